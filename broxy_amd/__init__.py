@@ -1,0 +1,235 @@
+"""GMorpho closing path on MI355X: ctypes access to the in-tree libraries.
+
+The product is the C-ABI library ``_lib/libgmorpho_hip.so`` (declared in
+``include/gmorpho_c.h``) and its C++ facade ``include/gmorpho.hpp``. This
+module only loads those libraries for Python callers (tests, bench); it has no
+compute path of its own and raises when the HIP library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "_lib")
+HIP_LIB = os.path.join(LIBDIR, "libgmorpho_hip.so")
+MESHGEN_LIB = os.path.join(LIBDIR, "libgmorpho_meshgen.so")
+
+GRID_INPUT, GRID_DILATION, GRID_CONTOUR, GRID_CLOSING = 0, 1, 2, 3
+SE_SPHERE, SE_CUBE = 0, 1
+
+_hip = None
+_meshgen = None
+_P = ctypes.c_void_p
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+class GMorphoError(RuntimeError):
+    pass
+
+
+def hip_lib():
+    """Load libgmorpho_hip.so (raises if it has not been built)."""
+    global _hip
+    if _hip is None:
+        if not os.path.exists(HIP_LIB):
+            raise GMorphoError("libgmorpho_hip.so is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(HIP_LIB)
+        L.gm_last_error.restype = ctypes.c_char_p
+        L.gm_last_error.argtypes = [_P]
+        _hip = L
+    return _hip
+
+
+def meshgen_lib():
+    global _meshgen
+    if _meshgen is None:
+        if not os.path.exists(MESHGEN_LIB):
+            raise GMorphoError("libgmorpho_meshgen.so is not built; run __graft_entry__.build()")
+        _meshgen = ctypes.CDLL(MESHGEN_LIB)
+    return _meshgen
+
+
+# ---------------------------------------------------------------------------
+# workload generators
+def icosphere_counts(subdiv):
+    nv, nf = ctypes.c_int(), ctypes.c_int()
+    if meshgen_lib().gmg_icosphere_counts(subdiv, ctypes.byref(nv), ctypes.byref(nf)):
+        raise ValueError("bad subdivision")
+    return nv.value, nf.value
+
+
+def icosphere(subdiv):
+    nv, nf = icosphere_counts(subdiv)
+    P = np.zeros((nv, 3), np.float32)
+    T = np.zeros((nf, 3), np.uint32)
+    meshgen_lib().gmg_icosphere(subdiv, _ptr(P), _ptr(T))
+    return P, T
+
+
+def sh_blob(subdiv, seed, lmax=8, amp=0.1):
+    nv, nf = icosphere_counts(subdiv)
+    P = np.zeros((nv, 3), np.float32)
+    T = np.zeros((nf, 3), np.uint32)
+    rc = meshgen_lib().gmg_sh_blob(subdiv, ctypes.c_uint64(seed), lmax, ctypes.c_double(amp), _ptr(P), _ptr(T))
+    if rc:
+        raise ValueError("gmg_sh_blob failed")
+    return P, T
+
+
+def sphere_soup(nspheres, subdiv, seed, cmin=0.15, cmax=0.85, rmin=0.03, rmax=0.12):
+    nv, nf = icosphere_counts(subdiv)
+    P = np.zeros((nspheres * nv, 3), np.float32)
+    T = np.zeros((nspheres * nf, 3), np.uint32)
+    rc = meshgen_lib().gmg_sphere_soup(nspheres, subdiv, ctypes.c_uint64(seed), ctypes.c_double(cmin),
+                                       ctypes.c_double(cmax), ctypes.c_double(rmin), ctypes.c_double(rmax), _ptr(P),
+                                       _ptr(T))
+    if rc:
+        raise ValueError("gmg_sphere_soup failed")
+    return P, T
+
+
+def scale_lattice(L, seed, lo=2.0, hi=24.0, smooth=3):
+    out = np.zeros(L ** 3, np.float32)
+    rc = meshgen_lib().gmg_scale_lattice(L, ctypes.c_uint64(seed), ctypes.c_double(lo), ctypes.c_double(hi), smooth,
+                                         _ptr(out))
+    if rc:
+        raise ValueError("gmg_scale_lattice failed")
+    return out
+
+
+# ---------------------------------------------------------------------------
+class Context:
+    """One gm_ctx (one device, one stream)."""
+
+    def __init__(self, device=0):
+        self._lib = hip_lib()
+        h = _P()
+        rc = self._lib.gm_create(device, ctypes.byref(h))
+        if rc:
+            raise GMorphoError("gm_create failed (%d): no usable HIP device" % rc)
+        self._h = h
+
+    def close(self):
+        if self._h:
+            self._lib.gm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, name, *args):
+        rc = getattr(self._lib, name)(self._h, *args)
+        if rc:
+            msg = self._lib.gm_last_error(self._h)
+            raise GMorphoError("%s -> %d: %s" % (name, rc, msg.decode() if msg else ""))
+        return rc
+
+    # GMorpho::Load
+    def load(self, P, T, base_res, se_size):
+        P = np.ascontiguousarray(P, np.float32)
+        T = np.ascontiguousarray(T, np.uint32)
+        self._call("gm_load", _ptr(P), None, len(P), _ptr(T), len(T), base_res, ctypes.c_float(se_size))
+
+    def load_mesh(self, P, T):
+        P = np.ascontiguousarray(P, np.float32)
+        T = np.ascontiguousarray(T, np.uint32)
+        self._call("gm_load_mesh", _ptr(P), None, len(P), _ptr(T), len(T))
+
+    def voxelize(self, base_res, margin):
+        self._call("gm_voxelize", base_res, ctypes.c_float(margin))
+
+    def grid_init(self, bmin, bmax, base_res, data_res, cell):
+        bmin = np.ascontiguousarray(bmin, np.float32)
+        bmax = np.ascontiguousarray(bmax, np.float32)
+        dr = np.ascontiguousarray(data_res, np.int32)
+        self._call("gm_grid_init", _ptr(bmin), _ptr(bmax), base_res, _ptr(dr), ctypes.c_float(cell))
+
+    def info(self):
+        bmin = np.zeros(3, np.float32)
+        bmax = np.zeros(3, np.float32)
+        res = np.zeros(3, np.int32)
+        dr = np.zeros(3, np.int32)
+        cell = ctypes.c_float()
+        self._call("gm_grid_info", _ptr(bmin), _ptr(bmax), _ptr(res), _ptr(dr), ctypes.byref(cell))
+        return bmin, bmax, int(res[0]), dr, cell.value
+
+    def set_cells(self, which, cells):
+        cells = np.ascontiguousarray(cells, np.uint8)
+        self._call("gm_grid_set_cells", which, _ptr(cells))
+
+    def get_cells(self, which):
+        R = self.info()[2]
+        out = np.zeros(R ** 3, np.uint8)
+        self._call("gm_grid_get_cells", which, _ptr(out))
+        return out
+
+    def scale_uniform(self, r):
+        self._call("gm_scale_uniform", int(r))
+
+    def scale_grid(self, grid_i8):
+        g = np.ascontiguousarray(grid_i8, np.int8)
+        self._call("gm_scale_grid_i8", _ptr(g))
+
+    def scale_lattice(self, lat, L):
+        lat = np.ascontiguousarray(lat, np.float32)
+        self._call("gm_scale_lattice_f32", _ptr(lat), L)
+
+    def dilate(self, se=SE_SPHERE):
+        self._call("gm_dilate", se)
+
+    def erode(self):
+        self._call("gm_erode")
+
+    def close_grid(self, se=SE_SPHERE):
+        self._call("gm_close", se)
+
+    def extract_mesh(self):
+        self._call("gm_extract_mesh")
+
+    def mesh_counts(self):
+        nv, nt = ctypes.c_uint32(), ctypes.c_uint32()
+        self._call("gm_mesh_counts", ctypes.byref(nv), ctypes.byref(nt))
+        return nv.value, nt.value
+
+    def mesh(self):
+        nv, nt = self.mesh_counts()
+        V = np.zeros((nv, 3), np.float32)
+        N = np.zeros((nv, 3), np.float32)
+        T = np.zeros((nt, 3), np.uint32)
+        self._call("gm_mesh_copy", _ptr(V), _ptr(N), _ptr(T))
+        return V, N, T
+
+    def build_pyramid(self, which):
+        self._call("gm_build_pyramid", which)
+        lv = ctypes.c_int()
+        tex = ctypes.c_uint64()
+        self._call("gm_pyramid_size", ctypes.byref(lv), ctypes.byref(tex))
+        out = np.zeros(tex.value, np.uint64)
+        self._call("gm_pyramid_get", _ptr(out))
+        return out
+
+    def set_timing(self, on=True):
+        self._call("gm_set_timing", 1 if on else 0)
+
+    def stage_times(self):
+        ms = np.zeros(10, np.float32)
+        self._call("gm_stage_times", _ptr(ms), 10)
+        return ms
+
+    def sync(self):
+        self._call("gm_sync")
+
+
+def mc_tables():
+    ntri = np.zeros(256, np.uint32)
+    ln = np.zeros(256, np.uint32)
+    tri = np.zeros(256, np.uint64)
+    hip_lib().gm_mc_tables(_ptr(ntri), _ptr(ln), _ptr(tri))
+    return ntri, ln, tri

@@ -1,0 +1,729 @@
+// C-ABI of the GMorpho closing path: context, device buffers and the
+// orchestration of GMorpho::Load / Update / ExtractClosingMesh
+// (GMorpho.cu:7731-7845, 6814-7001) on one HIP stream.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gmorpho_c.h"
+#include "gm_internal.h"
+
+namespace gm {
+std::vector<u32> build_se_table(int kind, int rho, bool cube);
+hipError_t mc_tables_host(u32* ntri, u32* len, unsigned long long* tri);
+}  // namespace gm
+
+using namespace gm;
+
+struct gm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // grid
+  bool has_grid = false;
+  int R = 0, N = 0, W = 0;
+  float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0}, cell = 0.f;
+  int data_res[3] = {0, 0, 0};
+  u32 *I = nullptr, *D = nullptr, *K = nullptr, *C = nullptr;
+  u16 *DF = nullptr, *TG = nullptr, *CL = nullptr, *UG = nullptr;
+  uint8_t* stage = nullptr;  // R^3 bytes for layout conversions
+  // mesh input
+  float* dP = nullptr;
+  u32* dT = nullptr;
+  int32_t* dSnap = nullptr;
+  int nv = 0, nf = 0;
+  size_t capP = 0, capTri = 0;
+  u32 *mark = nullptr, *shell = nullptr;
+  // radius field
+  int rmode = 0, r = 0, rlo = 0, rhi = 0, L = 0;
+  int8_t* dGrid = nullptr;
+  int32_t* dLat = nullptr;
+  // structuring-element tables
+  u32* dTables = nullptr;
+  size_t tablesCap = 0;
+  SETableSet tabs;
+  int tabLo = -1, tabHi = -1;
+  // marching cubes
+  MCBuffers mc{};
+  u32* hTotals = nullptr;
+  bool meshPending = false;
+  int Pmc = 0;
+  // pyramid
+  u64* dPyr = nullptr;
+  int pyrLevels = 0;
+  uint64_t pyrTexels = 0;
+  // timing
+  bool timing = false;
+  hipEvent_t ev[16] = {};
+  int evValid = 0;
+};
+
+namespace {
+
+int fail(gm_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = "GMorpho Error: " + msg;
+  return code;
+}
+
+#define HIPCALL(ctx, x)                                                                       \
+  do {                                                                                        \
+    hipError_t e_ = (x);                                                                      \
+    if (e_ != hipSuccess) return fail(ctx, GM_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+void dfree(T*& p) {
+  if (p) hipFree(p);
+  p = nullptr;
+}
+
+void free_grid(gm_ctx* c) {
+  dfree(c->I); dfree(c->D); dfree(c->K); dfree(c->C);
+  dfree(c->DF); dfree(c->TG); dfree(c->CL); dfree(c->UG);
+  dfree(c->stage); dfree(c->mark); dfree(c->shell);
+  dfree(c->mc.blockV); dfree(c->mc.blockT); dfree(c->mc.cellV); dfree(c->mc.cellT);
+  dfree(c->mc.V); dfree(c->mc.Nrm); dfree(c->mc.T); dfree(c->mc.late); dfree(c->mc.totals);
+  dfree(c->dPyr);
+  dfree(c->dGrid);
+  c->has_grid = false;
+  c->meshPending = false;
+  c->pyrLevels = 0;
+}
+
+int alloc_mesh_out(gm_ctx* c, u32 capV, u32 capT, u32 capLate) {
+  if (capV > c->mc.capV) {
+    dfree(c->mc.V);
+    dfree(c->mc.Nrm);
+    HIPCALL(c, hipMalloc(&c->mc.V, (size_t)capV * 12));
+    HIPCALL(c, hipMalloc(&c->mc.Nrm, (size_t)capV * 12));
+    c->mc.capV = capV;
+  }
+  if (capT > c->mc.capT) {
+    dfree(c->mc.T);
+    HIPCALL(c, hipMalloc(&c->mc.T, (size_t)capT * 12));
+    c->mc.capT = capT;
+  }
+  if (capLate > c->mc.capLate || !c->mc.late) {
+    dfree(c->mc.late);
+    HIPCALL(c, hipMalloc(&c->mc.late, (size_t)(1 + 4 * capLate) * 4));
+    c->mc.capLate = capLate;
+  }
+  return GM_OK;
+}
+
+int alloc_grid(gm_ctx* c, int R) {
+  if (R < 16 || R > 1024 || (R % 16) != 0)
+    return fail(c, GM_ERR_UNSUPPORTED, "base resolution must be a multiple of 16 in [16,1024]");
+  if (c->has_grid && c->R == R) return GM_OK;
+  free_grid(c);
+  c->R = R;
+  c->N = 2 * R;
+  c->W = c->N / 32;
+  size_t rowBytes = (size_t)c->N * c->N * c->W * 4;
+  size_t flagBytes = (size_t)R * R * c->W * 2;
+  HIPCALL(c, hipMalloc(&c->I, rowBytes));
+  HIPCALL(c, hipMalloc(&c->D, rowBytes));
+  HIPCALL(c, hipMalloc(&c->K, rowBytes));
+  HIPCALL(c, hipMalloc(&c->C, rowBytes));
+  HIPCALL(c, hipMalloc(&c->mark, rowBytes));
+  HIPCALL(c, hipMalloc(&c->shell, rowBytes));
+  HIPCALL(c, hipMalloc(&c->DF, flagBytes));
+  HIPCALL(c, hipMalloc(&c->TG, flagBytes));
+  HIPCALL(c, hipMalloc(&c->CL, flagBytes));
+  HIPCALL(c, hipMalloc(&c->UG, flagBytes));
+  HIPCALL(c, hipMalloc(&c->stage, (size_t)R * R * R));
+  HIPCALL(c, hipMemsetAsync(c->I, 0, rowBytes, c->stream));
+  HIPCALL(c, hipMemsetAsync(c->D, 0, rowBytes, c->stream));
+  HIPCALL(c, hipMemsetAsync(c->K, 0, rowBytes, c->stream));
+  HIPCALL(c, hipMemsetAsync(c->C, 0, rowBytes, c->stream));
+  int P = 1;
+  while (P < R) P <<= 1;
+  c->Pmc = P;
+  c->mc.P = P;
+  c->mc.nblocks = (P / 8) * (P / 8) * (P / 8);
+  HIPCALL(c, hipMalloc(&c->mc.blockV, (size_t)c->mc.nblocks * 4));
+  HIPCALL(c, hipMalloc(&c->mc.blockT, (size_t)c->mc.nblocks * 4));
+  HIPCALL(c, hipMalloc(&c->mc.cellV, (size_t)R * R * R * 4));
+  HIPCALL(c, hipMalloc(&c->mc.totals, 16));
+  c->mc.capV = c->mc.capT = c->mc.capLate = 0;
+  int rc = alloc_mesh_out(c, (u32)std::min<size_t>((size_t)4 * c->N * c->N, 1u << 30),
+                          (u32)std::min<size_t>((size_t)8 * c->N * c->N, 1u << 30), 4096);
+  if (rc) return rc;
+  c->has_grid = true;
+  // default radius: uniform 0 (closing = input) until a ScaleField is set
+  if (c->rmode != 0) {
+    c->rmode = 0;
+    c->r = 0;
+    c->rlo = c->rhi = 0;
+  }
+  return GM_OK;
+}
+
+int need_grid(gm_ctx* c) {
+  if (!c) return GM_ERR_ARG;
+  if (!c->has_grid) return fail(c, GM_ERR_STATE, "no grid: call gm_load, gm_voxelize or gm_grid_init first");
+  return GM_OK;
+}
+
+u32* grid_rows(gm_ctx* c, int which) {
+  switch (which) {
+    case GM_GRID_INPUT: return c->I;
+    case GM_GRID_DILATION: return c->D;
+    case GM_GRID_CONTOUR: return c->K;
+    case GM_GRID_CLOSING: return c->C;
+    default: return nullptr;
+  }
+}
+
+RadiusField radius_field(const gm_ctx* c) {
+  RadiusField f{};
+  f.mode = c->rmode;
+  f.r = c->r;
+  f.N = c->N;
+  f.grid = c->dGrid;
+  f.lat = c->dLat;
+  f.L = c->L;
+  f.s = c->L ? c->N / c->L : 1;
+  f.lshift = 0;
+  while ((1 << f.lshift) < f.s) ++f.lshift;
+  return f;
+}
+
+int ensure_tables(gm_ctx* c, int rlo, int rhi) {
+  if (c->tabLo == rlo && c->tabHi == rhi && c->dTables) return GM_OK;
+  std::vector<u32> all;
+  SETableSet t;
+  for (int i = 0; i <= 128; ++i) t.dil[i] = t.ero_corner[i] = t.ero_centre[i] = -1;
+  auto add = [&](const std::vector<u32>& v) {
+    int off = (int)all.size();
+    all.insert(all.end(), v.begin(), v.end());
+    return off;
+  };
+  // sphere dilation uses rho = r+1, cube uses rho = r; both are corner tables,
+  // so dil[] holds sphere tables at r+1 and cube tables are built separately.
+  for (int r = rlo; r <= rhi; ++r) t.dil[r + 1] = add(build_se_table(SE_CORNER, r + 1, false));
+  for (int r = rlo; r <= rhi; ++r) {
+    int rho = r - 1;
+    if (rho < 1) continue;
+    t.ero_corner[rho] = add(build_se_table(SE_CORNER, rho, false));
+    t.ero_centre[rho] = add(build_se_table(SE_CENTRE, rho, false));
+  }
+  c->tabs = t;
+  size_t bytes = all.size() * 4;
+  if (bytes > c->tablesCap) {
+    dfree(c->dTables);
+    HIPCALL(c, hipMalloc(&c->dTables, bytes));
+    c->tablesCap = bytes;
+  }
+  HIPCALL(c, hipMemcpyAsync(c->dTables, all.data(), bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  c->tabLo = rlo;
+  c->tabHi = rhi;
+  return GM_OK;
+}
+
+// Cube dilation tables live in a second table set (dil[] indexed by r).
+int cube_tables(gm_ctx* c, int rlo, int rhi, SETableSet& t, u32*& dtab) {
+  std::vector<u32> all;
+  for (int i = 0; i <= 128; ++i) t.dil[i] = t.ero_corner[i] = t.ero_centre[i] = -1;
+  for (int r = rlo; r <= rhi; ++r) {
+    int off = (int)all.size();
+    std::vector<u32> v = build_se_table(SE_CORNER, r, true);
+    all.insert(all.end(), v.begin(), v.end());
+    t.dil[r] = off;
+  }
+  HIPCALL(c, hipMalloc(&dtab, all.size() * 4));
+  HIPCALL(c, hipMemcpy(dtab, all.data(), all.size() * 4, hipMemcpyHostToDevice));
+  return GM_OK;
+}
+
+MorphBuffers morph_buffers(gm_ctx* c) {
+  MorphBuffers b{};
+  b.I = c->I;
+  b.D = c->D;
+  b.K = c->K;
+  b.C = c->C;
+  b.DF = c->DF;
+  b.TG = c->TG;
+  b.CL = c->CL;
+  b.UG = c->UG;
+  b.tables = c->dTables;
+  return b;
+}
+
+MorphDims morph_dims(gm_ctx* c) {
+  MorphDims d{};
+  d.R = c->R;
+  d.N = c->N;
+  d.W = c->W;
+  for (int a = 0; a < 3; ++a) d.dR[a] = std::max(0, std::min(c->data_res[a], c->R));
+  return d;
+}
+
+int check_radius(gm_ctx* c) {
+  if (c->rhi > 31) return fail(c, GM_ERR_UNSUPPORTED, "radius above 31 fine voxels is not supported by this build");
+  return GM_OK;
+}
+
+int do_dilate(gm_ctx* c, int se) {
+  int rc = check_radius(c);
+  if (rc) return rc;
+  int lo = std::max(c->rlo, 0), hi = c->rhi;
+  if (se == GM_SE_CUBE) {
+    SETableSet t;
+    u32* dtab = nullptr;
+    rc = cube_tables(c, lo, hi, t, dtab);
+    if (rc) return rc;
+    MorphBuffers b = morph_buffers(c);
+    b.tables = dtab;
+    hipError_t e = launch_dilate(morph_dims(c), b, radius_field(c), 1, t, lo, hi, c->stream,
+                                 c->timing ? &c->ev[0] : nullptr);
+    hipStreamSynchronize(c->stream);
+    hipFree(dtab);
+    if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("dilation: ") + hipGetErrorString(e));
+  } else {
+    rc = ensure_tables(c, lo, hi);
+    if (rc) return rc;
+    hipError_t e = launch_dilate(morph_dims(c), morph_buffers(c), radius_field(c), 0, c->tabs, lo, hi, c->stream,
+                                 c->timing ? &c->ev[0] : nullptr);
+    if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("dilation: ") + hipGetErrorString(e));
+  }
+  c->meshPending = false;
+  return GM_OK;
+}
+
+int do_erode(gm_ctx* c) {
+  int rc = check_radius(c);
+  if (rc) return rc;
+  rc = ensure_tables(c, std::max(c->rlo, 0), c->rhi);
+  if (rc) return rc;
+  hipError_t e = launch_erode(morph_dims(c), morph_buffers(c), radius_field(c), c->tabs, c->rlo, c->rhi, c->stream,
+                              c->timing ? &c->ev[4] : nullptr);
+  if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("erosion: ") + hipGetErrorString(e));
+  c->meshPending = false;
+  return GM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gm_create(int device, gm_ctx** out) {
+  if (!out) return GM_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return GM_ERR_HIP;
+  if (device < 0 || device >= n) return GM_ERR_ARG;
+  gm_ctx* c = new gm_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return GM_ERR_HIP;
+  }
+  for (auto& e : c->ev) hipEventCreate(&e);
+  hipHostMalloc((void**)&c->hTotals, 16, hipHostMallocDefault);
+  *out = c;
+  return GM_OK;
+}
+
+int gm_destroy(gm_ctx* c) {
+  if (!c) return GM_ERR_ARG;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  free_grid(c);
+  dfree(c->dP);
+  dfree(c->dT);
+  dfree(c->dSnap);
+  dfree(c->dLat);
+  dfree(c->dTables);
+  for (auto& e : c->ev) hipEventDestroy(e);
+  if (c->hTotals) hipHostFree(c->hTotals);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return GM_OK;
+}
+
+const char* gm_last_error(const gm_ctx* c) { return c ? c->err.c_str() : "GMorpho Error: null context"; }
+
+int gm_load_mesh(gm_ctx* c, const float* P, const float* N, int nv, const uint32_t* T, int nf) {
+  (void)N;
+  if (!c) return GM_ERR_ARG;
+  if (!P || !T || nv <= 0 || nf <= 0) return fail(c, GM_ERR_ARG, "empty mesh");
+  for (size_t i = 0; i < (size_t)3 * nf; ++i)
+    if (T[i] >= (uint32_t)nv) return fail(c, GM_ERR_ARG, "triangle index out of range");
+  hipSetDevice(c->device);
+  if ((size_t)nv > c->capP) {
+    dfree(c->dP);
+    dfree(c->dSnap);
+    HIPCALL(c, hipMalloc(&c->dP, (size_t)nv * 12));
+    HIPCALL(c, hipMalloc(&c->dSnap, (size_t)nv * 12));
+    c->capP = nv;
+  }
+  if ((size_t)nf > c->capTri) {
+    dfree(c->dT);
+    HIPCALL(c, hipMalloc(&c->dT, (size_t)nf * 12));
+    c->capTri = nf;
+  }
+  HIPCALL(c, hipMemcpyAsync(c->dP, P, (size_t)nv * 12, hipMemcpyHostToDevice, c->stream));
+  HIPCALL(c, hipMemcpyAsync(c->dT, T, (size_t)nf * 12, hipMemcpyHostToDevice, c->stream));
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  c->nv = nv;
+  c->nf = nf;
+  // bounding box of the mesh, kept for ComputeGridAttributes
+  float mn[3] = {P[0], P[1], P[2]}, mx[3] = {P[0], P[1], P[2]};
+  for (int i = 0; i < nv; ++i)
+    for (int a = 0; a < 3; ++a) {
+      float v = P[3 * i + a];
+      if (v > mx[a]) mx[a] = v;
+      if (v < mn[a]) mn[a] = v;
+    }
+  std::memcpy(c->bmin, mn, sizeof(mn));
+  std::memcpy(c->bmax, mx, sizeof(mx));
+  c->cell = -1.f;  // grid attributes not computed yet
+  return GM_OK;
+}
+
+int gm_voxelize(gm_ctx* c, int base_res, float margin) {
+  if (!c) return GM_ERR_ARG;
+  if (c->nv <= 0) return fail(c, GM_ERR_STATE, "no mesh: call gm_load_mesh first");
+  hipSetDevice(c->device);
+  // Voxelizer::ComputeGridAttributes (Voxelizer.cu:431-455), in float.
+  float mn[3], mx[3];
+  if (c->cell < 0.f) {
+    std::memcpy(mn, c->bmin, sizeof(mn));
+    std::memcpy(mx, c->bmax, sizeof(mx));
+  } else {
+    return fail(c, GM_ERR_STATE, "grid already voxelized for this mesh: call gm_load_mesh again");
+  }
+  int rc = alloc_grid(c, base_res);
+  if (rc) return rc;
+  float thick = std::max(std::fabs(mx[0] - mn[0]), std::max(std::fabs(mx[1] - mn[1]), std::fabs(mx[2] - mn[2]))) /
+                (float)base_res;
+  float off = 2 * thick + margin;
+  for (int a = 0; a < 3; ++a) {
+    mn[a] -= off;
+    mx[a] += off;
+  }
+  float cs = std::max(std::fabs(mx[0] - mn[0]), std::max(std::fabs(mx[1] - mn[1]), std::fabs(mx[2] - mn[2]))) /
+             (float)base_res;
+  for (int a = 0; a < 3; ++a) {
+    c->bmin[a] = mn[a];
+    c->bmax[a] = mx[a];
+    c->data_res[a] = (int)std::ceil((mx[a] - mn[a]) / cs);
+  }
+  c->cell = cs;
+  float maxext = std::max(std::fabs(mx[0] - mn[0]), std::max(std::fabs(mx[1] - mn[1]), std::fabs(mx[2] - mn[2])));
+  double scale = (double)c->N / (double)maxext;
+  hipError_t e = launch_voxelize(c->dP, c->nv, c->dT, c->nf, c->bmin, scale, c->R, c->dSnap, c->mark, c->shell, c->I,
+                                 c->stream);
+  if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("voxelize: ") + hipGetErrorString(e));
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  c->meshPending = false;
+  return GM_OK;
+}
+
+int gm_load(gm_ctx* c, const float* P, const float* N, int nv, const uint32_t* T, int nf, int base_res,
+            float se_size) {
+  int rc = gm_load_mesh(c, P, N, nv, T, nf);
+  if (rc) return rc;
+  // margin of GMorpho::Load (GMorpho.cu:7737)
+  float margin = se_size + 2.f / (2 * ((float)base_res));
+  return gm_voxelize(c, base_res, margin);
+}
+
+int gm_grid_init(gm_ctx* c, const float bmin[3], const float bmax[3], int base_res, const int data_res[3],
+                 float cell) {
+  if (!c || !bmin || !bmax || !data_res) return GM_ERR_ARG;
+  hipSetDevice(c->device);
+  int rc = alloc_grid(c, base_res);
+  if (rc) return rc;
+  for (int a = 0; a < 3; ++a) {
+    c->bmin[a] = bmin[a];
+    c->bmax[a] = bmax[a];
+    c->data_res[a] = data_res[a];
+  }
+  c->cell = cell;
+  size_t rowBytes = (size_t)c->N * c->N * c->W * 4;
+  HIPCALL(c, hipMemsetAsync(c->I, 0, rowBytes, c->stream));
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  return GM_OK;
+}
+
+int gm_grid_set_cells(gm_ctx* c, int which, const uint8_t* cells) {
+  int rc = need_grid(c);
+  if (rc) return rc;
+  u32* rows = grid_rows(c, which);
+  if (!rows || !cells) return fail(c, GM_ERR_ARG, "bad grid id or null buffer");
+  hipSetDevice(c->device);
+  HIPCALL(c, hipMemcpyAsync(c->stage, cells, (size_t)c->R * c->R * c->R, hipMemcpyHostToDevice, c->stream));
+  HIPCALL(c, launch_cells_to_rows(c->stage, rows, c->R, c->stream));
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  c->meshPending = false;
+  return GM_OK;
+}
+
+int gm_grid_get_cells(gm_ctx* c, int which, uint8_t* cells) {
+  int rc = need_grid(c);
+  if (rc) return rc;
+  u32* rows = grid_rows(c, which);
+  if (!rows || !cells) return fail(c, GM_ERR_ARG, "bad grid id or null buffer");
+  hipSetDevice(c->device);
+  HIPCALL(c, launch_rows_to_cells(rows, c->stage, c->R, c->stream));
+  HIPCALL(c, hipMemcpyAsync(cells, c->stage, (size_t)c->R * c->R * c->R, hipMemcpyDeviceToHost, c->stream));
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  return GM_OK;
+}
+
+int gm_grid_info(const gm_ctx* c, float bmin[3], float bmax[3], int res[3], int data_res[3], float* cell) {
+  if (!c) return GM_ERR_ARG;
+  if (!c->has_grid) return GM_ERR_STATE;
+  for (int a = 0; a < 3; ++a) {
+    if (bmin) bmin[a] = c->bmin[a];
+    if (bmax) bmax[a] = c->bmax[a];
+    if (res) res[a] = c->R;
+    if (data_res) data_res[a] = c->data_res[a];
+  }
+  if (cell) *cell = c->cell;
+  return GM_OK;
+}
+
+int gm_scale_uniform(gm_ctx* c, int r) {
+  int rc = need_grid(c);
+  if (rc) return rc;
+  if (r < 0 || r > 127) return fail(c, GM_ERR_ARG, "uniform radius must be in [0,127]");
+  c->rmode = 0;
+  c->r = r;
+  c->rlo = c->rhi = r;
+  c->meshPending = false;
+  return GM_OK;
+}
+
+int gm_scale_grid_i8(gm_ctx* c, const int8_t* grid) {
+  int rc = need_grid(c);
+  if (rc) return rc;
+  if (!grid) return fail(c, GM_ERR_ARG, "null scale grid");
+  hipSetDevice(c->device);
+  size_t n = (size_t)c->N * c->N * c->N;
+  if (!c->dGrid) HIPCALL(c, hipMalloc(&c->dGrid, n));
+  HIPCALL(c, hipMemcpyAsync(c->dGrid, grid, n, hipMemcpyHostToDevice, c->stream));
+  int lo = 127, hi = 2;
+  for (size_t i = 0; i < n; ++i) {
+    int v = grid[i];
+    v = v < 2 ? 2 : (v > 127 ? 127 : v);
+    lo = std::min(lo, v);
+    hi = std::max(hi, v);
+  }
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  c->rmode = 1;
+  c->rlo = lo;
+  c->rhi = hi;
+  c->meshPending = false;
+  return GM_OK;
+}
+
+int gm_scale_lattice_f32(gm_ctx* c, const float* lat, int L) {
+  int rc = need_grid(c);
+  if (rc) return rc;
+  if (!lat || L < 2 || L > c->N || (c->N % L) != 0 || ((c->N / L) & (c->N / L - 1)) != 0)
+    return fail(c, GM_ERR_ARG, "lattice size must divide N with a power-of-two ratio");
+  hipSetDevice(c->device);
+  size_t n = (size_t)L * L * L;
+  std::vector<int32_t> q(n);
+  long long mn = (1LL << 40), mx = -(1LL << 40);
+  for (size_t i = 0; i < n; ++i) {
+    double v = std::floor((double)lat[i] * 256.0 + 0.5);
+    if (v > 2e9 || v < -2e9) return fail(c, GM_ERR_ARG, "lattice value out of range");
+    q[i] = (int32_t)v;
+    mn = std::min<long long>(mn, q[i]);
+    mx = std::max<long long>(mx, q[i]);
+  }
+  if (c->L != L) {
+    dfree(c->dLat);
+    HIPCALL(c, hipMalloc(&c->dLat, n * 4));
+  }
+  HIPCALL(c, hipMemcpyAsync(c->dLat, q.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  auto fl = [](long long v) { return v >= 0 ? v / 256 : -((-v + 255) / 256); };
+  c->L = L;
+  c->rmode = 2;
+  c->rlo = (int)std::max<long long>(2, std::min<long long>(127, fl(mn)));
+  c->rhi = (int)std::max<long long>(2, std::min<long long>(127, fl(mx)));
+  c->meshPending = false;
+  return GM_OK;
+}
+
+int gm_dilate(gm_ctx* c, int se) {
+  int rc = need_grid(c);
+  if (rc) return rc;
+  if (se != GM_SE_SPHERE && se != GM_SE_CUBE) return fail(c, GM_ERR_ARG, "unknown structuring element");
+  hipSetDevice(c->device);
+  c->evValid = 0;
+  rc = do_dilate(c, se);
+  if (rc) return rc;
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  return GM_OK;
+}
+
+int gm_erode(gm_ctx* c) {
+  int rc = need_grid(c);
+  if (rc) return rc;
+  hipSetDevice(c->device);
+  c->evValid = 0;
+  rc = do_erode(c);
+  if (rc) return rc;
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  return GM_OK;
+}
+
+int gm_close(gm_ctx* c, int se) {
+  int rc = need_grid(c);
+  if (rc) return rc;
+  if (se != GM_SE_SPHERE && se != GM_SE_CUBE) return fail(c, GM_ERR_ARG, "unknown structuring element");
+  hipSetDevice(c->device);
+  c->evValid = 0;
+  size_t rowBytes = (size_t)c->N * c->N * c->W * 4;
+  if (c->rmode == 0 && c->r <= 1) {
+    // GMorpho.cu:7784, 7803: no morphology below two fine voxels.
+    HIPCALL(c, hipMemcpyAsync(c->D, c->I, rowBytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCALL(c, hipMemsetAsync(c->K, 0, rowBytes, c->stream));
+    HIPCALL(c, hipMemcpyAsync(c->C, c->I, rowBytes, hipMemcpyDeviceToDevice, c->stream));
+    c->meshPending = false;
+    return GM_OK;
+  }
+  rc = do_dilate(c, se);
+  if (rc) return rc;
+  rc = do_erode(c);
+  if (rc) return rc;
+  if (c->timing) c->evValid |= 1;
+  return GM_OK;
+}
+
+int gm_extract_mesh(gm_ctx* c) {
+  int rc = need_grid(c);
+  if (rc) return rc;
+  hipSetDevice(c->device);
+  hipError_t e = launch_mc(c->C, c->R, c->bmin, c->cell, c->mc, c->stream, c->timing ? &c->ev[9] : nullptr);
+  if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("mesh extraction: ") + hipGetErrorString(e));
+  HIPCALL(c, hipMemcpyAsync(c->hTotals, c->mc.totals, 16, hipMemcpyDeviceToHost, c->stream));
+  c->meshPending = true;
+  if (c->timing) c->evValid |= 2;
+  return GM_OK;
+}
+
+int gm_mesh_counts(gm_ctx* c, uint32_t* nv, uint32_t* nt) {
+  int rc = need_grid(c);
+  if (rc) return rc;
+  if (!c->meshPending) return fail(c, GM_ERR_STATE, "no mesh extracted since the last grid change");
+  hipSetDevice(c->device);
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  // grow and re-emit when the output did not fit
+  for (int attempt = 0; attempt < 3 && c->hTotals[2] != 0; ++attempt) {
+    u32 lateN = 0;
+    HIPCALL(c, hipMemcpy(&lateN, c->mc.late, 4, hipMemcpyDeviceToHost));
+    rc = alloc_mesh_out(c, std::max(c->mc.capV, c->hTotals[0] + c->hTotals[0] / 8 + 1024),
+                        std::max(c->mc.capT, c->hTotals[1] + c->hTotals[1] / 8 + 1024),
+                        std::max(c->mc.capLate, lateN + 1024));
+    if (rc) return rc;
+    u32 tot[4] = {c->hTotals[0], c->hTotals[1], 0u, 0u};
+    HIPCALL(c, hipMemcpy(c->mc.totals, tot, 16, hipMemcpyHostToDevice));
+    hipError_t e = launch_mc_emit(c->C, c->R, c->bmin, c->cell, c->mc, c->stream, nullptr);
+    if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("mesh extraction: ") + hipGetErrorString(e));
+    HIPCALL(c, hipMemcpyAsync(c->hTotals, c->mc.totals, 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+  }
+  if (c->hTotals[2] != 0) return fail(c, GM_ERR_CAPACITY, "mesh output overflow");
+  if (nv) *nv = c->hTotals[0];
+  if (nt) *nt = c->hTotals[1];
+  return GM_OK;
+}
+
+int gm_mesh_copy(gm_ctx* c, float* V, float* Nrm, uint32_t* T) {
+  uint32_t nv = 0, nt = 0;
+  int rc = gm_mesh_counts(c, &nv, &nt);
+  if (rc) return rc;
+  if (V && nv) HIPCALL(c, hipMemcpyAsync(V, c->mc.V, (size_t)nv * 12, hipMemcpyDeviceToHost, c->stream));
+  if (Nrm && nv) HIPCALL(c, hipMemcpyAsync(Nrm, c->mc.Nrm, (size_t)nv * 12, hipMemcpyDeviceToHost, c->stream));
+  if (T && nt) HIPCALL(c, hipMemcpyAsync(T, c->mc.T, (size_t)nt * 12, hipMemcpyDeviceToHost, c->stream));
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  return GM_OK;
+}
+
+int gm_build_pyramid(gm_ctx* c, int which) {
+  int rc = need_grid(c);
+  if (rc) return rc;
+  u32* rows = grid_rows(c, which);
+  if (!rows) return fail(c, GM_ERR_ARG, "bad grid id");
+  if (c->R & (c->R - 1)) return fail(c, GM_ERR_UNSUPPORTED, "the max-mipmap needs a power-of-two resolution");
+  hipSetDevice(c->device);
+  int levels = 0;
+  uint64_t tex = 0;
+  for (int s = c->R; s >= 1; s >>= 1) {
+    ++levels;
+    tex += (uint64_t)s * s * s;
+  }
+  if (!c->dPyr) HIPCALL(c, hipMalloc(&c->dPyr, tex * 8));
+  HIPCALL(c, launch_pyramid(rows, c->R, radius_field(c), c->dPyr, c->stream));
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  c->pyrLevels = levels;
+  c->pyrTexels = tex;
+  return GM_OK;
+}
+
+int gm_pyramid_size(const gm_ctx* c, int* levels, uint64_t* texels) {
+  if (!c) return GM_ERR_ARG;
+  if (!c->pyrLevels) return GM_ERR_STATE;
+  if (levels) *levels = c->pyrLevels;
+  if (texels) *texels = c->pyrTexels;
+  return GM_OK;
+}
+
+int gm_pyramid_get(gm_ctx* c, uint64_t* out) {
+  if (!c || !out) return GM_ERR_ARG;
+  if (!c->pyrLevels) return fail(c, GM_ERR_STATE, "no pyramid built");
+  HIPCALL(c, hipMemcpy(out, c->dPyr, c->pyrTexels * 8, hipMemcpyDeviceToHost));
+  return GM_OK;
+}
+
+int gm_set_timing(gm_ctx* c, int enable) {
+  if (!c) return GM_ERR_ARG;
+  c->timing = enable != 0;
+  c->evValid = 0;
+  return GM_OK;
+}
+
+int gm_stage_times(gm_ctx* c, float* ms, int n) {
+  if (!c || !ms) return GM_ERR_ARG;
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  // events: dilate 0..3, erode 4..8, mc 9..12
+  const int pairs[10][2] = {{0, 1}, {1, 2}, {2, 3}, {4, 5}, {5, 6}, {6, 7}, {7, 8}, {9, 10}, {10, 11}, {11, 12}};
+  for (int i = 0; i < n && i < 10; ++i) {
+    ms[i] = -1.f;
+    bool ok = (i < 7) ? (c->evValid & 1) : (c->evValid & 2);
+    if (!ok) continue;
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, c->ev[pairs[i][0]], c->ev[pairs[i][1]]) == hipSuccess) ms[i] = t;
+  }
+  return GM_OK;
+}
+
+int gm_sync(gm_ctx* c) {
+  if (!c) return GM_ERR_ARG;
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  return GM_OK;
+}
+
+int gm_get_stream(gm_ctx* c, void** s) {
+  if (!c || !s) return GM_ERR_ARG;
+  *s = (void*)c->stream;
+  return GM_OK;
+}
+
+int gm_mc_tables(uint32_t* ntri, uint32_t* len, uint64_t* tri) {
+  if (!ntri || !len || !tri) return GM_ERR_ARG;
+  mc_tables_host(ntri, len, (unsigned long long*)tri);
+  return GM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,149 @@
+// Internal declarations shared by the GMorpho HIP sources.
+//
+// Device layouts (N = 2R fine voxels per axis, R a multiple of 16):
+//  * Bit grid ("rows"): fine voxel (x,y,z) is bit x%32 of u32 word
+//    ((z*N + y)*W + x/32), W = N/32. One u32 row word spans 16 cells of the
+//    cell row (y/2, z/2) with sub-voxel bits (y&1, z&1).
+//  * Cell flags: u16 per (cell row, fine word): ((cz*R + cy)*W + wx), bit j =
+//    cell (16*wx + j, cy, cz). Used for the coarse-pass results and for the
+//    contour tags that gate the fine passes.
+//  * The reference's cell-byte layout (R^3 bytes, index x + R*(y + R*z), bit k =
+//    sub-voxel (k&1, (k>>1)&1, (k>>2)&1)) only exists at the C-ABI boundary.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace gm {
+
+typedef uint32_t u32;
+typedef uint64_t u64;
+typedef uint16_t u16;
+
+// ---------------------------------------------------------------------------
+// Radius field. mode 0: uniform r; 1: signed-char grid N^3 (x fastest),
+// clamped to [2,127]; 2: fixed-point lattice (L^3 int32 values in 1/256 units,
+// node i on fine voxel i*s), trilinear, floor, clamped to [2,127].
+struct RadiusField {
+  int mode;
+  int r;
+  int N;
+  const int8_t* grid;
+  const int32_t* lat;
+  int L;
+  int s;      // N / L (power of two)
+  int lshift; // log2(s)
+};
+
+__device__ __forceinline__ int radius_at(const RadiusField& f, int x, int y, int z) {
+  if (f.mode == 0) return f.r & 0x7f;
+  if (f.mode == 1) {
+    int v = f.grid[(size_t)x + (size_t)f.N * ((size_t)y + (size_t)f.N * (size_t)z)];
+    return v < 2 ? 2 : (v > 127 ? 127 : v);
+  }
+  int ix = x >> f.lshift, iy = y >> f.lshift, iz = z >> f.lshift;
+  long long fx = x - (ix << f.lshift), fy = y - (iy << f.lshift), fz = z - (iz << f.lshift);
+  long long s = f.s;
+  long long acc = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    int xx = ix + (k & 1), yy = iy + ((k >> 1) & 1), zz = iz + ((k >> 2) & 1);
+    xx = xx < f.L - 1 ? xx : f.L - 1;
+    yy = yy < f.L - 1 ? yy : f.L - 1;
+    zz = zz < f.L - 1 ? zz : f.L - 1;
+    long long w = ((k & 1) ? fx : s - fx) * ((k & 2) ? fy : s - fy) * ((k & 4) ? fz : s - fz);
+    acc += w * (long long)f.lat[(size_t)xx + (size_t)f.L * ((size_t)yy + (size_t)f.L * (size_t)zz)];
+  }
+  long long den = s * s * s * 256;
+  long long v = acc >= 0 ? acc / den : -((-acc + den - 1) / den);
+  v = v < -128 ? -128 : (v > 127 ? 127 : v);
+  return v < 2 ? 2 : (int)v;
+}
+
+// ---------------------------------------------------------------------------
+// Structuring-element chord tables. A table lists, per "dz class", the dz
+// values it stands for and the (dy class, x half-width m) chords. Two kinds:
+//  corner (query at a lattice point P; fine voxel q is at distance
+//          delta(q-P) per axis with delta(t) = t (t>=0), -t-1 (t<0)): a class
+//          value b stands for offsets {b, -b-1}; window along x is [-(m+1), m];
+//  centre (query at a voxel centre; per-axis distance |t|-1/2 for t != 0):
+//          class b stands for {b, -b} (just {0} for b = 0); window [-m, m].
+// Layout (u32): [0] kind, [1] nclass, [2] max m, [3] max |offset| in y/z,
+// then per class: header (b | npairs<<16), npairs words (a | m<<16).
+enum { SE_CORNER = 0, SE_CENTRE = 1 };
+constexpr int kMaxRhoFast = 32;  // streaming kernel supports m, |offset| <= 32
+
+// Launch configuration of the streaming morphology kernels.
+constexpr int kTileRows = 64;  // rows (y) per tile == threads per workgroup
+constexpr int kTileZ = 32;     // output planes per tile
+
+struct MorphDims {
+  int R, N, W;
+  int dR[3];  // data region in cells (clamped to R)
+};
+
+// Error plumbing: every host-side helper returns hipError_t.
+#define GM_CHECK(x)                          \
+  do {                                       \
+    hipError_t e_ = (x);                     \
+    if (e_ != hipSuccess) return e_;         \
+  } while (0)
+
+// ---- layout kernels (gm_layout.hip)
+hipError_t launch_cells_to_rows(const uint8_t* cells, u32* rows, int R, hipStream_t s);
+hipError_t launch_rows_to_cells(const u32* rows, uint8_t* cells, int R, hipStream_t s);
+
+// ---- voxelizer (gm_voxelize.hip)
+hipError_t launch_voxelize(const float* P, int nv, const u32* T, int nt, const float* bmin3, double scale,
+                           int R, int32_t* snap, u32* mark, u32* shell, u32* rows, hipStream_t s);
+
+// ---- morphology (gm_morph.hip)
+struct MorphBuffers {
+  u32* I;
+  u32* D;
+  u32* K;
+  u32* C;
+  u16* DF;  // coarse dilation: cell full
+  u16* TG;  // tight contour of DF
+  u16* CL;  // coarse erosion: cell cleared
+  u16* UG;  // base contour of the coarse erosion
+  u32* tables;       // device SE tables (concatenated)
+  int* flags;        // scratch counters
+};
+struct SETableSet {
+  // offsets (in u32) into MorphBuffers::tables for each rho, -1 if absent
+  int dil[129];
+  int ero_corner[129];
+  int ero_centre[129];
+};
+hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf, int se,
+                         const SETableSet& tabs, int rho_lo, int rho_hi, hipStream_t s, hipEvent_t* ev);
+hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf,
+                        const SETableSet& tabs, int rho_lo, int rho_hi, hipStream_t s, hipEvent_t* ev);
+
+// ---- marching cubes (gm_mc.hip)
+struct MCBuffers {
+  u32* blockV;   // per 8^3 Morton block: vertex count, then exclusive offset
+  u32* blockT;
+  u32* cellV;    // dense R^3: vertex base of cells in M
+  u32* cellT;
+  float* V;
+  float* Nrm;
+  u32* T;
+  u32* late;     // [0] count, then (slot, a, b, c) quadruples
+  u32* totals;   // [0] nV [1] nT [2] overflow [3] border flag
+  u32 capV, capT, capLate;
+  int nblocks;   // (P/8)^3
+  int P;         // padded power-of-two cells per axis
+};
+hipError_t launch_mc(const u32* C, int R, const float* bmin3, float cell, const MCBuffers& m, hipStream_t s,
+                     hipEvent_t* ev);
+hipError_t launch_mc_emit(const u32* C, int R, const float* bmin3, float cell, const MCBuffers& m,
+                          hipStream_t s, hipEvent_t* ev);
+
+// ---- reference-format max-mipmap (gm_pyramid.hip)
+hipError_t launch_pyramid(const u32* rows, int R, const RadiusField& rf, u64* levels, hipStream_t s);
+
+}  // namespace gm

@@ -1,0 +1,198 @@
+// Voxelizer: triangle mesh -> bit grid at N = 2R fine voxels per axis.
+//
+// Restates Voxelizer::VoxelizeConservative (Voxelizer.cu:724-796) without
+// OpenGL. The reference draws the mesh twice into an N x N x N/32 image:
+//  * volume pass (vol_vox.vert, vol_vox.geo, vol_vox_bit.frag:35-52):
+//    orthographic along z; every pixel centre a triangle covers XORs the
+//    column of voxels at or below the triangle's depth there -> parity fill;
+//  * surface pass (surf_vox_conserv.geo:41-122, surf_vox_conserv_bit.frag:35-47):
+//    each triangle whose squared aspect ratio is below 50^2 is rasterised
+//    conservatively on its dominant plane, and the voxel under the plane's
+//    depth at each touched pixel centre is set.
+// Rasterisation here is exact integer arithmetic on vertices snapped to
+// 1/256 voxel: pixel centres use a top-left tie rule, conservative coverage
+// is the closed triangle/pixel-square overlap, depths are floored plane
+// values. The parity pass writes one toggle bit per covered pixel at the
+// triangle's depth; a suffix XOR along z then yields the filled columns.
+#include "gm_internal.h"
+
+namespace gm {
+
+namespace {
+
+typedef long long i64;
+
+__device__ __forceinline__ i64 fdiv(i64 a, i64 b) {  // floor(a/b), b > 0
+  i64 q = a / b;
+  if ((a % b != 0) && (a < 0)) --q;
+  return q;
+}
+
+__global__ void k_snap(const float* __restrict__ P, int nv, float bx, float by, float bz, double sc,
+                       int32_t* __restrict__ X) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const float b[3] = {bx, by, bz};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    double v = __dmul_rn(__dsub_rn((double)P[3 * i + a], (double)b[a]), sc);
+    X[3 * i + a] = (int32_t)floor(__dadd_rn(__dmul_rn(v, 256.0), 0.5));
+  }
+}
+
+__device__ __forceinline__ void toggle(u32* g, int N, int W, int x, int y, int z) {
+  atomicXor(&g[((size_t)z * N + y) * W + (x >> 5)], 1u << (x & 31));
+}
+__device__ __forceinline__ void setbit(u32* g, int N, int W, int x, int y, int z) {
+  atomicOr(&g[((size_t)z * N + y) * W + (x >> 5)], 1u << (x & 31));
+}
+
+__global__ void k_vox_tris(const int32_t* __restrict__ X, const u32* __restrict__ T, int nt, int N,
+                           u32* __restrict__ mark, u32* __restrict__ shell) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const i64 S = 256;
+  const int W = N / 32;
+  i64 A[3], B[3], C[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    A[a] = X[3 * (size_t)T[3 * (size_t)t] + a];
+    B[a] = X[3 * (size_t)T[3 * (size_t)t + 1] + a];
+    C[a] = X[3 * (size_t)T[3 * (size_t)t + 2] + a];
+  }
+  // ---- volume pass
+  {
+    i64 a0 = A[0], a1 = A[1], a2 = A[2];
+    i64 b0 = B[0], b1 = B[1], b2 = B[2];
+    i64 c0 = C[0], c1 = C[1], c2 = C[2];
+    i64 area = (b0 - a0) * (c1 - a1) - (b1 - a1) * (c0 - a0);
+    if (area != 0) {
+      if (area < 0) {
+        i64 t0 = b0, t1 = b1, t2 = b2;
+        b0 = c0; b1 = c1; b2 = c2;
+        c0 = t0; c1 = t1; c2 = t2;
+        area = -area;
+      }
+      i64 xmn = min(a0, min(b0, c0)), xmx = max(a0, max(b0, c0));
+      i64 ymn = min(a1, min(b1, c1)), ymx = max(a1, max(b1, c1));
+      i64 i0 = max((i64)0, fdiv(xmn - S / 2 + S - 1, S)), i1 = min((i64)N - 1, fdiv(xmx - S / 2, S));
+      i64 j0 = max((i64)0, fdiv(ymn - S / 2 + S - 1, S)), j1 = min((i64)N - 1, fdiv(ymx - S / 2, S));
+      const i64 vx[3] = {a0, b0, c0}, vy[3] = {a1, b1, c1}, vz[3] = {a2, b2, c2};
+      for (i64 j = j0; j <= j1; ++j)
+        for (i64 i = i0; i <= i1; ++i) {
+          i64 px = i * S + S / 2, py = j * S + S / 2;
+          i64 w[3];
+          bool in = true;
+#pragma unroll
+          for (int e = 0; e < 3; ++e) {
+            int e1 = e == 2 ? 0 : e + 1;
+            i64 ex = vx[e1] - vx[e], ey = vy[e1] - vy[e];
+            i64 f = ex * (py - vy[e]) - ey * (px - vx[e]);
+            bool tl = (ey < 0) || (ey == 0 && ex < 0);
+            if (f < 0 || (f == 0 && !tl)) in = false;
+            w[e == 0 ? 2 : e - 1] = f;
+          }
+          if (!in) continue;
+          i64 num = w[0] * vz[0] + w[1] * vz[1] + w[2] * vz[2];
+          i64 zc = fdiv(num, area * S);
+          if (zc < 0) continue;
+          if (zc > N - 1) zc = N - 1;
+          toggle(mark, N, W, (int)i, (int)j, (int)zc);
+        }
+    }
+  }
+  // ---- surface pass
+  {
+    i64 e1[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+    i64 e2[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+    i64 n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    typedef unsigned __int128 u128;
+    u128 a2 = (u128)((__int128)n[0] * n[0]) + (u128)((__int128)n[1] * n[1]) + (u128)((__int128)n[2] * n[2]);
+    if (a2 == 0) return;
+    i64 d0[3] = {A[0] - B[0], A[1] - B[1], A[2] - B[2]};
+    i64 d1[3] = {B[0] - C[0], B[1] - C[1], B[2] - C[2]};
+    i64 l0 = d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2];
+    i64 l1 = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2];
+    i64 l2 = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
+    u128 ml = (u128)max(l0, max(l1, l2));
+    if (!(ml * ml < (u128)2500 * a2)) return;
+    i64 an0 = n[0] < 0 ? -n[0] : n[0], an1 = n[1] < 0 ? -n[1] : n[1], an2 = n[2] < 0 ? -n[2] : n[2];
+    i64 mxn = max(an0, max(an1, an2));
+    int ax = an0 == mxn ? 0 : (an1 == mxn ? 1 : 2);
+    int ua = ax == 0 ? 1 : (ax == 1 ? 2 : 0);
+    int va = ax == 0 ? 2 : (ax == 1 ? 0 : 1);
+    i64 U[3] = {A[ua], B[ua], C[ua]}, Vv[3] = {A[va], B[va], C[va]};
+    i64 area = (U[1] - U[0]) * (Vv[2] - Vv[0]) - (Vv[1] - Vv[0]) * (U[2] - U[0]);
+    if (area < 0) {
+      i64 t0 = U[1], t1 = Vv[1];
+      U[1] = U[2]; Vv[1] = Vv[2];
+      U[2] = t0; Vv[2] = t1;
+    }
+    i64 umn = min(U[0], min(U[1], U[2])), umx = max(U[0], max(U[1], U[2]));
+    i64 vmn = min(Vv[0], min(Vv[1], Vv[2])), vmx = max(Vv[0], max(Vv[1], Vv[2]));
+    i64 iu0 = max((i64)0, fdiv(umn - 1, S)), iu1 = min((i64)N - 1, fdiv(umx, S));
+    i64 iv0 = max((i64)0, fdiv(vmn - 1, S)), iv1 = min((i64)N - 1, fdiv(vmx, S));
+    i64 nd = n[ax], nu = n[ua], nvv = n[va];
+    i64 Ad = A[ax], Au = A[ua], Av = A[va];
+    for (i64 iv = iv0; iv <= iv1; ++iv)
+      for (i64 iu = iu0; iu <= iu1; ++iu) {
+        i64 ulo = iu * S, uhi = ulo + S, vlo = iv * S, vhi = vlo + S;
+        bool ok = true;
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+          int e1i = e == 2 ? 0 : e + 1;
+          i64 eu = U[e1i] - U[e], ev = Vv[e1i] - Vv[e];
+          i64 pu = (-ev > 0) ? uhi : ulo;
+          i64 pv = (eu > 0) ? vhi : vlo;
+          i64 f = eu * (pv - Vv[e]) - ev * (pu - U[e]);
+          if (f < 0) ok = false;
+        }
+        if (!ok) continue;
+        i64 pu = ulo + S / 2, pv = vlo + S / 2;
+        i64 num = Ad * nd - (nu * (pu - Au) + nvv * (pv - Av));
+        i64 q = nd > 0 ? fdiv(num, nd * S) : fdiv(-num, -nd * S);
+        if (q < 0 || q >= N) continue;
+        int c3[3];
+        c3[ua] = (int)iu;
+        c3[va] = (int)iv;
+        c3[ax] = (int)q;
+        setbit(shell, N, W, c3[0], c3[1], c3[2]);
+      }
+  }
+}
+
+// Suffix XOR along z per (row y, word wx) column, then OR with the shell.
+__global__ void k_vox_fill(const u32* __restrict__ mark, const u32* __restrict__ shell, u32* __restrict__ rows,
+                           int N) {
+  const int W = N / 32;
+  int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= N * W) return;
+  size_t col = (size_t)gid;  // = y*W + wx
+  size_t plane = (size_t)N * W;
+  u32 acc = 0;
+  for (int z = N - 1; z >= 0; --z) {
+    size_t i = (size_t)z * plane + col;
+    acc ^= mark[i];
+    rows[i] = acc | shell[i];
+  }
+}
+
+}  // namespace
+
+hipError_t launch_voxelize(const float* P, int nv, const u32* T, int nt, const float* bmin3, double scale,
+                           int R, int32_t* snap, u32* mark, u32* shell, u32* rows, hipStream_t s) {
+  const int N = 2 * R, W = N / 32;
+  size_t words = (size_t)N * N * W;
+  GM_CHECK(hipMemsetAsync(mark, 0, words * 4, s));
+  GM_CHECK(hipMemsetAsync(shell, 0, words * 4, s));
+  if (nv > 0)
+    hipLaunchKernelGGL(k_snap, dim3((nv + 255) / 256), dim3(256), 0, s, P, nv, bmin3[0], bmin3[1], bmin3[2],
+                       scale, snap);
+  GM_CHECK(hipGetLastError());
+  if (nt > 0) hipLaunchKernelGGL(k_vox_tris, dim3((nt + 127) / 128), dim3(128), 0, s, snap, T, nt, N, mark, shell);
+  GM_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_vox_fill, dim3((N * W + 255) / 256), dim3(256), 0, s, mark, shell, rows, N);
+  return hipGetLastError();
+}
+
+}  // namespace gm

@@ -1,0 +1,136 @@
+/* GMorpho closing path on MI355X: C-ABI boundary of libgmorpho_hip.so.
+ *
+ * Each entry point replaces a method of the reference's C++ classes
+ * (MorphoGraphics::GMorpho, Grid, Voxelizer, ScaleField, MarchingCubesMesher);
+ * the reference file:line it stands for is given beside it. The C++ facade in
+ * include/gmorpho.hpp wraps these calls under the reference's class and method
+ * names.
+ *
+ * Conventions
+ *  - Every function returns GM_OK (0) or a GM_ERR_* code; gm_last_error()
+ *    gives the message ("GMorpho Error: ..." as GMorpho::Exception formats it,
+ *    GMorpho.h:52). No C++ exception crosses this boundary.
+ *  - Host pointers are borrowed for the duration of the call only. Device
+ *    memory belongs to the context and is released by gm_destroy.
+ *  - One context = one HIP stream on one device. Calls are synchronous with
+ *    respect to the host unless noted ("enqueue"); contexts are
+ *    thread-compatible, not thread-safe.
+ *  - Grids cross the boundary in the reference's cell-byte layout: R^3 bytes,
+ *    index x + R*(y + R*z), bit k = fine voxel (2x+(k&1), 2y+((k>>1)&1),
+ *    2z+((k>>2)&1)) (Grid.h:61-68). N = 2R fine voxels per axis; R must be a
+ *    multiple of 16 (16 <= R <= 1024).
+ *  - Radii are in fine voxels.
+ */
+#ifndef GMORPHO_C_H
+#define GMORPHO_C_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gm_ctx gm_ctx;
+
+enum {
+  GM_OK = 0,
+  GM_ERR_ARG = 1,         /* invalid argument */
+  GM_ERR_STATE = 2,       /* call out of order (e.g. no grid loaded) */
+  GM_ERR_HIP = 3,         /* HIP runtime failure */
+  GM_ERR_CAPACITY = 4,    /* output does not fit the caller's limits */
+  GM_ERR_UNSUPPORTED = 5  /* parameter outside what this build supports */
+};
+
+/* Grids owned by a context (GMorpho.h:138-141). */
+enum { GM_GRID_INPUT = 0, GM_GRID_DILATION = 1, GM_GRID_CONTOUR = 2, GM_GRID_CLOSING = 3 };
+/* Structuring element of the dilation (use_asymmetric_closing_, GMorpho.h:69). */
+enum { GM_SE_SPHERE = 0, GM_SE_CUBE = 1 };
+
+/* GMorpho::GMorpho / ~GMorpho (GMorpho.cu:76-101). */
+int gm_create(int device, gm_ctx** out);
+int gm_destroy(gm_ctx* ctx);
+/* Message of the last failing call on ctx ("" if none). */
+const char* gm_last_error(const gm_ctx* ctx);
+
+/* GMorpho::Load (GMorpho.cu:7731-7773, GMorpho.h:76-78): copy the mesh,
+ * voxelize it at base_res cells per axis with margin se_size + 1/base_res,
+ * and set up the grids. N (normals) may be NULL; it is not used. */
+int gm_load(gm_ctx* ctx, const float* P, const float* N, int num_of_vertices, const uint32_t* T,
+            int num_of_faces, int base_res, float se_size);
+/* Voxelizer::Load (Voxelizer.cu:74-102). */
+int gm_load_mesh(gm_ctx* ctx, const float* P, const float* N, int num_of_vertices, const uint32_t* T,
+                 int num_of_faces);
+/* Voxelizer::VoxelizeConservative (Voxelizer.cu:724-796) with
+ * ComputeGridAttributes (431-455): fills GM_GRID_INPUT. */
+int gm_voxelize(gm_ctx* ctx, int base_res, float margin);
+
+/* Grid::Init (Grid.cu:107-131) for a caller-supplied input grid: bounding
+ * box, base resolution (cells per axis), data resolution (cells of the data
+ * region; coarse passes only visit cells below it) and cell size. */
+int gm_grid_init(gm_ctx* ctx, const float bbox_min[3], const float bbox_max[3], int base_res,
+                 const int data_res[3], float cell_size);
+/* Grid::CopyVoxelsFrom / voxels() transfer (Grid.cu:139-151): cell bytes in/out. */
+int gm_grid_set_cells(gm_ctx* ctx, int which, const uint8_t* cells);
+int gm_grid_get_cells(gm_ctx* ctx, int which, uint8_t* cells);
+/* GMorpho::bbox() / res() / cell_size() (GMorpho.h:63-65) and Grid::data_res(). */
+int gm_grid_info(const gm_ctx* ctx, float bbox_min[3], float bbox_max[3], int res[3], int data_res[3],
+                 float* cell_size);
+
+/* ScaleField (ScaleField.h:33-140). Per-voxel radius r(p) in fine voxels, as
+ * the max-mipmap level 0 stores it (Grid.cu:957-998):
+ *  - uniform: every voxel has radius r (ScaleField::UpdateGridGlobalScale,
+ *    ScaleField.cu:143; the fixed-size path of Grid.cu:725). r <= 1 makes
+ *    gm_close skip the morphology (GMorpho.cu:7784, 7803).
+ *  - int8 grid: N^3 signed chars (ScaleField::scale_grid(), x fastest),
+ *    clamped to [2,127] per voxel.
+ *  - FP32 lattice: L^3 values (x fastest), N/L a power of two; node i sits on
+ *    fine voxel i*N/L; values are rounded to 1/256, sampled trilinearly in
+ *    fixed point with clamped borders, floored, then clamped to [2,127]. */
+int gm_scale_uniform(gm_ctx* ctx, int r_fine);
+int gm_scale_grid_i8(gm_ctx* ctx, const int8_t* grid);
+int gm_scale_lattice_f32(gm_ctx* ctx, const float* lattice, int L);
+
+/* GMorpho::DilateBySphereMipmap / DilateByCubeMipmap (GMorpho.cu:3538, 3231):
+ * GM_GRID_DILATION from GM_GRID_INPUT. */
+int gm_dilate(gm_ctx* ctx, int se);
+/* GMorpho::ErodeBySphereMipmap (GMorpho.cu:6627): GM_GRID_CONTOUR and
+ * GM_GRID_CLOSING from GM_GRID_DILATION. */
+int gm_erode(gm_ctx* ctx);
+/* The morphology of GMorpho::Update (GMorpho.cu:7775-7806): dilate then
+ * erode, both skipped for a uniform radius <= 1 (closing = input). */
+int gm_close(gm_ctx* ctx, int se);
+/* GMorpho::ExtractClosingMesh (GMorpho.cu:6814-7001) on GM_GRID_CLOSING:
+ * vertices, area-weighted unit normals and triangles stay on the device.
+ * Enqueue only; gm_mesh_counts waits for it. */
+int gm_extract_mesh(gm_ctx* ctx);
+/* Sizes of the extracted mesh (waits for the extraction). */
+int gm_mesh_counts(gm_ctx* ctx, uint32_t* num_vertices, uint32_t* num_triangles);
+/* Copy the mesh out (GMorpho.cu:6985-6995): V, Nrm float[3*nv], T uint32[3*nt]. */
+int gm_mesh_copy(gm_ctx* ctx, float* V, float* Nrm, uint32_t* T);
+
+/* Grid::BuildTexMipmaps (Grid.cu:1000-1154) of grid `which` with the current
+ * radius field: levels 0..log2(R), level L = (R>>L)^3 64-bit texels (x
+ * fastest) concatenated. R must be a power of two. */
+int gm_build_pyramid(gm_ctx* ctx, int which);
+int gm_pyramid_size(const gm_ctx* ctx, int* levels, uint64_t* texels);
+int gm_pyramid_get(gm_ctx* ctx, uint64_t* texels);
+
+/* Per-stage device timings of the last gm_close / gm_extract_mesh, in ms,
+ * from HIP events on the context stream (the reference prints GET_TIME
+ * brackets, GMorpho.cu:7796-7815). Stage order: dilate coarse, tight contour,
+ * dilate fine, contour K, erode coarse, base contour, erode fine, mc count+scan,
+ * mc vertices+triangles, mc normals. Timing is off unless enabled. */
+int gm_set_timing(gm_ctx* ctx, int enable);
+int gm_stage_times(gm_ctx* ctx, float* ms, int n);
+/* Wait for all work enqueued on the context. */
+int gm_sync(gm_ctx* ctx);
+/* The context's HIP stream (hipStream_t), for callers that enqueue around it. */
+int gm_get_stream(gm_ctx* ctx, void** stream);
+
+/* The marching-cubes tables in use: per case the counted triangles, the listed
+ * triangles and the packed edge ids (nibble 3q+i = edge i of triangle q, 0xf
+ * terminates). Host only. */
+int gm_mc_tables(uint32_t* ntri256, uint32_t* len256, uint64_t* tri256);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,405 @@
+"""CPU tests: the oracle pinned against brute force, closed forms, invariants
+and the reference's fixtures; host-side generators; the C-ABI export list."""
+import os
+import re
+from collections import Counter
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import broxy_amd as B
+from _util import cells_from_fine, fine_grid, random_blobs, random_sparse, set_fine
+
+REF = "/root/reference"
+
+
+def morton(x, y, z):
+    m = 0
+    for i in range(10):
+        m |= ((x >> i) & 1) << (3 * i) | ((y >> i) & 1) << (3 * i + 1) | ((z >> i) & 1) << (3 * i + 2)
+    return m
+
+
+# ---------------------------------------------------------------- dilation
+@pytest.mark.parametrize("seed", range(4))
+def test_dilate_dfs_equals_brute(seed):
+    R = 16
+    rng = np.random.default_rng(seed)
+    I = random_sparse(R, seed, density=0.01, margin=0)
+    r = int(rng.integers(2, 7))
+    assert np.array_equal(O.dilate(I, R, r=r), O.dilate(I, R, r=r, brute=1))
+    assert np.array_equal(O.dilate(I, R, r=r, se=1), O.dilate(I, R, r=r, se=1, brute=1))
+    sc = rng.integers(-4, 9, (2 * R) ** 3).astype(np.int8)
+    assert np.array_equal(O.dilate(I, R, scale=sc), O.dilate(I, R, scale=sc, brute=1))
+
+
+@pytest.mark.parametrize("r", [2, 3, 4, 6])
+def test_single_voxel_sphere_is_digital_ball(r):
+    # SqDist(p, [q,q+1]^3) < (r+1)^2 per axis distance d(t) = t (t>=0), -t-1 (t<0)
+    R = 16
+    q = (15, 16, 15)
+    I = np.zeros(R ** 3, np.uint8)
+    set_fine(I, R, *q)
+    D = fine_grid(O.dilate(I, R, r=r), R)
+    N = 2 * R
+    z, y, x = np.mgrid[0:N, 0:N, 0:N]
+
+    def d(t):
+        return np.where(t >= 0, t, -t - 1)
+
+    ball = d(q[0] - x) ** 2 + d(q[1] - y) ** 2 + d(q[2] - z) ** 2 < (r + 1) ** 2
+    assert np.array_equal(D, ball)
+    # count closed form: 8 * #{a,b,c >= 0 : a^2+b^2+c^2 < (r+1)^2}
+    cnt = sum(1 for a in range(r + 1) for b in range(r + 1) for c in range(r + 1) if a * a + b * b + c * c < (r + 1) ** 2)
+    assert D.sum() == 8 * cnt
+
+
+@pytest.mark.parametrize("r", [2, 3, 5])
+def test_single_voxel_cube_is_box(r):
+    # closed box test: q in [p-r-1, p+r]  <=>  p in [q-r, q+r+1]
+    R = 16
+    q = (14, 17, 15)
+    I = np.zeros(R ** 3, np.uint8)
+    set_fine(I, R, *q)
+    D = fine_grid(O.dilate(I, R, r=r, se=1), R)
+    box = np.zeros_like(D)
+    box[q[2] - r:q[2] + r + 2, q[1] - r:q[1] + r + 2, q[0] - r:q[0] + r + 2] = True
+    assert np.array_equal(D, box)
+
+
+def test_dilation_contains_input_and_respects_data_res():
+    R = 24
+    I = random_blobs(R, seed=3)
+    dr = np.array([R - 4, R, R - 8], np.int32)
+    D = O.dilate(I, R, r=4, data_res=dr)
+    assert np.all((I & ~D) == 0)
+    # coarse fills only happen inside the data region: a cell beyond it is
+    # either untouched or a tight-contour cell refined voxel by voxel
+    Dfull = O.dilate(I, R, r=4)
+    assert np.all((D & ~Dfull) == 0)
+
+
+# ---------------------------------------------------------------- erosion
+@pytest.mark.parametrize("seed", range(3))
+def test_erode_dfs_equals_brute(seed):
+    R = 16
+    rng = np.random.default_rng(100 + seed)
+    I = random_sparse(R, seed + 50, density=0.02, margin=1)
+    r = int(rng.integers(2, 6))
+    D = O.dilate(I, R, r=r)
+    assert np.array_equal(O.erode(I, D, R, r=r), O.erode(I, D, R, r=r, brute=1))
+    sc = rng.integers(-2, 7, (2 * R) ** 3).astype(np.int8)
+    D2 = O.dilate(I, R, scale=sc)
+    assert np.array_equal(O.erode(I, D2, R, scale=sc), O.erode(I, D2, R, scale=sc, brute=1))
+
+
+def test_contour_K_definition():
+    R = 16
+    D = random_blobs(R, seed=9, rmin=3, rmax=7, margin=2)
+    K = fine_grid(O.contour_K(D, R), R)
+    f = fine_grid(D, R)
+    N = 2 * R
+    nb = np.zeros_like(f)
+    nb[1:] |= f[:-1]
+    nb[:-1] |= f[1:]
+    nb[:, 1:] |= f[:, :-1]
+    nb[:, :-1] |= f[:, 1:]
+    nb[:, :, 1:] |= f[:, :, :-1]
+    nb[:, :, :-1] |= f[:, :, 1:]
+    cellfull = np.repeat(np.repeat(np.repeat((D.reshape(R, R, R) == 255), 2, 0), 2, 1), 2, 2)
+    inner = np.zeros_like(f)
+    inner[2:N - 2, 2:N - 2, 2:N - 2] = True
+    assert np.array_equal(K, ~f & nb & ~cellfull & inner)
+
+
+def test_erosion_of_dilation_keeps_input():
+    # closing is extensive away from the grid border: every input voxel stays
+    R = 24
+    I = random_blobs(R, seed=4, margin=8)
+    for r in (2, 4):
+        D = O.dilate(I, R, r=r)
+        C = O.erode(I, D, R, r=r)
+        assert np.all((I & ~C) == 0)
+
+
+def test_closing_of_ball_bounds_it():
+    # The reference's closing dilates with r+1 and erodes with r-1 (the proxy
+    # must bound the shape): a ball comes back grown by about two voxels,
+    # never smaller, and inside the dilation.
+    R = 24
+    N = 2 * R
+    z, y, x = np.mgrid[0:N, 0:N, 0:N]
+    d2 = (x - 23.5) ** 2 + (y - 24.5) ** 2 + (z - 23.5) ** 2
+    ball = d2 < 10.3 ** 2
+    I = cells_from_fine(ball)
+    for r in (2, 3, 4):
+        D = O.dilate(I, R, r=r)
+        C = fine_grid(O.erode(I, D, R, r=r), R)
+        assert np.all(C[ball])
+        assert not np.any(C & ~fine_grid(D, R))
+        assert not np.any(C & (d2 >= (10.3 + 3.5) ** 2))
+
+
+# ---------------------------------------------------------------- contours / order
+def test_mcm_list_is_morton_sorted_and_complete():
+    R = 16
+    C = random_blobs(R, seed=2, margin=2)
+    lst = O.contour_list(C, R, 2)
+    xyz = [(c % R, (c // R) % R, c // (R * R)) for c in lst]
+    codes = [morton(*p) for p in xyz]
+    assert codes == sorted(codes) and len(set(codes)) == len(codes)
+    f = fine_grid(C, R)
+    expect = set()
+    for cz in range(1, R - 1):
+        for cy in range(1, R - 1):
+            for cx in range(1, R - 1):
+                blk = f[2 * cz:2 * cz + 3, 2 * cy:2 * cy + 3, 2 * cx:2 * cx + 3]
+                for j in range(8):
+                    cube = blk[(j >> 2) & 1:(j >> 2) + 2, (j >> 1) & 1:((j >> 1) & 1) + 2, j & 1:(j & 1) + 2]
+                    if cube.any() and not cube.all():
+                        expect.add(cx + R * (cy + R * cz))
+                        break
+    assert set(lst.tolist()) == expect
+
+
+# ---------------------------------------------------------------- marching cubes
+def test_mc_tables():
+    e, n, t = O.mc_tables()
+    corners_edges = [(0, 1), (1, 2), (2, 3), (3, 0), (4, 5), (5, 6), (6, 7), (7, 4), (0, 4), (1, 5), (2, 6), (3, 7)]
+    for c in range(256):
+        m = sum(1 << k for k, (a, b) in enumerate(corners_edges) if ((c >> a) & 1) != ((c >> b) & 1))
+        assert e[c] == m
+        listed = int((t[c] != -1).sum())
+        assert listed % 3 == 0
+        # every triangle uses crossing edges only
+        assert all((e[c] >> int(k)) & 1 for k in t[c][:listed])
+        # every crossing edge is used by some triangle
+        assert set(int(k) for k in t[c][:listed]) == {k for k in range(12) if (e[c] >> k) & 1}
+        if c != 91:
+            assert n[c] == listed // 3
+    assert n[91] == 4 and (t[91] != -1).sum() == 15
+    ntri, ln, tri = B.mc_tables()
+    for c in range(256):
+        listed = [int(x) for x in t[c] if x != -1]
+        packed = [(int(tri[c]) >> (4 * i)) & 0xF for i in range(len(listed))]
+        assert packed == listed and ln[c] == len(listed) // 3 and ntri[c] == n[c]
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "GMorpho/tables.h")), reason="reference tree absent")
+def test_mc_tables_equal_reference_text():
+    src = open(os.path.join(REF, "GMorpho/tables.h")).read()
+
+    def arr(name):
+        i = src.index(name)
+        j = src.index("{", i)
+        k = src.index("}", j)
+        return [int(x, 0) for x in re.findall(r"-?0x[0-9a-fA-F]+|-?\d+", src[j + 1:k])]
+
+    e, n, t = O.mc_tables()
+    assert arr("edgeTable") == [int(x) for x in e]
+    assert arr("numTriTable") == [int(x) for x in n]
+    assert arr("triTable") == [int(x) for x in t.reshape(-1)]
+
+
+def _mesh_checks(V, T):
+    assert T.max() < len(V)
+    E = Counter()
+    for a, b, c in T:
+        for u, v in ((a, b), (b, c), (c, a)):
+            E[(min(u, v), max(u, v))] += 1
+    return E
+
+
+def test_mc_closed_manifold_euler():
+    R = 24
+    N = 2 * R
+    z, y, x = np.mgrid[0:N, 0:N, 0:N]
+    ball = (x - 23.5) ** 2 + (y - 24.5) ** 2 + (z - 23.5) ** 2 < 13.0 ** 2
+    C = cells_from_fine(ball)
+    V, Nn, T = O.mc(C, R, np.zeros(3, np.float32), 1.0)
+    E = _mesh_checks(V, T)
+    assert all(v == 2 for v in E.values())
+    assert len(V) - len(E) + len(T) == 2
+    # every vertex sits at 0.05 or 0.95 along its fine edge (h = 0.5)
+    frac = (V.astype(np.float64) / 0.5) % 1.0
+    dist = np.min(np.abs(frac[..., None] - np.array([0.0, 0.05, 0.95, 1.0])), axis=-1)
+    assert np.all(dist < 1e-5)
+    assert np.all((np.abs(frac - 0.05) < 1e-5).sum(1) + (np.abs(frac - 0.95) < 1e-5).sum(1) == 1)
+    # outward normals: the normal points away from the ball centre
+    c = np.array([24.0, 25.0, 24.0]) * 0.5
+    assert np.mean(np.einsum("ij,ij->i", Nn, V - c) > 0) > 0.99
+    assert np.allclose(np.linalg.norm(Nn, axis=1), 1.0, atol=1e-5)
+
+
+def test_mc_single_voxel():
+    # one occupied fine voxel -> an octahedron-like closed surface: 8 cubes x 1 triangle
+    R = 16
+    I = np.zeros(R ** 3, np.uint8)
+    set_fine(I, R, 15, 16, 15)
+    V, Nn, T = O.mc(I, R, np.zeros(3, np.float32), 1.0)
+    assert len(T) == 8 and len(V) == 6
+    E = _mesh_checks(V, T)
+    assert all(v == 2 for v in E.values())
+
+
+def test_mc_vertex_count_closed_form():
+    R = 16
+    C = random_blobs(R, seed=12, margin=3)
+    nv, nt = O.mc_counts(C, R)
+    e, n, t = O.mc_tables()
+    f = fine_grid(C, R)
+    N = 2 * R
+    v = t_ = 0
+    for cz in range(1, R - 1):
+        for cy in range(1, R - 1):
+            for cx in range(1, R - 1):
+                for j in range(8):
+                    fx, fy, fz = 2 * cx + (j & 1), 2 * cy + ((j >> 1) & 1), 2 * cz + ((j >> 2) & 1)
+                    cube = [f[fz + dz, fy + dy, fx + dx] for dx, dy, dz in
+                            [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]]
+                    case = sum(int(b) << k for k, b in enumerate(cube))
+                    if case in (0, 255):
+                        continue
+                    v += bin(int(e[case]) & 0x109).count("1")
+                    t_ += int(n[case])
+    assert (nv, nt) == (v, t_)
+
+
+# ---------------------------------------------------------------- voxelizer
+def _snap(P, bmin, maxext, N):
+    v = (P.astype(np.float64) - bmin.astype(np.float64)) * (N / np.float64(maxext))
+    return np.floor(v * 256.0 + 0.5).astype(np.int64)
+
+
+def test_voxelize_convex_parity_matches_inside_test():
+    # For a closed convex mesh the parity fill sets voxel (i,j,k) iff the
+    # column through the pixel centre enters below k and leaves at or above
+    # it: checked against half-space tests on the snapped triangles.
+    P, T = B.icosphere(2)
+    P = (P * 0.4 + 0.5).astype(np.float32)
+    R = 16
+    N = 2 * R
+    bmin, bmax, cell, dr = O.grid_attributes(P, R, 0.05)
+    maxext = float((bmax - bmin).max())
+    I = fine_grid(O.voxelize(P, T, bmin, maxext, R), R)
+    X = _snap(P, bmin, maxext, N)
+    A, Bv, C = X[T[:, 0]], X[T[:, 1]], X[T[:, 2]]
+    n = np.cross(Bv - A, C - A)
+    inside = np.zeros((N, N, N), bool)
+    for k in range(N):
+        for j in range(N):
+            for i in range(N):
+                p = np.array([256 * i + 128, 256 * j + 128, 256 * k])
+                s = np.einsum("ij,ij->i", n, p - A)
+                inside[k, j, i] = np.all(s < 0) or (np.all(s <= 0) and np.any((s == 0) & (n[:, 2] > 0)))
+    # the conservative shell may add voxels, but parity voxels are exactly `inside`
+    assert np.all(I[inside])
+    extra = I & ~inside
+    # every extra voxel is a shell voxel: within 2 voxels of the surface
+    kk, jj, ii = np.nonzero(extra)
+    c = (np.array([0.5, 0.5, 0.5]) - bmin) * (N / maxext)
+    rad = 0.4 * (N / maxext)
+    d = np.sqrt((ii + 0.5 - c[0]) ** 2 + (jj + 0.5 - c[1]) ** 2 + (kk + 0.5 - c[2]) ** 2)
+    assert np.all(np.abs(d - rad) < 2.0)
+
+
+def test_voxelize_solid_soup_is_filled():
+    P, T = B.sphere_soup(4, 3, seed=5, rmin=0.08, rmax=0.12, cmin=0.3, cmax=0.7)
+    R = 32
+    bmin, bmax, cell, dr = O.grid_attributes(P, R, 0.3 + 1.0 / R)
+    I = O.voxelize(P, T, bmin, float((bmax - bmin).max()), R)
+    assert fine_grid(I, R).sum() > 1000
+    # voxelisation is deterministic and independent of triangle order
+    perm = np.random.default_rng(0).permutation(len(T))
+    I2 = O.voxelize(P, T[perm], bmin, float((bmax - bmin).max()), R)
+    assert np.array_equal(I, I2)
+
+
+def test_grid_attributes_match_reference_formula():
+    P, T = B.sh_blob(2, seed=3)
+    R = 64
+    margin = np.float32(0.3) + np.float32(2.0) / np.float32(2 * R)
+    bmin, bmax, cell, dr = O.grid_attributes(P, R, float(margin))
+    mn, mx = P.min(0), P.max(0)
+    thick = np.float32(np.max(np.abs(mx - mn)) / np.float32(R))
+    off = np.float32(np.float32(2) * thick + margin)
+    assert np.array_equal(bmin, (mn - off).astype(np.float32))
+    assert np.array_equal(bmax, (mx + off).astype(np.float32))
+    cs = np.float32(np.max(np.abs(bmax - bmin)) / np.float32(R))
+    assert np.float32(cell) == cs
+    assert list(dr) == [int(np.ceil(np.float32((bmax[a] - bmin[a]) / cs))) for a in range(3)]
+
+
+# ---------------------------------------------------------------- ScaleField lattice
+def test_lattice_constant_and_ramp():
+    L, N = 4, 32
+    lat = np.full((L, L, L), 7.75, np.float32)
+    assert np.all(O.scale_from_lattice(lat, N) == 7)
+    # trilinear interpolation reproduces a linear ramp exactly
+    s = N // L
+    ramp = np.zeros((L, L, L), np.float32)
+    for i in range(L):
+        ramp[:, :, i] = 3 + 2 * i
+    out = O.scale_from_lattice(ramp, N).reshape(N, N, N)
+    for x in range(N):
+        xc = min(x, s * (L - 1))  # clamped beyond the last node
+        exp = (256 * 3 * s + 512 * xc) // (256 * s) if x < s * (L - 1) else 3 + 2 * (L - 1)
+        assert np.all(out[:, :, x] == exp), x
+
+
+# ---------------------------------------------------------------- pyramid
+def test_pyramid_levels():
+    R = 16
+    I = random_blobs(R, seed=1, rmin=1.5, rmax=3.0, margin=2)
+    pyr = O.pyramid(I, R, r=5)
+    lv0 = pyr[:R ** 3]
+    assert np.all((lv0 >> np.uint64(56)).astype(np.uint8) == I)
+    assert np.all((lv0 & np.uint64(0x7F)) == 5)
+    top = pyr[-1]
+    nonempty = [(I.reshape(R, R, R)[(l >> 2) * 8:(l >> 2) * 8 + 8, ((l >> 1) & 1) * 8:((l >> 1) & 1) * 8 + 8,
+                 (l & 1) * 8:(l & 1) * 8 + 8] != 0).any() for l in range(8)]
+    assert [(int(top) >> (56 + l)) & 1 for l in range(8)] == [int(b) for b in nonempty]
+
+
+# ---------------------------------------------------------------- fixtures / generators
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "Bin/Resources/Models/sphere.off")),
+                    reason="reference tree absent")
+def test_icosphere_matches_reference_fixture():
+    lines = open(os.path.join(REF, "Bin/Resources/Models/sphere.off")).read().split("\n")
+    nv, nf = map(int, lines[1].split()[:2])
+    Q = np.array([[float(x) for x in l.split()] for l in lines[2:2 + nv]])
+    P, T = B.icosphere(3)
+    assert (len(P), len(T)) == (nv, nf)
+    d = np.sqrt(((P[:, None, :] - Q[None, :, :]) ** 2).sum(-1)).min(1)
+    assert d.max() < 2e-6
+
+
+def test_generators_deterministic_and_closed():
+    P1, T1 = B.sh_blob(3, seed=42)
+    P2, T2 = B.sh_blob(3, seed=42)
+    assert np.array_equal(P1, P2) and np.array_equal(T1, T2)
+    assert P1.min() >= 0 and np.isclose(P1.max(0).max(), 1.0)
+    E = _mesh_checks(P1, T1)
+    assert all(v == 2 for v in E.values())
+    lat = B.scale_lattice(8, seed=1)
+    assert lat.min() >= 2 and lat.max() <= 24
+
+
+# ---------------------------------------------------------------- C-ABI
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(os.path.dirname(B.HERE), "include", "gmorpho_c.h")).read()
+    names = set(re.findall(r"\b(gm_[a-z0-9_]+)\s*\(", hdr))
+    lib = B.hip_lib()
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing and len(names) >= 25
+    mg = open(os.path.join(os.path.dirname(B.HERE), "include", "gmorpho_meshgen.h")).read()
+    for n in set(re.findall(r"\b(gmg_[a-z0-9_]+)\s*\(", mg)):
+        assert hasattr(B.meshgen_lib(), n)
+
+
+def test_context_without_device_fails_loudly():
+    if os.environ.get("HIP_VISIBLE_DEVICES") is None and os.path.exists("/dev/kfd"):
+        pytest.skip("a HIP device may be present")
+    with pytest.raises(B.GMorphoError):
+        B.Context(0)

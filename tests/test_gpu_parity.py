@@ -1,0 +1,183 @@
+"""HIP path vs the oracle, stage by stage, through the C-ABI (bit-exact grids,
+bit-exact MC index/triangle lists, vertices within 1e-6 relative)."""
+import sys
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import broxy_amd as B
+from _util import random_blobs, random_sparse
+
+pytestmark = pytest.mark.gpu
+
+VERT_RTOL = 1e-6
+NORMAL_ATOL = 2e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def grid_setup(ctx, R, cells, data_res=None):
+    bmin = np.array([-1.0, -0.5, 0.25], np.float32)
+    cell = np.float32(2.0 / R)
+    bmax = bmin + cell * R
+    dr = np.array(data_res if data_res is not None else [R, R, R], np.int32)
+    ctx.grid_init(bmin, bmax, R, dr, float(cell))
+    ctx.set_cells(B.GRID_INPUT, cells)
+    return bmin, float(cell), dr
+
+
+def test_layout_roundtrip(ctx):
+    R = 32
+    rng = np.random.default_rng(0)
+    cells = rng.integers(0, 256, R ** 3, dtype=np.uint8)
+    grid_setup(ctx, R, cells)
+    say("layout roundtrip R=32")
+    assert np.array_equal(ctx.get_cells(B.GRID_INPUT), cells)
+
+
+@pytest.mark.parametrize("r", [2, 3, 5, 8])
+def test_dilate_erode_uniform(ctx, r):
+    R = 32
+    I = random_blobs(R, seed=10 + r)
+    dr = np.array([R, R - 3, R - 5], np.int32)
+    grid_setup(ctx, R, I, dr)
+    ctx.scale_uniform(r)
+    say("dilate/erode uniform r=%d R=%d" % (r, R))
+    ctx.dilate(B.SE_SPHERE)
+    D = ctx.get_cells(B.GRID_DILATION)
+    Dref = O.dilate(I, R, r=r, data_res=dr)
+    assert np.array_equal(D, Dref), "dilation differs in %d cells" % int((D != Dref).sum())
+    ctx.erode()
+    K = ctx.get_cells(B.GRID_CONTOUR)
+    assert np.array_equal(K, O.contour_K(Dref, R))
+    C = ctx.get_cells(B.GRID_CLOSING)
+    Cref = O.erode(I, Dref, R, r=r, data_res=dr)
+    assert np.array_equal(C, Cref), "erosion differs in %d cells" % int((C != Cref).sum())
+
+
+@pytest.mark.parametrize("r", [2, 4, 7])
+def test_sparse_input(ctx, r):
+    R = 32
+    I = random_sparse(R, seed=r, density=0.004)
+    grid_setup(ctx, R, I)
+    ctx.scale_uniform(r)
+    say("sparse input r=%d" % r)
+    ctx.close_grid(B.SE_SPHERE)
+    Dref = O.dilate(I, R, r=r)
+    assert np.array_equal(ctx.get_cells(B.GRID_DILATION), Dref)
+    assert np.array_equal(ctx.get_cells(B.GRID_CLOSING), O.erode(I, Dref, R, r=r))
+
+
+def test_cube_dilation(ctx):
+    R = 32
+    I = random_blobs(R, seed=3)
+    grid_setup(ctx, R, I)
+    for r in (2, 5):
+        ctx.scale_uniform(r)
+        say("cube dilation r=%d" % r)
+        ctx.dilate(B.SE_CUBE)
+        assert np.array_equal(ctx.get_cells(B.GRID_DILATION), O.dilate(I, R, r=r, se=1))
+
+
+def test_scale_grid_i8(ctx):
+    R = 32
+    N = 2 * R
+    I = random_blobs(R, seed=5)
+    grid_setup(ctx, R, I)
+    rng = np.random.default_rng(7)
+    sc = rng.integers(-3, 8, N ** 3).astype(np.int8)
+    ctx.scale_grid(sc)
+    say("per-voxel int8 radii")
+    ctx.close_grid(B.SE_SPHERE)
+    Dref = O.dilate(I, R, scale=sc)
+    assert np.array_equal(ctx.get_cells(B.GRID_DILATION), Dref)
+    assert np.array_equal(ctx.get_cells(B.GRID_CLOSING), O.erode(I, Dref, R, scale=sc))
+
+
+def test_scale_lattice(ctx):
+    R = 32
+    N = 2 * R
+    I = random_blobs(R, seed=6)
+    grid_setup(ctx, R, I)
+    L = 8
+    lat = B.scale_lattice(L, seed=11, lo=2.0, hi=7.0, smooth=1)
+    ctx.scale_lattice(lat, L)
+    sc = O.scale_from_lattice(lat.reshape(L, L, L), N)
+    say("lattice radii")
+    ctx.close_grid(B.SE_SPHERE)
+    Dref = O.dilate(I, R, scale=sc)
+    assert np.array_equal(ctx.get_cells(B.GRID_DILATION), Dref)
+    assert np.array_equal(ctx.get_cells(B.GRID_CLOSING), O.erode(I, Dref, R, scale=sc))
+
+
+def test_voxelize_soup(ctx):
+    P, T = B.sphere_soup(8, 3, seed=99)
+    R = 64
+    say("voxelize soup R=64")
+    ctx.load(P, T, R, 0.3)
+    bmin, bmax, res, dr, cell = ctx.info()
+    rb, rB, rc, rd = O.grid_attributes(P, R, 0.3 + 2.0 / (2 * R))
+    assert np.array_equal(bmin, rb) and np.array_equal(bmax, rB) and cell == np.float32(rc)
+    assert np.array_equal(dr, rd)
+    Iref = O.voxelize(P, T, rb, float((rB - rb).max()), R)
+    I = ctx.get_cells(B.GRID_INPUT)
+    assert np.array_equal(I, Iref), "voxelization differs in %d cells" % int((I != Iref).sum())
+
+
+def _mesh_compare(ctx, C, R, bmin, cell):
+    V, N, T = ctx.mesh()
+    Vr, Nr, Tr = O.mc(C, R, bmin, cell)
+    assert V.shape == Vr.shape and T.shape == Tr.shape
+    assert np.array_equal(T, Tr)
+    assert np.allclose(V, Vr, rtol=VERT_RTOL, atol=0)
+    assert np.allclose(N, Nr, atol=NORMAL_ATOL)
+    return V, T
+
+
+def test_mesh_blobs(ctx):
+    R = 32
+    I = random_blobs(R, seed=21, nballs=8)
+    bmin, cell, dr = grid_setup(ctx, R, I)
+    ctx.scale_uniform(3)
+    ctx.close_grid(B.SE_SPHERE)
+    ctx.extract_mesh()
+    say("mesh of closed blobs R=32")
+    C = ctx.get_cells(B.GRID_CLOSING)
+    _mesh_compare(ctx, C, R, bmin, cell)
+
+
+def test_config0_end_to_end(ctx):
+    P, T = B.sphere_soup(64, 3, seed=1234)
+    R = 64
+    say("config-0 end to end (N=128, r=4)")
+    ctx.load(P, T, R, 0.3)
+    bmin, bmax, res, dr, cell = ctx.info()
+    ctx.scale_uniform(4)
+    ctx.close_grid(B.SE_SPHERE)
+    ctx.extract_mesh()
+    I = ctx.get_cells(B.GRID_INPUT)
+    Iref = O.voxelize(P, T, bmin, float((bmax - bmin).max()), R)
+    assert np.array_equal(I, Iref)
+    Dref = O.dilate(Iref, R, r=4, data_res=dr)
+    assert np.array_equal(ctx.get_cells(B.GRID_DILATION), Dref)
+    Cref = O.erode(Iref, Dref, R, r=4, data_res=dr)
+    assert np.array_equal(ctx.get_cells(B.GRID_CLOSING), Cref)
+    V, Tt = _mesh_compare(ctx, Cref, R, bmin, cell)
+    assert len(Tt) > 1000
+
+
+def test_pyramid(ctx):
+    R = 32
+    I = random_blobs(R, seed=8)
+    grid_setup(ctx, R, I)
+    ctx.scale_uniform(6)
+    say("pyramid uniform")
+    assert np.array_equal(ctx.build_pyramid(B.GRID_INPUT), O.pyramid(I, R, r=6))
+    sc = np.random.default_rng(1).integers(-5, 40, (2 * R) ** 3).astype(np.int8)
+    ctx.scale_grid(sc)
+    say("pyramid int8 scale")
+    assert np.array_equal(ctx.build_pyramid(B.GRID_INPUT), O.pyramid(I, R, scale=sc))

@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Closing + mesh throughput of the GMorpho path on one MI355X.
+
+Workload (the headline configuration): SH-displaced icosphere (subdivision 7,
+327,680 triangles, lmax 8, amplitude 0.1, seeded), voxelized at N = 512
+(base_res 256, se_size 0.3), closed with a uniform ball of radius 8 fine
+voxels, then marching cubes. A step is one closing + mesh extraction
+(dilate -> erode -> MC) of the grid already resident in HBM; voxelization is
+timed and reported separately. value = N^3 / step time (Gvoxels/s).
+
+Prints one JSON line (rank 0). Single GPU only: --gpus > 1 exits.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import broxy_amd as B  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+STAGES = ["dilate_coarse", "tight_contour", "dilate_fine", "contour_K", "erode_coarse", "base_contour",
+          "erode_fine", "mc_count_scan", "mc_verts_tris", "mc_normals"]
+KERNEL_OF_STAGE = {"dilate_coarse": "k_dil_coarse", "tight_contour": "k_tight", "dilate_fine": "k_dil_fine",
+                   "contour_K": "k_contour_K", "erode_coarse": "k_ero_coarse", "base_contour": "k_base_contour",
+                   "erode_fine": "k_ero_fine", "mc_count_scan": "k_mc_count+k_mc_scan",
+                   "mc_verts_tris": "k_mc_verts+k_mc_tris", "mc_normals": "k_mc_normals+fixup+normalize"}
+
+
+def algorithmic_bytes(stage, N, nv, nt):
+    """Compulsory HBM bytes of one launch of a stage (bit grids N^3/8 bytes,
+    cell flags N^3/64 bytes, mesh 12 bytes per vertex / normal / triangle)."""
+    g = N ** 3 / 8.0
+    f = N ** 3 / 64.0
+    return {
+        "dilate_coarse": g + f,            # read I, write DF
+        "tight_contour": f + f,            # read DF, write TG
+        "dilate_fine": g + 2 * f + g,      # read I, DF, TG, write D
+        "contour_K": g + g,                # read D, write K
+        "erode_coarse": 3 * g + f,         # read K, D, I, write CL
+        "base_contour": g + f + f,         # read D, CL, write UG
+        "erode_fine": 2 * g + 2 * f + g,   # read K, D, CL, UG, write C
+        "mc_count_scan": g,                # read C
+        "mc_verts_tris": g + 12 * nv + 12 * nt,
+        "mc_normals": g + 12 * nv,
+    }[stage]
+
+
+def build_workload(args):
+    if args.config == "large":
+        P, T = B.sh_blob(7, seed=args.seed, lmax=8, amp=0.1)
+        return P, T, 256, 8, "sh-icosphere subdiv7 (327,680 tris), N=512, ball r=8"
+    if args.config == "plain":
+        P, T = B.sphere_soup(64, 3, seed=args.seed)
+        return P, T, 64, 4, "64-icosphere soup subdiv3, N=128, ball r=4"
+    raise SystemExit("unknown config")
+
+
+def cpu_baseline(I, R, r, dr, bmin, cell, N):
+    """The oracle (CPU restatement, OpenMP) on the same closing + mesh."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    t0 = time.perf_counter()
+    D = O.dilate(I, R, r=r, data_res=dr)
+    C = O.erode(I, D, R, r=r, data_res=dr)
+    V, Nn, T = O.mc(C, R, bmin, cell)
+    dt = time.perf_counter() - t0
+    return {"value": round(N ** 3 / dt / 1e9, 4), "unit": "Gvoxels/s", "cores": int(O.num_threads()),
+            "kind": "port", "sample": "full closing + mesh of the same grid, once (%.1f s)" % dt}, C, (V, T)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="large")
+    ap.add_argument("--seed", type=int, default=2017)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--check", action="store_true", help="compare the closing grid with the oracle")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the closing path is single-device (--gpus %d refused)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    P, T, R, r, desc = build_workload(args)
+    N = 2 * R
+    ctx = B.Context(0)
+    t0 = time.perf_counter()
+    ctx.load(P, T, R, 0.3)
+    vox_ms = (time.perf_counter() - t0) * 1e3
+    bmin, bmax, res, dr, cell = ctx.info()
+    ctx.scale_uniform(r)
+
+    for _ in range(args.warmup):
+        ctx.close_grid(B.SE_SPHERE)
+        ctx.extract_mesh()
+        nv, nt = ctx.mesh_counts()
+
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.close_grid(B.SE_SPHERE)
+        ctx.extract_mesh()
+    ctx.sync()
+    t1 = time.perf_counter()
+    nv, nt = ctx.mesh_counts()
+    ms = (t1 - t0) * 1e3 / args.steps
+
+    # per-stage device times from HIP events on the context stream
+    ctx.set_timing(True)
+    acc = np.zeros(len(STAGES))
+    reps = max(3, min(args.steps, 10))
+    for _ in range(reps):
+        ctx.close_grid(B.SE_SPHERE)
+        ctx.extract_mesh()
+        acc += ctx.stage_times()
+    ctx.set_timing(False)
+    stage_ms = acc / reps
+    dom = int(np.argmax(stage_ms))
+    dname = STAGES[dom]
+    dbytes = algorithmic_bytes(dname, N, nv, nt)
+    achieved = dbytes / (stage_ms[dom] * 1e-3) / 1e9
+    whole_bytes = 5 * N ** 3 / 8.0 + 12 * nv + 12 * nt
+    whole_gbs = whole_bytes / (ms * 1e-3) / 1e9
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.config)
+    if os.path.exists(tf):
+        with open(tf) as fh:
+            tr = json.load(fh)
+        traffic = tr.get(KERNEL_OF_STAGE[dname])
+
+    out = {
+        "metric": "Gvoxels/s closing 512^3 r=8 (+%HBM roof)" if args.config == "large" else "Gvoxels/s closing",
+        "value": round(N ** 3 / (ms * 1e-3) / 1e9, 3),
+        "unit": "Gvoxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded mesh generator)",
+        "config": {"workload": desc, "N": N, "radius": r, "se": "sphere", "mesh_vertices": nv,
+                   "mesh_triangles": nt, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": KERNEL_OF_STAGE[dname], "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "algorithmic_bytes": int(dbytes),
+                     "avg_launch_ms": round(float(stage_ms[dom]), 5)},
+        "pipeline_roofline": {"bytes": int(whole_bytes), "achieved_GBs": round(whole_gbs, 1),
+                              "frac": round(whole_gbs / HBM_PEAK_GBS, 4)},
+        "stage_ms": {s: round(float(v), 5) for s, v in zip(STAGES, stage_ms)},
+        "voxelize_ms_incl_h2d": round(vox_ms, 3),
+    }
+    if not args.no_cpu:
+        I = ctx.get_cells(B.GRID_INPUT)
+        cb, Cref, (Vr, Tr) = cpu_baseline(I, R, r, dr, bmin, cell, N)
+        out["cpu_baseline"] = cb
+        if args.check:
+            C = ctx.get_cells(B.GRID_CLOSING)
+            V, Nn, Tt = ctx.mesh()
+            out["parity"] = {"closing_bit_exact": bool(np.array_equal(C, Cref)),
+                             "triangles_exact": bool(np.array_equal(Tt, Tr)),
+                             "vertices_rtol1e-6": bool(V.shape == Vr.shape and np.allclose(V, Vr, rtol=1e-6, atol=0))}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -28,6 +28,7 @@ struct gm_ctx {
   int data_res[3] = {0, 0, 0};
   u32 *I = nullptr, *D = nullptr, *K = nullptr, *C = nullptr;
   u16 *DF = nullptr, *TG = nullptr, *CL = nullptr, *UG = nullptr;
+  uint8_t *brickI = nullptr, *brickK = nullptr;
   uint8_t* stage = nullptr;  // R^3 bytes for layout conversions
   // mesh input
   float* dP = nullptr;
@@ -45,6 +46,7 @@ struct gm_ctx {
   size_t tablesCap = 0;
   SETableSet tabs;
   int tabLo = -1, tabHi = -1;
+  int tabWords = 0;  // longest table of the current set
   // marching cubes
   MCBuffers mc{};
   u32* hTotals = nullptr;
@@ -81,10 +83,10 @@ void dfree(T*& p) {
 
 void free_grid(gm_ctx* c) {
   dfree(c->I); dfree(c->D); dfree(c->K); dfree(c->C);
-  dfree(c->DF); dfree(c->TG); dfree(c->CL); dfree(c->UG);
+  dfree(c->DF); dfree(c->TG); dfree(c->CL); dfree(c->UG); dfree(c->brickI); dfree(c->brickK);
   dfree(c->stage); dfree(c->mark); dfree(c->shell);
   dfree(c->mc.blockV); dfree(c->mc.blockT); dfree(c->mc.cellV); dfree(c->mc.cellT);
-  dfree(c->mc.V); dfree(c->mc.Nrm); dfree(c->mc.T); dfree(c->mc.late); dfree(c->mc.totals);
+  dfree(c->mc.V); dfree(c->mc.Nrm); dfree(c->mc.T); dfree(c->mc.late); dfree(c->mc.totals); dfree(c->mc.scanPart);
   dfree(c->dPyr);
   dfree(c->dGrid);
   c->has_grid = false;
@@ -133,6 +135,9 @@ int alloc_grid(gm_ctx* c, int R) {
   HIPCALL(c, hipMalloc(&c->TG, flagBytes));
   HIPCALL(c, hipMalloc(&c->CL, flagBytes));
   HIPCALL(c, hipMalloc(&c->UG, flagBytes));
+  size_t brickBytes = (size_t)(c->N / 8) * (c->N / 8) * c->W;
+  HIPCALL(c, hipMalloc(&c->brickI, brickBytes));
+  HIPCALL(c, hipMalloc(&c->brickK, brickBytes));
   HIPCALL(c, hipMalloc(&c->stage, (size_t)R * R * R));
   HIPCALL(c, hipMemsetAsync(c->I, 0, rowBytes, c->stream));
   HIPCALL(c, hipMemsetAsync(c->D, 0, rowBytes, c->stream));
@@ -147,6 +152,7 @@ int alloc_grid(gm_ctx* c, int R) {
   HIPCALL(c, hipMalloc(&c->mc.blockT, (size_t)c->mc.nblocks * 4));
   HIPCALL(c, hipMalloc(&c->mc.cellV, (size_t)R * R * R * 4));
   HIPCALL(c, hipMalloc(&c->mc.totals, 16));
+  HIPCALL(c, hipMalloc(&c->mc.scanPart, mc_scan_parts(c->mc.nblocks) * 8));
   c->mc.capV = c->mc.capT = c->mc.capLate = 0;
   int rc = alloc_mesh_out(c, (u32)std::min<size_t>((size_t)4 * c->N * c->N, 1u << 30),
                           (u32)std::min<size_t>((size_t)8 * c->N * c->N, 1u << 30), 4096);
@@ -196,9 +202,11 @@ int ensure_tables(gm_ctx* c, int rlo, int rhi) {
   std::vector<u32> all;
   SETableSet t;
   for (int i = 0; i <= 128; ++i) t.dil[i] = t.ero_corner[i] = t.ero_centre[i] = -1;
+  int longest = 0;
   auto add = [&](const std::vector<u32>& v) {
     int off = (int)all.size();
     all.insert(all.end(), v.begin(), v.end());
+    longest = std::max(longest, (int)v.size());
     return off;
   };
   // sphere dilation uses rho = r+1, cube uses rho = r; both are corner tables,
@@ -221,17 +229,20 @@ int ensure_tables(gm_ctx* c, int rlo, int rhi) {
   HIPCALL(c, hipStreamSynchronize(c->stream));
   c->tabLo = rlo;
   c->tabHi = rhi;
+  c->tabWords = longest;
   return GM_OK;
 }
 
 // Cube dilation tables live in a second table set (dil[] indexed by r).
-int cube_tables(gm_ctx* c, int rlo, int rhi, SETableSet& t, u32*& dtab) {
+int cube_tables(gm_ctx* c, int rlo, int rhi, SETableSet& t, u32*& dtab, int& longest) {
   std::vector<u32> all;
+  longest = 0;
   for (int i = 0; i <= 128; ++i) t.dil[i] = t.ero_corner[i] = t.ero_centre[i] = -1;
   for (int r = rlo; r <= rhi; ++r) {
     int off = (int)all.size();
     std::vector<u32> v = build_se_table(SE_CORNER, r, true);
     all.insert(all.end(), v.begin(), v.end());
+    longest = std::max(longest, (int)v.size());
     t.dil[r] = off;
   }
   HIPCALL(c, hipMalloc(&dtab, all.size() * 4));
@@ -250,6 +261,8 @@ MorphBuffers morph_buffers(gm_ctx* c) {
   b.CL = c->CL;
   b.UG = c->UG;
   b.tables = c->dTables;
+  b.brickI = c->brickI;
+  b.brickK = c->brickK;
   return b;
 }
 
@@ -274,11 +287,12 @@ int do_dilate(gm_ctx* c, int se) {
   if (se == GM_SE_CUBE) {
     SETableSet t;
     u32* dtab = nullptr;
-    rc = cube_tables(c, lo, hi, t, dtab);
+    int longest = 0;
+    rc = cube_tables(c, lo, hi, t, dtab, longest);
     if (rc) return rc;
     MorphBuffers b = morph_buffers(c);
     b.tables = dtab;
-    hipError_t e = launch_dilate(morph_dims(c), b, radius_field(c), 1, t, lo, hi, c->stream,
+    hipError_t e = launch_dilate(morph_dims(c), b, radius_field(c), 1, t, lo, hi, longest, c->stream,
                                  c->timing ? &c->ev[0] : nullptr);
     hipStreamSynchronize(c->stream);
     hipFree(dtab);
@@ -286,8 +300,8 @@ int do_dilate(gm_ctx* c, int se) {
   } else {
     rc = ensure_tables(c, lo, hi);
     if (rc) return rc;
-    hipError_t e = launch_dilate(morph_dims(c), morph_buffers(c), radius_field(c), 0, c->tabs, lo, hi, c->stream,
-                                 c->timing ? &c->ev[0] : nullptr);
+    hipError_t e = launch_dilate(morph_dims(c), morph_buffers(c), radius_field(c), 0, c->tabs, lo, hi, c->tabWords,
+                                 c->stream, c->timing ? &c->ev[0] : nullptr);
     if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("dilation: ") + hipGetErrorString(e));
   }
   c->meshPending = false;
@@ -299,8 +313,8 @@ int do_erode(gm_ctx* c) {
   if (rc) return rc;
   rc = ensure_tables(c, std::max(c->rlo, 0), c->rhi);
   if (rc) return rc;
-  hipError_t e = launch_erode(morph_dims(c), morph_buffers(c), radius_field(c), c->tabs, c->rlo, c->rhi, c->stream,
-                              c->timing ? &c->ev[4] : nullptr);
+  hipError_t e = launch_erode(morph_dims(c), morph_buffers(c), radius_field(c), c->tabs, c->rlo, c->rhi,
+                              c->tabWords, c->stream, c->timing ? &c->ev[4] : nullptr);
   if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("erosion: ") + hipGetErrorString(e));
   c->meshPending = false;
   return GM_OK;

@@ -76,8 +76,8 @@ enum { SE_CORNER = 0, SE_CENTRE = 1 };
 constexpr int kMaxRhoFast = 32;  // streaming kernel supports m, |offset| <= 32
 
 // Launch configuration of the streaming morphology kernels.
-constexpr int kTileRows = 64;  // rows (y) per tile == threads per workgroup
-constexpr int kTileZ = 32;     // output planes per tile
+constexpr int kTileRows = 64;  // lanes per workgroup (one output row each)
+constexpr int kTileZ = 32;     // planes per tile
 
 struct MorphDims {
   int R, N, W;
@@ -110,7 +110,8 @@ struct MorphBuffers {
   u16* CL;  // coarse erosion: cell cleared
   u16* UG;  // base contour of the coarse erosion
   u32* tables;       // device SE tables (concatenated)
-  int* flags;        // scratch counters
+  uint8_t* brickI;   // per 1 word x 8 rows x 8 planes brick of I: bit0 any, bit1 all
+  uint8_t* brickK;   // same for K (bit0 only)
 };
 struct SETableSet {
   // offsets (in u32) into MorphBuffers::tables for each rho, -1 if absent
@@ -119,9 +120,11 @@ struct SETableSet {
   int ero_centre[129];
 };
 hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf, int se,
-                         const SETableSet& tabs, int rho_lo, int rho_hi, hipStream_t s, hipEvent_t* ev);
+                         const SETableSet& tabs, int rho_lo, int rho_hi, int tabWords, hipStream_t s,
+                         hipEvent_t* ev);
 hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf,
-                        const SETableSet& tabs, int rho_lo, int rho_hi, hipStream_t s, hipEvent_t* ev);
+                        const SETableSet& tabs, int rho_lo, int rho_hi, int tabWords, hipStream_t s,
+                        hipEvent_t* ev);
 
 // ---- marching cubes (gm_mc.hip)
 struct MCBuffers {
@@ -134,12 +137,14 @@ struct MCBuffers {
   u32* T;
   u32* late;     // [0] count, then (slot, a, b, c) quadruples
   u32* totals;   // [0] nV [1] nT [2] overflow [3] border flag
+  u32* scanPart; // 2 u32 per 1024 blocks
   u32 capV, capT, capLate;
   int nblocks;   // (P/8)^3
   int P;         // padded power-of-two cells per axis
 };
 hipError_t launch_mc(const u32* C, int R, const float* bmin3, float cell, const MCBuffers& m, hipStream_t s,
                      hipEvent_t* ev);
+size_t mc_scan_parts(int nblocks);
 hipError_t launch_mc_emit(const u32* C, int R, const float* bmin3, float cell, const MCBuffers& m,
                           hipStream_t s, hipEvent_t* ev);
 

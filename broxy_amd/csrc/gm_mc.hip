@@ -8,14 +8,17 @@
 // triangles from the triangulation table with reversed winding.
 //
 // MI355X design: the Morton order is walked directly. A workgroup owns an
-// aligned 8x8x8 block of cells (512 consecutive Morton codes), so the
-// reference's tag / iota / copy_if / sort_by_key / scan chain becomes
-//   mc_count  (per block vertex/triangle totals)
-//   mc_scan   (exclusive scan of the block totals, one workgroup)
-//   mc_verts  (block-local scan, vertex positions, per-cell vertex base)
-//   mc_tris   (triangles; neighbour vertex ids from the per-cell base)
-//   mc_normals(per vertex gather over the <= 4 cubes sharing its edge)
-//   mc_fixup  (triangles the count table does not cover), mc_normalize.
+// aligned 8x8x8 block of cells (512 consecutive Morton codes) and stages the
+// closed grid around it in LDS, so the reference's tag / iota / copy_if /
+// sort_by_key / scan chain becomes
+//   mc_count   per block vertex/triangle totals (uniform blocks exit early)
+//   scan       exclusive scan of the block totals (reduce, top, down-sweep)
+//   mc_verts   cube cases in LDS, block scan over cells, then one thread per
+//              cube slot: vertex positions, area-weighted normal sums
+//              gathered from the <= 4 cubes around each vertex, and the
+//              per-cell vertex base for neighbouring blocks
+//   mc_tris    same staging, triangles with neighbour vertex ids
+//   mc_fixup   triangles the count table does not cover, mc_normalize.
 // The per-cell vertex base map is dense (R^3 u32) but only written and read
 // at surface cells, replacing the reference's 4*N^3-byte memset + splat.
 #include "gm_internal.h"
@@ -60,9 +63,14 @@ struct Box {
   __device__ __forceinline__ int at(int x, int y, int z) const {
     return (rows[(y - y0) + kBox * (z - z0)] >> (x - x0)) & 1;
   }
+  // three consecutive x bits starting at x
+  __device__ __forceinline__ u32 at3(int x, int y, int z) const {
+    return (rows[(y - y0) + kBox * (z - z0)] >> (x - x0)) & 7u;
+  }
 };
 
-__device__ void load_box(Box& bx, const u32* __restrict__ C, int N, int fx0, int fy0, int fz0) {
+// Loads rows [fy0, fy0+ny) x [fz0, fz0+nz) of the box (x range 24 bits from fx0).
+__device__ void load_box(Box& bx, const u32* __restrict__ C, int N, int fx0, int fy0, int fz0, int ny, int nz) {
   const int W = N / 32;
   if (threadIdx.x == 0) {
     bx.x0 = fx0;
@@ -71,8 +79,9 @@ __device__ void load_box(Box& bx, const u32* __restrict__ C, int N, int fx0, int
   }
   int w0 = (fx0 >= 0) ? (fx0 >> 5) : -((31 - fx0) >> 5);
   int o = fx0 - 32 * w0;
-  for (int i = threadIdx.x; i < kBox * kBox; i += blockDim.x) {
-    int y = fy0 + (i % kBox), z = fz0 + (i / kBox);
+  for (int i = threadIdx.x; i < ny * nz; i += blockDim.x) {
+    int j = i % ny, k = i / ny;
+    int y = fy0 + j, z = fz0 + k;
     u32 a = 0, b = 0;
     if ((unsigned)y < (unsigned)N && (unsigned)z < (unsigned)N) {
       size_t r = ((size_t)z * N + y) * W;
@@ -80,24 +89,18 @@ __device__ void load_box(Box& bx, const u32* __restrict__ C, int N, int fx0, int
       if ((unsigned)(w0 + 1) < (unsigned)W) b = C[r + w0 + 1];
     }
     u64 v = ((u64)b << 32) | a;
-    bx.rows[i] = (u32)(v >> o) & ((1u << kBox) - 1u);
+    bx.rows[j + kBox * k] = (u32)(v >> o) & ((1u << kBox) - 1u);
   }
   __syncthreads();
 }
 
-// Corner k of the cube at f: (k&1 ^ (k>>1)&1, (k>>1)&1, (k>>2)&1) in
-// MarchingCubesMesher.h:39-46 order (0,0,0),(1,0,0),(1,1,0),(0,1,0),...
+// Corner k of the cube at f in MarchingCubesMesher.h:39-46 order:
+// (0,0,0),(1,0,0),(1,1,0),(0,1,0),(0,0,1),(1,0,1),(1,1,1),(0,1,1).
 __device__ __forceinline__ int cube_case(const Box& b, int fx, int fy, int fz) {
-  int v = 0;
-  v |= b.at(fx, fy, fz) << 0;
-  v |= b.at(fx + 1, fy, fz) << 1;
-  v |= b.at(fx + 1, fy + 1, fz) << 2;
-  v |= b.at(fx, fy + 1, fz) << 3;
-  v |= b.at(fx, fy, fz + 1) << 4;
-  v |= b.at(fx + 1, fy, fz + 1) << 5;
-  v |= b.at(fx + 1, fy + 1, fz + 1) << 6;
-  v |= b.at(fx, fy + 1, fz + 1) << 7;
-  return v;
+  u32 r00 = b.at3(fx, fy, fz) , r10 = b.at3(fx, fy + 1, fz);
+  u32 r01 = b.at3(fx, fy, fz + 1), r11 = b.at3(fx, fy + 1, fz + 1);
+  return (int)((r00 & 1u) | ((r00 >> 1 & 1u) << 1) | ((r10 >> 1 & 1u) << 2) | ((r10 & 1u) << 3) |
+               ((r01 & 1u) << 4) | ((r01 >> 1 & 1u) << 5) | ((r11 >> 1 & 1u) << 6) | ((r11 & 1u) << 7));
 }
 __device__ __forceinline__ int own_verts(int v) {
   int c0 = v & 1;
@@ -107,18 +110,18 @@ __device__ __forceinline__ bool mixed(int v) { return v != 0 && v != 255; }
 __device__ __forceinline__ bool cell_in_m_range(int cx, int cy, int cz, int R) {
   return cx >= 1 && cy >= 1 && cz >= 1 && cx <= R - 2 && cy <= R - 2 && cz <= R - 2;
 }
-
-// Per-cell vertex / triangle counts (cells outside [1,R-2]^3 contribute 0).
-__device__ __forceinline__ void cell_counts(const Box& b, int cx, int cy, int cz, int R, u32& nv, u32& nt) {
-  nv = 0;
-  nt = 0;
-  if (!cell_in_m_range(cx, cy, cz, R)) return;
+// The 3x3x3 fine voxels of cell c are uniform (then no cube of c is mixed).
+__device__ __forceinline__ bool cell_uniform(const Box& b, int cx, int cy, int cz) {
+  u32 a = 7u, o = 0u;
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    int v = cube_case(b, 2 * cx + (j & 1), 2 * cy + ((j >> 1) & 1), 2 * cz + ((j >> 2) & 1));
-    nv += own_verts(v);
-    nt += g_mc.ntri[v];
-  }
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      u32 v = b.at3(2 * cx, 2 * cy + j, 2 * cz + k);
+      a &= v;
+      o |= v;
+    }
+  return a == 7u || o == 0u;
 }
 
 // Block index -> cell coordinates of local Morton index l.
@@ -129,10 +132,9 @@ __device__ __forceinline__ void block_cell(int blk, int l, int& cx, int& cy, int
   cz = (int)compact1by2(code >> 2);
 }
 
-// Exclusive scan of a (two values per thread) over the workgroup in thread
-// order; returns the exclusive prefix of the thread's first value and writes
-// the block total to *total.
-__device__ u32 block_exscan(u32 v, u32* lds8, u32* total) {
+// Exclusive scan over the workgroup in thread order of one value per thread;
+// writes the workgroup total to *total.
+__device__ u32 block_exscan(u32 v, u32* lds16, u32* total) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   u32 x = v;
 #pragma unroll
@@ -140,12 +142,12 @@ __device__ u32 block_exscan(u32 v, u32* lds8, u32* total) {
     u32 y = __shfl_up(x, d, 64);
     if (lane >= d) x += y;
   }
-  if (lane == 63) lds8[wv] = x;
+  if (lane == 63) lds16[wv] = x;
   __syncthreads();
   u32 off = 0, tot = 0;
   const int nw = blockDim.x >> 6;
   for (int i = 0; i < nw; ++i) {
-    u32 s = lds8[i];
+    u32 s = lds16[i];
     if (i < wv) off += s;
     tot += s;
   }
@@ -154,29 +156,97 @@ __device__ u32 block_exscan(u32 v, u32* lds8, u32* total) {
   return off + x - v;
 }
 
-__global__ void __launch_bounds__(kMcThreads) k_mc_count(const u32* __restrict__ C, int R, int P, u32* blockV,
+struct BlockLds {
+  Box box;
+  uint8_t cases[4096];  // [local cell * 8 + cube], 0 for cells outside M range
+  u32 vOff[512];
+  u32 tOff[512];
+  u32 red[16];
+};
+
+// Cases of the cubes of the block's cells (uniform cells and cells outside
+// [1,R-2]^3 get 0) and per-cell vertex / triangle counts into vOff/tOff.
+__device__ void stage_cases(BlockLds& S, int blk, int R) {
+  for (int l = threadIdx.x; l < 512; l += blockDim.x) {
+    int cx, cy, cz;
+    block_cell(blk, l, cx, cy, cz);
+    u32 nv = 0, nt = 0;
+    if (cell_in_m_range(cx, cy, cz, R) && !cell_uniform(S.box, cx, cy, cz)) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        int v = cube_case(S.box, 2 * cx + (j & 1), 2 * cy + ((j >> 1) & 1), 2 * cz + ((j >> 2) & 1));
+        S.cases[8 * l + j] = (uint8_t)v;
+        nv += own_verts(v);
+        nt += g_mc.ntri[v];
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) S.cases[8 * l + j] = 0;
+    }
+    S.vOff[l] = nv;
+    S.tOff[l] = nt;
+  }
+  __syncthreads();
+}
+
+// Exclusive scans of vOff/tOff over the 512 cells in Morton order.
+__device__ void scan_cells(BlockLds& S) {
+  // two cells per thread (workgroup of 256)
+  int l0 = 2 * threadIdx.x;
+  u32 v0 = S.vOff[l0], v1 = S.vOff[l0 + 1], t0 = S.tOff[l0], t1 = S.tOff[l0 + 1];
+  u32 tv, tt;
+  u32 ov = block_exscan(v0 + v1, S.red, &tv);
+  u32 ot = block_exscan(t0 + t1, S.red, &tt);
+  S.vOff[l0] = ov;
+  S.vOff[l0 + 1] = ov + v0;
+  S.tOff[l0] = ot;
+  S.tOff[l0 + 1] = ot + t0;
+  __syncthreads();
+}
+
+__device__ __forceinline__ bool block_outside(int blk, int R, int& c0x, int& c0y, int& c0z) {
+  block_cell(blk, 0, c0x, c0y, c0z);
+  return c0x >= R || c0y >= R || c0z >= R;
+}
+
+__global__ void __launch_bounds__(kMcThreads) k_mc_count(const u32* __restrict__ C, int R, u32* blockV,
                                                           u32* blockT) {
   __shared__ Box bx;
-  __shared__ u32 red[8];
+  __shared__ u32 red[16];
   const int blk = blockIdx.x;
   int c0x, c0y, c0z;
-  block_cell(blk, 0, c0x, c0y, c0z);
-  if (c0x >= R || c0y >= R || c0z >= R) {
+  if (block_outside(blk, R, c0x, c0y, c0z)) {
     if (threadIdx.x == 0) {
       blockV[blk] = 0;
       blockT[blk] = 0;
     }
     return;
   }
-  load_box(bx, C, 2 * R, 2 * c0x - kLo, 2 * c0y - kLo, 2 * c0z - kLo);
+  // cells of the block need fine voxels [16b, 16b+17) per axis
+  load_box(bx, C, 2 * R, 2 * c0x, 2 * c0y, 2 * c0z, 17, 17);
+  // uniform block: no mixed cube
+  u32 a = 0x1ffffu, o = 0u;
+  for (int i = threadIdx.x; i < 17 * 17; i += blockDim.x) {
+    u32 v = bx.rows[(i % 17) + kBox * (i / 17)] & 0x1ffffu;
+    a &= v;
+    o |= v;
+  }
+  const bool anyOne = __syncthreads_or(o != 0u) != 0;
+  const bool anyZero = __syncthreads_or(a != 0x1ffffu) != 0;
+  const bool nonuni = anyOne && anyZero;
   u32 nv = 0, nt = 0;
-  for (int h = 0; h < 2; ++h) {
-    int cx, cy, cz;
-    block_cell(blk, 2 * threadIdx.x + h, cx, cy, cz);
-    u32 a, b;
-    cell_counts(bx, cx, cy, cz, R, a, b);
-    nv += a;
-    nt += b;
+  if (nonuni) {
+    for (int h = 0; h < 2; ++h) {
+      int cx, cy, cz;
+      block_cell(blk, 2 * threadIdx.x + h, cx, cy, cz);
+      if (!cell_in_m_range(cx, cy, cz, R) || cell_uniform(bx, cx, cy, cz)) continue;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        int v = cube_case(bx, 2 * cx + (j & 1), 2 * cy + ((j >> 1) & 1), 2 * cz + ((j >> 2) & 1));
+        nv += own_verts(v);
+        nt += g_mc.ntri[v];
+      }
+    }
   }
   u32 tv, tt;
   block_exscan(nv, red, &tv);
@@ -187,43 +257,96 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_count(const u32* __restrict__
   }
 }
 
-// Exclusive scan of the block totals (single workgroup of 1024 threads).
-__global__ void __launch_bounds__(1024) k_mc_scan(u32* blockV, u32* blockT, int nblocks, u32* totals, u32 capV,
-                                                   u32 capT) {
+// ---- device-wide exclusive scan of (blockV, blockT), 1024 entries per chunk
+constexpr int kChunk = 1024;
+
+__global__ void __launch_bounds__(256) k_scan_reduce(const u32* __restrict__ a, const u32* __restrict__ b, int n,
+                                                     u32* part) {
   __shared__ u32 red[16];
-  const int t = threadIdx.x;
-  int per = (nblocks + 1023) / 1024;
-  int b0 = t * per, b1 = min(b0 + per, nblocks);
-  u32 sv = 0, st = 0;
-  for (int b = b0; b < b1; ++b) {
-    sv += blockV[b];
-    st += blockT[b];
+  int base = blockIdx.x * kChunk;
+  u32 sa = 0, sb = 0;
+  for (int i = base + threadIdx.x; i < min(base + kChunk, n); i += 256) {
+    sa += a[i];
+    sb += b[i];
   }
-  u32 tv, tt;
-  u32 ov = block_exscan(sv, red, &tv);
-  u32 ot = block_exscan(st, red, &tt);
-  for (int b = b0; b < b1; ++b) {
-    u32 v = blockV[b], w = blockT[b];
-    blockV[b] = ov;
-    blockT[b] = ot;
-    ov += v;
-    ot += w;
+  u32 ta, tb;
+  block_exscan(sa, red, &ta);
+  block_exscan(sb, red, &tb);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = ta;
+    part[2 * blockIdx.x + 1] = tb;
   }
-  if (t == 0) {
-    totals[0] = tv;
-    totals[1] = tt;
-    totals[2] = (tv > capV || tt > capT) ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(256) k_scan_top(u32* part, int nparts, u32* totals, u32 capV, u32 capT) {
+  __shared__ u32 red[16];
+  // up to 4 parts per thread, contiguous
+  int p0 = threadIdx.x * 4;
+  u32 va[4] = {0, 0, 0, 0}, vb[4] = {0, 0, 0, 0};
+  u32 sa = 0, sb = 0;
+  for (int k = 0; k < 4; ++k) {
+    int p = p0 + k;
+    if (p < nparts) {
+      va[k] = part[2 * p];
+      vb[k] = part[2 * p + 1];
+    }
+    sa += va[k];
+    sb += vb[k];
+  }
+  u32 ta, tb;
+  u32 oa = block_exscan(sa, red, &ta);
+  u32 ob = block_exscan(sb, red, &tb);
+  for (int k = 0; k < 4; ++k) {
+    int p = p0 + k;
+    if (p < nparts) {
+      part[2 * p] = oa;
+      part[2 * p + 1] = ob;
+    }
+    oa += va[k];
+    ob += vb[k];
+  }
+  if (threadIdx.x == 0) {
+    totals[0] = ta;
+    totals[1] = tb;
+    totals[2] = (ta > capV || tb > capT) ? 1u : 0u;
     totals[3] = 0u;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_scan_down(u32* a, u32* b, int n, const u32* __restrict__ part) {
+  __shared__ u32 red[16];
+  int base = blockIdx.x * kChunk + 4 * threadIdx.x;
+  u32 va[4], vb[4];
+  u32 sa = 0, sb = 0;
+  for (int k = 0; k < 4; ++k) {
+    int i = base + k;
+    va[k] = i < n ? a[i] : 0u;
+    vb[k] = i < n ? b[i] : 0u;
+    sa += va[k];
+    sb += vb[k];
+  }
+  u32 ta, tb;
+  u32 oa = block_exscan(sa, red, &ta) + part[2 * blockIdx.x];
+  u32 ob = block_exscan(sb, red, &tb) + part[2 * blockIdx.x + 1];
+  for (int k = 0; k < 4; ++k) {
+    int i = base + k;
+    if (i < n) {
+      a[i] = oa;
+      b[i] = ob;
+    }
+    oa += va[k];
+    ob += vb[k];
   }
 }
 
 // Position of the vertex on owned edge e (0: x, 3: y, 8: z) of cube f:
 // findIntersection (MarchingCubesMesher.cu:98-100) with {0,1} corner values.
+// The division by v2 - v1 = +-1 is exact, so it is applied as a sign.
 __device__ __forceinline__ float lerp_edge(float p1, float p2, float v1, float v2) {
   float d = __fsub_rn(p2, p1);
   float t = __fsub_rn(0.95f, v1);
   float m = __fmul_rn(t, d);
-  float q = __fdiv_rn(m, __fsub_rn(v2, v1));
+  float q = (v2 > v1) ? m : -m;
   return __fadd_rn(p1, q);
 }
 struct Geo {
@@ -252,143 +375,6 @@ __device__ __forceinline__ int rank_of(int v, int e) {  // index of owned edge e
   int e0 = c0 ^ ((v >> 1) & 1), e3 = c0 ^ ((v >> 3) & 1);
   return e == 0 ? 0 : (e == 3 ? e0 : e0 + e3);
 }
-__device__ __forceinline__ int cell_cube_prefix(const Box& b, int cx, int cy, int cz, int jlim) {
-  int s = 0;
-  for (int j = 0; j < jlim; ++j)
-    s += own_verts(cube_case(b, 2 * cx + (j & 1), 2 * cy + ((j >> 1) & 1), 2 * cz + ((j >> 2) & 1)));
-  return s;
-}
-
-__global__ void __launch_bounds__(kMcThreads) k_mc_verts(const u32* __restrict__ C, int R, Geo geo,
-                                                          const u32* __restrict__ blockV, u32* cellV, float* V,
-                                                          u32 capV, u32* totals) {
-  __shared__ Box bx;
-  __shared__ u32 red[8];
-  const int blk = blockIdx.x;
-  int c0x, c0y, c0z;
-  block_cell(blk, 0, c0x, c0y, c0z);
-  if (c0x >= R || c0y >= R || c0z >= R) return;
-  u32 base = blockV[blk];
-  u32 next = (blk + 1 < (int)gridDim.x) ? blockV[blk + 1] : totals[0];
-  if (next == base) return;  // no vertex in this block (uniform across the workgroup)
-  load_box(bx, C, 2 * R, 2 * c0x - kLo, 2 * c0y - kLo, 2 * c0z - kLo);
-  int cxs[2], cys[2], czs[2];
-  u32 nv[2];
-  for (int h = 0; h < 2; ++h) {
-    block_cell(blk, 2 * threadIdx.x + h, cxs[h], cys[h], czs[h]);
-    u32 t;
-    cell_counts(bx, cxs[h], cys[h], czs[h], R, nv[h], t);
-  }
-  u32 tot;
-  u32 off = block_exscan(nv[0] + nv[1], red, &tot) + base;
-  for (int h = 0; h < 2; ++h) {
-    if (nv[h] == 0) continue;
-    int cx = cxs[h], cy = cys[h], cz = czs[h];
-    cellV[(size_t)cx + (size_t)R * ((size_t)cy + (size_t)R * cz)] = off;
-    u32 vid = off;
-    for (int j = 0; j < 8; ++j) {
-      int fx = 2 * cx + (j & 1), fy = 2 * cy + ((j >> 1) & 1), fz = 2 * cz + ((j >> 2) & 1);
-      int v = cube_case(bx, fx, fy, fz);
-      int c0 = v & 1;
-      const int es[3] = {0, 3, 8};
-      const int on[3] = {c0 ^ ((v >> 1) & 1), c0 ^ ((v >> 3) & 1), c0 ^ ((v >> 4) & 1)};
-      for (int q = 0; q < 3; ++q) {
-        if (!on[q]) continue;
-        if (vid < capV) {
-          float p[3];
-          edge_pos(geo, bx, fx, fy, fz, es[q], p);
-          V[3 * (size_t)vid] = p[0];
-          V[3 * (size_t)vid + 1] = p[1];
-          V[3 * (size_t)vid + 2] = p[2];
-        }
-        ++vid;
-      }
-    }
-    off += nv[h];
-  }
-}
-
-// Vertex id of local edge e of the cube at fine (fx,fy,fz): 0 when the owning
-// cube's cell is outside [1,R-2]^3 (the reference's absent neighbour).
-__device__ __forceinline__ u32 edge_vid(const Box& b, const u32* __restrict__ cellV, int R, int fx, int fy, int fz,
-                                        int e, u32* border) {
-  int gx = fx + g_own[e][0], gy = fy + g_own[e][1], gz = fz + g_own[e][2], oe = g_own[e][3];
-  int cx = gx >> 1, cy = gy >> 1, cz = gz >> 1;
-  if (!cell_in_m_range(cx, cy, cz, R)) {
-    *border = 1u;
-    return 0u;
-  }
-  int j = (gx & 1) | ((gy & 1) << 1) | ((gz & 1) << 2);
-  u32 base = cellV[(size_t)cx + (size_t)R * ((size_t)cy + (size_t)R * cz)];
-  return base + (u32)cell_cube_prefix(b, cx, cy, cz, j) + (u32)rank_of(cube_case(b, gx, gy, gz), oe);
-}
-
-__global__ void __launch_bounds__(kMcThreads) k_mc_tris(const u32* __restrict__ C, int R,
-                                                         const u32* __restrict__ blockT,
-                                                         const u32* __restrict__ cellV, u32* T, u32 capT,
-                                                         u32* late, u32 capLate, u32* totals) {
-  __shared__ Box bx;
-  __shared__ u32 red[8];
-  const int blk = blockIdx.x;
-  int c0x, c0y, c0z;
-  block_cell(blk, 0, c0x, c0y, c0z);
-  if (c0x >= R || c0y >= R || c0z >= R) return;
-  u32 base = blockT[blk];
-  u32 next = (blk + 1 < (int)gridDim.x) ? blockT[blk + 1] : totals[1];
-  if (next == base) return;
-  load_box(bx, C, 2 * R, 2 * c0x - kLo, 2 * c0y - kLo, 2 * c0z - kLo);
-  int cxs[2], cys[2], czs[2];
-  u32 nt[2];
-  for (int h = 0; h < 2; ++h) {
-    block_cell(blk, 2 * threadIdx.x + h, cxs[h], cys[h], czs[h]);
-    u32 t;
-    cell_counts(bx, cxs[h], cys[h], czs[h], R, t, nt[h]);
-  }
-  u32 tot;
-  u32 off = block_exscan(nt[0] + nt[1], red, &tot) + base;
-  u32 border = 0u;
-  for (int h = 0; h < 2; ++h) {
-    if (nt[h] == 0) continue;
-    int cx = cxs[h], cy = cys[h], cz = czs[h];
-    u32 slot = off;
-    for (int j = 0; j < 8; ++j) {
-      int fx = 2 * cx + (j & 1), fy = 2 * cy + ((j >> 1) & 1), fz = 2 * cz + ((j >> 2) & 1);
-      int v = cube_case(bx, fx, fy, fz);
-      if (!mixed(v)) continue;
-      unsigned long long tri = g_tri[v];
-      u32 cnt = g_mc.ntri[v], len = g_mc.len[v];
-      for (u32 q = 0; q < len; ++q) {
-        int e0 = (int)((tri >> (12 * q)) & 0xf), e1 = (int)((tri >> (12 * q + 4)) & 0xf),
-            e2 = (int)((tri >> (12 * q + 8)) & 0xf);
-        u32 a = edge_vid(bx, cellV, R, fx, fy, fz, e2, &border);
-        u32 b = edge_vid(bx, cellV, R, fx, fy, fz, e1, &border);
-        u32 c = edge_vid(bx, cellV, R, fx, fy, fz, e0, &border);
-        u32 s = slot + q;
-        if (q < cnt) {
-          if (s < capT) {
-            T[3 * (size_t)s] = a;
-            T[3 * (size_t)s + 1] = b;
-            T[3 * (size_t)s + 2] = c;
-          }
-        } else {
-          // listed but not counted: overwrites slot s after all counted writes
-          u32 k = atomicAdd(&late[0], 1u);
-          if (k < capLate) {
-            late[1 + 4 * k] = s;
-            late[2 + 4 * k] = a;
-            late[3 + 4 * k] = b;
-            late[4 + 4 * k] = c;
-          } else {
-            atomicOr(&totals[2], 2u);
-          }
-        }
-      }
-      slot += cnt;
-    }
-    off += nt[h];
-  }
-  if (border) atomicOr(&totals[3], 1u);
-}
 
 // Area-weighted normal sums (GMorpho.cu:6498-6524) gathered per vertex: the
 // triangles that use the vertex's edge are those of the <= 4 cubes around it.
@@ -404,67 +390,169 @@ __device__ __forceinline__ void local_edge_pos(const Geo& g, const Box& b, int f
   edge_pos(g, b, fx + g_own[e][0], fy + g_own[e][1], fz + g_own[e][2], g_own[e][3], p);
 }
 
-__global__ void __launch_bounds__(kMcThreads) k_mc_normals(const u32* __restrict__ C, int R, Geo geo,
-                                                            const u32* __restrict__ blockV,
-                                                            const u32* __restrict__ cellV, float* Nrm, u32 capV,
-                                                            u32* totals) {
-  __shared__ Box bx;
+__device__ void normal_sum(const Geo& geo, const Box& bx, int R, int fx, int fy, int fz, int q, float* n) {
+  n[0] = n[1] = n[2] = 0.f;
+  for (int s = 0; s < 4; ++s) {
+    int gx = fx + g_share[q][s][0], gy = fy + g_share[q][s][1], gz = fz + g_share[q][s][2];
+    int le = g_share[q][s][3];
+    if (!cell_in_m_range(gx >> 1, gy >> 1, gz >> 1, R)) continue;
+    int gv = cube_case(bx, gx, gy, gz);
+    if (!mixed(gv)) continue;
+    unsigned long long tri = g_tri[gv];
+    u32 cnt = g_mc.ntri[gv];
+    for (u32 t = 0; t < cnt; ++t) {
+      int e0 = (int)((tri >> (12 * t)) & 0xf), e1 = (int)((tri >> (12 * t + 4)) & 0xf),
+          e2 = (int)((tri >> (12 * t + 8)) & 0xf);
+      if (e0 != le && e1 != le && e2 != le) continue;
+      float p0[3], p1[3], p2[3];
+      local_edge_pos(geo, bx, gx, gy, gz, e2, p0);
+      local_edge_pos(geo, bx, gx, gy, gz, e1, p1);
+      local_edge_pos(geo, bx, gx, gy, gz, e0, p2);
+      float a0 = __fsub_rn(p1[0], p0[0]), a1 = __fsub_rn(p1[1], p0[1]), a2 = __fsub_rn(p1[2], p0[2]);
+      float b0 = __fsub_rn(p2[0], p0[0]), b1 = __fsub_rn(p2[1], p0[1]), b2 = __fsub_rn(p2[2], p0[2]);
+      n[0] = __fadd_rn(n[0], __fsub_rn(__fmul_rn(a1, b2), __fmul_rn(a2, b1)));
+      n[1] = __fadd_rn(n[1], __fsub_rn(__fmul_rn(a2, b0), __fmul_rn(a0, b2)));
+      n[2] = __fadd_rn(n[2], __fsub_rn(__fmul_rn(a0, b1), __fmul_rn(a1, b0)));
+    }
+  }
+}
+
+__device__ __forceinline__ u32 prefix_in_cell(const BlockLds& S, int l, int j) {
+  u32 s = 0;
+  for (int q = 0; q < j; ++q) s += own_verts(S.cases[8 * l + q]);
+  return s;
+}
+
+__global__ void __launch_bounds__(kMcThreads) k_mc_verts(const u32* __restrict__ C, int R, Geo geo,
+                                                          const u32* __restrict__ blockV, u32* cellV, float* V,
+                                                          float* Nrm, u32 capV, const u32* totals) {
+  __shared__ BlockLds S;
   const int blk = blockIdx.x;
   int c0x, c0y, c0z;
-  block_cell(blk, 0, c0x, c0y, c0z);
-  if (c0x >= R || c0y >= R || c0z >= R) return;
-  u32 base = blockV[blk];
-  u32 next = (blk + 1 < (int)gridDim.x) ? blockV[blk + 1] : totals[0];
-  if (next == base) return;
-  load_box(bx, C, 2 * R, 2 * c0x - kLo, 2 * c0y - kLo, 2 * c0z - kLo);
-  for (int h = 0; h < 2; ++h) {
+  if (block_outside(blk, R, c0x, c0y, c0z)) return;
+  const u32 base = blockV[blk];
+  const u32 next = (blk + 1 < (int)gridDim.x) ? blockV[blk + 1] : totals[0];
+  if (next == base) return;  // no vertex in this block (uniform across the workgroup)
+  load_box(S.box, C, 2 * R, 2 * c0x - kLo, 2 * c0y - kLo, 2 * c0z - kLo, kBox, kBox);
+  stage_cases(S, blk, R);
+  scan_cells(S);
+  // per-cell vertex base, read by the triangle kernel of neighbouring blocks
+  for (int l = threadIdx.x; l < 512; l += blockDim.x) {
+    u32 nxt = (l + 1 < 512) ? S.vOff[l + 1] : (next - base);
+    if (nxt != S.vOff[l]) {
+      int cx, cy, cz;
+      block_cell(blk, l, cx, cy, cz);
+      cellV[(size_t)cx + (size_t)R * ((size_t)cy + (size_t)R * cz)] = base + S.vOff[l];
+    }
+  }
+  for (int s = threadIdx.x; s < 4096; s += blockDim.x) {
+    int v = S.cases[s];
+    if (!mixed(v)) continue;
+    int l = s >> 3, j = s & 7;
     int cx, cy, cz;
-    block_cell(blk, 2 * threadIdx.x + h, cx, cy, cz);
-    if (!cell_in_m_range(cx, cy, cz, R)) continue;
-    u32 nvc, ntc;
-    cell_counts(bx, cx, cy, cz, R, nvc, ntc);
-    if (nvc == 0) continue;
-    u32 vid = cellV[(size_t)cx + (size_t)R * ((size_t)cy + (size_t)R * cz)];
-    for (int j = 0; j < 8; ++j) {
-      int fx = 2 * cx + (j & 1), fy = 2 * cy + ((j >> 1) & 1), fz = 2 * cz + ((j >> 2) & 1);
-      int v = cube_case(bx, fx, fy, fz);
-      int c0 = v & 1;
-      const int on[3] = {c0 ^ ((v >> 1) & 1), c0 ^ ((v >> 3) & 1), c0 ^ ((v >> 4) & 1)};
-      for (int q = 0; q < 3; ++q) {
-        if (!on[q]) continue;
-        float n[3] = {0.f, 0.f, 0.f};
-        for (int s = 0; s < 4; ++s) {
-          int gx = fx + g_share[q][s][0], gy = fy + g_share[q][s][1], gz = fz + g_share[q][s][2];
-          int le = g_share[q][s][3];
-          if (!cell_in_m_range(gx >> 1, gy >> 1, gz >> 1, R)) continue;
-          int gv = cube_case(bx, gx, gy, gz);
-          if (!mixed(gv)) continue;
-          unsigned long long tri = g_tri[gv];
-          u32 cnt = g_mc.ntri[gv];
-          for (u32 t = 0; t < cnt; ++t) {
-            int e0 = (int)((tri >> (12 * t)) & 0xf), e1 = (int)((tri >> (12 * t + 4)) & 0xf),
-                e2 = (int)((tri >> (12 * t + 8)) & 0xf);
-            if (e0 != le && e1 != le && e2 != le) continue;
-            float p0[3], p1[3], p2[3];
-            local_edge_pos(geo, bx, gx, gy, gz, e2, p0);
-            local_edge_pos(geo, bx, gx, gy, gz, e1, p1);
-            local_edge_pos(geo, bx, gx, gy, gz, e0, p2);
-            float a0 = __fsub_rn(p1[0], p0[0]), a1 = __fsub_rn(p1[1], p0[1]), a2 = __fsub_rn(p1[2], p0[2]);
-            float b0 = __fsub_rn(p2[0], p0[0]), b1 = __fsub_rn(p2[1], p0[1]), b2 = __fsub_rn(p2[2], p0[2]);
-            n[0] = __fadd_rn(n[0], __fsub_rn(__fmul_rn(a1, b2), __fmul_rn(a2, b1)));
-            n[1] = __fadd_rn(n[1], __fsub_rn(__fmul_rn(a2, b0), __fmul_rn(a0, b2)));
-            n[2] = __fadd_rn(n[2], __fsub_rn(__fmul_rn(a0, b1), __fmul_rn(a1, b0)));
-          }
+    block_cell(blk, l, cx, cy, cz);
+    int fx = 2 * cx + (j & 1), fy = 2 * cy + ((j >> 1) & 1), fz = 2 * cz + ((j >> 2) & 1);
+    u32 vid = base + S.vOff[l] + prefix_in_cell(S, l, j);
+    int c0 = v & 1;
+    const int es[3] = {0, 3, 8};
+    const int on[3] = {c0 ^ ((v >> 1) & 1), c0 ^ ((v >> 3) & 1), c0 ^ ((v >> 4) & 1)};
+    for (int q = 0; q < 3; ++q) {
+      if (!on[q]) continue;
+      if (vid < capV) {
+        float p[3], n[3];
+        edge_pos(geo, S.box, fx, fy, fz, es[q], p);
+        normal_sum(geo, S.box, R, fx, fy, fz, q, n);
+        V[3 * (size_t)vid] = p[0];
+        V[3 * (size_t)vid + 1] = p[1];
+        V[3 * (size_t)vid + 2] = p[2];
+        Nrm[3 * (size_t)vid] = n[0];
+        Nrm[3 * (size_t)vid + 1] = n[1];
+        Nrm[3 * (size_t)vid + 2] = n[2];
+      }
+      ++vid;
+    }
+  }
+}
+
+// Vertex id of local edge e of the cube at fine (fx,fy,fz): 0 when the owning
+// cube's cell is outside [1,R-2]^3 (the reference's absent neighbour).
+__device__ __forceinline__ u32 edge_vid(const BlockLds& S, int blk, int c0x, int c0y, int c0z, u32 base,
+                                        const u32* __restrict__ cellV, int R, int fx, int fy, int fz, int e,
+                                        u32* border) {
+  int gx = fx + g_own[e][0], gy = fy + g_own[e][1], gz = fz + g_own[e][2], oe = g_own[e][3];
+  int cx = gx >> 1, cy = gy >> 1, cz = gz >> 1;
+  if (!cell_in_m_range(cx, cy, cz, R)) {
+    *border = 1u;
+    return 0u;
+  }
+  int j = (gx & 1) | ((gy & 1) << 1) | ((gz & 1) << 2);
+  int lx = cx - c0x, ly = cy - c0y, lz = cz - c0z;
+  if (lx < 8 && ly < 8 && lz < 8) {
+    int l = (int)(part1by2((u32)lx) | (part1by2((u32)ly) << 1) | (part1by2((u32)lz) << 2));
+    return base + S.vOff[l] + prefix_in_cell(S, l, j) + (u32)rank_of(S.cases[8 * l + j], oe);
+  }
+  u32 s = cellV[(size_t)cx + (size_t)R * ((size_t)cy + (size_t)R * cz)];
+  for (int q = 0; q < j; ++q)
+    s += own_verts(cube_case(S.box, 2 * cx + (q & 1), 2 * cy + ((q >> 1) & 1), 2 * cz + ((q >> 2) & 1)));
+  return s + (u32)rank_of(cube_case(S.box, gx, gy, gz), oe);
+}
+
+__global__ void __launch_bounds__(kMcThreads) k_mc_tris(const u32* __restrict__ C, int R,
+                                                         const u32* __restrict__ blockV,
+                                                         const u32* __restrict__ blockT,
+                                                         const u32* __restrict__ cellV, u32* T, u32 capT,
+                                                         u32* late, u32 capLate, u32* totals) {
+  __shared__ BlockLds S;
+  const int blk = blockIdx.x;
+  int c0x, c0y, c0z;
+  if (block_outside(blk, R, c0x, c0y, c0z)) return;
+  const u32 tbase = blockT[blk];
+  const u32 tnext = (blk + 1 < (int)gridDim.x) ? blockT[blk + 1] : totals[1];
+  if (tnext == tbase) return;
+  const u32 vbase = blockV[blk];
+  load_box(S.box, C, 2 * R, 2 * c0x - kLo, 2 * c0y - kLo, 2 * c0z - kLo, kBox, kBox);
+  stage_cases(S, blk, R);
+  scan_cells(S);
+  u32 border = 0u;
+  for (int s = threadIdx.x; s < 4096; s += blockDim.x) {
+    int v = S.cases[s];
+    if (!mixed(v)) continue;
+    int l = s >> 3, j = s & 7;
+    int cx, cy, cz;
+    block_cell(blk, l, cx, cy, cz);
+    int fx = 2 * cx + (j & 1), fy = 2 * cy + ((j >> 1) & 1), fz = 2 * cz + ((j >> 2) & 1);
+    u32 slot = tbase + S.tOff[l];
+    for (int q = 0; q < j; ++q) slot += g_mc.ntri[S.cases[8 * l + q]];
+    unsigned long long tri = g_tri[v];
+    u32 cnt = g_mc.ntri[v], len = g_mc.len[v];
+    for (u32 q = 0; q < len; ++q) {
+      int e0 = (int)((tri >> (12 * q)) & 0xf), e1 = (int)((tri >> (12 * q + 4)) & 0xf),
+          e2 = (int)((tri >> (12 * q + 8)) & 0xf);
+      u32 a = edge_vid(S, blk, c0x, c0y, c0z, vbase, cellV, R, fx, fy, fz, e2, &border);
+      u32 b = edge_vid(S, blk, c0x, c0y, c0z, vbase, cellV, R, fx, fy, fz, e1, &border);
+      u32 c = edge_vid(S, blk, c0x, c0y, c0z, vbase, cellV, R, fx, fy, fz, e0, &border);
+      u32 sl = slot + q;
+      if (q < cnt) {
+        if (sl < capT) {
+          T[3 * (size_t)sl] = a;
+          T[3 * (size_t)sl + 1] = b;
+          T[3 * (size_t)sl + 2] = c;
         }
-        if (vid < capV) {
-          Nrm[3 * (size_t)vid] = n[0];
-          Nrm[3 * (size_t)vid + 1] = n[1];
-          Nrm[3 * (size_t)vid + 2] = n[2];
+      } else {
+        // listed but not counted: overwrites slot sl after all counted writes
+        u32 k = atomicAdd(&late[0], 1u);
+        if (k < capLate) {
+          late[1 + 4 * k] = sl;
+          late[2 + 4 * k] = a;
+          late[3 + 4 * k] = b;
+          late[4 + 4 * k] = c;
+        } else {
+          atomicOr(&totals[2], 2u);
         }
-        ++vid;
       }
     }
   }
+  if (border) atomicOr(&totals[3], 1u);
 }
 
 __device__ __forceinline__ void face(const float* V, const u32* t, float* n) {
@@ -519,14 +607,14 @@ __global__ void k_mc_fixup(const float* V, float* Nrm, u32* T, u32* late, u32 ca
 __global__ void k_mc_normalize(float* Nrm, const u32* totals, u32 capV) {
   u32 nV = totals[0];
   if (nV > capV) nV = capV;
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nV) return;
-  float x = Nrm[3 * i], y = Nrm[3 * i + 1], z = Nrm[3 * i + 2];
-  float l = sqrtf(__fadd_rn(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y)), __fmul_rn(z, z)));
-  if (l > 0.f) {
-    Nrm[3 * i] = __fdiv_rn(x, l);
-    Nrm[3 * i + 1] = __fdiv_rn(y, l);
-    Nrm[3 * i + 2] = __fdiv_rn(z, l);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nV; i += (size_t)gridDim.x * blockDim.x) {
+    float x = Nrm[3 * i], y = Nrm[3 * i + 1], z = Nrm[3 * i + 2];
+    float l = sqrtf(__fadd_rn(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y)), __fmul_rn(z, z)));
+    if (l > 0.f) {
+      Nrm[3 * i] = __fdiv_rn(x, l);
+      Nrm[3 * i + 1] = __fdiv_rn(y, l);
+      Nrm[3 * i + 2] = __fdiv_rn(z, l);
+    }
   }
 }
 
@@ -564,13 +652,20 @@ hipError_t mc_tables_host(u32* ntri, u32* len, unsigned long long* tri) {
   return hipSuccess;
 }
 
+size_t mc_scan_parts(int nblocks) { return (size_t)((nblocks + kChunk - 1) / kChunk); }
+
 hipError_t launch_mc(const u32* C, int R, const float* bmin3, float cell, const MCBuffers& m, hipStream_t s,
                      hipEvent_t* ev) {
   GM_CHECK(upload_tables());
   if (ev) GM_CHECK(hipEventRecord(ev[0], s));
-  hipLaunchKernelGGL(k_mc_count, dim3(m.nblocks), dim3(kMcThreads), 0, s, C, R, m.P, m.blockV, m.blockT);
+  hipLaunchKernelGGL(k_mc_count, dim3(m.nblocks), dim3(kMcThreads), 0, s, C, R, m.blockV, m.blockT);
   GM_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_mc_scan, dim3(1), dim3(1024), 0, s, m.blockV, m.blockT, m.nblocks, m.totals, m.capV, m.capT);
+  int nparts = (int)mc_scan_parts(m.nblocks);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nparts), dim3(256), 0, s, m.blockV, m.blockT, m.nblocks, m.scanPart);
+  GM_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, m.scanPart, nparts, m.totals, m.capV, m.capT);
+  GM_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_scan_down, dim3(nparts), dim3(256), 0, s, m.blockV, m.blockT, m.nblocks, m.scanPart);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
   return launch_mc_emit(C, R, bmin3, cell, m, s, ev ? ev + 1 : nullptr);
@@ -581,20 +676,17 @@ hipError_t launch_mc_emit(const u32* C, int R, const float* bmin3, float cell, c
   GM_CHECK(upload_tables());
   Geo geo{bmin3[0], bmin3[1], bmin3[2], 0.5f * cell};
   GM_CHECK(hipMemsetAsync(m.late, 0, sizeof(u32), s));
-  hipLaunchKernelGGL(k_mc_verts, dim3(m.nblocks), dim3(kMcThreads), 0, s, C, R, geo, m.blockV, m.cellV, m.V, m.capV,
-                     m.totals);
-  GM_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_mc_tris, dim3(m.nblocks), dim3(kMcThreads), 0, s, C, R, m.blockT, m.cellV, m.T, m.capT,
-                     m.late, m.capLate, m.totals);
-  GM_CHECK(hipGetLastError());
-  if (ev) GM_CHECK(hipEventRecord(ev[1], s));
-  hipLaunchKernelGGL(k_mc_normals, dim3(m.nblocks), dim3(kMcThreads), 0, s, C, R, geo, m.blockV, m.cellV, m.Nrm,
+  hipLaunchKernelGGL(k_mc_verts, dim3(m.nblocks), dim3(kMcThreads), 0, s, C, R, geo, m.blockV, m.cellV, m.V, m.Nrm,
                      m.capV, m.totals);
   GM_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_mc_tris, dim3(m.nblocks), dim3(kMcThreads), 0, s, C, R, m.blockV, m.blockT, m.cellV, m.T,
+                     m.capT, m.late, m.capLate, m.totals);
+  GM_CHECK(hipGetLastError());
+  if (ev) GM_CHECK(hipEventRecord(ev[1], s));
   hipLaunchKernelGGL(k_mc_fixup, dim3(1), dim3(64), 0, s, m.V, m.Nrm, m.T, m.late, m.capLate, m.totals, m.capV,
                      m.capT);
   GM_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_mc_normalize, dim3((m.capV + 255) / 256), dim3(256), 0, s, m.Nrm, m.totals, m.capV);
+  hipLaunchKernelGGL(k_mc_normalize, dim3(1024), dim3(256), 0, s, m.Nrm, m.totals, m.capV);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   return hipSuccess;

@@ -21,13 +21,20 @@
 // predicate is an OR of x-runs: for every (dy, dz) of the structuring element
 // the x-extent is an interval, so F = OR_{dy,dz} G_m(src)(y+dy, z+dz) with G_m
 // the x-window dilation of a 32-bit row word. A 64-lane workgroup owns one
-// row word column x 64 rows and streams the input planes along z: per plane
-// it builds G_0..G_mmax for all rows it needs in LDS, then every lane ORs the
-// chords of each dz class into a per-lane ring of output-plane accumulators.
-// The chords are read from a table in uniform (scalar) memory. Radii that vary
-// per voxel are handled by one pass per distinct radius found in the tile:
-// gather radius (dilation) masks the output voxels, scatter radius (erosion)
-// masks the source voxels.
+// row-word column x 64 output rows and streams the input planes along z:
+//  * the chord table of the structuring element sits in LDS (uniform reads);
+//  * per plane, the rows the tile reads are prefetched into registers one
+//    plane ahead, and G_0..G_mmax of every row is built in LDS;
+//  * every lane ORs the chords of each dz class (classes b and -b-1 share
+//    their chords) into a power-of-two ring of output-plane accumulators;
+//  * a plane is handed to the pass's epilogue as soon as it is complete.
+// The coarse passes only need even (dilation) or odd (erosion) rows and
+// planes, so their lanes own every other row. Tiles whose source region is
+// empty (per-brick occupancy summaries) or whose fine pass has no tagged
+// cell skip the streaming and run the epilogue on F = 0.
+// Radii that vary per voxel are handled by one pass per distinct radius
+// found in the tile: gather radius (dilation) masks the output voxels,
+// scatter radius (erosion) masks the source voxels.
 #include <vector>
 
 #include "gm_internal.h"
@@ -68,128 +75,148 @@ __device__ __forceinline__ u32 flag_word(const u16* __restrict__ g, int R, int W
 }
 // Mask of cells 16w..16w+15 whose x index lies in [lo, hi).
 __device__ __forceinline__ u32 cell_range_mask(int w, int lo, int hi) {
-  u32 m = 0;
-  int base = 16 * w;
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    int cx = base + j;
-    if (cx >= lo && cx < hi) m |= 1u << j;
-  }
-  return m;
+  int a = lo - 16 * w, b = hi - 16 * w;
+  a = a < 0 ? 0 : (a > 16 ? 16 : a);
+  b = b < 0 ? 0 : (b > 16 ? 16 : b);
+  if (b <= a) return 0u;
+  u32 m = (b >= 16) ? 0xffffu : ((1u << b) - 1u);
+  return m & ~((1u << a) - 1u);
 }
 
 // shr(k): bit x reads source x+k; shl(k): bit x reads source x-k (1 <= k <= 32)
-__device__ __forceinline__ u32 shr_k(u32 L, u32 C, u32 Rw, int k) {
-  (void)L;
+__device__ __forceinline__ u32 shr_k(u32 C, u32 Rw, int k) {
   return k >= 32 ? Rw : __builtin_amdgcn_alignbit(Rw, C, (u32)k);
 }
-__device__ __forceinline__ u32 shl_k(u32 L, u32 C, u32 Rw, int k) {
-  (void)Rw;
+__device__ __forceinline__ u32 shl_k(u32 L, u32 C, int k) {
   return k >= 32 ? L : __builtin_amdgcn_alignbit(C, L, (u32)(32 - k));
 }
 
-struct TabHdr {
-  int kind, nclass, maxm, maxoff;
-};
-__device__ __forceinline__ TabHdr tab_hdr(const u32* tab) {
-  return TabHdr{(int)tab[0], (int)tab[1], (int)tab[2], (int)tab[3]};
-}
-__device__ __forceinline__ int dz_lo(const TabHdr& h) { return h.kind == SE_CORNER ? -h.maxoff : -h.maxoff; }
-__device__ __forceinline__ int dz_hi(const TabHdr& h) { return h.kind == SE_CORNER ? h.maxoff - 1 : h.maxoff; }
-
-// LDS carve-up for one workgroup (sizes in u32).
-struct Lds {
-  u32* gt;   // (maxm+1) * GR
-  u32* acc;  // ZR * 64
-  u32* out;  // Zt * 64
-  int GR, HY, ZR;
+// Output geometry of one streaming tile: lane i of every wave owns row
+// yfirst + ystep*i; output planes are zfirst, zfirst + zstep, ... <= zlast.
+struct TileGeo {
+  int wx;
+  int yfirst, ystep;
+  int tz0, zfirst, zlast, zstep;
+  __device__ int nout() const { return zlast < zfirst ? 0 : (zlast - zfirst) / zstep + 1; }
 };
 
-// One streaming pass: out[zo][lane] |= OR over the SE of the source around
-// (row ty0+lane, plane zo) (masked by omask when given). src(y, z, w) returns
-// the source word (0 outside the grid).
-template <class Src, class OMask>
-__device__ void stream_pass(const u32* __restrict__ tab, const Src& src, const OMask& omask, Lds L, int wx,
-                            int ty0, int tz0, int Zt) {
-  const int lane = threadIdx.x;
-  const TabHdr h = tab_hdr(tab);
-  const int dzl = dz_lo(h), dzh = dz_hi(h);
-  const int ZR = dzh - dzl + 1;
-  const int HY = h.maxoff;
-  const int GR = kTileRows + 2 * HY;
-  for (int s = 0; s < ZR; ++s) L.acc[s * kTileRows + lane] = 0u;
-  const int zbeg = tz0 + dzl, zend = tz0 + Zt - 1 + dzh;
-  for (int zi = zbeg; zi <= zend; ++zi) {
-    __syncthreads();
-    // stage 1: x-window dilations G_0..G_maxm of every row this tile reads
-    for (int gr = lane; gr < GR; gr += kTileRows) {
-      int y = ty0 - HY + gr;
-      u32 Lw = src(y, zi, wx - 1), Cw = src(y, zi, wx), Rw = src(y, zi, wx + 1);
-      u32 g;
-      if (h.kind == SE_CORNER) {
-        g = Cw | shl_k(Lw, Cw, Rw, 1);
-        L.gt[gr] = g;
-        for (int m = 1; m <= h.maxm; ++m) {
-          g |= shr_k(Lw, Cw, Rw, m) | shl_k(Lw, Cw, Rw, m + 1);
-          L.gt[m * GR + gr] = g;
-        }
-      } else {
-        g = Cw;
-        L.gt[gr] = g;
-        for (int m = 1; m <= h.maxm; ++m) {
-          g |= shr_k(Lw, Cw, Rw, m) | shl_k(Lw, Cw, Rw, m);
-          L.gt[m * GR + gr] = g;
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+__device__ __forceinline__ int wave_id() { return (int)(threadIdx.x >> 6); }
+__device__ __forceinline__ int num_waves() { return (int)(blockDim.x >> 6); }
+
+// One streaming pass of the table tl (in LDS) over the source src, ORed into
+// acc[plane index][lane] (plane index = (zo - zfirst) / zstep). The waves of
+// the workgroup take the input planes round robin; each builds the G table of
+// its plane in its own LDS slice (gtAll + wave * gtStride) and accumulates
+// with LDS atomics, so the plane loop needs no workgroup barrier.
+template <class Src>
+__device__ void stream(const u32* tl, const Src& src, u32* gtAll, int gtStride, u32* acc, const TileGeo& g) {
+  const int lane = lane_id(), wv = wave_id(), nw = num_waves();
+  const int kind = (int)tl[0], nclass = (int)tl[1], maxm = (int)tl[2], HY = (int)tl[3];
+  const int dzl = -HY, dzh = (kind == SE_CORNER) ? HY - 1 : HY;
+  const int GR = g.ystep * (kTileRows - 1) + 1 + 2 * HY;
+  const int ybase = g.yfirst - HY;
+  u32* gt = gtAll + wv * gtStride;
+  const int zbeg = g.zfirst + dzl, zend = g.zlast + dzh;
+  const int zshift = g.zstep == 2 ? 1 : 0;
+  const int base = g.ystep * lane + HY;
+  u32 pre[4][3];
+  int zi = zbeg + wv;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int gr = lane + kTileRows * k;
+    pre[k][0] = pre[k][1] = pre[k][2] = 0u;
+    if (gr < GR && zi <= zend) {
+      pre[k][0] = src(ybase + gr, zi, g.wx - 1);
+      pre[k][1] = src(ybase + gr, zi, g.wx);
+      pre[k][2] = src(ybase + gr, zi, g.wx + 1);
+    }
+  }
+  for (; zi <= zend; zi += nw) {
+    // stage 1: x-window dilations of this plane's rows into the wave's slice
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      int gr = lane + kTileRows * k;
+      if (gr < GR) {
+        const u32 Lw = pre[k][0], Cw = pre[k][1], Rw = pre[k][2];
+        u32 gg;
+        if (kind == SE_CORNER) {
+          gg = Cw | shl_k(Lw, Cw, 1);
+          gt[gr] = gg;
+          for (int m = 1; m <= maxm; ++m) {
+            gg |= shr_k(Cw, Rw, m) | shl_k(Lw, Cw, m + 1);
+            gt[m * GR + gr] = gg;
+          }
+        } else {
+          gg = Cw;
+          gt[gr] = gg;
+          for (int m = 1; m <= maxm; ++m) {
+            gg |= shr_k(Cw, Rw, m) | shl_k(Lw, Cw, m);
+            gt[m * GR + gr] = gg;
+          }
         }
       }
     }
-    __syncthreads();
+    // prefetch this wave's next plane while this one is reduced
+    const int zn = zi + nw;
+    if (zn <= zend) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        int gr = lane + kTileRows * k;
+        if (gr < GR) {
+          pre[k][0] = src(ybase + gr, zn, g.wx - 1);
+          pre[k][1] = src(ybase + gr, zn, g.wx);
+          pre[k][2] = src(ybase + gr, zn, g.wx + 1);
+        }
+      }
+    }
+    // the slice is written and read by this wave only: in-order LDS suffices
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     // stage 2: chords of each dz class
-    const u32* p = tab + 4;
-    const int base = lane + HY;
-    for (int c = 0; c < h.nclass; ++c) {
-      u32 hw = p[0];
-      int b = (int)(hw & 0xffffu), np = (int)(hw >> 16);
+    const u32* p = tl + 5;
+    for (int c = 0; c < nclass; ++c) {
+      const u32 hw = p[0];
+      const int b = (int)(hw & 0xffffu), np = (int)(hw >> 16);
       ++p;
       u32 disk = 0u;
-      if (h.kind == SE_CORNER) {
+      if (kind == SE_CORNER) {
+#pragma unroll 4
         for (int e = 0; e < np; ++e) {
-          u32 ew = p[e];
-          int a = (int)(ew & 0xffffu), m = (int)(ew >> 16);
-          const u32* gm = L.gt + m * GR;
+          const u32 ew = p[e];
+          const int a = (int)(ew & 0xffffu);
+          const u32* gm = gt + (ew >> 16) * GR;
           disk |= gm[base + a] | gm[base - a - 1];
         }
       } else {
+#pragma unroll 4
         for (int e = 0; e < np; ++e) {
-          u32 ew = p[e];
-          int a = (int)(ew & 0xffffu), m = (int)(ew >> 16);
-          const u32* gm = L.gt + m * GR;
+          const u32 ew = p[e];
+          const int a = (int)(ew & 0xffffu);
+          const u32* gm = gt + (ew >> 16) * GR;
           disk |= gm[base + a] | gm[base - a];
         }
       }
       p += np;
-      // dz values of the class; output plane zo = zi - dz
-      int dz0 = b, dz1 = (h.kind == SE_CORNER) ? -b - 1 : -b;
-      int zo = zi - dz0;
-      if (zo >= tz0 && zo < tz0 + Zt) L.acc[((zo - tz0) % ZR) * kTileRows + lane] |= disk;
-      if (dz1 != dz0) {
+      if (!disk) continue;
+      // output planes zo = zi - dz for the dz values of the class
+      int zo = zi - b;
+      if (zo >= g.zfirst && zo <= g.zlast && ((zo - g.zfirst) & (g.zstep - 1)) == 0)
+        atomicOr(&acc[((zo - g.zfirst) >> zshift) * kTileRows + lane], disk);
+      const int dz1 = (kind == SE_CORNER) ? -b - 1 : -b;
+      if (dz1 != b) {
         zo = zi - dz1;
-        if (zo >= tz0 && zo < tz0 + Zt) L.acc[((zo - tz0) % ZR) * kTileRows + lane] |= disk;
+        if (zo >= g.zfirst && zo <= g.zlast && ((zo - g.zfirst) & (g.zstep - 1)) == 0)
+          atomicOr(&acc[((zo - g.zfirst) >> zshift) * kTileRows + lane], disk);
       }
     }
-    // stage 3: plane zi - dzh has received all contributions
-    int zd = zi - dzh;
-    if (zd >= tz0 && zd < tz0 + Zt) {
-      int s = (zd - tz0) % ZR;
-      u32 v = L.acc[s * kTileRows + lane];
-      L.acc[s * kTileRows + lane] = 0u;
-      L.out[(zd - tz0) * kTileRows + lane] |= v & omask(ty0 + lane, zd);
-    }
+    // keep the next iteration's stage 1 behind this stage 2 in the slice
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
-
-struct AllOnes {
-  __device__ __forceinline__ u32 operator()(int, int) const { return 0xffffffffu; }
-};
 
 struct GridSrc {
   const u32* g;
@@ -215,83 +242,8 @@ struct MaskedSrc {
   }
 };
 
-// Output voxels whose dilation radius (r(p)+1 sphere, r(p) cube) equals rho.
-struct RhoOMask {
-  RadiusField rf;
-  int wx, N, rho, add, ystep, yoff, zstep, zoff;
-  __device__ __forceinline__ u32 operator()(int y, int z) const {
-    if (y >= N || z >= N) return 0u;
-    if (((y - yoff) % ystep) != 0 || ((z - zoff) % zstep) != 0) return 0u;
-    u32 m = 0u;
-    for (int b = 0; b < 32; ++b)
-      if (radius_at(rf, 32 * wx + b, y, z) + add == rho) m |= 1u << b;
-    return m;
-  }
-};
-
-__device__ __forceinline__ Lds carve(u32* smem, int maxm, int maxoff, int Zt) {
-  Lds L;
-  L.HY = maxoff;
-  L.GR = kTileRows + 2 * maxoff;
-  L.ZR = 2 * maxoff + 1;
-  L.gt = smem;
-  L.acc = L.gt + (maxm + 1) * L.GR;
-  L.out = L.acc + L.ZR * kTileRows;
-  return L;
-}
-
-__device__ __forceinline__ void clear_out(Lds L, int Zt) {
-  for (int i = threadIdx.x; i < Zt * kTileRows; i += kTileRows) L.out[i] = 0u;
-}
-
-// Collect the distinct radii relevant to a tile into a 128-bit set in LDS.
-// Dilation: radii of the tile's output voxels (+add). Erosion: radii-1 of the
-// source voxels of K within the tile's input window (halo h).
-__device__ void rho_set_dilate(const RadiusField& rf, int add, int wx, int ty0, int tz0, int Zt, int N, int ystep,
-                               int yoff, int zstep, int zoff, u32* set4) {
-  const int lane = threadIdx.x;
-  if (lane < 4) set4[lane] = 0u;
-  __syncthreads();
-  int y = ty0 + lane;
-  u32 loc[4] = {0, 0, 0, 0};
-  if (y < N && ((y - yoff) % ystep) == 0)
-    for (int z = tz0; z < tz0 + Zt && z < N; ++z) {
-      if (((z - zoff) % zstep) != 0) continue;
-      for (int b = 0; b < 32; ++b) {
-        int r = radius_at(rf, 32 * wx + b, y, z) + add;
-        loc[r >> 5] |= 1u << (r & 31);
-      }
-    }
-  for (int i = 0; i < 4; ++i)
-    if (loc[i]) atomicOr(&set4[i], loc[i]);
-  __syncthreads();
-}
-
-__device__ void rho_set_erode(const RadiusField& rf, const u32* K, int N, int W, int wx, int ty0, int tz0, int Zt,
-                              int halo, u32* set4) {
-  const int lane = threadIdx.x;
-  if (lane < 4) set4[lane] = 0u;
-  __syncthreads();
-  u32 loc[4] = {0, 0, 0, 0};
-  for (int y = ty0 - halo + lane; y < ty0 + kTileRows + halo; y += kTileRows)
-    for (int z = tz0 - halo; z < tz0 + Zt + halo; ++z)
-      for (int w = wx - 1; w <= wx + 1; ++w) {
-        u32 v = row_word(K, N, W, y, z, w);
-        while (v) {
-          int b = __builtin_ctz(v);
-          v &= v - 1u;
-          int r = radius_at(rf, 32 * w + b, y, z) - 1;
-          r = r < 0 ? 0 : r;
-          loc[r >> 5] |= 1u << (r & 31);
-        }
-      }
-  for (int i = 0; i < 4; ++i)
-    if (loc[i]) atomicOr(&set4[i], loc[i]);
-  __syncthreads();
-}
-
 // ---------------------------------------------------------------------------
-// Kernel parameters shared by the four streaming kernels.
+// Kernel parameters shared by the streaming kernels.
 struct MorphArgs {
   int R, N, W;
   int dR0, dR1, dR2;
@@ -300,8 +252,8 @@ struct MorphArgs {
   int se;  // 0 sphere, 1 cube (dilation)
   const u32* tables;
   SETableSet tabs;
-  int rho_lo, rho_hi;  // radius range over the whole grid (for LDS sizing and uniform fast path)
-  int maxm, maxoff;    // LDS carve sizes
+  int maxm, maxoff, tabWords;  // LDS carve sizes
+  int ystep;                   // row step of this launch's lanes
   const u32* I;
   const u32* D;
   const u32* K;
@@ -310,86 +262,316 @@ struct MorphArgs {
   const u16* TG;
   const u16* CL;
   const u16* UG;
+  const uint8_t* brickI;  // per brick (1 word x 8 rows x 8 planes): bit0 any, bit1 all
+  const uint8_t* brickK;
   u16* outFlags;
 };
 
-__device__ __forceinline__ const u32* dil_table(const MorphArgs& a, int rho) { return a.tables + a.tabs.dil[rho]; }
-
-// Runs the dilation predicate F over the tile into L.out. ystep/zstep select
-// the output rows/planes that matter (coarse pass: even only).
-__device__ void dilate_tile(const MorphArgs& a, Lds L, u32* set4, int wx, int ty0, int tz0, int ystep, int zstep) {
-  GridSrc src{a.I, a.N, a.W};
-  clear_out(L, a.Zt);
-  const int add = a.se == 0 ? 1 : 0;
-  if (a.rf.mode == 0) {
-    int rho = (a.rf.r & 0x7f) + add;
-    if (rho > 128 || a.tabs.dil[rho] < 0) return;
-    stream_pass(dil_table(a, rho), src, AllOnes(), L, wx, ty0, tz0, a.Zt);
-    return;
-  }
-  rho_set_dilate(a.rf, add, wx, ty0, tz0, a.Zt, a.N, ystep, 0, zstep, 0, set4);
-  for (int i = 0; i < 4; ++i) {
-    u32 m = set4[i];
-    while (m) {
-      int bit = __builtin_ctz(m);
-      m &= m - 1u;
-      int rho = 32 * i + bit;
-      if (rho > 128 || a.tabs.dil[rho] < 0) continue;
-      RhoOMask om{a.rf, wx, a.N, rho, add, ystep, 0, zstep, 0};
-      stream_pass(dil_table(a, rho), src, om, L, wx, ty0, tz0, a.Zt);
-    }
-  }
+struct Lds {
+  u32* tl;
+  u32* gt;
+  int gtStride;
+  u32* out;  // output planes x 64
+  u32* acc;  // per-radius pass accumulator (per-voxel radii only)
+};
+__device__ __forceinline__ int gt_stride(const MorphArgs& a) {
+  return (a.maxm + 1) * (a.ystep * (kTileRows - 1) + 1 + 2 * a.maxoff);
+}
+__device__ __forceinline__ Lds carve(u32* smem, const MorphArgs& a, int nplanes) {
+  Lds L;
+  L.tl = smem;
+  L.gt = L.tl + a.tabWords;
+  L.gtStride = gt_stride(a);
+  L.out = L.gt + L.gtStride * num_waves();
+  L.acc = L.out + nplanes * kTileRows;
+  return L;
 }
 
-template <bool CENTRE>
-__device__ void erode_tile(const MorphArgs& a, Lds L, u32* set4, int wx, int ty0, int tz0) {
-  clear_out(L, a.Zt);
-  if (a.rf.mode == 0) {
-    int rho = (a.rf.r & 0x7f) - 1;
-    if (rho <= 0 || (CENTRE ? a.tabs.ero_centre[rho] : a.tabs.ero_corner[rho]) < 0) return;
-    const u32* tab = a.tables + (CENTRE ? a.tabs.ero_centre[rho] : a.tabs.ero_corner[rho]);
-    stream_pass(tab, GridSrc{a.K, a.N, a.W}, AllOnes(), L, wx, ty0, tz0, a.Zt);
-    return;
-  }
-  rho_set_erode(a.rf, a.K, a.N, a.W, wx, ty0, tz0, a.Zt, a.maxoff, set4);
-  for (int i = 0; i < 4; ++i) {
-    u32 m = set4[i];
-    while (m) {
-      int bit = __builtin_ctz(m);
-      m &= m - 1u;
-      int rho = 32 * i + bit;
-      if (rho <= 0 || (CENTRE ? a.tabs.ero_centre[rho] : a.tabs.ero_corner[rho]) < 0) continue;
-      const u32* tab = a.tables + (CENTRE ? a.tabs.ero_centre[rho] : a.tabs.ero_corner[rho]);
-      MaskedSrc src{a.K, a.N, a.W, a.rf, rho};
-      stream_pass(tab, src, AllOnes(), L, wx, ty0, tz0, a.Zt);
+__device__ __forceinline__ void load_table(u32* tl, const u32* src) {
+  __syncthreads();
+  const int n = (int)src[4];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) tl[i] = src[i];
+  __syncthreads();
+}
+
+__device__ __forceinline__ int table_off(const MorphArgs& a, int which, int rho) {
+  if (rho < 0 || rho > 128) return -1;
+  return which == 0 ? a.tabs.dil[rho] : (which == 1 ? a.tabs.ero_corner[rho] : a.tabs.ero_centre[rho]);
+}
+
+// Any brick flag (bit) set in the brick region covering rows [y0,y1),
+// planes [z0,z1), words [w0,w1] (clamped to the grid).
+__device__ bool bricks_any(const uint8_t* br, int N, int W, int y0, int y1, int z0, int z1, int w0, int w1,
+                           int bit) {
+  const int NB = N / 8;
+  int by0 = max(y0, 0) >> 3, by1 = (min(y1, N) - 1) >> 3;
+  int bz0 = max(z0, 0) >> 3, bz1 = (min(z1, N) - 1) >> 3;
+  w0 = max(w0, 0);
+  w1 = min(w1, W - 1);
+  int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1, nw = w1 - w0 + 1;
+  int found = 0;
+  if (ny > 0 && nz > 0 && nw > 0) {
+    int n = ny * nz * nw;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      int w = w0 + i % nw, t = i / nw;
+      int by = by0 + t % ny, bz = bz0 + t / ny;
+      if ((br[((size_t)bz * NB + by) * W + w] >> bit) & 1) found = 1;
     }
   }
+  return __syncthreads_or(found) != 0;
+}
+// Every brick of the region has the "all" bit.
+__device__ bool bricks_all(const uint8_t* br, int N, int W, int y0, int y1, int z0, int z1, int w) {
+  const int NB = N / 8;
+  int by0 = max(y0, 0) >> 3, by1 = (min(y1, N) - 1) >> 3;
+  int bz0 = max(z0, 0) >> 3, bz1 = (min(z1, N) - 1) >> 3;
+  int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1;
+  int miss = 0;
+  for (int i = threadIdx.x; i < ny * nz; i += blockDim.x) {
+    int by = by0 + i % ny, bz = bz0 + i / ny;
+    if (!((br[((size_t)bz * NB + by) * W + w] >> 1) & 1)) miss = 1;
+  }
+  return __syncthreads_or(miss) == 0;
+}
+
+__device__ __forceinline__ bool tile_has_flags(const u16* F, int R, int W, int wx, int y0, int y1, int z0, int z1) {
+  int cy0 = y0 >> 1, cy1 = min((y1 + 1) >> 1, R);
+  int cz0 = z0 >> 1, cz1 = min((z1 + 1) >> 1, R);
+  int ny = cy1 - cy0, nz = cz1 - cz0;
+  u32 acc = 0u;
+  if (ny > 0 && nz > 0)
+    for (int i = threadIdx.x; i < ny * nz; i += blockDim.x) acc |= flag_word(F, R, W, cy0 + i % ny, cz0 + i / ny, wx);
+  return __syncthreads_or(acc != 0u) != 0;
+}
+
+// Distinct radii of a tile (128-bit set in LDS). Dilation: radius+add of the
+// output voxels; erosion: radius-1 of the K voxels within the halo.
+__device__ void rho_set_dilate(const RadiusField& rf, int add, const TileGeo& g, int N, u32* set4) {
+  if (threadIdx.x < 4) set4[threadIdx.x] = 0u;
+  __syncthreads();
+  const int y = g.yfirst + g.ystep * lane_id();
+  u32 loc[4] = {0, 0, 0, 0};
+  if (y < N)
+    for (int p = wave_id(); p < g.nout(); p += num_waves()) {
+      const int z = g.zfirst + p * g.zstep;
+      for (int b = 0; b < 32; ++b) {
+        int r = radius_at(rf, 32 * g.wx + b, y, z) + add;
+        loc[r >> 5] |= 1u << (r & 31);
+      }
+    }
+  for (int i = 0; i < 4; ++i)
+    if (loc[i]) atomicOr(&set4[i], loc[i]);
+  __syncthreads();
+}
+__device__ void rho_set_erode(const RadiusField& rf, const u32* K, int N, int W, const TileGeo& g, int halo,
+                              u32* set4) {
+  if (threadIdx.x < 4) set4[threadIdx.x] = 0u;
+  __syncthreads();
+  u32 loc[4] = {0, 0, 0, 0};
+  const int y0 = g.yfirst - halo, y1 = g.yfirst + g.ystep * (kTileRows - 1) + halo;
+  const int z0 = g.zfirst - halo, z1 = g.zlast + halo;
+  const int ny = y1 - y0 + 1;
+  for (int i = threadIdx.x; i < ny * (z1 - z0 + 1); i += blockDim.x) {
+    const int y = y0 + i % ny, z = z0 + i / ny;
+    for (int w = g.wx - 1; w <= g.wx + 1; ++w) {
+      u32 v = row_word(K, N, W, y, z, w);
+      while (v) {
+        int b = __builtin_ctz(v);
+        v &= v - 1u;
+        int r = radius_at(rf, 32 * w + b, y, z) - 1;
+        r = r < 0 ? 0 : r;
+        loc[r >> 5] |= 1u << (r & 31);
+      }
+    }
+  }
+  for (int i = 0; i < 4; ++i)
+    if (loc[i]) atomicOr(&set4[i], loc[i]);
+  __syncthreads();
+}
+
+// Runs the predicate of a pass over the tile and calls fin(zo, F) for every
+// output plane of the thread's row (F = 0 when skip is set). which: 0
+// dilation table (radius+add), 1 erosion corner, 2 erosion centre (radius-1).
+template <class Fin>
+__device__ void run_tile(const MorphArgs& a, Lds L, u32* set4, const TileGeo& g, const u32* src, int which,
+                         bool skip, Fin& fin) {
+  const int lane = lane_id(), wv = wave_id(), nw = num_waves();
+  const int nout = g.nout();
+  const int add = which == 0 ? (a.se == 0 ? 1 : 0) : -1;
+  if (!skip) {
+    for (int i = threadIdx.x; i < nout * kTileRows; i += blockDim.x) L.out[i] = 0u;
+    if (a.rf.mode == 0) {
+      const int rho = (a.rf.r & 0x7f) + add;
+      const int off = table_off(a, which, rho);
+      if (off >= 0 && rho > 0) {
+        load_table(L.tl, a.tables + off);
+        stream(L.tl, GridSrc{src, a.N, a.W}, L.gt, L.gtStride, L.out, g);
+      }
+    } else {
+      // per-voxel radii: one pass per distinct radius, masked into out
+      const int y = g.yfirst + g.ystep * lane;
+      if (which == 0) rho_set_dilate(a.rf, add, g, a.N, set4);
+      else rho_set_erode(a.rf, src, a.N, a.W, g, a.maxoff, set4);
+      for (int i = 0; i < 4; ++i) {
+        u32 m = set4[i];
+        while (m) {
+          const int bit = __builtin_ctz(m);
+          m &= m - 1u;
+          const int rho = 32 * i + bit;
+          const int off = table_off(a, which, rho);
+          if (off < 0 || rho <= 0) continue;
+          load_table(L.tl, a.tables + off);
+          for (int k = threadIdx.x; k < nout * kTileRows; k += blockDim.x) L.acc[k] = 0u;
+          __syncthreads();
+          if (which == 0) stream(L.tl, GridSrc{src, a.N, a.W}, L.gt, L.gtStride, L.acc, g);
+          else stream(L.tl, MaskedSrc{src, a.N, a.W, a.rf, rho}, L.gt, L.gtStride, L.acc, g);
+          __syncthreads();
+          for (int p = wv; p < nout; p += nw) {
+            u32 v = L.acc[p * kTileRows + lane];
+            if (!v) continue;
+            if (which == 0) {
+              const int z = g.zfirst + p * g.zstep;
+              u32 mk = 0u;
+              if (y < a.N)
+                for (int b = 0; b < 32; ++b)
+                  if (radius_at(a.rf, 32 * g.wx + b, y, z) + add == rho) mk |= 1u << b;
+              v &= mk;
+            }
+            L.out[p * kTileRows + lane] |= v;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int p = wv; p < nout; p += nw) fin(g.zfirst + p * g.zstep, skip ? 0u : L.out[p * kTileRows + lane]);
+}
+
+__device__ __forceinline__ TileGeo tile_geo(const MorphArgs& a, int yoff, int zoff, int zstep) {
+  TileGeo g;
+  g.wx = blockIdx.x;
+  const int rows = a.ystep * kTileRows;
+  g.yfirst = blockIdx.y * rows + yoff;
+  g.ystep = a.ystep;
+  g.tz0 = blockIdx.z * a.Zt;
+  g.zfirst = g.tz0 + zoff;
+  int zl = min(g.tz0 + a.Zt, a.N) - 1;
+  g.zlast = zl < g.zfirst ? g.zfirst - zstep : g.zfirst + ((zl - g.zfirst) / zstep) * zstep;
+  g.zstep = zstep;
+  return g;
 }
 
 // ---------------------------------------------------------------------------
-// Coarse dilation -> DF flags.
-__global__ void __launch_bounds__(kTileRows) k_dil_coarse(MorphArgs a) {
-  extern __shared__ u32 smem[];
-  __shared__ u32 set4[4];
-  const int wx = blockIdx.x, ty0 = blockIdx.y * kTileRows, tz0 = blockIdx.z * a.Zt;
-  Lds L = carve(smem, a.maxm, a.maxoff, a.Zt);
-  dilate_tile(a, L, set4, wx, ty0, tz0, 2, 2);
-  __syncthreads();
-  const int lane = threadIdx.x;
-  const int y = ty0 + lane;
-  if (y >= a.N || (y & 1)) return;
-  const int cy = y >> 1;
-  const u32 xin = cell_range_mask(wx, 0, a.dR0);
-  for (int z = tz0; z < tz0 + a.Zt && z < a.N; z += 2) {
-    const int cz = z >> 1;
-    u32 f = gather2(L.out[(z - tz0) * kTileRows + lane]);  // F at even x
-    u32 i0 = row_word(a.I, a.N, a.W, y, z, wx), i1 = row_word(a.I, a.N, a.W, y + 1, z, wx);
-    u32 i2 = row_word(a.I, a.N, a.W, y, z + 1, wx), i3 = row_word(a.I, a.N, a.W, y + 1, z + 1, wx);
+// Epilogues.
+struct DilCoarseFin {  // DF(c) = I(c) full | (c < dR & F(2c))
+  const MorphArgs* a;
+  int y;
+  u32 xin;
+  __device__ void operator()(int z, u32 f) const {
+    if (y >= a->N) return;
+    const int cy = y >> 1, cz = z >> 1;
+    u32 i0 = row_word(a->I, a->N, a->W, y, z, blockIdx.x), i1 = row_word(a->I, a->N, a->W, y + 1, z, blockIdx.x);
+    u32 i2 = row_word(a->I, a->N, a->W, y, z + 1, blockIdx.x);
+    u32 i3 = row_word(a->I, a->N, a->W, y + 1, z + 1, blockIdx.x);
     u32 full = i0 & i1 & i2 & i3;
     u32 ifull = gather2(full & (full >> 1));
-    u32 inr = (cy < a.dR1 && cz < a.dR2) ? xin : 0u;
-    a.outFlags[((size_t)cz * a.R + cy) * a.W + wx] = (u16)(ifull | (f & inr));
+    u32 inr = (cy < a->dR1 && cz < a->dR2) ? xin : 0u;
+    a->outFlags[((size_t)cz * a->R + cy) * a->W + blockIdx.x] = (u16)(ifull | (gather2(f) & inr));
   }
+};
+struct DilFineFin {  // D = TG ? F : (DF ? 1 : I)
+  const MorphArgs* a;
+  int y;
+  __device__ void operator()(int z, u32 f) const {
+    if (y >= a->N) return;
+    const int cy = y >> 1, cz = z >> 1;
+    u32 df = expand2(flag_word(a->DF, a->R, a->W, cy, cz, blockIdx.x));
+    u32 tg = expand2(flag_word(a->TG, a->R, a->W, cy, cz, blockIdx.x));
+    u32 d = df | row_word(a->I, a->N, a->W, y, z, blockIdx.x);
+    a->outRows[((size_t)z * a->N + y) * a->W + blockIdx.x] = (tg & f) | (~tg & d);
+  }
+};
+struct EroCoarseFin {  // CL(c) = c < dR & D(c) != 0 & I(c) != 0xff & Ec(2c+1)
+  const MorphArgs* a;
+  int y;  // odd row 2cy+1
+  u32 xin;
+  __device__ void operator()(int z, u32 e) const {
+    if (y >= a->N) return;
+    const int cy = y >> 1, cz = z >> 1, y0 = y - 1, z0 = z - 1, w = blockIdx.x;
+    u32 d0 = row_word(a->D, a->N, a->W, y0, z0, w), d1 = row_word(a->D, a->N, a->W, y0 + 1, z0, w);
+    u32 d2 = row_word(a->D, a->N, a->W, y0, z0 + 1, w), d3 = row_word(a->D, a->N, a->W, y0 + 1, z0 + 1, w);
+    u32 dany = d0 | d1 | d2 | d3;
+    u32 i0 = row_word(a->I, a->N, a->W, y0, z0, w), i1 = row_word(a->I, a->N, a->W, y0 + 1, z0, w);
+    u32 i2 = row_word(a->I, a->N, a->W, y0, z0 + 1, w), i3 = row_word(a->I, a->N, a->W, y0 + 1, z0 + 1, w);
+    u32 full = i0 & i1 & i2 & i3;
+    u32 inr = (cy < a->dR1 && cz < a->dR2) ? xin : 0u;
+    a->outFlags[((size_t)cz * a->R + cy) * a->W + w] =
+        (u16)(gather2(e >> 1) & gather2(dany | (dany >> 1)) & ~gather2(full & (full >> 1)) & inr);
+  }
+};
+struct EroFineFin {  // C = UG ? ~Ef : C'
+  const MorphArgs* a;
+  int y;
+  __device__ void operator()(int z, u32 e) const {
+    if (y >= a->N) return;
+    const int cy = y >> 1, cz = z >> 1;
+    u32 cl = expand2(flag_word(a->CL, a->R, a->W, cy, cz, blockIdx.x));
+    u32 ug = expand2(flag_word(a->UG, a->R, a->W, cy, cz, blockIdx.x));
+    u32 cw = row_word(a->D, a->N, a->W, y, z, blockIdx.x) & ~cl;
+    a->outRows[((size_t)z * a->N + y) * a->W + blockIdx.x] = (ug & ~e) | (~ug & cw);
+  }
+};
+
+constexpr int kMaxWaves = 8;
+
+// Coarse dilation -> DF flags (lanes own the even rows, even planes).
+__global__ void __launch_bounds__(64 * kMaxWaves) k_dil_coarse(MorphArgs a) {
+  extern __shared__ u32 smem[];
+  __shared__ u32 set4[4];
+  TileGeo g = tile_geo(a, 0, 0, 2);
+  Lds L = carve(smem, a, g.nout());
+  const int y1 = g.yfirst + a.ystep * kTileRows;
+  // F = 0 when no occupied voxel is in reach; F unused when every cell is full
+  bool skip = !bricks_any(a.brickI, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff, g.zfirst - a.maxoff,
+                          g.zlast + 2 + a.maxoff, g.wx - 1, g.wx + 1, 0) ||
+              bricks_all(a.brickI, a.N, a.W, g.yfirst, y1, g.zfirst, g.zlast + 2, g.wx);
+  DilCoarseFin fin{&a, g.yfirst + 2 * lane_id(), cell_range_mask(g.wx, 0, a.dR0)};
+  run_tile(a, L, set4, g, a.I, 0, skip, fin);
+}
+
+// Fine dilation -> D rows.
+__global__ void __launch_bounds__(64 * kMaxWaves) k_dil_fine(MorphArgs a) {
+  extern __shared__ u32 smem[];
+  __shared__ u32 set4[4];
+  TileGeo g = tile_geo(a, 0, 0, 1);
+  Lds L = carve(smem, a, g.nout());
+  bool skip = !tile_has_flags(a.TG, a.R, a.W, g.wx, g.yfirst, g.yfirst + kTileRows, g.zfirst, g.zlast + 1);
+  DilFineFin fin{&a, g.yfirst + lane_id()};
+  run_tile(a, L, set4, g, a.I, 0, skip, fin);
+}
+
+// Coarse erosion -> CL flags (lanes own the odd rows, odd planes).
+__global__ void __launch_bounds__(64 * kMaxWaves) k_ero_coarse(MorphArgs a) {
+  extern __shared__ u32 smem[];
+  __shared__ u32 set4[4];
+  TileGeo g = tile_geo(a, 1, 1, 2);
+  Lds L = carve(smem, a, g.nout());
+  const int y1 = g.yfirst + a.ystep * kTileRows;
+  bool skip = !bricks_any(a.brickK, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff, g.zfirst - a.maxoff,
+                          g.zlast + 1 + a.maxoff, g.wx - 1, g.wx + 1, 0);
+  EroCoarseFin fin{&a, g.yfirst + 2 * lane_id(), cell_range_mask(g.wx, 0, a.dR0)};
+  run_tile(a, L, set4, g, a.K, 1, skip, fin);
+}
+
+// Fine erosion -> C rows.
+__global__ void __launch_bounds__(64 * kMaxWaves) k_ero_fine(MorphArgs a) {
+  extern __shared__ u32 smem[];
+  __shared__ u32 set4[4];
+  TileGeo g = tile_geo(a, 0, 0, 1);
+  Lds L = carve(smem, a, g.nout());
+  bool skip = !tile_has_flags(a.UG, a.R, a.W, g.wx, g.yfirst, g.yfirst + kTileRows, g.zfirst, g.zlast + 1);
+  EroFineFin fin{&a, g.yfirst + lane_id()};
+  run_tile(a, L, set4, g, a.K, 2, skip, fin);
 }
 
 // Tight contour of DF -> TG flags (one thread per cell-row word).
@@ -415,97 +597,46 @@ __global__ void k_tight(MorphArgs a) {
   a.outFlags[gid] = (u16)(any & ~all & valid);
 }
 
-__device__ __forceinline__ bool tile_has_flags(const u16* F, int R, int W, int wx, int ty0, int tz0, int Zt, int N) {
-  // cells covered by the tile's rows/planes
-  int cy0 = ty0 >> 1, cy1 = min((ty0 + kTileRows) >> 1, R);
-  int cz0 = tz0 >> 1, cz1 = min((tz0 + Zt + 1) >> 1, R);
-  u32 acc = 0u;
-  for (int cz = cz0; cz < cz1; ++cz)
-    for (int cy = cy0 + (int)threadIdx.x; cy < cy1; cy += kTileRows) acc |= flag_word(F, R, W, cy, cz, wx);
-  return __syncthreads_or(acc != 0u) != 0;
+// Brick summary of a grid: one wave per brick of 1 word x 8 rows x 8 planes
+// (lane = row + 8*plane); bit0 = any voxel set, bit1 = all set.
+__global__ void __launch_bounds__(64) k_brick_summary(const u32* __restrict__ G, int N, int W, uint8_t* out) {
+  const int NB = N / 8;
+  const long long brick = blockIdx.x;
+  const int w = (int)(brick % W);
+  const long long t = brick / W;
+  const int by = (int)(t % NB), bz = (int)(t / NB);
+  const int y = 8 * by + (threadIdx.x & 7), z = 8 * bz + (threadIdx.x >> 3);
+  const u32 v = G[((size_t)z * N + y) * W + w];
+  const bool any = __any(v != 0u);
+  const bool all = __all(v == 0xffffffffu);
+  if (threadIdx.x == 0) out[brick] = (uint8_t)((any ? 1 : 0) | (all ? 2 : 0));
 }
 
-// Fine dilation -> D rows.
-__global__ void __launch_bounds__(kTileRows) k_dil_fine(MorphArgs a) {
-  extern __shared__ u32 smem[];
-  __shared__ u32 set4[4];
-  const int wx = blockIdx.x, ty0 = blockIdx.y * kTileRows, tz0 = blockIdx.z * a.Zt;
-  const int lane = threadIdx.x;
-  const int y = ty0 + lane;
-  Lds L = carve(smem, a.maxm, a.maxoff, a.Zt);
-  bool work = tile_has_flags(a.TG, a.R, a.W, wx, ty0, tz0, a.Zt, a.N);
-  if (work) {
-    dilate_tile(a, L, set4, wx, ty0, tz0, 1, 1);
-    __syncthreads();
-  }
-  if (y >= a.N) return;
-  const int cy = y >> 1;
-  for (int z = tz0; z < tz0 + a.Zt && z < a.N; ++z) {
-    const int cz = z >> 1;
-    u32 df = expand2(flag_word(a.DF, a.R, a.W, cy, cz, wx));
-    u32 iw = row_word(a.I, a.N, a.W, y, z, wx);
-    u32 d = df | iw;
-    if (work) {
-      u32 tg = expand2(flag_word(a.TG, a.R, a.W, cy, cz, wx));
-      d = (tg & L.out[(z - tz0) * kTileRows + lane]) | (~tg & d);
-    }
-    a.outRows[((size_t)z * a.N + y) * a.W + wx] = d;
-  }
-}
-
-// Outer contour K of D (Grid.cu:1230-1327), one thread per row word.
-__global__ void k_contour_K(MorphArgs a) {
-  long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long total = (long long)a.N * a.N * a.W;
-  if (gid >= total) return;
-  int wx = (int)(gid % a.W);
-  long long t = gid / a.W;
-  int y = (int)(t % a.N), z = (int)(t / a.N);
+// Outer contour K of D (Grid.cu:1230-1327) and its brick summary; one wave
+// per brick, one lane per row word.
+__global__ void __launch_bounds__(64) k_contour_K(MorphArgs a, uint8_t* brickK) {
+  const int N = a.N, W = a.W, NB = N / 8;
+  const long long brick = blockIdx.x;
+  const int wx = (int)(brick % W);
+  const long long t = brick / W;
+  const int by = (int)(t % NB), bz = (int)(t / NB);
+  const int y = 8 * by + (threadIdx.x & 7), z = 8 * bz + (threadIdx.x >> 3);
   const u32* D = a.D;
-  int N = a.N, W = a.W;
   u32 c = row_word(D, N, W, y, z, wx);
   u32 l = row_word(D, N, W, y, z, wx - 1), r = row_word(D, N, W, y, z, wx + 1);
   u32 xn = (c << 1) | (l >> 31) | (c >> 1) | (r << 31);
   u32 n6 = xn | row_word(D, N, W, y - 1, z, wx) | row_word(D, N, W, y + 1, z, wx) |
            row_word(D, N, W, y, z - 1, wx) | row_word(D, N, W, y, z + 1, wx);
-  // cell fullness over the 4 rows of this cell row
   int y0 = y & ~1, z0 = z & ~1;
   u32 f = row_word(D, N, W, y0, z0, wx) & row_word(D, N, W, y0 + 1, z0, wx) & row_word(D, N, W, y0, z0 + 1, wx) &
           row_word(D, N, W, y0 + 1, z0 + 1, wx);
   u32 cfull = expand2(gather2(f & (f >> 1)));
   int cy = y >> 1, cz = z >> 1;
   u32 inner = (cy >= 1 && cy <= a.R - 2 && cz >= 1 && cz <= a.R - 2) ? expand2(cell_range_mask(wx, 1, a.R - 1)) : 0u;
-  a.outRows[gid] = ~c & n6 & ~cfull & inner;
-}
-
-// Coarse erosion -> CL flags.
-__global__ void __launch_bounds__(kTileRows) k_ero_coarse(MorphArgs a) {
-  extern __shared__ u32 smem[];
-  __shared__ u32 set4[4];
-  const int wx = blockIdx.x, ty0 = blockIdx.y * kTileRows, tz0 = blockIdx.z * a.Zt;
-  Lds L = carve(smem, a.maxm, a.maxoff, a.Zt);
-  erode_tile<false>(a, L, set4, wx, ty0, tz0);
-  __syncthreads();
-  const int lane = threadIdx.x;
-  const int y = ty0 + lane;
-  if (y >= a.N || !(y & 1)) return;
-  const int cy = y >> 1;
-  const u32 xin = cell_range_mask(wx, 0, a.dR0);
-  for (int z = tz0 + ((tz0 & 1) ? 0 : 1); z < tz0 + a.Zt && z < a.N; z += 2) {
-    const int cz = z >> 1;
-    u32 e = gather2(L.out[(z - tz0) * kTileRows + lane] >> 1);  // Ec at odd x = 2c+1
-    int y0 = y - 1, z0 = z - 1;
-    u32 d0 = row_word(a.D, a.N, a.W, y0, z0, wx), d1 = row_word(a.D, a.N, a.W, y0 + 1, z0, wx);
-    u32 d2 = row_word(a.D, a.N, a.W, y0, z0 + 1, wx), d3 = row_word(a.D, a.N, a.W, y0 + 1, z0 + 1, wx);
-    u32 dany = d0 | d1 | d2 | d3;
-    u32 dnz = gather2(dany | (dany >> 1));
-    u32 i0 = row_word(a.I, a.N, a.W, y0, z0, wx), i1 = row_word(a.I, a.N, a.W, y0 + 1, z0, wx);
-    u32 i2 = row_word(a.I, a.N, a.W, y0, z0 + 1, wx), i3 = row_word(a.I, a.N, a.W, y0 + 1, z0 + 1, wx);
-    u32 full = i0 & i1 & i2 & i3;
-    u32 ifull = gather2(full & (full >> 1));
-    u32 inr = (cy < a.dR1 && cz < a.dR2) ? xin : 0u;
-    a.outFlags[((size_t)cz * a.R + cy) * a.W + wx] = (u16)(e & dnz & ~ifull & inr);
-  }
+  u32 k = ~c & n6 & ~cfull & inner;
+  a.outRows[((size_t)z * N + y) * W + wx] = k;
+  const bool any = __any(k != 0u);
+  if (threadIdx.x == 0) brickK[brick] = any ? 1 : 0;
 }
 
 // C' rows of a cell row (4 fine rows), masked by CL.
@@ -550,41 +681,22 @@ __global__ void k_base_contour(MorphArgs a) {
   a.outFlags[gid] = (u16)(gather2(diff | (diff >> 1)) & valid);
 }
 
-// Fine erosion -> C rows.
-__global__ void __launch_bounds__(kTileRows) k_ero_fine(MorphArgs a) {
-  extern __shared__ u32 smem[];
-  __shared__ u32 set4[4];
-  const int wx = blockIdx.x, ty0 = blockIdx.y * kTileRows, tz0 = blockIdx.z * a.Zt;
-  const int lane = threadIdx.x;
-  const int y = ty0 + lane;
-  Lds L = carve(smem, a.maxm, a.maxoff, a.Zt);
-  bool work = tile_has_flags(a.UG, a.R, a.W, wx, ty0, tz0, a.Zt, a.N);
-  if (work) {
-    erode_tile<true>(a, L, set4, wx, ty0, tz0);
-    __syncthreads();
-  }
-  if (y >= a.N) return;
-  const int cy = y >> 1;
-  for (int z = tz0; z < tz0 + a.Zt && z < a.N; ++z) {
-    const int cz = z >> 1;
-    u32 cl = expand2(flag_word(a.CL, a.R, a.W, cy, cz, wx));
-    u32 cw = row_word(a.D, a.N, a.W, y, z, wx) & ~cl;
-    if (work) {
-      u32 ug = expand2(flag_word(a.UG, a.R, a.W, cy, cz, wx));
-      cw = (ug & ~L.out[(z - tz0) * kTileRows + lane]) | (~ug & cw);
-    }
-    a.outRows[((size_t)z * a.N + y) * a.W + wx] = cw;
-  }
+// LDS bytes of a streaming launch with `waves` waves per workgroup.
+size_t lds_bytes(const MorphArgs& a, bool var, int waves) {
+  const int GR = a.ystep * (kTileRows - 1) + 1 + 2 * a.maxoff;
+  const int nplanes = a.ystep == 2 ? a.Zt / 2 : a.Zt;
+  size_t words = (size_t)a.tabWords + (size_t)(a.maxm + 1) * GR * waves + (size_t)nplanes * kTileRows;
+  if (var) words += (size_t)nplanes * kTileRows;
+  return words * 4;
+}
+// Waves per workgroup: as many as fit a 64 KiB LDS budget, at most 8.
+int stream_waves(const MorphArgs& a, bool var) {
+  int w = kMaxWaves;
+  while (w > 1 && lds_bytes(a, var, w) > 64 * 1024) --w;
+  return w;
 }
 
-size_t lds_bytes(int maxm, int maxoff, int Zt) {
-  int GR = kTileRows + 2 * maxoff;
-  int ZR = 2 * maxoff + 1;
-  return (size_t)4 * ((size_t)(maxm + 1) * GR + (size_t)ZR * kTileRows + (size_t)Zt * kTileRows);
-}
-
-MorphArgs base_args(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf, const SETableSet& tabs,
-                    int rho_lo, int rho_hi) {
+MorphArgs base_args(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf, const SETableSet& tabs) {
   MorphArgs a{};
   a.R = d.R;
   a.N = d.N;
@@ -596,8 +708,6 @@ MorphArgs base_args(const MorphDims& d, const MorphBuffers& b, const RadiusField
   a.rf = rf;
   a.tables = b.tables;
   a.tabs = tabs;
-  a.rho_lo = rho_lo;
-  a.rho_hi = rho_hi;
   a.I = b.I;
   a.D = b.D;
   a.K = b.K;
@@ -605,67 +715,88 @@ MorphArgs base_args(const MorphDims& d, const MorphBuffers& b, const RadiusField
   a.TG = b.TG;
   a.CL = b.CL;
   a.UG = b.UG;
+  a.brickI = b.brickI;
+  a.brickK = b.brickK;
   return a;
+}
+
+dim3 stream_grid(const MorphArgs& a) {
+  const int rows = a.ystep * kTileRows;
+  return dim3(a.W, (a.N + rows - 1) / rows, (a.N + a.Zt - 1) / a.Zt);
 }
 
 }  // namespace
 
-// rho_lo/rho_hi: range of the per-voxel radius r over the field.
+// rho_lo/rho_hi: range of the per-voxel radius r over the field; tabWords:
+// longest SE table of the launch.
 hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf, int se,
-                         const SETableSet& tabs, int rho_lo, int rho_hi, hipStream_t s, hipEvent_t* ev) {
-  MorphArgs a = base_args(d, b, rf, tabs, rho_lo, rho_hi);
+                         const SETableSet& tabs, int rho_lo, int rho_hi, int tabWords, hipStream_t s,
+                         hipEvent_t* ev) {
+  (void)rho_lo;
+  MorphArgs a = base_args(d, b, rf, tabs);
   a.se = se;
-  int rmax = rho_hi + (se == 0 ? 1 : 0);
-  // corner tables: maxm <= rmax-1 (sphere) or r (cube); offsets <= rmax (+1 for cube)
+  const int rmax = rho_hi + (se == 0 ? 1 : 0);
+  // corner tables: m <= rmax-1 (sphere) or r (cube); offsets <= rmax (+1 for cube)
   a.maxm = se == 0 ? rmax - 1 : rho_hi;
   a.maxoff = se == 0 ? rmax : rho_hi + 1;
-  size_t lds = lds_bytes(a.maxm, a.maxoff, a.Zt);
-  dim3 tiles(d.W, (d.N + kTileRows - 1) / kTileRows, (d.N + a.Zt - 1) / a.Zt);
+  a.tabWords = tabWords;
+  const bool var = rf.mode != 0;
   long long cellwords = (long long)d.R * d.R * d.W;
-  dim3 ewg((unsigned)((cellwords + 255) / 256));
+  long long bricks = (long long)(d.N / 8) * (d.N / 8) * d.W;
   if (ev) GM_CHECK(hipEventRecord(ev[0], s));
+  hipLaunchKernelGGL(k_brick_summary, dim3((unsigned)bricks), dim3(64), 0, s, b.I, d.N, d.W, b.brickI);
+  GM_CHECK(hipGetLastError());
+  a.ystep = 2;
   a.outFlags = b.DF;
-  hipLaunchKernelGGL(k_dil_coarse, tiles, dim3(kTileRows), lds, s, a);
+  int nw = stream_waves(a, var);
+  hipLaunchKernelGGL(k_dil_coarse, stream_grid(a), dim3(64 * nw), lds_bytes(a, var, nw), s, a);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
   a.outFlags = b.TG;
-  hipLaunchKernelGGL(k_tight, ewg, dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_tight, dim3((unsigned)((cellwords + 255) / 256)), dim3(256), 0, s, a);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
+  a.ystep = 1;
   a.outRows = b.D;
-  hipLaunchKernelGGL(k_dil_fine, tiles, dim3(kTileRows), lds, s, a);
+  nw = stream_waves(a, var);
+  hipLaunchKernelGGL(k_dil_fine, stream_grid(a), dim3(64 * nw), lds_bytes(a, var, nw), s, a);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[3], s));
   return hipSuccess;
 }
 
-hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf,
-                        const SETableSet& tabs, int rho_lo, int rho_hi, hipStream_t s, hipEvent_t* ev) {
-  MorphArgs a = base_args(d, b, rf, tabs, rho_lo, rho_hi);
+hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf, const SETableSet& tabs,
+                        int rho_lo, int rho_hi, int tabWords, hipStream_t s, hipEvent_t* ev) {
+  (void)rho_lo;
+  MorphArgs a = base_args(d, b, rf, tabs);
   int rmax = rho_hi - 1;  // erosion radius r-1
   if (rmax < 1) rmax = 1;
   // corner: m <= rmax-1, offsets <= rmax; centre: m <= rmax, offsets <= rmax
   a.maxm = rmax;
   a.maxoff = rmax;
-  size_t lds = lds_bytes(a.maxm, a.maxoff, a.Zt);
-  dim3 tiles(d.W, (d.N + kTileRows - 1) / kTileRows, (d.N + a.Zt - 1) / a.Zt);
-  long long rowwords = (long long)d.N * d.N * d.W;
+  a.tabWords = tabWords;
+  const bool var = rf.mode != 0;
   long long cellwords = (long long)d.R * d.R * d.W;
+  long long bricks = (long long)(d.N / 8) * (d.N / 8) * d.W;
   if (ev) GM_CHECK(hipEventRecord(ev[0], s));
   a.outRows = b.K;
-  hipLaunchKernelGGL(k_contour_K, dim3((unsigned)((rowwords + 255) / 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_contour_K, dim3((unsigned)bricks), dim3(64), 0, s, a, b.brickK);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
+  a.ystep = 2;
   a.outFlags = b.CL;
-  hipLaunchKernelGGL(k_ero_coarse, tiles, dim3(kTileRows), lds, s, a);
+  int nw = stream_waves(a, var);
+  hipLaunchKernelGGL(k_ero_coarse, stream_grid(a), dim3(64 * nw), lds_bytes(a, var, nw), s, a);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   a.outFlags = b.UG;
   hipLaunchKernelGGL(k_base_contour, dim3((unsigned)((cellwords + 255) / 256)), dim3(256), 0, s, a);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[3], s));
+  a.ystep = 1;
   a.outRows = b.C;
-  hipLaunchKernelGGL(k_ero_fine, tiles, dim3(kTileRows), lds, s, a);
+  nw = stream_waves(a, var);
+  hipLaunchKernelGGL(k_ero_fine, stream_grid(a), dim3(64 * nw), lds_bytes(a, var, nw), s, a);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[4], s));
   return hipSuccess;
@@ -680,8 +811,11 @@ static int isq_lt(long long X) {  // max m >= 0 with m*m < X (X >= 1)
 }
 static long long ecen(long long t) { return t == 0 ? 0 : (2 * t - 1) * (2 * t - 1); }
 
+// Layout (u32): [0] kind, [1] nclass, [2] max m, [3] max |offset| (HY),
+// [4] total length in words, then per class: header (b | npairs<<16) and
+// npairs chords (a | m<<16).
 std::vector<u32> build_se_table(int kind, int rho, bool cube) {
-  std::vector<u32> out = {(u32)kind, 0, 0, 0};
+  std::vector<u32> out = {(u32)kind, 0, 0, 0, 0};
   int nclass = 0, maxm = 0, maxab = 0;
   if (kind == SE_CORNER) {
     int bmax = cube ? rho : rho - 1;
@@ -724,6 +858,7 @@ std::vector<u32> build_se_table(int kind, int rho, bool cube) {
   }
   out[1] = (u32)nclass;
   out[2] = (u32)maxm;
+  out[4] = (u32)out.size();
   return out;
 }
 

@@ -83,7 +83,18 @@ def main():
     ap.add_argument("--seed", type=int, default=2017)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="compare the closing grid with the oracle")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        src = torch.ones(args.copy_probe << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe << 20}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the closing path is single-device (--gpus %d refused)" % args.gpus,
               file=sys.stderr)

@@ -37,6 +37,21 @@ __constant__ McConst g_mc;
 __constant__ unsigned long long g_tri[256];
 
 constexpr int kMcThreads = 256;
+
+// Per-workgroup LDS copy of the case tables: the lookups are per lane, so
+// they would otherwise be divergent constant-memory loads.
+struct McTab {
+  u32 ntri[256];
+  u32 len[256];
+  unsigned long long tri[256];
+};
+__device__ __forceinline__ void load_mc_tab(McTab& t) {
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+    t.ntri[i] = g_mc.ntri[i];
+    t.len[i] = g_mc.len[i];
+    t.tri[i] = g_tri[i];
+  }
+}
 constexpr int kBox = 24;  // fine voxels per axis held in LDS (block 16 + halo)
 constexpr int kLo = 2;    // halo below the block
 
@@ -156,12 +171,16 @@ __device__ u32 block_exscan(u32 v, u32* lds16, u32* total) {
   return off + x - v;
 }
 
+constexpr int kListCap = 4096;  // work-list window (vertices or triangles of a block)
+
 struct BlockLds {
+  McTab tab;
   Box box;
   uint8_t cases[4096];  // [local cell * 8 + cube], 0 for cells outside M range
   u32 vOff[512];
   u32 tOff[512];
   u32 red[16];
+  uint16_t list[kListCap];  // cube index << 3 | item, by local output index
 };
 
 // Cases of the cubes of the block's cells (uniform cells and cells outside
@@ -177,7 +196,7 @@ __device__ void stage_cases(BlockLds& S, int blk, int R) {
         int v = cube_case(S.box, 2 * cx + (j & 1), 2 * cy + ((j >> 1) & 1), 2 * cz + ((j >> 2) & 1));
         S.cases[8 * l + j] = (uint8_t)v;
         nv += own_verts(v);
-        nt += g_mc.ntri[v];
+        nt += S.tab.ntri[v];
       }
     } else {
 #pragma unroll
@@ -213,6 +232,7 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_count(const u32* __restrict__
                                                           u32* blockT) {
   __shared__ Box bx;
   __shared__ u32 red[16];
+  __shared__ McTab tab;
   const int blk = blockIdx.x;
   int c0x, c0y, c0z;
   if (block_outside(blk, R, c0x, c0y, c0z)) {
@@ -222,6 +242,7 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_count(const u32* __restrict__
     }
     return;
   }
+  load_mc_tab(tab);
   // cells of the block need fine voxels [16b, 16b+17) per axis
   load_box(bx, C, 2 * R, 2 * c0x, 2 * c0y, 2 * c0z, 17, 17);
   // uniform block: no mixed cube
@@ -244,7 +265,7 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_count(const u32* __restrict__
       for (int j = 0; j < 8; ++j) {
         int v = cube_case(bx, 2 * cx + (j & 1), 2 * cy + ((j >> 1) & 1), 2 * cz + ((j >> 2) & 1));
         nv += own_verts(v);
-        nt += g_mc.ntri[v];
+        nt += tab.ntri[v];
       }
     }
   }
@@ -365,10 +386,21 @@ __device__ __forceinline__ void edge_pos(const Geo& g, const Box& b, int fx, int
   else o[2] = lerp_edge(pz, __fadd_rn(pz, g.h), v0, (float)b.at(fx, fy, fz + 1));
 }
 
-// Owner of local edge e of a cube: offset of the owning cube and its edge.
-__constant__ int8_t g_own[12][4] = {{0, 0, 0, 0}, {1, 0, 0, 3}, {0, 1, 0, 0}, {0, 0, 0, 3},
-                                    {0, 0, 1, 0}, {1, 0, 1, 3}, {0, 1, 1, 0}, {0, 0, 1, 3},
-                                    {0, 0, 0, 8}, {1, 0, 0, 8}, {1, 1, 0, 8}, {0, 1, 0, 8}};
+// Owner of local edge e of a cube: offset (dx,dy,dz) of the owning cube and
+// which of its own edges (0, 3, 8) it is, packed 5 bits per edge:
+// dx | dy<<1 | dz<<2 | k<<3 with k = 0, 1, 2 for edges 0, 3, 8.
+constexpr unsigned long long kOwnPacked =
+    (0ull << 0) | ((1ull | (1ull << 3)) << 5) | (2ull << 10) | ((1ull << 3) << 15) | (4ull << 20) |
+    ((5ull | (1ull << 3)) << 25) | (6ull << 30) | ((4ull | (1ull << 3)) << 35) | ((2ull << 3) << 40) |
+    ((1ull | (2ull << 3)) << 45) | ((3ull | (2ull << 3)) << 50) | ((2ull | (2ull << 3)) << 55);
+__device__ __forceinline__ void own_of(int e, int& dx, int& dy, int& dz, int& oe) {
+  const int v = (int)((kOwnPacked >> (5 * e)) & 31ull);
+  dx = v & 1;
+  dy = (v >> 1) & 1;
+  dz = (v >> 2) & 1;
+  const int k = v >> 3;
+  oe = k == 0 ? 0 : (k == 1 ? 3 : 8);
+}
 
 __device__ __forceinline__ int rank_of(int v, int e) {  // index of owned edge e among the cube's own vertices
   int c0 = v & 1;
@@ -387,19 +419,23 @@ __constant__ int8_t g_share[3][4][4] = {
     {{0, 0, 0, 8}, {-1, 0, 0, 9}, {0, -1, 0, 11}, {-1, -1, 0, 10}}};
 
 __device__ __forceinline__ void local_edge_pos(const Geo& g, const Box& b, int fx, int fy, int fz, int e, float* p) {
-  edge_pos(g, b, fx + g_own[e][0], fy + g_own[e][1], fz + g_own[e][2], g_own[e][3], p);
+  int dx, dy, dz, oe;
+  own_of(e, dx, dy, dz, oe);
+  edge_pos(g, b, fx + dx, fy + dy, fz + dz, oe, p);
 }
 
-__device__ void normal_sum(const Geo& geo, const Box& bx, int R, int fx, int fy, int fz, int q, float* n) {
+__device__ void normal_sum(const McTab& tab, const Geo& geo, const Box& bx, int R, int fx, int fy, int fz, int q,
+                           float* n) {
   n[0] = n[1] = n[2] = 0.f;
+#pragma unroll
   for (int s = 0; s < 4; ++s) {
     int gx = fx + g_share[q][s][0], gy = fy + g_share[q][s][1], gz = fz + g_share[q][s][2];
     int le = g_share[q][s][3];
     if (!cell_in_m_range(gx >> 1, gy >> 1, gz >> 1, R)) continue;
     int gv = cube_case(bx, gx, gy, gz);
     if (!mixed(gv)) continue;
-    unsigned long long tri = g_tri[gv];
-    u32 cnt = g_mc.ntri[gv];
+    unsigned long long tri = tab.tri[gv];
+    u32 cnt = tab.ntri[gv];
     for (u32 t = 0; t < cnt; ++t) {
       int e0 = (int)((tri >> (12 * t)) & 0xf), e1 = (int)((tri >> (12 * t + 4)) & 0xf),
           e2 = (int)((tri >> (12 * t + 8)) & 0xf);
@@ -423,6 +459,9 @@ __device__ __forceinline__ u32 prefix_in_cell(const BlockLds& S, int l, int j) {
   return s;
 }
 
+// Vertices of a block, in two phases per window of kListCap local vertex
+// ids: the mixed cubes write (cube, owned edge) at their local id, then every
+// thread takes consecutive ids (position + gathered normal sum).
 __global__ void __launch_bounds__(kMcThreads) k_mc_verts(const u32* __restrict__ C, int R, Geo geo,
                                                           const u32* __restrict__ blockV, u32* cellV, float* V,
                                                           float* Nrm, u32 capV, const u32* totals) {
@@ -433,6 +472,7 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_verts(const u32* __restrict__
   const u32 base = blockV[blk];
   const u32 next = (blk + 1 < (int)gridDim.x) ? blockV[blk + 1] : totals[0];
   if (next == base) return;  // no vertex in this block (uniform across the workgroup)
+  load_mc_tab(S.tab);
   load_box(S.box, C, 2 * R, 2 * c0x - kLo, 2 * c0y - kLo, 2 * c0z - kLo, kBox, kBox);
   stage_cases(S, blk, R);
   scan_cells(S);
@@ -445,32 +485,43 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_verts(const u32* __restrict__
       cellV[(size_t)cx + (size_t)R * ((size_t)cy + (size_t)R * cz)] = base + S.vOff[l];
     }
   }
-  for (int s = threadIdx.x; s < 4096; s += blockDim.x) {
-    int v = S.cases[s];
-    if (!mixed(v)) continue;
-    int l = s >> 3, j = s & 7;
-    int cx, cy, cz;
-    block_cell(blk, l, cx, cy, cz);
-    int fx = 2 * cx + (j & 1), fy = 2 * cy + ((j >> 1) & 1), fz = 2 * cz + ((j >> 2) & 1);
-    u32 vid = base + S.vOff[l] + prefix_in_cell(S, l, j);
-    int c0 = v & 1;
-    const int es[3] = {0, 3, 8};
-    const int on[3] = {c0 ^ ((v >> 1) & 1), c0 ^ ((v >> 3) & 1), c0 ^ ((v >> 4) & 1)};
-    for (int q = 0; q < 3; ++q) {
-      if (!on[q]) continue;
-      if (vid < capV) {
-        float p[3], n[3];
-        edge_pos(geo, S.box, fx, fy, fz, es[q], p);
-        normal_sum(geo, S.box, R, fx, fy, fz, q, n);
-        V[3 * (size_t)vid] = p[0];
-        V[3 * (size_t)vid + 1] = p[1];
-        V[3 * (size_t)vid + 2] = p[2];
-        Nrm[3 * (size_t)vid] = n[0];
-        Nrm[3 * (size_t)vid + 1] = n[1];
-        Nrm[3 * (size_t)vid + 2] = n[2];
+  const u32 nverts = next - base;
+  for (u32 w0 = 0; w0 < nverts; w0 += kListCap) {
+    for (int s = threadIdx.x; s < 4096; s += blockDim.x) {
+      const int v = S.cases[s];
+      if (!mixed(v)) continue;
+      const int l = s >> 3, j = s & 7;
+      u32 lv = S.vOff[l] + prefix_in_cell(S, l, j) - w0;
+      const int c0 = v & 1;
+      const int on[3] = {c0 ^ ((v >> 1) & 1), c0 ^ ((v >> 3) & 1), c0 ^ ((v >> 4) & 1)};
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        if (!on[q]) continue;
+        if (lv < (u32)kListCap) S.list[lv] = (uint16_t)((s << 3) | q);
+        ++lv;
       }
-      ++vid;
     }
+    __syncthreads();
+    const u32 n = min((u32)kListCap, nverts - w0);
+    for (u32 i = threadIdx.x; i < n; i += blockDim.x) {
+      const u32 vid = base + w0 + i;
+      if (vid >= capV) break;
+      const int e = S.list[i];
+      const int s = e >> 3, q = e & 7, l = s >> 3, j = s & 7;
+      int cx, cy, cz;
+      block_cell(blk, l, cx, cy, cz);
+      const int fx = 2 * cx + (j & 1), fy = 2 * cy + ((j >> 1) & 1), fz = 2 * cz + ((j >> 2) & 1);
+      float p[3], nn[3];
+      edge_pos(geo, S.box, fx, fy, fz, q == 0 ? 0 : (q == 1 ? 3 : 8), p);
+      normal_sum(S.tab, geo, S.box, R, fx, fy, fz, q, nn);
+      V[3 * (size_t)vid] = p[0];
+      V[3 * (size_t)vid + 1] = p[1];
+      V[3 * (size_t)vid + 2] = p[2];
+      Nrm[3 * (size_t)vid] = nn[0];
+      Nrm[3 * (size_t)vid + 1] = nn[1];
+      Nrm[3 * (size_t)vid + 2] = nn[2];
+    }
+    __syncthreads();
   }
 }
 
@@ -479,7 +530,9 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_verts(const u32* __restrict__
 __device__ __forceinline__ u32 edge_vid(const BlockLds& S, int blk, int c0x, int c0y, int c0z, u32 base,
                                         const u32* __restrict__ cellV, int R, int fx, int fy, int fz, int e,
                                         u32* border) {
-  int gx = fx + g_own[e][0], gy = fy + g_own[e][1], gz = fz + g_own[e][2], oe = g_own[e][3];
+  int dx, dy, dz, oe;
+  own_of(e, dx, dy, dz, oe);
+  int gx = fx + dx, gy = fy + dy, gz = fz + dz;
   int cx = gx >> 1, cy = gy >> 1, cz = gz >> 1;
   if (!cell_in_m_range(cx, cy, cz, R)) {
     *border = 1u;
@@ -497,6 +550,15 @@ __device__ __forceinline__ u32 edge_vid(const BlockLds& S, int blk, int c0x, int
   return s + (u32)rank_of(cube_case(S.box, gx, gy, gz), oe);
 }
 
+__device__ __forceinline__ u32 prefix_ntri(const BlockLds& S, int l, int j) {
+  u32 s = 0;
+  for (int q = 0; q < j; ++q) s += S.tab.ntri[S.cases[8 * l + q]];
+  return s;
+}
+
+// Triangles of a block, in the same two phases as k_mc_verts over local
+// triangle slots. Listed-but-uncounted triangles (len > ntri) go to the late
+// list from the fill phase of the first window.
 __global__ void __launch_bounds__(kMcThreads) k_mc_tris(const u32* __restrict__ C, int R,
                                                          const u32* __restrict__ blockV,
                                                          const u32* __restrict__ blockT,
@@ -510,47 +572,63 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_tris(const u32* __restrict__ 
   const u32 tnext = (blk + 1 < (int)gridDim.x) ? blockT[blk + 1] : totals[1];
   if (tnext == tbase) return;
   const u32 vbase = blockV[blk];
+  load_mc_tab(S.tab);
   load_box(S.box, C, 2 * R, 2 * c0x - kLo, 2 * c0y - kLo, 2 * c0z - kLo, kBox, kBox);
   stage_cases(S, blk, R);
   scan_cells(S);
   u32 border = 0u;
-  for (int s = threadIdx.x; s < 4096; s += blockDim.x) {
-    int v = S.cases[s];
-    if (!mixed(v)) continue;
-    int l = s >> 3, j = s & 7;
-    int cx, cy, cz;
-    block_cell(blk, l, cx, cy, cz);
-    int fx = 2 * cx + (j & 1), fy = 2 * cy + ((j >> 1) & 1), fz = 2 * cz + ((j >> 2) & 1);
-    u32 slot = tbase + S.tOff[l];
-    for (int q = 0; q < j; ++q) slot += g_mc.ntri[S.cases[8 * l + q]];
-    unsigned long long tri = g_tri[v];
-    u32 cnt = g_mc.ntri[v], len = g_mc.len[v];
-    for (u32 q = 0; q < len; ++q) {
-      int e0 = (int)((tri >> (12 * q)) & 0xf), e1 = (int)((tri >> (12 * q + 4)) & 0xf),
-          e2 = (int)((tri >> (12 * q + 8)) & 0xf);
-      u32 a = edge_vid(S, blk, c0x, c0y, c0z, vbase, cellV, R, fx, fy, fz, e2, &border);
-      u32 b = edge_vid(S, blk, c0x, c0y, c0z, vbase, cellV, R, fx, fy, fz, e1, &border);
-      u32 c = edge_vid(S, blk, c0x, c0y, c0z, vbase, cellV, R, fx, fy, fz, e0, &border);
-      u32 sl = slot + q;
-      if (q < cnt) {
-        if (sl < capT) {
-          T[3 * (size_t)sl] = a;
-          T[3 * (size_t)sl + 1] = b;
-          T[3 * (size_t)sl + 2] = c;
-        }
-      } else {
-        // listed but not counted: overwrites slot sl after all counted writes
-        u32 k = atomicAdd(&late[0], 1u);
-        if (k < capLate) {
-          late[1 + 4 * k] = sl;
-          late[2 + 4 * k] = a;
-          late[3 + 4 * k] = b;
-          late[4 + 4 * k] = c;
-        } else {
-          atomicOr(&totals[2], 2u);
+  const u32 ntris = tnext - tbase;
+  for (u32 w0 = 0; w0 < ntris; w0 += kListCap) {
+    for (int s = threadIdx.x; s < 4096; s += blockDim.x) {
+      const int v = S.cases[s];
+      if (!mixed(v)) continue;
+      const int l = s >> 3, j = s & 7;
+      const u32 lt = S.tOff[l] + prefix_ntri(S, l, j);
+      const u32 cnt = S.tab.ntri[v], len = S.tab.len[v];
+      for (u32 q = 0; q < cnt; ++q)
+        if (lt + q - w0 < (u32)kListCap) S.list[lt + q - w0] = (uint16_t)((s << 3) | q);
+      if (w0 == 0 && len > cnt) {
+        int cx, cy, cz;
+        block_cell(blk, l, cx, cy, cz);
+        const int fx = 2 * cx + (j & 1), fy = 2 * cy + ((j >> 1) & 1), fz = 2 * cz + ((j >> 2) & 1);
+        const unsigned long long tri = S.tab.tri[v];
+        for (u32 q = cnt; q < len; ++q) {
+          const int e0 = (int)((tri >> (12 * q)) & 0xf), e1 = (int)((tri >> (12 * q + 4)) & 0xf),
+                    e2 = (int)((tri >> (12 * q + 8)) & 0xf);
+          const u32 a = edge_vid(S, blk, c0x, c0y, c0z, vbase, cellV, R, fx, fy, fz, e2, &border);
+          const u32 b = edge_vid(S, blk, c0x, c0y, c0z, vbase, cellV, R, fx, fy, fz, e1, &border);
+          const u32 c = edge_vid(S, blk, c0x, c0y, c0z, vbase, cellV, R, fx, fy, fz, e0, &border);
+          // listed but not counted: overwrites slot sl after all counted writes
+          const u32 k = atomicAdd(&late[0], 1u);
+          if (k < capLate) {
+            late[1 + 4 * k] = tbase + lt + q;
+            late[2 + 4 * k] = a;
+            late[3 + 4 * k] = b;
+            late[4 + 4 * k] = c;
+          } else {
+            atomicOr(&totals[2], 2u);
+          }
         }
       }
     }
+    __syncthreads();
+    const u32 n = min((u32)kListCap, ntris - w0);
+    for (u32 i = threadIdx.x; i < n; i += blockDim.x) {
+      const u32 sl = tbase + w0 + i;
+      if (sl >= capT) break;
+      const int e = S.list[i];
+      const int s = e >> 3, q = e & 7, l = s >> 3, j = s & 7;
+      int cx, cy, cz;
+      block_cell(blk, l, cx, cy, cz);
+      const int fx = 2 * cx + (j & 1), fy = 2 * cy + ((j >> 1) & 1), fz = 2 * cz + ((j >> 2) & 1);
+      const unsigned long long tri = S.tab.tri[S.cases[s]];
+      const int e0 = (int)((tri >> (12 * q)) & 0xf), e1 = (int)((tri >> (12 * q + 4)) & 0xf),
+                e2 = (int)((tri >> (12 * q + 8)) & 0xf);
+      T[3 * (size_t)sl] = edge_vid(S, blk, c0x, c0y, c0z, vbase, cellV, R, fx, fy, fz, e2, &border);
+      T[3 * (size_t)sl + 1] = edge_vid(S, blk, c0x, c0y, c0z, vbase, cellV, R, fx, fy, fz, e1, &border);
+      T[3 * (size_t)sl + 2] = edge_vid(S, blk, c0x, c0y, c0z, vbase, cellV, R, fx, fy, fz, e0, &border);
+    }
+    __syncthreads();
   }
   if (border) atomicOr(&totals[3], 1u);
 }

@@ -447,13 +447,27 @@ __device__ void run_tile(const MorphArgs& a, Lds L, u32* set4, const TileGeo& g,
   for (int p = wv; p < nout; p += nw) fin(g.zfirst + p * g.zstep, skip ? 0u : L.out[p * kTileRows + lane]);
 }
 
+// Tiles are numbered slab-major (word fastest, then row tile, then plane
+// tile). Workgroups are dealt round-robin to the 8 XCDs, so workgroup i is
+// given tile (i % 8) * (T / 8) + i / 8: each XCD walks a contiguous run of
+// tiles, and the W word tiles of one slab, which share every 64-byte row
+// segment, meet in the same L2 at about the same time.
+__device__ __forceinline__ int xcd_tile(int i, int T) {
+  const int per = T >> 3;
+  return i < (per << 3) ? (i & 7) * per + (i >> 3) : i;
+}
+
 __device__ __forceinline__ TileGeo tile_geo(const MorphArgs& a, int yoff, int zoff, int zstep) {
   TileGeo g;
-  g.wx = blockIdx.x;
   const int rows = a.ystep * kTileRows;
-  g.yfirst = blockIdx.y * rows + yoff;
+  const int nty = (a.N + rows - 1) / rows, ntz = (a.N + a.Zt - 1) / a.Zt;
+  const int j = xcd_tile(blockIdx.x, a.W * nty * ntz);
+  const int slab = j / a.W;
+  g.wx = j - slab * a.W;
+  const int ty = slab % nty, tz = slab / nty;
+  g.yfirst = ty * rows + yoff;
   g.ystep = a.ystep;
-  g.tz0 = blockIdx.z * a.Zt;
+  g.tz0 = tz * a.Zt;
   g.zfirst = g.tz0 + zoff;
   int zl = min(g.tz0 + a.Zt, a.N) - 1;
   g.zlast = zl < g.zfirst ? g.zfirst - zstep : g.zfirst + ((zl - g.zfirst) / zstep) * zstep;
@@ -465,39 +479,42 @@ __device__ __forceinline__ TileGeo tile_geo(const MorphArgs& a, int yoff, int zo
 // Epilogues.
 struct DilCoarseFin {  // DF(c) = I(c) full | (c < dR & F(2c))
   const MorphArgs* a;
+  int w;
   int y;
   u32 xin;
   __device__ void operator()(int z, u32 f) const {
     if (y >= a->N) return;
     const int cy = y >> 1, cz = z >> 1;
-    u32 i0 = row_word(a->I, a->N, a->W, y, z, blockIdx.x), i1 = row_word(a->I, a->N, a->W, y + 1, z, blockIdx.x);
-    u32 i2 = row_word(a->I, a->N, a->W, y, z + 1, blockIdx.x);
-    u32 i3 = row_word(a->I, a->N, a->W, y + 1, z + 1, blockIdx.x);
+    u32 i0 = row_word(a->I, a->N, a->W, y, z, w), i1 = row_word(a->I, a->N, a->W, y + 1, z, w);
+    u32 i2 = row_word(a->I, a->N, a->W, y, z + 1, w);
+    u32 i3 = row_word(a->I, a->N, a->W, y + 1, z + 1, w);
     u32 full = i0 & i1 & i2 & i3;
     u32 ifull = gather2(full & (full >> 1));
     u32 inr = (cy < a->dR1 && cz < a->dR2) ? xin : 0u;
-    a->outFlags[((size_t)cz * a->R + cy) * a->W + blockIdx.x] = (u16)(ifull | (gather2(f) & inr));
+    a->outFlags[((size_t)cz * a->R + cy) * a->W + w] = (u16)(ifull | (gather2(f) & inr));
   }
 };
 struct DilFineFin {  // D = TG ? F : (DF ? 1 : I)
   const MorphArgs* a;
+  int w;
   int y;
   __device__ void operator()(int z, u32 f) const {
     if (y >= a->N) return;
     const int cy = y >> 1, cz = z >> 1;
-    u32 df = expand2(flag_word(a->DF, a->R, a->W, cy, cz, blockIdx.x));
-    u32 tg = expand2(flag_word(a->TG, a->R, a->W, cy, cz, blockIdx.x));
-    u32 d = df | row_word(a->I, a->N, a->W, y, z, blockIdx.x);
-    a->outRows[((size_t)z * a->N + y) * a->W + blockIdx.x] = (tg & f) | (~tg & d);
+    u32 df = expand2(flag_word(a->DF, a->R, a->W, cy, cz, w));
+    u32 tg = expand2(flag_word(a->TG, a->R, a->W, cy, cz, w));
+    u32 d = df | row_word(a->I, a->N, a->W, y, z, w);
+    a->outRows[((size_t)z * a->N + y) * a->W + w] = (tg & f) | (~tg & d);
   }
 };
 struct EroCoarseFin {  // CL(c) = c < dR & D(c) != 0 & I(c) != 0xff & Ec(2c+1)
   const MorphArgs* a;
+  int w;
   int y;  // odd row 2cy+1
   u32 xin;
   __device__ void operator()(int z, u32 e) const {
     if (y >= a->N) return;
-    const int cy = y >> 1, cz = z >> 1, y0 = y - 1, z0 = z - 1, w = blockIdx.x;
+    const int cy = y >> 1, cz = z >> 1, y0 = y - 1, z0 = z - 1;
     u32 d0 = row_word(a->D, a->N, a->W, y0, z0, w), d1 = row_word(a->D, a->N, a->W, y0 + 1, z0, w);
     u32 d2 = row_word(a->D, a->N, a->W, y0, z0 + 1, w), d3 = row_word(a->D, a->N, a->W, y0 + 1, z0 + 1, w);
     u32 dany = d0 | d1 | d2 | d3;
@@ -511,14 +528,15 @@ struct EroCoarseFin {  // CL(c) = c < dR & D(c) != 0 & I(c) != 0xff & Ec(2c+1)
 };
 struct EroFineFin {  // C = UG ? ~Ef : C'
   const MorphArgs* a;
+  int w;
   int y;
   __device__ void operator()(int z, u32 e) const {
     if (y >= a->N) return;
     const int cy = y >> 1, cz = z >> 1;
-    u32 cl = expand2(flag_word(a->CL, a->R, a->W, cy, cz, blockIdx.x));
-    u32 ug = expand2(flag_word(a->UG, a->R, a->W, cy, cz, blockIdx.x));
-    u32 cw = row_word(a->D, a->N, a->W, y, z, blockIdx.x) & ~cl;
-    a->outRows[((size_t)z * a->N + y) * a->W + blockIdx.x] = (ug & ~e) | (~ug & cw);
+    u32 cl = expand2(flag_word(a->CL, a->R, a->W, cy, cz, w));
+    u32 ug = expand2(flag_word(a->UG, a->R, a->W, cy, cz, w));
+    u32 cw = row_word(a->D, a->N, a->W, y, z, w) & ~cl;
+    a->outRows[((size_t)z * a->N + y) * a->W + w] = (ug & ~e) | (~ug & cw);
   }
 };
 
@@ -535,7 +553,7 @@ __global__ void __launch_bounds__(64 * kMaxWaves) k_dil_coarse(MorphArgs a) {
   bool skip = !bricks_any(a.brickI, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff, g.zfirst - a.maxoff,
                           g.zlast + 2 + a.maxoff, g.wx - 1, g.wx + 1, 0) ||
               bricks_all(a.brickI, a.N, a.W, g.yfirst, y1, g.zfirst, g.zlast + 2, g.wx);
-  DilCoarseFin fin{&a, g.yfirst + 2 * lane_id(), cell_range_mask(g.wx, 0, a.dR0)};
+  DilCoarseFin fin{&a, g.wx, g.yfirst + 2 * lane_id(), cell_range_mask(g.wx, 0, a.dR0)};
   run_tile(a, L, set4, g, a.I, 0, skip, fin);
 }
 
@@ -546,7 +564,7 @@ __global__ void __launch_bounds__(64 * kMaxWaves) k_dil_fine(MorphArgs a) {
   TileGeo g = tile_geo(a, 0, 0, 1);
   Lds L = carve(smem, a, g.nout());
   bool skip = !tile_has_flags(a.TG, a.R, a.W, g.wx, g.yfirst, g.yfirst + kTileRows, g.zfirst, g.zlast + 1);
-  DilFineFin fin{&a, g.yfirst + lane_id()};
+  DilFineFin fin{&a, g.wx, g.yfirst + lane_id()};
   run_tile(a, L, set4, g, a.I, 0, skip, fin);
 }
 
@@ -559,7 +577,7 @@ __global__ void __launch_bounds__(64 * kMaxWaves) k_ero_coarse(MorphArgs a) {
   const int y1 = g.yfirst + a.ystep * kTileRows;
   bool skip = !bricks_any(a.brickK, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff, g.zfirst - a.maxoff,
                           g.zlast + 1 + a.maxoff, g.wx - 1, g.wx + 1, 0);
-  EroCoarseFin fin{&a, g.yfirst + 2 * lane_id(), cell_range_mask(g.wx, 0, a.dR0)};
+  EroCoarseFin fin{&a, g.wx, g.yfirst + 2 * lane_id(), cell_range_mask(g.wx, 0, a.dR0)};
   run_tile(a, L, set4, g, a.K, 1, skip, fin);
 }
 
@@ -570,7 +588,7 @@ __global__ void __launch_bounds__(64 * kMaxWaves) k_ero_fine(MorphArgs a) {
   TileGeo g = tile_geo(a, 0, 0, 1);
   Lds L = carve(smem, a, g.nout());
   bool skip = !tile_has_flags(a.UG, a.R, a.W, g.wx, g.yfirst, g.yfirst + kTileRows, g.zfirst, g.zlast + 1);
-  EroFineFin fin{&a, g.yfirst + lane_id()};
+  EroFineFin fin{&a, g.wx, g.yfirst + lane_id()};
   run_tile(a, L, set4, g, a.K, 2, skip, fin);
 }
 
@@ -597,46 +615,80 @@ __global__ void k_tight(MorphArgs a) {
   a.outFlags[gid] = (u16)(any & ~all & valid);
 }
 
-// Brick summary of a grid: one wave per brick of 1 word x 8 rows x 8 planes
-// (lane = row + 8*plane); bit0 = any voxel set, bit1 = all set.
-__global__ void __launch_bounds__(64) k_brick_summary(const u32* __restrict__ G, int N, int W, uint8_t* out) {
+// Brick summary of a grid (brick = 1 word x 8 rows x 8 planes; bit0 = any
+// voxel set, bit1 = all set). Workgroup = one 8x8 (row, plane) brick column
+// over every word: thread (w, row) walks the 8 planes, so each plane step is
+// one contiguous read of 8 rows; the 8 rows of a word meet in LDS.
+__global__ void __launch_bounds__(512) k_brick_summary(const u32* __restrict__ G, int N, int W,
+                                                        uint8_t* __restrict__ out) {
+  __shared__ u32 anyW[64], allW[64];
   const int NB = N / 8;
-  const long long brick = blockIdx.x;
-  const int w = (int)(brick % W);
-  const long long t = brick / W;
-  const int by = (int)(t % NB), bz = (int)(t / NB);
-  const int y = 8 * by + (threadIdx.x & 7), z = 8 * bz + (threadIdx.x >> 3);
-  const u32 v = G[((size_t)z * N + y) * W + w];
-  const bool any = __any(v != 0u);
-  const bool all = __all(v == 0xffffffffu);
-  if (threadIdx.x == 0) out[brick] = (uint8_t)((any ? 1 : 0) | (all ? 2 : 0));
+  const int by = blockIdx.x, bz = blockIdx.y;
+  const int w = threadIdx.x % W, yl = threadIdx.x / W;
+  if (threadIdx.x < W) {
+    anyW[threadIdx.x] = 0u;
+    allW[threadIdx.x] = 1u;
+  }
+  __syncthreads();
+  const int y = 8 * by + yl;
+  u32 any = 0u, all = 0xffffffffu;
+  const u32* g = G + ((size_t)(8 * bz) * N + y) * W + w;
+#pragma unroll
+  for (int zl = 0; zl < 8; ++zl) {
+    u32 v = g[(size_t)zl * N * W];
+    any |= v;
+    all &= v;
+  }
+  if (any) atomicOr(&anyW[w], 1u);
+  if (all != 0xffffffffu) atomicAnd(&allW[w], 0u);
+  __syncthreads();
+  if (threadIdx.x < W)
+    out[((size_t)bz * NB + by) * W + threadIdx.x] = (uint8_t)(anyW[threadIdx.x] | (allW[threadIdx.x] << 1));
 }
 
-// Outer contour K of D (Grid.cu:1230-1327) and its brick summary; one wave
-// per brick, one lane per row word.
-__global__ void __launch_bounds__(64) k_contour_K(MorphArgs a, uint8_t* brickK) {
+// Outer contour K of D (Grid.cu:1230-1327) and its brick summary: voxels
+// outside D, 6-adjacent to D, in a cell that is not full, away from the data
+// border. Same workgroup mapping as k_brick_summary.
+__global__ void __launch_bounds__(512) k_contour_K(MorphArgs a, uint8_t* __restrict__ brickK) {
+  __shared__ u32 anyW[64];
   const int N = a.N, W = a.W, NB = N / 8;
-  const long long brick = blockIdx.x;
-  const int wx = (int)(brick % W);
-  const long long t = brick / W;
-  const int by = (int)(t % NB), bz = (int)(t / NB);
-  const int y = 8 * by + (threadIdx.x & 7), z = 8 * bz + (threadIdx.x >> 3);
+  const int by = blockIdx.x, bz = blockIdx.y;
+  const int wx = threadIdx.x % W, yl = threadIdx.x / W;
+  if (threadIdx.x < W) anyW[threadIdx.x] = 0u;
+  __syncthreads();
   const u32* D = a.D;
-  u32 c = row_word(D, N, W, y, z, wx);
-  u32 l = row_word(D, N, W, y, z, wx - 1), r = row_word(D, N, W, y, z, wx + 1);
-  u32 xn = (c << 1) | (l >> 31) | (c >> 1) | (r << 31);
-  u32 n6 = xn | row_word(D, N, W, y - 1, z, wx) | row_word(D, N, W, y + 1, z, wx) |
-           row_word(D, N, W, y, z - 1, wx) | row_word(D, N, W, y, z + 1, wx);
-  int y0 = y & ~1, z0 = z & ~1;
-  u32 f = row_word(D, N, W, y0, z0, wx) & row_word(D, N, W, y0 + 1, z0, wx) & row_word(D, N, W, y0, z0 + 1, wx) &
-          row_word(D, N, W, y0 + 1, z0 + 1, wx);
-  u32 cfull = expand2(gather2(f & (f >> 1)));
-  int cy = y >> 1, cz = z >> 1;
-  u32 inner = (cy >= 1 && cy <= a.R - 2 && cz >= 1 && cz <= a.R - 2) ? expand2(cell_range_mask(wx, 1, a.R - 1)) : 0u;
-  u32 k = ~c & n6 & ~cfull & inner;
-  a.outRows[((size_t)z * N + y) * W + wx] = k;
-  const bool any = __any(k != 0u);
-  if (threadIdx.x == 0) brickK[brick] = any ? 1 : 0;
+  const int y = 8 * by + yl;
+  const int cy = y >> 1;
+  const bool yin = cy >= 1 && cy <= a.R - 2;
+  const u32 xin = expand2(cell_range_mask(wx, 1, a.R - 1));
+  u32 any = 0u;
+  u32 zm = row_word(D, N, W, y, 8 * bz - 1, wx);
+  u32 c = row_word(D, N, W, y, 8 * bz, wx);
+#pragma unroll 2
+  for (int zl = 0; zl < 8; ++zl) {
+    const int z = 8 * bz + zl;
+    const u32 zp = row_word(D, N, W, y, z + 1, wx);
+    const u32 l = row_word(D, N, W, y, z, wx - 1), r = row_word(D, N, W, y, z, wx + 1);
+    const u32 xn = (c << 1) | (l >> 31) | (c >> 1) | (r << 31);
+    const u32 ym = row_word(D, N, W, y - 1, z, wx), yp = row_word(D, N, W, y + 1, z, wx);
+    const u32 n6 = xn | ym | yp | zm | zp;
+    // the 2x2 fine rows of this voxel's cell row
+    const u32 ysib = (y & 1) ? ym : yp;
+    const u32 zsib = (z & 1) ? zm : zp;
+    const u32 diag = row_word(D, N, W, y ^ 1, z ^ 1, wx);
+    const u32 f = c & ysib & zsib & diag;
+    const u32 cfull = expand2(gather2(f & (f >> 1)));
+    const int cz = z >> 1;
+    const u32 inner = (yin && cz >= 1 && cz <= a.R - 2) ? xin : 0u;
+    const u32 k = ~c & n6 & ~cfull & inner;
+    a.outRows[((size_t)z * N + y) * W + wx] = k;
+    any |= k;
+    zm = c;
+    c = zp;
+  }
+  if (any) atomicOr(&anyW[wx], 1u);
+  __syncthreads();
+  if (threadIdx.x < W) brickK[((size_t)bz * NB + by) * W + threadIdx.x] = (uint8_t)anyW[threadIdx.x];
 }
 
 // C' rows of a cell row (4 fine rows), masked by CL.
@@ -722,7 +774,7 @@ MorphArgs base_args(const MorphDims& d, const MorphBuffers& b, const RadiusField
 
 dim3 stream_grid(const MorphArgs& a) {
   const int rows = a.ystep * kTileRows;
-  return dim3(a.W, (a.N + rows - 1) / rows, (a.N + a.Zt - 1) / a.Zt);
+  return dim3((unsigned)(a.W * ((a.N + rows - 1) / rows) * ((a.N + a.Zt - 1) / a.Zt)));
 }
 
 }  // namespace
@@ -742,9 +794,8 @@ hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const Radius
   a.tabWords = tabWords;
   const bool var = rf.mode != 0;
   long long cellwords = (long long)d.R * d.R * d.W;
-  long long bricks = (long long)(d.N / 8) * (d.N / 8) * d.W;
   if (ev) GM_CHECK(hipEventRecord(ev[0], s));
-  hipLaunchKernelGGL(k_brick_summary, dim3((unsigned)bricks), dim3(64), 0, s, b.I, d.N, d.W, b.brickI);
+  hipLaunchKernelGGL(k_brick_summary, dim3(d.N / 8, d.N / 8), dim3(8 * d.W), 0, s, b.I, d.N, d.W, b.brickI);
   GM_CHECK(hipGetLastError());
   a.ystep = 2;
   a.outFlags = b.DF;
@@ -777,10 +828,9 @@ hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusF
   a.tabWords = tabWords;
   const bool var = rf.mode != 0;
   long long cellwords = (long long)d.R * d.R * d.W;
-  long long bricks = (long long)(d.N / 8) * (d.N / 8) * d.W;
   if (ev) GM_CHECK(hipEventRecord(ev[0], s));
   a.outRows = b.K;
-  hipLaunchKernelGGL(k_contour_K, dim3((unsigned)bricks), dim3(64), 0, s, a, b.brickK);
+  hipLaunchKernelGGL(k_contour_K, dim3(d.N / 8, d.N / 8), dim3(8 * d.W), 0, s, a, b.brickK);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
   a.ystep = 2;

@@ -14,11 +14,14 @@ LIBDIR = os.path.join(HERE, "_lib")
 
 HIP_SOURCES = ["gm_context.hip", "gm_layout.hip", "gm_voxelize.hip", "gm_morph.hip", "gm_mc.hip",
                "gm_pyramid.hip"]
+# gm_morph_ct.hip is compiled once per part (the compiled-radius kernels)
+CT_SOURCE = "gm_morph_ct.hip"
+CT_PARTS = 8
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GM_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [
-    "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+    "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-value", "-Wno-unused-result",
 ]
@@ -37,11 +40,32 @@ def build_meshgen():
     return out
 
 
-def build_hip():
+def build_hip(jobs=None):
+    """Compiles every HIP source to an object (in parallel), then links."""
     os.makedirs(LIBDIR, exist_ok=True)
+    objdir = os.path.join(LIBDIR, "obj")
+    os.makedirs(objdir, exist_ok=True)
+    units = [(os.path.join(CSRC, s), os.path.join(objdir, s.replace(".hip", ".o")), []) for s in HIP_SOURCES]
+    for p in range(CT_PARTS):
+        units.append((os.path.join(CSRC, CT_SOURCE), os.path.join(objdir, "gm_morph_ct%d.o" % p),
+                      ["-DGM_CT_PART=%d" % p]))
+    cmds = [[HIPCC] + HIP_FLAGS + extra + ["-c", src, "-o", obj] for src, obj, extra in units]
+    jobs = jobs or min(len(cmds), max(1, os.cpu_count() or 1))
+    running, failed = [], []
+    pending = list(cmds)
+    while pending or running:
+        while pending and len(running) < jobs:
+            cmd = pending.pop(0)
+            print("+", " ".join(cmd), flush=True)
+            running.append((cmd, subprocess.Popen(cmd)))
+        cmd, proc = running.pop(0)
+        if proc.wait() != 0:
+            failed.append(cmd)
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
     out = os.path.join(LIBDIR, "libgmorpho_hip.so")
     tmp = out + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + [os.path.join(CSRC, s) for s in HIP_SOURCES] + ["-o", tmp])
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC"] + [obj for _, obj, _ in units] + ["-o", tmp])
     os.replace(tmp, out)
     return out
 

@@ -199,6 +199,8 @@ RadiusField radius_field(const gm_ctx* c) {
 
 int ensure_tables(gm_ctx* c, int rlo, int rhi) {
   if (c->tabLo == rlo && c->tabHi == rhi && c->dTables) return GM_OK;
+  static const bool ctOk = ct_tables_ok();
+  if (!ctOk) return fail(c, GM_ERR_UNSUPPORTED, "compiled chord tables differ from the structuring element");
   std::vector<u32> all;
   SETableSet t;
   for (int i = 0; i <= 128; ++i) t.dil[i] = t.ero_corner[i] = t.ero_centre[i] = -1;

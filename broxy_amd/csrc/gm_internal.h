@@ -126,6 +126,28 @@ hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusF
                         const SETableSet& tabs, int rho_lo, int rho_hi, int tabWords, hipStream_t s,
                         hipEvent_t* ev);
 
+// Host check that the chord tables compiled into the streaming kernels equal
+// the ones build_se_table produces.
+bool ct_tables_ok();
+
+// Streaming kernels compiled for one uniform radius (gm_morph_ct.hip, built
+// as kCtParts objects, radius rho in part rho % kCtParts): fn[pass][rho] is
+// the kernel of pass 0..3 (coarse dilation, fine dilation, coarse erosion,
+// fine erosion) for chord-table radius rho, null where none is compiled.
+constexpr int kCtMaxRho = 32;
+constexpr int kCtParts = 8;
+struct CtKernels {
+  const void* fn[4][kCtMaxRho + 1];
+};
+void ct_register_part0(CtKernels&);
+void ct_register_part1(CtKernels&);
+void ct_register_part2(CtKernels&);
+void ct_register_part3(CtKernels&);
+void ct_register_part4(CtKernels&);
+void ct_register_part5(CtKernels&);
+void ct_register_part6(CtKernels&);
+void ct_register_part7(CtKernels&);
+
 // ---- marching cubes (gm_mc.hip)
 struct MCBuffers {
   u32* blockV;   // per 8^3 Morton block: vertex count, then exclusive offset

@@ -37,560 +37,17 @@
 // scatter radius (erosion) masks the source voxels.
 #include <vector>
 
-#include "gm_internal.h"
+#include "gm_morph_dev.h"
 
 namespace gm {
 
 namespace {
 
-__device__ __forceinline__ u32 spread2(u32 v) {
-  v &= 0xffffu;
-  v = (v | (v << 8)) & 0x00ff00ffu;
-  v = (v | (v << 4)) & 0x0f0f0f0fu;
-  v = (v | (v << 2)) & 0x33333333u;
-  v = (v | (v << 1)) & 0x55555555u;
-  return v;
-}
-__device__ __forceinline__ u32 gather2(u32 v) {
-  v &= 0x55555555u;
-  v = (v | (v >> 1)) & 0x33333333u;
-  v = (v | (v >> 2)) & 0x0f0f0f0fu;
-  v = (v | (v >> 4)) & 0x00ff00ffu;
-  v = (v | (v >> 8)) & 0x0000ffffu;
-  return v;
-}
-// 16 cell flags -> 32 fine bits (both x sub-voxels of each cell)
-__device__ __forceinline__ u32 expand2(u32 v) {
-  u32 s = spread2(v);
-  return s | (s << 1);
-}
-
-__device__ __forceinline__ u32 row_word(const u32* __restrict__ g, int N, int W, int y, int z, int w) {
-  if ((unsigned)y >= (unsigned)N || (unsigned)z >= (unsigned)N || (unsigned)w >= (unsigned)W) return 0u;
-  return g[((size_t)z * N + y) * W + w];
-}
-__device__ __forceinline__ u32 flag_word(const u16* __restrict__ g, int R, int W, int cy, int cz, int w) {
-  if ((unsigned)cy >= (unsigned)R || (unsigned)cz >= (unsigned)R || (unsigned)w >= (unsigned)W) return 0u;
-  return g[((size_t)cz * R + cy) * W + w];
-}
-// Mask of cells 16w..16w+15 whose x index lies in [lo, hi).
-__device__ __forceinline__ u32 cell_range_mask(int w, int lo, int hi) {
-  int a = lo - 16 * w, b = hi - 16 * w;
-  a = a < 0 ? 0 : (a > 16 ? 16 : a);
-  b = b < 0 ? 0 : (b > 16 ? 16 : b);
-  if (b <= a) return 0u;
-  u32 m = (b >= 16) ? 0xffffu : ((1u << b) - 1u);
-  return m & ~((1u << a) - 1u);
-}
-
-// shr(k): bit x reads source x+k; shl(k): bit x reads source x-k (1 <= k <= 32)
-__device__ __forceinline__ u32 shr_k(u32 C, u32 Rw, int k) {
-  return k >= 32 ? Rw : __builtin_amdgcn_alignbit(Rw, C, (u32)k);
-}
-__device__ __forceinline__ u32 shl_k(u32 L, u32 C, int k) {
-  return k >= 32 ? L : __builtin_amdgcn_alignbit(C, L, (u32)(32 - k));
-}
-
-// Output geometry of one streaming tile: lane i of every wave owns row
-// yfirst + ystep*i; output planes are zfirst, zfirst + zstep, ... <= zlast.
-struct TileGeo {
-  int wx;
-  int yfirst, ystep;
-  int tz0, zfirst, zlast, zstep;
-  __device__ int nout() const { return zlast < zfirst ? 0 : (zlast - zfirst) / zstep + 1; }
-};
-
-__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
-__device__ __forceinline__ int wave_id() { return (int)(threadIdx.x >> 6); }
-__device__ __forceinline__ int num_waves() { return (int)(blockDim.x >> 6); }
-
-// One streaming pass of the table tl (in LDS) over the source src, ORed into
-// acc[plane index][lane] (plane index = (zo - zfirst) / zstep). The waves of
-// the workgroup take the input planes round robin; each builds the G table of
-// its plane in its own LDS slice (gtAll + wave * gtStride) and accumulates
-// with LDS atomics, so the plane loop needs no workgroup barrier.
-template <class Src>
-__device__ void stream(const u32* tl, const Src& src, u32* gtAll, int gtStride, u32* acc, const TileGeo& g) {
-  const int lane = lane_id(), wv = wave_id(), nw = num_waves();
-  const int kind = (int)tl[0], nclass = (int)tl[1], maxm = (int)tl[2], HY = (int)tl[3];
-  const int dzl = -HY, dzh = (kind == SE_CORNER) ? HY - 1 : HY;
-  const int GR = g.ystep * (kTileRows - 1) + 1 + 2 * HY;
-  const int ybase = g.yfirst - HY;
-  u32* gt = gtAll + wv * gtStride;
-  const int zbeg = g.zfirst + dzl, zend = g.zlast + dzh;
-  const int zshift = g.zstep == 2 ? 1 : 0;
-  const int base = g.ystep * lane + HY;
-  u32 pre[4][3];
-  int zi = zbeg + wv;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    int gr = lane + kTileRows * k;
-    pre[k][0] = pre[k][1] = pre[k][2] = 0u;
-    if (gr < GR && zi <= zend) {
-      pre[k][0] = src(ybase + gr, zi, g.wx - 1);
-      pre[k][1] = src(ybase + gr, zi, g.wx);
-      pre[k][2] = src(ybase + gr, zi, g.wx + 1);
-    }
-  }
-  for (; zi <= zend; zi += nw) {
-    // stage 1: x-window dilations of this plane's rows into the wave's slice
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      int gr = lane + kTileRows * k;
-      if (gr < GR) {
-        const u32 Lw = pre[k][0], Cw = pre[k][1], Rw = pre[k][2];
-        u32 gg;
-        if (kind == SE_CORNER) {
-          gg = Cw | shl_k(Lw, Cw, 1);
-          gt[gr] = gg;
-          for (int m = 1; m <= maxm; ++m) {
-            gg |= shr_k(Cw, Rw, m) | shl_k(Lw, Cw, m + 1);
-            gt[m * GR + gr] = gg;
-          }
-        } else {
-          gg = Cw;
-          gt[gr] = gg;
-          for (int m = 1; m <= maxm; ++m) {
-            gg |= shr_k(Cw, Rw, m) | shl_k(Lw, Cw, m);
-            gt[m * GR + gr] = gg;
-          }
-        }
-      }
-    }
-    // prefetch this wave's next plane while this one is reduced
-    const int zn = zi + nw;
-    if (zn <= zend) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        int gr = lane + kTileRows * k;
-        if (gr < GR) {
-          pre[k][0] = src(ybase + gr, zn, g.wx - 1);
-          pre[k][1] = src(ybase + gr, zn, g.wx);
-          pre[k][2] = src(ybase + gr, zn, g.wx + 1);
-        }
-      }
-    }
-    // the slice is written and read by this wave only: in-order LDS suffices
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // stage 2: chords of each dz class
-    const u32* p = tl + 5;
-    for (int c = 0; c < nclass; ++c) {
-      const u32 hw = p[0];
-      const int b = (int)(hw & 0xffffu), np = (int)(hw >> 16);
-      ++p;
-      u32 disk = 0u;
-      if (kind == SE_CORNER) {
-#pragma unroll 4
-        for (int e = 0; e < np; ++e) {
-          const u32 ew = p[e];
-          const int a = (int)(ew & 0xffffu);
-          const u32* gm = gt + (ew >> 16) * GR;
-          disk |= gm[base + a] | gm[base - a - 1];
-        }
-      } else {
-#pragma unroll 4
-        for (int e = 0; e < np; ++e) {
-          const u32 ew = p[e];
-          const int a = (int)(ew & 0xffffu);
-          const u32* gm = gt + (ew >> 16) * GR;
-          disk |= gm[base + a] | gm[base - a];
-        }
-      }
-      p += np;
-      if (!disk) continue;
-      // output planes zo = zi - dz for the dz values of the class
-      int zo = zi - b;
-      if (zo >= g.zfirst && zo <= g.zlast && ((zo - g.zfirst) & (g.zstep - 1)) == 0)
-        atomicOr(&acc[((zo - g.zfirst) >> zshift) * kTileRows + lane], disk);
-      const int dz1 = (kind == SE_CORNER) ? -b - 1 : -b;
-      if (dz1 != b) {
-        zo = zi - dz1;
-        if (zo >= g.zfirst && zo <= g.zlast && ((zo - g.zfirst) & (g.zstep - 1)) == 0)
-          atomicOr(&acc[((zo - g.zfirst) >> zshift) * kTileRows + lane], disk);
-      }
-    }
-    // keep the next iteration's stage 1 behind this stage 2 in the slice
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  }
-}
-
-struct GridSrc {
-  const u32* g;
-  int N, W;
-  __device__ __forceinline__ u32 operator()(int y, int z, int w) const { return row_word(g, N, W, y, z, w); }
-};
-
-// Source voxels whose erosion radius r(q)-1 equals rho.
-struct MaskedSrc {
-  const u32* g;
-  int N, W;
-  RadiusField rf;
-  int rho;
-  __device__ __forceinline__ u32 operator()(int y, int z, int w) const {
-    u32 v = row_word(g, N, W, y, z, w);
-    u32 out = 0u;
-    while (v) {
-      int b = __builtin_ctz(v);
-      v &= v - 1u;
-      if (radius_at(rf, 32 * w + b, y, z) - 1 == rho) out |= 1u << b;
-    }
-    return out;
-  }
-};
-
-// ---------------------------------------------------------------------------
-// Kernel parameters shared by the streaming kernels.
-struct MorphArgs {
-  int R, N, W;
-  int dR0, dR1, dR2;
-  int Zt;
-  RadiusField rf;
-  int se;  // 0 sphere, 1 cube (dilation)
-  const u32* tables;
-  SETableSet tabs;
-  int maxm, maxoff, tabWords;  // LDS carve sizes
-  int ystep;                   // row step of this launch's lanes
-  const u32* I;
-  const u32* D;
-  const u32* K;
-  u32* outRows;
-  const u16* DF;
-  const u16* TG;
-  const u16* CL;
-  const u16* UG;
-  const uint8_t* brickI;  // per brick (1 word x 8 rows x 8 planes): bit0 any, bit1 all
-  const uint8_t* brickK;
-  u16* outFlags;
-};
-
-struct Lds {
-  u32* tl;
-  u32* gt;
-  int gtStride;
-  u32* out;  // output planes x 64
-  u32* acc;  // per-radius pass accumulator (per-voxel radii only)
-};
-__device__ __forceinline__ int gt_stride(const MorphArgs& a) {
-  return (a.maxm + 1) * (a.ystep * (kTileRows - 1) + 1 + 2 * a.maxoff);
-}
-__device__ __forceinline__ Lds carve(u32* smem, const MorphArgs& a, int nplanes) {
-  Lds L;
-  L.tl = smem;
-  L.gt = L.tl + a.tabWords;
-  L.gtStride = gt_stride(a);
-  L.out = L.gt + L.gtStride * num_waves();
-  L.acc = L.out + nplanes * kTileRows;
-  return L;
-}
-
-__device__ __forceinline__ void load_table(u32* tl, const u32* src) {
-  __syncthreads();
-  const int n = (int)src[4];
-  for (int i = threadIdx.x; i < n; i += blockDim.x) tl[i] = src[i];
-  __syncthreads();
-}
-
-__device__ __forceinline__ int table_off(const MorphArgs& a, int which, int rho) {
-  if (rho < 0 || rho > 128) return -1;
-  return which == 0 ? a.tabs.dil[rho] : (which == 1 ? a.tabs.ero_corner[rho] : a.tabs.ero_centre[rho]);
-}
-
-// Any brick flag (bit) set in the brick region covering rows [y0,y1),
-// planes [z0,z1), words [w0,w1] (clamped to the grid).
-__device__ bool bricks_any(const uint8_t* br, int N, int W, int y0, int y1, int z0, int z1, int w0, int w1,
-                           int bit) {
-  const int NB = N / 8;
-  int by0 = max(y0, 0) >> 3, by1 = (min(y1, N) - 1) >> 3;
-  int bz0 = max(z0, 0) >> 3, bz1 = (min(z1, N) - 1) >> 3;
-  w0 = max(w0, 0);
-  w1 = min(w1, W - 1);
-  int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1, nw = w1 - w0 + 1;
-  int found = 0;
-  if (ny > 0 && nz > 0 && nw > 0) {
-    int n = ny * nz * nw;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      int w = w0 + i % nw, t = i / nw;
-      int by = by0 + t % ny, bz = bz0 + t / ny;
-      if ((br[((size_t)bz * NB + by) * W + w] >> bit) & 1) found = 1;
-    }
-  }
-  return __syncthreads_or(found) != 0;
-}
-// Every brick of the region has the "all" bit.
-__device__ bool bricks_all(const uint8_t* br, int N, int W, int y0, int y1, int z0, int z1, int w) {
-  const int NB = N / 8;
-  int by0 = max(y0, 0) >> 3, by1 = (min(y1, N) - 1) >> 3;
-  int bz0 = max(z0, 0) >> 3, bz1 = (min(z1, N) - 1) >> 3;
-  int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1;
-  int miss = 0;
-  for (int i = threadIdx.x; i < ny * nz; i += blockDim.x) {
-    int by = by0 + i % ny, bz = bz0 + i / ny;
-    if (!((br[((size_t)bz * NB + by) * W + w] >> 1) & 1)) miss = 1;
-  }
-  return __syncthreads_or(miss) == 0;
-}
-
-__device__ __forceinline__ bool tile_has_flags(const u16* F, int R, int W, int wx, int y0, int y1, int z0, int z1) {
-  int cy0 = y0 >> 1, cy1 = min((y1 + 1) >> 1, R);
-  int cz0 = z0 >> 1, cz1 = min((z1 + 1) >> 1, R);
-  int ny = cy1 - cy0, nz = cz1 - cz0;
-  u32 acc = 0u;
-  if (ny > 0 && nz > 0)
-    for (int i = threadIdx.x; i < ny * nz; i += blockDim.x) acc |= flag_word(F, R, W, cy0 + i % ny, cz0 + i / ny, wx);
-  return __syncthreads_or(acc != 0u) != 0;
-}
-
-// Distinct radii of a tile (128-bit set in LDS). Dilation: radius+add of the
-// output voxels; erosion: radius-1 of the K voxels within the halo.
-__device__ void rho_set_dilate(const RadiusField& rf, int add, const TileGeo& g, int N, u32* set4) {
-  if (threadIdx.x < 4) set4[threadIdx.x] = 0u;
-  __syncthreads();
-  const int y = g.yfirst + g.ystep * lane_id();
-  u32 loc[4] = {0, 0, 0, 0};
-  if (y < N)
-    for (int p = wave_id(); p < g.nout(); p += num_waves()) {
-      const int z = g.zfirst + p * g.zstep;
-      for (int b = 0; b < 32; ++b) {
-        int r = radius_at(rf, 32 * g.wx + b, y, z) + add;
-        loc[r >> 5] |= 1u << (r & 31);
-      }
-    }
-  for (int i = 0; i < 4; ++i)
-    if (loc[i]) atomicOr(&set4[i], loc[i]);
-  __syncthreads();
-}
-__device__ void rho_set_erode(const RadiusField& rf, const u32* K, int N, int W, const TileGeo& g, int halo,
-                              u32* set4) {
-  if (threadIdx.x < 4) set4[threadIdx.x] = 0u;
-  __syncthreads();
-  u32 loc[4] = {0, 0, 0, 0};
-  const int y0 = g.yfirst - halo, y1 = g.yfirst + g.ystep * (kTileRows - 1) + halo;
-  const int z0 = g.zfirst - halo, z1 = g.zlast + halo;
-  const int ny = y1 - y0 + 1;
-  for (int i = threadIdx.x; i < ny * (z1 - z0 + 1); i += blockDim.x) {
-    const int y = y0 + i % ny, z = z0 + i / ny;
-    for (int w = g.wx - 1; w <= g.wx + 1; ++w) {
-      u32 v = row_word(K, N, W, y, z, w);
-      while (v) {
-        int b = __builtin_ctz(v);
-        v &= v - 1u;
-        int r = radius_at(rf, 32 * w + b, y, z) - 1;
-        r = r < 0 ? 0 : r;
-        loc[r >> 5] |= 1u << (r & 31);
-      }
-    }
-  }
-  for (int i = 0; i < 4; ++i)
-    if (loc[i]) atomicOr(&set4[i], loc[i]);
-  __syncthreads();
-}
-
-// Runs the predicate of a pass over the tile and calls fin(zo, F) for every
-// output plane of the thread's row (F = 0 when skip is set). which: 0
-// dilation table (radius+add), 1 erosion corner, 2 erosion centre (radius-1).
-template <class Fin>
-__device__ void run_tile(const MorphArgs& a, Lds L, u32* set4, const TileGeo& g, const u32* src, int which,
-                         bool skip, Fin& fin) {
-  const int lane = lane_id(), wv = wave_id(), nw = num_waves();
-  const int nout = g.nout();
-  const int add = which == 0 ? (a.se == 0 ? 1 : 0) : -1;
-  if (!skip) {
-    for (int i = threadIdx.x; i < nout * kTileRows; i += blockDim.x) L.out[i] = 0u;
-    if (a.rf.mode == 0) {
-      const int rho = (a.rf.r & 0x7f) + add;
-      const int off = table_off(a, which, rho);
-      if (off >= 0 && rho > 0) {
-        load_table(L.tl, a.tables + off);
-        stream(L.tl, GridSrc{src, a.N, a.W}, L.gt, L.gtStride, L.out, g);
-      }
-    } else {
-      // per-voxel radii: one pass per distinct radius, masked into out
-      const int y = g.yfirst + g.ystep * lane;
-      if (which == 0) rho_set_dilate(a.rf, add, g, a.N, set4);
-      else rho_set_erode(a.rf, src, a.N, a.W, g, a.maxoff, set4);
-      for (int i = 0; i < 4; ++i) {
-        u32 m = set4[i];
-        while (m) {
-          const int bit = __builtin_ctz(m);
-          m &= m - 1u;
-          const int rho = 32 * i + bit;
-          const int off = table_off(a, which, rho);
-          if (off < 0 || rho <= 0) continue;
-          load_table(L.tl, a.tables + off);
-          for (int k = threadIdx.x; k < nout * kTileRows; k += blockDim.x) L.acc[k] = 0u;
-          __syncthreads();
-          if (which == 0) stream(L.tl, GridSrc{src, a.N, a.W}, L.gt, L.gtStride, L.acc, g);
-          else stream(L.tl, MaskedSrc{src, a.N, a.W, a.rf, rho}, L.gt, L.gtStride, L.acc, g);
-          __syncthreads();
-          for (int p = wv; p < nout; p += nw) {
-            u32 v = L.acc[p * kTileRows + lane];
-            if (!v) continue;
-            if (which == 0) {
-              const int z = g.zfirst + p * g.zstep;
-              u32 mk = 0u;
-              if (y < a.N)
-                for (int b = 0; b < 32; ++b)
-                  if (radius_at(a.rf, 32 * g.wx + b, y, z) + add == rho) mk |= 1u << b;
-              v &= mk;
-            }
-            L.out[p * kTileRows + lane] |= v;
-          }
-        }
-      }
-    }
-    __syncthreads();
-  }
-  for (int p = wv; p < nout; p += nw) fin(g.zfirst + p * g.zstep, skip ? 0u : L.out[p * kTileRows + lane]);
-}
-
-// Tiles are numbered slab-major (word fastest, then row tile, then plane
-// tile). Workgroups are dealt round-robin to the 8 XCDs, so workgroup i is
-// given tile (i % 8) * (T / 8) + i / 8: each XCD walks a contiguous run of
-// tiles, and the W word tiles of one slab, which share every 64-byte row
-// segment, meet in the same L2 at about the same time.
-__device__ __forceinline__ int xcd_tile(int i, int T) {
-  const int per = T >> 3;
-  return i < (per << 3) ? (i & 7) * per + (i >> 3) : i;
-}
-
-__device__ __forceinline__ TileGeo tile_geo(const MorphArgs& a, int yoff, int zoff, int zstep) {
-  TileGeo g;
-  const int rows = a.ystep * kTileRows;
-  const int nty = (a.N + rows - 1) / rows, ntz = (a.N + a.Zt - 1) / a.Zt;
-  const int j = xcd_tile(blockIdx.x, a.W * nty * ntz);
-  const int slab = j / a.W;
-  g.wx = j - slab * a.W;
-  const int ty = slab % nty, tz = slab / nty;
-  g.yfirst = ty * rows + yoff;
-  g.ystep = a.ystep;
-  g.tz0 = tz * a.Zt;
-  g.zfirst = g.tz0 + zoff;
-  int zl = min(g.tz0 + a.Zt, a.N) - 1;
-  g.zlast = zl < g.zfirst ? g.zfirst - zstep : g.zfirst + ((zl - g.zfirst) / zstep) * zstep;
-  g.zstep = zstep;
-  return g;
-}
-
-// ---------------------------------------------------------------------------
-// Epilogues.
-struct DilCoarseFin {  // DF(c) = I(c) full | (c < dR & F(2c))
-  const MorphArgs* a;
-  int w;
-  int y;
-  u32 xin;
-  __device__ void operator()(int z, u32 f) const {
-    if (y >= a->N) return;
-    const int cy = y >> 1, cz = z >> 1;
-    u32 i0 = row_word(a->I, a->N, a->W, y, z, w), i1 = row_word(a->I, a->N, a->W, y + 1, z, w);
-    u32 i2 = row_word(a->I, a->N, a->W, y, z + 1, w);
-    u32 i3 = row_word(a->I, a->N, a->W, y + 1, z + 1, w);
-    u32 full = i0 & i1 & i2 & i3;
-    u32 ifull = gather2(full & (full >> 1));
-    u32 inr = (cy < a->dR1 && cz < a->dR2) ? xin : 0u;
-    a->outFlags[((size_t)cz * a->R + cy) * a->W + w] = (u16)(ifull | (gather2(f) & inr));
-  }
-};
-struct DilFineFin {  // D = TG ? F : (DF ? 1 : I)
-  const MorphArgs* a;
-  int w;
-  int y;
-  __device__ void operator()(int z, u32 f) const {
-    if (y >= a->N) return;
-    const int cy = y >> 1, cz = z >> 1;
-    u32 df = expand2(flag_word(a->DF, a->R, a->W, cy, cz, w));
-    u32 tg = expand2(flag_word(a->TG, a->R, a->W, cy, cz, w));
-    u32 d = df | row_word(a->I, a->N, a->W, y, z, w);
-    a->outRows[((size_t)z * a->N + y) * a->W + w] = (tg & f) | (~tg & d);
-  }
-};
-struct EroCoarseFin {  // CL(c) = c < dR & D(c) != 0 & I(c) != 0xff & Ec(2c+1)
-  const MorphArgs* a;
-  int w;
-  int y;  // odd row 2cy+1
-  u32 xin;
-  __device__ void operator()(int z, u32 e) const {
-    if (y >= a->N) return;
-    const int cy = y >> 1, cz = z >> 1, y0 = y - 1, z0 = z - 1;
-    u32 d0 = row_word(a->D, a->N, a->W, y0, z0, w), d1 = row_word(a->D, a->N, a->W, y0 + 1, z0, w);
-    u32 d2 = row_word(a->D, a->N, a->W, y0, z0 + 1, w), d3 = row_word(a->D, a->N, a->W, y0 + 1, z0 + 1, w);
-    u32 dany = d0 | d1 | d2 | d3;
-    u32 i0 = row_word(a->I, a->N, a->W, y0, z0, w), i1 = row_word(a->I, a->N, a->W, y0 + 1, z0, w);
-    u32 i2 = row_word(a->I, a->N, a->W, y0, z0 + 1, w), i3 = row_word(a->I, a->N, a->W, y0 + 1, z0 + 1, w);
-    u32 full = i0 & i1 & i2 & i3;
-    u32 inr = (cy < a->dR1 && cz < a->dR2) ? xin : 0u;
-    a->outFlags[((size_t)cz * a->R + cy) * a->W + w] =
-        (u16)(gather2(e >> 1) & gather2(dany | (dany >> 1)) & ~gather2(full & (full >> 1)) & inr);
-  }
-};
-struct EroFineFin {  // C = UG ? ~Ef : C'
-  const MorphArgs* a;
-  int w;
-  int y;
-  __device__ void operator()(int z, u32 e) const {
-    if (y >= a->N) return;
-    const int cy = y >> 1, cz = z >> 1;
-    u32 cl = expand2(flag_word(a->CL, a->R, a->W, cy, cz, w));
-    u32 ug = expand2(flag_word(a->UG, a->R, a->W, cy, cz, w));
-    u32 cw = row_word(a->D, a->N, a->W, y, z, w) & ~cl;
-    a->outRows[((size_t)z * a->N + y) * a->W + w] = (ug & ~e) | (~ug & cw);
-  }
-};
-
-constexpr int kMaxWaves = 8;
-
-// Coarse dilation -> DF flags (lanes own the even rows, even planes).
-__global__ void __launch_bounds__(64 * kMaxWaves) k_dil_coarse(MorphArgs a) {
-  extern __shared__ u32 smem[];
-  __shared__ u32 set4[4];
-  TileGeo g = tile_geo(a, 0, 0, 2);
-  Lds L = carve(smem, a, g.nout());
-  const int y1 = g.yfirst + a.ystep * kTileRows;
-  // F = 0 when no occupied voxel is in reach; F unused when every cell is full
-  bool skip = !bricks_any(a.brickI, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff, g.zfirst - a.maxoff,
-                          g.zlast + 2 + a.maxoff, g.wx - 1, g.wx + 1, 0) ||
-              bricks_all(a.brickI, a.N, a.W, g.yfirst, y1, g.zfirst, g.zlast + 2, g.wx);
-  DilCoarseFin fin{&a, g.wx, g.yfirst + 2 * lane_id(), cell_range_mask(g.wx, 0, a.dR0)};
-  run_tile(a, L, set4, g, a.I, 0, skip, fin);
-}
-
-// Fine dilation -> D rows.
-__global__ void __launch_bounds__(64 * kMaxWaves) k_dil_fine(MorphArgs a) {
-  extern __shared__ u32 smem[];
-  __shared__ u32 set4[4];
-  TileGeo g = tile_geo(a, 0, 0, 1);
-  Lds L = carve(smem, a, g.nout());
-  bool skip = !tile_has_flags(a.TG, a.R, a.W, g.wx, g.yfirst, g.yfirst + kTileRows, g.zfirst, g.zlast + 1);
-  DilFineFin fin{&a, g.wx, g.yfirst + lane_id()};
-  run_tile(a, L, set4, g, a.I, 0, skip, fin);
-}
-
-// Coarse erosion -> CL flags (lanes own the odd rows, odd planes).
-__global__ void __launch_bounds__(64 * kMaxWaves) k_ero_coarse(MorphArgs a) {
-  extern __shared__ u32 smem[];
-  __shared__ u32 set4[4];
-  TileGeo g = tile_geo(a, 1, 1, 2);
-  Lds L = carve(smem, a, g.nout());
-  const int y1 = g.yfirst + a.ystep * kTileRows;
-  bool skip = !bricks_any(a.brickK, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff, g.zfirst - a.maxoff,
-                          g.zlast + 1 + a.maxoff, g.wx - 1, g.wx + 1, 0);
-  EroCoarseFin fin{&a, g.wx, g.yfirst + 2 * lane_id(), cell_range_mask(g.wx, 0, a.dR0)};
-  run_tile(a, L, set4, g, a.K, 1, skip, fin);
-}
-
-// Fine erosion -> C rows.
-__global__ void __launch_bounds__(64 * kMaxWaves) k_ero_fine(MorphArgs a) {
-  extern __shared__ u32 smem[];
-  __shared__ u32 set4[4];
-  TileGeo g = tile_geo(a, 0, 0, 1);
-  Lds L = carve(smem, a, g.nout());
-  bool skip = !tile_has_flags(a.UG, a.R, a.W, g.wx, g.yfirst, g.yfirst + kTileRows, g.zfirst, g.zlast + 1);
-  EroFineFin fin{&a, g.wx, g.yfirst + lane_id()};
-  run_tile(a, L, set4, g, a.K, 2, skip, fin);
-}
+// Runtime-table kernels (any radius field, sphere or cube).
+__global__ void __launch_bounds__(64 * kMaxWaves) k_dil_coarse(MorphArgs a) { run_pass<0, 0>(a); }
+__global__ void __launch_bounds__(64 * kMaxWaves) k_dil_fine(MorphArgs a) { run_pass<1, 0>(a); }
+__global__ void __launch_bounds__(64 * kMaxWaves) k_ero_coarse(MorphArgs a) { run_pass<2, 0>(a); }
+__global__ void __launch_bounds__(64 * kMaxWaves) k_ero_fine(MorphArgs a) { run_pass<3, 0>(a); }
 
 // Tight contour of DF -> TG flags (one thread per cell-row word).
 __global__ void k_tight(MorphArgs a) {
@@ -613,6 +70,13 @@ __global__ void k_tight(MorphArgs a) {
   u32 valid = cell_range_mask(wx, 0, a.R - 1);
   if (cy > a.R - 2 || cz > a.R - 2) valid = 0;
   a.outFlags[gid] = (u16)(any & ~all & valid);
+  // D outside the tight contour: DF ? 1 : I (the fine pass rewrites TG words)
+  const u32 df = expand2(flag_word(a.DF, a.R, a.W, cy, cz, wx));
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const size_t o = ((size_t)(2 * cz + (s >> 1)) * a.N + 2 * cy + (s & 1)) * a.W + wx;
+    a.outRows[o] = df | a.I[o];
+  }
 }
 
 // Brick summary of a grid (brick = 1 word x 8 rows x 8 planes; bit0 = any
@@ -708,12 +172,15 @@ __global__ void k_base_contour(MorphArgs a) {
   int cy = (int)(t % a.R), cz = (int)(t / a.R);
   u32 valid = cell_range_mask(wx, 1, a.R - 1);
   if (cy < 1 || cy > a.R - 2 || cz < 1 || cz > a.R - 2) valid = 0;
+  u32 c[4], l[4], r[4], o[4];
+  cprime_rows(a, cy, cz, wx, c);
+  // C outside the base contour is C' (the fine pass rewrites UG words)
+#pragma unroll
+  for (int s = 0; s < 4; ++s) a.outRows[((size_t)(2 * cz + (s >> 1)) * a.N + 2 * cy + (s & 1)) * a.W + wx] = c[s];
   if (!valid) {
     a.outFlags[gid] = 0;
     return;
   }
-  u32 c[4], l[4], r[4], o[4];
-  cprime_rows(a, cy, cz, wx, c);
   cprime_rows(a, cy, cz, wx - 1, l);
   cprime_rows(a, cy, cz, wx + 1, r);
   u32 diff = 0u;
@@ -735,9 +202,9 @@ __global__ void k_base_contour(MorphArgs a) {
 
 // LDS bytes of a streaming launch with `waves` waves per workgroup.
 size_t lds_bytes(const MorphArgs& a, bool var, int waves) {
-  const int GR = a.ystep * (kTileRows - 1) + 1 + 2 * a.maxoff;
   const int nplanes = a.ystep == 2 ? a.Zt / 2 : a.Zt;
-  size_t words = (size_t)a.tabWords + (size_t)(a.maxm + 1) * GR * waves + (size_t)nplanes * kTileRows;
+  size_t words = (size_t)a.tabWords + (size_t)ct_slice_words(a.maxm, a.maxoff, a.ystep) * waves +
+                 (size_t)nplanes * kTileRows;
   if (var) words += (size_t)nplanes * kTileRows;
   return words * 4;
 }
@@ -777,6 +244,38 @@ dim3 stream_grid(const MorphArgs& a) {
   return dim3((unsigned)(a.W * ((a.N + rows - 1) / rows) * ((a.N + a.Zt - 1) / a.Zt)));
 }
 
+const CtKernels& ct_kernels() {
+  static const CtKernels k = [] {
+    CtKernels t{};
+    ct_register_part0(t);
+    ct_register_part1(t);
+    ct_register_part2(t);
+    ct_register_part3(t);
+    ct_register_part4(t);
+    ct_register_part5(t);
+    ct_register_part6(t);
+    ct_register_part7(t);
+    return t;
+  }();
+  return k;
+}
+
+// Launches streaming pass `pass` (0..3): the kernel compiled for chord-table
+// radius rho when there is one (fixedOk: uniform radius, sphere), else the
+// runtime-table kernel `generic`.
+hipError_t launch_pass(int pass, const void* generic, MorphArgs a, bool var, bool fixedOk, int rho,
+                       hipStream_t s) {
+  const void* fn = generic;
+  if (fixedOk && rho >= 1 && rho <= kCtMaxRho && ct_kernels().fn[pass][rho]) {
+    fn = ct_kernels().fn[pass][rho];
+    a.tabWords = 0;  // the chords are compiled in: no table in LDS
+  }
+  const int nw = stream_waves(a, var);
+  void* args[] = {&a};
+  GM_CHECK(hipLaunchKernel(fn, stream_grid(a), dim3(64 * nw), args, lds_bytes(a, var, nw), s));
+  return hipGetLastError();
+}
+
 }  // namespace
 
 // rho_lo/rho_hi: range of the per-voxel radius r over the field; tabWords:
@@ -797,21 +296,20 @@ hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const Radius
   if (ev) GM_CHECK(hipEventRecord(ev[0], s));
   hipLaunchKernelGGL(k_brick_summary, dim3(d.N / 8, d.N / 8), dim3(8 * d.W), 0, s, b.I, d.N, d.W, b.brickI);
   GM_CHECK(hipGetLastError());
+  const bool fixedOk = !var && se == 0;
+  const int rho = (rf.r & 0x7f) + 1;
   a.ystep = 2;
   a.outFlags = b.DF;
-  int nw = stream_waves(a, var);
-  hipLaunchKernelGGL(k_dil_coarse, stream_grid(a), dim3(64 * nw), lds_bytes(a, var, nw), s, a);
-  GM_CHECK(hipGetLastError());
+  GM_CHECK(launch_pass(0, reinterpret_cast<const void*>(&k_dil_coarse), a, var, fixedOk, rho, s));
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
   a.outFlags = b.TG;
+  a.outRows = b.D;
   hipLaunchKernelGGL(k_tight, dim3((unsigned)((cellwords + 255) / 256)), dim3(256), 0, s, a);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   a.ystep = 1;
   a.outRows = b.D;
-  nw = stream_waves(a, var);
-  hipLaunchKernelGGL(k_dil_fine, stream_grid(a), dim3(64 * nw), lds_bytes(a, var, nw), s, a);
-  GM_CHECK(hipGetLastError());
+  GM_CHECK(launch_pass(1, reinterpret_cast<const void*>(&k_dil_fine), a, var, fixedOk, rho, s));
   if (ev) GM_CHECK(hipEventRecord(ev[3], s));
   return hipSuccess;
 }
@@ -833,21 +331,20 @@ hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusF
   hipLaunchKernelGGL(k_contour_K, dim3(d.N / 8, d.N / 8), dim3(8 * d.W), 0, s, a, b.brickK);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
+  const bool fixedOk = !var;
+  const int rho = (rf.r & 0x7f) - 1;
   a.ystep = 2;
   a.outFlags = b.CL;
-  int nw = stream_waves(a, var);
-  hipLaunchKernelGGL(k_ero_coarse, stream_grid(a), dim3(64 * nw), lds_bytes(a, var, nw), s, a);
-  GM_CHECK(hipGetLastError());
+  GM_CHECK(launch_pass(2, reinterpret_cast<const void*>(&k_ero_coarse), a, var, fixedOk, rho, s));
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   a.outFlags = b.UG;
+  a.outRows = b.C;
   hipLaunchKernelGGL(k_base_contour, dim3((unsigned)((cellwords + 255) / 256)), dim3(256), 0, s, a);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[3], s));
   a.ystep = 1;
   a.outRows = b.C;
-  nw = stream_waves(a, var);
-  hipLaunchKernelGGL(k_ero_fine, stream_grid(a), dim3(64 * nw), lds_bytes(a, var, nw), s, a);
-  GM_CHECK(hipGetLastError());
+  GM_CHECK(launch_pass(3, reinterpret_cast<const void*>(&k_ero_fine), a, var, fixedOk, rho, s));
   if (ev) GM_CHECK(hipEventRecord(ev[4], s));
   return hipSuccess;
 }
@@ -911,5 +408,32 @@ std::vector<u32> build_se_table(int kind, int rho, bool cube) {
   out[4] = (u32)out.size();
   return out;
 }
+
+// The compiled chord tables must list exactly the chords of build_se_table.
+template <int KIND, int RHO>
+static bool ct_matches() {
+  constexpr SETabCT<KIND, RHO> T{};
+  const std::vector<u32> h = build_se_table(KIND, RHO, false);
+  if ((int)h[2] != T.maxm || (int)h[3] != T.hy) return false;
+  size_t p = 5;
+  int seen = 0;
+  for (u32 c = 0; c < h[1]; ++c) {
+    const int b = (int)(h[p] & 0xffffu), np = (int)(h[p] >> 16);
+    ++p;
+    if (b >= SETabCT<KIND, RHO>::NMAX || T.np[b] != np) return false;
+    for (int e = 0; e < np; ++e)
+      if ((int)(h[p + e] & 0xffffu) != e || T.m[b][e] != (int)(h[p + e] >> 16)) return false;
+    p += np;
+    ++seen;
+  }
+  int listed = 0;
+  for (int b = 0; b < SETabCT<KIND, RHO>::NMAX; ++b) listed += T.np[b] > 0;
+  return listed == seen && (int)p == (int)h[4];
+}
+template <int... Rs>
+static bool ct_matches_all(std::integer_sequence<int, Rs...>) {
+  return ((ct_matches<SE_CORNER, Rs + 1>() && ct_matches<SE_CENTRE, Rs + 1>()) && ...);
+}
+bool ct_tables_ok() { return ct_matches_all(std::make_integer_sequence<int, kCtMaxRho>{}); }
 
 }  // namespace gm

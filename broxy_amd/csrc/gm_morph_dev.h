@@ -1,0 +1,864 @@
+// Device code shared by the GMorpho morphology kernels: bit-row helpers, the
+// streaming chord engine (runtime and compiled tables), tile geometry, skip
+// tests, epilogues and the pass bodies. Included by gm_morph.hip (runtime-
+// table kernels, host launchers) and gm_morph_ct.hip (one kernel per pass and
+// compiled radius). Everything here has internal linkage.
+#pragma once
+#include <utility>
+
+#include "gm_internal.h"
+
+namespace gm {
+namespace {
+
+
+__device__ __forceinline__ u32 spread2(u32 v) {
+  v &= 0xffffu;
+  v = (v | (v << 8)) & 0x00ff00ffu;
+  v = (v | (v << 4)) & 0x0f0f0f0fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+__device__ __forceinline__ u32 gather2(u32 v) {
+  v &= 0x55555555u;
+  v = (v | (v >> 1)) & 0x33333333u;
+  v = (v | (v >> 2)) & 0x0f0f0f0fu;
+  v = (v | (v >> 4)) & 0x00ff00ffu;
+  v = (v | (v >> 8)) & 0x0000ffffu;
+  return v;
+}
+// 16 cell flags -> 32 fine bits (both x sub-voxels of each cell)
+__device__ __forceinline__ u32 expand2(u32 v) {
+  u32 s = spread2(v);
+  return s | (s << 1);
+}
+
+__device__ __forceinline__ u32 row_word(const u32* __restrict__ g, int N, int W, int y, int z, int w) {
+  if ((unsigned)y >= (unsigned)N || (unsigned)z >= (unsigned)N || (unsigned)w >= (unsigned)W) return 0u;
+  return g[((size_t)z * N + y) * W + w];
+}
+__device__ __forceinline__ u32 flag_word(const u16* __restrict__ g, int R, int W, int cy, int cz, int w) {
+  if ((unsigned)cy >= (unsigned)R || (unsigned)cz >= (unsigned)R || (unsigned)w >= (unsigned)W) return 0u;
+  return g[((size_t)cz * R + cy) * W + w];
+}
+// Branch-free forms of row_word / flag_word (clamped address, masked value):
+// a run of them issues its loads back to back.
+__device__ __forceinline__ u32 ld_row(const u32* __restrict__ g, int N, int W, int y, int z, int w) {
+  const bool in = (unsigned)y < (unsigned)N && (unsigned)z < (unsigned)N && (unsigned)w < (unsigned)W;
+  const int yc = min(max(y, 0), N - 1), zc = min(max(z, 0), N - 1), wc = min(max(w, 0), W - 1);
+  const u32 v = g[((size_t)zc * N + yc) * W + wc];
+  return in ? v : 0u;
+}
+__device__ __forceinline__ u32 ld_flag(const u16* __restrict__ g, int R, int W, int cy, int cz, int w) {
+  const bool in = (unsigned)cy < (unsigned)R && (unsigned)cz < (unsigned)R && (unsigned)w < (unsigned)W;
+  const int yc = min(max(cy, 0), R - 1), zc = min(max(cz, 0), R - 1), wc = min(max(w, 0), W - 1);
+  const u32 v = g[((size_t)zc * R + yc) * W + wc];
+  return in ? v : 0u;
+}
+
+// Mask of cells 16w..16w+15 whose x index lies in [lo, hi).
+__device__ __forceinline__ u32 cell_range_mask(int w, int lo, int hi) {
+  int a = lo - 16 * w, b = hi - 16 * w;
+  a = a < 0 ? 0 : (a > 16 ? 16 : a);
+  b = b < 0 ? 0 : (b > 16 ? 16 : b);
+  if (b <= a) return 0u;
+  u32 m = (b >= 16) ? 0xffffu : ((1u << b) - 1u);
+  return m & ~((1u << a) - 1u);
+}
+
+// shr(k): bit x reads source x+k; shl(k): bit x reads source x-k (1 <= k <= 32)
+__device__ __forceinline__ u32 shr_k(u32 C, u32 Rw, int k) {
+  return k >= 32 ? Rw : __builtin_amdgcn_alignbit(Rw, C, (u32)k);
+}
+__device__ __forceinline__ u32 shl_k(u32 L, u32 C, int k) {
+  return k >= 32 ? L : __builtin_amdgcn_alignbit(C, L, (u32)(32 - k));
+}
+
+// Output geometry of one streaming tile: lane i of every wave owns row
+// yfirst + ystep*i; output planes are zfirst, zfirst + zstep, ... <= zlast.
+struct TileGeo {
+  int wx;
+  int yfirst, ystep;
+  int tz0, zfirst, zlast, zstep;
+  __device__ int nout() const { return zlast < zfirst ? 0 : (zlast - zfirst) / zstep + 1; }
+};
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+// uniform per wave; readfirstlane lets the compiler keep it in an SGPR
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+__device__ __forceinline__ int num_waves() { return (int)(blockDim.x >> 6); }
+
+// One streaming pass of the table tl (in LDS) over the source src, ORed into
+// acc[plane index][lane] (plane index = (zo - zfirst) / zstep). The waves of
+// the workgroup take the input planes round robin; each builds the G table of
+// its plane in its own LDS slice (gtAll + wave * gtStride) and accumulates
+// with LDS atomics, so the plane loop needs no workgroup barrier.
+template <class Src>
+__device__ void stream(const u32* tl, const Src& src, u32* gtAll, int gtStride, u32* acc, const TileGeo& g) {
+  const int lane = lane_id(), wv = wave_id(), nw = num_waves();
+  const int kind = (int)tl[0], nclass = (int)tl[1], maxm = (int)tl[2], HY = (int)tl[3];
+  const int dzl = -HY, dzh = (kind == SE_CORNER) ? HY - 1 : HY;
+  const int GR = g.ystep * (kTileRows - 1) + 1 + 2 * HY;
+  const int ybase = g.yfirst - HY;
+  u32* gt = gtAll + wv * gtStride;
+  const int zbeg = g.zfirst + dzl, zend = g.zlast + dzh;
+  const int zshift = g.zstep == 2 ? 1 : 0;
+  const int base = g.ystep * lane + HY;
+  u32 pre[4][3];
+  int zi = zbeg + wv;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int gr = lane + kTileRows * k;
+    pre[k][0] = pre[k][1] = pre[k][2] = 0u;
+    if (gr < GR && zi <= zend) {
+      pre[k][0] = src(ybase + gr, zi, g.wx - 1);
+      pre[k][1] = src(ybase + gr, zi, g.wx);
+      pre[k][2] = src(ybase + gr, zi, g.wx + 1);
+    }
+  }
+  for (; zi <= zend; zi += nw) {
+    // stage 1: x-window dilations of this plane's rows into the wave's slice
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      int gr = lane + kTileRows * k;
+      if (gr < GR) {
+        const u32 Lw = pre[k][0], Cw = pre[k][1], Rw = pre[k][2];
+        u32 gg;
+        if (kind == SE_CORNER) {
+          gg = Cw | shl_k(Lw, Cw, 1);
+          gt[gr] = gg;
+          for (int m = 1; m <= maxm; ++m) {
+            gg |= shr_k(Cw, Rw, m) | shl_k(Lw, Cw, m + 1);
+            gt[m * GR + gr] = gg;
+          }
+        } else {
+          gg = Cw;
+          gt[gr] = gg;
+          for (int m = 1; m <= maxm; ++m) {
+            gg |= shr_k(Cw, Rw, m) | shl_k(Lw, Cw, m);
+            gt[m * GR + gr] = gg;
+          }
+        }
+      }
+    }
+    // prefetch this wave's next plane while this one is reduced
+    const int zn = zi + nw;
+    if (zn <= zend) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        int gr = lane + kTileRows * k;
+        if (gr < GR) {
+          pre[k][0] = src(ybase + gr, zn, g.wx - 1);
+          pre[k][1] = src(ybase + gr, zn, g.wx);
+          pre[k][2] = src(ybase + gr, zn, g.wx + 1);
+        }
+      }
+    }
+    // the slice is written and read by this wave only: in-order LDS suffices
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // stage 2: chords of each dz class
+    const u32* p = tl + 5;
+    for (int c = 0; c < nclass; ++c) {
+      const u32 hw = p[0];
+      const int b = (int)(hw & 0xffffu), np = (int)(hw >> 16);
+      ++p;
+      u32 disk = 0u;
+      if (kind == SE_CORNER) {
+#pragma unroll 4
+        for (int e = 0; e < np; ++e) {
+          const u32 ew = p[e];
+          const int a = (int)(ew & 0xffffu);
+          const u32* gm = gt + (ew >> 16) * GR;
+          disk |= gm[base + a] | gm[base - a - 1];
+        }
+      } else {
+#pragma unroll 4
+        for (int e = 0; e < np; ++e) {
+          const u32 ew = p[e];
+          const int a = (int)(ew & 0xffffu);
+          const u32* gm = gt + (ew >> 16) * GR;
+          disk |= gm[base + a] | gm[base - a];
+        }
+      }
+      p += np;
+      if (!disk) continue;
+      // output planes zo = zi - dz for the dz values of the class
+      int zo = zi - b;
+      if (zo >= g.zfirst && zo <= g.zlast && ((zo - g.zfirst) & (g.zstep - 1)) == 0)
+        atomicOr(&acc[((zo - g.zfirst) >> zshift) * kTileRows + lane], disk);
+      const int dz1 = (kind == SE_CORNER) ? -b - 1 : -b;
+      if (dz1 != b) {
+        zo = zi - dz1;
+        if (zo >= g.zfirst && zo <= g.zlast && ((zo - g.zfirst) & (g.zstep - 1)) == 0)
+          atomicOr(&acc[((zo - g.zfirst) >> zshift) * kTileRows + lane], disk);
+      }
+    }
+    // keep the next iteration's stage 1 behind this stage 2 in the slice
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+struct GridSrc {
+  const u32* g;
+  int N, W;
+  __device__ __forceinline__ u32 operator()(int y, int z, int w) const { return row_word(g, N, W, y, z, w); }
+};
+
+// ---------------------------------------------------------------------------
+// Compile-time chord tables. The same chords as build_se_table (host), built
+// by a constexpr constructor so that the chord loop of a fixed radius unrolls
+// into LDS reads at immediate offsets (ds_read2_b32 pairs).
+constexpr long long ct_ecen(long long t) { return t == 0 ? 0 : (2 * t - 1) * (2 * t - 1); }
+constexpr int ct_isq_lt(long long X) {  // max m >= 0 with m*m < X (X >= 1)
+  int m = 0;
+  while ((long long)(m + 1) * (m + 1) < X) ++m;
+  return m;
+}
+template <int KIND, int RHO>
+struct SETabCT {
+  static constexpr int NMAX = RHO + 2;
+  int m[NMAX][NMAX];  // chord half-width of (class b, dy class a); -1: no chord
+  int np[NMAX];
+  int nb, hy, maxm;
+  constexpr SETabCT() : m{}, np{}, nb(0), hy(0), maxm(0) {
+    int maxab = 0;
+    for (int b = 0; b < NMAX; ++b) {
+      np[b] = 0;
+      for (int a = 0; a < NMAX; ++a) m[b][a] = -1;
+    }
+    if (KIND == SE_CORNER) {
+      const long long r2 = (long long)RHO * RHO;
+      for (int b = 0; b <= RHO - 1; ++b) {
+        for (int a = 0; (long long)a * a + (long long)b * b < r2; ++a) {
+          m[b][a] = ct_isq_lt(r2 - (long long)a * a - (long long)b * b);
+          np[b] = a + 1;
+          maxm = m[b][a] > maxm ? m[b][a] : maxm;
+          maxab = a > maxab ? a : maxab;
+        }
+        if (np[b]) {
+          maxab = b > maxab ? b : maxab;
+          nb = b + 1;
+        }
+      }
+      hy = maxab + 1;
+    } else {
+      const long long lim = 4LL * RHO * RHO;
+      for (int b = 0; ct_ecen(b) < lim; ++b) {
+        for (int a = 0; ct_ecen(a) + ct_ecen(b) < lim; ++a) {
+          const long long rem = lim - ct_ecen(a) - ct_ecen(b);
+          int mm = 0;
+          while (ct_ecen(mm + 1) < rem) ++mm;
+          m[b][a] = mm;
+          np[b] = a + 1;
+          maxm = mm > maxm ? mm : maxm;
+          maxab = a > maxab ? a : maxab;
+        }
+        maxab = b > maxab ? b : maxab;
+        nb = b + 1;
+      }
+      hy = maxab;
+    }
+  }
+};
+
+// G-table geometry of a fixed-radius stream. Rows of the tile window are
+// r = 0..GR-1 (row yfirst - HY + r); row r holds G_0..G_MAXM in consecutive
+// words (stride M1, odd, so that lanes reading rows lane + c hit distinct
+// banks). ystep 1: row r at r*M1. ystep 2: rows are split by parity (r even
+// at (r/2)*M1, odd at (GRH + r/2)*M1) so that the lanes, which own rows
+// 2*lane, again read rows one stride apart. Every chord of a ball then lies
+// within (2*HY+1)*M1 words of the lane's base.
+template <int KIND, int RHO, int YS>
+struct StreamCT {
+  static constexpr SETabCT<KIND, RHO> T{};
+  static constexpr int HY = T.hy, MAXM = T.maxm, NB = T.nb, NA = SETabCT<KIND, RHO>::NMAX;
+  static constexpr int M1 = (MAXM + 1) | 1;
+  static constexpr int GR = YS * (kTileRows - 1) + 1 + 2 * HY;
+  static constexpr int GRH = (GR + 1) / 2;
+  static constexpr int NK = (GR + kTileRows - 1) / kTileRows;
+  static constexpr int row_at(int r) { return YS == 1 ? r * M1 : ((r & 1) * GRH + (r >> 1)) * M1; }
+  // word of G_m at the row offset d from the lane's row, relative to gt + lane*M1
+  static constexpr int off(int m, int d) {
+    return YS == 1 ? (HY + d) * M1 + m : (((HY + d) & 1) * GRH + ((HY + d) >> 1)) * M1 + m;
+  }
+};
+// Words per wave slice of a fixed-radius stream (upper bound, any kind).
+__host__ __device__ constexpr int ct_slice_words(int maxm, int maxoff, int ystep) {
+  return (maxm + 2) * (((ystep * (kTileRows - 1) + 1 + 2 * maxoff) + 1) & ~1);
+}
+
+template <class C, int KIND, int B, int A>
+__device__ __forceinline__ u32 ct_pair(const u32* gl) {
+  constexpr int m = C::T.m[B][A];
+  if constexpr (m < 0) {
+    return 0u;
+  } else if constexpr (KIND == SE_CENTRE && A == 0) {
+    return gl[C::off(m, 0)];
+  } else {
+    constexpr int d2 = KIND == SE_CORNER ? -A - 1 : -A;
+    return gl[C::off(m, A)] | gl[C::off(m, d2)];
+  }
+}
+template <class C, int KIND, int B, int... As>
+__device__ __forceinline__ u32 ct_disk(const u32* gl, std::integer_sequence<int, As...>) {
+  return (0u | ... | ct_pair<C, KIND, B, As>(gl));
+}
+// Chord ORs of every dz class into disk[] (all LDS reads independent).
+// Chords of class B that class B+1 does not already cover. The half-width
+// of a dy class never grows with |dz| (m[b][a] >= m[b+1][a], checked when
+// the table is compiled), and G_m grows with m, so disk(B) = disk(B+1) | the
+// chords of B whose half-width differs from B+1's.
+template <class C, int B, int A>
+constexpr bool ct_new_chord() {
+  return C::T.m[B][A] >= 0 && (B + 1 >= C::NB || C::T.m[B + 1][A] != C::T.m[B][A]);
+}
+template <class C, int KIND, int B, int... As>
+__device__ __forceinline__ u32 ct_new(const u32* gl, std::integer_sequence<int, As...>) {
+  return (0u | ... | (ct_new_chord<C, B, As>() ? ct_pair<C, KIND, B, As>(gl) : 0u));
+}
+template <class C, int KIND, int B>
+__device__ __forceinline__ void ct_disks_from(const u32* gl, u32* disk) {
+  u32 d = ct_new<C, KIND, B>(gl, std::make_integer_sequence<int, C::NA>{});
+  if constexpr (B + 1 < C::NB) d |= disk[B + 1];
+  disk[B] = d;
+  if constexpr (B > 0) ct_disks_from<C, KIND, B - 1>(gl, disk);
+}
+template <class C>
+constexpr bool ct_monotone() {
+  for (int b = 0; b + 1 < C::NB; ++b)
+    for (int a = 0; a < C::NA; ++a)
+      if (C::T.m[b + 1][a] >= 0 && C::T.m[b][a] < C::T.m[b + 1][a]) return false;
+  for (int b = 0; b < C::NB; ++b)
+    if (C::T.np[b] == 0) return false;
+  return true;
+}
+// OR of class B's disk into the output planes zi - dz for its dz values; the
+// plane tests are uniform over the wave.
+template <class C, int KIND, int B>
+__device__ __forceinline__ void ct_scatter(u32 disk, int zi, const TileGeo& g, u32* acc, int lane, int zshift) {
+  if constexpr (C::T.np[B] > 0) {
+    int zo = zi - B;
+    if (zo >= g.zfirst && zo <= g.zlast && ((zo - g.zfirst) & (g.zstep - 1)) == 0)
+      atomicOr(&acc[((zo - g.zfirst) >> zshift) * kTileRows + lane], disk);
+    constexpr int dz1 = KIND == SE_CORNER ? -B - 1 : -B;
+    if constexpr (dz1 != B) {
+      zo = zi - dz1;
+      if (zo >= g.zfirst && zo <= g.zlast && ((zo - g.zfirst) & (g.zstep - 1)) == 0)
+        atomicOr(&acc[((zo - g.zfirst) >> zshift) * kTileRows + lane], disk);
+    }
+  }
+}
+template <class C, int KIND, int... Bs>
+__device__ __forceinline__ void ct_scatter_all(const u32* disk, int zi, const TileGeo& g, u32* acc, int lane,
+                                               int zshift, std::integer_sequence<int, Bs...>) {
+  (ct_scatter<C, KIND, Bs>(disk[Bs], zi, g, acc, lane, zshift), ...);
+}
+
+// Words wx-1, wx, wx+1 of row (y, z) of a bit grid, zero outside the grid.
+// Branch-free (clamped address, masked value) so that the loads of a plane
+// issue back to back.
+__device__ __forceinline__ void load3(const u32* __restrict__ G, int N, int W, int y, int z, int wx, u32& l, u32& c,
+                                      u32& r) {
+  const bool in = (unsigned)y < (unsigned)N && (unsigned)z < (unsigned)N;
+  const int yc = min(max(y, 0), N - 1), zc = min(max(z, 0), N - 1);
+  const u32* row = G + ((size_t)zc * N + yc) * W;
+  const u32 lv = row[max(wx - 1, 0)], cv = row[wx], rv = row[min(wx + 1, W - 1)];
+  l = (in && wx > 0) ? lv : 0u;
+  c = in ? cv : 0u;
+  r = (in && wx + 1 < W) ? rv : 0u;
+}
+
+// Fixed-radius version of stream(): same plane schedule and accumulation,
+// chords and G-table geometry fixed at compile time.
+template <int KIND, int RHO, int YS>
+__device__ void stream_ct(const GridSrc& src, u32* gtAll, int gtStride, u32* acc, const TileGeo& g) {
+  using C = StreamCT<KIND, RHO, YS>;
+  const int lane = lane_id(), wv = wave_id(), nw = num_waves();
+  constexpr int HY = C::HY;
+  constexpr int dzl = -HY, dzh = (KIND == SE_CORNER) ? HY - 1 : HY;
+  const int ybase = g.yfirst - HY;
+  u32* gt = gtAll + wv * gtStride;
+  const u32* gl = gt + lane * C::M1;
+  const int zbeg = g.zfirst + dzl, zend = g.zlast + dzh;
+  const int zshift = YS == 2 ? 1 : 0;
+  static_assert(ct_monotone<C>(), "chord half-widths must not grow with |dz|");
+  static_assert((size_t)C::M1 * (YS == 1 ? C::GR : 2 * C::GRH) <= (size_t)ct_slice_words(C::MAXM, C::HY, YS),
+                "slice too small");
+  // the rows of the wave's next two planes are in flight while a plane is
+  // reduced: buffer a holds plane zi, buffer b plane zi + nw
+  u32 pa[C::NK][3], pb[C::NK][3];
+  int zi = zbeg + wv;
+  auto load = [&](u32 (&buf)[C::NK][3], int z) {
+#pragma unroll
+    for (int k = 0; k < C::NK; ++k)
+      load3(src.g, src.N, src.W, ybase + lane + kTileRows * k, z, g.wx, buf[k][0], buf[k][1], buf[k][2]);
+  };
+  // x-window dilations of one plane's rows into the wave's slice
+  auto build = [&](const u32 (&buf)[C::NK][3]) {
+#pragma unroll
+    for (int k = 0; k < C::NK; ++k) {
+      const int gr = lane + kTileRows * k;
+      if (k + 1 < C::NK || gr < C::GR) {
+        const u32 Lw = buf[k][0], Cw = buf[k][1], Rw = buf[k][2];
+        u32* gw = gt + (YS == 1 ? gr * C::M1 : ((gr & 1) * C::GRH + (gr >> 1)) * C::M1);
+        u32 gg = KIND == SE_CORNER ? (Cw | shl_k(Lw, Cw, 1)) : Cw;
+        gw[0] = gg;
+#pragma unroll
+        for (int m = 1; m <= C::MAXM; ++m) {
+          gg |= shr_k(Cw, Rw, m) | shl_k(Lw, Cw, KIND == SE_CORNER ? m + 1 : m);
+          gw[m] = gg;
+        }
+      }
+    }
+  };
+  // chords of every class, ORed into the output planes
+  auto reduce = [&](int z) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    u32 disk[C::NB];
+    ct_disks_from<C, KIND, C::NB - 1>(gl, disk);
+    ct_scatter_all<C, KIND>(disk, z, g, acc, lane, zshift, std::make_integer_sequence<int, C::NB>{});
+    // the next build writes the slice only after these reads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  // planes past zend are loaded (in bounds, masked) and not used
+  load(pa, zi);
+  load(pb, zi + nw);
+  zi = __builtin_amdgcn_readfirstlane(zi);
+  while (zi <= zend) {
+    build(pa);
+    load(pa, zi + 2 * nw);
+    reduce(zi);
+    zi += nw;
+    if (zi > zend) break;
+    build(pb);
+    load(pb, zi + 2 * nw);
+    reduce(zi);
+    zi += nw;
+  }
+}
+
+// Source voxels whose erosion radius r(q)-1 equals rho.
+struct MaskedSrc {
+  const u32* g;
+  int N, W;
+  RadiusField rf;
+  int rho;
+  __device__ __forceinline__ u32 operator()(int y, int z, int w) const {
+    u32 v = row_word(g, N, W, y, z, w);
+    u32 out = 0u;
+    while (v) {
+      int b = __builtin_ctz(v);
+      v &= v - 1u;
+      if (radius_at(rf, 32 * w + b, y, z) - 1 == rho) out |= 1u << b;
+    }
+    return out;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Kernel parameters shared by the streaming kernels.
+struct MorphArgs {
+  int R, N, W;
+  int dR0, dR1, dR2;
+  int Zt;
+  RadiusField rf;
+  int se;  // 0 sphere, 1 cube (dilation)
+  const u32* tables;
+  SETableSet tabs;
+  int maxm, maxoff, tabWords;  // LDS carve sizes
+  int ystep;                   // row step of this launch's lanes
+  const u32* I;
+  const u32* D;
+  const u32* K;
+  u32* outRows;
+  const u16* DF;
+  const u16* TG;
+  const u16* CL;
+  const u16* UG;
+  const uint8_t* brickI;  // per brick (1 word x 8 rows x 8 planes): bit0 any, bit1 all
+  const uint8_t* brickK;
+  u16* outFlags;
+};
+
+struct Lds {
+  u32* tl;
+  u32* gt;
+  int gtStride;
+  u32* out;  // output planes x 64
+  u32* acc;  // per-radius pass accumulator (per-voxel radii only)
+};
+__device__ __forceinline__ int gt_stride(const MorphArgs& a) { return ct_slice_words(a.maxm, a.maxoff, a.ystep); }
+__device__ __forceinline__ Lds carve(u32* smem, const MorphArgs& a, int nplanes) {
+  Lds L;
+  L.tl = smem;
+  L.gt = L.tl + a.tabWords;
+  L.gtStride = gt_stride(a);
+  L.out = L.gt + L.gtStride * num_waves();
+  L.acc = L.out + nplanes * kTileRows;
+  return L;
+}
+
+__device__ __forceinline__ void load_table(u32* tl, const u32* src) {
+  __syncthreads();
+  const int n = (int)src[4];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) tl[i] = src[i];
+  __syncthreads();
+}
+
+__device__ __forceinline__ int table_off(const MorphArgs& a, int which, int rho) {
+  if (rho < 0 || rho > 128) return -1;
+  return which == 0 ? a.tabs.dil[rho] : (which == 1 ? a.tabs.ero_corner[rho] : a.tabs.ero_centre[rho]);
+}
+
+// Any brick flag (bit) set in the brick region covering rows [y0,y1),
+// planes [z0,z1), words [w0,w1] (clamped to the grid).
+__device__ bool bricks_any(const uint8_t* br, int N, int W, int y0, int y1, int z0, int z1, int w0, int w1,
+                           int bit) {
+  const int NB = N / 8;
+  int by0 = max(y0, 0) >> 3, by1 = (min(y1, N) - 1) >> 3;
+  int bz0 = max(z0, 0) >> 3, bz1 = (min(z1, N) - 1) >> 3;
+  w0 = max(w0, 0);
+  w1 = min(w1, W - 1);
+  int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1, nw = w1 - w0 + 1;
+  int found = 0;
+  if (ny > 0 && nz > 0 && nw > 0) {
+    int n = ny * nz * nw;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      int w = w0 + i % nw, t = i / nw;
+      int by = by0 + t % ny, bz = bz0 + t / ny;
+      if ((br[((size_t)bz * NB + by) * W + w] >> bit) & 1) found = 1;
+    }
+  }
+  return __syncthreads_or(found) != 0;
+}
+// Every brick of the region has the "all" bit.
+__device__ bool bricks_all(const uint8_t* br, int N, int W, int y0, int y1, int z0, int z1, int w) {
+  const int NB = N / 8;
+  int by0 = max(y0, 0) >> 3, by1 = (min(y1, N) - 1) >> 3;
+  int bz0 = max(z0, 0) >> 3, bz1 = (min(z1, N) - 1) >> 3;
+  int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1;
+  int miss = 0;
+  for (int i = threadIdx.x; i < ny * nz; i += blockDim.x) {
+    int by = by0 + i % ny, bz = bz0 + i / ny;
+    if (!((br[((size_t)bz * NB + by) * W + w] >> 1) & 1)) miss = 1;
+  }
+  return __syncthreads_or(miss) == 0;
+}
+
+__device__ __forceinline__ bool tile_has_flags(const u16* F, int R, int W, int wx, int y0, int y1, int z0, int z1) {
+  int cy0 = y0 >> 1, cy1 = min((y1 + 1) >> 1, R);
+  int cz0 = z0 >> 1, cz1 = min((z1 + 1) >> 1, R);
+  int ny = cy1 - cy0, nz = cz1 - cz0;
+  u32 acc = 0u;
+  if (ny > 0 && nz > 0)
+    for (int i = threadIdx.x; i < ny * nz; i += blockDim.x) acc |= flag_word(F, R, W, cy0 + i % ny, cz0 + i / ny, wx);
+  return __syncthreads_or(acc != 0u) != 0;
+}
+
+// Distinct radii of a tile (128-bit set in LDS). Dilation: radius+add of the
+// output voxels; erosion: radius-1 of the K voxels within the halo.
+__device__ void rho_set_dilate(const RadiusField& rf, int add, const TileGeo& g, int N, u32* set4) {
+  if (threadIdx.x < 4) set4[threadIdx.x] = 0u;
+  __syncthreads();
+  const int y = g.yfirst + g.ystep * lane_id();
+  u32 loc[4] = {0, 0, 0, 0};
+  if (y < N)
+    for (int p = wave_id(); p < g.nout(); p += num_waves()) {
+      const int z = g.zfirst + p * g.zstep;
+      for (int b = 0; b < 32; ++b) {
+        int r = radius_at(rf, 32 * g.wx + b, y, z) + add;
+        loc[r >> 5] |= 1u << (r & 31);
+      }
+    }
+  for (int i = 0; i < 4; ++i)
+    if (loc[i]) atomicOr(&set4[i], loc[i]);
+  __syncthreads();
+}
+__device__ void rho_set_erode(const RadiusField& rf, const u32* K, int N, int W, const TileGeo& g, int halo,
+                              u32* set4) {
+  if (threadIdx.x < 4) set4[threadIdx.x] = 0u;
+  __syncthreads();
+  u32 loc[4] = {0, 0, 0, 0};
+  const int y0 = g.yfirst - halo, y1 = g.yfirst + g.ystep * (kTileRows - 1) + halo;
+  const int z0 = g.zfirst - halo, z1 = g.zlast + halo;
+  const int ny = y1 - y0 + 1;
+  for (int i = threadIdx.x; i < ny * (z1 - z0 + 1); i += blockDim.x) {
+    const int y = y0 + i % ny, z = z0 + i / ny;
+    for (int w = g.wx - 1; w <= g.wx + 1; ++w) {
+      u32 v = row_word(K, N, W, y, z, w);
+      while (v) {
+        int b = __builtin_ctz(v);
+        v &= v - 1u;
+        int r = radius_at(rf, 32 * w + b, y, z) - 1;
+        r = r < 0 ? 0 : r;
+        loc[r >> 5] |= 1u << (r & 31);
+      }
+    }
+  }
+  for (int i = 0; i < 4; ++i)
+    if (loc[i]) atomicOr(&set4[i], loc[i]);
+  __syncthreads();
+}
+
+// Runs the predicate of a pass over the tile and calls fin(zo, F) for every
+// output plane of the thread's row (F = 0 when skip is set). which: 0
+// dilation table (radius+add), 1 erosion corner, 2 erosion centre (radius-1).
+template <int WHICH, int YS, int RHO, class Fin>
+__device__ void run_tile(const MorphArgs& a, Lds L, u32* set4, const TileGeo& g, const u32* src, bool skip,
+                         Fin& fin) {
+  constexpr int which = WHICH;
+  const int lane = lane_id(), wv = wave_id(), nw = num_waves();
+  const int nout = g.nout();
+  const int add = which == 0 ? (a.se == 0 ? 1 : 0) : -1;
+  if (!skip) {
+    for (int i = threadIdx.x; i < nout * kTileRows; i += blockDim.x) L.out[i] = 0u;
+    if constexpr (RHO > 0) {
+      // uniform radius with a compiled chord table (the host picks the kernel)
+      __syncthreads();
+      stream_ct<WHICH == 2 ? SE_CENTRE : SE_CORNER, RHO, YS>(GridSrc{src, a.N, a.W}, L.gt, L.gtStride, L.out, g);
+    } else if (a.rf.mode == 0) {
+      const int rho = (a.rf.r & 0x7f) + add;
+      const int off = table_off(a, which, rho);
+      if (off >= 0 && rho > 0) {
+        load_table(L.tl, a.tables + off);
+        stream(L.tl, GridSrc{src, a.N, a.W}, L.gt, L.gtStride, L.out, g);
+      }
+    } else {
+      // per-voxel radii: one pass per distinct radius, masked into out
+      const int y = g.yfirst + g.ystep * lane;
+      if (which == 0) rho_set_dilate(a.rf, add, g, a.N, set4);
+      else rho_set_erode(a.rf, src, a.N, a.W, g, a.maxoff, set4);
+      for (int i = 0; i < 4; ++i) {
+        u32 m = set4[i];
+        while (m) {
+          const int bit = __builtin_ctz(m);
+          m &= m - 1u;
+          const int rho = 32 * i + bit;
+          const int off = table_off(a, which, rho);
+          if (off < 0 || rho <= 0) continue;
+          load_table(L.tl, a.tables + off);
+          for (int k = threadIdx.x; k < nout * kTileRows; k += blockDim.x) L.acc[k] = 0u;
+          __syncthreads();
+          if (which == 0) stream(L.tl, GridSrc{src, a.N, a.W}, L.gt, L.gtStride, L.acc, g);
+          else stream(L.tl, MaskedSrc{src, a.N, a.W, a.rf, rho}, L.gt, L.gtStride, L.acc, g);
+          __syncthreads();
+          for (int p = wv; p < nout; p += nw) {
+            u32 v = L.acc[p * kTileRows + lane];
+            if (!v) continue;
+            if (which == 0) {
+              const int z = g.zfirst + p * g.zstep;
+              u32 mk = 0u;
+              if (y < a.N)
+                for (int b = 0; b < 32; ++b)
+                  if (radius_at(a.rf, 32 * g.wx + b, y, z) + add == rho) mk |= 1u << b;
+              v &= mk;
+            }
+            L.out[p * kTileRows + lane] |= v;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (skip) {
+    for (int p = wv; p < nout; p += nw) fin.skipped(g.zfirst + p * g.zstep);
+    return;
+  }
+  // the wave's output planes, 4 at a time: all loads first, then the stores
+  for (int p0 = wv; p0 < nout; p0 += 4 * nw) {
+    typename Fin::In in[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) fin.load(g.zfirst + min(p0 + k * nw, nout - 1) * g.zstep, in[k]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int p = p0 + k * nw;
+      if (p < nout) fin.store(g.zfirst + p * g.zstep, L.out[p * kTileRows + lane], in[k]);
+    }
+  }
+}
+
+// Tiles are numbered slab-major (word fastest, then row tile, then plane
+// tile). Workgroups are dealt round-robin to the 8 XCDs; workgroup i runs on
+// XCD i % 8 as that XCD's (i / 8)-th workgroup. Slab 8q + j goes to XCD
+// (j + q) % 8: the W word tiles of a slab, which share every 64-byte row
+// segment, meet in one L2 at about the same time, and the rotation by q keeps
+// any one row band or plane band (where a surface puts its work) from
+// landing on a single XCD.
+__device__ __forceinline__ int xcd_tile(int i, int W, int nslab) {
+  if (nslab & 7) return i;
+  const int k = i >> 3, x = i & 7;
+  const int q = k / W;
+  return ((q << 3) + ((x - q) & 7)) * W + (k - q * W);
+}
+
+__device__ __forceinline__ TileGeo tile_geo(const MorphArgs& a, int yoff, int zoff, int zstep) {
+  TileGeo g;
+  const int rows = a.ystep * kTileRows;
+  const int nty = (a.N + rows - 1) / rows, ntz = (a.N + a.Zt - 1) / a.Zt;
+  const int j = xcd_tile(blockIdx.x, a.W, nty * ntz);
+  const int slab = j / a.W;
+  g.wx = j - slab * a.W;
+  const int ty = slab % nty, tz = slab / nty;
+  g.yfirst = ty * rows + yoff;
+  g.ystep = a.ystep;
+  g.tz0 = tz * a.Zt;
+  g.zfirst = g.tz0 + zoff;
+  int zl = min(g.tz0 + a.Zt, a.N) - 1;
+  g.zlast = zl < g.zfirst ? g.zfirst - zstep : g.zfirst + ((zl - g.zfirst) / zstep) * zstep;
+  g.zstep = zstep;
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// Epilogues.
+// Each epilogue loads what its output word needs (load), then combines it
+// with the tile result (store); skipped() writes the output of a tile whose
+// result is known without the predicate (nothing to load).
+struct DilCoarseFin {  // DF(c) = I(c) full | (c < dR & F(2c))
+  const MorphArgs* a;
+  int w;
+  int y;
+  u32 xin;
+  u16 skipValue;  // 0: no voxel in reach (I empty here); 0xffff: every cell full
+  struct In {
+    u32 full;
+  };
+  __device__ void load(int z, In& in) const {
+    const u32* I = a->I;
+    in.full = ld_row(I, a->N, a->W, y, z, w) & ld_row(I, a->N, a->W, y + 1, z, w) &
+              ld_row(I, a->N, a->W, y, z + 1, w) & ld_row(I, a->N, a->W, y + 1, z + 1, w);
+  }
+  __device__ void store(int z, u32 f, const In& in) const {
+    if (y >= a->N) return;
+    const int cy = y >> 1, cz = z >> 1;
+    const u32 ifull = gather2(in.full & (in.full >> 1));
+    const u32 inr = (cy < a->dR1 && cz < a->dR2) ? xin : 0u;
+    a->outFlags[((size_t)cz * a->R + cy) * a->W + w] = (u16)(ifull | (gather2(f) & inr));
+  }
+  __device__ void skipped(int z) const {
+    if (y >= a->N) return;
+    a->outFlags[((size_t)(z >> 1) * a->R + (y >> 1)) * a->W + w] = skipValue;
+  }
+};
+struct DilFineFin {  // D = TG ? F : D, D already holds DF ? 1 : I
+  const MorphArgs* a;
+  int w;
+  int y;
+  struct In {
+    u32 tg, d;
+  };
+  __device__ void load(int z, In& in) const {
+    in.tg = ld_flag(a->TG, a->R, a->W, y >> 1, z >> 1, w);
+    in.d = ld_row(a->outRows, a->N, a->W, y, z, w);
+  }
+  __device__ void store(int z, u32 f, const In& in) const {
+    if (y >= a->N) return;
+    const u32 tg = expand2(in.tg);
+    a->outRows[((size_t)z * a->N + y) * a->W + w] = (tg & f) | (~tg & in.d);
+  }
+  __device__ void skipped(int) const {}
+};
+struct EroCoarseFin {  // CL(c) = c < dR & D(c) != 0 & I(c) != 0xff & Ec(2c+1)
+  const MorphArgs* a;
+  int w;
+  int y;  // odd row 2cy+1
+  u32 xin;
+  struct In {
+    u32 dany, full;
+  };
+  __device__ void load(int z, In& in) const {
+    const int y0 = y - 1, z0 = z - 1, N = a->N, W = a->W;
+    in.dany = ld_row(a->D, N, W, y0, z0, w) | ld_row(a->D, N, W, y0 + 1, z0, w) | ld_row(a->D, N, W, y0, z0 + 1, w) |
+              ld_row(a->D, N, W, y0 + 1, z0 + 1, w);
+    in.full = ld_row(a->I, N, W, y0, z0, w) & ld_row(a->I, N, W, y0 + 1, z0, w) & ld_row(a->I, N, W, y0, z0 + 1, w) &
+              ld_row(a->I, N, W, y0 + 1, z0 + 1, w);
+  }
+  __device__ void store(int z, u32 e, const In& in) const {
+    if (y >= a->N) return;
+    const int cy = y >> 1, cz = z >> 1;
+    const u32 inr = (cy < a->dR1 && cz < a->dR2) ? xin : 0u;
+    a->outFlags[((size_t)cz * a->R + cy) * a->W + w] =
+        (u16)(gather2(e >> 1) & gather2(in.dany | (in.dany >> 1)) & ~gather2(in.full & (in.full >> 1)) & inr);
+  }
+  __device__ void skipped(int z) const {  // no K in reach: E = 0
+    if (y >= a->N) return;
+    a->outFlags[((size_t)(z >> 1) * a->R + (y >> 1)) * a->W + w] = 0;
+  }
+};
+struct EroFineFin {  // C = UG ? ~Ef : C, C already holds C' = D & ~CL
+  const MorphArgs* a;
+  int w;
+  int y;
+  struct In {
+    u32 ug, c;
+  };
+  __device__ void load(int z, In& in) const {
+    in.ug = ld_flag(a->UG, a->R, a->W, y >> 1, z >> 1, w);
+    in.c = ld_row(a->outRows, a->N, a->W, y, z, w);
+  }
+  __device__ void store(int z, u32 e, const In& in) const {
+    if (y >= a->N) return;
+    const u32 ug = expand2(in.ug);
+    a->outRows[((size_t)z * a->N + y) * a->W + w] = (ug & ~e) | (~ug & in.c);
+  }
+  __device__ void skipped(int) const {}
+};
+
+constexpr int kMaxWaves = 8;
+
+// Pass bodies. PASS 0: coarse dilation -> DF flags (lanes own the even rows,
+// even planes); 1: fine dilation -> D rows; 2: coarse erosion -> CL flags
+// (odd rows, odd planes); 3: fine erosion -> C rows. RHO > 0: uniform radius
+// with the compiled chord table of that radius; RHO = 0: runtime tables
+// (uniform or per-voxel radii, sphere or cube).
+template <int PASS, int RHO>
+__device__ __forceinline__ void run_pass(const MorphArgs& a) {
+  extern __shared__ u32 smem[];
+  __shared__ u32 set4[4];
+  if constexpr (PASS == 0) {
+    TileGeo g = tile_geo(a, 0, 0, 2);
+    Lds L = carve(smem, a, g.nout());
+    const int y1 = g.yfirst + a.ystep * kTileRows;
+    // F = 0 when no occupied voxel is in reach; F unused when every cell is full
+    const bool empty = !bricks_any(a.brickI, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff, g.zfirst - a.maxoff,
+                                   g.zlast + 2 + a.maxoff, g.wx - 1, g.wx + 1, 0);
+    const bool full = !empty && bricks_all(a.brickI, a.N, a.W, g.yfirst, y1, g.zfirst, g.zlast + 2, g.wx);
+    DilCoarseFin fin{&a, g.wx, g.yfirst + 2 * lane_id(), cell_range_mask(g.wx, 0, a.dR0),
+                     (u16)(full ? 0xffffu : 0u)};
+    run_tile<0, 2, RHO>(a, L, set4, g, a.I, empty || full, fin);
+  } else if constexpr (PASS == 1) {
+    TileGeo g = tile_geo(a, 0, 0, 1);
+    Lds L = carve(smem, a, g.nout());
+    const bool skip =
+        !tile_has_flags(a.TG, a.R, a.W, g.wx, g.yfirst, g.yfirst + kTileRows, g.zfirst, g.zlast + 1);
+    DilFineFin fin{&a, g.wx, g.yfirst + lane_id()};
+    run_tile<0, 1, RHO>(a, L, set4, g, a.I, skip, fin);
+  } else if constexpr (PASS == 2) {
+    TileGeo g = tile_geo(a, 1, 1, 2);
+    Lds L = carve(smem, a, g.nout());
+    const int y1 = g.yfirst + a.ystep * kTileRows;
+    const bool skip = !bricks_any(a.brickK, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff, g.zfirst - a.maxoff,
+                                  g.zlast + 1 + a.maxoff, g.wx - 1, g.wx + 1, 0);
+    EroCoarseFin fin{&a, g.wx, g.yfirst + 2 * lane_id(), cell_range_mask(g.wx, 0, a.dR0)};
+    run_tile<1, 2, RHO>(a, L, set4, g, a.K, skip, fin);
+  } else {
+    TileGeo g = tile_geo(a, 0, 0, 1);
+    Lds L = carve(smem, a, g.nout());
+    const bool skip =
+        !tile_has_flags(a.UG, a.R, a.W, g.wx, g.yfirst, g.yfirst + kTileRows, g.zfirst, g.zlast + 1);
+    EroFineFin fin{&a, g.wx, g.yfirst + lane_id()};
+    run_tile<2, 1, RHO>(a, L, set4, g, a.K, skip, fin);
+  }
+}
+
+}  // namespace
+}  // namespace gm

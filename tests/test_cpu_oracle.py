@@ -398,6 +398,15 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(B.meshgen_lib(), n)
 
 
+def test_compiled_chord_tables_equal_runtime_tables():
+    """The streaming kernels' compile-time chord tables (one per fixed radius)
+    list exactly the chords the host table builder emits (host-only check)."""
+    import ctypes
+    f = getattr(B.hip_lib(), "_ZN2gm12ct_tables_okEv")
+    f.restype = ctypes.c_bool
+    assert f()
+
+
 def test_context_without_device_fails_loudly():
     if os.environ.get("HIP_VISIBLE_DEVICES") is None and os.path.exists("/dev/kfd"):
         pytest.skip("a HIP device may be present")

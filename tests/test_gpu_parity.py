@@ -59,6 +59,27 @@ def test_dilate_erode_uniform(ctx, r):
     assert np.array_equal(C, Cref), "erosion differs in %d cells" % int((C != Cref).sum())
 
 
+@pytest.mark.parametrize("r", [1, 11, 15, 16, 17, 23, 31])
+def test_dilate_erode_radius_range(ctx, r):
+    """Radii across the supported range (both the compiled chord tables and
+    the runtime-table path), a grid whose side is not a power of two
+    (R = 48: 3 row words), a few isolated seeds and a clipped data region."""
+    R = 48
+    I = random_sparse(R, seed=100 + r, density=0.00003)
+    dr = np.array([R - 2, R, R - 7], np.int32)
+    grid_setup(ctx, R, I, dr)
+    ctx.scale_uniform(r)
+    say("dilate/erode radius range r=%d R=%d" % (r, R))
+    ctx.dilate(B.SE_SPHERE)
+    Dref = O.dilate(I, R, r=r, data_res=dr)
+    D = ctx.get_cells(B.GRID_DILATION)
+    assert np.array_equal(D, Dref), "dilation differs in %d cells" % int((D != Dref).sum())
+    ctx.erode()
+    C = ctx.get_cells(B.GRID_CLOSING)
+    Cref = O.erode(I, Dref, R, r=r, data_res=dr)
+    assert np.array_equal(C, Cref), "erosion differs in %d cells" % int((C != Cref).sum())
+
+
 @pytest.mark.parametrize("r", [2, 4, 7])
 def test_sparse_input(ctx, r):
     R = 32

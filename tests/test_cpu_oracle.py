@@ -11,7 +11,7 @@ import _oracle as O
 import broxy_amd as B
 from _util import cells_from_fine, fine_grid, random_blobs, random_sparse, set_fine
 
-REF = "/root/reference"
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def morton(x, y, z):
@@ -186,20 +186,14 @@ def test_mc_tables():
         assert packed == listed and ln[c] == len(listed) // 3 and ntri[c] == n[c]
 
 
-@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "GMorpho/tables.h")), reason="reference tree absent")
-def test_mc_tables_equal_reference_text():
-    src = open(os.path.join(REF, "GMorpho/tables.h")).read()
-
-    def arr(name):
-        i = src.index(name)
-        j = src.index("{", i)
-        k = src.index("}", j)
-        return [int(x, 0) for x in re.findall(r"-?0x[0-9a-fA-F]+|-?\d+", src[j + 1:k])]
-
+def test_mc_tables_equal_reference_fixture():
+    """Case tables equal the reference's tables.h data (fixture written by
+    tests/golden/make_golden.py)."""
+    g = np.load(os.path.join(GOLDEN, "mc_tables.npz"))
     e, n, t = O.mc_tables()
-    assert arr("edgeTable") == [int(x) for x in e]
-    assert arr("numTriTable") == [int(x) for x in n]
-    assert arr("triTable") == [int(x) for x in t.reshape(-1)]
+    assert np.array_equal(g["edgeTable"], np.asarray(e, np.uint16))
+    assert np.array_equal(g["numTriTable"], np.asarray(n, np.uint8))
+    assert np.array_equal(g["triTable"].reshape(-1), np.asarray(t, np.int8).reshape(-1))
 
 
 def _mesh_checks(V, T):
@@ -363,16 +357,35 @@ def test_pyramid_levels():
 
 
 # ---------------------------------------------------------------- fixtures / generators
-@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "Bin/Resources/Models/sphere.off")),
-                    reason="reference tree absent")
 def test_icosphere_matches_reference_fixture():
-    lines = open(os.path.join(REF, "Bin/Resources/Models/sphere.off")).read().split("\n")
-    nv, nf = map(int, lines[1].split()[:2])
-    Q = np.array([[float(x) for x in l.split()] for l in lines[2:2 + nv]])
+    """The generator's subdivision-3 icosphere is the reference's sphere.off
+    (same counts, same vertex set to float precision)."""
+    g = np.load(os.path.join(GOLDEN, "sphere_off.npz"))
+    Q, F = g["V"], g["F"]
     P, T = B.icosphere(3)
-    assert (len(P), len(T)) == (nv, nf)
+    assert (len(P), len(T)) == (len(Q), len(F))
     d = np.sqrt(((P[:, None, :] - Q[None, :, :]) ** 2).sum(-1)).min(1)
     assert d.max() < 2e-6
+
+
+def test_oracle_matches_golden_config0():
+    """The oracle's config-0 closing and mesh still hash to the committed
+    record (regression pin of the oracle; written by make_golden.py)."""
+    import hashlib
+    import json
+    g = json.load(open(os.path.join(GOLDEN, "oracle_config0.json")))
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+    P, T = B.sphere_soup(64, 3, seed=2017)
+    R = 64
+    margin = np.float32(0.3) + np.float32(2.0) / np.float32(2 * R)
+    bmin, bmax, cell, dr = O.grid_attributes(P, R, float(margin))
+    assert [float(x) for x in bmin] == g["bmin"] and [int(x) for x in dr] == g["data_res"]
+    I = O.voxelize(P, T, bmin, float((bmax - bmin).max()), R)
+    D = O.dilate(I, R, r=4, data_res=dr)
+    C = O.erode(I, D, R, r=4, data_res=dr)
+    V, Nn, Tt = O.mc(C, R, bmin, cell)
+    assert (sha(I), sha(D), sha(C)) == (g["I"], g["D"], g["C"])
+    assert (len(V), len(Tt), sha(Tt), sha(V)) == (g["nV"], g["nT"], g["T"], g["V"])
 
 
 def test_generators_deterministic_and_closed():

@@ -191,6 +191,30 @@ def test_config0_end_to_end(ctx):
     assert len(Tt) > 1000
 
 
+def test_config0_golden(ctx):
+    """The GPU path reproduces the committed oracle record of the config-0
+    workload (seed 2017) without running the oracle."""
+    import hashlib
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_config0.json")))
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+    P, T = B.sphere_soup(64, 3, seed=2017)
+    say("config-0 golden record (seed 2017)")
+    ctx.load(P, T, 64, 0.3)
+    bmin, bmax, res, dr, cell = ctx.info()
+    assert [float(x) for x in bmin] == g["bmin"] and [int(x) for x in dr] == g["data_res"]
+    ctx.scale_uniform(4)
+    ctx.close_grid(B.SE_SPHERE)
+    ctx.extract_mesh()
+    assert sha(ctx.get_cells(B.GRID_INPUT)) == g["I"]
+    assert sha(ctx.get_cells(B.GRID_DILATION)) == g["D"]
+    assert sha(ctx.get_cells(B.GRID_CLOSING)) == g["C"]
+    V, Nn, Tt = ctx.mesh()
+    assert (len(V), len(Tt)) == (g["nV"], g["nT"])
+    assert sha(np.ascontiguousarray(Tt, np.uint32)) == g["T"]
+
+
 def test_pyramid(ctx):
     R = 32
     I = random_blobs(R, seed=8)

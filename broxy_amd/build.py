@@ -70,6 +70,19 @@ def build_hip(jobs=None):
     return out
 
 
+def build_driver():
+    """The C++ facade driver (tests/cpp/facade_closing.cpp), linked against the
+    two libraries; run by the GPU tests."""
+    bindir = os.path.join(LIBDIR, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    out = os.path.join(bindir, "facade_closing")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+          os.path.join(ROOT, "tests", "cpp", "facade_closing.cpp"), "-o", out,
+          "-L", LIBDIR, "-lgmorpho_hip", "-lgmorpho_meshgen", "-L/opt/rocm/lib",
+          "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib"])
+    return out
+
+
 def build_oracle():
     _run(["make", "-C", os.path.join(ROOT, "oracle")])
 
@@ -77,10 +90,12 @@ def build_oracle():
 def build_all():
     build_meshgen()
     build_hip()
+    build_driver()
     build_oracle()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "hip": build_hip, "meshgen": build_meshgen, "oracle": build_oracle}[w]()
+        {"all": build_all, "hip": build_hip, "meshgen": build_meshgen, "driver": build_driver,
+         "oracle": build_oracle}[w]()

@@ -420,6 +420,19 @@ def test_compiled_chord_tables_equal_runtime_tables():
     assert f()
 
 
+def test_facade_driver_fails_loudly_without_device():
+    """The C++ facade (include/gmorpho.hpp) compiles and links against the
+    library; without a device its GMorpho constructor throws the reference's
+    exception type, which the driver reports."""
+    import subprocess
+    exe = os.path.join(B.LIBDIR, "bin", "facade_closing")
+    assert os.path.exists(exe), "run broxy_amd/build.py driver"
+    if os.environ.get("HIP_VISIBLE_DEVICES") is None and os.path.exists("/dev/kfd"):
+        pytest.skip("a HIP device may be present")
+    p = subprocess.run([exe, "/tmp/gm_facade_nodev"], capture_output=True, text=True, timeout=120)
+    assert p.returncode != 0 and "GMorpho Error:" in p.stderr
+
+
 def test_context_without_device_fails_loudly():
     if os.environ.get("HIP_VISIBLE_DEVICES") is None and os.path.exists("/dev/kfd"):
         pytest.skip("a HIP device may be present")

@@ -215,6 +215,26 @@ def test_config0_golden(ctx):
     assert sha(np.ascontiguousarray(Tt, np.uint32)) == g["T"]
 
 
+def test_facade_driver_matches_golden(tmp_path):
+    """The C++ facade, used as the reference's viewer uses GMorpho (Load,
+    ScaleField global scale, Update -> Mesh), reproduces the oracle's record."""
+    import hashlib
+    import json
+    import os
+    import subprocess
+    g = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_config0.json")))
+    exe = os.path.join(B.LIBDIR, "bin", "facade_closing")
+    say("C++ facade driver, config-0 seed 2017")
+    pre = str(tmp_path / "facade")
+    p = subprocess.run([exe, pre, "2017"], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    nv, nt = map(int, p.stdout.split())
+    assert (nv, nt) == (g["nV"], g["nT"])
+    h = lambda f: hashlib.sha256(open(f, "rb").read()).hexdigest()
+    assert h(pre + "_closing.u8") == g["C"]
+    assert h(pre + "_tris.u32") == g["T"]
+
+
 def test_pyramid(ctx):
     R = 32
     I = random_blobs(R, seed=8)

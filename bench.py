@@ -24,31 +24,42 @@ sys.path.insert(0, ROOT)
 import broxy_amd as B  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-STAGES = ["dilate_coarse", "tight_contour", "dilate_fine", "contour_K", "erode_coarse", "base_contour",
-          "erode_fine", "mc_count_scan", "mc_verts_tris", "mc_normals"]
-KERNEL_OF_STAGE = {"dilate_coarse": "k_dil_coarse", "tight_contour": "k_tight", "dilate_fine": "k_dil_fine",
-                   "contour_K": "k_contour_K", "erode_coarse": "k_ero_coarse", "base_contour": "k_base_contour",
-                   "erode_fine": "k_ero_fine", "mc_count_scan": "k_mc_count+k_mc_scan",
-                   "mc_verts_tris": "k_mc_verts+k_mc_tris", "mc_normals": "k_mc_normals+fixup+normalize"}
+STAGES = list(B.Context.STAGES)
+# kernel (name in a rocprof trace) timed by each stage
+KERNEL_OF_STAGE = {"brick_summary": "k_brick_summary", "dilate_coarse": "k_pass_ct<0,rho>",
+                   "tight_contour": "k_tight", "dilate_fine": "k_pass_ct<1,rho>", "contour_K": "k_contour_K",
+                   "erode_coarse": "k_pass_ct<2,rho>", "base_contour": "k_base_contour",
+                   "erode_fine": "k_pass_ct<3,rho>", "mc_count": "k_mc_count",
+                   "mc_scan": "k_scan_reduce+k_scan_top+k_scan_down", "mc_verts": "k_mc_verts",
+                   "mc_tris": "k_mc_tris", "mc_fixup_normalize": "k_mc_fixup+k_mc_normalize"}
 
 
 def algorithmic_bytes(stage, N, nv, nt):
-    """Compulsory HBM bytes of one launch of a stage (bit grids N^3/8 bytes,
-    cell flags N^3/64 bytes, mesh 12 bytes per vertex / normal / triangle)."""
+    """Compulsory HBM bytes of one launch of a stage's kernel: bit grids N^3/8
+    bytes, cell flags N^3/64 bytes, mesh 12 bytes per vertex position, normal
+    and triangle."""
     g = N ** 3 / 8.0
     f = N ** 3 / 64.0
     return {
+        "brick_summary": g,                 # read I
         "dilate_coarse": g + f,            # read I, write DF
-        "tight_contour": f + f,            # read DF, write TG
-        "dilate_fine": g + 2 * f + g,      # read I, DF, TG, write D
-        "contour_K": g + g,                # read D, write K
-        "erode_coarse": 3 * g + f,         # read K, D, I, write CL
-        "base_contour": g + f + f,         # read D, CL, write UG
-        "erode_fine": 2 * g + 2 * f + g,   # read K, D, CL, UG, write C
-        "mc_count_scan": g,                # read C
-        "mc_verts_tris": g + 12 * nv + 12 * nt,
-        "mc_normals": g + 12 * nv,
+        "tight_contour": 2 * f + 2 * g,    # read DF, I; write TG, D
+        "dilate_fine": 2 * g + f,          # read I, TG; rewrite D (upper bound: every tile tagged)
+        "contour_K": 2 * g,                # read D, write K
+        "erode_coarse": 3 * g + f,         # read K, D, I; write CL
+        "base_contour": 2 * g + 2 * f,     # read D, CL; write UG, C
+        "erode_fine": 3 * g + f,           # read K, UG, C; rewrite C (upper bound)
+        "mc_count": g,                     # read C
+        "mc_scan": 16.0 * N ** 3 / 4096,   # block counts and bases (2 u32 per 8^3 cells, read + write)
+        "mc_verts": g + 24 * nv,           # read C; write positions, normal sums
+        "mc_tris": g + 12 * nt,            # read C; write triangles
+        "mc_fixup_normalize": 24 * nv,     # normals read + write
     }[stage]
+
+
+def rho_of_stage(stage, r):
+    """Chord-table radius of a streaming pass (dilation r+1, erosion r-1)."""
+    return r + 1 if stage.startswith("dilate") else r - 1
 
 
 def build_workload(args):
@@ -141,12 +152,16 @@ def main():
     whole_bytes = 5 * N ** 3 / 8.0 + 12 * nv + 12 * nt
     whole_gbs = whole_bytes / (ms * 1e-3) / 1e9
 
+    # HBM bytes per launch of that kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # passes (scaled by a copy of known size), committed beside the profiles
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.config)
     if os.path.exists(tf):
         with open(tf) as fh:
             tr = json.load(fh)
-        traffic = tr.get(KERNEL_OF_STAGE[dname])
+        ent = tr.get("stages", {}).get(dname)
+        if ent:
+            traffic = ent.get("bytes")
 
     out = {
         "metric": "Gvoxels/s closing 512^3 r=8 (+%HBM roof)" if args.config == "large" else "Gvoxels/s closing",
@@ -163,7 +178,8 @@ def main():
         "data": "synthetic (seeded mesh generator)",
         "config": {"workload": desc, "N": N, "radius": r, "se": "sphere", "mesh_vertices": nv,
                    "mesh_triangles": nt, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": KERNEL_OF_STAGE[dname], "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": KERNEL_OF_STAGE[dname].replace("rho", str(rho_of_stage(dname, r))),
+                     "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes": int(dbytes),
                      "avg_launch_ms": round(float(stage_ms[dom]), 5)},

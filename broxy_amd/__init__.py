@@ -218,9 +218,14 @@ class Context:
     def set_timing(self, on=True):
         self._call("gm_set_timing", 1 if on else 0)
 
+    STAGES = ("brick_summary", "dilate_coarse", "tight_contour", "dilate_fine", "contour_K", "erode_coarse",
+              "base_contour", "erode_fine", "mc_count", "mc_scan", "mc_verts", "mc_tris", "mc_fixup_normalize")
+
     def stage_times(self):
-        ms = np.zeros(10, np.float32)
-        self._call("gm_stage_times", _ptr(ms), 10)
+        """Device ms per stage (one kernel each, see STAGES) of the last
+        close_grid / extract_mesh with timing on."""
+        ms = np.zeros(len(self.STAGES), np.float32)
+        self._call("gm_stage_times", _ptr(ms), len(self.STAGES))
         return ms
 
     def sync(self):

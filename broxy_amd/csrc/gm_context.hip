@@ -316,7 +316,7 @@ int do_erode(gm_ctx* c) {
   rc = ensure_tables(c, std::max(c->rlo, 0), c->rhi);
   if (rc) return rc;
   hipError_t e = launch_erode(morph_dims(c), morph_buffers(c), radius_field(c), c->tabs, c->rlo, c->rhi,
-                              c->tabWords, c->stream, c->timing ? &c->ev[4] : nullptr);
+                              c->tabWords, c->stream, c->timing ? &c->ev[5] : nullptr);
   if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("erosion: ") + hipGetErrorString(e));
   c->meshPending = false;
   return GM_OK;
@@ -620,7 +620,7 @@ int gm_extract_mesh(gm_ctx* c) {
   int rc = need_grid(c);
   if (rc) return rc;
   hipSetDevice(c->device);
-  hipError_t e = launch_mc(c->C, c->R, c->bmin, c->cell, c->mc, c->stream, c->timing ? &c->ev[9] : nullptr);
+  hipError_t e = launch_mc(c->C, c->R, c->bmin, c->cell, c->mc, c->stream, c->timing ? &c->ev[10] : nullptr);
   if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("mesh extraction: ") + hipGetErrorString(e));
   HIPCALL(c, hipMemcpyAsync(c->hTotals, c->mc.totals, 16, hipMemcpyDeviceToHost, c->stream));
   c->meshPending = true;
@@ -709,14 +709,16 @@ int gm_set_timing(gm_ctx* c, int enable) {
   return GM_OK;
 }
 
+constexpr int kNumStages = 13;
 int gm_stage_times(gm_ctx* c, float* ms, int n) {
   if (!c || !ms) return GM_ERR_ARG;
   HIPCALL(c, hipStreamSynchronize(c->stream));
-  // events: dilate 0..3, erode 4..8, mc 9..12
-  const int pairs[10][2] = {{0, 1}, {1, 2}, {2, 3}, {4, 5}, {5, 6}, {6, 7}, {7, 8}, {9, 10}, {10, 11}, {11, 12}};
-  for (int i = 0; i < n && i < 10; ++i) {
+  // events: dilate 0..4, erode 5..9, mc 10..15 (one stage per kernel or scan)
+  const int pairs[kNumStages][2] = {{0, 1}, {1, 2}, {2, 3},   {3, 4},   {5, 6},   {6, 7},  {7, 8},
+                                    {8, 9}, {10, 11}, {11, 12}, {12, 13}, {13, 14}, {14, 15}};
+  for (int i = 0; i < n && i < kNumStages; ++i) {
     ms[i] = -1.f;
-    bool ok = (i < 7) ? (c->evValid & 1) : (c->evValid & 2);
+    bool ok = (i < 8) ? (c->evValid & 1) : (c->evValid & 2);
     if (!ok) continue;
     float t = 0.f;
     if (hipEventElapsedTime(&t, c->ev[pairs[i][0]], c->ev[pairs[i][1]]) == hipSuccess) ms[i] = t;

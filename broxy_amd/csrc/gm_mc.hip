@@ -738,6 +738,7 @@ hipError_t launch_mc(const u32* C, int R, const float* bmin3, float cell, const 
   if (ev) GM_CHECK(hipEventRecord(ev[0], s));
   hipLaunchKernelGGL(k_mc_count, dim3(m.nblocks), dim3(kMcThreads), 0, s, C, R, m.blockV, m.blockT);
   GM_CHECK(hipGetLastError());
+  if (ev) GM_CHECK(hipEventRecord(ev[1], s));
   int nparts = (int)mc_scan_parts(m.nblocks);
   hipLaunchKernelGGL(k_scan_reduce, dim3(nparts), dim3(256), 0, s, m.blockV, m.blockT, m.nblocks, m.scanPart);
   GM_CHECK(hipGetLastError());
@@ -745,8 +746,8 @@ hipError_t launch_mc(const u32* C, int R, const float* bmin3, float cell, const 
   GM_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_scan_down, dim3(nparts), dim3(256), 0, s, m.blockV, m.blockT, m.nblocks, m.scanPart);
   GM_CHECK(hipGetLastError());
-  if (ev) GM_CHECK(hipEventRecord(ev[1], s));
-  return launch_mc_emit(C, R, bmin3, cell, m, s, ev ? ev + 1 : nullptr);
+  if (ev) GM_CHECK(hipEventRecord(ev[2], s));
+  return launch_mc_emit(C, R, bmin3, cell, m, s, ev ? ev + 2 : nullptr);
 }
 
 hipError_t launch_mc_emit(const u32* C, int R, const float* bmin3, float cell, const MCBuffers& m, hipStream_t s,
@@ -757,16 +758,17 @@ hipError_t launch_mc_emit(const u32* C, int R, const float* bmin3, float cell, c
   hipLaunchKernelGGL(k_mc_verts, dim3(m.nblocks), dim3(kMcThreads), 0, s, C, R, geo, m.blockV, m.cellV, m.V, m.Nrm,
                      m.capV, m.totals);
   GM_CHECK(hipGetLastError());
+  if (ev) GM_CHECK(hipEventRecord(ev[1], s));
   hipLaunchKernelGGL(k_mc_tris, dim3(m.nblocks), dim3(kMcThreads), 0, s, C, R, m.blockV, m.blockT, m.cellV, m.T,
                      m.capT, m.late, m.capLate, m.totals);
   GM_CHECK(hipGetLastError());
-  if (ev) GM_CHECK(hipEventRecord(ev[1], s));
+  if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   hipLaunchKernelGGL(k_mc_fixup, dim3(1), dim3(64), 0, s, m.V, m.Nrm, m.T, m.late, m.capLate, m.totals, m.capV,
                      m.capT);
   GM_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_mc_normalize, dim3(1024), dim3(256), 0, s, m.Nrm, m.totals, m.capV);
   GM_CHECK(hipGetLastError());
-  if (ev) GM_CHECK(hipEventRecord(ev[2], s));
+  if (ev) GM_CHECK(hipEventRecord(ev[3], s));
   return hipSuccess;
 }
 

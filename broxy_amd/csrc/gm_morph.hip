@@ -296,21 +296,22 @@ hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const Radius
   if (ev) GM_CHECK(hipEventRecord(ev[0], s));
   hipLaunchKernelGGL(k_brick_summary, dim3(d.N / 8, d.N / 8), dim3(8 * d.W), 0, s, b.I, d.N, d.W, b.brickI);
   GM_CHECK(hipGetLastError());
+  if (ev) GM_CHECK(hipEventRecord(ev[1], s));
   const bool fixedOk = !var && se == 0;
   const int rho = (rf.r & 0x7f) + 1;
   a.ystep = 2;
   a.outFlags = b.DF;
   GM_CHECK(launch_pass(0, reinterpret_cast<const void*>(&k_dil_coarse), a, var, fixedOk, rho, s));
-  if (ev) GM_CHECK(hipEventRecord(ev[1], s));
+  if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   a.outFlags = b.TG;
   a.outRows = b.D;
   hipLaunchKernelGGL(k_tight, dim3((unsigned)((cellwords + 255) / 256)), dim3(256), 0, s, a);
   GM_CHECK(hipGetLastError());
-  if (ev) GM_CHECK(hipEventRecord(ev[2], s));
+  if (ev) GM_CHECK(hipEventRecord(ev[3], s));
   a.ystep = 1;
   a.outRows = b.D;
   GM_CHECK(launch_pass(1, reinterpret_cast<const void*>(&k_dil_fine), a, var, fixedOk, rho, s));
-  if (ev) GM_CHECK(hipEventRecord(ev[3], s));
+  if (ev) GM_CHECK(hipEventRecord(ev[4], s));
   return hipSuccess;
 }
 

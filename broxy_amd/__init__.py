@@ -238,3 +238,90 @@ def mc_tables():
     tri = np.zeros(256, np.uint64)
     hip_lib().gm_mc_tables(_ptr(ntri), _ptr(ln), _ptr(tri))
     return ntri, ln, tri
+
+
+class _Job(ctypes.Structure):
+    _fields_ = [("P", _P), ("num_of_vertices", ctypes.c_int), ("T", _P), ("num_of_faces", ctypes.c_int),
+                ("base_res", ctypes.c_int), ("se_size", ctypes.c_float), ("radius", ctypes.c_int),
+                ("lattice", _P), ("lattice_res", ctypes.c_int), ("se", ctypes.c_int)]
+
+
+class Engine:
+    """gm_engine: a pool of `slots` contexts running closing jobs concurrently
+    on one device under an HBM budget (batched and served use)."""
+
+    def __init__(self, device=0, slots=4, hbm_budget=200 << 30):
+        self._lib = hip_lib()
+        self._lib.gm_engine_last_error.restype = ctypes.c_char_p
+        self._lib.gm_engine_last_error.argtypes = [_P]
+        h = _P()
+        rc = self._lib.gm_engine_create(device, slots, ctypes.c_uint64(hbm_budget), ctypes.byref(h))
+        if rc:
+            raise GMorphoError("gm_engine_create failed (%d): no usable HIP device" % rc)
+        self._h = h
+
+    def close(self):
+        if self._h:
+            self._lib.gm_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, name, *args):
+        rc = getattr(self._lib, name)(self._h, *args)
+        if rc:
+            msg = self._lib.gm_engine_last_error(self._h)
+            raise GMorphoError("%s -> %d: %s" % (name, rc, msg.decode() if msg else ""))
+        return rc
+
+    def submit(self, P, T, base_res, se_size=0.3, radius=0, lattice=None, se=SE_SPHERE):
+        """Queue one closing job (voxelize at base_res, close, mesh); returns its id."""
+        P = np.ascontiguousarray(P, np.float32)
+        T = np.ascontiguousarray(T, np.uint32)
+        L = 0
+        lat = None
+        if lattice is not None:
+            lat = np.ascontiguousarray(lattice, np.float32)
+            L = int(round(lat.size ** (1.0 / 3.0)))
+        job = _Job(P.ctypes.data, len(P), T.ctypes.data, len(T), base_res, se_size, radius,
+                   None if lat is None else lat.ctypes.data, L, se)
+        jid = ctypes.c_uint64()
+        self._call("gm_engine_submit", ctypes.byref(job), ctypes.byref(jid))
+        return jid.value
+
+    def wait(self, jid):
+        nv = ctypes.c_uint32()
+        nt = ctypes.c_uint32()
+        self._call("gm_engine_wait", ctypes.c_uint64(jid), ctypes.byref(nv), ctypes.byref(nt))
+        return nv.value, nt.value
+
+    def times(self, jid):
+        """(submit, start, done) seconds on the engine clock."""
+        t = np.zeros(3, np.float64)
+        self._call("gm_engine_job_times", ctypes.c_uint64(jid), _ptr(t))
+        return t
+
+    def now(self):
+        t = ctypes.c_double()
+        self._call("gm_engine_now", ctypes.byref(t))
+        return t.value
+
+    def take(self, jid):
+        """Mesh (V, N, T) of a finished job; releases it."""
+        nv, nt = self.wait(jid)
+        V = np.zeros((nv, 3), np.float32)
+        N = np.zeros((nv, 3), np.float32)
+        T = np.zeros((nt, 3), np.uint32)
+        self._call("gm_engine_take", ctypes.c_uint64(jid), _ptr(V), _ptr(N), _ptr(T))
+        return V, N, T
+
+    def stats(self):
+        b = ctypes.c_uint64()
+        r = ctypes.c_int()
+        q = ctypes.c_int()
+        self._call("gm_engine_stats", ctypes.byref(b), ctypes.byref(r), ctypes.byref(q))
+        return b.value, r.value, q.value

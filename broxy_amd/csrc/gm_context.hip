@@ -24,6 +24,7 @@ struct gm_ctx {
   // grid
   bool has_grid = false;
   int R = 0, N = 0, W = 0;
+  int capR = 0;  // grid buffers are sized for capR; smaller grids reuse them
   float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0}, cell = 0.f;
   int data_res[3] = {0, 0, 0};
   u32 *I = nullptr, *D = nullptr, *K = nullptr, *C = nullptr;
@@ -40,6 +41,7 @@ struct gm_ctx {
   // radius field
   int rmode = 0, r = 0, rlo = 0, rhi = 0, L = 0;
   int8_t* dGrid = nullptr;
+  size_t capGrid = 0;  // bytes of dGrid
   int32_t* dLat = nullptr;
   // structuring-element tables
   u32* dTables = nullptr;
@@ -47,6 +49,10 @@ struct gm_ctx {
   SETableSet tabs;
   int tabLo = -1, tabHi = -1;
   int tabWords = 0;  // longest table of the current set
+  u32* dCube = nullptr;  // cube dilation tables
+  size_t cubeCap = 0;
+  SETableSet cubeTabs;
+  int cubeLo = -1, cubeHi = -1, cubeWords = 0;
   // marching cubes
   MCBuffers mc{};
   u32* hTotals = nullptr;
@@ -54,6 +60,7 @@ struct gm_ctx {
   int Pmc = 0;
   // pyramid
   u64* dPyr = nullptr;
+  uint64_t capPyr = 0;  // texels of dPyr
   int pyrLevels = 0;
   uint64_t pyrTexels = 0;
   // timing
@@ -88,8 +95,11 @@ void free_grid(gm_ctx* c) {
   dfree(c->mc.blockV); dfree(c->mc.blockT); dfree(c->mc.cellV); dfree(c->mc.cellT);
   dfree(c->mc.V); dfree(c->mc.Nrm); dfree(c->mc.T); dfree(c->mc.late); dfree(c->mc.totals); dfree(c->mc.scanPart);
   dfree(c->dPyr);
+  c->capPyr = 0;
   dfree(c->dGrid);
+  c->capGrid = 0;
   c->has_grid = false;
+  c->capR = 0;
   c->meshPending = false;
   c->pyrLevels = 0;
 }
@@ -115,10 +125,35 @@ int alloc_mesh_out(gm_ctx* c, u32 capV, u32 capT, u32 capLate) {
   return GM_OK;
 }
 
+// Grid dimensions of base resolution R inside buffers sized for c->capR.
+void set_dims(gm_ctx* c, int R) {
+  if (c->R != R && c->rmode != 0) {
+    // a per-voxel radius field belongs to the previous grid size
+    c->rmode = 0;
+    c->r = 0;
+    c->rlo = c->rhi = 0;
+  }
+  c->R = R;
+  c->N = 2 * R;
+  c->W = c->N / 32;
+  int P = 1;
+  while (P < R) P <<= 1;
+  c->Pmc = P;
+  c->mc.P = P;
+  c->mc.nblocks = (P / 8) * (P / 8) * (P / 8);
+}
+
+// Buffers for base resolution R. A context keeps the largest buffers it has
+// allocated and reuses them for smaller grids, so a sequence of jobs of mixed
+// sizes allocates (and frees, which waits for the device) only when a new
+// largest size comes.
 int alloc_grid(gm_ctx* c, int R) {
   if (R < 16 || R > 1024 || (R % 16) != 0)
     return fail(c, GM_ERR_UNSUPPORTED, "base resolution must be a multiple of 16 in [16,1024]");
-  if (c->has_grid && c->R == R) return GM_OK;
+  if (c->has_grid && R <= c->capR) {
+    set_dims(c, R);
+    return GM_OK;
+  }
   free_grid(c);
   c->R = R;
   c->N = 2 * R;
@@ -158,6 +193,7 @@ int alloc_grid(gm_ctx* c, int R) {
                           (u32)std::min<size_t>((size_t)8 * c->N * c->N, 1u << 30), 4096);
   if (rc) return rc;
   c->has_grid = true;
+  c->capR = R;
   // default radius: uniform 0 (closing = input) until a ScaleField is set
   if (c->rmode != 0) {
     c->rmode = 0;
@@ -235,10 +271,13 @@ int ensure_tables(gm_ctx* c, int rlo, int rhi) {
   return GM_OK;
 }
 
-// Cube dilation tables live in a second table set (dil[] indexed by r).
-int cube_tables(gm_ctx* c, int rlo, int rhi, SETableSet& t, u32*& dtab, int& longest) {
+// Cube dilation tables live in a second table set (dil[] indexed by r),
+// kept by the context like the sphere tables.
+int cube_tables(gm_ctx* c, int rlo, int rhi) {
+  if (c->cubeLo == rlo && c->cubeHi == rhi && c->dCube) return GM_OK;
   std::vector<u32> all;
-  longest = 0;
+  SETableSet& t = c->cubeTabs;
+  int longest = 0;
   for (int i = 0; i <= 128; ++i) t.dil[i] = t.ero_corner[i] = t.ero_centre[i] = -1;
   for (int r = rlo; r <= rhi; ++r) {
     int off = (int)all.size();
@@ -247,8 +286,18 @@ int cube_tables(gm_ctx* c, int rlo, int rhi, SETableSet& t, u32*& dtab, int& lon
     longest = std::max(longest, (int)v.size());
     t.dil[r] = off;
   }
-  HIPCALL(c, hipMalloc(&dtab, all.size() * 4));
-  HIPCALL(c, hipMemcpy(dtab, all.data(), all.size() * 4, hipMemcpyHostToDevice));
+  const size_t bytes = all.size() * 4;
+  if (bytes > c->cubeCap) {
+    dfree(c->dCube);
+    c->cubeCap = 0;
+    HIPCALL(c, hipMalloc(&c->dCube, bytes));
+    c->cubeCap = bytes;
+  }
+  HIPCALL(c, hipMemcpyAsync(c->dCube, all.data(), bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  c->cubeLo = rlo;
+  c->cubeHi = rhi;
+  c->cubeWords = longest;
   return GM_OK;
 }
 
@@ -287,17 +336,12 @@ int do_dilate(gm_ctx* c, int se) {
   if (rc) return rc;
   int lo = std::max(c->rlo, 0), hi = c->rhi;
   if (se == GM_SE_CUBE) {
-    SETableSet t;
-    u32* dtab = nullptr;
-    int longest = 0;
-    rc = cube_tables(c, lo, hi, t, dtab, longest);
+    rc = cube_tables(c, lo, hi);
     if (rc) return rc;
     MorphBuffers b = morph_buffers(c);
-    b.tables = dtab;
-    hipError_t e = launch_dilate(morph_dims(c), b, radius_field(c), 1, t, lo, hi, longest, c->stream,
+    b.tables = c->dCube;
+    hipError_t e = launch_dilate(morph_dims(c), b, radius_field(c), 1, c->cubeTabs, lo, hi, c->cubeWords, c->stream,
                                  c->timing ? &c->ev[0] : nullptr);
-    hipStreamSynchronize(c->stream);
-    hipFree(dtab);
     if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("dilation: ") + hipGetErrorString(e));
   } else {
     rc = ensure_tables(c, lo, hi);
@@ -354,6 +398,7 @@ int gm_destroy(gm_ctx* c) {
   dfree(c->dSnap);
   dfree(c->dLat);
   dfree(c->dTables);
+  dfree(c->dCube);
   for (auto& e : c->ev) hipEventDestroy(e);
   if (c->hTotals) hipHostFree(c->hTotals);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -522,7 +567,12 @@ int gm_scale_grid_i8(gm_ctx* c, const int8_t* grid) {
   if (!grid) return fail(c, GM_ERR_ARG, "null scale grid");
   hipSetDevice(c->device);
   size_t n = (size_t)c->N * c->N * c->N;
-  if (!c->dGrid) HIPCALL(c, hipMalloc(&c->dGrid, n));
+  if (n > c->capGrid) {
+    dfree(c->dGrid);
+    c->capGrid = 0;
+    HIPCALL(c, hipMalloc(&c->dGrid, n));
+    c->capGrid = n;
+  }
   HIPCALL(c, hipMemcpyAsync(c->dGrid, grid, n, hipMemcpyHostToDevice, c->stream));
   int lo = 127, hi = 2;
   for (size_t i = 0; i < n; ++i) {
@@ -637,13 +687,14 @@ int gm_mesh_counts(gm_ctx* c, uint32_t* nv, uint32_t* nt) {
   // grow and re-emit when the output did not fit
   for (int attempt = 0; attempt < 3 && c->hTotals[2] != 0; ++attempt) {
     u32 lateN = 0;
-    HIPCALL(c, hipMemcpy(&lateN, c->mc.late, 4, hipMemcpyDeviceToHost));
+    HIPCALL(c, hipMemcpyAsync(&lateN, c->mc.late, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(c, hipStreamSynchronize(c->stream));
     rc = alloc_mesh_out(c, std::max(c->mc.capV, c->hTotals[0] + c->hTotals[0] / 8 + 1024),
                         std::max(c->mc.capT, c->hTotals[1] + c->hTotals[1] / 8 + 1024),
                         std::max(c->mc.capLate, lateN + 1024));
     if (rc) return rc;
     u32 tot[4] = {c->hTotals[0], c->hTotals[1], 0u, 0u};
-    HIPCALL(c, hipMemcpy(c->mc.totals, tot, 16, hipMemcpyHostToDevice));
+    HIPCALL(c, hipMemcpyAsync(c->mc.totals, tot, 16, hipMemcpyHostToDevice, c->stream));
     hipError_t e = launch_mc_emit(c->C, c->R, c->bmin, c->cell, c->mc, c->stream, nullptr);
     if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("mesh extraction: ") + hipGetErrorString(e));
     HIPCALL(c, hipMemcpyAsync(c->hTotals, c->mc.totals, 16, hipMemcpyDeviceToHost, c->stream));
@@ -679,7 +730,12 @@ int gm_build_pyramid(gm_ctx* c, int which) {
     ++levels;
     tex += (uint64_t)s * s * s;
   }
-  if (!c->dPyr) HIPCALL(c, hipMalloc(&c->dPyr, tex * 8));
+  if (tex > c->capPyr) {
+    dfree(c->dPyr);
+    c->capPyr = 0;
+    HIPCALL(c, hipMalloc(&c->dPyr, tex * 8));
+    c->capPyr = tex;
+  }
   HIPCALL(c, launch_pyramid(rows, c->R, radius_field(c), c->dPyr, c->stream));
   HIPCALL(c, hipStreamSynchronize(c->stream));
   c->pyrLevels = levels;
@@ -698,7 +754,8 @@ int gm_pyramid_size(const gm_ctx* c, int* levels, uint64_t* texels) {
 int gm_pyramid_get(gm_ctx* c, uint64_t* out) {
   if (!c || !out) return GM_ERR_ARG;
   if (!c->pyrLevels) return fail(c, GM_ERR_STATE, "no pyramid built");
-  HIPCALL(c, hipMemcpy(out, c->dPyr, c->pyrTexels * 8, hipMemcpyDeviceToHost));
+  HIPCALL(c, hipMemcpyAsync(out, c->dPyr, c->pyrTexels * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCALL(c, hipStreamSynchronize(c->stream));
   return GM_OK;
 }
 

@@ -1,0 +1,312 @@
+// Job engine: many closing jobs on one device at once.
+//
+// The reference runs one closing at a time from its viewer (GMorpho::Update,
+// GMorpho.cu:7775). Here a pool of slots, each one gm_ctx (its own HIP stream
+// and buffers) driven by its own host thread, takes jobs from a FIFO queue.
+// A job is admitted to a free slot when the device memory the slot would hold
+// afterwards fits the engine's HBM budget; a job that does not fit yet lets
+// smaller jobs behind it go first (backfill). Slots keep their largest
+// buffers, so once warm a stream of mixed sizes allocates nothing. Work of
+// different slots overlaps on the device. Every job runs exactly the kernels
+// of a single-context run, so its outputs are identical to running it alone.
+//
+// Batched use submits all jobs at once; served use submits them as they
+// arrive. Host inputs are copied at submit; meshes stay in the engine until
+// taken.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/gmorpho_c.h"
+
+namespace {
+
+struct Job {
+  gm_job desc{};
+  std::vector<float> P, lattice;
+  std::vector<uint32_t> T;
+  int R = 0;
+  int status = GM_OK;
+  std::string err;
+  bool done = false;
+  double tSubmit = 0, tStart = 0, tDone = 0;
+  std::vector<float> V, Nrm;
+  std::vector<uint32_t> Tri;
+  uint32_t nv = 0, nt = 0;
+  int slot = -1;
+};
+
+struct Slot {
+  gm_ctx* ctx = nullptr;
+  int capR = 0;            // largest base resolution this slot has run
+  uint64_t reserved = 0;   // device bytes the slot holds (estimate)
+  std::thread th;
+};
+
+// Device bytes of one context at base resolution R (the buffers alloc_grid,
+// the mesh output and the radius field allocate), plus the mesh input.
+uint64_t context_bytes(int R, int L, const gm_job& j) {
+  const uint64_t N = 2ull * (uint64_t)R, W = N / 32;
+  uint64_t P = 1;
+  while (P < (uint64_t)R) P <<= 1;
+  uint64_t b = 0;
+  b += 6 * N * N * W * 4;                         // I, D, K, C, voxelizer marks and shell
+  b += 4 * (uint64_t)R * R * W * 2;               // DF, TG, CL, UG
+  b += 2 * (N / 8) * (N / 8) * W;                 // brick summaries
+  b += (uint64_t)R * R * R;                       // cell-byte staging
+  b += 2 * (P / 8) * (P / 8) * (P / 8) * 4;       // MC block counts
+  b += (uint64_t)R * R * R * 4;                   // MC cell bases
+  b += 4 * N * N * 24 + 8 * N * N * 12;           // mesh output (initial capacity)
+  if (L > 0) b += (uint64_t)L * L * L * 4;        // radius lattice
+  b += (uint64_t)j.num_of_vertices * 24 + (uint64_t)j.num_of_faces * 12;
+  return b + (16ull << 20);                       // tables, scans, slack
+}
+
+}  // namespace
+
+struct gm_engine {
+  int device = 0;
+  uint64_t budget = 0;
+  std::vector<Slot> slots;
+  std::mutex m;
+  std::condition_variable cvWork, cvDone;
+  std::deque<Job*> queue;
+  std::unordered_map<uint64_t, Job*> jobs;
+  uint64_t nextId = 1;
+  uint64_t reservedTotal = 0;
+  int running = 0;
+  bool stop = false;
+  std::string err;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+
+  // First queued job this slot can run within the budget (caller holds m).
+  Job* pick(Slot& s) {
+    for (auto it = queue.begin(); it != queue.end(); ++it) {
+      Job* j = *it;
+      const int R = std::max(s.capR, j->R);
+      const uint64_t need = std::max(s.reserved, context_bytes(R, j->desc.lattice_res, j->desc));
+      const uint64_t grow = need - s.reserved;
+      if (reservedTotal + grow > budget) continue;
+      reservedTotal += grow;
+      s.reserved = need;
+      s.capR = R;
+      queue.erase(it);
+      return j;
+    }
+    return nullptr;
+  }
+
+  void run(Slot& s, Job* j) {
+    gm_ctx* c = s.ctx;
+    const gm_job& d = j->desc;
+    auto step = [&](int rc) {
+      if (rc != GM_OK && j->status == GM_OK) {
+        j->status = rc;
+        j->err = gm_last_error(c);
+      }
+      return rc == GM_OK;
+    };
+    if (!step(gm_load(c, j->P.data(), nullptr, d.num_of_vertices, j->T.data(), d.num_of_faces, d.base_res,
+                      d.se_size)))
+      return;
+    if (!j->lattice.empty()) {
+      if (!step(gm_scale_lattice_f32(c, j->lattice.data(), d.lattice_res))) return;
+    } else if (!step(gm_scale_uniform(c, d.radius))) {
+      return;
+    }
+    if (!step(gm_close(c, d.se))) return;
+    if (!step(gm_extract_mesh(c))) return;
+    if (!step(gm_mesh_counts(c, &j->nv, &j->nt))) return;
+    j->V.resize(3 * (size_t)j->nv);
+    j->Nrm.resize(3 * (size_t)j->nv);
+    j->Tri.resize(3 * (size_t)j->nt);
+    step(gm_mesh_copy(c, j->V.data(), j->Nrm.data(), j->Tri.data()));
+  }
+
+  void worker(int idx) {
+    Slot& s = slots[idx];
+    std::unique_lock<std::mutex> lk(m);
+    for (;;) {
+      Job* j = nullptr;
+      cvWork.wait(lk, [&] { return stop || (j = pick(s)) != nullptr; });
+      if (!j) return;  // stopping
+      j->slot = idx;
+      j->tStart = now();
+      ++running;
+      lk.unlock();
+      run(s, j);
+      lk.lock();
+      --running;
+      j->tDone = now();
+      j->done = true;
+      cvDone.notify_all();
+      cvWork.notify_all();
+    }
+  }
+};
+
+namespace {
+int efail(gm_engine* e, int code, const std::string& msg) {
+  e->err = "GMorpho Error: " + msg;
+  return code;
+}
+}  // namespace
+
+extern "C" {
+
+int gm_engine_create(int device, int slots, uint64_t hbm_budget_bytes, gm_engine** out) {
+  if (!out || slots < 1 || slots > 64) return GM_ERR_ARG;
+  *out = nullptr;
+  gm_engine* e = new gm_engine();
+  e->device = device;
+  e->budget = hbm_budget_bytes;
+  e->slots.resize(slots);
+  for (auto& s : e->slots) {
+    if (gm_create(device, &s.ctx) != GM_OK) {
+      for (auto& t : e->slots)
+        if (t.ctx) gm_destroy(t.ctx);
+      delete e;
+      return GM_ERR_HIP;
+    }
+  }
+  for (int i = 0; i < slots; ++i) e->slots[i].th = std::thread([e, i] { e->worker(i); });
+  *out = e;
+  return GM_OK;
+}
+
+int gm_engine_destroy(gm_engine* e) {
+  if (!e) return GM_ERR_ARG;
+  {
+    std::lock_guard<std::mutex> lk(e->m);
+    e->stop = true;
+    // queued jobs are dropped; running ones finish first
+    e->queue.clear();
+  }
+  e->cvWork.notify_all();
+  for (auto& s : e->slots)
+    if (s.th.joinable()) s.th.join();
+  for (auto& s : e->slots)
+    if (s.ctx) gm_destroy(s.ctx);
+  for (auto& kv : e->jobs) delete kv.second;
+  delete e;
+  return GM_OK;
+}
+
+const char* gm_engine_last_error(const gm_engine* e) { return e ? e->err.c_str() : ""; }
+
+int gm_engine_submit(gm_engine* e, const gm_job* job, uint64_t* id) {
+  if (!e || !job || !id) return GM_ERR_ARG;
+  const gm_job& d = *job;
+  if (!d.P || !d.T || d.num_of_vertices <= 0 || d.num_of_faces <= 0) return efail(e, GM_ERR_ARG, "empty mesh");
+  if (d.base_res < 16 || d.base_res > 1024 || (d.base_res % 16) != 0)
+    return efail(e, GM_ERR_UNSUPPORTED, "base resolution must be a multiple of 16 in [16,1024]");
+  if (d.lattice && d.lattice_res < 2) return efail(e, GM_ERR_ARG, "lattice resolution below 2");
+  Job* j = new Job();
+  j->desc = d;
+  j->R = d.base_res;
+  j->P.assign(d.P, d.P + 3 * (size_t)d.num_of_vertices);
+  j->T.assign(d.T, d.T + 3 * (size_t)d.num_of_faces);
+  if (d.lattice) j->lattice.assign(d.lattice, d.lattice + (size_t)d.lattice_res * d.lattice_res * d.lattice_res);
+  else j->desc.lattice_res = 0;
+  j->desc.P = nullptr;
+  j->desc.T = nullptr;
+  j->desc.lattice = nullptr;
+  if (context_bytes(j->R, j->desc.lattice_res, j->desc) > e->budget) {
+    delete j;
+    return efail(e, GM_ERR_CAPACITY, "job does not fit the engine's HBM budget");
+  }
+  {
+    std::lock_guard<std::mutex> lk(e->m);
+    if (e->stop) {
+      delete j;
+      return efail(e, GM_ERR_STATE, "engine is stopping");
+    }
+    j->tSubmit = e->now();
+    *id = e->nextId++;
+    e->jobs[*id] = j;
+    e->queue.push_back(j);
+  }
+  e->cvWork.notify_all();
+  return GM_OK;
+}
+
+int gm_engine_wait(gm_engine* e, uint64_t id, uint32_t* nv, uint32_t* nt) {
+  if (!e) return GM_ERR_ARG;
+  std::unique_lock<std::mutex> lk(e->m);
+  auto it = e->jobs.find(id);
+  if (it == e->jobs.end()) return efail(e, GM_ERR_ARG, "unknown job id");
+  Job* j = it->second;
+  e->cvDone.wait(lk, [&] { return j->done; });
+  if (j->status != GM_OK) {
+    e->err = j->err;
+    return j->status;
+  }
+  if (nv) *nv = j->nv;
+  if (nt) *nt = j->nt;
+  return GM_OK;
+}
+
+int gm_engine_take(gm_engine* e, uint64_t id, float* V, float* Nrm, uint32_t* T) {
+  if (!e) return GM_ERR_ARG;
+  Job* j = nullptr;
+  {
+    std::unique_lock<std::mutex> lk(e->m);
+    auto it = e->jobs.find(id);
+    if (it == e->jobs.end()) return efail(e, GM_ERR_ARG, "unknown job id");
+    j = it->second;
+    e->cvDone.wait(lk, [&] { return j->done; });
+    e->jobs.erase(it);
+  }
+  int rc = j->status;
+  if (rc == GM_OK) {
+    if (V) std::memcpy(V, j->V.data(), j->V.size() * 4);
+    if (Nrm) std::memcpy(Nrm, j->Nrm.data(), j->Nrm.size() * 4);
+    if (T) std::memcpy(T, j->Tri.data(), j->Tri.size() * 4);
+  } else {
+    e->err = j->err;
+  }
+  delete j;
+  return rc;
+}
+
+int gm_engine_job_times(gm_engine* e, uint64_t id, double t[3]) {
+  if (!e || !t) return GM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(e->m);
+  auto it = e->jobs.find(id);
+  if (it == e->jobs.end()) return efail(e, GM_ERR_ARG, "unknown job id");
+  t[0] = it->second->tSubmit;
+  t[1] = it->second->tStart;
+  t[2] = it->second->tDone;
+  return GM_OK;
+}
+
+int gm_engine_now(gm_engine* e, double* t) {
+  if (!e || !t) return GM_ERR_ARG;
+  *t = e->now();
+  return GM_OK;
+}
+
+int gm_engine_stats(gm_engine* e, uint64_t* reserved_bytes, int* running, int* queued) {
+  if (!e) return GM_ERR_ARG;
+  std::lock_guard<std::mutex> lk(e->m);
+  if (reserved_bytes) *reserved_bytes = e->reservedTotal;
+  if (running) *running = e->running;
+  if (queued) *queued = (int)e->queue.size();
+  return GM_OK;
+}
+
+}  // extern "C"

@@ -21,6 +21,8 @@
 //   mc_fixup   triangles the count table does not cover, mc_normalize.
 // The per-cell vertex base map is dense (R^3 u32) but only written and read
 // at surface cells, replacing the reference's 4*N^3-byte memset + splat.
+#include <mutex>
+
 #include "gm_internal.h"
 
 namespace gm {
@@ -696,10 +698,8 @@ __global__ void k_mc_normalize(float* Nrm, const u32* totals, u32 capV) {
   }
 }
 
-bool g_tables_uploaded = false;
 
-hipError_t upload_tables() {
-  if (g_tables_uploaded) return hipSuccess;
+hipError_t upload_tables_once() {
   McConst h{};
   for (int c = 0; c < 256; ++c) {
     int n = 0;
@@ -712,8 +712,14 @@ hipError_t upload_tables() {
   h.ntri[91] = 4;
   GM_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(g_mc), &h, sizeof(h)));
   GM_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(g_tri), kTriPacked, sizeof(kTriPacked)));
-  g_tables_uploaded = true;
   return hipSuccess;
+}
+// Once per process (contexts on several host threads share the tables).
+hipError_t upload_tables() {
+  static std::once_flag once;
+  static hipError_t status = hipSuccess;
+  std::call_once(once, [] { status = upload_tables_once(); });
+  return status;
 }
 
 }  // namespace

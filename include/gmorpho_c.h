@@ -131,6 +131,46 @@ int gm_get_stream(gm_ctx* ctx, void** stream);
  * terminates). Host only. */
 int gm_mc_tables(uint32_t* ntri256, uint32_t* len256, uint64_t* tri256);
 
+/* ---- Job engine. No counterpart in the reference, which runs one closing
+ * at a time from its viewer (GMorpho::Update, GMorpho.cu:7775); it is for
+ * callers with many jobs: batches of uneven grids and radii, and a serving
+ * loop. A pool of `slots` contexts (one HIP stream and one host thread each)
+ * runs jobs concurrently on one device. A job is admitted when a slot is free
+ * and the device memory the slot would then hold fits hbm_budget_bytes;
+ * smaller jobs may pass one that does not fit yet. Slots keep their largest
+ * buffers. A job runs the same kernels as a single-context run, so its mesh
+ * is identical to running it alone. Inputs are copied at submit. */
+typedef struct gm_engine gm_engine;
+typedef struct {
+  const float* P;          /* float[3*num_of_vertices] */
+  int num_of_vertices;
+  const uint32_t* T;       /* uint32[3*num_of_faces] */
+  int num_of_faces;
+  int base_res;            /* cells per axis (multiple of 16, 16..1024) */
+  float se_size;           /* the margin parameter of gm_load */
+  int radius;              /* uniform radius in fine voxels (lattice == NULL) */
+  const float* lattice;    /* optional FP32 radius lattice, lattice_res^3 */
+  int lattice_res;
+  int se;                  /* GM_SE_SPHERE or GM_SE_CUBE */
+} gm_job;
+int gm_engine_create(int device, int slots, uint64_t hbm_budget_bytes, gm_engine** out);
+/* Running jobs finish; queued ones are dropped. */
+int gm_engine_destroy(gm_engine* e);
+const char* gm_engine_last_error(const gm_engine* e);
+/* Queue a job; returns at once with its id. */
+int gm_engine_submit(gm_engine* e, const gm_job* job, uint64_t* id);
+/* Wait for a job; returns its status and mesh sizes. */
+int gm_engine_wait(gm_engine* e, uint64_t id, uint32_t* num_vertices, uint32_t* num_triangles);
+/* Copy a finished job's mesh out (V, Nrm float[3*nv], T uint32[3*nt]) and
+ * release the job. */
+int gm_engine_take(gm_engine* e, uint64_t id, float* V, float* Nrm, uint32_t* T);
+/* Submit, start and completion time of a job (seconds on the engine's clock). */
+int gm_engine_job_times(gm_engine* e, uint64_t id, double t[3]);
+/* The engine's clock now (seconds). */
+int gm_engine_now(gm_engine* e, double* t);
+/* Device bytes reserved by the slots, jobs running, jobs queued. */
+int gm_engine_stats(gm_engine* e, uint64_t* reserved_bytes, int* running, int* queued);
+
 #ifdef __cplusplus
 }
 #endif

@@ -85,6 +85,113 @@ def cpu_baseline(I, R, r, dr, bmin, cell, N):
             "kind": "port", "sample": "full closing + mesh of the same grid, once (%.1f s)" % dt}, C, (V, T)
 
 
+def _mesh_hash(V, N, T):
+    import hashlib
+    m = hashlib.sha256()
+    for a in (V, N, T):
+        m.update(np.ascontiguousarray(a).tobytes())
+    return m.hexdigest()
+
+
+def run_alone(ctx, j):
+    ctx.load(j["P"], j["T"], j["base_res"], 0.3)
+    if j["lattice"] is not None:
+        ctx.scale_lattice(j["lattice"], int(round(j["lattice"].size ** (1 / 3.0))))
+    else:
+        ctx.scale_uniform(j["radius"])
+    ctx.close_grid(B.SE_SPHERE)
+    ctx.extract_mesh()
+    return ctx.mesh()
+
+
+def run_engine(eng, jobs):
+    ids = [eng.submit(j["P"], j["T"], j["base_res"], 0.3, radius=j["radius"], lattice=j["lattice"]) for j in jobs]
+    return [eng.take(i) for i in ids]
+
+
+def batched(args):
+    """256 closing jobs of uneven size and radius: one-by-one on one context
+    vs the job engine (slot count tuned on the first 64 jobs). Every job's
+    mesh must be identical between the two."""
+    from broxy_amd import workloads
+    jobs = workloads.batched_jobs(seed=args.seed, n=args.batch_jobs)
+    big = max(jobs, key=lambda j: j["base_res"])
+    ctx = B.Context(0)
+    run_alone(ctx, big)  # buffers at their largest size before timing
+    t0 = time.perf_counter()
+    ref = [_mesh_hash(*run_alone(ctx, j)) for j in jobs]
+    t_seq = time.perf_counter() - t0
+    ctx.close()
+    # tune the slot count on a prefix of the batch
+    tune = {}
+    for k in (2, 4, 8, 16):
+        eng = B.Engine(0, slots=k, hbm_budget=args.hbm_budget << 30)
+        run_engine(eng, [big] * k)
+        t0 = time.perf_counter()
+        run_engine(eng, jobs[:64])
+        tune[k] = time.perf_counter() - t0
+        eng.close()
+    kbest = min(tune, key=tune.get)
+    eng = B.Engine(0, slots=kbest, hbm_budget=args.hbm_budget << 30)
+    run_engine(eng, [big] * kbest)
+    t0 = time.perf_counter()
+    res = run_engine(eng, jobs)
+    t_bat = time.perf_counter() - t0
+    eng.close()
+    same = all(_mesh_hash(*m) == h for m, h in zip(res, ref))
+    vox = sum((2 * j["base_res"]) ** 3 for j in jobs)
+    return {"jobs": len(jobs), "jobs_per_s": round(len(jobs) / t_bat, 2),
+            "one_by_one_jobs_per_s": round(len(jobs) / t_seq, 2), "speedup": round(t_seq / t_bat, 3),
+            "slots": kbest, "slot_tuning_s_per_64_jobs": {str(k): round(v, 4) for k, v in tune.items()},
+            "Gvoxels_per_s": round(vox / t_bat / 1e9, 3), "bit_exact_vs_one_by_one": bool(same),
+            "workload": "sphere-union jobs (8-128 icospheres), N log-uniform [64,384] step 32, r U{2..16}, "
+                        "voxelize + close + mesh + copy-back per job"}
+
+
+def served(args):
+    """Poisson arrivals at several rates: continuous admission into the job
+    engine vs one request at a time, same request sequence. Latency is
+    arrival (submit) to mesh on host."""
+    from broxy_amd import workloads
+    out = {}
+    for lam in args.rates:
+        # rate 0: every request arrives at t = 0 (serving capacity)
+        reqs = workloads.served_requests(lam if lam > 0 else 1e9, args.requests)
+        row = {}
+        hashes = {}
+        for mode, slots in (("continuous", args.slots), ("one_at_a_time", 1)):
+            eng = B.Engine(0, slots=slots, hbm_budget=args.hbm_budget << 30)
+            # warm every slot at the largest size of the mix
+            big = max(reqs, key=lambda q: q["base_res"])
+            run_engine(eng, [big] * slots)
+            start = eng.now()
+            ids = []
+            for q in reqs:
+                while eng.now() - start < q["t"]:
+                    time.sleep(min(0.002, max(0.0, q["t"] - (eng.now() - start))))
+                ids.append(eng.submit(q["P"], q["T"], q["base_res"], 0.3, radius=q["radius"], lattice=q["lattice"]))
+            lat, done = [], []
+            hs = []
+            for i in ids:
+                eng.wait(i)
+                t = eng.times(i)
+                lat.append(t[2] - t[0])
+                done.append(t[2])
+                hs.append(_mesh_hash(*eng.take(i)))
+            eng.close()
+            lat = np.array(lat) * 1e3
+            span = max(done) - start
+            row[mode] = {"p50_ms": round(float(np.percentile(lat, 50)), 2),
+                         "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                         "jobs_per_s": round(len(reqs) / span, 2), "slots": slots}
+            hashes[mode] = hs
+        row["bit_exact_mix_vs_alone"] = hashes["continuous"] == hashes["one_at_a_time"]
+        row["throughput_ratio"] = round(row["continuous"]["jobs_per_s"] / row["one_at_a_time"]["jobs_per_s"], 3)
+        out["lambda_%d" % lam if lam > 0 else "burst"] = row
+    return {"requests_per_rate": args.requests, "rates": out,
+            "workload": "N in {128,256,512,1024} p {.4,.3,.2,.1}, r U[2,24], 20% with a 64^3 radius lattice"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +201,14 @@ def main():
     ap.add_argument("--seed", type=int, default=2017)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="compare the closing grid with the oracle")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-jobs measurement")
+    ap.add_argument("--no-served", action="store_true", help="skip the served-requests measurement")
+    ap.add_argument("--batch-jobs", type=int, default=256)
+    ap.add_argument("--requests", type=int, default=60, help="served requests per arrival rate")
+    ap.add_argument("--rates", type=int, nargs="+", default=[20, 50, 100, 200, 0],
+                    help="Poisson arrival rates (req/s); 0 = all requests at once")
+    ap.add_argument("--slots", type=int, default=8, help="engine slots for the served measurement")
+    ap.add_argument("--hbm-budget", type=int, default=200, help="engine HBM budget in GiB")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -164,7 +279,8 @@ def main():
             traffic = ent.get("bytes")
 
     out = {
-        "metric": "Gvoxels/s closing 512^3 r=8 (+%HBM roof)" if args.config == "large" else "Gvoxels/s closing",
+        "metric": ("Gvoxels/s closing 512^3 r=8 (+%HBM roof); batched jobs/s vs 1-by-1; p99 lat"
+                   if args.config == "large" else "Gvoxels/s closing"),
         "value": round(N ** 3 / (ms * 1e-3) / 1e9, 3),
         "unit": "Gvoxels/s",
         "n_gpus": 1,
@@ -199,6 +315,10 @@ def main():
                              "triangles_exact": bool(np.array_equal(Tt, Tr)),
                              "vertices_rtol1e-6": bool(V.shape == Vr.shape and np.allclose(V, Vr, rtol=1e-6, atol=0))}
     ctx.close()
+    if not args.no_batched:
+        out["batched"] = batched(args)
+    if not args.no_served:
+        out["served"] = served(args)
     print(json.dumps(out), flush=True)
 
 

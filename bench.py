@@ -34,6 +34,11 @@ KERNEL_OF_STAGE = {"brick_summary": "k_brick_summary", "dilate_coarse": "k_pass_
                    "mc_tris": "k_mc_tris", "mc_fixup_normalize": "k_mc_fixup+k_mc_normalize"}
 
 
+# per-voxel radii run the runtime-table kernels
+VAR_KERNEL = {"dilate_coarse": "k_dil_coarse", "dilate_fine": "k_dil_fine", "erode_coarse": "k_ero_coarse",
+              "erode_fine": "k_ero_fine"}
+
+
 def algorithmic_bytes(stage, N, nv, nt):
     """Compulsory HBM bytes of one launch of a stage's kernel: bit grids N^3/8
     bytes, cell flags N^3/64 bytes, mesh 12 bytes per vertex position, normal
@@ -63,22 +68,32 @@ def rho_of_stage(stage, r):
 
 
 def build_workload(args):
+    """(P, T, base_res, uniform radius or None, radius lattice or None, description)."""
     if args.config == "large":
         P, T = B.sh_blob(7, seed=args.seed, lmax=8, amp=0.1)
-        return P, T, 256, 8, "sh-icosphere subdiv7 (327,680 tris), N=512, ball r=8"
+        return P, T, 256, 8, None, "sh-icosphere subdiv7 (327,680 tris), N=512, ball r=8"
     if args.config == "plain":
         P, T = B.sphere_soup(64, 3, seed=args.seed)
-        return P, T, 64, 4, "64-icosphere soup subdiv3, N=128, ball r=4"
+        return P, T, 64, 4, None, "64-icosphere soup subdiv3, N=128, ball r=4"
+    if args.config == "varying":
+        P, T = B.sh_blob(8, seed=args.seed, lmax=8, amp=0.1)
+        lat = B.scale_lattice(64, seed=args.seed, lo=2.0, hi=24.0, smooth=3)
+        return P, T, 512, None, lat, ("sh-icosphere subdiv8 (1,310,720 tris), N=1024, radii from a 64^3 FP32 "
+                                      "lattice U[2,24] box-smoothed 3x, trilinear")
     raise SystemExit("unknown config")
 
 
-def cpu_baseline(I, R, r, dr, bmin, cell, N):
+def cpu_baseline(I, R, r, dr, bmin, cell, N, lat=None):
     """The oracle (CPU restatement, OpenMP) on the same closing + mesh."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
+    sc = None
+    if lat is not None:
+        L = int(round(lat.size ** (1 / 3.0)))
+        sc = O.scale_from_lattice(lat.reshape(L, L, L), N)
     t0 = time.perf_counter()
-    D = O.dilate(I, R, r=r, data_res=dr)
-    C = O.erode(I, D, R, r=r, data_res=dr)
+    D = O.dilate(I, R, r=r or 0, data_res=dr, scale=sc)
+    C = O.erode(I, D, R, r=r or 0, data_res=dr, scale=sc)
     V, Nn, T = O.mc(C, R, bmin, cell)
     dt = time.perf_counter() - t0
     return {"value": round(N ** 3 / dt / 1e9, 4), "unit": "Gvoxels/s", "cores": int(O.num_threads()),
@@ -226,14 +241,17 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    P, T, R, r, desc = build_workload(args)
+    P, T, R, r, lat, desc = build_workload(args)
     N = 2 * R
     ctx = B.Context(0)
     t0 = time.perf_counter()
     ctx.load(P, T, R, 0.3)
     vox_ms = (time.perf_counter() - t0) * 1e3
     bmin, bmax, res, dr, cell = ctx.info()
-    ctx.scale_uniform(r)
+    if lat is not None:
+        ctx.scale_lattice(lat, int(round(lat.size ** (1 / 3.0))))
+    else:
+        ctx.scale_uniform(r)
 
     for _ in range(args.warmup):
         ctx.close_grid(B.SE_SPHERE)
@@ -292,9 +310,12 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic (seeded mesh generator)",
-        "config": {"workload": desc, "N": N, "radius": r, "se": "sphere", "mesh_vertices": nv,
+        "config": {"workload": desc, "N": N, "radius": r if r is not None else "lattice [2,24]", "se": "sphere",
+                   "mesh_vertices": nv,
                    "mesh_triangles": nt, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": KERNEL_OF_STAGE[dname].replace("rho", str(rho_of_stage(dname, r))),
+        "roofline": {"bound": "hbm",
+                     "kernel": (KERNEL_OF_STAGE[dname].replace("rho", str(rho_of_stage(dname, r))) if r
+                                else VAR_KERNEL.get(dname, KERNEL_OF_STAGE[dname])),
                      "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes": int(dbytes),
@@ -306,7 +327,7 @@ def main():
     }
     if not args.no_cpu:
         I = ctx.get_cells(B.GRID_INPUT)
-        cb, Cref, (Vr, Tr) = cpu_baseline(I, R, r, dr, bmin, cell, N)
+        cb, Cref, (Vr, Tr) = cpu_baseline(I, R, r, dr, bmin, cell, N, lat)
         out["cpu_baseline"] = cb
         if args.check:
             C = ctx.get_cells(B.GRID_CLOSING)

@@ -94,6 +94,7 @@ void free_grid(gm_ctx* c) {
   dfree(c->stage); dfree(c->mark); dfree(c->shell);
   dfree(c->mc.blockV); dfree(c->mc.blockT); dfree(c->mc.cellV); dfree(c->mc.cellT);
   dfree(c->mc.V); dfree(c->mc.Nrm); dfree(c->mc.T); dfree(c->mc.late); dfree(c->mc.totals); dfree(c->mc.scanPart);
+  dfree(c->mc.brick); dfree(c->mc.list);
   dfree(c->dPyr);
   c->capPyr = 0;
   dfree(c->dGrid);
@@ -188,6 +189,8 @@ int alloc_grid(gm_ctx* c, int R) {
   HIPCALL(c, hipMalloc(&c->mc.cellV, (size_t)R * R * R * 4));
   HIPCALL(c, hipMalloc(&c->mc.totals, 16));
   HIPCALL(c, hipMalloc(&c->mc.scanPart, mc_scan_parts(c->mc.nblocks) * 8));
+  HIPCALL(c, hipMalloc(&c->mc.brick, brickBytes));
+  HIPCALL(c, hipMalloc(&c->mc.list, ((size_t)c->mc.nblocks + 1) * 4));
   c->mc.capV = c->mc.capT = c->mc.capLate = 0;
   int rc = alloc_mesh_out(c, (u32)std::min<size_t>((size_t)4 * c->N * c->N, 1u << 30),
                           (u32)std::min<size_t>((size_t)8 * c->N * c->N, 1u << 30), 4096);

@@ -122,6 +122,9 @@ struct SETableSet {
 hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf, int se,
                          const SETableSet& tabs, int rho_lo, int rho_hi, int tabWords, hipStream_t s,
                          hipEvent_t* ev);
+// Brick summary of a bit grid of edge N: one byte per 1 word x 8 rows x 8
+// planes, bit0 any voxel set, bit1 all set.
+hipError_t launch_brick_summary(const u32* G, int N, uint8_t* out, hipStream_t s);
 hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf,
                         const SETableSet& tabs, int rho_lo, int rho_hi, int tabWords, hipStream_t s,
                         hipEvent_t* ev);
@@ -160,6 +163,8 @@ struct MCBuffers {
   u32* late;     // [0] count, then (slot, a, b, c) quadruples
   u32* totals;   // [0] nV [1] nT [2] overflow [3] border flag
   u32* scanPart; // 2 u32 per 1024 blocks
+  uint8_t* brick; // brick summary of the closing grid (1 word x 8 rows x 8 planes)
+  u32* list;      // [0] number of active blocks, then their ids
   u32 capV, capT, capLate;
   int nblocks;   // (P/8)^3
   int P;         // padded power-of-two cells per axis

@@ -230,53 +230,96 @@ __device__ __forceinline__ bool block_outside(int blk, int R, int& c0x, int& c0y
   return c0x >= R || c0y >= R || c0z >= R;
 }
 
-__global__ void __launch_bounds__(kMcThreads) k_mc_count(const u32* __restrict__ C, int R, u32* blockV,
+// Morton blocks that can hold a mixed cube: the bricks (1 word x 8 rows x 8
+// planes of C) around the block's 17^3 fine voxels are not all empty and not
+// all full. Active blocks are appended to list (list[0] = count, then block
+// ids, unordered); the others get zero counts. One thread per block.
+__global__ void __launch_bounds__(256) k_mc_blocks(const uint8_t* __restrict__ brick, int R, int nblocks,
+                                                   u32* blockV, u32* blockT, u32* list) {
+  const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+  bool act = false;
+  if (blk < nblocks) {
+    int c0x, c0y, c0z;
+    if (!block_outside(blk, R, c0x, c0y, c0z)) {
+      const int N = 2 * R, W = N / 32, NB = N / 8;
+      const int fx0 = 2 * c0x, fy0 = 2 * c0y, fz0 = 2 * c0z;
+      const int w0 = fx0 >> 5, w1 = min((fx0 + 16) >> 5, W - 1);
+      const int y0 = fy0 >> 3, y1 = min((fy0 + 16) >> 3, NB - 1);
+      const int z0 = fz0 >> 3, z1 = min((fz0 + 16) >> 3, NB - 1);
+      u32 any = 0u, all = 2u;
+      for (int z = z0; z <= z1; ++z)
+        for (int y = y0; y <= y1; ++y)
+          for (int w = w0; w <= w1; ++w) {
+            const u32 v = brick[((size_t)z * NB + y) * W + w];
+            any |= v & 1u;
+            all &= v & 2u;
+          }
+      act = any != 0u && all == 0u;
+    }
+    if (!act) {
+      blockV[blk] = 0u;
+      blockT[blk] = 0u;
+    }
+  }
+  const unsigned long long m = __ballot(act);
+  if (m) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int leader = __ffsll((long long)m) - 1;
+    u32 base = 0u;
+    if (lane == leader) base = atomicAdd(&list[0], (u32)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (act) list[1 + base + (u32)__popcll(m & ((1ull << lane) - 1ull))] = (u32)blk;
+  }
+}
+
+// Vertex and triangle counts of the active blocks; the workgroups walk the
+// list (each ends when the list does).
+__global__ void __launch_bounds__(kMcThreads) k_mc_count(const u32* __restrict__ C, int R,
+                                                          const u32* __restrict__ list, u32* blockV,
                                                           u32* blockT) {
   __shared__ Box bx;
   __shared__ u32 red[16];
   __shared__ McTab tab;
-  const int blk = blockIdx.x;
-  int c0x, c0y, c0z;
-  if (block_outside(blk, R, c0x, c0y, c0z)) {
-    if (threadIdx.x == 0) {
-      blockV[blk] = 0;
-      blockT[blk] = 0;
-    }
-    return;
-  }
   load_mc_tab(tab);
-  // cells of the block need fine voxels [16b, 16b+17) per axis
-  load_box(bx, C, 2 * R, 2 * c0x, 2 * c0y, 2 * c0z, 17, 17);
-  // uniform block: no mixed cube
-  u32 a = 0x1ffffu, o = 0u;
-  for (int i = threadIdx.x; i < 17 * 17; i += blockDim.x) {
-    u32 v = bx.rows[(i % 17) + kBox * (i / 17)] & 0x1ffffu;
-    a &= v;
-    o |= v;
-  }
-  const bool anyOne = __syncthreads_or(o != 0u) != 0;
-  const bool anyZero = __syncthreads_or(a != 0x1ffffu) != 0;
-  const bool nonuni = anyOne && anyZero;
-  u32 nv = 0, nt = 0;
-  if (nonuni) {
-    for (int h = 0; h < 2; ++h) {
-      int cx, cy, cz;
-      block_cell(blk, 2 * threadIdx.x + h, cx, cy, cz);
-      if (!cell_in_m_range(cx, cy, cz, R) || cell_uniform(bx, cx, cy, cz)) continue;
+  const u32 n = list[0];
+  for (u32 it = blockIdx.x; it < n; it += gridDim.x) {
+    const int blk = (int)list[1 + it];
+    int c0x, c0y, c0z;
+    block_outside(blk, R, c0x, c0y, c0z);
+    // cells of the block need fine voxels [16b, 16b+17) per axis
+    load_box(bx, C, 2 * R, 2 * c0x, 2 * c0y, 2 * c0z, 17, 17);
+    // uniform block: no mixed cube
+    u32 a = 0x1ffffu, o = 0u;
+    for (int i = threadIdx.x; i < 17 * 17; i += blockDim.x) {
+      u32 v = bx.rows[(i % 17) + kBox * (i / 17)] & 0x1ffffu;
+      a &= v;
+      o |= v;
+    }
+    const bool anyOne = __syncthreads_or(o != 0u) != 0;
+    const bool anyZero = __syncthreads_or(a != 0x1ffffu) != 0;
+    const bool nonuni = anyOne && anyZero;
+    u32 nv = 0, nt = 0;
+    if (nonuni) {
+      for (int h = 0; h < 2; ++h) {
+        int cx, cy, cz;
+        block_cell(blk, 2 * threadIdx.x + h, cx, cy, cz);
+        if (!cell_in_m_range(cx, cy, cz, R) || cell_uniform(bx, cx, cy, cz)) continue;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        int v = cube_case(bx, 2 * cx + (j & 1), 2 * cy + ((j >> 1) & 1), 2 * cz + ((j >> 2) & 1));
-        nv += own_verts(v);
-        nt += tab.ntri[v];
+        for (int j = 0; j < 8; ++j) {
+          int v = cube_case(bx, 2 * cx + (j & 1), 2 * cy + ((j >> 1) & 1), 2 * cz + ((j >> 2) & 1));
+          nv += own_verts(v);
+          nt += tab.ntri[v];
+        }
       }
     }
-  }
-  u32 tv, tt;
-  block_exscan(nv, red, &tv);
-  block_exscan(nt, red, &tt);
-  if (threadIdx.x == 0) {
-    blockV[blk] = tv;
-    blockT[blk] = tt;
+    u32 tv, tt;
+    block_exscan(nv, red, &tv);
+    block_exscan(nt, red, &tt);
+    if (threadIdx.x == 0) {
+      blockV[blk] = tv;
+      blockT[blk] = tt;
+    }
+    __syncthreads();  // the box is reloaded by the next iteration
   }
 }
 
@@ -464,17 +507,31 @@ __device__ __forceinline__ u32 prefix_in_cell(const BlockLds& S, int l, int j) {
 // Vertices of a block, in two phases per window of kListCap local vertex
 // ids: the mixed cubes write (cube, owned edge) at their local id, then every
 // thread takes consecutive ids (position + gathered normal sum).
+__device__ void mc_verts_block(BlockLds& S, const u32* __restrict__ C, int R, Geo geo, int blk, int nblocks,
+                               const u32* __restrict__ blockV, u32* cellV, float* V, float* Nrm, u32 capV,
+                               const u32* totals);
+
 __global__ void __launch_bounds__(kMcThreads) k_mc_verts(const u32* __restrict__ C, int R, Geo geo,
+                                                          const u32* __restrict__ list, int nblocks,
                                                           const u32* __restrict__ blockV, u32* cellV, float* V,
                                                           float* Nrm, u32 capV, const u32* totals) {
   __shared__ BlockLds S;
-  const int blk = blockIdx.x;
-  int c0x, c0y, c0z;
-  if (block_outside(blk, R, c0x, c0y, c0z)) return;
-  const u32 base = blockV[blk];
-  const u32 next = (blk + 1 < (int)gridDim.x) ? blockV[blk + 1] : totals[0];
-  if (next == base) return;  // no vertex in this block (uniform across the workgroup)
   load_mc_tab(S.tab);
+  const u32 nlist = list[0];
+  for (u32 it = blockIdx.x; it < nlist; it += gridDim.x) {
+    __syncthreads();  // LDS of the previous block is no longer read
+    mc_verts_block(S, C, R, geo, (int)list[1 + it], nblocks, blockV, cellV, V, Nrm, capV, totals);
+  }
+}
+
+__device__ void mc_verts_block(BlockLds& S, const u32* __restrict__ C, int R, Geo geo, int blk, int nblocks,
+                               const u32* __restrict__ blockV, u32* cellV, float* V, float* Nrm, u32 capV,
+                               const u32* totals) {
+  int c0x, c0y, c0z;
+  block_outside(blk, R, c0x, c0y, c0z);
+  const u32 base = blockV[blk];
+  const u32 next = (blk + 1 < nblocks) ? blockV[blk + 1] : totals[0];
+  if (next == base) return;  // no vertex in this block (uniform across the workgroup)
   load_box(S.box, C, 2 * R, 2 * c0x - kLo, 2 * c0y - kLo, 2 * c0z - kLo, kBox, kBox);
   stage_cases(S, blk, R);
   scan_cells(S);
@@ -561,20 +618,36 @@ __device__ __forceinline__ u32 prefix_ntri(const BlockLds& S, int l, int j) {
 // Triangles of a block, in the same two phases as k_mc_verts over local
 // triangle slots. Listed-but-uncounted triangles (len > ntri) go to the late
 // list from the fill phase of the first window.
+__device__ void mc_tris_block(BlockLds& S, const u32* __restrict__ C, int R, int blk, int nblocks,
+                              const u32* __restrict__ blockV, const u32* __restrict__ blockT,
+                              const u32* __restrict__ cellV, u32* T, u32 capT, u32* late, u32 capLate,
+                              u32* totals);
+
 __global__ void __launch_bounds__(kMcThreads) k_mc_tris(const u32* __restrict__ C, int R,
+                                                         const u32* __restrict__ list, int nblocks,
                                                          const u32* __restrict__ blockV,
                                                          const u32* __restrict__ blockT,
                                                          const u32* __restrict__ cellV, u32* T, u32 capT,
                                                          u32* late, u32 capLate, u32* totals) {
   __shared__ BlockLds S;
-  const int blk = blockIdx.x;
+  load_mc_tab(S.tab);
+  const u32 nlist = list[0];
+  for (u32 it = blockIdx.x; it < nlist; it += gridDim.x) {
+    __syncthreads();  // LDS of the previous block is no longer read
+    mc_tris_block(S, C, R, (int)list[1 + it], nblocks, blockV, blockT, cellV, T, capT, late, capLate, totals);
+  }
+}
+
+__device__ void mc_tris_block(BlockLds& S, const u32* __restrict__ C, int R, int blk, int nblocks,
+                              const u32* __restrict__ blockV, const u32* __restrict__ blockT,
+                              const u32* __restrict__ cellV, u32* T, u32 capT, u32* late, u32 capLate,
+                              u32* totals) {
   int c0x, c0y, c0z;
-  if (block_outside(blk, R, c0x, c0y, c0z)) return;
+  block_outside(blk, R, c0x, c0y, c0z);
   const u32 tbase = blockT[blk];
-  const u32 tnext = (blk + 1 < (int)gridDim.x) ? blockT[blk + 1] : totals[1];
+  const u32 tnext = (blk + 1 < nblocks) ? blockT[blk + 1] : totals[1];
   if (tnext == tbase) return;
   const u32 vbase = blockV[blk];
-  load_mc_tab(S.tab);
   load_box(S.box, C, 2 * R, 2 * c0x - kLo, 2 * c0y - kLo, 2 * c0z - kLo, kBox, kBox);
   stage_cases(S, blk, R);
   scan_cells(S);
@@ -738,11 +811,24 @@ hipError_t mc_tables_host(u32* ntri, u32* len, unsigned long long* tri) {
 
 size_t mc_scan_parts(int nblocks) { return (size_t)((nblocks + kChunk - 1) / kChunk); }
 
+// Workgroups of the kernels that walk the active-block list: enough to fill
+// the chip several times over, never more than there are blocks. The emit
+// kernels are long per block, so they get one workgroup per active block up
+// to a larger cap.
+int mc_list_grid(int nblocks, int cap) { return nblocks < cap ? nblocks : cap; }
+
 hipError_t launch_mc(const u32* C, int R, const float* bmin3, float cell, const MCBuffers& m, hipStream_t s,
                      hipEvent_t* ev) {
   GM_CHECK(upload_tables());
   if (ev) GM_CHECK(hipEventRecord(ev[0], s));
-  hipLaunchKernelGGL(k_mc_count, dim3(m.nblocks), dim3(kMcThreads), 0, s, C, R, m.blockV, m.blockT);
+  // active Morton blocks from a brick summary of C
+  GM_CHECK(launch_brick_summary(C, 2 * R, m.brick, s));
+  GM_CHECK(hipMemsetAsync(m.list, 0, sizeof(u32), s));
+  hipLaunchKernelGGL(k_mc_blocks, dim3((m.nblocks + 255) / 256), dim3(256), 0, s, m.brick, R, m.nblocks, m.blockV,
+                     m.blockT, m.list);
+  GM_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_mc_count, dim3(mc_list_grid(m.nblocks, 2048)), dim3(kMcThreads), 0, s, C, R, m.list, m.blockV,
+                     m.blockT);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
   int nparts = (int)mc_scan_parts(m.nblocks);
@@ -761,12 +847,12 @@ hipError_t launch_mc_emit(const u32* C, int R, const float* bmin3, float cell, c
   GM_CHECK(upload_tables());
   Geo geo{bmin3[0], bmin3[1], bmin3[2], 0.5f * cell};
   GM_CHECK(hipMemsetAsync(m.late, 0, sizeof(u32), s));
-  hipLaunchKernelGGL(k_mc_verts, dim3(m.nblocks), dim3(kMcThreads), 0, s, C, R, geo, m.blockV, m.cellV, m.V, m.Nrm,
-                     m.capV, m.totals);
+  hipLaunchKernelGGL(k_mc_verts, dim3(mc_list_grid(m.nblocks, 16384)), dim3(kMcThreads), 0, s, C, R, geo, m.list,
+                     m.nblocks, m.blockV, m.cellV, m.V, m.Nrm, m.capV, m.totals);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
-  hipLaunchKernelGGL(k_mc_tris, dim3(m.nblocks), dim3(kMcThreads), 0, s, C, R, m.blockV, m.blockT, m.cellV, m.T,
-                     m.capT, m.late, m.capLate, m.totals);
+  hipLaunchKernelGGL(k_mc_tris, dim3(mc_list_grid(m.nblocks, 16384)), dim3(kMcThreads), 0, s, C, R, m.list, m.nblocks,
+                     m.blockV, m.blockT, m.cellV, m.T, m.capT, m.late, m.capLate, m.totals);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   hipLaunchKernelGGL(k_mc_fixup, dim3(1), dim3(64), 0, s, m.V, m.Nrm, m.T, m.late, m.capLate, m.totals, m.capV,

@@ -278,6 +278,11 @@ hipError_t launch_pass(int pass, const void* generic, MorphArgs a, bool var, boo
 
 }  // namespace
 
+hipError_t launch_brick_summary(const u32* G, int N, uint8_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_brick_summary, dim3(N / 8, N / 8), dim3(8 * (N / 32)), 0, s, G, N, N / 32, out);
+  return hipGetLastError();
+}
+
 // rho_lo/rho_hi: range of the per-voxel radius r over the field; tabWords:
 // longest SE table of the launch.
 hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf, int se,

@@ -30,6 +30,8 @@ struct gm_ctx {
   u32 *I = nullptr, *D = nullptr, *K = nullptr, *C = nullptr;
   u16 *DF = nullptr, *TG = nullptr, *CL = nullptr, *UG = nullptr;
   uint8_t *brickI = nullptr, *brickK = nullptr;
+  u32* queue = nullptr;  // active-tile queues of the streaming passes
+  int qsel = 0;          // counter block of the next streaming pass
   uint8_t* stage = nullptr;  // R^3 bytes for layout conversions
   // mesh input
   float* dP = nullptr;
@@ -90,7 +92,7 @@ void dfree(T*& p) {
 
 void free_grid(gm_ctx* c) {
   dfree(c->I); dfree(c->D); dfree(c->K); dfree(c->C);
-  dfree(c->DF); dfree(c->TG); dfree(c->CL); dfree(c->UG); dfree(c->brickI); dfree(c->brickK);
+  dfree(c->DF); dfree(c->TG); dfree(c->CL); dfree(c->UG); dfree(c->brickI); dfree(c->brickK); dfree(c->queue);
   dfree(c->stage); dfree(c->mark); dfree(c->shell);
   dfree(c->mc.blockV); dfree(c->mc.blockT); dfree(c->mc.cellV); dfree(c->mc.cellT);
   dfree(c->mc.V); dfree(c->mc.Nrm); dfree(c->mc.T); dfree(c->mc.late); dfree(c->mc.totals); dfree(c->mc.scanPart);
@@ -174,6 +176,10 @@ int alloc_grid(gm_ctx* c, int R) {
   size_t brickBytes = (size_t)(c->N / 8) * (c->N / 8) * c->W;
   HIPCALL(c, hipMalloc(&c->brickI, brickBytes));
   HIPCALL(c, hipMalloc(&c->brickK, brickBytes));
+  HIPCALL(c, hipMalloc(&c->queue, tile_queue_words(R) * 4));
+  // both counter blocks start at zero (each pass clears the other one)
+  HIPCALL(c, hipMemsetAsync(c->queue, 0, 32 * 4, c->stream));
+  c->qsel = 0;
   HIPCALL(c, hipMalloc(&c->stage, (size_t)R * R * R));
   HIPCALL(c, hipMemsetAsync(c->I, 0, rowBytes, c->stream));
   HIPCALL(c, hipMemsetAsync(c->D, 0, rowBytes, c->stream));
@@ -317,6 +323,9 @@ MorphBuffers morph_buffers(gm_ctx* c) {
   b.tables = c->dTables;
   b.brickI = c->brickI;
   b.brickK = c->brickK;
+  b.queue = c->queue;
+  b.queueCap = (int)((tile_queue_words(c->R) - 32) / 8);
+  b.qsel = &c->qsel;
   return b;
 }
 

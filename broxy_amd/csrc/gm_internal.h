@@ -112,7 +112,16 @@ struct MorphBuffers {
   u32* tables;       // device SE tables (concatenated)
   uint8_t* brickI;   // per 1 word x 8 rows x 8 planes brick of I: bit0 any, bit1 all
   uint8_t* brickK;   // same for K (bit0 only)
+  u32* queue;        // active-tile queues of the streaming passes
+  int queueCap;      // tiles per queue list (the most tiles of any pass)
+  int* qsel;         // counter block of the next pass (flips every pass)
 };
+// u32 words of the active-tile queues for base resolution R.
+inline size_t tile_queue_words(int R) {
+  const size_t N = 2 * (size_t)R, W = N / 32;
+  const size_t tiles = W * ((N + kTileRows - 1) / kTileRows) * ((N + kTileZ - 1) / kTileZ);
+  return 32 + 8 * tiles;
+}
 struct SETableSet {
   // offsets (in u32) into MorphBuffers::tables for each rho, -1 if absent
   int dil[129];

@@ -43,6 +43,12 @@ namespace gm {
 
 namespace {
 
+// Tile selection of each pass (one wave per tile).
+template <int PASS>
+__global__ void __launch_bounds__(256) k_select(MorphArgs a) {
+  select_tiles<PASS>(a);
+}
+
 // Runtime-table kernels (any radius field, sphere or cube).
 __global__ void __launch_bounds__(64 * kMaxWaves) k_dil_coarse(MorphArgs a) { run_pass<0, 0>(a); }
 __global__ void __launch_bounds__(64 * kMaxWaves) k_dil_fine(MorphArgs a) { run_pass<1, 0>(a); }
@@ -236,6 +242,8 @@ MorphArgs base_args(const MorphDims& d, const MorphBuffers& b, const RadiusField
   a.UG = b.UG;
   a.brickI = b.brickI;
   a.brickK = b.brickK;
+  a.queue = b.queue;
+  a.queueCap = b.queueCap;
   return a;
 }
 
@@ -260,19 +268,41 @@ const CtKernels& ct_kernels() {
   return k;
 }
 
-// Launches streaming pass `pass` (0..3): the kernel compiled for chord-table
-// radius rho when there is one (fixedOk: uniform radius, sphere), else the
-// runtime-table kernel `generic`.
-hipError_t launch_pass(int pass, const void* generic, MorphArgs a, bool var, bool fixedOk, int rho,
+int device_cus() {
+  static int cus = [] {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1)
+      n = 256;
+    return n;
+  }();
+  return cus;
+}
+
+// Launches streaming pass `pass` (0..3): tile selection (skipped tiles are
+// finished there, active ones queued), then persistent workgroups of the
+// kernel compiled for chord-table radius rho when there is one (fixedOk:
+// uniform radius, sphere), else of the runtime-table kernel `generic`.
+hipError_t launch_pass(int pass, const void* generic, MorphArgs a, int* qsel, bool var, bool fixedOk, int rho,
                        hipStream_t s) {
+  a.qsel = *qsel;
+  *qsel = 1 - *qsel;
   const void* fn = generic;
   if (fixedOk && rho >= 1 && rho <= kCtMaxRho && ct_kernels().fn[pass][rho]) {
     fn = ct_kernels().fn[pass][rho];
     a.tabWords = 0;  // the chords are compiled in: no table in LDS
   }
-  const int nw = stream_waves(a, var);
+  const void* sel[4] = {reinterpret_cast<const void*>(&k_select<0>), reinterpret_cast<const void*>(&k_select<1>),
+                        reinterpret_cast<const void*>(&k_select<2>), reinterpret_cast<const void*>(&k_select<3>)};
+  const unsigned tiles = stream_grid(a).x;
   void* args[] = {&a};
-  GM_CHECK(hipLaunchKernel(fn, stream_grid(a), dim3(64 * nw), args, lds_bytes(a, var, nw), s));
+  GM_CHECK(hipLaunchKernel(sel[pass], dim3((tiles + 3) / 4), dim3(256), args, 0, s));
+  const int nw = stream_waves(a, var);
+  const size_t lds = lds_bytes(a, var, nw);
+  // one workgroup per tile at most: workgroup i takes entries i/8, i/8 + G/8,
+  // ... of queue i % 8 (G, the grid, is a multiple of 8)
+  unsigned grid = (tiles + 7u) & ~7u;
+  GM_CHECK(hipLaunchKernel(fn, dim3(grid), dim3(64 * nw), args, lds, s));
   return hipGetLastError();
 }
 
@@ -306,7 +336,7 @@ hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const Radius
   const int rho = (rf.r & 0x7f) + 1;
   a.ystep = 2;
   a.outFlags = b.DF;
-  GM_CHECK(launch_pass(0, reinterpret_cast<const void*>(&k_dil_coarse), a, var, fixedOk, rho, s));
+  GM_CHECK(launch_pass(0, reinterpret_cast<const void*>(&k_dil_coarse), a, b.qsel, var, fixedOk, rho, s));
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   a.outFlags = b.TG;
   a.outRows = b.D;
@@ -315,7 +345,7 @@ hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const Radius
   if (ev) GM_CHECK(hipEventRecord(ev[3], s));
   a.ystep = 1;
   a.outRows = b.D;
-  GM_CHECK(launch_pass(1, reinterpret_cast<const void*>(&k_dil_fine), a, var, fixedOk, rho, s));
+  GM_CHECK(launch_pass(1, reinterpret_cast<const void*>(&k_dil_fine), a, b.qsel, var, fixedOk, rho, s));
   if (ev) GM_CHECK(hipEventRecord(ev[4], s));
   return hipSuccess;
 }
@@ -341,7 +371,7 @@ hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusF
   const int rho = (rf.r & 0x7f) - 1;
   a.ystep = 2;
   a.outFlags = b.CL;
-  GM_CHECK(launch_pass(2, reinterpret_cast<const void*>(&k_ero_coarse), a, var, fixedOk, rho, s));
+  GM_CHECK(launch_pass(2, reinterpret_cast<const void*>(&k_ero_coarse), a, b.qsel, var, fixedOk, rho, s));
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   a.outFlags = b.UG;
   a.outRows = b.C;
@@ -350,7 +380,7 @@ hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusF
   if (ev) GM_CHECK(hipEventRecord(ev[3], s));
   a.ystep = 1;
   a.outRows = b.C;
-  GM_CHECK(launch_pass(3, reinterpret_cast<const void*>(&k_ero_fine), a, var, fixedOk, rho, s));
+  GM_CHECK(launch_pass(3, reinterpret_cast<const void*>(&k_ero_fine), a, b.qsel, var, fixedOk, rho, s));
   if (ev) GM_CHECK(hipEventRecord(ev[4], s));
   return hipSuccess;
 }

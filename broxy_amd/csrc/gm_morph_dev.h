@@ -487,6 +487,13 @@ struct MorphArgs {
   const uint8_t* brickI;  // per brick (1 word x 8 rows x 8 planes): bit0 any, bit1 all
   const uint8_t* brickK;
   u16* outFlags;
+  // active-tile queues: two blocks of counters ([0..7] tiles per queue,
+  // [8..15] next tile to hand out) at words 0 and 16, then (from word 32) 8
+  // lists of queueCap tile ids. A launch uses counter block qsel; its tile
+  // selection clears the other block for the next launch.
+  u32* queue;
+  int queueCap;
+  int qsel;
 };
 
 struct Lds {
@@ -519,49 +526,60 @@ __device__ __forceinline__ int table_off(const MorphArgs& a, int which, int rho)
   return which == 0 ? a.tabs.dil[rho] : (which == 1 ? a.tabs.ero_corner[rho] : a.tabs.ero_centre[rho]);
 }
 
+// Skip tests of one tile, evaluated by one wave (the result is uniform over
+// the wave).
 // Any brick flag (bit) set in the brick region covering rows [y0,y1),
 // planes [z0,z1), words [w0,w1] (clamped to the grid).
-__device__ bool bricks_any(const uint8_t* br, int N, int W, int y0, int y1, int z0, int z1, int w0, int w1,
-                           int bit) {
+__device__ bool wave_bricks_any(const uint8_t* br, int N, int W, int y0, int y1, int z0, int z1, int w0, int w1,
+                                int bit) {
   const int NB = N / 8;
-  int by0 = max(y0, 0) >> 3, by1 = (min(y1, N) - 1) >> 3;
-  int bz0 = max(z0, 0) >> 3, bz1 = (min(z1, N) - 1) >> 3;
+  const int by0 = max(y0, 0) >> 3, by1 = (min(y1, N) - 1) >> 3;
+  const int bz0 = max(z0, 0) >> 3, bz1 = (min(z1, N) - 1) >> 3;
   w0 = max(w0, 0);
   w1 = min(w1, W - 1);
-  int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1, nw = w1 - w0 + 1;
-  int found = 0;
+  const int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1, nw = w1 - w0 + 1;
+  u32 found = 0u;
   if (ny > 0 && nz > 0 && nw > 0) {
-    int n = ny * nz * nw;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      int w = w0 + i % nw, t = i / nw;
-      int by = by0 + t % ny, bz = bz0 + t / ny;
-      if ((br[((size_t)bz * NB + by) * W + w] >> bit) & 1) found = 1;
+    const int n = ny * nz * nw;
+    // branch-free: every lane loads (clamped index), out-of-range lanes mask
+    for (int i0 = 0; i0 < n; i0 += 64) {
+      const int i = min(i0 + lane_id(), n - 1);
+      const int w = w0 + i % nw, t = i / nw;
+      const int by = by0 + t % ny, bz = bz0 + t / ny;
+      found |= (u32)br[((size_t)bz * NB + by) * W + w] >> bit;
     }
   }
-  return __syncthreads_or(found) != 0;
+  return __any((found & 1u) != 0u) != 0;
 }
 // Every brick of the region has the "all" bit.
-__device__ bool bricks_all(const uint8_t* br, int N, int W, int y0, int y1, int z0, int z1, int w) {
+__device__ bool wave_bricks_all(const uint8_t* br, int N, int W, int y0, int y1, int z0, int z1, int w) {
   const int NB = N / 8;
-  int by0 = max(y0, 0) >> 3, by1 = (min(y1, N) - 1) >> 3;
-  int bz0 = max(z0, 0) >> 3, bz1 = (min(z1, N) - 1) >> 3;
-  int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1;
-  int miss = 0;
-  for (int i = threadIdx.x; i < ny * nz; i += blockDim.x) {
-    int by = by0 + i % ny, bz = bz0 + i / ny;
-    if (!((br[((size_t)bz * NB + by) * W + w] >> 1) & 1)) miss = 1;
+  const int by0 = max(y0, 0) >> 3, by1 = (min(y1, N) - 1) >> 3;
+  const int bz0 = max(z0, 0) >> 3, bz1 = (min(z1, N) - 1) >> 3;
+  const int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1;
+  u32 all = 2u;
+  const int n = ny * nz;
+  for (int i0 = 0; i0 < n; i0 += 64) {
+    const int i = min(i0 + lane_id(), n - 1);
+    const int by = by0 + i % ny, bz = bz0 + i / ny;
+    all &= (u32)br[((size_t)bz * NB + by) * W + w];
   }
-  return __syncthreads_or(miss) == 0;
+  return __any((all & 2u) == 0u) == 0;
 }
-
-__device__ __forceinline__ bool tile_has_flags(const u16* F, int R, int W, int wx, int y0, int y1, int z0, int z1) {
-  int cy0 = y0 >> 1, cy1 = min((y1 + 1) >> 1, R);
-  int cz0 = z0 >> 1, cz1 = min((z1 + 1) >> 1, R);
-  int ny = cy1 - cy0, nz = cz1 - cz0;
+// Some cell-row flag of word column wx set in rows [y0,y1), planes [z0,z1).
+__device__ bool wave_tile_has_flags(const u16* F, int R, int W, int wx, int y0, int y1, int z0, int z1) {
+  const int cy0 = y0 >> 1, cy1 = min((y1 + 1) >> 1, R);
+  const int cz0 = z0 >> 1, cz1 = min((z1 + 1) >> 1, R);
+  const int ny = cy1 - cy0, nz = cz1 - cz0;
   u32 acc = 0u;
-  if (ny > 0 && nz > 0)
-    for (int i = threadIdx.x; i < ny * nz; i += blockDim.x) acc |= flag_word(F, R, W, cy0 + i % ny, cz0 + i / ny, wx);
-  return __syncthreads_or(acc != 0u) != 0;
+  if (ny > 0 && nz > 0) {
+    const int n = ny * nz;
+    for (int i0 = 0; i0 < n; i0 += 64) {
+      const int i = min(i0 + lane_id(), n - 1);
+      acc |= ld_flag(F, R, W, cy0 + i % ny, cz0 + i / ny, wx);
+    }
+  }
+  return __any(acc != 0u) != 0;
 }
 
 // Distinct radii of a tile (128-bit set in LDS). Dilation: radius+add of the
@@ -686,25 +704,22 @@ __device__ void run_tile(const MorphArgs& a, Lds L, u32* set4, const TileGeo& g,
   }
 }
 
-// Tiles are numbered slab-major (word fastest, then row tile, then plane
-// tile). Workgroups are dealt round-robin to the 8 XCDs; workgroup i runs on
-// XCD i % 8 as that XCD's (i / 8)-th workgroup. Slab 8q + j goes to XCD
-// (j + q) % 8: the W word tiles of a slab, which share every 64-byte row
-// segment, meet in one L2 at about the same time, and the rotation by q keeps
-// any one row band or plane band (where a surface puts its work) from
-// landing on a single XCD.
-__device__ __forceinline__ int xcd_tile(int i, int W, int nslab) {
-  if (nslab & 7) return i;
-  const int k = i >> 3, x = i & 7;
-  const int q = k / W;
-  return ((q << 3) + ((x - q) & 7)) * W + (k - q * W);
-}
+constexpr int kQueueList = 32;  // first word of the queue lists
 
-__device__ __forceinline__ TileGeo tile_geo(const MorphArgs& a, int yoff, int zoff, int zstep) {
+__device__ __forceinline__ int tile_count(const MorphArgs& a) {
+  const int rows = a.ystep * kTileRows;
+  return a.W * ((a.N + rows - 1) / rows) * ((a.N + a.Zt - 1) / a.Zt);
+}
+// Queue (one per XCD) of the tiles of slab s: slab 8q + j goes to (j + q) % 8,
+// so that the word tiles of a slab, which share every 64-byte row segment,
+// run in one L2, and a row band or plane band spreads over all XCDs.
+__device__ __forceinline__ int slab_queue(int slab) { return ((slab & 7) + (slab >> 3)) & 7; }
+
+// Geometry of tile j (word fastest, then row tile, then plane tile).
+__device__ __forceinline__ TileGeo tile_at(const MorphArgs& a, int j, int yoff, int zoff, int zstep) {
   TileGeo g;
   const int rows = a.ystep * kTileRows;
-  const int nty = (a.N + rows - 1) / rows, ntz = (a.N + a.Zt - 1) / a.Zt;
-  const int j = xcd_tile(blockIdx.x, a.W, nty * ntz);
+  const int nty = (a.N + rows - 1) / rows;
   const int slab = j / a.W;
   g.wx = j - slab * a.W;
   const int ty = slab % nty, tz = slab / nty;
@@ -815,48 +830,97 @@ struct EroFineFin {  // C = UG ? ~Ef : C, C already holds C' = D & ~CL
 
 constexpr int kMaxWaves = 8;
 
-// Pass bodies. PASS 0: coarse dilation -> DF flags (lanes own the even rows,
-// even planes); 1: fine dilation -> D rows; 2: coarse erosion -> CL flags
-// (odd rows, odd planes); 3: fine erosion -> C rows. RHO > 0: uniform radius
+// Pass geometry. PASS 0: coarse dilation -> DF flags (lanes own the even
+// rows, even planes); 1: fine dilation -> D rows; 2: coarse erosion -> CL
+// flags (odd rows, odd planes); 3: fine erosion -> C rows.
+template <int PASS>
+__device__ __forceinline__ TileGeo pass_tile(const MorphArgs& a, int t) {
+  if constexpr (PASS == 0) return tile_at(a, t, 0, 0, 2);
+  else if constexpr (PASS == 2) return tile_at(a, t, 1, 1, 2);
+  else return tile_at(a, t, 0, 0, 1);
+}
+
+// Tile selection, one wave per tile: a tile whose result is known without the
+// predicate is finished here (coarse passes store their constant; the fine
+// passes' default output is already written), every other tile is appended to
+// the queue of its slab.
+template <int PASS>
+__device__ void select_tiles(const MorphArgs& a) {
+  if (blockIdx.x == 0 && threadIdx.x < 16) a.queue[16 * (1 - a.qsel) + threadIdx.x] = 0u;
+  u32* qc = a.queue + 16 * a.qsel;
+  const int t = blockIdx.x * (int)(blockDim.x >> 6) + wave_id();
+  if (t >= tile_count(a)) return;
+  const TileGeo g = pass_tile<PASS>(a, t);
+  const int lane = lane_id();
+  bool skip;
+  if constexpr (PASS == 0) {
+    const int y1 = g.yfirst + a.ystep * kTileRows;
+    // F = 0 when no occupied voxel is in reach; F unused when every cell is full
+    const bool empty = !wave_bricks_any(a.brickI, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff,
+                                        g.zfirst - a.maxoff, g.zlast + 2 + a.maxoff, g.wx - 1, g.wx + 1, 0);
+    const bool full = !empty && wave_bricks_all(a.brickI, a.N, a.W, g.yfirst, y1, g.zfirst, g.zlast + 2, g.wx);
+    skip = empty || full;
+    if (skip) {
+      DilCoarseFin fin{&a, g.wx, g.yfirst + 2 * lane, 0u, (u16)(full ? 0xffffu : 0u)};
+      for (int p = 0; p < g.nout(); ++p) fin.skipped(g.zfirst + p * g.zstep);
+    }
+  } else if constexpr (PASS == 2) {
+    const int y1 = g.yfirst + a.ystep * kTileRows;
+    skip = !wave_bricks_any(a.brickK, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff, g.zfirst - a.maxoff,
+                            g.zlast + 1 + a.maxoff, g.wx - 1, g.wx + 1, 0);
+    if (skip) {
+      EroCoarseFin fin{&a, g.wx, g.yfirst + 2 * lane, 0u};
+      for (int p = 0; p < g.nout(); ++p) fin.skipped(g.zfirst + p * g.zstep);
+    }
+  } else {
+    skip = !wave_tile_has_flags(PASS == 1 ? a.TG : a.UG, a.R, a.W, g.wx, g.yfirst, g.yfirst + kTileRows, g.zfirst,
+                                g.zlast + 1);
+  }
+  if (!skip && lane == 0) {
+    const int x = slab_queue(t / a.W);
+    const u32 k = atomicAdd(&qc[x], 1u);
+    a.queue[kQueueList + x * a.queueCap + k] = (u32)t;
+  }
+}
+
+// The predicate of one active tile and its epilogue. RHO > 0: uniform radius
 // with the compiled chord table of that radius; RHO = 0: runtime tables
 // (uniform or per-voxel radii, sphere or cube).
 template <int PASS, int RHO>
-__device__ __forceinline__ void run_pass(const MorphArgs& a) {
+__device__ __forceinline__ void run_pass_tile(const MorphArgs& a, int t, u32* smem, u32* set4) {
+  const TileGeo g = pass_tile<PASS>(a, t);
+  Lds L = carve(smem, a, g.nout());
+  if constexpr (PASS == 0) {
+    DilCoarseFin fin{&a, g.wx, g.yfirst + 2 * lane_id(), cell_range_mask(g.wx, 0, a.dR0), 0};
+    run_tile<0, 2, RHO>(a, L, set4, g, a.I, false, fin);
+  } else if constexpr (PASS == 1) {
+    DilFineFin fin{&a, g.wx, g.yfirst + lane_id()};
+    run_tile<0, 1, RHO>(a, L, set4, g, a.I, false, fin);
+  } else if constexpr (PASS == 2) {
+    EroCoarseFin fin{&a, g.wx, g.yfirst + 2 * lane_id(), cell_range_mask(g.wx, 0, a.dR0)};
+    run_tile<1, 2, RHO>(a, L, set4, g, a.K, false, fin);
+  } else {
+    EroFineFin fin{&a, g.wx, g.yfirst + lane_id()};
+    run_tile<2, 1, RHO>(a, L, set4, g, a.K, false, fin);
+  }
+}
+
+// Persistent workgroups over the active tiles. Workgroup i serves queue
+// i % 8 (the grid is a multiple of 8, so every queue has workgroups and is
+// drained); workgroups are dealt round-robin to the 8 XCDs, so a queue's
+// tiles stay in one L2. Tiles are not taken from other queues: the word
+// tiles of a slab write 4-byte pieces of the same 64-byte row segments and
+// run several times slower when their slab is spread over XCDs. A workgroup
+// ends when its queue is drained, so every wave reaches the exit.
+template <int PASS, int RHO>
+__device__ void run_pass(const MorphArgs& a) {
   extern __shared__ u32 smem[];
   __shared__ u32 set4[4];
-  if constexpr (PASS == 0) {
-    TileGeo g = tile_geo(a, 0, 0, 2);
-    Lds L = carve(smem, a, g.nout());
-    const int y1 = g.yfirst + a.ystep * kTileRows;
-    // F = 0 when no occupied voxel is in reach; F unused when every cell is full
-    const bool empty = !bricks_any(a.brickI, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff, g.zfirst - a.maxoff,
-                                   g.zlast + 2 + a.maxoff, g.wx - 1, g.wx + 1, 0);
-    const bool full = !empty && bricks_all(a.brickI, a.N, a.W, g.yfirst, y1, g.zfirst, g.zlast + 2, g.wx);
-    DilCoarseFin fin{&a, g.wx, g.yfirst + 2 * lane_id(), cell_range_mask(g.wx, 0, a.dR0),
-                     (u16)(full ? 0xffffu : 0u)};
-    run_tile<0, 2, RHO>(a, L, set4, g, a.I, empty || full, fin);
-  } else if constexpr (PASS == 1) {
-    TileGeo g = tile_geo(a, 0, 0, 1);
-    Lds L = carve(smem, a, g.nout());
-    const bool skip =
-        !tile_has_flags(a.TG, a.R, a.W, g.wx, g.yfirst, g.yfirst + kTileRows, g.zfirst, g.zlast + 1);
-    DilFineFin fin{&a, g.wx, g.yfirst + lane_id()};
-    run_tile<0, 1, RHO>(a, L, set4, g, a.I, skip, fin);
-  } else if constexpr (PASS == 2) {
-    TileGeo g = tile_geo(a, 1, 1, 2);
-    Lds L = carve(smem, a, g.nout());
-    const int y1 = g.yfirst + a.ystep * kTileRows;
-    const bool skip = !bricks_any(a.brickK, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff, g.zfirst - a.maxoff,
-                                  g.zlast + 1 + a.maxoff, g.wx - 1, g.wx + 1, 0);
-    EroCoarseFin fin{&a, g.wx, g.yfirst + 2 * lane_id(), cell_range_mask(g.wx, 0, a.dR0)};
-    run_tile<1, 2, RHO>(a, L, set4, g, a.K, skip, fin);
-  } else {
-    TileGeo g = tile_geo(a, 0, 0, 1);
-    Lds L = carve(smem, a, g.nout());
-    const bool skip =
-        !tile_has_flags(a.UG, a.R, a.W, g.wx, g.yfirst, g.yfirst + kTileRows, g.zfirst, g.zlast + 1);
-    EroFineFin fin{&a, g.wx, g.yfirst + lane_id()};
-    run_tile<2, 1, RHO>(a, L, set4, g, a.K, skip, fin);
+  const int x = (int)(blockIdx.x & 7u);
+  const u32* qc = a.queue + 16 * a.qsel;
+  for (u32 k = blockIdx.x >> 3; k < qc[x]; k += gridDim.x >> 3) {
+    run_pass_tile<PASS, RHO>(a, (int)a.queue[kQueueList + x * a.queueCap + k], smem, set4);
+    __syncthreads();  // LDS is reused by the next tile
   }
 }
 

@@ -179,6 +179,8 @@ struct BlockLds {
   McTab tab;
   Box box;
   uint8_t cases[4096];  // [local cell * 8 + cube], 0 for cells outside M range
+  uint8_t vpre[4096];   // vertices owned by the cell's earlier cubes
+  uint8_t tpre[4096];   // counted triangles of the cell's earlier cubes
   u32 vOff[512];
   u32 tOff[512];
   u32 red[16];
@@ -197,12 +199,18 @@ __device__ void stage_cases(BlockLds& S, int blk, int R) {
       for (int j = 0; j < 8; ++j) {
         int v = cube_case(S.box, 2 * cx + (j & 1), 2 * cy + ((j >> 1) & 1), 2 * cz + ((j >> 2) & 1));
         S.cases[8 * l + j] = (uint8_t)v;
+        S.vpre[8 * l + j] = (uint8_t)nv;
+        S.tpre[8 * l + j] = (uint8_t)nt;
         nv += own_verts(v);
         nt += S.tab.ntri[v];
       }
     } else {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) S.cases[8 * l + j] = 0;
+      for (int j = 0; j < 8; ++j) {
+        S.cases[8 * l + j] = 0;
+        S.vpre[8 * l + j] = 0;
+        S.tpre[8 * l + j] = 0;
+      }
     }
     S.vOff[l] = nv;
     S.tOff[l] = nt;
@@ -498,11 +506,7 @@ __device__ void normal_sum(const McTab& tab, const Geo& geo, const Box& bx, int 
   }
 }
 
-__device__ __forceinline__ u32 prefix_in_cell(const BlockLds& S, int l, int j) {
-  u32 s = 0;
-  for (int q = 0; q < j; ++q) s += own_verts(S.cases[8 * l + q]);
-  return s;
-}
+__device__ __forceinline__ u32 prefix_in_cell(const BlockLds& S, int l, int j) { return S.vpre[8 * l + j]; }
 
 // Vertices of a block, in two phases per window of kListCap local vertex
 // ids: the mixed cubes write (cube, owned edge) at their local id, then every
@@ -609,11 +613,7 @@ __device__ __forceinline__ u32 edge_vid(const BlockLds& S, int blk, int c0x, int
   return s + (u32)rank_of(cube_case(S.box, gx, gy, gz), oe);
 }
 
-__device__ __forceinline__ u32 prefix_ntri(const BlockLds& S, int l, int j) {
-  u32 s = 0;
-  for (int q = 0; q < j; ++q) s += S.tab.ntri[S.cases[8 * l + q]];
-  return s;
-}
+__device__ __forceinline__ u32 prefix_ntri(const BlockLds& S, int l, int j) { return S.tpre[8 * l + j]; }
 
 // Triangles of a block, in the same two phases as k_mc_verts over local
 // triangle slots. Listed-but-uncounted triangles (len > ntri) go to the late

@@ -471,10 +471,29 @@ __constant__ int8_t g_share[3][4][4] = {
     // owned edge 8 (z): cubes f, f-x, f-y, f-x-y with local edges 8, 9, 11, 10
     {{0, 0, 0, 8}, {-1, 0, 0, 9}, {0, -1, 0, 11}, {-1, -1, 0, 10}}};
 
-__device__ __forceinline__ void local_edge_pos(const Geo& g, const Box& b, int fx, int fy, int fz, int e, float* p) {
+// edge_pos with the two voxel values given (v0 at (fx,fy,fz), v1 one step
+// along the edge's axis): the same float operations.
+__device__ __forceinline__ void edge_pos_v(const Geo& g, int fx, int fy, int fz, int e, u32 v0, u32 v1, float* o) {
+  float px = fmaf((float)fx, g.h, g.bx);
+  float py = fmaf((float)fy, g.h, g.by);
+  float pz = fmaf((float)fz, g.h, g.bz);
+  o[0] = px;
+  o[1] = py;
+  o[2] = pz;
+  if (e == 0) o[0] = lerp_edge(px, __fadd_rn(px, g.h), (float)v0, (float)v1);
+  else if (e == 3) o[1] = lerp_edge(__fadd_rn(py, g.h), py, (float)v1, (float)v0);
+  else o[2] = lerp_edge(pz, __fadd_rn(pz, g.h), (float)v0, (float)v1);
+}
+// Corner index (cube_case order) of the corner at offset (ox, oy, oz).
+__device__ __forceinline__ int corner_of(int ox, int oy, int oz) { return 4 * oz + (oy ? (ox ? 2 : 3) : ox); }
+// Position of the vertex on local edge e of cube (fx,fy,fz) with case v: both
+// ends of the edge are corners of the cube, so their values are bits of v.
+__device__ __forceinline__ void local_edge_pos(const Geo& g, int fx, int fy, int fz, int v, int e, float* p) {
   int dx, dy, dz, oe;
   own_of(e, dx, dy, dz, oe);
-  edge_pos(g, b, fx + dx, fy + dy, fz + dz, oe, p);
+  const int ca = corner_of(dx, dy, dz);
+  const int cb = corner_of(dx + (oe == 0), dy + (oe == 3), dz + (oe == 8));
+  edge_pos_v(g, fx + dx, fy + dy, fz + dz, oe, (u32)(v >> ca) & 1u, (u32)(v >> cb) & 1u, p);
 }
 
 __device__ void normal_sum(const McTab& tab, const Geo& geo, const Box& bx, int R, int fx, int fy, int fz, int q,
@@ -489,14 +508,21 @@ __device__ void normal_sum(const McTab& tab, const Geo& geo, const Box& bx, int 
     if (!mixed(gv)) continue;
     unsigned long long tri = tab.tri[gv];
     u32 cnt = tab.ntri[gv];
+    // the cube's counted triangles that use edge le, then only those (in order)
+    u32 use = 0u;
     for (u32 t = 0; t < cnt; ++t) {
+      const u32 w = (u32)(tri >> (12 * t)) & 0xfffu;
+      if ((w & 0xfu) == (u32)le || ((w >> 4) & 0xfu) == (u32)le || (w >> 8) == (u32)le) use |= 1u << t;
+    }
+    while (use) {
+      const u32 t = (u32)__builtin_ctz(use);
+      use &= use - 1u;
       int e0 = (int)((tri >> (12 * t)) & 0xf), e1 = (int)((tri >> (12 * t + 4)) & 0xf),
           e2 = (int)((tri >> (12 * t + 8)) & 0xf);
-      if (e0 != le && e1 != le && e2 != le) continue;
       float p0[3], p1[3], p2[3];
-      local_edge_pos(geo, bx, gx, gy, gz, e2, p0);
-      local_edge_pos(geo, bx, gx, gy, gz, e1, p1);
-      local_edge_pos(geo, bx, gx, gy, gz, e0, p2);
+      local_edge_pos(geo, gx, gy, gz, gv, e2, p0);
+      local_edge_pos(geo, gx, gy, gz, gv, e1, p1);
+      local_edge_pos(geo, gx, gy, gz, gv, e0, p2);
       float a0 = __fsub_rn(p1[0], p0[0]), a1 = __fsub_rn(p1[1], p0[1]), a2 = __fsub_rn(p1[2], p0[2]);
       float b0 = __fsub_rn(p2[0], p0[0]), b1 = __fsub_rn(p2[1], p0[1]), b2 = __fsub_rn(p2[2], p0[2]);
       n[0] = __fadd_rn(n[0], __fsub_rn(__fmul_rn(a1, b2), __fmul_rn(a2, b1)));
@@ -575,7 +601,10 @@ __device__ void mc_verts_block(BlockLds& S, const u32* __restrict__ C, int R, Ge
       block_cell(blk, l, cx, cy, cz);
       const int fx = 2 * cx + (j & 1), fy = 2 * cy + ((j >> 1) & 1), fz = 2 * cz + ((j >> 2) & 1);
       float p[3], nn[3];
-      edge_pos(geo, S.box, fx, fy, fz, q == 0 ? 0 : (q == 1 ? 3 : 8), p);
+      const int v = S.cases[s];
+      // own edge q runs from corner 0 to corner 1 (x), 3 (y) or 4 (z)
+      edge_pos_v(geo, fx, fy, fz, q == 0 ? 0 : (q == 1 ? 3 : 8), (u32)v & 1u,
+                 (u32)(v >> (q == 0 ? 1 : (q == 1 ? 3 : 4))) & 1u, p);
       normal_sum(S.tab, geo, S.box, R, fx, fy, fz, q, nn);
       V[3 * (size_t)vid] = p[0];
       V[3 * (size_t)vid + 1] = p[1];

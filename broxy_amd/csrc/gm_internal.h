@@ -132,8 +132,9 @@ hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const Radius
                          const SETableSet& tabs, int rho_lo, int rho_hi, int tabWords, hipStream_t s,
                          hipEvent_t* ev);
 // Brick summary of a bit grid of edge N: one byte per 1 word x 8 rows x 8
-// planes, bit0 any voxel set, bit1 all set.
-hipError_t launch_brick_summary(const u32* G, int N, uint8_t* out, hipStream_t s);
+// planes, bit0 any voxel set, bit1 all set. clear (optional): a u32 counter
+// set to zero by the same launch.
+hipError_t launch_brick_summary(const u32* G, int N, uint8_t* out, u32* clear, hipStream_t s);
 hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf,
                         const SETableSet& tabs, int rho_lo, int rho_hi, int tabWords, hipStream_t s,
                         hipEvent_t* ev);

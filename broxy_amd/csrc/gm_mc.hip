@@ -544,8 +544,10 @@ __device__ void mc_verts_block(BlockLds& S, const u32* __restrict__ C, int R, Ge
 __global__ void __launch_bounds__(kMcThreads) k_mc_verts(const u32* __restrict__ C, int R, Geo geo,
                                                           const u32* __restrict__ list, int nblocks,
                                                           const u32* __restrict__ blockV, u32* cellV, float* V,
-                                                          float* Nrm, u32 capV, const u32* totals) {
+                                                          float* Nrm, u32 capV, const u32* totals, u32* late) {
   __shared__ BlockLds S;
+  // the late-triangle count starts from zero in the triangle kernel that follows
+  if (blockIdx.x == 0 && threadIdx.x == 0) late[0] = 0u;
   load_mc_tab(S.tab);
   const u32 nlist = list[0];
   for (u32 it = blockIdx.x; it < nlist; it += gridDim.x) {
@@ -851,8 +853,7 @@ hipError_t launch_mc(const u32* C, int R, const float* bmin3, float cell, const 
   GM_CHECK(upload_tables());
   if (ev) GM_CHECK(hipEventRecord(ev[0], s));
   // active Morton blocks from a brick summary of C
-  GM_CHECK(launch_brick_summary(C, 2 * R, m.brick, s));
-  GM_CHECK(hipMemsetAsync(m.list, 0, sizeof(u32), s));
+  GM_CHECK(launch_brick_summary(C, 2 * R, m.brick, m.list, s));
   hipLaunchKernelGGL(k_mc_blocks, dim3((m.nblocks + 255) / 256), dim3(256), 0, s, m.brick, R, m.nblocks, m.blockV,
                      m.blockT, m.list);
   GM_CHECK(hipGetLastError());
@@ -875,9 +876,8 @@ hipError_t launch_mc_emit(const u32* C, int R, const float* bmin3, float cell, c
                           hipEvent_t* ev) {
   GM_CHECK(upload_tables());
   Geo geo{bmin3[0], bmin3[1], bmin3[2], 0.5f * cell};
-  GM_CHECK(hipMemsetAsync(m.late, 0, sizeof(u32), s));
   hipLaunchKernelGGL(k_mc_verts, dim3(mc_list_grid(m.nblocks, 16384)), dim3(kMcThreads), 0, s, C, R, geo, m.list,
-                     m.nblocks, m.blockV, m.cellV, m.V, m.Nrm, m.capV, m.totals);
+                     m.nblocks, m.blockV, m.cellV, m.V, m.Nrm, m.capV, m.totals, m.late);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
   hipLaunchKernelGGL(k_mc_tris, dim3(mc_list_grid(m.nblocks, 16384)), dim3(kMcThreads), 0, s, C, R, m.list, m.nblocks,

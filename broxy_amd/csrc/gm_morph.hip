@@ -89,8 +89,11 @@ __global__ void k_tight(MorphArgs a) {
 // voxel set, bit1 = all set). Workgroup = one 8x8 (row, plane) brick column
 // over every word: thread (w, row) walks the 8 planes, so each plane step is
 // one contiguous read of 8 rows; the 8 rows of a word meet in LDS.
+// clear (optional): a counter the next kernel starts from, zeroed here so
+// that no separate memset is needed.
 __global__ void __launch_bounds__(512) k_brick_summary(const u32* __restrict__ G, int N, int W,
-                                                        uint8_t* __restrict__ out) {
+                                                        uint8_t* __restrict__ out, u32* clear) {
+  if (clear && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *clear = 0u;
   __shared__ u32 anyW[64], allW[64];
   const int NB = N / 8;
   const int by = blockIdx.x, bz = blockIdx.y;
@@ -308,8 +311,8 @@ hipError_t launch_pass(int pass, const void* generic, MorphArgs a, int* qsel, bo
 
 }  // namespace
 
-hipError_t launch_brick_summary(const u32* G, int N, uint8_t* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_brick_summary, dim3(N / 8, N / 8), dim3(8 * (N / 32)), 0, s, G, N, N / 32, out);
+hipError_t launch_brick_summary(const u32* G, int N, uint8_t* out, u32* clear, hipStream_t s) {
+  hipLaunchKernelGGL(k_brick_summary, dim3(N / 8, N / 8), dim3(8 * (N / 32)), 0, s, G, N, N / 32, out, clear);
   return hipGetLastError();
 }
 
@@ -329,7 +332,8 @@ hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const Radius
   const bool var = rf.mode != 0;
   long long cellwords = (long long)d.R * d.R * d.W;
   if (ev) GM_CHECK(hipEventRecord(ev[0], s));
-  hipLaunchKernelGGL(k_brick_summary, dim3(d.N / 8, d.N / 8), dim3(8 * d.W), 0, s, b.I, d.N, d.W, b.brickI);
+  hipLaunchKernelGGL(k_brick_summary, dim3(d.N / 8, d.N / 8), dim3(8 * d.W), 0, s, b.I, d.N, d.W, b.brickI,
+                     nullptr);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
   const bool fixedOk = !var && se == 0;

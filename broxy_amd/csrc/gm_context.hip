@@ -45,6 +45,11 @@ struct gm_ctx {
   int8_t* dGrid = nullptr;
   size_t capGrid = 0;  // bytes of dGrid
   int32_t* dLat = nullptr;
+  // per-voxel radii by value: radius planes, per-brick value sets, masked
+  // erosion source, K brick list (sized for capVarR)
+  u32 *rp = nullptr, *rvset = nullptr, *kv = nullptr, *kbl = nullptr;
+  int capVarR = 0;
+  bool rpValid = false;  // rp/rvset hold the current radius field
   // structuring-element tables
   u32* dTables = nullptr;
   size_t tablesCap = 0;
@@ -101,6 +106,9 @@ void free_grid(gm_ctx* c) {
   c->capPyr = 0;
   dfree(c->dGrid);
   c->capGrid = 0;
+  dfree(c->rp); dfree(c->rvset); dfree(c->kv); dfree(c->kbl);
+  c->capVarR = 0;
+  c->rpValid = false;
   c->has_grid = false;
   c->capR = 0;
   c->meshPending = false;
@@ -133,6 +141,7 @@ void set_dims(gm_ctx* c, int R) {
   if (c->R != R && c->rmode != 0) {
     // a per-voxel radius field belongs to the previous grid size
     c->rmode = 0;
+    c->rpValid = false;
     c->r = 0;
     c->rlo = c->rhi = 0;
   }
@@ -178,7 +187,7 @@ int alloc_grid(gm_ctx* c, int R) {
   HIPCALL(c, hipMalloc(&c->brickK, brickBytes));
   HIPCALL(c, hipMalloc(&c->queue, tile_queue_words(R) * 4));
   // both counter blocks start at zero (each pass clears the other one)
-  HIPCALL(c, hipMemsetAsync(c->queue, 0, 32 * 4, c->stream));
+  HIPCALL(c, hipMemsetAsync(c->queue, 0, kQueueLists * 4, c->stream));
   c->qsel = 0;
   HIPCALL(c, hipMalloc(&c->stage, (size_t)R * R * R));
   HIPCALL(c, hipMemsetAsync(c->I, 0, rowBytes, c->stream));
@@ -324,8 +333,15 @@ MorphBuffers morph_buffers(gm_ctx* c) {
   b.brickI = c->brickI;
   b.brickK = c->brickK;
   b.queue = c->queue;
-  b.queueCap = (int)((tile_queue_words(c->R) - 32) / 8);
+  b.queueCap = (int)tile_count_max(c->R);
   b.qsel = &c->qsel;
+  if (c->rmode != 0 && c->capVarR >= c->R) {
+    b.rp = c->rp;
+    b.rvset = c->rvset;
+    b.kv = c->kv;
+    b.kbl = c->kbl;
+    b.rpBuild = c->rpValid ? 0 : 1;
+  }
   return b;
 }
 
@@ -340,6 +356,22 @@ MorphDims morph_dims(gm_ctx* c) {
 
 int check_radius(gm_ctx* c) {
   if (c->rhi > 31) return fail(c, GM_ERR_UNSUPPORTED, "radius above 31 fine voxels is not supported by this build");
+  return GM_OK;
+}
+
+// Buffers of the per-value passes of a per-voxel radius field (kept at the
+// largest size used, like the grid buffers).
+int ensure_var_buffers(gm_ctx* c) {
+  if (c->rmode == 0 || c->capVarR >= c->R) return GM_OK;
+  dfree(c->rp); dfree(c->rvset); dfree(c->kv); dfree(c->kbl);
+  c->capVarR = 0;
+  c->rpValid = false;
+  const int R = c->R;
+  HIPCALL(c, hipMalloc(&c->rp, radius_plane_words(R) * 4));
+  HIPCALL(c, hipMalloc(&c->rvset, brick_words(R) * 4));
+  HIPCALL(c, hipMalloc(&c->kv, (size_t)R * R * R / 4 * 4));
+  HIPCALL(c, hipMalloc(&c->kbl, (brick_words(R) + 1) * 4));
+  c->capVarR = R;
   return GM_OK;
 }
 
@@ -358,9 +390,12 @@ int do_dilate(gm_ctx* c, int se) {
   } else {
     rc = ensure_tables(c, lo, hi);
     if (rc) return rc;
+    rc = ensure_var_buffers(c);
+    if (rc) return rc;
     hipError_t e = launch_dilate(morph_dims(c), morph_buffers(c), radius_field(c), 0, c->tabs, lo, hi, c->tabWords,
                                  c->stream, c->timing ? &c->ev[0] : nullptr);
     if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("dilation: ") + hipGetErrorString(e));
+    if (c->rmode != 0) c->rpValid = true;
   }
   c->meshPending = false;
   return GM_OK;
@@ -371,9 +406,12 @@ int do_erode(gm_ctx* c) {
   if (rc) return rc;
   rc = ensure_tables(c, std::max(c->rlo, 0), c->rhi);
   if (rc) return rc;
+  rc = ensure_var_buffers(c);
+  if (rc) return rc;
   hipError_t e = launch_erode(morph_dims(c), morph_buffers(c), radius_field(c), c->tabs, c->rlo, c->rhi,
                               c->tabWords, c->stream, c->timing ? &c->ev[5] : nullptr);
   if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("erosion: ") + hipGetErrorString(e));
+  if (c->rmode != 0) c->rpValid = true;
   c->meshPending = false;
   return GM_OK;
 }
@@ -570,6 +608,7 @@ int gm_scale_uniform(gm_ctx* c, int r) {
   c->r = r;
   c->rlo = c->rhi = r;
   c->meshPending = false;
+  c->rpValid = false;
   return GM_OK;
 }
 
@@ -598,6 +637,7 @@ int gm_scale_grid_i8(gm_ctx* c, const int8_t* grid) {
   c->rlo = lo;
   c->rhi = hi;
   c->meshPending = false;
+  c->rpValid = false;
   return GM_OK;
 }
 
@@ -626,6 +666,7 @@ int gm_scale_lattice_f32(gm_ctx* c, const float* lat, int L) {
   auto fl = [](long long v) { return v >= 0 ? v / 256 : -((-v + 255) / 256); };
   c->L = L;
   c->rmode = 2;
+  c->rpValid = false;
   c->rlo = (int)std::max<long long>(2, std::min<long long>(127, fl(mn)));
   c->rhi = (int)std::max<long long>(2, std::min<long long>(127, fl(mx)));
   c->meshPending = false;

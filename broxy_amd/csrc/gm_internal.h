@@ -115,13 +115,40 @@ struct MorphBuffers {
   u32* queue;        // active-tile queues of the streaming passes
   int queueCap;      // tiles per queue list (the most tiles of any pass)
   int* qsel;         // counter block of the next pass (flips every pass)
+  // Per-voxel radii with the sphere (null otherwise): the radius value v of
+  // every voxel as 5 bit planes (plane k = bit k, planes one grid apart,
+  // each brick-major: the 64 words of a 1 word x 8 rows x 8 planes brick
+  // contiguous), the set of values present per brick (u32, bit v), the
+  // source grid of one erosion radius (K masked to r = v) and the bricks
+  // holding K ([0] count, then brick ids).
+  u32* rp;
+  u32* rvset;
+  u32* kv;
+  u32* kbl;
+  int rpBuild;  // rebuild rp and rvset from the radius field first
 };
-// u32 words of the active-tile queues for base resolution R.
-inline size_t tile_queue_words(int R) {
+// Active-tile queues for base resolution R (u32 words): two blocks of 16
+// counters (uniform radius) at word 0, two blocks of 32 x 8 counters (one
+// per radius value and XCD queue) at kVarCounters, then from kQueueLists the
+// 8 lists of every tile (uniform radius) and the 32 x 8 lists of
+// var_queue_cap tiles (per radius value). The counters do not move with R.
+constexpr int kVarCounters = 32;
+constexpr int kQueueLists = 544;
+inline size_t tile_count_max(int R) {
   const size_t N = 2 * (size_t)R, W = N / 32;
-  const size_t tiles = W * ((N + kTileRows - 1) / kTileRows) * ((N + kTileZ - 1) / kTileZ);
-  return 32 + 8 * tiles;
+  return W * ((N + kTileRows - 1) / kTileRows) * ((N + kTileZ - 1) / kTileZ);
 }
+inline size_t var_queue_base(int R) { return kQueueLists + 8 * tile_count_max(R); }
+// Tiles of one per-XCD queue: each run of 8 slabs puts one slab in every queue.
+inline size_t var_queue_cap(int R) {
+  const size_t N = 2 * (size_t)R, W = N / 32;
+  const size_t slabs = ((N + kTileRows - 1) / kTileRows) * ((N + kTileZ - 1) / kTileZ);
+  return (slabs + 7) / 8 * W;
+}
+inline size_t tile_queue_words(int R) { return var_queue_base(R) + 256 * var_queue_cap(R); }
+// Device words of the per-voxel radius buffers of base resolution R.
+inline size_t radius_plane_words(int R) { return 5 * (size_t)R * R * R / 4; }
+inline size_t brick_words(int R) { return (size_t)R * R * R / 256; }  // (N/8)^2 * W bricks
 struct SETableSet {
   // offsets (in u32) into MorphBuffers::tables for each rho, -1 if absent
   int dil[129];
@@ -138,6 +165,9 @@ hipError_t launch_brick_summary(const u32* G, int N, uint8_t* out, u32* clear, h
 hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf,
                         const SETableSet& tabs, int rho_lo, int rho_hi, int tabWords, hipStream_t s,
                         hipEvent_t* ev);
+// Radius planes and per-brick value sets of a per-voxel radius field
+// (values clamped to [2,31]; the caller has checked the range).
+hipError_t launch_radius_planes(const RadiusField& rf, int N, u32* rp, u32* rvset, hipStream_t s);
 
 // Host check that the chord tables compiled into the streaming kernels equal
 // the ones build_se_table produces.

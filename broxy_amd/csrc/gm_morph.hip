@@ -35,6 +35,7 @@
 // Radii that vary per voxel are handled by one pass per distinct radius
 // found in the tile: gather radius (dilation) masks the output voxels,
 // scatter radius (erosion) masks the source voxels.
+#include <algorithm>
 #include <vector>
 
 #include "gm_morph_dev.h"
@@ -75,9 +76,11 @@ __global__ void k_tight(MorphArgs a) {
   }
   u32 valid = cell_range_mask(wx, 0, a.R - 1);
   if (cy > a.R - 2 || cz > a.R - 2) valid = 0;
-  a.outFlags[gid] = (u16)(any & ~all & valid);
-  // D outside the tight contour: DF ? 1 : I (the fine pass rewrites TG words)
-  const u32 df = expand2(flag_word(a.DF, a.R, a.W, cy, cz, wx));
+  const u32 tg = any & ~all & valid;
+  a.outFlags[gid] = (u16)tg;
+  // D outside the tight contour: DF ? 1 : I; on it: I (the fine pass ORs in
+  // F, which contains I)
+  const u32 df = expand2(flag_word(a.DF, a.R, a.W, cy, cz, wx) & ~tg);
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     const size_t o = ((size_t)(2 * cz + (s >> 1)) * a.N + 2 * cy + (s & 1)) * a.W + wx;
@@ -122,7 +125,8 @@ __global__ void __launch_bounds__(512) k_brick_summary(const u32* __restrict__ G
 // Outer contour K of D (Grid.cu:1230-1327) and its brick summary: voxels
 // outside D, 6-adjacent to D, in a cell that is not full, away from the data
 // border. Same workgroup mapping as k_brick_summary.
-__global__ void __launch_bounds__(512) k_contour_K(MorphArgs a, uint8_t* __restrict__ brickK) {
+// kbl (optional): the bricks holding K are appended to it ([0] count).
+__global__ void __launch_bounds__(512) k_contour_K(MorphArgs a, uint8_t* __restrict__ brickK, u32* kbl) {
   __shared__ u32 anyW[64];
   const int N = a.N, W = a.W, NB = N / 8;
   const int by = blockIdx.x, bz = blockIdx.y;
@@ -161,7 +165,128 @@ __global__ void __launch_bounds__(512) k_contour_K(MorphArgs a, uint8_t* __restr
   }
   if (any) atomicOr(&anyW[wx], 1u);
   __syncthreads();
-  if (threadIdx.x < W) brickK[((size_t)bz * NB + by) * W + threadIdx.x] = (uint8_t)anyW[threadIdx.x];
+  if (threadIdx.x < W) {
+    const size_t b = ((size_t)bz * NB + by) * W + threadIdx.x;
+    brickK[b] = (uint8_t)anyW[threadIdx.x];
+    if (kbl && anyW[threadIdx.x]) kbl[1 + atomicAdd(kbl, 1u)] = (u32)b;
+  }
+}
+
+// Radius value v in [2,31] of every voxel as 5 bit planes (plane k at
+// rp + k*N*N*W holds bit k, brick-major layout) and the set of values of each brick
+// (vset, bit v). Same workgroup mapping as k_brick_summary. Lattice fields
+// follow radius_at: trilinear in fixed point, floor, clamped; the interpolant
+// along x is (s - fx) A(ix) + fx A(ix+1), A the y/z-weighted column sums, and
+// the division by s^3 * 256 (a power of two) is an arithmetic shift.
+__global__ void __launch_bounds__(512) k_radius_planes(RadiusField rf, int N, int W, u32* __restrict__ rp,
+                                                       u32* __restrict__ vset) {
+  __shared__ u32 setW[64];
+  const int NB = N / 8;
+  const int by = blockIdx.x, bz = blockIdx.y;
+  const int w = threadIdx.x % W, yl = threadIdx.x / W;
+  if (threadIdx.x < W) setW[threadIdx.x] = 0u;
+  __syncthreads();
+  const int y = 8 * by + yl;
+  const size_t plane = (size_t)N * N * W;
+  u32 set = 0u;
+  for (int zl = 0; zl < 8; ++zl) {
+    const int z = 8 * bz + zl;
+    u32 p[5] = {0u, 0u, 0u, 0u, 0u};
+    if (rf.mode == 1) {
+      const int8_t* row = rf.grid + ((size_t)z * N + y) * (size_t)N + 32 * w;
+      const int4 q0 = reinterpret_cast<const int4*>(row)[0];
+      const int4 q1 = reinterpret_cast<const int4*>(row)[1];
+      const int qw[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+#pragma unroll
+      for (int b = 0; b < 32; ++b) {
+        int v = (int)(int8_t)((u32)qw[b >> 2] >> (8 * (b & 3)));
+        v = v < 2 ? 2 : (v > 31 ? 31 : v);
+        set |= 1u << v;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) p[k] |= (u32)((v >> k) & 1) << b;
+      }
+    } else {
+      const int ls = rf.lshift, L = rf.L;
+      const long long s = rf.s;
+      const int iy = y >> ls, iz = z >> ls;
+      const long long fy = y - (iy << ls), fz = z - (iz << ls);
+      const int y0 = min(iy, L - 1), y1 = min(iy + 1, L - 1), z0 = min(iz, L - 1), z1 = min(iz + 1, L - 1);
+      const long long w00 = (s - fy) * (s - fz), w10 = fy * (s - fz), w01 = (s - fy) * fz, w11 = fy * fz;
+      auto colsum = [&](int ix) {
+        const int xx = min(ix, L - 1);
+        const int32_t* l = rf.lat + xx;
+        return w00 * l[(size_t)L * (y0 + (size_t)L * z0)] + w10 * l[(size_t)L * (y1 + (size_t)L * z0)] +
+               w01 * l[(size_t)L * (y0 + (size_t)L * z1)] + w11 * l[(size_t)L * (y1 + (size_t)L * z1)];
+      };
+      const int sh = 3 * ls + 8;
+      int cur = -1;
+      long long A0 = 0, A1 = 0;
+      for (int b = 0; b < 32; ++b) {
+        const int x = 32 * w + b;
+        const int ix = x >> ls;
+        if (ix != cur) {  // same b for every lane: 32w is a multiple of s or s > 32
+          A0 = colsum(ix);
+          A1 = colsum(ix + 1);
+          cur = ix;
+        }
+        const long long fx = x - (ix << ls);
+        long long v = ((s - fx) * A0 + fx * A1) >> sh;
+        v = v < 2 ? 2 : (v > 31 ? 31 : v);
+        set |= 1u << (int)v;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) p[k] |= (u32)((v >> k) & 1) << b;
+      }
+    }
+    const size_t o = brick_major(N, W, y, z, w);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) rp[k * plane + o] = p[k];
+  }
+  atomicOr(&setW[w], set);
+  __syncthreads();
+  if (threadIdx.x < W) vset[((size_t)bz * NB + by) * W + threadIdx.x] = setW[threadIdx.x];
+}
+
+// DF = I-full for every cell word (the per-value coarse dilation ORs its
+// launches into it): one thread per cell-row word.
+__global__ void k_ifull(const u32* __restrict__ I, int R, int N, int W, u16* __restrict__ DF) {
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long long)R * R * W) return;
+  const int wx = (int)(gid % W);
+  const long long t = gid / W;
+  const int cy = (int)(t % R), cz = (int)(t / R);
+  const u32* r = I + ((size_t)(2 * cz) * N + 2 * cy) * W + wx;
+  const size_t pz = (size_t)N * W;
+  const u32 f = r[0] & r[W] & r[pz] & r[pz + W];
+  DF[gid] = (u16)gather2(f & (f >> 1));
+}
+
+// Clears the masked-source grid of the per-value erosion and the K brick list.
+__global__ void k_kv_clear(uint4* __restrict__ kv, size_t n16, u32* kbl) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) kbl[0] = 0u;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+    kv[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// Source of the erosion launch of radius value v: kv = K & (r == v) on the
+// bricks of K whose value set holds v, or held vprev (the previous launch's
+// value, whose bits are cleared here). Every other word of kv is zero. One
+// wave per listed brick (lane = row + 8 * plane); a grid of about one wave
+// per brick keeps the loads of all bricks in flight at once.
+__global__ void __launch_bounds__(256) k_kv(const u32* __restrict__ K, const u32* __restrict__ rp,
+                                            const u32* __restrict__ vset, const u32* __restrict__ kbl,
+                                            u32* __restrict__ kv, int N, int W, int v, int vprev) {
+  const int NB = N / 8;
+  const int lane = lane_id();
+  const u32 n = kbl[0];
+  const u32 want = (1u << v) | (vprev >= 0 ? 1u << vprev : 0u);
+  for (u32 i = blockIdx.x * 4u + (threadIdx.x >> 6); i < n; i += gridDim.x * 4u) {
+    const u32 b = kbl[1 + i];
+    if (!(vset[b] & want)) continue;
+    const int w = (int)(b % (u32)W), t = (int)(b / (u32)W);
+    const int y = 8 * (t % NB) + (lane & 7), z = 8 * (t / NB) + (lane >> 3);
+    const size_t o = ((size_t)z * N + y) * W + w;
+    kv[o] = K[o] & radius_eq(rp, N, W, y, z, w, v);
+  }
 }
 
 // C' rows of a cell row (4 fine rows), masked by CL.
@@ -183,11 +308,16 @@ __global__ void k_base_contour(MorphArgs a) {
   if (cy < 1 || cy > a.R - 2 || cz < 1 || cz > a.R - 2) valid = 0;
   u32 c[4], l[4], r[4], o[4];
   cprime_rows(a, cy, cz, wx, c);
-  // C outside the base contour is C' (the fine pass rewrites UG words)
+  // C outside the base contour is C'; on it all ones (the fine pass clears Ef)
+  auto put = [&](u32 ug) {
+    a.outFlags[gid] = (u16)ug;
+    const u32 ones = expand2(ug);
 #pragma unroll
-  for (int s = 0; s < 4; ++s) a.outRows[((size_t)(2 * cz + (s >> 1)) * a.N + 2 * cy + (s & 1)) * a.W + wx] = c[s];
+    for (int s = 0; s < 4; ++s)
+      a.outRows[((size_t)(2 * cz + (s >> 1)) * a.N + 2 * cy + (s & 1)) * a.W + wx] = c[s] | ones;
+  };
   if (!valid) {
-    a.outFlags[gid] = 0;
+    put(0u);
     return;
   }
   cprime_rows(a, cy, cz, wx - 1, l);
@@ -206,7 +336,7 @@ __global__ void k_base_contour(MorphArgs a) {
 #pragma unroll
     for (int s = 0; s < 4; ++s) diff |= c[s] ^ o[s];
   }
-  a.outFlags[gid] = (u16)(gather2(diff | (diff >> 1)) & valid);
+  put(gather2(diff | (diff >> 1)) & valid);
 }
 
 // LDS bytes of a streaming launch with `waves` waves per workgroup.
@@ -247,6 +377,8 @@ MorphArgs base_args(const MorphDims& d, const MorphBuffers& b, const RadiusField
   a.brickK = b.brickK;
   a.queue = b.queue;
   a.queueCap = b.queueCap;
+  a.vqBase = (int)var_queue_base(d.R);
+  a.vqCap = (int)var_queue_cap(d.R);
   return a;
 }
 
@@ -298,6 +430,10 @@ hipError_t launch_pass(int pass, const void* generic, MorphArgs a, int* qsel, bo
   const void* sel[4] = {reinterpret_cast<const void*>(&k_select<0>), reinterpret_cast<const void*>(&k_select<1>),
                         reinterpret_cast<const void*>(&k_select<2>), reinterpret_cast<const void*>(&k_select<3>)};
   const unsigned tiles = stream_grid(a).x;
+  a.rvar = 0;
+  a.qcOff = 16 * a.qsel;
+  a.qlOff = kQueueList;
+  a.qCap = a.queueCap;
   void* args[] = {&a};
   GM_CHECK(hipLaunchKernel(sel[pass], dim3((tiles + 3) / 4), dim3(256), args, 0, s));
   const int nw = stream_waves(a, var);
@@ -309,7 +445,63 @@ hipError_t launch_pass(int pass, const void* generic, MorphArgs a, int* qsel, bo
   return hipGetLastError();
 }
 
+// Streaming pass `pass` with per-voxel radii (sphere): one tile selection
+// queues every tile under each radius value that can reach it, then one
+// launch per value v in [vlo, vhi] of the kernel compiled for the chord-table
+// radius of v (dilation v + 1, erosion v - 1). Erosion launches read the
+// source kv, rebuilt for each value first (*vprev: the value kv holds).
+hipError_t launch_pass_var(int pass, MorphArgs a, const MorphBuffers& b, int vlo, int vhi, int* vprev,
+                           hipStream_t s) {
+  a.qsel = *b.qsel;
+  *b.qsel = 1 - *b.qsel;
+  a.rvar = 1;
+  a.rp = b.rp;
+  a.rvset = b.rvset;
+  const void* sel[4] = {reinterpret_cast<const void*>(&k_select<0>), reinterpret_cast<const void*>(&k_select<1>),
+                        reinterpret_cast<const void*>(&k_select<2>), reinterpret_cast<const void*>(&k_select<3>)};
+  const unsigned tiles = stream_grid(a).x;
+  {
+    void* args[] = {&a};
+    GM_CHECK(hipLaunchKernel(sel[pass], dim3((tiles + 3) / 4), dim3(256), args, 0, s));
+  }
+  const bool ero = pass >= 2;
+  // k_kv: 4 waves per workgroup, about one wave per brick of K (bounded)
+  const unsigned bricks = (unsigned)((a.N / 8) * (a.N / 8) * a.W);
+  const unsigned kvGrid = std::max(64u, std::min(bricks / 32u, 16384u));
+  // a value reaches few tiles: fewer persistent workgroups than tiles
+  const unsigned grid = std::min((tiles + 7u) & ~7u, 2048u);
+  for (int v = vlo; v <= vhi; ++v) {
+    const int rho = ero ? v - 1 : v + 1;
+    if (rho < 1 || rho > kCtMaxRho || !ct_kernels().fn[pass][rho]) return hipErrorInvalidValue;
+    MorphArgs c = a;
+    c.vcur = v;
+    c.qcOff = kVarCounters + 256 * a.qsel + 8 * v;
+    c.qlOff = a.vqBase + 8 * v * a.vqCap;
+    c.qCap = a.vqCap;
+    c.tabWords = 0;
+    c.maxm = ero ? rho : rho - 1;
+    c.maxoff = rho;
+    if (ero) {
+      hipLaunchKernelGGL(k_kv, dim3(kvGrid), dim3(256), 0, s, b.K, b.rp, b.rvset, b.kbl, b.kv, a.N, a.W, v, *vprev);
+      GM_CHECK(hipGetLastError());
+      *vprev = v;
+      c.K = b.kv;
+    }
+    const int nw = stream_waves(c, false);
+    void* args[] = {&c};
+    GM_CHECK(hipLaunchKernel(ct_kernels().fn[pass][rho], dim3(grid), dim3(64 * nw), args, lds_bytes(c, false, nw),
+                             s));
+  }
+  return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t launch_radius_planes(const RadiusField& rf, int N, u32* rp, u32* rvset, hipStream_t s) {
+  if (rf.mode != 1 && rf.mode != 2) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_radius_planes, dim3(N / 8, N / 8), dim3(8 * (N / 32)), 0, s, rf, N, N / 32, rp, rvset);
+  return hipGetLastError();
+}
 
 hipError_t launch_brick_summary(const u32* G, int N, uint8_t* out, u32* clear, hipStream_t s) {
   hipLaunchKernelGGL(k_brick_summary, dim3(N / 8, N / 8), dim3(8 * (N / 32)), 0, s, G, N, N / 32, out, clear);
@@ -317,11 +509,12 @@ hipError_t launch_brick_summary(const u32* G, int N, uint8_t* out, u32* clear, h
 }
 
 // rho_lo/rho_hi: range of the per-voxel radius r over the field; tabWords:
-// longest SE table of the launch.
+// longest SE table of the launch. A per-voxel radius field with the sphere
+// (b.rp set) runs one launch per radius value; the cube, with any field, runs
+// the runtime-table kernels.
 hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf, int se,
                          const SETableSet& tabs, int rho_lo, int rho_hi, int tabWords, hipStream_t s,
                          hipEvent_t* ev) {
-  (void)rho_lo;
   MorphArgs a = base_args(d, b, rf, tabs);
   a.se = se;
   const int rmax = rho_hi + (se == 0 ? 1 : 0);
@@ -330,17 +523,27 @@ hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const Radius
   a.maxoff = se == 0 ? rmax : rho_hi + 1;
   a.tabWords = tabWords;
   const bool var = rf.mode != 0;
+  const bool perValue = var && se == 0 && b.rp;
   long long cellwords = (long long)d.R * d.R * d.W;
   if (ev) GM_CHECK(hipEventRecord(ev[0], s));
+  if (perValue && b.rpBuild) GM_CHECK(launch_radius_planes(rf, d.N, b.rp, b.rvset, s));
   hipLaunchKernelGGL(k_brick_summary, dim3(d.N / 8, d.N / 8), dim3(8 * d.W), 0, s, b.I, d.N, d.W, b.brickI,
                      nullptr);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
   const bool fixedOk = !var && se == 0;
   const int rho = (rf.r & 0x7f) + 1;
+  const int vlo = std::max(rho_lo, 2), vhi = rho_hi;
+  int vprev = -1;
   a.ystep = 2;
   a.outFlags = b.DF;
-  GM_CHECK(launch_pass(0, reinterpret_cast<const void*>(&k_dil_coarse), a, b.qsel, var, fixedOk, rho, s));
+  if (perValue) {
+    hipLaunchKernelGGL(k_ifull, dim3((unsigned)((cellwords + 255) / 256)), dim3(256), 0, s, b.I, d.R, d.N, d.W,
+                       b.DF);
+    GM_CHECK(hipGetLastError());
+  }
+  if (perValue) GM_CHECK(launch_pass_var(0, a, b, vlo, vhi, &vprev, s));
+  else GM_CHECK(launch_pass(0, reinterpret_cast<const void*>(&k_dil_coarse), a, b.qsel, var, fixedOk, rho, s));
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   a.outFlags = b.TG;
   a.outRows = b.D;
@@ -349,14 +552,14 @@ hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const Radius
   if (ev) GM_CHECK(hipEventRecord(ev[3], s));
   a.ystep = 1;
   a.outRows = b.D;
-  GM_CHECK(launch_pass(1, reinterpret_cast<const void*>(&k_dil_fine), a, b.qsel, var, fixedOk, rho, s));
+  if (perValue) GM_CHECK(launch_pass_var(1, a, b, vlo, vhi, &vprev, s));
+  else GM_CHECK(launch_pass(1, reinterpret_cast<const void*>(&k_dil_fine), a, b.qsel, var, fixedOk, rho, s));
   if (ev) GM_CHECK(hipEventRecord(ev[4], s));
   return hipSuccess;
 }
 
 hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf, const SETableSet& tabs,
                         int rho_lo, int rho_hi, int tabWords, hipStream_t s, hipEvent_t* ev) {
-  (void)rho_lo;
   MorphArgs a = base_args(d, b, rf, tabs);
   int rmax = rho_hi - 1;  // erosion radius r-1
   if (rmax < 1) rmax = 1;
@@ -365,17 +568,28 @@ hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusF
   a.maxoff = rmax;
   a.tabWords = tabWords;
   const bool var = rf.mode != 0;
+  const bool perValue = var && b.rp;
   long long cellwords = (long long)d.R * d.R * d.W;
   if (ev) GM_CHECK(hipEventRecord(ev[0], s));
+  if (perValue) {
+    if (b.rpBuild) GM_CHECK(launch_radius_planes(rf, d.N, b.rp, b.rvset, s));
+    const size_t n16 = (size_t)d.N * d.N * d.W / 4;
+    hipLaunchKernelGGL(k_kv_clear, dim3(2048), dim3(256), 0, s, reinterpret_cast<uint4*>(b.kv), n16, b.kbl);
+    GM_CHECK(hipGetLastError());
+  }
   a.outRows = b.K;
-  hipLaunchKernelGGL(k_contour_K, dim3(d.N / 8, d.N / 8), dim3(8 * d.W), 0, s, a, b.brickK);
+  hipLaunchKernelGGL(k_contour_K, dim3(d.N / 8, d.N / 8), dim3(8 * d.W), 0, s, a, b.brickK,
+                     perValue ? b.kbl : nullptr);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
   const bool fixedOk = !var;
   const int rho = (rf.r & 0x7f) - 1;
+  const int vlo = std::max(rho_lo, 2), vhi = rho_hi;
+  int vprev = -1;
   a.ystep = 2;
   a.outFlags = b.CL;
-  GM_CHECK(launch_pass(2, reinterpret_cast<const void*>(&k_ero_coarse), a, b.qsel, var, fixedOk, rho, s));
+  if (perValue) GM_CHECK(launch_pass_var(2, a, b, vlo, vhi, &vprev, s));
+  else GM_CHECK(launch_pass(2, reinterpret_cast<const void*>(&k_ero_coarse), a, b.qsel, var, fixedOk, rho, s));
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   a.outFlags = b.UG;
   a.outRows = b.C;
@@ -384,7 +598,8 @@ hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusF
   if (ev) GM_CHECK(hipEventRecord(ev[3], s));
   a.ystep = 1;
   a.outRows = b.C;
-  GM_CHECK(launch_pass(3, reinterpret_cast<const void*>(&k_ero_fine), a, b.qsel, var, fixedOk, rho, s));
+  if (perValue) GM_CHECK(launch_pass_var(3, a, b, vlo, vhi, &vprev, s));
+  else GM_CHECK(launch_pass(3, reinterpret_cast<const void*>(&k_ero_fine), a, b.qsel, var, fixedOk, rho, s));
   if (ev) GM_CHECK(hipEventRecord(ev[4], s));
   return hipSuccess;
 }

@@ -494,6 +494,18 @@ struct MorphArgs {
   u32* queue;
   int queueCap;
   int qsel;
+  // queue a streaming launch drains: 8 counters at word qcOff, 8 lists of
+  // qCap entries from word qlOff
+  int qcOff, qlOff, qCap;
+  // per-value queues: counter blocks of 32 values x 8 queues at
+  // kVarCounters + 256*qsel, lists of vqCap entries at vqBase + (8v + x)*vqCap
+  int vqBase, vqCap;
+  // per-voxel radii by radius value (rvar != 0): radius planes, per-brick
+  // value sets, and the value vcur a streaming launch serves
+  const u32* rp;
+  const u32* rvset;
+  int rvar;
+  int vcur;
 };
 
 struct Lds {
@@ -627,6 +639,54 @@ __device__ void rho_set_erode(const RadiusField& rf, const u32* K, int N, int W,
   __syncthreads();
 }
 
+// Word of row (y, z, w) in the brick-major layout of the radius planes: the
+// 64 words of a brick (1 word x 8 rows x 8 planes) are contiguous, row
+// fastest, bricks in grid order.
+__device__ __forceinline__ size_t brick_major(int N, int W, int y, int z, int w) {
+  return (((size_t)(z >> 3) * (N >> 3) + (y >> 3)) * W + w) * 64 + (z & 7) * 8 + (y & 7);
+}
+// Bits of row word (y, z, w) whose radius value is v (radius planes rp; the
+// address is clamped, callers mask rows outside the grid).
+__device__ __forceinline__ u32 radius_eq(const u32* __restrict__ rp, int N, int W, int y, int z, int w, int v) {
+  const int yc = min(max(y, 0), N - 1), zc = min(max(z, 0), N - 1);
+  const size_t plane = (size_t)N * N * W;
+  const u32* p = rp + brick_major(N, W, yc, zc, w);
+  u32 b[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) b[k] = p[k * plane];
+  u32 m = ~0u;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) m &= ((v >> k) & 1) ? b[k] : ~b[k];
+  return m;
+}
+
+// Radius values present in the bricks of rows [y0,y1), planes [z0,z1), words
+// [w0,w1] (clamped to the grid); with gate, only bricks whose gate bit0 is set.
+__device__ u32 wave_bricks_vset(const u32* vset, const uint8_t* gate, int N, int W, int y0, int y1, int z0, int z1,
+                                int w0, int w1) {
+  const int NB = N / 8;
+  const int by0 = max(y0, 0) >> 3, by1 = (min(y1, N) - 1) >> 3;
+  const int bz0 = max(z0, 0) >> 3, bz1 = (min(z1, N) - 1) >> 3;
+  w0 = max(w0, 0);
+  w1 = min(w1, W - 1);
+  const int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1, nw = w1 - w0 + 1;
+  u32 found = 0u;
+  if (ny > 0 && nz > 0 && nw > 0) {
+    const int n = ny * nz * nw;
+    for (int i0 = 0; i0 < n; i0 += 64) {
+      const int i = min(i0 + lane_id(), n - 1);
+      const int w = w0 + i % nw, t = i / nw;
+      const int by = by0 + t % ny, bz = bz0 + t / ny;
+      const size_t b = ((size_t)bz * NB + by) * W + w;
+      const u32 v = vset[b];
+      found |= (gate == nullptr || (gate[b] & 1u)) ? v : 0u;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) found |= (u32)__shfl_xor((int)found, off);
+  return found;
+}
+
 // Runs the predicate of a pass over the tile and calls fin(zo, F) for every
 // output plane of the thread's row (F = 0 when skip is set). which: 0
 // dilation table (radius+add), 1 erosion corner, 2 erosion centre (radius-1).
@@ -704,7 +764,7 @@ __device__ void run_tile(const MorphArgs& a, Lds L, u32* set4, const TileGeo& g,
   }
 }
 
-constexpr int kQueueList = 32;  // first word of the queue lists
+constexpr int kQueueList = kQueueLists;  // first word of the uniform-radius lists
 
 __device__ __forceinline__ int tile_count(const MorphArgs& a) {
   const int rows = a.ystep * kTileRows;
@@ -738,6 +798,12 @@ __device__ __forceinline__ TileGeo tile_at(const MorphArgs& a, int j, int yoff, 
 // Each epilogue loads what its output word needs (load), then combines it
 // with the tile result (store); skipped() writes the output of a tile whose
 // result is known without the predicate (nothing to load).
+// With per-voxel radii (a->rvar) one launch serves one radius value: the
+// coarse outputs were initialised before the launches and every launch ORs
+// in its bits (dilation: output voxels of value vcur; erosion: the source is
+// K restricted to that value); the fine outputs start from their default
+// (D: I on tagged cells; C: all ones on tagged cells) and accumulate the same
+// way. A uniform radius is the one-launch case of the same formulas.
 struct DilCoarseFin {  // DF(c) = I(c) full | (c < dR & F(2c))
   const MorphArgs* a;
   int w;
@@ -745,40 +811,51 @@ struct DilCoarseFin {  // DF(c) = I(c) full | (c < dR & F(2c))
   u32 xin;
   u16 skipValue;  // 0: no voxel in reach (I empty here); 0xffff: every cell full
   struct In {
-    u32 full;
+    u32 full;  // uniform: I-full bits of the 4 rows; per value: DF so far
+    u32 m;     // per value: output voxels of value vcur
   };
-  __device__ void load(int z, In& in) const {
+  __device__ u32 ifull(int z) const {
     const u32* I = a->I;
-    in.full = ld_row(I, a->N, a->W, y, z, w) & ld_row(I, a->N, a->W, y + 1, z, w) &
-              ld_row(I, a->N, a->W, y, z + 1, w) & ld_row(I, a->N, a->W, y + 1, z + 1, w);
+    const u32 f = ld_row(I, a->N, a->W, y, z, w) & ld_row(I, a->N, a->W, y + 1, z, w) &
+                  ld_row(I, a->N, a->W, y, z + 1, w) & ld_row(I, a->N, a->W, y + 1, z + 1, w);
+    return gather2(f & (f >> 1));
+  }
+  __device__ size_t at(int z) const { return ((size_t)(z >> 1) * a->R + (y >> 1)) * a->W + w; }
+  __device__ void load(int z, In& in) const {
+    if (a->rvar) {
+      in.full = ld_flag(a->outFlags, a->R, a->W, y >> 1, z >> 1, w);
+      in.m = radius_eq(a->rp, a->N, a->W, y, z, w, a->vcur);
+    } else {
+      in.full = ifull(z);
+      in.m = ~0u;
+    }
   }
   __device__ void store(int z, u32 f, const In& in) const {
     if (y >= a->N) return;
     const int cy = y >> 1, cz = z >> 1;
-    const u32 ifull = gather2(in.full & (in.full >> 1));
     const u32 inr = (cy < a->dR1 && cz < a->dR2) ? xin : 0u;
-    a->outFlags[((size_t)cz * a->R + cy) * a->W + w] = (u16)(ifull | (gather2(f) & inr));
+    a->outFlags[at(z)] = (u16)(in.full | (gather2(f & in.m) & inr));
   }
   __device__ void skipped(int z) const {
     if (y >= a->N) return;
-    a->outFlags[((size_t)(z >> 1) * a->R + (y >> 1)) * a->W + w] = skipValue;
+    a->outFlags[at(z)] = skipValue;
   }
 };
-struct DilFineFin {  // D = TG ? F : D, D already holds DF ? 1 : I
+struct DilFineFin {  // D |= TG & F (& value mask); D holds TG ? I : (DF ? 1 : I)
   const MorphArgs* a;
   int w;
   int y;
   struct In {
-    u32 tg, d;
+    u32 tg, d, m;
   };
   __device__ void load(int z, In& in) const {
     in.tg = ld_flag(a->TG, a->R, a->W, y >> 1, z >> 1, w);
     in.d = ld_row(a->outRows, a->N, a->W, y, z, w);
+    in.m = a->rvar ? radius_eq(a->rp, a->N, a->W, y, z, w, a->vcur) : ~0u;
   }
   __device__ void store(int z, u32 f, const In& in) const {
     if (y >= a->N) return;
-    const u32 tg = expand2(in.tg);
-    a->outRows[((size_t)z * a->N + y) * a->W + w] = (tg & f) | (~tg & in.d);
+    a->outRows[((size_t)z * a->N + y) * a->W + w] = in.d | (expand2(in.tg) & f & in.m);
   }
   __device__ void skipped(int) const {}
 };
@@ -788,28 +865,30 @@ struct EroCoarseFin {  // CL(c) = c < dR & D(c) != 0 & I(c) != 0xff & Ec(2c+1)
   int y;  // odd row 2cy+1
   u32 xin;
   struct In {
-    u32 dany, full;
+    u32 dany, full, old;
   };
+  __device__ size_t at(int z) const { return ((size_t)(z >> 1) * a->R + (y >> 1)) * a->W + w; }
   __device__ void load(int z, In& in) const {
     const int y0 = y - 1, z0 = z - 1, N = a->N, W = a->W;
     in.dany = ld_row(a->D, N, W, y0, z0, w) | ld_row(a->D, N, W, y0 + 1, z0, w) | ld_row(a->D, N, W, y0, z0 + 1, w) |
               ld_row(a->D, N, W, y0 + 1, z0 + 1, w);
     in.full = ld_row(a->I, N, W, y0, z0, w) & ld_row(a->I, N, W, y0 + 1, z0, w) & ld_row(a->I, N, W, y0, z0 + 1, w) &
               ld_row(a->I, N, W, y0 + 1, z0 + 1, w);
+    in.old = a->rvar ? ld_flag(a->outFlags, a->R, a->W, y >> 1, z >> 1, w) : 0u;
   }
   __device__ void store(int z, u32 e, const In& in) const {
     if (y >= a->N) return;
     const int cy = y >> 1, cz = z >> 1;
     const u32 inr = (cy < a->dR1 && cz < a->dR2) ? xin : 0u;
-    a->outFlags[((size_t)cz * a->R + cy) * a->W + w] =
-        (u16)(gather2(e >> 1) & gather2(in.dany | (in.dany >> 1)) & ~gather2(in.full & (in.full >> 1)) & inr);
+    a->outFlags[at(z)] = (u16)(in.old | (gather2(e >> 1) & gather2(in.dany | (in.dany >> 1)) &
+                                         ~gather2(in.full & (in.full >> 1)) & inr));
   }
   __device__ void skipped(int z) const {  // no K in reach: E = 0
     if (y >= a->N) return;
-    a->outFlags[((size_t)(z >> 1) * a->R + (y >> 1)) * a->W + w] = 0;
+    a->outFlags[at(z)] = 0;
   }
 };
-struct EroFineFin {  // C = UG ? ~Ef : C, C already holds C' = D & ~CL
+struct EroFineFin {  // C &= ~(UG & Ef); C holds C' = D & ~CL, all ones on UG cells
   const MorphArgs* a;
   int w;
   int y;
@@ -822,8 +901,7 @@ struct EroFineFin {  // C = UG ? ~Ef : C, C already holds C' = D & ~CL
   }
   __device__ void store(int z, u32 e, const In& in) const {
     if (y >= a->N) return;
-    const u32 ug = expand2(in.ug);
-    a->outRows[((size_t)z * a->N + y) * a->W + w] = (ug & ~e) | (~ug & in.c);
+    a->outRows[((size_t)z * a->N + y) * a->W + w] = in.c & ~(expand2(in.ug) & e);
   }
   __device__ void skipped(int) const {}
 };
@@ -843,41 +921,70 @@ __device__ __forceinline__ TileGeo pass_tile(const MorphArgs& a, int t) {
 // Tile selection, one wave per tile: a tile whose result is known without the
 // predicate is finished here (coarse passes store their constant; the fine
 // passes' default output is already written), every other tile is appended to
-// the queue of its slab.
+// the queue of its slab. With per-voxel radii the tile goes to the queue of
+// every radius value that can reach it (per-brick value sets; erosion: of the
+// bricks holding K); the coarse erosion output of every tile is cleared here
+// for the per-value launches to OR into (the coarse dilation output is set
+// to I-full beforehand).
 template <int PASS>
 __device__ void select_tiles(const MorphArgs& a) {
-  if (blockIdx.x == 0 && threadIdx.x < 16) a.queue[16 * (1 - a.qsel) + threadIdx.x] = 0u;
+  if (blockIdx.x == 0) {
+    // the counter blocks of the next pass (the previous pass has finished)
+    if (threadIdx.x < 16) a.queue[16 * (1 - a.qsel) + threadIdx.x] = 0u;
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) a.queue[kVarCounters + 256 * (1 - a.qsel) + i] = 0u;
+  }
   u32* qc = a.queue + 16 * a.qsel;
   const int t = blockIdx.x * (int)(blockDim.x >> 6) + wave_id();
   if (t >= tile_count(a)) return;
   const TileGeo g = pass_tile<PASS>(a, t);
   const int lane = lane_id();
+  const int y1 = g.yfirst + a.ystep * kTileRows;
   bool skip;
+  u32 vals = 0u;  // radius values to queue the tile for (per-voxel radii)
   if constexpr (PASS == 0) {
-    const int y1 = g.yfirst + a.ystep * kTileRows;
     // F = 0 when no occupied voxel is in reach; F unused when every cell is full
     const bool empty = !wave_bricks_any(a.brickI, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff,
                                         g.zfirst - a.maxoff, g.zlast + 2 + a.maxoff, g.wx - 1, g.wx + 1, 0);
     const bool full = !empty && wave_bricks_all(a.brickI, a.N, a.W, g.yfirst, y1, g.zfirst, g.zlast + 2, g.wx);
     skip = empty || full;
-    if (skip) {
+    if (a.rvar) {
+      // DF already holds I-full, which is also the value of a skipped tile
+      if (!skip) vals = wave_bricks_vset(a.rvset, nullptr, a.N, a.W, g.yfirst, y1, g.zfirst, g.zlast + 1, g.wx, g.wx);
+    } else if (skip) {
       DilCoarseFin fin{&a, g.wx, g.yfirst + 2 * lane, 0u, (u16)(full ? 0xffffu : 0u)};
       for (int p = 0; p < g.nout(); ++p) fin.skipped(g.zfirst + p * g.zstep);
     }
   } else if constexpr (PASS == 2) {
-    const int y1 = g.yfirst + a.ystep * kTileRows;
     skip = !wave_bricks_any(a.brickK, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff, g.zfirst - a.maxoff,
                             g.zlast + 1 + a.maxoff, g.wx - 1, g.wx + 1, 0);
-    if (skip) {
+    if (skip || a.rvar) {
       EroCoarseFin fin{&a, g.wx, g.yfirst + 2 * lane, 0u};
       for (int p = 0; p < g.nout(); ++p) fin.skipped(g.zfirst + p * g.zstep);
     }
+    if (!skip && a.rvar)
+      vals = wave_bricks_vset(a.rvset, a.brickK, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff, g.zfirst - a.maxoff,
+                              g.zlast + 1 + a.maxoff, g.wx - 1, g.wx + 1);
   } else {
     skip = !wave_tile_has_flags(PASS == 1 ? a.TG : a.UG, a.R, a.W, g.wx, g.yfirst, g.yfirst + kTileRows, g.zfirst,
                                 g.zlast + 1);
+    if (!skip && a.rvar) {
+      if constexpr (PASS == 1)
+        vals = wave_bricks_vset(a.rvset, nullptr, a.N, a.W, g.yfirst, y1, g.zfirst, g.zlast + 1, g.wx, g.wx);
+      else
+        vals = wave_bricks_vset(a.rvset, a.brickK, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff,
+                                g.zfirst - a.maxoff, g.zlast + 1 + a.maxoff, g.wx - 1, g.wx + 1);
+    }
   }
-  if (!skip && lane == 0) {
-    const int x = slab_queue(t / a.W);
+  if (skip) return;
+  const int x = slab_queue(t / a.W);
+  if (a.rvar) {
+    // lane v queues the tile for radius value v
+    if (lane < 32 && ((vals >> lane) & 1u)) {
+      const int q = lane * 8 + x;
+      const u32 k = atomicAdd(&a.queue[kVarCounters + 256 * a.qsel + q], 1u);
+      a.queue[a.vqBase + (size_t)q * a.vqCap + k] = (u32)t;
+    }
+  } else if (lane == 0) {
     const u32 k = atomicAdd(&qc[x], 1u);
     a.queue[kQueueList + x * a.queueCap + k] = (u32)t;
   }
@@ -917,9 +1024,10 @@ __device__ void run_pass(const MorphArgs& a) {
   extern __shared__ u32 smem[];
   __shared__ u32 set4[4];
   const int x = (int)(blockIdx.x & 7u);
-  const u32* qc = a.queue + 16 * a.qsel;
+  const u32* qc = a.queue + a.qcOff;
+  const u32* ql = a.queue + a.qlOff + (size_t)x * a.qCap;
   for (u32 k = blockIdx.x >> 3; k < qc[x]; k += gridDim.x >> 3) {
-    run_pass_tile<PASS, RHO>(a, (int)a.queue[kQueueList + x * a.queueCap + k], smem, set4);
+    run_pass_tile<PASS, RHO>(a, (int)ql[k], smem, set4);
     __syncthreads();  // LDS is reused by the next tile
   }
 }

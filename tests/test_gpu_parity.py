@@ -135,6 +135,65 @@ def test_scale_lattice(ctx):
     assert np.array_equal(ctx.get_cells(B.GRID_CLOSING), O.erode(I, Dref, R, scale=sc))
 
 
+def test_scale_lattice_wide_clamped(ctx):
+    # radii 2..23 over a grid that is not a multiple of the tile sizes, with
+    # a data region smaller than the grid on two axes
+    R = 80
+    N = 2 * R
+    I = random_blobs(R, seed=8, nballs=24, rmin=3.0, rmax=14.0)
+    dr = [72, R, 61]
+    grid_setup(ctx, R, I, data_res=dr)
+    L = 10
+    lat = B.scale_lattice(L, seed=12, lo=2.0, hi=24.0, smooth=0)
+    ctx.scale_lattice(lat, L)
+    sc = O.scale_from_lattice(lat.reshape(L, L, L), N)
+    assert sc.min() == 2 and sc.max() >= 20
+    say("lattice radii 2..23, R=80, data region", dr)
+    ctx.close_grid(B.SE_SPHERE)
+    Dref = O.dilate(I, R, scale=sc, data_res=dr)
+    assert np.array_equal(ctx.get_cells(B.GRID_DILATION), Dref)
+    assert np.array_equal(ctx.get_cells(B.GRID_CLOSING), O.erode(I, Dref, R, scale=sc, data_res=dr))
+
+
+def test_scale_grid_cube(ctx):
+    R = 32
+    N = 2 * R
+    I = random_blobs(R, seed=9)
+    grid_setup(ctx, R, I)
+    rng = np.random.default_rng(13)
+    sc = rng.integers(0, 6, N ** 3).astype(np.int8)
+    ctx.scale_grid(sc)
+    say("per-voxel int8 radii, cube dilation")
+    ctx.close_grid(B.SE_CUBE)
+    Dref = O.dilate(I, R, scale=sc, se=1)
+    assert np.array_equal(ctx.get_cells(B.GRID_DILATION), Dref)
+    assert np.array_equal(ctx.get_cells(B.GRID_CLOSING), O.erode(I, Dref, R, scale=sc))
+
+
+def test_scale_field_changes(ctx):
+    # a new field (or a uniform radius) replaces the radius planes of the last one
+    R = 48
+    N = 2 * R
+    I = random_blobs(R, seed=10)
+    grid_setup(ctx, R, I)
+    L = 6
+    for seed, lo, hi in ((21, 2.0, 9.0), (22, 5.0, 14.0)):
+        lat = B.scale_lattice(L, seed=seed, lo=lo, hi=hi, smooth=1)
+        ctx.scale_lattice(lat, L)
+        sc = O.scale_from_lattice(lat.reshape(L, L, L), N)
+        say("lattice seed", seed)
+        ctx.dilate(B.SE_SPHERE)
+        Dref = O.dilate(I, R, scale=sc)
+        assert np.array_equal(ctx.get_cells(B.GRID_DILATION), Dref)
+        ctx.erode()
+        assert np.array_equal(ctx.get_cells(B.GRID_CLOSING), O.erode(I, Dref, R, scale=sc))
+    ctx.scale_uniform(6)
+    say("uniform after lattice")
+    ctx.close_grid(B.SE_SPHERE)
+    Dref = O.dilate(I, R, r=6)
+    assert np.array_equal(ctx.get_cells(B.GRID_CLOSING), O.erode(I, Dref, R, r=6))
+
+
 def test_voxelize_soup(ctx):
     P, T = B.sphere_soup(8, 3, seed=99)
     R = 64

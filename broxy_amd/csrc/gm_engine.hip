@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "../../include/gmorpho_c.h"
+#include "gm_internal.h"
 
 namespace {
 
@@ -67,7 +68,12 @@ uint64_t context_bytes(int R, int L, const gm_job& j) {
   b += 2 * (P / 8) * (P / 8) * (P / 8) * 4;       // MC block counts
   b += (uint64_t)R * R * R * 4;                   // MC cell bases
   b += 4 * N * N * 24 + 8 * N * N * 12;           // mesh output (initial capacity)
-  if (L > 0) b += (uint64_t)L * L * L * 4;        // radius lattice
+  b += gm::tile_queue_words(R) * 4;               // active-tile queues
+  if (L > 0) {
+    b += (uint64_t)L * L * L * 4;                         // radius lattice
+    b += (gm::radius_plane_words(R) + (uint64_t)R * R * R / 4) * 4;  // radius planes, masked K
+    b += 2 * (gm::brick_words(R) + 1) * 4;                // value sets, K brick list
+  }
   b += (uint64_t)j.num_of_vertices * 24 + (uint64_t)j.num_of_faces * 12;
   return b + (16ull << 20);                       // tables, scans, slack
 }

@@ -77,10 +77,14 @@ __device__ __forceinline__ u32 shl_k(u32 L, u32 C, int k) {
 
 // Output geometry of one streaming tile: lane i of every wave owns row
 // yfirst + ystep*i; output planes are zfirst, zfirst + zstep, ... <= zlast.
+// [rlo, rhi]: the planes that can contribute, of the output (rout != 0: a
+// mask keeps outputs of those planes only) or of the input (the source is
+// zero elsewhere); the compiled streams skip every other input plane.
 struct TileGeo {
   int wx;
   int yfirst, ystep;
   int tz0, zfirst, zlast, zstep;
+  int rlo, rhi, rout;
   __device__ int nout() const { return zlast < zfirst ? 0 : (zlast - zfirst) / zstep + 1; }
 };
 
@@ -384,7 +388,8 @@ __device__ void stream_ct(const GridSrc& src, u32* gtAll, int gtStride, u32* acc
   const int ybase = g.yfirst - HY;
   u32* gt = gtAll + wv * gtStride;
   const u32* gl = gt + lane * C::M1;
-  const int zbeg = g.zfirst + dzl, zend = g.zlast + dzh;
+  const int zbeg = max(g.zfirst + dzl, g.rout ? g.rlo + dzl : g.rlo);
+  const int zend = min(g.zlast + dzh, g.rout ? g.rhi + dzh : g.rhi);
   const int zshift = YS == 2 ? 1 : 0;
   static_assert(ct_monotone<C>(), "chord half-widths must not grow with |dz|");
   static_assert((size_t)C::M1 * (YS == 1 ? C::GR : 2 * C::GRH) <= (size_t)ct_slice_words(C::MAXM, C::HY, YS),
@@ -687,6 +692,43 @@ __device__ u32 wave_bricks_vset(const u32* vset, const uint8_t* gate, int N, int
   return found;
 }
 
+// wave_bricks_vset plane by plane: the union of the value sets, and in lane
+// v (< 32) the first and last brick plane (bz) holding value v (lo > hi when
+// none does).
+__device__ u32 wave_vset_planes(const u32* vset, const uint8_t* gate, int N, int W, int y0, int y1, int z0, int z1,
+                                int w0, int w1, int& lo, int& hi) {
+  const int NB = N / 8;
+  const int by0 = max(y0, 0) >> 3, by1 = (min(y1, N) - 1) >> 3;
+  const int bz0 = max(z0, 0) >> 3, bz1 = (min(z1, N) - 1) >> 3;
+  w0 = max(w0, 0);
+  w1 = min(w1, W - 1);
+  const int ny = by1 - by0 + 1, nw = w1 - w0 + 1;
+  const int lane = lane_id();
+  lo = 1 << 20;
+  hi = -1;
+  u32 all = 0u;
+  if (ny <= 0 || nw <= 0) return 0u;
+  const int n = ny * nw;
+  for (int bz = bz0; bz <= bz1; ++bz) {
+    u32 found = 0u;
+    for (int i0 = 0; i0 < n; i0 += 64) {
+      const int i = min(i0 + lane, n - 1);
+      const int w = w0 + i % nw, by = by0 + i / nw;
+      const size_t b = ((size_t)bz * NB + by) * W + w;
+      const u32 v = vset[b];
+      found |= (gate == nullptr || (gate[b] & 1u)) ? v : 0u;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) found |= (u32)__shfl_xor((int)found, off);
+    all |= found;
+    if ((found >> (lane & 31)) & 1u) {
+      lo = min(lo, bz);
+      hi = max(hi, bz);
+    }
+  }
+  return all;
+}
+
 // Runs the predicate of a pass over the tile and calls fin(zo, F) for every
 // output plane of the thread's row (F = 0 when skip is set). which: 0
 // dilation table (radius+add), 1 erosion corner, 2 erosion centre (radius-1).
@@ -790,7 +832,26 @@ __device__ __forceinline__ TileGeo tile_at(const MorphArgs& a, int j, int yoff, 
   int zl = min(g.tz0 + a.Zt, a.N) - 1;
   g.zlast = zl < g.zfirst ? g.zfirst - zstep : g.zfirst + ((zl - g.zfirst) / zstep) * zstep;
   g.zstep = zstep;
+  g.rlo = -(1 << 20);
+  g.rhi = 1 << 20;
+  g.rout = 0;
   return g;
+}
+
+// Queue entries: tile id in the low kEntryTileBits bits, then the plane range
+// [rlo, rhi] that can contribute, 7 bits each, relative to tz0 - kEntryZBase
+// (clamped, which only widens the range).
+constexpr int kEntryTileBits = 18;
+constexpr int kEntryZBase = 40;
+__device__ __forceinline__ u32 entry_pack(int t, int tz0, int rlo, int rhi) {
+  const int lo = min(max(rlo - tz0 + kEntryZBase, 0), 127), hi = min(max(rhi - tz0 + kEntryZBase, 0), 127);
+  return (u32)t | ((u32)lo << kEntryTileBits) | ((u32)hi << (kEntryTileBits + 7));
+}
+__device__ __forceinline__ int entry_tile(u32 e) { return (int)(e & ((1u << kEntryTileBits) - 1u)); }
+__device__ __forceinline__ void entry_range(u32 e, TileGeo& g, int rout) {
+  g.rlo = (int)((e >> kEntryTileBits) & 127u) + g.tz0 - kEntryZBase;
+  g.rhi = (int)((e >> (kEntryTileBits + 7)) & 127u) + g.tz0 - kEntryZBase;
+  g.rout = rout;
 }
 
 // ---------------------------------------------------------------------------
@@ -941,6 +1002,7 @@ __device__ void select_tiles(const MorphArgs& a) {
   const int y1 = g.yfirst + a.ystep * kTileRows;
   bool skip;
   u32 vals = 0u;  // radius values to queue the tile for (per-voxel radii)
+  int blo = 0, bhi = -1;  // lane v: brick planes holding value v
   if constexpr (PASS == 0) {
     // F = 0 when no occupied voxel is in reach; F unused when every cell is full
     const bool empty = !wave_bricks_any(a.brickI, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff,
@@ -949,7 +1011,8 @@ __device__ void select_tiles(const MorphArgs& a) {
     skip = empty || full;
     if (a.rvar) {
       // DF already holds I-full, which is also the value of a skipped tile
-      if (!skip) vals = wave_bricks_vset(a.rvset, nullptr, a.N, a.W, g.yfirst, y1, g.zfirst, g.zlast + 1, g.wx, g.wx);
+      if (!skip)
+        vals = wave_vset_planes(a.rvset, nullptr, a.N, a.W, g.yfirst, y1, g.zfirst, g.zlast + 1, g.wx, g.wx, blo, bhi);
     } else if (skip) {
       DilCoarseFin fin{&a, g.wx, g.yfirst + 2 * lane, 0u, (u16)(full ? 0xffffu : 0u)};
       for (int p = 0; p < g.nout(); ++p) fin.skipped(g.zfirst + p * g.zstep);
@@ -962,31 +1025,38 @@ __device__ void select_tiles(const MorphArgs& a) {
       for (int p = 0; p < g.nout(); ++p) fin.skipped(g.zfirst + p * g.zstep);
     }
     if (!skip && a.rvar)
-      vals = wave_bricks_vset(a.rvset, a.brickK, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff, g.zfirst - a.maxoff,
-                              g.zlast + 1 + a.maxoff, g.wx - 1, g.wx + 1);
+      vals = wave_vset_planes(a.rvset, a.brickK, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff, g.zfirst - a.maxoff,
+                              g.zlast + 1 + a.maxoff, g.wx - 1, g.wx + 1, blo, bhi);
   } else {
     skip = !wave_tile_has_flags(PASS == 1 ? a.TG : a.UG, a.R, a.W, g.wx, g.yfirst, g.yfirst + kTileRows, g.zfirst,
                                 g.zlast + 1);
     if (!skip && a.rvar) {
       if constexpr (PASS == 1)
-        vals = wave_bricks_vset(a.rvset, nullptr, a.N, a.W, g.yfirst, y1, g.zfirst, g.zlast + 1, g.wx, g.wx);
+        vals = wave_vset_planes(a.rvset, nullptr, a.N, a.W, g.yfirst, y1, g.zfirst, g.zlast + 1, g.wx, g.wx, blo, bhi);
       else
-        vals = wave_bricks_vset(a.rvset, a.brickK, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff,
-                                g.zfirst - a.maxoff, g.zlast + 1 + a.maxoff, g.wx - 1, g.wx + 1);
+        vals = wave_vset_planes(a.rvset, a.brickK, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff,
+                                g.zfirst - a.maxoff, g.zlast + 1 + a.maxoff, g.wx - 1, g.wx + 1, blo, bhi);
     }
   }
   if (skip) return;
   const int x = slab_queue(t / a.W);
   if (a.rvar) {
-    // lane v queues the tile for radius value v
+    // lane v queues the tile for radius value v, with the planes that can
+    // contribute: the output planes of value v (dilation, outputs masked by
+    // value) or the input planes of K of value v (erosion, source masked)
     if (lane < 32 && ((vals >> lane) & 1u)) {
+      int rlo = 8 * blo, rhi = 8 * bhi + 7;
+      if constexpr (PASS < 2) {
+        rlo = max(rlo, g.zfirst);
+        rhi = min(rhi, g.zlast);
+      }
       const int q = lane * 8 + x;
       const u32 k = atomicAdd(&a.queue[kVarCounters + 256 * a.qsel + q], 1u);
-      a.queue[a.vqBase + (size_t)q * a.vqCap + k] = (u32)t;
+      a.queue[a.vqBase + (size_t)q * a.vqCap + k] = entry_pack(t, g.tz0, rlo, rhi);
     }
   } else if (lane == 0) {
     const u32 k = atomicAdd(&qc[x], 1u);
-    a.queue[kQueueList + x * a.queueCap + k] = (u32)t;
+    a.queue[kQueueList + x * a.queueCap + k] = entry_pack(t, g.tz0, -(1 << 20), 1 << 20);
   }
 }
 
@@ -994,8 +1064,9 @@ __device__ void select_tiles(const MorphArgs& a) {
 // with the compiled chord table of that radius; RHO = 0: runtime tables
 // (uniform or per-voxel radii, sphere or cube).
 template <int PASS, int RHO>
-__device__ __forceinline__ void run_pass_tile(const MorphArgs& a, int t, u32* smem, u32* set4) {
-  const TileGeo g = pass_tile<PASS>(a, t);
+__device__ __forceinline__ void run_pass_tile(const MorphArgs& a, u32 entry, u32* smem, u32* set4) {
+  TileGeo g = pass_tile<PASS>(a, entry_tile(entry));
+  entry_range(entry, g, PASS < 2 ? 1 : 0);
   Lds L = carve(smem, a, g.nout());
   if constexpr (PASS == 0) {
     DilCoarseFin fin{&a, g.wx, g.yfirst + 2 * lane_id(), cell_range_mask(g.wx, 0, a.dR0), 0};
@@ -1027,7 +1098,7 @@ __device__ void run_pass(const MorphArgs& a) {
   const u32* qc = a.queue + a.qcOff;
   const u32* ql = a.queue + a.qlOff + (size_t)x * a.qCap;
   for (u32 k = blockIdx.x >> 3; k < qc[x]; k += gridDim.x >> 3) {
-    run_pass_tile<PASS, RHO>(a, (int)ql[k], smem, set4);
+    run_pass_tile<PASS, RHO>(a, ql[k], smem, set4);
     __syncthreads();  // LDS is reused by the next tile
   }
 }

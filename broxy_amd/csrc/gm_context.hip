@@ -3,6 +3,7 @@
 // (GMorpho.cu:7731-7845, 6814-7001) on one HIP stream.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -45,6 +46,7 @@ struct gm_ctx {
   int8_t* dGrid = nullptr;
   size_t capGrid = 0;  // bytes of dGrid
   int32_t* dLat = nullptr;
+  long long latMaxAbs = 0;  // largest |fixed-point lattice value|
   // per-voxel radii by value: radius planes, per-brick value sets, masked
   // erosion source, K brick list (sized for capVarR)
   u32 *rp = nullptr, *rvset = nullptr, *kv = nullptr, *kbl = nullptr;
@@ -248,6 +250,7 @@ RadiusField radius_field(const gm_ctx* c) {
   f.s = c->L ? c->N / c->L : 1;
   f.lshift = 0;
   while ((1 << f.lshift) < f.s) ++f.lshift;
+  f.narrow = (double)f.s * f.s * f.s * (double)c->latMaxAbs < (double)(1 << 30) ? 1 : 0;
   return f;
 }
 
@@ -665,6 +668,7 @@ int gm_scale_lattice_f32(gm_ctx* c, const float* lat, int L) {
   HIPCALL(c, hipStreamSynchronize(c->stream));
   auto fl = [](long long v) { return v >= 0 ? v / 256 : -((-v + 255) / 256); };
   c->L = L;
+  c->latMaxAbs = std::max(std::llabs(mn), std::llabs(mx));
   c->rmode = 2;
   c->rpValid = false;
   c->rlo = (int)std::max<long long>(2, std::min<long long>(127, fl(mn)));

@@ -35,6 +35,7 @@ struct RadiusField {
   int L;
   int s;      // N / L (power of two)
   int lshift; // log2(s)
+  int narrow; // s^3 * max |lat| < 2^30: lattice sums fit 32-bit integers
 };
 
 __device__ __forceinline__ int radius_at(const RadiusField& f, int x, int y, int z) {

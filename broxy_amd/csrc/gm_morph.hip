@@ -172,78 +172,93 @@ __global__ void __launch_bounds__(512) k_contour_K(MorphArgs a, uint8_t* __restr
   }
 }
 
+// Radius values of the 32 voxels of row word (y, z, w) of a lattice field
+// into the bit planes p and the value set: T is int when s^3 times the
+// largest lattice magnitude fits 31 bits (rf.narrow), else long long.
+template <class T>
+__device__ __forceinline__ void lattice_word(const RadiusField& rf, int y, int z, int w, u32 p[5], u32& set) {
+  const int ls = rf.lshift, L = rf.L;
+  const T s = (T)rf.s;
+  const int iy = y >> ls, iz = z >> ls;
+  const T fy = (T)(y - (iy << ls)), fz = (T)(z - (iz << ls));
+  const int y0 = min(iy, L - 1), y1 = min(iy + 1, L - 1), z0 = min(iz, L - 1), z1 = min(iz + 1, L - 1);
+  const T w00 = (s - fy) * (s - fz), w10 = fy * (s - fz), w01 = (s - fy) * fz, w11 = fy * fz;
+  const int32_t* l00 = rf.lat + (size_t)L * (y0 + (size_t)L * z0);
+  const int32_t* l10 = rf.lat + (size_t)L * (y1 + (size_t)L * z0);
+  const int32_t* l01 = rf.lat + (size_t)L * (y0 + (size_t)L * z1);
+  const int32_t* l11 = rf.lat + (size_t)L * (y1 + (size_t)L * z1);
+  auto colsum = [&](int ix) {
+    const int xx = min(ix, L - 1);
+    return w00 * (T)l00[xx] + w10 * (T)l10[xx] + w01 * (T)l01[xx] + w11 * (T)l11[xx];
+  };
+  const int sh = 3 * ls + 8;
+  int cur = -1;
+  T acc = 0, dA = 0;
+#pragma unroll
+  for (int b = 0; b < 32; ++b) {
+    const int x = 32 * w + b;
+    const int ix = x >> ls;
+    if (ix != cur) {  // same b for every lane: 32w is a multiple of s or s > 32
+      const T A0 = colsum(ix), A1 = colsum(ix + 1);
+      const T fx = (T)(x - (ix << ls));
+      acc = (s - fx) * A0 + fx * A1;  // s A0 + fx (A1 - A0), then + (A1 - A0) per voxel
+      dA = A1 - A0;
+      cur = ix;
+    } else {
+      acc += dA;
+    }
+    int v = (int)(acc >> sh);
+    v = min(max(v, 2), 31);
+    set |= 1u << v;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) p[k] |= (u32)((v >> k) & 1) << b;
+  }
+}
+
 // Radius value v in [2,31] of every voxel as 5 bit planes (plane k at
-// rp + k*N*N*W holds bit k, brick-major layout) and the set of values of each brick
-// (vset, bit v). Same workgroup mapping as k_brick_summary. Lattice fields
-// follow radius_at: trilinear in fixed point, floor, clamped; the interpolant
-// along x is (s - fx) A(ix) + fx A(ix+1), A the y/z-weighted column sums, and
-// the division by s^3 * 256 (a power of two) is an arithmetic shift.
-__global__ void __launch_bounds__(512) k_radius_planes(RadiusField rf, int N, int W, u32* __restrict__ rp,
+// rp + k*N*N*W holds bit k, brick-major layout) and the set of values of
+// each brick (vset, bit v). Lattice fields follow radius_at: trilinear in
+// fixed point, floor, clamped; the interpolant along x is
+// (s - fx) A(ix) + fx A(ix+1), A the y/z-weighted column sums, and the
+// division by s^3 * 256 (a power of two) is an arithmetic shift.
+__global__ void __launch_bounds__(256) k_radius_planes(RadiusField rf, int N, int W, u32* __restrict__ rp,
                                                        u32* __restrict__ vset) {
-  __shared__ u32 setW[64];
   const int NB = N / 8;
-  const int by = blockIdx.x, bz = blockIdx.y;
-  const int w = threadIdx.x % W, yl = threadIdx.x / W;
-  if (threadIdx.x < W) setW[threadIdx.x] = 0u;
-  __syncthreads();
-  const int y = 8 * by + yl;
+  const size_t nbricks = (size_t)NB * NB * W;
+  // one wave per brick, lane = row + 8 * plane: the 5 plane stores of a wave
+  // are each one contiguous 256-byte brick
+  const size_t b = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (b >= nbricks) return;
+  const int lane = lane_id();
+  const int w = (int)(b % W);
+  const size_t t = b / W;
+  const int y = 8 * (int)(t % NB) + (lane & 7), z = 8 * (int)(t / NB) + (lane >> 3);
   const size_t plane = (size_t)N * N * W;
   u32 set = 0u;
-  for (int zl = 0; zl < 8; ++zl) {
-    const int z = 8 * bz + zl;
-    u32 p[5] = {0u, 0u, 0u, 0u, 0u};
-    if (rf.mode == 1) {
-      const int8_t* row = rf.grid + ((size_t)z * N + y) * (size_t)N + 32 * w;
-      const int4 q0 = reinterpret_cast<const int4*>(row)[0];
-      const int4 q1 = reinterpret_cast<const int4*>(row)[1];
-      const int qw[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+  u32 p[5] = {0u, 0u, 0u, 0u, 0u};
+  if (rf.mode == 1) {
+    const int8_t* row = rf.grid + ((size_t)z * N + y) * (size_t)N + 32 * w;
+    const int4 q0 = reinterpret_cast<const int4*>(row)[0];
+    const int4 q1 = reinterpret_cast<const int4*>(row)[1];
+    const int qw[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
 #pragma unroll
-      for (int b = 0; b < 32; ++b) {
-        int v = (int)(int8_t)((u32)qw[b >> 2] >> (8 * (b & 3)));
-        v = v < 2 ? 2 : (v > 31 ? 31 : v);
-        set |= 1u << v;
+    for (int k = 0; k < 32; ++k) {
+      int v = (int)(int8_t)((u32)qw[k >> 2] >> (8 * (k & 3)));
+      v = v < 2 ? 2 : (v > 31 ? 31 : v);
+      set |= 1u << v;
 #pragma unroll
-        for (int k = 0; k < 5; ++k) p[k] |= (u32)((v >> k) & 1) << b;
-      }
-    } else {
-      const int ls = rf.lshift, L = rf.L;
-      const long long s = rf.s;
-      const int iy = y >> ls, iz = z >> ls;
-      const long long fy = y - (iy << ls), fz = z - (iz << ls);
-      const int y0 = min(iy, L - 1), y1 = min(iy + 1, L - 1), z0 = min(iz, L - 1), z1 = min(iz + 1, L - 1);
-      const long long w00 = (s - fy) * (s - fz), w10 = fy * (s - fz), w01 = (s - fy) * fz, w11 = fy * fz;
-      auto colsum = [&](int ix) {
-        const int xx = min(ix, L - 1);
-        const int32_t* l = rf.lat + xx;
-        return w00 * l[(size_t)L * (y0 + (size_t)L * z0)] + w10 * l[(size_t)L * (y1 + (size_t)L * z0)] +
-               w01 * l[(size_t)L * (y0 + (size_t)L * z1)] + w11 * l[(size_t)L * (y1 + (size_t)L * z1)];
-      };
-      const int sh = 3 * ls + 8;
-      int cur = -1;
-      long long A0 = 0, A1 = 0;
-      for (int b = 0; b < 32; ++b) {
-        const int x = 32 * w + b;
-        const int ix = x >> ls;
-        if (ix != cur) {  // same b for every lane: 32w is a multiple of s or s > 32
-          A0 = colsum(ix);
-          A1 = colsum(ix + 1);
-          cur = ix;
-        }
-        const long long fx = x - (ix << ls);
-        long long v = ((s - fx) * A0 + fx * A1) >> sh;
-        v = v < 2 ? 2 : (v > 31 ? 31 : v);
-        set |= 1u << (int)v;
-#pragma unroll
-        for (int k = 0; k < 5; ++k) p[k] |= (u32)((v >> k) & 1) << b;
-      }
+      for (int j = 0; j < 5; ++j) p[j] |= (u32)((v >> j) & 1) << k;
     }
-    const size_t o = brick_major(N, W, y, z, w);
-#pragma unroll
-    for (int k = 0; k < 5; ++k) rp[k * plane + o] = p[k];
+  } else if (rf.narrow) {
+    lattice_word<int>(rf, y, z, w, p, set);
+  } else {
+    lattice_word<long long>(rf, y, z, w, p, set);
   }
-  atomicOr(&setW[w], set);
-  __syncthreads();
-  if (threadIdx.x < W) vset[((size_t)bz * NB + by) * W + threadIdx.x] = setW[threadIdx.x];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) rp[j * plane + b * 64 + lane] = p[j];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) set |= (u32)__shfl_xor((int)set, off);
+  if (lane == 0) vset[b] = set;
 }
 
 // DF = I-full for every cell word (the per-value coarse dilation ORs its
@@ -499,7 +514,8 @@ hipError_t launch_pass_var(int pass, MorphArgs a, const MorphBuffers& b, int vlo
 
 hipError_t launch_radius_planes(const RadiusField& rf, int N, u32* rp, u32* rvset, hipStream_t s) {
   if (rf.mode != 1 && rf.mode != 2) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_radius_planes, dim3(N / 8, N / 8), dim3(8 * (N / 32)), 0, s, rf, N, N / 32, rp, rvset);
+  const size_t bricks = (size_t)(N / 8) * (N / 8) * (N / 32);
+  hipLaunchKernelGGL(k_radius_planes, dim3((unsigned)((bricks + 3) / 4)), dim3(256), 0, s, rf, N, N / 32, rp, rvset);
   return hipGetLastError();
 }
 

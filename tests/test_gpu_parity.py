@@ -155,6 +155,26 @@ def test_scale_lattice_wide_clamped(ctx):
     assert np.array_equal(ctx.get_cells(B.GRID_CLOSING), O.erode(I, Dref, R, scale=sc, data_res=dr))
 
 
+def test_scale_lattice_large_magnitudes(ctx):
+    # lattice values far below zero (radius 2 there): the interpolation sums
+    # exceed 32 bits and take the 64-bit path
+    R = 48
+    N = 2 * R
+    I = random_blobs(R, seed=14, nballs=10, rmin=3.0, rmax=10.0)
+    grid_setup(ctx, R, I)
+    L = 6
+    rng = np.random.default_rng(15)
+    lat = rng.uniform(2.0, 16.0, L ** 3).astype(np.float32)
+    lat[rng.random(L ** 3) < 0.3] = -4.0e6
+    ctx.scale_lattice(lat, L)
+    sc = O.scale_from_lattice(lat.reshape(L, L, L), N)
+    say("lattice with large negative values")
+    ctx.close_grid(B.SE_SPHERE)
+    Dref = O.dilate(I, R, scale=sc)
+    assert np.array_equal(ctx.get_cells(B.GRID_DILATION), Dref)
+    assert np.array_equal(ctx.get_cells(B.GRID_CLOSING), O.erode(I, Dref, R, scale=sc))
+
+
 def test_scale_grid_cube(ctx):
     R = 32
     N = 2 * R

@@ -30,8 +30,8 @@ KERNEL_OF_STAGE = {"brick_summary": "k_brick_summary", "dilate_coarse": "k_pass_
                    "tight_contour": "k_tight", "dilate_fine": "k_pass_ct<1,rho>", "contour_K": "k_contour_K",
                    "erode_coarse": "k_pass_ct<2,rho>", "base_contour": "k_base_contour",
                    "erode_fine": "k_pass_ct<3,rho>", "mc_count": "k_mc_count",
-                   "mc_scan": "k_scan_reduce+k_scan_top+k_scan_down", "mc_verts": "k_mc_verts",
-                   "mc_tris": "k_mc_tris", "mc_fixup_normalize": "k_mc_fixup+k_mc_normalize"}
+                   "mc_scan": "k_scan_reduce+k_scan_top+k_scan_down", "mc_emit": "k_mc_emit",
+                   "mc_fixup_normalize": "k_mc_fixup+k_mc_normalize"}
 
 
 # per-voxel radii run the runtime-table kernels
@@ -56,8 +56,7 @@ def algorithmic_bytes(stage, N, nv, nt):
         "erode_fine": 3 * g + f,           # read K, UG, C; rewrite C (upper bound)
         "mc_count": g,                     # read C
         "mc_scan": 16.0 * N ** 3 / 4096,   # block counts and bases (2 u32 per 8^3 cells, read + write)
-        "mc_verts": g + 24 * nv,           # read C; write positions, normal sums
-        "mc_tris": g + 12 * nt,            # read C; write triangles
+        "mc_emit": g + 24 * nv + 12 * nt,  # read C; write positions, normal sums, triangles
         "mc_fixup_normalize": 24 * nv,     # normals read + write
     }[stage]
 

@@ -219,7 +219,7 @@ class Context:
         self._call("gm_set_timing", 1 if on else 0)
 
     STAGES = ("brick_summary", "dilate_coarse", "tight_contour", "dilate_fine", "contour_K", "erode_coarse",
-              "base_contour", "erode_fine", "mc_count", "mc_scan", "mc_verts", "mc_tris", "mc_fixup_normalize")
+              "base_contour", "erode_fine", "mc_count", "mc_scan", "mc_emit", "mc_fixup_normalize")
 
     def stage_times(self):
         """Device ms per stage (one kernel each, see STAGES) of the last

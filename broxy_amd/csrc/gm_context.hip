@@ -823,13 +823,13 @@ int gm_set_timing(gm_ctx* c, int enable) {
   return GM_OK;
 }
 
-constexpr int kNumStages = 13;
+constexpr int kNumStages = 12;
 int gm_stage_times(gm_ctx* c, float* ms, int n) {
   if (!c || !ms) return GM_ERR_ARG;
   HIPCALL(c, hipStreamSynchronize(c->stream));
-  // events: dilate 0..4, erode 5..9, mc 10..15 (one stage per kernel or scan)
-  const int pairs[kNumStages][2] = {{0, 1}, {1, 2}, {2, 3},   {3, 4},   {5, 6},   {6, 7},  {7, 8},
-                                    {8, 9}, {10, 11}, {11, 12}, {12, 13}, {13, 14}, {14, 15}};
+  // events: dilate 0..4, erode 5..9, mc 10..14 (one stage per kernel or scan)
+  const int pairs[kNumStages][2] = {{0, 1}, {1, 2}, {2, 3},   {3, 4},   {5, 6},   {6, 7},
+                                    {7, 8}, {8, 9}, {10, 11}, {11, 12}, {12, 13}, {13, 14}};
   for (int i = 0; i < n && i < kNumStages; ++i) {
     ms[i] = -1.f;
     bool ok = (i < 8) ? (c->evValid & 1) : (c->evValid & 2);

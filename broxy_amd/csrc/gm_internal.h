@@ -196,7 +196,7 @@ void ct_register_part7(CtKernels&);
 struct MCBuffers {
   u32* blockV;   // per 8^3 Morton block: vertex count, then exclusive offset
   u32* blockT;
-  u32* cellV;    // dense R^3: vertex base of cells in M
+  u32* cellV;    // dense R^3: block-local vertex base of cells with vertices
   u32* cellT;
   float* V;
   float* Nrm;

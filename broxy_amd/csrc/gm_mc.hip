@@ -11,13 +11,15 @@
 // aligned 8x8x8 block of cells (512 consecutive Morton codes) and stages the
 // closed grid around it in LDS, so the reference's tag / iota / copy_if /
 // sort_by_key / scan chain becomes
-//   mc_count   per block vertex/triangle totals (uniform blocks exit early)
+//   mc_blocks  list of the blocks that can hold a mixed cube
+//   mc_count   per block vertex/triangle totals, and the block-local vertex
+//              base of every cell with vertices
 //   scan       exclusive scan of the block totals (reduce, top, down-sweep)
-//   mc_verts   cube cases in LDS, block scan over cells, then one thread per
-//              cube slot: vertex positions, area-weighted normal sums
-//              gathered from the <= 4 cubes around each vertex, and the
-//              per-cell vertex base for neighbouring blocks
-//   mc_tris    same staging, triangles with neighbour vertex ids
+//   mc_emit    cube cases in LDS, block scan over cells, then one thread per
+//              output slot: vertex positions with area-weighted normal sums
+//              gathered from the <= 4 cubes around each vertex, then
+//              triangles (vertex ids of other blocks' edges from their block
+//              base + cell base)
 //   mc_fixup   triangles the count table does not cover, mc_normalize.
 // The per-cell vertex base map is dense (R^3 u32) but only written and read
 // at surface cells, replacing the reference's 4*N^3-byte memset + splat.
@@ -284,10 +286,12 @@ __global__ void __launch_bounds__(256) k_mc_blocks(const uint8_t* __restrict__ b
 // list (each ends when the list does).
 __global__ void __launch_bounds__(kMcThreads) k_mc_count(const u32* __restrict__ C, int R,
                                                           const u32* __restrict__ list, u32* blockV,
-                                                          u32* blockT) {
+                                                          u32* blockT, u32* cellL, u32* late) {
   __shared__ Box bx;
   __shared__ u32 red[16];
   __shared__ McTab tab;
+  // the late-triangle count starts from zero in the emit kernel that follows
+  if (blockIdx.x == 0 && threadIdx.x == 0) late[0] = 0u;
   load_mc_tab(tab);
   const u32 n = list[0];
   for (u32 it = blockIdx.x; it < n; it += gridDim.x) {
@@ -306,7 +310,7 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_count(const u32* __restrict__
     const bool anyOne = __syncthreads_or(o != 0u) != 0;
     const bool anyZero = __syncthreads_or(a != 0x1ffffu) != 0;
     const bool nonuni = anyOne && anyZero;
-    u32 nv = 0, nt = 0;
+    u32 nvh[2] = {0u, 0u}, nt = 0;
     if (nonuni) {
       for (int h = 0; h < 2; ++h) {
         int cx, cy, cz;
@@ -315,14 +319,22 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_count(const u32* __restrict__
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
           int v = cube_case(bx, 2 * cx + (j & 1), 2 * cy + ((j >> 1) & 1), 2 * cz + ((j >> 2) & 1));
-          nv += own_verts(v);
+          nvh[h] += own_verts(v);
           nt += tab.ntri[v];
         }
       }
     }
     u32 tv, tt;
-    block_exscan(nv, red, &tv);
+    const u32 off = block_exscan(nvh[0] + nvh[1], red, &tv);
     block_exscan(nt, red, &tt);
+    // block-local vertex base of the cells with vertices (cells are in Morton
+    // order, two per thread), read for the edges owned by another block's cube
+    for (int h = 0; h < 2; ++h)
+      if (nvh[h]) {
+        int cx, cy, cz;
+        block_cell(blk, 2 * threadIdx.x + h, cx, cy, cz);
+        cellL[(size_t)cx + (size_t)R * ((size_t)cy + (size_t)R * cz)] = off + (h ? nvh[0] : 0u);
+      }
     if (threadIdx.x == 0) {
       blockV[blk] = tv;
       blockT[blk] = tt;
@@ -534,49 +546,11 @@ __device__ void normal_sum(const McTab& tab, const Geo& geo, const Box& bx, int 
 
 __device__ __forceinline__ u32 prefix_in_cell(const BlockLds& S, int l, int j) { return S.vpre[8 * l + j]; }
 
-// Vertices of a block, in two phases per window of kListCap local vertex
-// ids: the mixed cubes write (cube, owned edge) at their local id, then every
-// thread takes consecutive ids (position + gathered normal sum).
-__device__ void mc_verts_block(BlockLds& S, const u32* __restrict__ C, int R, Geo geo, int blk, int nblocks,
-                               const u32* __restrict__ blockV, u32* cellV, float* V, float* Nrm, u32 capV,
-                               const u32* totals);
-
-__global__ void __launch_bounds__(kMcThreads) k_mc_verts(const u32* __restrict__ C, int R, Geo geo,
-                                                          const u32* __restrict__ list, int nblocks,
-                                                          const u32* __restrict__ blockV, u32* cellV, float* V,
-                                                          float* Nrm, u32 capV, const u32* totals, u32* late) {
-  __shared__ BlockLds S;
-  // the late-triangle count starts from zero in the triangle kernel that follows
-  if (blockIdx.x == 0 && threadIdx.x == 0) late[0] = 0u;
-  load_mc_tab(S.tab);
-  const u32 nlist = list[0];
-  for (u32 it = blockIdx.x; it < nlist; it += gridDim.x) {
-    __syncthreads();  // LDS of the previous block is no longer read
-    mc_verts_block(S, C, R, geo, (int)list[1 + it], nblocks, blockV, cellV, V, Nrm, capV, totals);
-  }
-}
-
-__device__ void mc_verts_block(BlockLds& S, const u32* __restrict__ C, int R, Geo geo, int blk, int nblocks,
-                               const u32* __restrict__ blockV, u32* cellV, float* V, float* Nrm, u32 capV,
-                               const u32* totals) {
-  int c0x, c0y, c0z;
-  block_outside(blk, R, c0x, c0y, c0z);
-  const u32 base = blockV[blk];
-  const u32 next = (blk + 1 < nblocks) ? blockV[blk + 1] : totals[0];
-  if (next == base) return;  // no vertex in this block (uniform across the workgroup)
-  load_box(S.box, C, 2 * R, 2 * c0x - kLo, 2 * c0y - kLo, 2 * c0z - kLo, kBox, kBox);
-  stage_cases(S, blk, R);
-  scan_cells(S);
-  // per-cell vertex base, read by the triangle kernel of neighbouring blocks
-  for (int l = threadIdx.x; l < 512; l += blockDim.x) {
-    u32 nxt = (l + 1 < 512) ? S.vOff[l + 1] : (next - base);
-    if (nxt != S.vOff[l]) {
-      int cx, cy, cz;
-      block_cell(blk, l, cx, cy, cz);
-      cellV[(size_t)cx + (size_t)R * ((size_t)cy + (size_t)R * cz)] = base + S.vOff[l];
-    }
-  }
-  const u32 nverts = next - base;
+// Vertices of a staged block, in two phases per window of kListCap local
+// vertex ids: the mixed cubes write (cube, owned edge) at their local id,
+// then every thread takes consecutive ids (position + gathered normal sum).
+__device__ void mc_verts_block(BlockLds& S, int R, Geo geo, int blk, u32 base, u32 nverts, float* V, float* Nrm,
+                               u32 capV) {
   for (u32 w0 = 0; w0 < nverts; w0 += kListCap) {
     for (int s = threadIdx.x; s < 4096; s += blockDim.x) {
       const int v = S.cases[s];
@@ -622,8 +596,8 @@ __device__ void mc_verts_block(BlockLds& S, const u32* __restrict__ C, int R, Ge
 // Vertex id of local edge e of the cube at fine (fx,fy,fz): 0 when the owning
 // cube's cell is outside [1,R-2]^3 (the reference's absent neighbour).
 __device__ __forceinline__ u32 edge_vid(const BlockLds& S, int blk, int c0x, int c0y, int c0z, u32 base,
-                                        const u32* __restrict__ cellV, int R, int fx, int fy, int fz, int e,
-                                        u32* border) {
+                                        const u32* __restrict__ blockV, const u32* __restrict__ cellL, int R, int fx,
+                                        int fy, int fz, int e, u32* border) {
   int dx, dy, dz, oe;
   own_of(e, dx, dy, dz, oe);
   int gx = fx + dx, gy = fy + dy, gz = fz + dz;
@@ -638,7 +612,9 @@ __device__ __forceinline__ u32 edge_vid(const BlockLds& S, int blk, int c0x, int
     int l = (int)(part1by2((u32)lx) | (part1by2((u32)ly) << 1) | (part1by2((u32)lz) << 2));
     return base + S.vOff[l] + prefix_in_cell(S, l, j) + (u32)rank_of(S.cases[8 * l + j], oe);
   }
-  u32 s = cellV[(size_t)cx + (size_t)R * ((size_t)cy + (size_t)R * cz)];
+  // another block's cell: that block's vertex base + the cell's local base
+  const u32 code = part1by2((u32)cx) | (part1by2((u32)cy) << 1) | (part1by2((u32)cz) << 2);
+  u32 s = blockV[code >> 9] + cellL[(size_t)cx + (size_t)R * ((size_t)cy + (size_t)R * cz)];
   for (int q = 0; q < j; ++q)
     s += own_verts(cube_case(S.box, 2 * cx + (q & 1), 2 * cy + ((q >> 1) & 1), 2 * cz + ((q >> 2) & 1)));
   return s + (u32)rank_of(cube_case(S.box, gx, gy, gz), oe);
@@ -646,44 +622,13 @@ __device__ __forceinline__ u32 edge_vid(const BlockLds& S, int blk, int c0x, int
 
 __device__ __forceinline__ u32 prefix_ntri(const BlockLds& S, int l, int j) { return S.tpre[8 * l + j]; }
 
-// Triangles of a block, in the same two phases as k_mc_verts over local
-// triangle slots. Listed-but-uncounted triangles (len > ntri) go to the late
-// list from the fill phase of the first window.
-__device__ void mc_tris_block(BlockLds& S, const u32* __restrict__ C, int R, int blk, int nblocks,
-                              const u32* __restrict__ blockV, const u32* __restrict__ blockT,
-                              const u32* __restrict__ cellV, u32* T, u32 capT, u32* late, u32 capLate,
-                              u32* totals);
-
-__global__ void __launch_bounds__(kMcThreads) k_mc_tris(const u32* __restrict__ C, int R,
-                                                         const u32* __restrict__ list, int nblocks,
-                                                         const u32* __restrict__ blockV,
-                                                         const u32* __restrict__ blockT,
-                                                         const u32* __restrict__ cellV, u32* T, u32 capT,
-                                                         u32* late, u32 capLate, u32* totals) {
-  __shared__ BlockLds S;
-  load_mc_tab(S.tab);
-  const u32 nlist = list[0];
-  for (u32 it = blockIdx.x; it < nlist; it += gridDim.x) {
-    __syncthreads();  // LDS of the previous block is no longer read
-    mc_tris_block(S, C, R, (int)list[1 + it], nblocks, blockV, blockT, cellV, T, capT, late, capLate, totals);
-  }
-}
-
-__device__ void mc_tris_block(BlockLds& S, const u32* __restrict__ C, int R, int blk, int nblocks,
-                              const u32* __restrict__ blockV, const u32* __restrict__ blockT,
-                              const u32* __restrict__ cellV, u32* T, u32 capT, u32* late, u32 capLate,
-                              u32* totals) {
-  int c0x, c0y, c0z;
-  block_outside(blk, R, c0x, c0y, c0z);
-  const u32 tbase = blockT[blk];
-  const u32 tnext = (blk + 1 < nblocks) ? blockT[blk + 1] : totals[1];
-  if (tnext == tbase) return;
-  const u32 vbase = blockV[blk];
-  load_box(S.box, C, 2 * R, 2 * c0x - kLo, 2 * c0y - kLo, 2 * c0z - kLo, kBox, kBox);
-  stage_cases(S, blk, R);
-  scan_cells(S);
+// Triangles of a staged block, in the same two phases over local triangle
+// slots. Listed-but-uncounted triangles (len > ntri) go to the late list
+// from the fill phase of the first window.
+__device__ void mc_tris_block(BlockLds& S, int R, int blk, int c0x, int c0y, int c0z, u32 vbase, u32 tbase,
+                              u32 ntris, const u32* __restrict__ blockV, const u32* __restrict__ cellL, u32* T,
+                              u32 capT, u32* late, u32 capLate, u32* totals) {
   u32 border = 0u;
-  const u32 ntris = tnext - tbase;
   for (u32 w0 = 0; w0 < ntris; w0 += kListCap) {
     for (int s = threadIdx.x; s < 4096; s += blockDim.x) {
       const int v = S.cases[s];
@@ -701,9 +646,9 @@ __device__ void mc_tris_block(BlockLds& S, const u32* __restrict__ C, int R, int
         for (u32 q = cnt; q < len; ++q) {
           const int e0 = (int)((tri >> (12 * q)) & 0xf), e1 = (int)((tri >> (12 * q + 4)) & 0xf),
                     e2 = (int)((tri >> (12 * q + 8)) & 0xf);
-          const u32 a = edge_vid(S, blk, c0x, c0y, c0z, vbase, cellV, R, fx, fy, fz, e2, &border);
-          const u32 b = edge_vid(S, blk, c0x, c0y, c0z, vbase, cellV, R, fx, fy, fz, e1, &border);
-          const u32 c = edge_vid(S, blk, c0x, c0y, c0z, vbase, cellV, R, fx, fy, fz, e0, &border);
+          const u32 a = edge_vid(S, blk, c0x, c0y, c0z, vbase, blockV, cellL, R, fx, fy, fz, e2, &border);
+          const u32 b = edge_vid(S, blk, c0x, c0y, c0z, vbase, blockV, cellL, R, fx, fy, fz, e1, &border);
+          const u32 c = edge_vid(S, blk, c0x, c0y, c0z, vbase, blockV, cellL, R, fx, fy, fz, e0, &border);
           // listed but not counted: overwrites slot sl after all counted writes
           const u32 k = atomicAdd(&late[0], 1u);
           if (k < capLate) {
@@ -730,13 +675,47 @@ __device__ void mc_tris_block(BlockLds& S, const u32* __restrict__ C, int R, int
       const unsigned long long tri = S.tab.tri[S.cases[s]];
       const int e0 = (int)((tri >> (12 * q)) & 0xf), e1 = (int)((tri >> (12 * q + 4)) & 0xf),
                 e2 = (int)((tri >> (12 * q + 8)) & 0xf);
-      T[3 * (size_t)sl] = edge_vid(S, blk, c0x, c0y, c0z, vbase, cellV, R, fx, fy, fz, e2, &border);
-      T[3 * (size_t)sl + 1] = edge_vid(S, blk, c0x, c0y, c0z, vbase, cellV, R, fx, fy, fz, e1, &border);
-      T[3 * (size_t)sl + 2] = edge_vid(S, blk, c0x, c0y, c0z, vbase, cellV, R, fx, fy, fz, e0, &border);
+      T[3 * (size_t)sl] = edge_vid(S, blk, c0x, c0y, c0z, vbase, blockV, cellL, R, fx, fy, fz, e2, &border);
+      T[3 * (size_t)sl + 1] = edge_vid(S, blk, c0x, c0y, c0z, vbase, blockV, cellL, R, fx, fy, fz, e1, &border);
+      T[3 * (size_t)sl + 2] = edge_vid(S, blk, c0x, c0y, c0z, vbase, blockV, cellL, R, fx, fy, fz, e0, &border);
     }
     __syncthreads();
   }
   if (border) atomicOr(&totals[3], 1u);
+}
+
+// Vertices and triangles of the active blocks: each listed block is staged
+// once (voxel box, cube cases, per-cell prefix sums), then its vertices and
+// its triangles are written. Vertex ids on edges owned by another block's
+// cube come from that block's base and the per-cell local bases written by
+// k_mc_count, so no block waits for another.
+__global__ void __launch_bounds__(kMcThreads) k_mc_emit(const u32* __restrict__ C, int R, Geo geo,
+                                                         const u32* __restrict__ list, int nblocks,
+                                                         const u32* __restrict__ blockV,
+                                                         const u32* __restrict__ blockT,
+                                                         const u32* __restrict__ cellL, float* V, float* Nrm,
+                                                         u32 capV, u32* T, u32 capT, u32* late, u32 capLate,
+                                                         u32* totals) {
+  __shared__ BlockLds S;
+  load_mc_tab(S.tab);
+  const u32 nlist = list[0];
+  for (u32 it = blockIdx.x; it < nlist; it += gridDim.x) {
+    __syncthreads();  // LDS of the previous block is no longer read
+    const int blk = (int)list[1 + it];
+    int c0x, c0y, c0z;
+    block_outside(blk, R, c0x, c0y, c0z);
+    const u32 base = blockV[blk];
+    const u32 next = (blk + 1 < nblocks) ? blockV[blk + 1] : totals[0];
+    const u32 tbase = blockT[blk];
+    const u32 tnext = (blk + 1 < nblocks) ? blockT[blk + 1] : totals[1];
+    if (next == base && tnext == tbase) continue;  // uniform across the workgroup
+    load_box(S.box, C, 2 * R, 2 * c0x - kLo, 2 * c0y - kLo, 2 * c0z - kLo, kBox, kBox);
+    stage_cases(S, blk, R);
+    scan_cells(S);
+    mc_verts_block(S, R, geo, blk, base, next - base, V, Nrm, capV);
+    mc_tris_block(S, R, blk, c0x, c0y, c0z, base, tbase, tnext - tbase, blockV, cellL, T, capT, late, capLate,
+                  totals);
+  }
 }
 
 __device__ __forceinline__ void face(const float* V, const u32* t, float* n) {
@@ -858,7 +837,7 @@ hipError_t launch_mc(const u32* C, int R, const float* bmin3, float cell, const 
                      m.blockT, m.list);
   GM_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_mc_count, dim3(mc_list_grid(m.nblocks, 2048)), dim3(kMcThreads), 0, s, C, R, m.list, m.blockV,
-                     m.blockT);
+                     m.blockT, m.cellV, m.late);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
   int nparts = (int)mc_scan_parts(m.nblocks);
@@ -876,20 +855,17 @@ hipError_t launch_mc_emit(const u32* C, int R, const float* bmin3, float cell, c
                           hipEvent_t* ev) {
   GM_CHECK(upload_tables());
   Geo geo{bmin3[0], bmin3[1], bmin3[2], 0.5f * cell};
-  hipLaunchKernelGGL(k_mc_verts, dim3(mc_list_grid(m.nblocks, 16384)), dim3(kMcThreads), 0, s, C, R, geo, m.list,
-                     m.nblocks, m.blockV, m.cellV, m.V, m.Nrm, m.capV, m.totals, m.late);
+  hipLaunchKernelGGL(k_mc_emit, dim3(mc_list_grid(m.nblocks, 16384)), dim3(kMcThreads), 0, s, C, R, geo, m.list,
+                     m.nblocks, m.blockV, m.blockT, m.cellV, m.V, m.Nrm, m.capV, m.T, m.capT, m.late, m.capLate,
+                     m.totals);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
-  hipLaunchKernelGGL(k_mc_tris, dim3(mc_list_grid(m.nblocks, 16384)), dim3(kMcThreads), 0, s, C, R, m.list, m.nblocks,
-                     m.blockV, m.blockT, m.cellV, m.T, m.capT, m.late, m.capLate, m.totals);
-  GM_CHECK(hipGetLastError());
-  if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   hipLaunchKernelGGL(k_mc_fixup, dim3(1), dim3(64), 0, s, m.V, m.Nrm, m.T, m.late, m.capLate, m.totals, m.capV,
                      m.capT);
   GM_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_mc_normalize, dim3(1024), dim3(256), 0, s, m.Nrm, m.totals, m.capV);
   GM_CHECK(hipGetLastError());
-  if (ev) GM_CHECK(hipEventRecord(ev[3], s));
+  if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   return hipSuccess;
 }
 

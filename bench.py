@@ -130,12 +130,26 @@ def batched(args):
     from broxy_amd import workloads
     jobs = workloads.batched_jobs(seed=args.seed, n=args.batch_jobs)
     big = max(jobs, key=lambda j: j["base_res"])
+    # one by one, two ways: the plain context API, and the engine with one
+    # slot (same copies and kernels, no overlap); the faster is the baseline.
+    # Meshes are hashed after the timed region in every leg.
     ctx = B.Context(0)
     run_alone(ctx, big)  # buffers at their largest size before timing
     t0 = time.perf_counter()
-    ref = [_mesh_hash(*run_alone(ctx, j)) for j in jobs]
-    t_seq = time.perf_counter() - t0
+    alone = [run_alone(ctx, j) for j in jobs]
+    t_ctx = time.perf_counter() - t0
     ctx.close()
+    ref = [_mesh_hash(*m) for m in alone]
+    del alone
+    eng = B.Engine(0, slots=1, hbm_budget=args.hbm_budget << 30)
+    run_engine(eng, [big])
+    t0 = time.perf_counter()
+    one = run_engine(eng, jobs)
+    t_one = time.perf_counter() - t0
+    eng.close()
+    same_one = all(_mesh_hash(*m) == h for m, h in zip(one, ref))
+    del one
+    t_seq = min(t_ctx, t_one)
     # tune the slot count on a prefix of the batch
     tune = {}
     for k in (2, 4, 8, 16):
@@ -156,8 +170,10 @@ def batched(args):
     vox = sum((2 * j["base_res"]) ** 3 for j in jobs)
     return {"jobs": len(jobs), "jobs_per_s": round(len(jobs) / t_bat, 2),
             "one_by_one_jobs_per_s": round(len(jobs) / t_seq, 2), "speedup": round(t_seq / t_bat, 3),
+            "one_by_one_context_jobs_per_s": round(len(jobs) / t_ctx, 2),
+            "one_by_one_engine_1slot_jobs_per_s": round(len(jobs) / t_one, 2),
             "slots": kbest, "slot_tuning_s_per_64_jobs": {str(k): round(v, 4) for k, v in tune.items()},
-            "Gvoxels_per_s": round(vox / t_bat / 1e9, 3), "bit_exact_vs_one_by_one": bool(same),
+            "Gvoxels_per_s": round(vox / t_bat / 1e9, 3), "bit_exact_vs_one_by_one": bool(same and same_one),
             "workload": "sphere-union jobs (8-128 icospheres), N log-uniform [64,384] step 32, r U{2..16}, "
                         "voxelize + close + mesh + copy-back per job"}
 

@@ -47,11 +47,41 @@ struct Job {
   int slot = -1;
 };
 
+// Pinned host buffer, grown on demand. Copies between pageable memory and
+// the device go through the runtime's staging buffers, which the slots'
+// threads would share; from pinned memory each slot's copies are plain DMA
+// on its own stream.
+template <class T>
+struct Pinned {
+  T* p = nullptr;
+  size_t cap = 0;
+  bool reserve(size_t n) {
+    if (n <= cap) return true;
+    if (p) hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t want = n + n / 4;
+    if (hipHostMalloc((void**)&p, want * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      return false;
+    }
+    cap = want;
+    return true;
+  }
+  void release() {
+    if (p) hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
 struct Slot {
   gm_ctx* ctx = nullptr;
   int capR = 0;            // largest base resolution this slot has run
   uint64_t reserved = 0;   // device bytes the slot holds (estimate)
   std::thread th;
+  Pinned<float> inP, outV, outN;
+  Pinned<uint32_t> inT, outT;
 };
 
 // Device bytes of one context at base resolution R (the buffers alloc_grid,
@@ -126,8 +156,14 @@ struct gm_engine {
       }
       return rc == GM_OK;
     };
-    if (!step(gm_load(c, j->P.data(), nullptr, d.num_of_vertices, j->T.data(), d.num_of_faces, d.base_res,
-                      d.se_size)))
+    auto oom = [&]() {
+      j->status = GM_ERR_HIP;
+      j->err = "GMorpho Error: pinned host buffer allocation failed";
+    };
+    if (!s.inP.reserve(j->P.size()) || !s.inT.reserve(j->T.size())) return oom();
+    std::memcpy(s.inP.p, j->P.data(), j->P.size() * sizeof(float));
+    std::memcpy(s.inT.p, j->T.data(), j->T.size() * sizeof(uint32_t));
+    if (!step(gm_load(c, s.inP.p, nullptr, d.num_of_vertices, s.inT.p, d.num_of_faces, d.base_res, d.se_size)))
       return;
     if (!j->lattice.empty()) {
       if (!step(gm_scale_lattice_f32(c, j->lattice.data(), d.lattice_res))) return;
@@ -137,10 +173,12 @@ struct gm_engine {
     if (!step(gm_close(c, d.se))) return;
     if (!step(gm_extract_mesh(c))) return;
     if (!step(gm_mesh_counts(c, &j->nv, &j->nt))) return;
-    j->V.resize(3 * (size_t)j->nv);
-    j->Nrm.resize(3 * (size_t)j->nv);
-    j->Tri.resize(3 * (size_t)j->nt);
-    step(gm_mesh_copy(c, j->V.data(), j->Nrm.data(), j->Tri.data()));
+    const size_t nv3 = 3 * (size_t)j->nv, nt3 = 3 * (size_t)j->nt;
+    if (!s.outV.reserve(nv3) || !s.outN.reserve(nv3) || !s.outT.reserve(nt3)) return oom();
+    if (!step(gm_mesh_copy(c, s.outV.p, s.outN.p, s.outT.p))) return;
+    j->V.assign(s.outV.p, s.outV.p + nv3);
+    j->Nrm.assign(s.outN.p, s.outN.p + nv3);
+    j->Tri.assign(s.outT.p, s.outT.p + nt3);
   }
 
   void worker(int idx) {
@@ -205,8 +243,14 @@ int gm_engine_destroy(gm_engine* e) {
   e->cvWork.notify_all();
   for (auto& s : e->slots)
     if (s.th.joinable()) s.th.join();
-  for (auto& s : e->slots)
+  for (auto& s : e->slots) {
     if (s.ctx) gm_destroy(s.ctx);
+    s.inP.release();
+    s.inT.release();
+    s.outV.release();
+    s.outN.release();
+    s.outT.release();
+  }
   for (auto& kv : e->jobs) delete kv.second;
   delete e;
   return GM_OK;

@@ -124,33 +124,73 @@ def run_engine(eng, jobs):
 
 
 def batched(args):
-    """256 closing jobs of uneven size and radius: one-by-one on one context
-    vs the job engine (slot count tuned on the first 64 jobs). Every job's
-    mesh must be identical between the two."""
+    """256 closing + mesh jobs of uneven size and radius (config 3).
+
+    resident: every job is voxelized into its own context first (untimed),
+    then closing + mesh extraction of all of them is timed, one by one (one
+    job, wait, next) against gm_batch_close_mesh (all jobs, one launch per
+    step). end_to_end: voxelize + close + mesh + copy-back per job from host
+    meshes, one by one against the job engine (slot count tuned on the first
+    64 jobs). Every job's mesh must be identical between the modes."""
     from broxy_amd import workloads
     jobs = workloads.batched_jobs(seed=args.seed, n=args.batch_jobs)
+    vox = sum((2 * j["base_res"]) ** 3 for j in jobs)
+    # ---- resident
+    ctxs = []
+    for j in jobs:
+        c = B.Context(0)
+        c.load(j["P"], j["T"], j["base_res"], 0.3)
+        c.scale_uniform(j["radius"])
+        ctxs.append(c)
+    bt = B.Batch(0)
+
+    def one_by_one():
+        for c in ctxs:
+            c.close_grid(B.SE_SPHERE)
+            c.extract_mesh()
+            c.mesh_counts()
+
+    def all_at_once():
+        bt.close_mesh(ctxs)
+        for c in ctxs:
+            c.mesh_counts()
+
+    one_by_one()
+    all_at_once()  # warm both paths (allocations, code objects)
+    t_one, t_all = [], []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        one_by_one()
+        t_one.append(time.perf_counter() - t0)
+    ref = [_mesh_hash(*c.mesh()) for c in ctxs]
+    for _ in range(3):
+        t0 = time.perf_counter()
+        all_at_once()
+        t_all.append(time.perf_counter() - t0)
+    same_res = all(_mesh_hash(*c.mesh()) == hsh for c, hsh in zip(ctxs, ref))
+    bt.close()
+    for c in ctxs:
+        c.close()
+    del ctxs
+    t_one, t_all = float(np.median(t_one)), float(np.median(t_all))
+    # ---- end to end from host meshes (the faster one-by-one is the baseline)
     big = max(jobs, key=lambda j: j["base_res"])
-    # one by one, two ways: the plain context API, and the engine with one
-    # slot (same copies and kernels, no overlap); the faster is the baseline.
-    # Meshes are hashed after the timed region in every leg.
     ctx = B.Context(0)
-    run_alone(ctx, big)  # buffers at their largest size before timing
+    run_alone(ctx, big)
     t0 = time.perf_counter()
     alone = [run_alone(ctx, j) for j in jobs]
     t_ctx = time.perf_counter() - t0
     ctx.close()
-    ref = [_mesh_hash(*m) for m in alone]
+    ref_e = [_mesh_hash(*m) for m in alone]
     del alone
     eng = B.Engine(0, slots=1, hbm_budget=args.hbm_budget << 30)
     run_engine(eng, [big])
     t0 = time.perf_counter()
     one = run_engine(eng, jobs)
-    t_one = time.perf_counter() - t0
+    t_eng1 = time.perf_counter() - t0
     eng.close()
-    same_one = all(_mesh_hash(*m) == h for m, h in zip(one, ref))
+    same_e = all(_mesh_hash(*m) == hsh for m, hsh in zip(one, ref_e))
     del one
-    t_seq = min(t_ctx, t_one)
-    # tune the slot count on a prefix of the batch
     tune = {}
     for k in (2, 4, 8, 16):
         eng = B.Engine(0, slots=k, hbm_budget=args.hbm_budget << 30)
@@ -164,18 +204,24 @@ def batched(args):
     run_engine(eng, [big] * kbest)
     t0 = time.perf_counter()
     res = run_engine(eng, jobs)
-    t_bat = time.perf_counter() - t0
+    t_eng = time.perf_counter() - t0
     eng.close()
-    same = all(_mesh_hash(*m) == h for m, h in zip(res, ref))
-    vox = sum((2 * j["base_res"]) ** 3 for j in jobs)
-    return {"jobs": len(jobs), "jobs_per_s": round(len(jobs) / t_bat, 2),
-            "one_by_one_jobs_per_s": round(len(jobs) / t_seq, 2), "speedup": round(t_seq / t_bat, 3),
-            "one_by_one_context_jobs_per_s": round(len(jobs) / t_ctx, 2),
-            "one_by_one_engine_1slot_jobs_per_s": round(len(jobs) / t_one, 2),
-            "slots": kbest, "slot_tuning_s_per_64_jobs": {str(k): round(v, 4) for k, v in tune.items()},
-            "Gvoxels_per_s": round(vox / t_bat / 1e9, 3), "bit_exact_vs_one_by_one": bool(same and same_one),
-            "workload": "sphere-union jobs (8-128 icospheres), N log-uniform [64,384] step 32, r U{2..16}, "
-                        "voxelize + close + mesh + copy-back per job"}
+    same_e = same_e and all(_mesh_hash(*m) == hsh for m, hsh in zip(res, ref_e))
+    t_seq = min(t_ctx, t_eng1)
+    n = len(jobs)
+    return {"jobs": n, "jobs_per_s": round(n / t_all, 2), "one_by_one_jobs_per_s": round(n / t_one, 2),
+            "speedup": round(t_one / t_all, 3), "Gvoxels_per_s": round(vox / t_all / 1e9, 3),
+            "bit_exact_vs_one_by_one": bool(same_res),
+            "mode": "resident: grids voxelized into one context per job beforehand; timed = closing + mesh of "
+                    "every job with counts on the host (median of 3)",
+            "end_to_end": {"jobs_per_s": round(n / t_eng, 2), "one_by_one_jobs_per_s": round(n / t_seq, 2),
+                           "speedup": round(t_seq / t_eng, 3),
+                           "one_by_one_context_jobs_per_s": round(n / t_ctx, 2),
+                           "one_by_one_engine_1slot_jobs_per_s": round(n / t_eng1, 2), "slots": kbest,
+                           "slot_tuning_s_per_64_jobs": {str(k): round(v, 4) for k, v in tune.items()},
+                           "bit_exact_vs_one_by_one": bool(same_e),
+                           "mode": "job engine from host meshes: voxelize + close + mesh + copy-back per job"},
+            "workload": "sphere-union jobs (8-128 icospheres), N log-uniform [64,384] step 32, r U{2..16}"}
 
 
 def served(args):

@@ -325,3 +325,38 @@ class Engine:
         q = ctypes.c_int()
         self._call("gm_engine_stats", ctypes.byref(b), ctypes.byref(r), ctypes.byref(q))
         return b.value, r.value, q.value
+
+
+class Batch:
+    """gm_batch: closing + mesh extraction of many contexts at once, one
+    kernel launch per step for the uniform-radius sphere contexts."""
+
+    def __init__(self, device=0):
+        self._lib = hip_lib()
+        self._lib.gm_batch_last_error.restype = ctypes.c_char_p
+        self._lib.gm_batch_last_error.argtypes = [_P]
+        h = _P()
+        rc = self._lib.gm_batch_create(device, ctypes.byref(h))
+        if rc:
+            raise GMorphoError("gm_batch_create failed (%d): no usable HIP device" % rc)
+        self._h = h
+
+    def close(self):
+        if self._h:
+            self._lib.gm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def close_mesh(self, contexts, se=SE_SPHERE):
+        """gm_close + gm_extract_mesh for every context; meshes are then read
+        with each context's mesh_counts / mesh."""
+        arr = (_P * len(contexts))(*[c._h for c in contexts])
+        rc = self._lib.gm_batch_close_mesh(self._h, arr, len(contexts), se)
+        if rc:
+            msg = self._lib.gm_batch_last_error(self._h)
+            raise GMorphoError("gm_batch_close_mesh -> %d: %s" % (rc, msg.decode() if msg else ""))

@@ -840,6 +840,47 @@ int gm_stage_times(gm_ctx* c, float* ms, int n) {
   return GM_OK;
 }
 
+// ---- Batched closing + mesh of many contexts (one launch per step).
+struct gm_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  void *hMorph = nullptr, *dMorph = nullptr;
+  size_t capMorph = 0;
+  void *hMc = nullptr, *dMc = nullptr;
+  size_t capMc = 0;
+  u32* hTot = nullptr;
+  size_t capTot = 0;
+  u32* queue = nullptr;
+  size_t capQueue = 0;  // words
+  int qsel = 0;
+};
+
+namespace {
+int bfail(gm_batch* b, int code, const std::string& msg) {
+  b->err = "GMorpho Error: " + msg;
+  return code;
+}
+#define BHIP(b, x)                                                                                  \
+  do {                                                                                              \
+    hipError_t e_ = (x);                                                                            \
+    if (e_ != hipSuccess) return bfail(b, GM_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+// Pinned host + device pair of at least `bytes`, kept at the largest size.
+int batch_blobs(gm_batch* b, void*& h, void*& d, size_t& cap, size_t bytes) {
+  if (bytes <= cap) return GM_OK;
+  if (h) hipHostFree(h);
+  if (d) hipFree(d);
+  h = d = nullptr;
+  cap = 0;
+  const size_t want = bytes + bytes / 4;
+  BHIP(b, hipHostMalloc(&h, want, hipHostMallocDefault));
+  BHIP(b, hipMalloc(&d, want));
+  cap = want;
+  return GM_OK;
+}
+}  // namespace
+
 int gm_sync(gm_ctx* c) {
   if (!c) return GM_ERR_ARG;
   HIPCALL(c, hipStreamSynchronize(c->stream));
@@ -855,6 +896,115 @@ int gm_get_stream(gm_ctx* c, void** s) {
 int gm_mc_tables(uint32_t* ntri, uint32_t* len, uint64_t* tri) {
   if (!ntri || !len || !tri) return GM_ERR_ARG;
   mc_tables_host(ntri, len, (unsigned long long*)tri);
+  return GM_OK;
+}
+
+
+int gm_batch_create(int device, gm_batch** out) {
+  if (!out) return GM_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return GM_ERR_HIP;
+  if (device < 0 || device >= n) return GM_ERR_ARG;
+  gm_batch* b = new gm_batch();
+  b->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return GM_ERR_HIP;
+  }
+  *out = b;
+  return GM_OK;
+}
+
+int gm_batch_destroy(gm_batch* b) {
+  if (!b) return GM_ERR_ARG;
+  hipSetDevice(b->device);
+  if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->hMorph) hipHostFree(b->hMorph);
+  if (b->hMc) hipHostFree(b->hMc);
+  if (b->hTot) hipHostFree(b->hTot);
+  dfree(b->dMorph);
+  dfree(b->dMc);
+  dfree(b->queue);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return GM_OK;
+}
+
+const char* gm_batch_last_error(const gm_batch* b) { return b ? b->err.c_str() : ""; }
+
+int gm_batch_close_mesh(gm_batch* b, gm_ctx* const* ctxs, int n, int se) {
+  if (!b || (!ctxs && n > 0) || n < 0) return GM_ERR_ARG;
+  if (se != GM_SE_SPHERE && se != GM_SE_CUBE) return bfail(b, GM_ERR_ARG, "unknown structuring element");
+  if (n > (1 << 12)) return bfail(b, GM_ERR_UNSUPPORTED, "more than 4096 contexts in one batch");
+  hipSetDevice(b->device);
+  static const bool ctOk = ct_tables_ok();
+  if (!ctOk) return bfail(b, GM_ERR_UNSUPPORTED, "compiled chord tables differ from the structuring element");
+  std::vector<gm_ctx*> grouped, single;
+  for (int i = 0; i < n; ++i) {
+    gm_ctx* c = ctxs[i];
+    if (!c || !c->has_grid) return bfail(b, GM_ERR_STATE, "context without a grid in the batch");
+    if (c->device != b->device) return bfail(b, GM_ERR_ARG, "context on another device");
+    if (c->rhi > 31) return bfail(b, GM_ERR_UNSUPPORTED, "radius above 31 fine voxels is not supported by this build");
+    c->evValid = 0;
+    if (se == GM_SE_SPHERE && c->rmode == 0 && c->r >= 2 && c->r <= 31) grouped.push_back(c);
+    else single.push_back(c);
+  }
+  // the contexts' own streams are idle before the batch stream touches their buffers
+  for (int i = 0; i < n; ++i) BHIP(b, hipStreamSynchronize(ctxs[i]->stream));
+  for (gm_ctx* c : single) {
+    int rc = gm_close(c, se);
+    if (rc == GM_OK) rc = gm_extract_mesh(c);
+    if (rc != GM_OK) return bfail(b, rc, c->err);
+  }
+  const int m = (int)grouped.size();
+  if (m > 0) {
+    std::vector<BatchMorphJob> mj(m);
+    std::vector<BatchMCJob> cj(m);
+    for (int k = 0; k < m; ++k) {
+      gm_ctx* c = grouped[k];
+      mj[k].d = morph_dims(c);
+      mj[k].b = morph_buffers(c);
+      mj[k].r = c->r;
+      cj[k].C = c->C;
+      cj[k].R = c->R;
+      for (int a = 0; a < 3; ++a) cj[k].bmin[a] = c->bmin[a];
+      cj[k].cell = c->cell;
+      cj[k].m = c->mc;
+    }
+    int rc = batch_blobs(b, b->hMorph, b->dMorph, b->capMorph, batch_morph_bytes(m));
+    if (rc) return rc;
+    rc = batch_blobs(b, b->hMc, b->dMc, b->capMc, batch_mc_bytes(m));
+    if (rc) return rc;
+    if ((size_t)m * 4 > b->capTot) {
+      if (b->hTot) hipHostFree(b->hTot);
+      b->hTot = nullptr;
+      b->capTot = 0;
+      BHIP(b, hipHostMalloc((void**)&b->hTot, (size_t)m * 16, hipHostMallocDefault));
+      b->capTot = (size_t)m * 4;
+    }
+    const size_t cap = batch_queue_cap(mj.data(), m);
+    const size_t words = kQueueLists + 256 * cap;
+    if (words > b->capQueue) {
+      dfree(b->queue);
+      b->capQueue = 0;
+      BHIP(b, hipMalloc(&b->queue, words * 4));
+      // both counter blocks start at zero (each selection clears the other)
+      BHIP(b, hipMemsetAsync(b->queue, 0, kQueueLists * 4, b->stream));
+      b->capQueue = words;
+      b->qsel = 0;
+    }
+    hipError_t e = launch_close_batched(mj.data(), m, b->hMorph, b->dMorph, b->queue, (int)cap, &b->qsel, b->stream);
+    if (e != hipSuccess) return bfail(b, GM_ERR_HIP, std::string("batched closing: ") + hipGetErrorString(e));
+    e = launch_mc_batched(cj.data(), m, b->hMc, b->dMc, b->hTot, b->stream);
+    if (e != hipSuccess) return bfail(b, GM_ERR_HIP, std::string("batched mesh extraction: ") + hipGetErrorString(e));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    for (int k = 0; k < m; ++k) {
+      std::memcpy(grouped[k]->hTotals, b->hTot + 4 * k, 16);
+      grouped[k]->meshPending = true;
+    }
+  }
+  for (gm_ctx* c : single) BHIP(b, hipStreamSynchronize(c->stream));
   return GM_OK;
 }
 

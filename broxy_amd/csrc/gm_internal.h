@@ -63,6 +63,19 @@ __device__ __forceinline__ int radius_at(const RadiusField& f, int x, int y, int
   return v < 2 ? 2 : (int)v;
 }
 
+// Many jobs per launch: unit u (a tile, workgroup or list entry) belongs to
+// the job j with pre[j] <= u < pre[j+1] (pre: prefix of the per-job counts;
+// u is uniform over the caller's wave).
+__device__ __forceinline__ int batch_job(const u32* __restrict__ pre, int n, u32 u) {
+  int lo = 0, hi = n - 1;  // largest j with pre[j] <= u
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= u) lo = mid;
+    else hi = mid - 1;
+  }
+  return __builtin_amdgcn_readfirstlane(lo);
+}
+
 // ---------------------------------------------------------------------------
 // Structuring-element chord tables. A table lists, per "dz class", the dz
 // values it stands for and the (dy class, x half-width m) chords. Two kinds:
@@ -166,6 +179,31 @@ hipError_t launch_brick_summary(const u32* G, int N, uint8_t* out, u32* clear, h
 hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf,
                         const SETableSet& tabs, int rho_lo, int rho_hi, int tabWords, hipStream_t s,
                         hipEvent_t* ev);
+// Batched closing of uniform-radius sphere jobs (radius r in [2, 31]), all
+// jobs on stream s. The caller provides a pinned host blob and a device blob
+// of batch_morph_bytes(n) (per-job argument tables and work prefixes), and a
+// queue of kQueueLists + 256 * cap words, cap >= batch_queue_cap, whose
+// counters start at zero; qsel is the batch's counter-block toggle.
+struct BatchMorphJob {
+  MorphDims d;
+  MorphBuffers b;
+  int r;
+};
+size_t batch_morph_bytes(int n);
+// Brick summaries of many grids in one launch (dPre: prefix of (N/8)^2 per
+// job); clear0 / clear1 (optional) are zeroed by the job's first workgroup.
+struct BrickJob {
+  const u32* G;
+  uint8_t* out;
+  u32* clear0;
+  u32* clear1;
+  int N;
+};
+hipError_t launch_brick_summary_batched(const BrickJob* dJobs, const u32* dPre, int n, u32 blocks, int wmax,
+                                        hipStream_t s);
+size_t batch_queue_cap(const BatchMorphJob* jobs, int n);
+hipError_t launch_close_batched(const BatchMorphJob* jobs, int n, void* hBlob, void* dBlob, u32* queue, int cap,
+                                int* qsel, hipStream_t s);
 // Radius planes and per-brick value sets of a per-voxel radius field
 // (values clamped to [2,31]; the caller has checked the range).
 hipError_t launch_radius_planes(const RadiusField& rf, int N, u32* rp, u32* rvset, hipStream_t s);
@@ -182,6 +220,7 @@ constexpr int kCtMaxRho = 32;
 constexpr int kCtParts = 8;
 struct CtKernels {
   const void* fn[4][kCtMaxRho + 1];
+  const void* fnB[4][kCtMaxRho + 1];  // batched forms (argument table + job/tile queue)
 };
 void ct_register_part0(CtKernels&);
 void ct_register_part1(CtKernels&);
@@ -215,6 +254,18 @@ hipError_t launch_mc(const u32* C, int R, const float* bmin3, float cell, const 
 size_t mc_scan_parts(int nblocks);
 hipError_t launch_mc_emit(const u32* C, int R, const float* bmin3, float cell, const MCBuffers& m,
                           hipStream_t s, hipEvent_t* ev);
+// Marching cubes of many closed grids, all on stream s: the same kernels per
+// job, one launch per step. hBlob (pinned) and dBlob hold batch_mc_bytes(n);
+// hTotals (pinned, 4 u32 per job) receives each job's totals.
+struct BatchMCJob {
+  const u32* C;
+  int R;
+  float bmin[3];
+  float cell;
+  MCBuffers m;
+};
+size_t batch_mc_bytes(int n);
+hipError_t launch_mc_batched(const BatchMCJob* jobs, int n, void* hBlob, void* dBlob, u32* hTotals, hipStream_t s);
 
 // ---- reference-format max-mipmap (gm_pyramid.hip)
 hipError_t launch_pyramid(const u32* rows, int R, const RadiusField& rf, u64* levels, hipStream_t s);

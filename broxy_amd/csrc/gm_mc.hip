@@ -23,6 +23,7 @@
 //   mc_fixup   triangles the count table does not cover, mc_normalize.
 // The per-cell vertex base map is dense (R^3 u32) but only written and read
 // at surface cells, replacing the reference's 4*N^3-byte memset + splat.
+#include <algorithm>
 #include <mutex>
 
 #include "gm_internal.h"
@@ -244,9 +245,8 @@ __device__ __forceinline__ bool block_outside(int blk, int R, int& c0x, int& c0y
 // planes of C) around the block's 17^3 fine voxels are not all empty and not
 // all full. Active blocks are appended to list (list[0] = count, then block
 // ids, unordered); the others get zero counts. One thread per block.
-__global__ void __launch_bounds__(256) k_mc_blocks(const uint8_t* __restrict__ brick, int R, int nblocks,
-                                                   u32* blockV, u32* blockT, u32* list) {
-  const int blk = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void mc_blocks_body(const uint8_t* __restrict__ brick, int R, int nblocks, u32* blockV,
+                                               u32* blockT, u32* list, int blk) {
   bool act = false;
   if (blk < nblocks) {
     int c0x, c0y, c0z;
@@ -281,6 +281,59 @@ __global__ void __launch_bounds__(256) k_mc_blocks(const uint8_t* __restrict__ b
     if (act) list[1 + base + (u32)__popcll(m & ((1ull << lane) - 1ull))] = (u32)blk;
   }
 }
+__global__ void __launch_bounds__(256) k_mc_blocks(const uint8_t* __restrict__ brick, int R, int nblocks,
+                                                   u32* blockV, u32* blockT, u32* list) {
+  mc_blocks_body(brick, R, nblocks, blockV, blockT, list, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Vertex and triangle counts of one listed block, the block-local vertex base
+// of each of its cells with vertices.
+__device__ void mc_count_block(const u32* __restrict__ C, int R, int blk, u32* blockV, u32* blockT, u32* cellL,
+                               Box& bx, u32* red, const McTab& tab) {
+  int c0x, c0y, c0z;
+  block_outside(blk, R, c0x, c0y, c0z);
+  // cells of the block need fine voxels [16b, 16b+17) per axis
+  load_box(bx, C, 2 * R, 2 * c0x, 2 * c0y, 2 * c0z, 17, 17);
+  // uniform block: no mixed cube
+  u32 a = 0x1ffffu, o = 0u;
+  for (int i = threadIdx.x; i < 17 * 17; i += blockDim.x) {
+    u32 v = bx.rows[(i % 17) + kBox * (i / 17)] & 0x1ffffu;
+    a &= v;
+    o |= v;
+  }
+  const bool anyOne = __syncthreads_or(o != 0u) != 0;
+  const bool anyZero = __syncthreads_or(a != 0x1ffffu) != 0;
+  const bool nonuni = anyOne && anyZero;
+  u32 nvh[2] = {0u, 0u}, nt = 0;
+  if (nonuni) {
+    for (int h = 0; h < 2; ++h) {
+      int cx, cy, cz;
+      block_cell(blk, 2 * threadIdx.x + h, cx, cy, cz);
+      if (!cell_in_m_range(cx, cy, cz, R) || cell_uniform(bx, cx, cy, cz)) continue;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        int v = cube_case(bx, 2 * cx + (j & 1), 2 * cy + ((j >> 1) & 1), 2 * cz + ((j >> 2) & 1));
+        nvh[h] += own_verts(v);
+        nt += tab.ntri[v];
+      }
+    }
+  }
+  u32 tv, tt;
+  const u32 off = block_exscan(nvh[0] + nvh[1], red, &tv);
+  block_exscan(nt, red, &tt);
+  // block-local vertex base of the cells with vertices (cells are in Morton
+  // order, two per thread), read for the edges owned by another block's cube
+  for (int h = 0; h < 2; ++h)
+    if (nvh[h]) {
+      int cx, cy, cz;
+      block_cell(blk, 2 * threadIdx.x + h, cx, cy, cz);
+      cellL[(size_t)cx + (size_t)R * ((size_t)cy + (size_t)R * cz)] = off + (h ? nvh[0] : 0u);
+    }
+  if (threadIdx.x == 0) {
+    blockV[blk] = tv;
+    blockT[blk] = tt;
+  }
+}
 
 // Vertex and triangle counts of the active blocks; the workgroups walk the
 // list (each ends when the list does).
@@ -295,50 +348,7 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_count(const u32* __restrict__
   load_mc_tab(tab);
   const u32 n = list[0];
   for (u32 it = blockIdx.x; it < n; it += gridDim.x) {
-    const int blk = (int)list[1 + it];
-    int c0x, c0y, c0z;
-    block_outside(blk, R, c0x, c0y, c0z);
-    // cells of the block need fine voxels [16b, 16b+17) per axis
-    load_box(bx, C, 2 * R, 2 * c0x, 2 * c0y, 2 * c0z, 17, 17);
-    // uniform block: no mixed cube
-    u32 a = 0x1ffffu, o = 0u;
-    for (int i = threadIdx.x; i < 17 * 17; i += blockDim.x) {
-      u32 v = bx.rows[(i % 17) + kBox * (i / 17)] & 0x1ffffu;
-      a &= v;
-      o |= v;
-    }
-    const bool anyOne = __syncthreads_or(o != 0u) != 0;
-    const bool anyZero = __syncthreads_or(a != 0x1ffffu) != 0;
-    const bool nonuni = anyOne && anyZero;
-    u32 nvh[2] = {0u, 0u}, nt = 0;
-    if (nonuni) {
-      for (int h = 0; h < 2; ++h) {
-        int cx, cy, cz;
-        block_cell(blk, 2 * threadIdx.x + h, cx, cy, cz);
-        if (!cell_in_m_range(cx, cy, cz, R) || cell_uniform(bx, cx, cy, cz)) continue;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          int v = cube_case(bx, 2 * cx + (j & 1), 2 * cy + ((j >> 1) & 1), 2 * cz + ((j >> 2) & 1));
-          nvh[h] += own_verts(v);
-          nt += tab.ntri[v];
-        }
-      }
-    }
-    u32 tv, tt;
-    const u32 off = block_exscan(nvh[0] + nvh[1], red, &tv);
-    block_exscan(nt, red, &tt);
-    // block-local vertex base of the cells with vertices (cells are in Morton
-    // order, two per thread), read for the edges owned by another block's cube
-    for (int h = 0; h < 2; ++h)
-      if (nvh[h]) {
-        int cx, cy, cz;
-        block_cell(blk, 2 * threadIdx.x + h, cx, cy, cz);
-        cellL[(size_t)cx + (size_t)R * ((size_t)cy + (size_t)R * cz)] = off + (h ? nvh[0] : 0u);
-      }
-    if (threadIdx.x == 0) {
-      blockV[blk] = tv;
-      blockT[blk] = tt;
-    }
+    mc_count_block(C, R, (int)list[1 + it], blockV, blockT, cellL, bx, red, tab);
     __syncthreads();  // the box is reloaded by the next iteration
   }
 }
@@ -346,10 +356,10 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_count(const u32* __restrict__
 // ---- device-wide exclusive scan of (blockV, blockT), 1024 entries per chunk
 constexpr int kChunk = 1024;
 
-__global__ void __launch_bounds__(256) k_scan_reduce(const u32* __restrict__ a, const u32* __restrict__ b, int n,
-                                                     u32* part) {
+__device__ __forceinline__ void scan_reduce_body(const u32* __restrict__ a, const u32* __restrict__ b, int n, u32* part,
+                                                 int chunk) {
   __shared__ u32 red[16];
-  int base = blockIdx.x * kChunk;
+  int base = chunk * kChunk;
   u32 sa = 0, sb = 0;
   for (int i = base + threadIdx.x; i < min(base + kChunk, n); i += 256) {
     sa += a[i];
@@ -359,37 +369,35 @@ __global__ void __launch_bounds__(256) k_scan_reduce(const u32* __restrict__ a, 
   block_exscan(sa, red, &ta);
   block_exscan(sb, red, &tb);
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = ta;
-    part[2 * blockIdx.x + 1] = tb;
+    part[2 * chunk] = ta;
+    part[2 * chunk + 1] = tb;
   }
 }
+__global__ void __launch_bounds__(256) k_scan_reduce(const u32* __restrict__ a, const u32* __restrict__ b, int n,
+                                                     u32* part) {
+  scan_reduce_body(a, b, n, part, blockIdx.x);
+}
 
-__global__ void __launch_bounds__(256) k_scan_top(u32* part, int nparts, u32* totals, u32 capV, u32 capT) {
+// Exclusive scan of the chunk totals (each thread a contiguous run of
+// ceil(nparts / 256) parts), then the overall totals and overflow flag.
+__device__ __forceinline__ void scan_top_body(u32* part, int nparts, u32* totals, u32 capV, u32 capT) {
   __shared__ u32 red[16];
-  // up to 4 parts per thread, contiguous
-  int p0 = threadIdx.x * 4;
-  u32 va[4] = {0, 0, 0, 0}, vb[4] = {0, 0, 0, 0};
+  const int per = (nparts + 255) / 256;
+  const int p0 = threadIdx.x * per, p1 = min(p0 + per, nparts);
   u32 sa = 0, sb = 0;
-  for (int k = 0; k < 4; ++k) {
-    int p = p0 + k;
-    if (p < nparts) {
-      va[k] = part[2 * p];
-      vb[k] = part[2 * p + 1];
-    }
-    sa += va[k];
-    sb += vb[k];
+  for (int p = p0; p < p1; ++p) {
+    sa += part[2 * p];
+    sb += part[2 * p + 1];
   }
   u32 ta, tb;
   u32 oa = block_exscan(sa, red, &ta);
   u32 ob = block_exscan(sb, red, &tb);
-  for (int k = 0; k < 4; ++k) {
-    int p = p0 + k;
-    if (p < nparts) {
-      part[2 * p] = oa;
-      part[2 * p + 1] = ob;
-    }
-    oa += va[k];
-    ob += vb[k];
+  for (int p = p0; p < p1; ++p) {
+    const u32 va = part[2 * p], vb = part[2 * p + 1];
+    part[2 * p] = oa;
+    part[2 * p + 1] = ob;
+    oa += va;
+    ob += vb;
   }
   if (threadIdx.x == 0) {
     totals[0] = ta;
@@ -398,10 +406,13 @@ __global__ void __launch_bounds__(256) k_scan_top(u32* part, int nparts, u32* to
     totals[3] = 0u;
   }
 }
+__global__ void __launch_bounds__(256) k_scan_top(u32* part, int nparts, u32* totals, u32 capV, u32 capT) {
+  scan_top_body(part, nparts, totals, capV, capT);
+}
 
-__global__ void __launch_bounds__(256) k_scan_down(u32* a, u32* b, int n, const u32* __restrict__ part) {
+__device__ __forceinline__ void scan_down_body(u32* a, u32* b, int n, const u32* __restrict__ part, int chunk) {
   __shared__ u32 red[16];
-  int base = blockIdx.x * kChunk + 4 * threadIdx.x;
+  int base = chunk * kChunk + 4 * threadIdx.x;
   u32 va[4], vb[4];
   u32 sa = 0, sb = 0;
   for (int k = 0; k < 4; ++k) {
@@ -412,8 +423,8 @@ __global__ void __launch_bounds__(256) k_scan_down(u32* a, u32* b, int n, const 
     sb += vb[k];
   }
   u32 ta, tb;
-  u32 oa = block_exscan(sa, red, &ta) + part[2 * blockIdx.x];
-  u32 ob = block_exscan(sb, red, &tb) + part[2 * blockIdx.x + 1];
+  u32 oa = block_exscan(sa, red, &ta) + part[2 * chunk];
+  u32 ob = block_exscan(sb, red, &tb) + part[2 * chunk + 1];
   for (int k = 0; k < 4; ++k) {
     int i = base + k;
     if (i < n) {
@@ -423,6 +434,9 @@ __global__ void __launch_bounds__(256) k_scan_down(u32* a, u32* b, int n, const 
     oa += va[k];
     ob += vb[k];
   }
+}
+__global__ void __launch_bounds__(256) k_scan_down(u32* a, u32* b, int n, const u32* __restrict__ part) {
+  scan_down_body(a, b, n, part, blockIdx.x);
 }
 
 // Position of the vertex on owned edge e (0: x, 3: y, 8: z) of cube f:
@@ -684,6 +698,25 @@ __device__ void mc_tris_block(BlockLds& S, int R, int blk, int c0x, int c0y, int
   if (border) atomicOr(&totals[3], 1u);
 }
 
+// One listed block: staged once, then its vertices and its triangles.
+__device__ void mc_emit_block(BlockLds& S, const u32* __restrict__ C, int R, Geo geo, int blk, int nblocks,
+                              const u32* __restrict__ blockV, const u32* __restrict__ blockT,
+                              const u32* __restrict__ cellL, float* V, float* Nrm, u32 capV, u32* T, u32 capT,
+                              u32* late, u32 capLate, u32* totals) {
+  int c0x, c0y, c0z;
+  block_outside(blk, R, c0x, c0y, c0z);
+  const u32 base = blockV[blk];
+  const u32 next = (blk + 1 < nblocks) ? blockV[blk + 1] : totals[0];
+  const u32 tbase = blockT[blk];
+  const u32 tnext = (blk + 1 < nblocks) ? blockT[blk + 1] : totals[1];
+  if (next == base && tnext == tbase) return;  // uniform across the workgroup
+  load_box(S.box, C, 2 * R, 2 * c0x - kLo, 2 * c0y - kLo, 2 * c0z - kLo, kBox, kBox);
+  stage_cases(S, blk, R);
+  scan_cells(S);
+  mc_verts_block(S, R, geo, blk, base, next - base, V, Nrm, capV);
+  mc_tris_block(S, R, blk, c0x, c0y, c0z, base, tbase, tnext - tbase, blockV, cellL, T, capT, late, capLate, totals);
+}
+
 // Vertices and triangles of the active blocks: each listed block is staged
 // once (voxel box, cube cases, per-cell prefix sums), then its vertices and
 // its triangles are written. Vertex ids on edges owned by another block's
@@ -701,20 +734,8 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_emit(const u32* __restrict__ 
   const u32 nlist = list[0];
   for (u32 it = blockIdx.x; it < nlist; it += gridDim.x) {
     __syncthreads();  // LDS of the previous block is no longer read
-    const int blk = (int)list[1 + it];
-    int c0x, c0y, c0z;
-    block_outside(blk, R, c0x, c0y, c0z);
-    const u32 base = blockV[blk];
-    const u32 next = (blk + 1 < nblocks) ? blockV[blk + 1] : totals[0];
-    const u32 tbase = blockT[blk];
-    const u32 tnext = (blk + 1 < nblocks) ? blockT[blk + 1] : totals[1];
-    if (next == base && tnext == tbase) continue;  // uniform across the workgroup
-    load_box(S.box, C, 2 * R, 2 * c0x - kLo, 2 * c0y - kLo, 2 * c0z - kLo, kBox, kBox);
-    stage_cases(S, blk, R);
-    scan_cells(S);
-    mc_verts_block(S, R, geo, blk, base, next - base, V, Nrm, capV);
-    mc_tris_block(S, R, blk, c0x, c0y, c0z, base, tbase, tnext - tbase, blockV, cellL, T, capT, late, capLate,
-                  totals);
+    mc_emit_block(S, C, R, geo, (int)list[1 + it], nblocks, blockV, blockT, cellL, V, Nrm, capV, T, capT, late,
+                  capLate, totals);
   }
 }
 
@@ -731,9 +752,8 @@ __device__ __forceinline__ void face(const float* V, const u32* t, float* n) {
 
 // Applies the listed-but-uncounted triangles in slot order (single thread):
 // each replaces the triangle in its slot, and the normal sums follow.
-__global__ void k_mc_fixup(const float* V, float* Nrm, u32* T, u32* late, u32 capLate, u32* totals, u32 capV,
-                           u32 capT) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ void mc_fixup_body(const float* V, float* Nrm, u32* T, u32* late, u32 capLate, u32* totals, u32 capV,
+                              u32 capT) {
   u32 n = late[0];
   if (n > capLate) n = capLate;
   if (n == 0) return;
@@ -765,12 +785,18 @@ __global__ void k_mc_fixup(const float* V, float* Nrm, u32* T, u32* late, u32 ca
       for (int q = 0; q < 3; ++q) Nrm[3 * (size_t)t[k] + q] = __fadd_rn(Nrm[3 * (size_t)t[k] + q], f[q]);
   }
 }
+__global__ void k_mc_fixup(const float* V, float* Nrm, u32* T, u32* late, u32 capLate, u32* totals, u32 capV,
+                           u32 capT) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  mc_fixup_body(V, Nrm, T, late, capLate, totals, capV, capT);
+}
 
 // NormalizeMeshNormals (GMorpho.cu:6526-6537).
-__global__ void k_mc_normalize(float* Nrm, const u32* totals, u32 capV) {
+__device__ __forceinline__ void mc_normalize_body(float* Nrm, const u32* totals, u32 capV, size_t first,
+                                                  size_t stride) {
   u32 nV = totals[0];
   if (nV > capV) nV = capV;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nV; i += (size_t)gridDim.x * blockDim.x) {
+  for (size_t i = first; i < nV; i += stride) {
     float x = Nrm[3 * i], y = Nrm[3 * i + 1], z = Nrm[3 * i + 2];
     float l = sqrtf(__fadd_rn(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y)), __fmul_rn(z, z)));
     if (l > 0.f) {
@@ -780,7 +806,93 @@ __global__ void k_mc_normalize(float* Nrm, const u32* totals, u32 capV) {
     }
   }
 }
+__global__ void k_mc_normalize(float* Nrm, const u32* totals, u32 capV) {
+  mc_normalize_body(Nrm, totals, capV, (size_t)blockIdx.x * blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x);
+}
 
+
+// ---- Batched forms: many closed grids in one launch per step.
+struct MCJob {
+  const u32* C;
+  int R;
+  Geo geo;
+  MCBuffers m;
+};
+__global__ void __launch_bounds__(256) k_mc_blocks_b(const MCJob* __restrict__ jobs, const u32* __restrict__ pre,
+                                                     int n) {
+  const int j = batch_job(pre, n, blockIdx.x);
+  const MCJob& J = jobs[j];
+  mc_blocks_body(J.m.brick, J.R, J.m.nblocks, J.m.blockV, J.m.blockT, J.m.list,
+                 (int)(blockIdx.x - pre[j]) * 256 + (int)threadIdx.x);
+}
+// Prefix of the jobs' active-block list lengths (one workgroup).
+__global__ void k_list_prefix(const MCJob* __restrict__ jobs, int n, u32* lpre) {
+  if (threadIdx.x != 0) return;
+  u32 acc = 0;
+  for (int j = 0; j < n; ++j) {
+    lpre[j] = acc;
+    acc += jobs[j].m.list[0];
+  }
+  lpre[n] = acc;
+}
+__global__ void __launch_bounds__(kMcThreads) k_mc_count_b(const MCJob* __restrict__ jobs, const u32* __restrict__ lpre,
+                                                            int n) {
+  __shared__ Box bx;
+  __shared__ u32 red[16];
+  __shared__ McTab tab;
+  load_mc_tab(tab);
+  for (u32 u = blockIdx.x; u < lpre[n]; u += gridDim.x) {
+    const int j = batch_job(lpre, n, u);
+    const MCJob& J = jobs[j];
+    mc_count_block(J.C, J.R, (int)J.m.list[1 + u - lpre[j]], J.m.blockV, J.m.blockT, J.m.cellV, bx, red, tab);
+    __syncthreads();
+  }
+}
+__global__ void __launch_bounds__(256) k_scan_reduce_b(const MCJob* __restrict__ jobs, const u32* __restrict__ pre,
+                                                       int n) {
+  const int j = batch_job(pre, n, blockIdx.x);
+  const MCJob& J = jobs[j];
+  scan_reduce_body(J.m.blockV, J.m.blockT, J.m.nblocks, J.m.scanPart, (int)(blockIdx.x - pre[j]));
+}
+__global__ void __launch_bounds__(256) k_scan_top_b(const MCJob* __restrict__ jobs) {
+  const MCJob& J = jobs[blockIdx.x];
+  scan_top_body(J.m.scanPart, (J.m.nblocks + kChunk - 1) / kChunk, J.m.totals, J.m.capV, J.m.capT);
+}
+__global__ void __launch_bounds__(256) k_scan_down_b(const MCJob* __restrict__ jobs, const u32* __restrict__ pre,
+                                                     int n) {
+  const int j = batch_job(pre, n, blockIdx.x);
+  const MCJob& J = jobs[j];
+  scan_down_body(J.m.blockV, J.m.blockT, J.m.nblocks, J.m.scanPart, (int)(blockIdx.x - pre[j]));
+}
+__global__ void __launch_bounds__(kMcThreads) k_mc_emit_b(const MCJob* __restrict__ jobs, const u32* __restrict__ lpre,
+                                                           int n) {
+  __shared__ BlockLds S;
+  load_mc_tab(S.tab);
+  for (u32 u = blockIdx.x; u < lpre[n]; u += gridDim.x) {
+    __syncthreads();  // LDS of the previous block is no longer read
+    const int j = batch_job(lpre, n, u);
+    const MCJob& J = jobs[j];
+    mc_emit_block(S, J.C, J.R, J.geo, (int)J.m.list[1 + u - lpre[j]], J.m.nblocks, J.m.blockV, J.m.blockT, J.m.cellV,
+                  J.m.V, J.m.Nrm, J.m.capV, J.m.T, J.m.capT, J.m.late, J.m.capLate, J.m.totals);
+  }
+}
+__global__ void k_mc_fixup_b(const MCJob* __restrict__ jobs) {
+  if (threadIdx.x != 0) return;
+  const MCJob& J = jobs[blockIdx.x];
+  mc_fixup_body(J.m.V, J.m.Nrm, J.m.T, J.m.late, J.m.capLate, J.m.totals, J.m.capV, J.m.capT);
+}
+constexpr int kNormWgPerJob = 16;
+__global__ void __launch_bounds__(256) k_mc_normalize_b(const MCJob* __restrict__ jobs) {
+  const MCJob& J = jobs[blockIdx.x / kNormWgPerJob];
+  const int l = (int)(blockIdx.x % kNormWgPerJob);
+  mc_normalize_body(J.m.Nrm, J.m.totals, J.m.capV, (size_t)l * blockDim.x + threadIdx.x,
+                    (size_t)kNormWgPerJob * blockDim.x);
+}
+__global__ void k_totals_b(const MCJob* __restrict__ jobs, int n, u32* out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  for (int k = 0; k < 4; ++k) out[4 * j + k] = jobs[j].m.totals[k];
+}
 
 hipError_t upload_tables_once() {
   McConst h{};
@@ -849,6 +961,89 @@ hipError_t launch_mc(const u32* C, int R, const float* bmin3, float cell, const 
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   return launch_mc_emit(C, R, bmin3, cell, m, s, ev ? ev + 2 : nullptr);
+}
+
+namespace {
+size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+struct McBlobLayout {
+  size_t jobs, bricks, preB, preK, preC, up, lpre, totals, all;
+  explicit McBlobLayout(int n) {
+    jobs = 0;
+    bricks = align16(jobs + (size_t)n * sizeof(MCJob));
+    preB = align16(bricks + (size_t)n * sizeof(BrickJob));
+    preK = align16(preB + (size_t)(n + 1) * 4);
+    preC = align16(preK + (size_t)(n + 1) * 4);
+    up = align16(preC + (size_t)(n + 1) * 4);  // bytes uploaded from the host blob
+    lpre = up;
+    totals = align16(lpre + (size_t)(n + 1) * 4);
+    all = align16(totals + (size_t)n * 16);
+  }
+};
+}  // namespace
+
+size_t batch_mc_bytes(int n) { return McBlobLayout(n).all; }
+
+hipError_t launch_mc_batched(const BatchMCJob* in, int n, void* hBlob, void* dBlob, u32* hTotals, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  GM_CHECK(upload_tables());
+  const McBlobLayout L(n);
+  char* h = static_cast<char*>(hBlob);
+  char* d = static_cast<char*>(dBlob);
+  MCJob* hj = reinterpret_cast<MCJob*>(h + L.jobs);
+  BrickJob* hb = reinterpret_cast<BrickJob*>(h + L.bricks);
+  u32* preB = reinterpret_cast<u32*>(h + L.preB);
+  u32* preK = reinterpret_cast<u32*>(h + L.preK);
+  u32* preC = reinterpret_cast<u32*>(h + L.preC);
+  preB[0] = preK[0] = preC[0] = 0;
+  long long blocksAll = 0;
+  int wmax = 1;
+  for (int j = 0; j < n; ++j) {
+    const BatchMCJob& J = in[j];
+    const int N = 2 * J.R;
+    hj[j].C = J.C;
+    hj[j].R = J.R;
+    hj[j].geo = Geo{J.bmin[0], J.bmin[1], J.bmin[2], 0.5f * J.cell};
+    hj[j].m = J.m;
+    hb[j] = BrickJob{J.C, J.m.brick, J.m.list, J.m.late, N};
+    preB[j + 1] = preB[j] + (u32)((N / 8) * (N / 8));
+    preK[j + 1] = preK[j] + (u32)((J.m.nblocks + 255) / 256);
+    preC[j + 1] = preC[j] + (u32)mc_scan_parts(J.m.nblocks);
+    blocksAll += J.m.nblocks;
+    wmax = std::max(wmax, N / 32);
+  }
+  GM_CHECK(hipMemcpyAsync(d, h, L.up, hipMemcpyHostToDevice, s));
+  const MCJob* dj = reinterpret_cast<const MCJob*>(d + L.jobs);
+  const u32* dPreK = reinterpret_cast<const u32*>(d + L.preK);
+  const u32* dPreC = reinterpret_cast<const u32*>(d + L.preC);
+  u32* lpre = reinterpret_cast<u32*>(d + L.lpre);
+  u32* tot = reinterpret_cast<u32*>(d + L.totals);
+  // active Morton blocks from brick summaries of the closed grids (their
+  // list and late counters cleared there)
+  GM_CHECK(launch_brick_summary_batched(reinterpret_cast<const BrickJob*>(d + L.bricks),
+                                        reinterpret_cast<const u32*>(d + L.preB), n, preB[n], wmax, s));
+  hipLaunchKernelGGL(k_mc_blocks_b, dim3(preK[n]), dim3(256), 0, s, dj, dPreK, n);
+  GM_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_list_prefix, dim3(1), dim3(64), 0, s, dj, n, lpre);
+  GM_CHECK(hipGetLastError());
+  const unsigned listGrid = (unsigned)std::max<long long>(1, std::min<long long>(blocksAll, 16384));
+  hipLaunchKernelGGL(k_mc_count_b, dim3(std::min(listGrid, 4096u)), dim3(kMcThreads), 0, s, dj, lpre, n);
+  GM_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_scan_reduce_b, dim3(preC[n]), dim3(256), 0, s, dj, dPreC, n);
+  GM_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_scan_top_b, dim3(n), dim3(256), 0, s, dj);
+  GM_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_scan_down_b, dim3(preC[n]), dim3(256), 0, s, dj, dPreC, n);
+  GM_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_mc_emit_b, dim3(listGrid), dim3(kMcThreads), 0, s, dj, lpre, n);
+  GM_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_mc_fixup_b, dim3(n), dim3(64), 0, s, dj);
+  GM_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_mc_normalize_b, dim3(n * kNormWgPerJob), dim3(256), 0, s, dj);
+  GM_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_totals_b, dim3((n + 255) / 256), dim3(256), 0, s, dj, n, tot);
+  GM_CHECK(hipGetLastError());
+  GM_CHECK(hipMemcpyAsync(hTotals, tot, (size_t)n * 16, hipMemcpyDeviceToHost, s));
+  return hipSuccess;
 }
 
 hipError_t launch_mc_emit(const u32* C, int R, const float* bmin3, float cell, const MCBuffers& m, hipStream_t s,

@@ -57,8 +57,7 @@ __global__ void __launch_bounds__(64 * kMaxWaves) k_ero_coarse(MorphArgs a) { ru
 __global__ void __launch_bounds__(64 * kMaxWaves) k_ero_fine(MorphArgs a) { run_pass<3, 0>(a); }
 
 // Tight contour of DF -> TG flags (one thread per cell-row word).
-__global__ void k_tight(MorphArgs a) {
-  long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void tight_body(const MorphArgs& a, long long gid) {
   long long total = (long long)a.R * a.R * a.W;
   if (gid >= total) return;
   int wx = (int)(gid % a.W);
@@ -87,6 +86,7 @@ __global__ void k_tight(MorphArgs a) {
     a.outRows[o] = df | a.I[o];
   }
 }
+__global__ void k_tight(MorphArgs a) { tight_body(a, (long long)blockIdx.x * blockDim.x + threadIdx.x); }
 
 // Brick summary of a grid (brick = 1 word x 8 rows x 8 planes; bit0 = any
 // voxel set, bit1 = all set). Workgroup = one 8x8 (row, plane) brick column
@@ -94,45 +94,69 @@ __global__ void k_tight(MorphArgs a) {
 // one contiguous read of 8 rows; the 8 rows of a word meet in LDS.
 // clear (optional): a counter the next kernel starts from, zeroed here so
 // that no separate memset is needed.
-__global__ void __launch_bounds__(512) k_brick_summary(const u32* __restrict__ G, int N, int W,
-                                                        uint8_t* __restrict__ out, u32* clear) {
-  if (clear && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *clear = 0u;
+// Workgroups may have more than 8*W threads (batched launches size them for
+// the widest job); the extra threads only meet the barriers.
+__device__ __forceinline__ void brick_summary_body(const u32* __restrict__ G, int N, int W, uint8_t* __restrict__ out,
+                                                   int by, int bz) {
   __shared__ u32 anyW[64], allW[64];
   const int NB = N / 8;
-  const int by = blockIdx.x, bz = blockIdx.y;
+  const bool act = threadIdx.x < 8u * (unsigned)W;
   const int w = threadIdx.x % W, yl = threadIdx.x / W;
   if (threadIdx.x < W) {
     anyW[threadIdx.x] = 0u;
     allW[threadIdx.x] = 1u;
   }
   __syncthreads();
-  const int y = 8 * by + yl;
-  u32 any = 0u, all = 0xffffffffu;
-  const u32* g = G + ((size_t)(8 * bz) * N + y) * W + w;
+  if (act) {
+    const int y = 8 * by + yl;
+    u32 any = 0u, all = 0xffffffffu;
+    const u32* g = G + ((size_t)(8 * bz) * N + y) * W + w;
 #pragma unroll
-  for (int zl = 0; zl < 8; ++zl) {
-    u32 v = g[(size_t)zl * N * W];
-    any |= v;
-    all &= v;
+    for (int zl = 0; zl < 8; ++zl) {
+      u32 v = g[(size_t)zl * N * W];
+      any |= v;
+      all &= v;
+    }
+    if (any) atomicOr(&anyW[w], 1u);
+    if (all != 0xffffffffu) atomicAnd(&allW[w], 0u);
   }
-  if (any) atomicOr(&anyW[w], 1u);
-  if (all != 0xffffffffu) atomicAnd(&allW[w], 0u);
   __syncthreads();
   if (threadIdx.x < W)
     out[((size_t)bz * NB + by) * W + threadIdx.x] = (uint8_t)(anyW[threadIdx.x] | (allW[threadIdx.x] << 1));
+}
+__global__ void __launch_bounds__(512) k_brick_summary(const u32* __restrict__ G, int N, int W,
+                                                        uint8_t* __restrict__ out, u32* clear) {
+  if (clear && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *clear = 0u;
+  brick_summary_body(G, N, W, out, blockIdx.x, blockIdx.y);
 }
 
 // Outer contour K of D (Grid.cu:1230-1327) and its brick summary: voxels
 // outside D, 6-adjacent to D, in a cell that is not full, away from the data
 // border. Same workgroup mapping as k_brick_summary.
 // kbl (optional): the bricks holding K are appended to it ([0] count).
-__global__ void __launch_bounds__(512) k_contour_K(MorphArgs a, uint8_t* __restrict__ brickK, u32* kbl) {
+__device__ void contour_K_rows(const MorphArgs& a, int wx, int yl, int by, int bz, u32* anyW);
+__device__ __forceinline__ void contour_K_body(const MorphArgs& a, uint8_t* __restrict__ brickK, u32* kbl, int by,
+                                               int bz) {
   __shared__ u32 anyW[64];
   const int N = a.N, W = a.W, NB = N / 8;
-  const int by = blockIdx.x, bz = blockIdx.y;
   const int wx = threadIdx.x % W, yl = threadIdx.x / W;
   if (threadIdx.x < W) anyW[threadIdx.x] = 0u;
   __syncthreads();
+  if (threadIdx.x < 8u * (unsigned)W) contour_K_rows(a, wx, yl, by, bz, anyW);
+  __syncthreads();
+  if (threadIdx.x < W) {
+    const size_t b = ((size_t)bz * NB + by) * W + threadIdx.x;
+    brickK[b] = (uint8_t)anyW[threadIdx.x];
+    if (kbl && anyW[threadIdx.x]) kbl[1 + atomicAdd(kbl, 1u)] = (u32)b;
+  }
+}
+__global__ void __launch_bounds__(512) k_contour_K(MorphArgs a, uint8_t* __restrict__ brickK, u32* kbl) {
+  contour_K_body(a, brickK, kbl, blockIdx.x, blockIdx.y);
+}
+
+// The rows of thread (wx, yl) of brick row (by, bz) for contour_K_body.
+__device__ void contour_K_rows(const MorphArgs& a, int wx, int yl, int by, int bz, u32* anyW) {
+  const int N = a.N, W = a.W;
   const u32* D = a.D;
   const int y = 8 * by + yl;
   const int cy = y >> 1;
@@ -164,12 +188,6 @@ __global__ void __launch_bounds__(512) k_contour_K(MorphArgs a, uint8_t* __restr
     c = zp;
   }
   if (any) atomicOr(&anyW[wx], 1u);
-  __syncthreads();
-  if (threadIdx.x < W) {
-    const size_t b = ((size_t)bz * NB + by) * W + threadIdx.x;
-    brickK[b] = (uint8_t)anyW[threadIdx.x];
-    if (kbl && anyW[threadIdx.x]) kbl[1 + atomicAdd(kbl, 1u)] = (u32)b;
-  }
 }
 
 // Radius values of the 32 voxels of row word (y, z, w) of a lattice field
@@ -312,8 +330,7 @@ __device__ __forceinline__ void cprime_rows(const MorphArgs& a, int cy, int cz, 
 }
 
 // Base (6-connected) contour of C' -> UG flags, one thread per cell-row word.
-__global__ void k_base_contour(MorphArgs a) {
-  long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void base_contour_body(const MorphArgs& a, long long gid) {
   long long total = (long long)a.R * a.R * a.W;
   if (gid >= total) return;
   int wx = (int)(gid % a.W);
@@ -352,6 +369,53 @@ __global__ void k_base_contour(MorphArgs a) {
     for (int s = 0; s < 4; ++s) diff |= c[s] ^ o[s];
   }
   put(gather2(diff | (diff >> 1)) & valid);
+}
+__global__ void k_base_contour(MorphArgs a) {
+  base_contour_body(a, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// ---- Batched forms (many jobs per launch): the job of a workgroup or wave
+// comes from the prefix array of the per-job counts (batch_job).
+__global__ void __launch_bounds__(512) k_brick_summary_b(const MorphArgs* __restrict__ tab,
+                                                          const u32* __restrict__ pre, int n) {
+  const int j = batch_job(pre, n, blockIdx.x);
+  const MorphArgs& a = tab[j];
+  const int NB = a.N / 8, l = (int)(blockIdx.x - pre[j]);
+  brick_summary_body(a.I, a.N, a.W, const_cast<uint8_t*>(a.brickI), l % NB, l / NB);
+}
+__global__ void __launch_bounds__(512) k_brick_summary_gb(const BrickJob* __restrict__ jobs,
+                                                           const u32* __restrict__ pre, int n) {
+  const int j = batch_job(pre, n, blockIdx.x);
+  const BrickJob& J = jobs[j];
+  const int NB = J.N / 8, l = (int)(blockIdx.x - pre[j]);
+  if (l == 0 && threadIdx.x == 0) {
+    if (J.clear0) *J.clear0 = 0u;
+    if (J.clear1) *J.clear1 = 0u;
+  }
+  brick_summary_body(J.G, J.N, J.N / 32, J.out, l % NB, l / NB);
+}
+__global__ void __launch_bounds__(512) k_contour_K_b(const MorphArgs* __restrict__ tab, const u32* __restrict__ pre,
+                                                      int n) {
+  const int j = batch_job(pre, n, blockIdx.x);
+  const MorphArgs& a = tab[j];
+  const int NB = a.N / 8, l = (int)(blockIdx.x - pre[j]);
+  contour_K_body(a, const_cast<uint8_t*>(a.brickK), nullptr, l % NB, l / NB);
+}
+__global__ void __launch_bounds__(256) k_tight_b(const MorphArgs* __restrict__ tab, const u32* __restrict__ pre,
+                                                  int n) {
+  const int j = batch_job(pre, n, blockIdx.x);
+  tight_body(tab[j], (long long)(blockIdx.x - pre[j]) * blockDim.x + threadIdx.x);
+}
+__global__ void __launch_bounds__(256) k_base_contour_b(const MorphArgs* __restrict__ tab,
+                                                         const u32* __restrict__ pre, int n) {
+  const int j = batch_job(pre, n, blockIdx.x);
+  base_contour_body(tab[j], (long long)(blockIdx.x - pre[j]) * blockDim.x + threadIdx.x);
+}
+template <int PASS>
+__global__ void __launch_bounds__(256) k_select_b(const MorphArgs* __restrict__ tab, const u32* __restrict__ pre, int n,
+                                                   const int* __restrict__ rad, u32* queue, int qsel, int listBase,
+                                                   int cap) {
+  select_tiles_batched<PASS>(tab, pre, n, rad, queue, qsel, listBase, cap);
 }
 
 // LDS bytes of a streaming launch with `waves` waves per workgroup.
@@ -519,6 +583,13 @@ hipError_t launch_radius_planes(const RadiusField& rf, int N, u32* rp, u32* rvse
   return hipGetLastError();
 }
 
+hipError_t launch_brick_summary_batched(const BrickJob* dJobs, const u32* dPre, int n, u32 blocks, int wmax,
+                                        hipStream_t s) {
+  if (n <= 0 || blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_brick_summary_gb, dim3(blocks), dim3(8 * wmax), 0, s, dJobs, dPre, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_brick_summary(const u32* G, int N, uint8_t* out, u32* clear, hipStream_t s) {
   hipLaunchKernelGGL(k_brick_summary, dim3(N / 8, N / 8), dim3(8 * (N / 32)), 0, s, G, N, N / 32, out, clear);
   return hipGetLastError();
@@ -617,6 +688,131 @@ hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusF
   if (perValue) GM_CHECK(launch_pass_var(3, a, b, vlo, vhi, &vprev, s));
   else GM_CHECK(launch_pass(3, reinterpret_cast<const void*>(&k_ero_fine), a, b.qsel, var, fixedOk, rho, s));
   if (ev) GM_CHECK(hipEventRecord(ev[4], s));
+  return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// Batched closing of many uniform-radius sphere jobs: every step is one
+// launch over all jobs (per radius value for the streaming passes), with the
+// per-job arguments built exactly as launch_dilate / launch_erode build them.
+size_t batch_morph_bytes(int n) {
+  return (size_t)7 * n * sizeof(MorphArgs) + (size_t)4 * (n + 1) * sizeof(u32) + (size_t)n * sizeof(int);
+}
+size_t batch_queue_cap(const BatchMorphJob* jobs, int n) {
+  size_t cap = 0;
+  for (int j = 0; j < n; ++j) cap += var_queue_cap(jobs[j].d.R);
+  return cap;
+}
+
+hipError_t launch_close_batched(const BatchMorphJob* jobs, int n, void* hBlob, void* dBlob, u32* queue, int cap,
+                                int* qsel, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  MorphArgs* hT = static_cast<MorphArgs*>(hBlob);
+  u32* hP = reinterpret_cast<u32*>(hT + 7 * n);
+  int* hR = reinterpret_cast<int*>(hP + 4 * (n + 1));
+  MorphArgs* dT = static_cast<MorphArgs*>(dBlob);
+  u32* dP = reinterpret_cast<u32*>(dT + 7 * n);
+  int* dR = reinterpret_cast<int*>(dP + 4 * (n + 1));
+  u32* preB = hP;                  // brick rows (N/8)^2 per job
+  u32* preW = hP + (n + 1);        // cell-row words / 256
+  u32* preT2 = hP + 2 * (n + 1);   // tiles of the coarse passes (ystep 2)
+  u32* preT1 = hP + 3 * (n + 1);   // tiles of the fine passes
+  long long tilesOf[2][kCtMaxRho + 1] = {};
+  bool present[kCtMaxRho + 1] = {};
+  int wmax = 1;
+  preB[0] = preW[0] = preT2[0] = preT1[0] = 0;
+  const SETableSet noTabs{};
+  for (int j = 0; j < n; ++j) {
+    const BatchMorphJob& J = jobs[j];
+    const int r = J.r;
+    if (r < 2 || r > kCtMaxRho - 1) return hipErrorInvalidValue;
+    RadiusField rf{};
+    rf.mode = 0;
+    rf.r = r;
+    rf.N = J.d.N;
+    MorphArgs a = base_args(J.d, J.b, rf, noTabs);
+    a.se = 0;
+    a.tabWords = 0;
+    a.maxm = r;        // dilation rho = r + 1: corner m <= rho - 1, offsets <= rho
+    a.maxoff = r + 1;
+    MorphArgs e = a;   // erosion rho = r - 1
+    e.maxm = e.maxoff = r - 1;
+    MorphArgs t[7] = {a, a, a, e, e, e, e};
+    t[0].ystep = 2;
+    t[0].outFlags = J.b.DF;
+    t[1].outFlags = J.b.TG;
+    t[1].outRows = J.b.D;
+    t[2].ystep = 1;
+    t[2].outFlags = J.b.TG;
+    t[2].outRows = J.b.D;
+    t[3].outRows = J.b.K;
+    t[4].ystep = 2;
+    t[4].outFlags = J.b.CL;
+    t[5].outFlags = J.b.UG;
+    t[5].outRows = J.b.C;
+    t[6].ystep = 1;
+    t[6].outFlags = J.b.UG;
+    t[6].outRows = J.b.C;
+    for (int k = 0; k < 7; ++k) hT[k * n + j] = t[k];
+    const int N = J.d.N, W = J.d.W, R = J.d.R;
+    const u32 t2 = (u32)(W * ((N + 2 * kTileRows - 1) / (2 * kTileRows)) * ((N + kTileZ - 1) / kTileZ));
+    const u32 t1 = (u32)tile_count_max(R);
+    preB[j + 1] = preB[j] + (u32)((N / 8) * (N / 8));
+    preW[j + 1] = preW[j] + (u32)(((long long)R * R * W + 255) / 256);
+    preT2[j + 1] = preT2[j] + t2;
+    preT1[j + 1] = preT1[j] + t1;
+    hR[j] = r;
+    present[r] = true;
+    tilesOf[0][r] += t2;
+    tilesOf[1][r] += t1;
+    wmax = std::max(wmax, W);
+  }
+  GM_CHECK(hipMemcpyAsync(dBlob, hBlob, batch_morph_bytes(n), hipMemcpyHostToDevice, s));
+  const CtKernels& ct = ct_kernels();
+  // one streaming pass: selection into per-radius queues, then one launch
+  // per radius present
+  auto pass = [&](int p, const MorphArgs* tab, const u32* pre, u32 units) -> hipError_t {
+    const int q = *qsel;
+    *qsel = 1 - q;
+    const void* sel[4] = {reinterpret_cast<const void*>(&k_select_b<0>), reinterpret_cast<const void*>(&k_select_b<1>),
+                          reinterpret_cast<const void*>(&k_select_b<2>), reinterpret_cast<const void*>(&k_select_b<3>)};
+    int nn = n, qq = q, lb = kQueueLists, cp = cap;
+    void* sargs[] = {&tab, &pre, &nn, &dR, &queue, &qq, &lb, &cp};
+    GM_CHECK(hipLaunchKernel(sel[p], dim3((units + 3) / 4), dim3(256), sargs, 0, s));
+    for (int r = 2; r <= kCtMaxRho - 1; ++r) {
+      if (!present[r]) continue;
+      const int rho = p < 2 ? r + 1 : r - 1;
+      const void* fn = ct.fnB[p][rho];
+      if (!fn) return hipErrorInvalidValue;
+      MorphArgs sz{};
+      sz.Zt = kTileZ;
+      sz.ystep = (p == 0 || p == 2) ? 2 : 1;
+      sz.maxm = p < 2 ? rho - 1 : rho;
+      sz.maxoff = rho;
+      sz.tabWords = 0;
+      const int nw = stream_waves(sz, false);
+      const size_t lds = lds_bytes(sz, false, nw);
+      const long long tl = tilesOf[(p == 0 || p == 2) ? 0 : 1][r];
+      const unsigned grid = (unsigned)std::min<long long>((tl + 7) & ~7LL, 8192);
+      int qc = kVarCounters + 256 * q + 8 * r, ql = kQueueLists + 8 * r * cap, qcap = cap;
+      const u32* cq = queue;
+      void* args[] = {&tab, &cq, &qc, &ql, &qcap};
+      GM_CHECK(hipLaunchKernel(fn, dim3(grid), dim3(64 * nw), args, lds, s));
+    }
+    return hipGetLastError();
+  };
+  hipLaunchKernelGGL(k_brick_summary_b, dim3(preB[n]), dim3(8 * wmax), 0, s, dT, dP, n);
+  GM_CHECK(hipGetLastError());
+  GM_CHECK(pass(0, dT, dP + 2 * (n + 1), preT2[n]));
+  hipLaunchKernelGGL(k_tight_b, dim3(preW[n]), dim3(256), 0, s, dT + n, dP + (n + 1), n);
+  GM_CHECK(hipGetLastError());
+  GM_CHECK(pass(1, dT + 2 * n, dP + 3 * (n + 1), preT1[n]));
+  hipLaunchKernelGGL(k_contour_K_b, dim3(preB[n]), dim3(8 * wmax), 0, s, dT + 3 * n, dP, n);
+  GM_CHECK(hipGetLastError());
+  GM_CHECK(pass(2, dT + 4 * n, dP + 2 * (n + 1), preT2[n]));
+  hipLaunchKernelGGL(k_base_contour_b, dim3(preW[n]), dim3(256), 0, s, dT + 5 * n, dP + (n + 1), n);
+  GM_CHECK(hipGetLastError());
+  GM_CHECK(pass(3, dT + 6 * n, dP + 3 * (n + 1), preT1[n]));
   return hipSuccess;
 }
 

@@ -18,6 +18,14 @@ __global__ void __launch_bounds__(64 * kMaxWaves) k_pass_ct(MorphArgs a) {
   run_pass<PASS, RHO>(a);
 }
 
+// Many jobs of radius RHO in one launch (entries name job and tile).
+template <int PASS, int RHO>
+__global__ void __launch_bounds__(64 * kMaxWaves) k_pass_ct_b(const MorphArgs* __restrict__ tab,
+                                                               const u32* __restrict__ queue, int qcOff, int qlOff,
+                                                               int qCap) {
+  run_pass_batched<PASS, RHO>(tab, queue, qcOff, qlOff, qCap);
+}
+
 // Dilation uses rho = r + 1 <= 32 (corner tables); erosion rho = r - 1 <= 30
 // (corner tables for the coarse pass, centre tables for the fine pass).
 template <int RHO>
@@ -25,9 +33,13 @@ void register_rho(CtKernels& k) {
   if constexpr (RHO % kCtParts == GM_CT_PART) {
     k.fn[0][RHO] = reinterpret_cast<const void*>(&k_pass_ct<0, RHO>);
     k.fn[1][RHO] = reinterpret_cast<const void*>(&k_pass_ct<1, RHO>);
+    k.fnB[0][RHO] = reinterpret_cast<const void*>(&k_pass_ct_b<0, RHO>);
+    k.fnB[1][RHO] = reinterpret_cast<const void*>(&k_pass_ct_b<1, RHO>);
     if constexpr (RHO <= kCtMaxRho - 2) {
       k.fn[2][RHO] = reinterpret_cast<const void*>(&k_pass_ct<2, RHO>);
       k.fn[3][RHO] = reinterpret_cast<const void*>(&k_pass_ct<3, RHO>);
+      k.fnB[2][RHO] = reinterpret_cast<const void*>(&k_pass_ct_b<2, RHO>);
+      k.fnB[3][RHO] = reinterpret_cast<const void*>(&k_pass_ct_b<3, RHO>);
     }
   }
 }

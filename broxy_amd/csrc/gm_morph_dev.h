@@ -987,22 +987,18 @@ __device__ __forceinline__ TileGeo pass_tile(const MorphArgs& a, int t) {
 // bricks holding K); the coarse erosion output of every tile is cleared here
 // for the per-value launches to OR into (the coarse dilation output is set
 // to I-full beforehand).
+// Skip test of tile t for pass PASS (one wave): finishes a skipped tile (a
+// coarse pass stores its constant), returns whether the tile is active. With
+// per-voxel radii, vals gets the radius values that reach the tile and lane v
+// the brick planes [blo, bhi] of value v.
 template <int PASS>
-__device__ void select_tiles(const MorphArgs& a) {
-  if (blockIdx.x == 0) {
-    // the counter blocks of the next pass (the previous pass has finished)
-    if (threadIdx.x < 16) a.queue[16 * (1 - a.qsel) + threadIdx.x] = 0u;
-    for (int i = threadIdx.x; i < 256; i += blockDim.x) a.queue[kVarCounters + 256 * (1 - a.qsel) + i] = 0u;
-  }
-  u32* qc = a.queue + 16 * a.qsel;
-  const int t = blockIdx.x * (int)(blockDim.x >> 6) + wave_id();
-  if (t >= tile_count(a)) return;
-  const TileGeo g = pass_tile<PASS>(a, t);
+__device__ bool select_tile(const MorphArgs& a, int t, const TileGeo& g, u32& vals, int& blo, int& bhi) {
   const int lane = lane_id();
   const int y1 = g.yfirst + a.ystep * kTileRows;
   bool skip;
-  u32 vals = 0u;  // radius values to queue the tile for (per-voxel radii)
-  int blo = 0, bhi = -1;  // lane v: brick planes holding value v
+  vals = 0u;
+  blo = 0;
+  bhi = -1;
   if constexpr (PASS == 0) {
     // F = 0 when no occupied voxel is in reach; F unused when every cell is full
     const bool empty = !wave_bricks_any(a.brickI, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff,
@@ -1038,7 +1034,32 @@ __device__ void select_tiles(const MorphArgs& a) {
                                 g.zfirst - a.maxoff, g.zlast + 1 + a.maxoff, g.wx - 1, g.wx + 1, blo, bhi);
     }
   }
-  if (skip) return;
+  return !skip;
+}
+
+// Tile selection, one wave per tile: a tile whose result is known without the
+// predicate is finished here (coarse passes store their constant; the fine
+// passes' default output is already written), every other tile is appended to
+// the queue of its slab. With per-voxel radii the tile goes to the queue of
+// every radius value that can reach it (per-brick value sets; erosion: of the
+// bricks holding K); the coarse erosion output of every tile is cleared here
+// for the per-value launches to OR into (the coarse dilation output is set
+// to I-full beforehand).
+template <int PASS>
+__device__ void select_tiles(const MorphArgs& a) {
+  if (blockIdx.x == 0) {
+    // the counter blocks of the next pass (the previous pass has finished)
+    if (threadIdx.x < 16) a.queue[16 * (1 - a.qsel) + threadIdx.x] = 0u;
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) a.queue[kVarCounters + 256 * (1 - a.qsel) + i] = 0u;
+  }
+  u32* qc = a.queue + 16 * a.qsel;
+  const int t = blockIdx.x * (int)(blockDim.x >> 6) + wave_id();
+  if (t >= tile_count(a)) return;
+  const TileGeo g = pass_tile<PASS>(a, t);
+  const int lane = lane_id();
+  u32 vals;
+  int blo, bhi;  // lane v: brick planes holding value v
+  if (!select_tile<PASS>(a, t, g, vals, blo, bhi)) return;
   const int x = slab_queue(t / a.W);
   if (a.rvar) {
     // lane v queues the tile for radius value v, with the planes that can
@@ -1057,6 +1078,40 @@ __device__ void select_tiles(const MorphArgs& a) {
   } else if (lane == 0) {
     const u32 k = atomicAdd(&qc[x], 1u);
     a.queue[kQueueList + x * a.queueCap + k] = entry_pack(t, g.tz0, -(1 << 20), 1 << 20);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Many jobs in one launch. A batched kernel gets the per-job arguments as a
+// table in device memory (a job's entry is exactly its single-job launch
+// argument) and a prefix array pre[0..n] of the per-job work units (tiles,
+// blocks, words); unit u belongs to the job j with pre[j] <= u < pre[j+1].
+
+// Batched queue entries: job in the high kBatchJobBits bits, tile below.
+constexpr int kBatchTileBits = 20;
+constexpr int kBatchJobBits = 12;
+
+// Batched tile selection (uniform radius per job): the tile of job j is
+// queued under the job's radius value v = rad[j] (per-value counters at
+// kVarCounters + 256*qsel + 8v + x, lists at listBase + (8v + x)*cap).
+template <int PASS>
+__device__ void select_tiles_batched(const MorphArgs* __restrict__ tab, const u32* __restrict__ pre, int n,
+                                     const int* __restrict__ rad, u32* queue, int qsel, int listBase, int cap) {
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) queue[kVarCounters + 256 * (1 - qsel) + i] = 0u;
+  const u32 u = blockIdx.x * (blockDim.x >> 6) + (u32)wave_id();
+  if (u >= pre[n]) return;
+  const int j = batch_job(pre, n, u);
+  const MorphArgs& a = tab[j];
+  const int t = (int)(u - pre[j]);
+  const TileGeo g = pass_tile<PASS>(a, t);
+  u32 vals;
+  int blo, bhi;
+  if (!select_tile<PASS>(a, t, g, vals, blo, bhi)) return;
+  if (lane_id() == 0) {
+    const int q = rad[j] * 8 + slab_queue(t / a.W);
+    const u32 k = atomicAdd(&queue[kVarCounters + 256 * qsel + q], 1u);
+    queue[listBase + (size_t)q * cap + k] = ((u32)j << kBatchTileBits) | (u32)t;
   }
 }
 
@@ -1100,6 +1155,25 @@ __device__ void run_pass(const MorphArgs& a) {
   for (u32 k = blockIdx.x >> 3; k < qc[x]; k += gridDim.x >> 3) {
     run_pass_tile<PASS, RHO>(a, ql[k], smem, set4);
     __syncthreads();  // LDS is reused by the next tile
+  }
+}
+
+// Batched form of run_pass: the entries name (job, tile) and the job's
+// arguments come from the table; same queue walk.
+template <int PASS, int RHO>
+__device__ void run_pass_batched(const MorphArgs* __restrict__ tab, const u32* __restrict__ queue, int qcOff,
+                                 int qlOff, int qCap) {
+  extern __shared__ u32 smem[];
+  __shared__ u32 set4[4];
+  const int x = (int)(blockIdx.x & 7u);
+  const u32* qc = queue + qcOff;
+  const u32* ql = queue + qlOff + (size_t)x * qCap;
+  for (u32 k = blockIdx.x >> 3; k < qc[x]; k += gridDim.x >> 3) {
+    const u32 e = __builtin_amdgcn_readfirstlane(ql[k]);
+    const MorphArgs& a = tab[e >> kBatchTileBits];
+    const int t = (int)(e & ((1u << kBatchTileBits) - 1u));
+    run_pass_tile<PASS, RHO>(a, entry_pack(t, pass_tile<PASS>(a, t).tz0, -(1 << 20), 1 << 20), smem, set4);
+    __syncthreads();
   }
 }
 

@@ -134,6 +134,21 @@ int gm_get_stream(gm_ctx* ctx, void** stream);
  * terminates). Host only. */
 int gm_mc_tables(uint32_t* ntri256, uint32_t* len256, uint64_t* tri256);
 
+/* ---- Batched closing. No counterpart in the reference (one closing per
+ * GMorpho::Update, GMorpho.cu:7775); for callers with many grids at once.
+ * gm_batch_close_mesh does, for each context, what gm_close(ctx, se)
+ * followed by gm_extract_mesh(ctx) does, with identical results: contexts
+ * with a uniform radius in [2, 31] and the sphere run together, one kernel
+ * launch per step for all of them on the batch's stream; the others run on
+ * their own streams. The contexts must be on the batch's device and hold a
+ * grid and a radius; it returns when every mesh is ready for gm_mesh_counts
+ * and gm_mesh_copy. One batch runs one call at a time. */
+typedef struct gm_batch gm_batch;
+int gm_batch_create(int device, gm_batch** out);
+int gm_batch_destroy(gm_batch* b);
+const char* gm_batch_last_error(const gm_batch* b);
+int gm_batch_close_mesh(gm_batch* b, gm_ctx* const* ctxs, int n, int se);
+
 /* ---- Job engine. No counterpart in the reference, which runs one closing
  * at a time from its viewer (GMorpho::Update, GMorpho.cu:7775); it is for
  * callers with many jobs: batches of uneven grids and radii, and a serving

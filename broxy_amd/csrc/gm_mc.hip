@@ -288,8 +288,8 @@ __global__ void __launch_bounds__(256) k_mc_blocks(const uint8_t* __restrict__ b
 
 // Vertex and triangle counts of one listed block, the block-local vertex base
 // of each of its cells with vertices.
-__device__ void mc_count_block(const u32* __restrict__ C, int R, int blk, u32* blockV, u32* blockT, u32* cellL,
-                               Box& bx, u32* red, const McTab& tab) {
+__device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R, int blk, u32* blockV, u32* blockT,
+                                               u32* cellL, Box& bx, u32* red, const McTab& tab) {
   int c0x, c0y, c0z;
   block_outside(blk, R, c0x, c0y, c0z);
   // cells of the block need fine voxels [16b, 16b+17) per axis
@@ -699,10 +699,11 @@ __device__ void mc_tris_block(BlockLds& S, int R, int blk, int c0x, int c0y, int
 }
 
 // One listed block: staged once, then its vertices and its triangles.
-__device__ void mc_emit_block(BlockLds& S, const u32* __restrict__ C, int R, Geo geo, int blk, int nblocks,
-                              const u32* __restrict__ blockV, const u32* __restrict__ blockT,
-                              const u32* __restrict__ cellL, float* V, float* Nrm, u32 capV, u32* T, u32 capT,
-                              u32* late, u32 capLate, u32* totals) {
+__device__ __forceinline__ void mc_emit_block(BlockLds& S, const u32* __restrict__ C, int R, Geo geo, int blk,
+                                              int nblocks, const u32* __restrict__ blockV,
+                                              const u32* __restrict__ blockT, const u32* __restrict__ cellL, float* V,
+                                              float* Nrm, u32 capV, u32* T, u32 capT, u32* late, u32 capLate,
+                                              u32* totals) {
   int c0x, c0y, c0z;
   block_outside(blk, R, c0x, c0y, c0z);
   const u32 base = blockV[blk];
@@ -752,8 +753,8 @@ __device__ __forceinline__ void face(const float* V, const u32* t, float* n) {
 
 // Applies the listed-but-uncounted triangles in slot order (single thread):
 // each replaces the triangle in its slot, and the normal sums follow.
-__device__ void mc_fixup_body(const float* V, float* Nrm, u32* T, u32* late, u32 capLate, u32* totals, u32 capV,
-                              u32 capT) {
+__device__ __forceinline__ void mc_fixup_body(const float* V, float* Nrm, u32* T, u32* late, u32 capLate,
+                                              u32* totals, u32 capV, u32 capT) {
   u32 n = late[0];
   if (n > capLate) n = capLate;
   if (n == 0) return;

@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(512) k_brick_summary(const u32* __restrict__ G
 // outside D, 6-adjacent to D, in a cell that is not full, away from the data
 // border. Same workgroup mapping as k_brick_summary.
 // kbl (optional): the bricks holding K are appended to it ([0] count).
-__device__ void contour_K_rows(const MorphArgs& a, int wx, int yl, int by, int bz, u32* anyW);
+__device__ __forceinline__ void contour_K_rows(const MorphArgs& a, int wx, int yl, int by, int bz, u32* anyW);
 __device__ __forceinline__ void contour_K_body(const MorphArgs& a, uint8_t* __restrict__ brickK, u32* kbl, int by,
                                                int bz) {
   __shared__ u32 anyW[64];
@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(512) k_contour_K(MorphArgs a, uint8_t* __restr
 }
 
 // The rows of thread (wx, yl) of brick row (by, bz) for contour_K_body.
-__device__ void contour_K_rows(const MorphArgs& a, int wx, int yl, int by, int bz, u32* anyW) {
+__device__ __forceinline__ void contour_K_rows(const MorphArgs& a, int wx, int yl, int by, int bz, u32* anyW) {
   const int N = a.N, W = a.W;
   const u32* D = a.D;
   const int y = 8 * by + yl;

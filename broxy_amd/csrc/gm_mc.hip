@@ -35,8 +35,9 @@ namespace {
 #include "mc_tables.inc"
 
 struct McConst {
-  u32 ntri[256];  // triangles counted per case (numTriTable semantics)
-  u32 len[256];   // triangles listed per case
+  u32 ntri[256];          // triangles counted per case (numTriTable semantics)
+  u32 len[256];           // triangles listed per case
+  uint8_t use[256][12];   // bit t: counted triangle t of the case uses edge e
 };
 __constant__ McConst g_mc;
 __constant__ unsigned long long g_tri[256];
@@ -46,15 +47,18 @@ constexpr int kMcThreads = 256;
 // Per-workgroup LDS copy of the case tables: the lookups are per lane, so
 // they would otherwise be divergent constant-memory loads.
 struct McTab {
-  u32 ntri[256];
-  u32 len[256];
   unsigned long long tri[256];
+  uint8_t ntri[256];
+  uint8_t len[256];
+  uint8_t use[256][12];
 };
 __device__ __forceinline__ void load_mc_tab(McTab& t) {
   for (int i = threadIdx.x; i < 256; i += blockDim.x) {
-    t.ntri[i] = g_mc.ntri[i];
-    t.len[i] = g_mc.len[i];
+    t.ntri[i] = (uint8_t)g_mc.ntri[i];
+    t.len[i] = (uint8_t)g_mc.len[i];
     t.tri[i] = g_tri[i];
+#pragma unroll
+    for (int e = 0; e < 12; ++e) t.use[i][e] = g_mc.use[i][e];
   }
 }
 constexpr int kBox = 24;  // fine voxels per axis held in LDS (block 16 + halo)
@@ -176,7 +180,7 @@ __device__ u32 block_exscan(u32 v, u32* lds16, u32* total) {
   return off + x - v;
 }
 
-constexpr int kListCap = 4096;  // work-list window (vertices or triangles of a block)
+constexpr int kListCap = 2048;  // work-list window (vertices or triangles of a block)
 
 struct BlockLds {
   McTab tab;
@@ -533,13 +537,8 @@ __device__ void normal_sum(const McTab& tab, const Geo& geo, const Box& bx, int 
     int gv = cube_case(bx, gx, gy, gz);
     if (!mixed(gv)) continue;
     unsigned long long tri = tab.tri[gv];
-    u32 cnt = tab.ntri[gv];
-    // the cube's counted triangles that use edge le, then only those (in order)
-    u32 use = 0u;
-    for (u32 t = 0; t < cnt; ++t) {
-      const u32 w = (u32)(tri >> (12 * t)) & 0xfffu;
-      if ((w & 0xfu) == (u32)le || ((w >> 4) & 0xfu) == (u32)le || (w >> 8) == (u32)le) use |= 1u << t;
-    }
+    // the cube's counted triangles that use edge le, in order
+    u32 use = tab.use[gv][le];
     while (use) {
       const u32 t = (u32)__builtin_ctz(use);
       use &= use - 1u;
@@ -906,6 +905,9 @@ hipError_t upload_tables_once() {
   // The reference's per-case triangle count (tables.h:61-318) lists 4 for
   // case 91, whose triangulation has 5 triangles.
   h.ntri[91] = 4;
+  for (int c = 0; c < 256; ++c)
+    for (u32 t = 0; t < h.ntri[c]; ++t)
+      for (int k = 0; k < 3; ++k) h.use[c][(kTriPacked[c] >> (12 * t + 4 * k)) & 0xf] |= (uint8_t)(1u << t);
   GM_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(g_mc), &h, sizeof(h)));
   GM_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(g_tri), kTriPacked, sizeof(kTriPacked)));
   return hipSuccess;

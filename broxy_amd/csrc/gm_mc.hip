@@ -526,15 +526,17 @@ __device__ __forceinline__ void local_edge_pos(const Geo& g, int fx, int fy, int
   edge_pos_v(g, fx + dx, fy + dy, fz + dz, oe, (u32)(v >> ca) & 1u, (u32)(v >> cb) & 1u, p);
 }
 
+// v0: the case of cube f; pv: the vertex's own position (the same float
+// values local_edge_pos gives for its edge in every sharing cube).
 __device__ void normal_sum(const McTab& tab, const Geo& geo, const Box& bx, int R, int fx, int fy, int fz, int q,
-                           float* n) {
+                           int v0, const float* pv, float* n) {
   n[0] = n[1] = n[2] = 0.f;
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     int gx = fx + g_share[q][s][0], gy = fy + g_share[q][s][1], gz = fz + g_share[q][s][2];
     int le = g_share[q][s][3];
-    if (!cell_in_m_range(gx >> 1, gy >> 1, gz >> 1, R)) continue;
-    int gv = cube_case(bx, gx, gy, gz);
+    if (s > 0 && !cell_in_m_range(gx >> 1, gy >> 1, gz >> 1, R)) continue;
+    int gv = s == 0 ? v0 : cube_case(bx, gx, gy, gz);
     if (!mixed(gv)) continue;
     unsigned long long tri = tab.tri[gv];
     // the cube's counted triangles that use edge le, in order
@@ -545,9 +547,18 @@ __device__ void normal_sum(const McTab& tab, const Geo& geo, const Box& bx, int 
       int e0 = (int)((tri >> (12 * t)) & 0xf), e1 = (int)((tri >> (12 * t + 4)) & 0xf),
           e2 = (int)((tri >> (12 * t + 8)) & 0xf);
       float p0[3], p1[3], p2[3];
-      local_edge_pos(geo, gx, gy, gz, gv, e2, p0);
-      local_edge_pos(geo, gx, gy, gz, gv, e1, p1);
-      local_edge_pos(geo, gx, gy, gz, gv, e0, p2);
+      auto pos = [&](int e, float* o) {
+        if (e == le) {
+          o[0] = pv[0];
+          o[1] = pv[1];
+          o[2] = pv[2];
+        } else {
+          local_edge_pos(geo, gx, gy, gz, gv, e, o);
+        }
+      };
+      pos(e2, p0);
+      pos(e1, p1);
+      pos(e0, p2);
       float a0 = __fsub_rn(p1[0], p0[0]), a1 = __fsub_rn(p1[1], p0[1]), a2 = __fsub_rn(p1[2], p0[2]);
       float b0 = __fsub_rn(p2[0], p0[0]), b1 = __fsub_rn(p2[1], p0[1]), b2 = __fsub_rn(p2[2], p0[2]);
       n[0] = __fadd_rn(n[0], __fsub_rn(__fmul_rn(a1, b2), __fmul_rn(a2, b1)));
@@ -594,7 +605,7 @@ __device__ void mc_verts_block(BlockLds& S, int R, Geo geo, int blk, u32 base, u
       // own edge q runs from corner 0 to corner 1 (x), 3 (y) or 4 (z)
       edge_pos_v(geo, fx, fy, fz, q == 0 ? 0 : (q == 1 ? 3 : 8), (u32)v & 1u,
                  (u32)(v >> (q == 0 ? 1 : (q == 1 ? 3 : 4))) & 1u, p);
-      normal_sum(S.tab, geo, S.box, R, fx, fy, fz, q, nn);
+      normal_sum(S.tab, geo, S.box, R, fx, fy, fz, q, v, p, nn);
       V[3 * (size_t)vid] = p[0];
       V[3 * (size_t)vid + 1] = p[1];
       V[3 * (size_t)vid + 2] = p[2];

@@ -558,12 +558,19 @@ __device__ bool wave_bricks_any(const uint8_t* br, int N, int W, int y0, int y1,
   u32 found = 0u;
   if (ny > 0 && nz > 0 && nw > 0) {
     const int n = ny * nz * nw;
-    // branch-free: every lane loads (clamped index), out-of-range lanes mask
-    for (int i0 = 0; i0 < n; i0 += 64) {
-      const int i = min(i0 + lane_id(), n - 1);
-      const int w = w0 + i % nw, t = i / nw;
-      const int by = by0 + t % ny, bz = bz0 + t / ny;
-      found |= (u32)br[((size_t)bz * NB + by) * W + w] >> bit;
+    // branch-free: every lane loads (clamped index; a repeated brick does not
+    // change the OR), four loads in flight per lane
+    for (int i0 = 0; i0 < n; i0 += 256) {
+      u32 v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = min(i0 + 64 * q + lane_id(), n - 1);
+        const int w = w0 + i % nw, t = i / nw;
+        const int by = by0 + t % ny, bz = bz0 + t / ny;
+        v[q] = br[((size_t)bz * NB + by) * W + w];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) found |= v[q] >> bit;
     }
   }
   return __any((found & 1u) != 0u) != 0;
@@ -576,10 +583,16 @@ __device__ bool wave_bricks_all(const uint8_t* br, int N, int W, int y0, int y1,
   const int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1;
   u32 all = 2u;
   const int n = ny * nz;
-  for (int i0 = 0; i0 < n; i0 += 64) {
-    const int i = min(i0 + lane_id(), n - 1);
-    const int by = by0 + i % ny, bz = bz0 + i / ny;
-    all &= (u32)br[((size_t)bz * NB + by) * W + w];
+  for (int i0 = 0; i0 < n; i0 += 256) {
+    u32 v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = min(i0 + 64 * q + lane_id(), n - 1);
+      const int by = by0 + i % ny, bz = bz0 + i / ny;
+      v[q] = br[((size_t)bz * NB + by) * W + w];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) all &= v[q];
   }
   return __any((all & 2u) == 0u) == 0;
 }
@@ -591,9 +604,15 @@ __device__ bool wave_tile_has_flags(const u16* F, int R, int W, int wx, int y0, 
   u32 acc = 0u;
   if (ny > 0 && nz > 0) {
     const int n = ny * nz;
-    for (int i0 = 0; i0 < n; i0 += 64) {
-      const int i = min(i0 + lane_id(), n - 1);
-      acc |= ld_flag(F, R, W, cy0 + i % ny, cz0 + i / ny, wx);
+    for (int i0 = 0; i0 < n; i0 += 256) {
+      u32 v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = min(i0 + 64 * q + lane_id(), n - 1);
+        v[q] = ld_flag(F, R, W, cy0 + i % ny, cz0 + i / ny, wx);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc |= v[q];
     }
   }
   return __any(acc != 0u) != 0;

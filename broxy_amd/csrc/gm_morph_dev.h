@@ -575,6 +575,47 @@ __device__ bool wave_bricks_any(const uint8_t* br, int N, int W, int y0, int y1,
   }
   return __any((found & 1u) != 0u) != 0;
 }
+// wave_bricks_any that also returns the first and last brick plane (bz)
+// holding a set bit (lo > hi when none does; uniform over the wave).
+__device__ bool wave_bricks_zrange(const uint8_t* br, int N, int W, int y0, int y1, int z0, int z1, int w0, int w1,
+                                   int bit, int& lo, int& hi) {
+  const int NB = N / 8;
+  const int by0 = max(y0, 0) >> 3, by1 = (min(y1, N) - 1) >> 3;
+  const int bz0 = max(z0, 0) >> 3, bz1 = (min(z1, N) - 1) >> 3;
+  w0 = max(w0, 0);
+  w1 = min(w1, W - 1);
+  const int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1, nw = w1 - w0 + 1;
+  int llo = 1 << 20, lhi = -1;
+  if (ny > 0 && nz > 0 && nw > 0) {
+    const int n = ny * nz * nw;
+    for (int i0 = 0; i0 < n; i0 += 256) {
+      u32 v[4];
+      int zb[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = min(i0 + 64 * q + lane_id(), n - 1);
+        const int w = w0 + i % nw, t = i / nw;
+        const int by = by0 + t % ny, bz = bz0 + t / ny;
+        v[q] = br[((size_t)bz * NB + by) * W + w];
+        zb[q] = bz;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if ((v[q] >> bit) & 1u) {
+          llo = min(llo, zb[q]);
+          lhi = max(lhi, zb[q]);
+        }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    llo = min(llo, __shfl_xor(llo, off));
+    lhi = max(lhi, __shfl_xor(lhi, off));
+  }
+  lo = __builtin_amdgcn_readfirstlane(llo);
+  hi = __builtin_amdgcn_readfirstlane(lhi);
+  return hi >= lo;
+}
 // Every brick of the region has the "all" bit.
 __device__ bool wave_bricks_all(const uint8_t* br, int N, int W, int y0, int y1, int z0, int z1, int w) {
   const int NB = N / 8;
@@ -1011,17 +1052,21 @@ __device__ __forceinline__ TileGeo pass_tile(const MorphArgs& a, int t) {
 // per-voxel radii, vals gets the radius values that reach the tile and lane v
 // the brick planes [blo, bhi] of value v.
 template <int PASS>
-__device__ bool select_tile(const MorphArgs& a, int t, const TileGeo& g, u32& vals, int& blo, int& bhi) {
+__device__ bool select_tile(const MorphArgs& a, int t, const TileGeo& g, u32& vals, int& blo, int& bhi, int& slo,
+                            int& shi) {
   const int lane = lane_id();
   const int y1 = g.yfirst + a.ystep * kTileRows;
   bool skip;
   vals = 0u;
   blo = 0;
   bhi = -1;
+  slo = 0;
+  shi = -1;
   if constexpr (PASS == 0) {
     // F = 0 when no occupied voxel is in reach; F unused when every cell is full
-    const bool empty = !wave_bricks_any(a.brickI, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff,
-                                        g.zfirst - a.maxoff, g.zlast + 2 + a.maxoff, g.wx - 1, g.wx + 1, 0);
+    const bool empty = !wave_bricks_zrange(a.brickI, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff,
+                                           g.zfirst - a.maxoff, g.zlast + 2 + a.maxoff, g.wx - 1, g.wx + 1, 0, slo,
+                                           shi);
     const bool full = !empty && wave_bricks_all(a.brickI, a.N, a.W, g.yfirst, y1, g.zfirst, g.zlast + 2, g.wx);
     skip = empty || full;
     if (a.rvar) {
@@ -1033,8 +1078,8 @@ __device__ bool select_tile(const MorphArgs& a, int t, const TileGeo& g, u32& va
       for (int p = 0; p < g.nout(); ++p) fin.skipped(g.zfirst + p * g.zstep);
     }
   } else if constexpr (PASS == 2) {
-    skip = !wave_bricks_any(a.brickK, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff, g.zfirst - a.maxoff,
-                            g.zlast + 1 + a.maxoff, g.wx - 1, g.wx + 1, 0);
+    skip = !wave_bricks_zrange(a.brickK, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff, g.zfirst - a.maxoff,
+                               g.zlast + 1 + a.maxoff, g.wx - 1, g.wx + 1, 0, slo, shi);
     if (skip || a.rvar) {
       EroCoarseFin fin{&a, g.wx, g.yfirst + 2 * lane, 0u};
       for (int p = 0; p < g.nout(); ++p) fin.skipped(g.zfirst + p * g.zstep);
@@ -1045,6 +1090,9 @@ __device__ bool select_tile(const MorphArgs& a, int t, const TileGeo& g, u32& va
   } else {
     skip = !wave_tile_has_flags(PASS == 1 ? a.TG : a.UG, a.R, a.W, g.wx, g.yfirst, g.yfirst + kTileRows, g.zfirst,
                                 g.zlast + 1);
+    if (!skip && !a.rvar)  // the source planes in reach (I, or K)
+      wave_bricks_zrange(PASS == 1 ? a.brickI : a.brickK, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff,
+                         g.zfirst - a.maxoff, g.zlast + 1 + a.maxoff, g.wx - 1, g.wx + 1, 0, slo, shi);
     if (!skip && a.rvar) {
       if constexpr (PASS == 1)
         vals = wave_vset_planes(a.rvset, nullptr, a.N, a.W, g.yfirst, y1, g.zfirst, g.zlast + 1, g.wx, g.wx, blo, bhi);
@@ -1078,7 +1126,8 @@ __device__ void select_tiles(const MorphArgs& a) {
   const int lane = lane_id();
   u32 vals;
   int blo, bhi;  // lane v: brick planes holding value v
-  if (!select_tile<PASS>(a, t, g, vals, blo, bhi)) return;
+  int slo, shi;  // brick planes holding source voxels in reach (uniform radius)
+  if (!select_tile<PASS>(a, t, g, vals, blo, bhi, slo, shi)) return;
   const int x = slab_queue(t / a.W);
   if (a.rvar) {
     // lane v queues the tile for radius value v, with the planes that can
@@ -1095,8 +1144,9 @@ __device__ void select_tiles(const MorphArgs& a) {
       a.queue[a.vqBase + (size_t)q * a.vqCap + k] = entry_pack(t, g.tz0, rlo, rhi);
     }
   } else if (lane == 0) {
+    // the stream reads only the input planes holding source voxels
     const u32 k = atomicAdd(&qc[x], 1u);
-    a.queue[kQueueList + x * a.queueCap + k] = entry_pack(t, g.tz0, -(1 << 20), 1 << 20);
+    a.queue[kQueueList + x * a.queueCap + k] = entry_pack(t, g.tz0, 8 * slo, 8 * shi + 7);
   }
 }
 
@@ -1125,8 +1175,8 @@ __device__ void select_tiles_batched(const MorphArgs* __restrict__ tab, const u3
   const int t = (int)(u - pre[j]);
   const TileGeo g = pass_tile<PASS>(a, t);
   u32 vals;
-  int blo, bhi;
-  if (!select_tile<PASS>(a, t, g, vals, blo, bhi)) return;
+  int blo, bhi, slo, shi;
+  if (!select_tile<PASS>(a, t, g, vals, blo, bhi, slo, shi)) return;
   if (lane_id() == 0) {
     const int q = rad[j] * 8 + slab_queue(t / a.W);
     const u32 k = atomicAdd(&queue[kVarCounters + 256 * qsel + q], 1u);
@@ -1140,7 +1190,9 @@ __device__ void select_tiles_batched(const MorphArgs* __restrict__ tab, const u3
 template <int PASS, int RHO>
 __device__ __forceinline__ void run_pass_tile(const MorphArgs& a, u32 entry, u32* smem, u32* set4) {
   TileGeo g = pass_tile<PASS>(a, entry_tile(entry));
-  entry_range(entry, g, PASS < 2 ? 1 : 0);
+  // per-voxel radii: a dilation entry holds the output planes of its value;
+  // otherwise entries hold the input planes with source voxels
+  entry_range(entry, g, (PASS < 2 && a.rvar) ? 1 : 0);
   Lds L = carve(smem, a, g.nout());
   if constexpr (PASS == 0) {
     DilCoarseFin fin{&a, g.wx, g.yfirst + 2 * lane_id(), cell_range_mask(g.wx, 0, a.dR0), 0};

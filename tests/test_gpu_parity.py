@@ -325,3 +325,35 @@ def test_pyramid(ctx):
     ctx.scale_grid(sc)
     say("pyramid int8 scale")
     assert np.array_equal(ctx.build_pyramid(B.GRID_INPUT), O.pyramid(I, R, scale=sc))
+
+
+def test_empty_grid(ctx):
+    R = 32
+    grid_setup(ctx, R, np.zeros(R ** 3, np.uint8))
+    ctx.scale_uniform(5)
+    say("empty grid")
+    ctx.close_grid(B.SE_SPHERE)
+    assert not ctx.get_cells(B.GRID_CLOSING).any()
+    ctx.extract_mesh()
+    V, N, T = ctx.mesh()
+    assert len(V) == 0 and len(T) == 0
+
+
+def test_largest_grid_closed_surface():
+    """The largest supported grid (base_res 1024, N = 2048): a sphere closes
+    to one closed genus-0 surface (2V - T = 4), every index in range, unit
+    normals; the oracle is too slow at this size, so these invariants stand
+    in for it."""
+    c = B.Context(0)
+    P, T = B.icosphere(5)
+    say("base_res 1024 sphere")
+    c.load(P, T, 1024, 0.3)
+    c.scale_uniform(8)
+    c.close_grid(B.SE_SPHERE)
+    c.extract_mesh()
+    V, N, Tm = c.mesh()
+    c.close()
+    assert len(Tm) > 0 and Tm.max() < len(V)
+    assert 2 * len(V) - len(Tm) == 4
+    n = np.linalg.norm(N[:: max(1, len(N) // 100000)], axis=1)
+    assert np.allclose(n, 1.0, atol=1e-5)

@@ -545,38 +545,10 @@ __device__ __forceinline__ int table_off(const MorphArgs& a, int which, int rho)
 
 // Skip tests of one tile, evaluated by one wave (the result is uniform over
 // the wave).
-// Any brick flag (bit) set in the brick region covering rows [y0,y1),
-// planes [z0,z1), words [w0,w1] (clamped to the grid).
-__device__ bool wave_bricks_any(const uint8_t* br, int N, int W, int y0, int y1, int z0, int z1, int w0, int w1,
-                                int bit) {
-  const int NB = N / 8;
-  const int by0 = max(y0, 0) >> 3, by1 = (min(y1, N) - 1) >> 3;
-  const int bz0 = max(z0, 0) >> 3, bz1 = (min(z1, N) - 1) >> 3;
-  w0 = max(w0, 0);
-  w1 = min(w1, W - 1);
-  const int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1, nw = w1 - w0 + 1;
-  u32 found = 0u;
-  if (ny > 0 && nz > 0 && nw > 0) {
-    const int n = ny * nz * nw;
-    // branch-free: every lane loads (clamped index; a repeated brick does not
-    // change the OR), four loads in flight per lane
-    for (int i0 = 0; i0 < n; i0 += 256) {
-      u32 v[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int i = min(i0 + 64 * q + lane_id(), n - 1);
-        const int w = w0 + i % nw, t = i / nw;
-        const int by = by0 + t % ny, bz = bz0 + t / ny;
-        v[q] = br[((size_t)bz * NB + by) * W + w];
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) found |= v[q] >> bit;
-    }
-  }
-  return __any((found & 1u) != 0u) != 0;
-}
-// wave_bricks_any that also returns the first and last brick plane (bz)
-// holding a set bit (lo > hi when none does; uniform over the wave).
+// Whether a brick flag (bit) is set in the brick region covering rows
+// [y0,y1), planes [z0,z1), words [w0,w1] (clamped to the grid), and the
+// first and last brick plane (bz) holding one (lo > hi when none does;
+// uniform over the wave).
 __device__ bool wave_bricks_zrange(const uint8_t* br, int N, int W, int y0, int y1, int z0, int z1, int w0, int w1,
                                    int bit, int& lo, int& hi) {
   const int NB = N / 8;
@@ -725,36 +697,11 @@ __device__ __forceinline__ u32 radius_eq(const u32* __restrict__ rp, int N, int 
   return m;
 }
 
-// Radius values present in the bricks of rows [y0,y1), planes [z0,z1), words
-// [w0,w1] (clamped to the grid); with gate, only bricks whose gate bit0 is set.
-__device__ u32 wave_bricks_vset(const u32* vset, const uint8_t* gate, int N, int W, int y0, int y1, int z0, int z1,
-                                int w0, int w1) {
-  const int NB = N / 8;
-  const int by0 = max(y0, 0) >> 3, by1 = (min(y1, N) - 1) >> 3;
-  const int bz0 = max(z0, 0) >> 3, bz1 = (min(z1, N) - 1) >> 3;
-  w0 = max(w0, 0);
-  w1 = min(w1, W - 1);
-  const int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1, nw = w1 - w0 + 1;
-  u32 found = 0u;
-  if (ny > 0 && nz > 0 && nw > 0) {
-    const int n = ny * nz * nw;
-    for (int i0 = 0; i0 < n; i0 += 64) {
-      const int i = min(i0 + lane_id(), n - 1);
-      const int w = w0 + i % nw, t = i / nw;
-      const int by = by0 + t % ny, bz = bz0 + t / ny;
-      const size_t b = ((size_t)bz * NB + by) * W + w;
-      const u32 v = vset[b];
-      found |= (gate == nullptr || (gate[b] & 1u)) ? v : 0u;
-    }
-  }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) found |= (u32)__shfl_xor((int)found, off);
-  return found;
-}
 
-// wave_bricks_vset plane by plane: the union of the value sets, and in lane
-// v (< 32) the first and last brick plane (bz) holding value v (lo > hi when
-// none does).
+// Radius values present in the bricks of rows [y0,y1), planes [z0,z1), words
+// [w0,w1] (clamped to the grid; with gate, only bricks whose gate bit0 is
+// set), and in lane v (< 32) the first and last brick plane (bz) holding
+// value v (lo > hi when none does).
 __device__ u32 wave_vset_planes(const u32* vset, const uint8_t* gate, int N, int W, int y0, int y1, int z0, int z1,
                                 int w0, int w1, int& lo, int& hi) {
   const int NB = N / 8;

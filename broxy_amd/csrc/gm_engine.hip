@@ -10,9 +10,14 @@
 // different slots overlaps on the device. Every job runs exactly the kernels
 // of a single-context run, so its outputs are identical to running it alone.
 //
-// Batched use submits all jobs at once; served use submits them as they
-// arrive. Host inputs are copied at submit; meshes stay in the engine until
-// taken.
+// Continuous batching: a slot voxelizes its job on its own stream; a small
+// grid is then handed to the batcher thread, which closes and meshes every
+// context that is ready at that moment in one gm_batch_close_mesh call (one
+// kernel launch per step for all of them), while jobs that arrive meanwhile
+// are loaded by the other slots and go into the next batch. A large grid is
+// closed on its slot's stream, concurrently with the batches. Batched use submits
+// all jobs at once; served use submits them as they arrive. Host inputs are
+// copied at submit; meshes stay in the engine until taken.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -82,10 +87,20 @@ struct Slot {
   std::thread th;
   Pinned<float> inP, outV, outN;
   Pinned<uint32_t> inT, outT;
+  // hand-over to the batcher: the loaded job's structuring element, then the
+  // outcome of its closing
+  int se = 0;
+  bool closed = false;
+  int closeRc = 0;
+  std::string closeErr;
 };
 
 // Device bytes of one context at base resolution R (the buffers alloc_grid,
 // the mesh output and the radius field allocate), plus the mesh input.
+// Jobs up to this base resolution are closed in batches; larger grids keep
+// the GPU busy on their own.
+constexpr int kBatchMaxRes = 192;
+
 uint64_t context_bytes(int R, int L, const gm_job& j) {
   const uint64_t N = 2ull * (uint64_t)R, W = N / 32;
   uint64_t P = 1;
@@ -123,6 +138,16 @@ struct gm_engine {
   int running = 0;
   bool stop = false;
   std::string err;
+  // continuous batching: loaded slots wait in `ready`; the batcher thread
+  // closes and meshes everything that is ready in one gm_batch_close_mesh
+  // call (one launch per step for all of them) while the slots load the
+  // next jobs
+  gm_batch* batch = nullptr;
+  std::thread batcher;
+  std::mutex bm;
+  std::condition_variable cvReady, cvClosed;
+  std::vector<Slot*> ready;
+  bool bstop = false;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
   double now() const {
@@ -170,8 +195,26 @@ struct gm_engine {
     } else if (!step(gm_scale_uniform(c, d.radius))) {
       return;
     }
-    if (!step(gm_close(c, d.se))) return;
-    if (!step(gm_extract_mesh(c))) return;
+    if (d.base_res <= kBatchMaxRes) {
+      // small grid: closed with whatever else is ready
+      std::unique_lock<std::mutex> lk(bm);
+      s.se = d.se;
+      s.closed = false;
+      ready.push_back(&s);
+      cvReady.notify_one();
+      cvClosed.wait(lk, [&] { return s.closed; });
+    } else {
+      // large grid: enough work of its own, closed on the slot's stream
+      // concurrently with the batches
+      s.closeRc = gm_close(c, d.se);
+      if (s.closeRc == GM_OK) s.closeRc = gm_extract_mesh(c);
+      if (s.closeRc != GM_OK) s.closeErr = gm_last_error(c);
+    }
+    if (s.closeRc != GM_OK) {
+      j->status = s.closeRc;
+      j->err = s.closeErr;
+      return;
+    }
     if (!step(gm_mesh_counts(c, &j->nv, &j->nt))) return;
     const size_t nv3 = 3 * (size_t)j->nv, nt3 = 3 * (size_t)j->nt;
     if (!s.outV.reserve(nv3) || !s.outN.reserve(nv3) || !s.outT.reserve(nt3)) return oom();
@@ -179,6 +222,38 @@ struct gm_engine {
     j->V.assign(s.outV.p, s.outV.p + nv3);
     j->Nrm.assign(s.outN.p, s.outN.p + nv3);
     j->Tri.assign(s.outT.p, s.outT.p + nt3);
+  }
+
+  void batch_loop() {
+    for (;;) {
+      std::vector<Slot*> take;
+      {
+        std::unique_lock<std::mutex> lk(bm);
+        cvReady.wait(lk, [&] { return bstop || !ready.empty(); });
+        if (ready.empty()) return;  // stopping
+        take.swap(ready);
+      }
+      // one call per structuring element present
+      for (int se : {GM_SE_SPHERE, GM_SE_CUBE}) {
+        std::vector<gm_ctx*> ctxs;
+        std::vector<Slot*> group;
+        for (Slot* s : take)
+          if (s->se == se) {
+            ctxs.push_back(s->ctx);
+            group.push_back(s);
+          }
+        if (group.empty()) continue;
+        const int rc = gm_batch_close_mesh(batch, ctxs.data(), (int)ctxs.size(), se);
+        const std::string msg = rc == GM_OK ? std::string() : std::string(gm_batch_last_error(batch));
+        std::lock_guard<std::mutex> lk(bm);
+        for (Slot* s : group) {
+          s->closeRc = rc;
+          s->closeErr = msg;
+          s->closed = true;
+        }
+      }
+      cvClosed.notify_all();
+    }
   }
 
   void worker(int idx) {
@@ -227,6 +302,13 @@ int gm_engine_create(int device, int slots, uint64_t hbm_budget_bytes, gm_engine
       return GM_ERR_HIP;
     }
   }
+  if (gm_batch_create(device, &e->batch) != GM_OK) {
+    for (auto& t : e->slots)
+      if (t.ctx) gm_destroy(t.ctx);
+    delete e;
+    return GM_ERR_HIP;
+  }
+  e->batcher = std::thread([e] { e->batch_loop(); });
   for (int i = 0; i < slots; ++i) e->slots[i].th = std::thread([e, i] { e->worker(i); });
   *out = e;
   return GM_OK;
@@ -243,6 +325,14 @@ int gm_engine_destroy(gm_engine* e) {
   e->cvWork.notify_all();
   for (auto& s : e->slots)
     if (s.th.joinable()) s.th.join();
+  // the workers are gone (the batcher served their last jobs): stop it
+  {
+    std::lock_guard<std::mutex> lk(e->bm);
+    e->bstop = true;
+  }
+  e->cvReady.notify_all();
+  if (e->batcher.joinable()) e->batcher.join();
+  if (e->batch) gm_batch_destroy(e->batch);
   for (auto& s : e->slots) {
     if (s.ctx) gm_destroy(s.ctx);
     s.inP.release();

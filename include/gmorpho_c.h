@@ -153,11 +153,14 @@ int gm_batch_close_mesh(gm_batch* b, gm_ctx* const* ctxs, int n, int se);
  * at a time from its viewer (GMorpho::Update, GMorpho.cu:7775); it is for
  * callers with many jobs: batches of uneven grids and radii, and a serving
  * loop. A pool of `slots` contexts (one HIP stream and one host thread each)
- * runs jobs concurrently on one device. A job is admitted when a slot is free
- * and the device memory the slot would then hold fits hbm_budget_bytes;
- * smaller jobs may pass one that does not fit yet. Slots keep their largest
- * buffers. A job runs the same kernels as a single-context run, so its mesh
- * is identical to running it alone. Inputs are copied at submit. */
+ * loads (voxelizes) jobs concurrently on one device; the closings and meshes
+ * of the loaded small grids (base_res <= 192) run together through
+ * gm_batch_close_mesh (continuous batching: jobs loaded while a batch runs
+ * join the next one), larger grids on their own slot's stream.
+ * A job is admitted when a slot is free and the device memory the slot would
+ * then hold fits hbm_budget_bytes; smaller jobs may pass one that does not
+ * fit yet. Slots keep their largest buffers. Every job's mesh is identical to
+ * running it alone. Inputs are copied at submit. */
 typedef struct gm_engine gm_engine;
 typedef struct {
   const float* P;          /* float[3*num_of_vertices] */

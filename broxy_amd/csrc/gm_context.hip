@@ -101,7 +101,7 @@ void free_grid(gm_ctx* c) {
   dfree(c->I); dfree(c->D); dfree(c->K); dfree(c->C);
   dfree(c->DF); dfree(c->TG); dfree(c->CL); dfree(c->UG); dfree(c->brickI); dfree(c->brickK); dfree(c->queue);
   dfree(c->stage); dfree(c->mark); dfree(c->shell);
-  dfree(c->mc.blockV); dfree(c->mc.blockT); dfree(c->mc.cellV); dfree(c->mc.cellT);
+  dfree(c->mc.blockV); dfree(c->mc.blockT); dfree(c->mc.cellV); dfree(c->mc.cellCase); dfree(c->mc.posOf); dfree(c->mc.order);
   dfree(c->mc.V); dfree(c->mc.Nrm); dfree(c->mc.T); dfree(c->mc.late); dfree(c->mc.totals); dfree(c->mc.scanPart);
   dfree(c->mc.brick); dfree(c->mc.list);
   dfree(c->dPyr);
@@ -204,6 +204,9 @@ int alloc_grid(gm_ctx* c, int R) {
   HIPCALL(c, hipMalloc(&c->mc.blockV, (size_t)c->mc.nblocks * 4));
   HIPCALL(c, hipMalloc(&c->mc.blockT, (size_t)c->mc.nblocks * 4));
   HIPCALL(c, hipMalloc(&c->mc.cellV, (size_t)R * R * R * 4));
+  HIPCALL(c, hipMalloc(&c->mc.cellCase, (size_t)c->mc.nblocks * 512 * 8));
+  HIPCALL(c, hipMalloc(&c->mc.posOf, (size_t)c->mc.nblocks * 4));
+  HIPCALL(c, hipMalloc(&c->mc.order, ((size_t)4 * c->mc.nblocks + 4) * 4));
   HIPCALL(c, hipMalloc(&c->mc.totals, 16));
   HIPCALL(c, hipMalloc(&c->mc.scanPart, mc_scan_parts(c->mc.nblocks) * 8));
   HIPCALL(c, hipMalloc(&c->mc.brick, brickBytes));

@@ -110,7 +110,8 @@ uint64_t context_bytes(int R, int L, const gm_job& j) {
   b += 4 * (uint64_t)R * R * W * 2;               // DF, TG, CL, UG
   b += 2 * (N / 8) * (N / 8) * W;                 // brick summaries
   b += (uint64_t)R * R * R;                       // cell-byte staging
-  b += 2 * (P / 8) * (P / 8) * (P / 8) * 4;       // MC block counts
+  b += 7 * (P / 8) * (P / 8) * (P / 8) * 4;       // MC block counts, list positions, emit order
+  b += (P / 8) * (P / 8) * (P / 8) * 512 * 8;     // MC cube cases of listed blocks
   b += (uint64_t)R * R * R * 4;                   // MC cell bases
   b += 4 * N * N * 24 + 8 * N * N * 12;           // mesh output (initial capacity)
   b += gm::tile_queue_words(R) * 4;               // active-tile queues

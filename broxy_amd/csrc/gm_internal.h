@@ -236,7 +236,9 @@ struct MCBuffers {
   u32* blockV;   // per 8^3 Morton block: vertex count, then exclusive offset
   u32* blockT;
   u32* cellV;    // dense R^3: block-local vertex base of cells with vertices
-  u32* cellT;
+  unsigned long long* cellCase;  // per listed block (list order) and cell: its 8 cube cases
+  u32* posOf;    // per block: its index in list, ~0u when not listed
+  u32* order;    // emit order of the listed blocks: 4 class counts, then 4 x nblocks list positions
   float* V;
   float* Nrm;
   u32* T;

@@ -148,6 +148,47 @@ __device__ __forceinline__ bool cell_uniform(const Box& b, int cx, int cy, int c
   return a == 7u || o == 0u;
 }
 
+// swap of the two low bits of p (p < 4)
+__device__ __forceinline__ u32 swap2(u32 p) { return ((p * 5u) >> 1) & 3u; }
+
+// The 8 cube cases of cell (cx,cy,cz) as the bytes of one word (byte j =
+// cube (j&1, j>>1&1, j>>2) of the cell, cube_case corner order), 0 when the
+// cell's 3x3x3 fine voxels are uniform: the 9 rows of 3 bits are read once
+// and each case takes two bits from four of them.
+__device__ __forceinline__ unsigned long long cell_cases_box(const Box& b, int cx, int cy, int cz) {
+  u32 r[3][3];
+  u32 a = 7u, o = 0u;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      r[j][k] = b.at3(2 * cx, 2 * cy + j, 2 * cz + k);
+      a &= r[j][k];
+      o |= r[j][k];
+    }
+  if (a == 7u || o == 0u) return 0ull;
+  unsigned long long w = 0ull;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int x = j & 1, y = (j >> 1) & 1, z = j >> 2;
+    const u32 c = ((r[y][z] >> x) & 3u) | (swap2((r[y + 1][z] >> x) & 3u) << 2) | (((r[y][z + 1] >> x) & 3u) << 4) |
+                  (swap2((r[y + 1][z + 1] >> x) & 3u) << 6);
+    w |= (unsigned long long)c << (8 * j);
+  }
+  return w;
+}
+
+// Local Morton index l in [0,512) <-> local cell coordinates in [0,8)^3.
+__device__ __forceinline__ void local_cell(int l, int& lx, int& ly, int& lz) {
+  lx = (l & 1) | ((l >> 2) & 2) | ((l >> 4) & 4);
+  ly = ((l >> 1) & 1) | ((l >> 3) & 2) | ((l >> 5) & 4);
+  lz = ((l >> 2) & 1) | ((l >> 4) & 2) | ((l >> 6) & 4);
+}
+__device__ __forceinline__ int spread3(int x) { return (x & 1) | ((x & 2) << 2) | ((x & 4) << 4); }
+__device__ __forceinline__ int local_morton(int lx, int ly, int lz) {
+  return spread3(lx) | (spread3(ly) << 1) | (spread3(lz) << 2);
+}
+
 // Block index -> cell coordinates of local Morton index l.
 __device__ __forceinline__ void block_cell(int blk, int l, int& cx, int& cy, int& cz) {
   u32 code = ((u32)blk << 9) | (u32)l;
@@ -181,46 +222,110 @@ __device__ u32 block_exscan(u32 v, u32* lds16, u32* total) {
 }
 
 constexpr int kListCap = 2048;  // work-list window (vertices or triangles of a block)
+constexpr int kFaceCap = 1024;  // face normals kept in LDS (blocks with at most this many triangles)
+
+constexpr int kNbCells = 9 * 9 * 9 - 8 * 8 * 8;
+static_assert(kNbCells <= kMcThreads, "one thread per cell just above a block");
+
+// Index of a cell just above a block from its local coordinates in [0,8]^3
+// with at least one equal to 8: the x = 8 face (81), then y = 8 with x < 8
+// (72), then z = 8 with x, y < 8 (64).
+__device__ __forceinline__ int nb_index(int lx, int ly, int lz) {
+  return lx == 8 ? ly + 9 * lz : (ly == 8 ? 81 + lx + 8 * lz : 153 + lx + 8 * ly);
+}
+__device__ __forceinline__ void nb_cell(int k, int& lx, int& ly, int& lz) {
+  if (k < 81) {
+    lx = 8;
+    ly = k % 9;
+    lz = k / 9;
+  } else if (k < 153) {
+    lx = (k - 81) & 7;
+    ly = 8;
+    lz = (k - 81) >> 3;
+  } else {
+    lx = (k - 153) & 7;
+    ly = (k - 153) >> 3;
+    lz = 8;
+  }
+}
+
+// Vertices owned by the cubes before cube j of a cell whose 8 cube cases are
+// the bytes of w (own_verts per byte, summed over bytes < j).
+__device__ __forceinline__ u32 own_prefix8(unsigned long long w, int j) {
+  constexpr unsigned long long m1 = 0x0101010101010101ull;
+  const unsigned long long c0 = w & m1;
+  const unsigned long long x = (c0 ^ ((w >> 1) & m1)) + (c0 ^ ((w >> 3) & m1)) + (c0 ^ ((w >> 4) & m1));
+  const unsigned long long keep = j ? (~0ull >> (64 - 8 * j)) : 0ull;
+  return (u32)(((x & keep) * m1) >> 56);
+}
 
 struct BlockLds {
   McTab tab;
-  Box box;
-  uint8_t cases[4096];  // [local cell * 8 + cube], 0 for cells outside M range
-  uint8_t vpre[4096];   // vertices owned by the cell's earlier cubes
-  uint8_t tpre[4096];   // counted triangles of the cell's earlier cubes
+  alignas(8) uint8_t cases[4096];  // [local cell * 8 + cube], 0 for cells outside M range
+  alignas(8) uint8_t tpre[4096];   // counted triangles of the cell's earlier cubes
+  // the 217 cells just above the block (one local coordinate 8, index
+  // nb_index) that own vertices of the block's triangles: their 8 cube cases
+  // and global vertex base; and the cells just below it (local coordinate
+  // -1, index nb_index of 7 - l) whose cubes share vertices of the block.
+  // Cases are 0 for cells outside [1,R-2]^3.
+  unsigned long long nbAbove[kNbCells];
+  unsigned long long nbBelow[kNbCells];
+  u32 nbBase[kNbCells];
   u32 vOff[512];
   u32 tOff[512];
   u32 red[16];
   uint16_t list[kListCap];  // cube index << 3 | item, by local output index
+  float fn[kFaceCap][3];     // face normal of each local triangle (when it fits)
 };
 
-// Cases of the cubes of the block's cells (uniform cells and cells outside
-// [1,R-2]^3 get 0) and per-cell vertex / triangle counts into vOff/tOff.
-__device__ void stage_cases(BlockLds& S, int blk, int R) {
-  for (int l = threadIdx.x; l < 512; l += blockDim.x) {
-    int cx, cy, cz;
-    block_cell(blk, l, cx, cy, cz);
-    u32 nv = 0, nt = 0;
-    if (cell_in_m_range(cx, cy, cz, R) && !cell_uniform(S.box, cx, cy, cz)) {
+// Cube cases of a listed neighbour cell (0 outside [1,R-2]^3 or in a block
+// without mixed cubes); *blkOut = its block.
+__device__ __forceinline__ unsigned long long listed_cell_cases(const unsigned long long* __restrict__ cellCase,
+                                                                const u32* __restrict__ posOf, int R, int cx, int cy,
+                                                                int cz, u32* code) {
+  if (!cell_in_m_range(cx, cy, cz, R)) return 0ull;
+  *code = part1by2((u32)cx) | (part1by2((u32)cy) << 1) | (part1by2((u32)cz) << 2);
+  const u32 pos = posOf[*code >> 9];
+  return pos == ~0u ? 0ull : cellCase[(size_t)pos * 512 + (*code & 511u)];
+}
+
+// Stages listed block `it` (origin cell c0): the cube cases of its 512 cells
+// (written by the count kernel), per-cell vertex / triangle counts into
+// vOff/tOff with the in-cell triangle prefix, and the cells just above and
+// below it.
+__device__ void stage_block(BlockLds& S, const unsigned long long* __restrict__ cellCase,
+                            const u32* __restrict__ posOf, const u32* __restrict__ blockV,
+                            const u32* __restrict__ cellL, int R, u32 it, int c0x, int c0y, int c0z) {
+  const int t = (int)threadIdx.x;
+  unsigned long long above = 0ull, below = 0ull;
+  u32 nbB = 0u;
+  if (t < kNbCells) {
+    int lx, ly, lz;
+    nb_cell(t, lx, ly, lz);
+    u32 code = 0u;
+    above = listed_cell_cases(cellCase, posOf, R, c0x + lx, c0y + ly, c0z + lz, &code);
+    if (above) nbB = blockV[code >> 9] + cellL[(size_t)(c0x + lx) + (size_t)R * ((size_t)(c0y + ly) + (size_t)R * (c0z + lz))];
+    below = listed_cell_cases(cellCase, posOf, R, c0x + 7 - lx, c0y + 7 - ly, c0z + 7 - lz, &code);
+  }
+  const unsigned long long* own = cellCase + (size_t)it * 512;
+  for (int l = t; l < 512; l += blockDim.x) {
+    const unsigned long long w = own[l];
+    u32 nt = 0u;
+    unsigned long long tp = 0ull;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        int v = cube_case(S.box, 2 * cx + (j & 1), 2 * cy + ((j >> 1) & 1), 2 * cz + ((j >> 2) & 1));
-        S.cases[8 * l + j] = (uint8_t)v;
-        S.vpre[8 * l + j] = (uint8_t)nv;
-        S.tpre[8 * l + j] = (uint8_t)nt;
-        nv += own_verts(v);
-        nt += S.tab.ntri[v];
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        S.cases[8 * l + j] = 0;
-        S.vpre[8 * l + j] = 0;
-        S.tpre[8 * l + j] = 0;
-      }
+    for (int j = 0; j < 8; ++j) {
+      tp |= (unsigned long long)nt << (8 * j);
+      nt += S.tab.ntri[(w >> (8 * j)) & 0xffu];
     }
-    S.vOff[l] = nv;
+    *reinterpret_cast<unsigned long long*>(&S.cases[8 * l]) = w;
+    *reinterpret_cast<unsigned long long*>(&S.tpre[8 * l]) = tp;
+    S.vOff[l] = own_prefix8(w, 8);
     S.tOff[l] = nt;
+  }
+  if (t < kNbCells) {
+    S.nbAbove[t] = above;
+    S.nbBelow[t] = below;
+    S.nbBase[t] = nbB;
   }
   __syncthreads();
 }
@@ -245,12 +350,28 @@ __device__ __forceinline__ bool block_outside(int blk, int R, int& c0x, int& c0y
   return c0x >= R || c0y >= R || c0z >= R;
 }
 
+// Emit order: the blocks with output in kOrderClasses classes by work
+// (triangles + 2 x vertices), heaviest class first, so that the longest
+// blocks start early. order[0..kOrderClasses) count the members of each
+// class; class c lists its members' list positions at
+// order[kOrderClasses + c * nblocks ...]. Built by the scan's down-sweep.
+constexpr int kOrderClasses = 4;
+__device__ __forceinline__ int order_class(u32 work) {
+  return work >= 2048u ? 0 : (work >= 1024u ? 1 : (work >= 512u ? 2 : 3));
+}
+// List position of the i-th block in emit order.
+__device__ __forceinline__ u32 order_at(const u32* __restrict__ order, int nblocks, u32 i) {
+  u32 c = 0;
+  while (c + 1 < (u32)kOrderClasses && i >= order[c]) i -= order[c++];
+  return order[kOrderClasses + (size_t)c * nblocks + i];
+}
+
 // Morton blocks that can hold a mixed cube: the bricks (1 word x 8 rows x 8
 // planes of C) around the block's 17^3 fine voxels are not all empty and not
 // all full. Active blocks are appended to list (list[0] = count, then block
 // ids, unordered); the others get zero counts. One thread per block.
 __device__ __forceinline__ void mc_blocks_body(const uint8_t* __restrict__ brick, int R, int nblocks, u32* blockV,
-                                               u32* blockT, u32* list, int blk) {
+                                               u32* blockT, u32* list, u32* posOf, int blk) {
   bool act = false;
   if (blk < nblocks) {
     int c0x, c0y, c0z;
@@ -273,6 +394,7 @@ __device__ __forceinline__ void mc_blocks_body(const uint8_t* __restrict__ brick
     if (!act) {
       blockV[blk] = 0u;
       blockT[blk] = 0u;
+      posOf[blk] = ~0u;
     }
   }
   const unsigned long long m = __ballot(act);
@@ -282,18 +404,26 @@ __device__ __forceinline__ void mc_blocks_body(const uint8_t* __restrict__ brick
     u32 base = 0u;
     if (lane == leader) base = atomicAdd(&list[0], (u32)__popcll(m));
     base = __shfl(base, leader, 64);
-    if (act) list[1 + base + (u32)__popcll(m & ((1ull << lane) - 1ull))] = (u32)blk;
+    if (act) {
+      const u32 pos = base + (u32)__popcll(m & ((1ull << lane) - 1ull));
+      list[1 + pos] = (u32)blk;
+      posOf[blk] = pos;
+    }
   }
 }
 __global__ void __launch_bounds__(256) k_mc_blocks(const uint8_t* __restrict__ brick, int R, int nblocks,
-                                                   u32* blockV, u32* blockT, u32* list) {
-  mc_blocks_body(brick, R, nblocks, blockV, blockT, list, blockIdx.x * blockDim.x + threadIdx.x);
+                                                   u32* blockV, u32* blockT, u32* list, u32* posOf, u32* order) {
+  // the emit-order class counts start from zero in the scan that follows
+  if (blockIdx.x == 0 && threadIdx.x < kOrderClasses) order[threadIdx.x] = 0u;
+  mc_blocks_body(brick, R, nblocks, blockV, blockT, list, posOf, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// Vertex and triangle counts of one listed block, the block-local vertex base
-// of each of its cells with vertices.
-__device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R, int blk, u32* blockV, u32* blockT,
-                                               u32* cellL, Box& bx, u32* red, const McTab& tab) {
+// Vertex and triangle counts of listed block `it`, the block-local vertex
+// base of each of its cells with vertices, and the cube cases of its cells
+// (for the emit kernel).
+__device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R, u32 it, int blk, u32* blockV,
+                                               u32* blockT, u32* cellL, unsigned long long* cellCase, Box& bx,
+                                               u32* red, const McTab& tab) {
   int c0x, c0y, c0z;
   block_outside(blk, R, c0x, c0y, c0z);
   // cells of the block need fine voxels [16b, 16b+17) per axis
@@ -309,18 +439,22 @@ __device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R,
   const bool anyZero = __syncthreads_or(a != 0x1ffffu) != 0;
   const bool nonuni = anyOne && anyZero;
   u32 nvh[2] = {0u, 0u}, nt = 0;
+  unsigned long long w[2] = {0ull, 0ull};
   if (nonuni) {
     for (int h = 0; h < 2; ++h) {
       int cx, cy, cz;
       block_cell(blk, 2 * threadIdx.x + h, cx, cy, cz);
-      if (!cell_in_m_range(cx, cy, cz, R) || cell_uniform(bx, cx, cy, cz)) continue;
+      if (!cell_in_m_range(cx, cy, cz, R)) continue;
+      w[h] = cell_cases_box(bx, cx, cy, cz);
+      nvh[h] = own_prefix8(w[h], 8);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        int v = cube_case(bx, 2 * cx + (j & 1), 2 * cy + ((j >> 1) & 1), 2 * cz + ((j >> 2) & 1));
-        nvh[h] += own_verts(v);
-        nt += tab.ntri[v];
-      }
+      for (int j = 0; j < 8; ++j) nt += tab.ntri[(w[h] >> (8 * j)) & 0xffu];
     }
+  }
+  {
+    unsigned long long* dst = cellCase + (size_t)it * 512 + 2 * threadIdx.x;
+    dst[0] = w[0];
+    dst[1] = w[1];
   }
   u32 tv, tt;
   const u32 off = block_exscan(nvh[0] + nvh[1], red, &tv);
@@ -343,16 +477,18 @@ __device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R,
 // list (each ends when the list does).
 __global__ void __launch_bounds__(kMcThreads) k_mc_count(const u32* __restrict__ C, int R,
                                                           const u32* __restrict__ list, u32* blockV,
-                                                          u32* blockT, u32* cellL, u32* late) {
+                                                          u32* blockT, u32* cellL, unsigned long long* cellCase,
+                                                          u32* late) {
   __shared__ Box bx;
   __shared__ u32 red[16];
   __shared__ McTab tab;
   // the late-triangle count starts from zero in the emit kernel that follows
   if (blockIdx.x == 0 && threadIdx.x == 0) late[0] = 0u;
-  load_mc_tab(tab);
   const u32 n = list[0];
+  if (blockIdx.x >= n) return;
+  load_mc_tab(tab);
   for (u32 it = blockIdx.x; it < n; it += gridDim.x) {
-    mc_count_block(C, R, (int)list[1 + it], blockV, blockT, cellL, bx, red, tab);
+    mc_count_block(C, R, it, (int)list[1 + it], blockV, blockT, cellL, cellCase, bx, red, tab);
     __syncthreads();  // the box is reloaded by the next iteration
   }
 }
@@ -414,8 +550,10 @@ __global__ void __launch_bounds__(256) k_scan_top(u32* part, int nparts, u32* to
   scan_top_body(part, nparts, totals, capV, capT);
 }
 
-__device__ __forceinline__ void scan_down_body(u32* a, u32* b, int n, const u32* __restrict__ part, int chunk) {
+__device__ __forceinline__ void scan_down_body(u32* a, u32* b, int n, const u32* __restrict__ part, int chunk,
+                                               const u32* __restrict__ posOf = nullptr, u32* order = nullptr) {
   __shared__ u32 red[16];
+  __shared__ u32 cls[2 * kOrderClasses];
   int base = chunk * kChunk + 4 * threadIdx.x;
   u32 va[4], vb[4];
   u32 sa = 0, sb = 0;
@@ -426,6 +564,7 @@ __device__ __forceinline__ void scan_down_body(u32* a, u32* b, int n, const u32*
     sa += va[k];
     sb += vb[k];
   }
+  if (order && threadIdx.x < kOrderClasses) cls[threadIdx.x] = 0u;
   u32 ta, tb;
   u32 oa = block_exscan(sa, red, &ta) + part[2 * chunk];
   u32 ob = block_exscan(sb, red, &tb) + part[2 * chunk + 1];
@@ -438,9 +577,31 @@ __device__ __forceinline__ void scan_down_body(u32* a, u32* b, int n, const u32*
     oa += va[k];
     ob += vb[k];
   }
+  if (order) {
+    // the chunk's blocks with output join their emit-order class: slots
+    // within the chunk from LDS counters, one global reservation per class
+    u32 slot[4];
+    for (int k = 0; k < 4; ++k) {
+      const u32 work = vb[k] + 2u * va[k];
+      slot[k] = work ? atomicAdd(&cls[order_class(work)], 1u) : 0u;
+    }
+    __syncthreads();
+    if (threadIdx.x < kOrderClasses) {
+      const u32 c = cls[threadIdx.x];
+      cls[kOrderClasses + threadIdx.x] = c ? atomicAdd(&order[threadIdx.x], c) : 0u;
+    }
+    __syncthreads();
+    for (int k = 0; k < 4; ++k) {
+      const u32 work = vb[k] + 2u * va[k];
+      if (!work) continue;
+      const int c = order_class(work);
+      order[kOrderClasses + (size_t)c * n + cls[kOrderClasses + c] + slot[k]] = posOf[base + k];
+    }
+  }
 }
-__global__ void __launch_bounds__(256) k_scan_down(u32* a, u32* b, int n, const u32* __restrict__ part) {
-  scan_down_body(a, b, n, part, blockIdx.x);
+__global__ void __launch_bounds__(256) k_scan_down(u32* a, u32* b, int n, const u32* __restrict__ part,
+                                                   const u32* __restrict__ posOf, u32* order) {
+  scan_down_body(a, b, n, part, blockIdx.x, posOf, order);
 }
 
 // Position of the vertex on owned edge e (0: x, 3: y, 8: z) of cube f:
@@ -456,18 +617,6 @@ __device__ __forceinline__ float lerp_edge(float p1, float p2, float v1, float v
 struct Geo {
   float bx, by, bz, h;
 };
-__device__ __forceinline__ void edge_pos(const Geo& g, const Box& b, int fx, int fy, int fz, int e, float* o) {
-  float px = fmaf((float)fx, g.h, g.bx);
-  float py = fmaf((float)fy, g.h, g.by);
-  float pz = fmaf((float)fz, g.h, g.bz);
-  float v0 = (float)b.at(fx, fy, fz);
-  o[0] = px;
-  o[1] = py;
-  o[2] = pz;
-  if (e == 0) o[0] = lerp_edge(px, __fadd_rn(px, g.h), v0, (float)b.at(fx + 1, fy, fz));
-  else if (e == 3) o[1] = lerp_edge(__fadd_rn(py, g.h), py, (float)b.at(fx, fy + 1, fz), v0);
-  else o[2] = lerp_edge(pz, __fadd_rn(pz, g.h), v0, (float)b.at(fx, fy, fz + 1));
-}
 
 // Owner of local edge e of a cube: offset (dx,dy,dz) of the owning cube and
 // which of its own edges (0, 3, 8) it is, packed 5 bits per edge:
@@ -526,21 +675,57 @@ __device__ __forceinline__ void local_edge_pos(const Geo& g, int fx, int fy, int
   edge_pos_v(g, fx + dx, fy + dy, fz + dz, oe, (u32)(v >> ca) & 1u, (u32)(v >> cb) & 1u, p);
 }
 
+// Face normal of a triangle from its three corner positions (the reference's
+// cross product, GMorpho.cu:6498-6524).
+__device__ __forceinline__ void face_cross(const float* p0, const float* p1, const float* p2, float* f) {
+  float a0 = __fsub_rn(p1[0], p0[0]), a1 = __fsub_rn(p1[1], p0[1]), a2 = __fsub_rn(p1[2], p0[2]);
+  float b0 = __fsub_rn(p2[0], p0[0]), b1 = __fsub_rn(p2[1], p0[1]), b2 = __fsub_rn(p2[2], p0[2]);
+  f[0] = __fsub_rn(__fmul_rn(a1, b2), __fmul_rn(a2, b1));
+  f[1] = __fsub_rn(__fmul_rn(a2, b0), __fmul_rn(a0, b2));
+  f[2] = __fsub_rn(__fmul_rn(a0, b1), __fmul_rn(a1, b0));
+}
+
+__device__ __forceinline__ u32 prefix_ntri(const BlockLds& S, int l, int j) { return S.tpre[8 * l + j]; }
+
 // v0: the case of cube f; pv: the vertex's own position (the same float
-// values local_edge_pos gives for its edge in every sharing cube).
-__device__ void normal_sum(const McTab& tab, const Geo& geo, const Box& bx, int R, int fx, int fy, int fz, int q,
-                           int v0, const float* pv, float* n) {
+// values local_edge_pos gives for its edge in every sharing cube). With
+// useFn, the face normals of the block's own cubes come from S.fn (written
+// by the triangle phase from the same positions); cubes of other blocks are
+// computed here. The sum runs in the same order either way.
+__device__ void normal_sum(const BlockLds& S, const Geo& geo, int R, int c0x, int c0y, int c0z, bool useFn, int fx,
+                           int fy, int fz, int q, int v0, const float* pv, float* n) {
   n[0] = n[1] = n[2] = 0.f;
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     int gx = fx + g_share[q][s][0], gy = fy + g_share[q][s][1], gz = fz + g_share[q][s][2];
     int le = g_share[q][s][3];
-    if (s > 0 && !cell_in_m_range(gx >> 1, gy >> 1, gz >> 1, R)) continue;
-    int gv = s == 0 ? v0 : cube_case(bx, gx, gy, gz);
+    const int cx = gx >> 1, cy = gy >> 1, cz = gz >> 1;
+    if (s > 0 && !cell_in_m_range(cx, cy, cz, R)) continue;
+    const int lx = cx - c0x, ly = cy - c0y, lz = cz - c0z;
+    const int j = (gx & 1) | ((gy & 1) << 1) | ((gz & 1) << 2);
+    const bool inBlock = (unsigned)lx < 8u && (unsigned)ly < 8u && (unsigned)lz < 8u;
+    const int l = inBlock ? local_morton(lx, ly, lz) : 0;
+    // a cube of the block, or of a cell just below it
+    const int gv = s == 0 ? v0
+                          : (inBlock ? (int)S.cases[8 * l + j]
+                                     : (int)((S.nbBelow[nb_index(7 - lx, 7 - ly, 7 - lz)] >> (8 * j)) & 0xffu));
     if (!mixed(gv)) continue;
-    unsigned long long tri = tab.tri[gv];
+    if (useFn && inBlock) {
+      const u32 lt = S.tOff[l] + prefix_ntri(S, l, j);
+      u32 use = S.tab.use[gv][le];
+      while (use) {
+        const u32 t = (u32)__builtin_ctz(use);
+        use &= use - 1u;
+        const float* f = S.fn[lt + t];
+        n[0] = __fadd_rn(n[0], f[0]);
+        n[1] = __fadd_rn(n[1], f[1]);
+        n[2] = __fadd_rn(n[2], f[2]);
+      }
+      continue;
+    }
+    unsigned long long tri = S.tab.tri[gv];
     // the cube's counted triangles that use edge le, in order
-    u32 use = tab.use[gv][le];
+    u32 use = S.tab.use[gv][le];
     while (use) {
       const u32 t = (u32)__builtin_ctz(use);
       use &= use - 1u;
@@ -559,35 +744,44 @@ __device__ void normal_sum(const McTab& tab, const Geo& geo, const Box& bx, int 
       pos(e2, p0);
       pos(e1, p1);
       pos(e0, p2);
-      float a0 = __fsub_rn(p1[0], p0[0]), a1 = __fsub_rn(p1[1], p0[1]), a2 = __fsub_rn(p1[2], p0[2]);
-      float b0 = __fsub_rn(p2[0], p0[0]), b1 = __fsub_rn(p2[1], p0[1]), b2 = __fsub_rn(p2[2], p0[2]);
-      n[0] = __fadd_rn(n[0], __fsub_rn(__fmul_rn(a1, b2), __fmul_rn(a2, b1)));
-      n[1] = __fadd_rn(n[1], __fsub_rn(__fmul_rn(a2, b0), __fmul_rn(a0, b2)));
-      n[2] = __fadd_rn(n[2], __fsub_rn(__fmul_rn(a0, b1), __fmul_rn(a1, b0)));
+      float f[3];
+      face_cross(p0, p1, p2, f);
+      n[0] = __fadd_rn(n[0], f[0]);
+      n[1] = __fadd_rn(n[1], f[1]);
+      n[2] = __fadd_rn(n[2], f[2]);
     }
   }
 }
 
-__device__ __forceinline__ u32 prefix_in_cell(const BlockLds& S, int l, int j) { return S.vpre[8 * l + j]; }
+__device__ __forceinline__ unsigned long long cell_cases(const BlockLds& S, int l) {
+  return *reinterpret_cast<const unsigned long long*>(&S.cases[8 * l]);
+}
+__device__ __forceinline__ u32 prefix_in_cell(const BlockLds& S, int l, int j) {
+  return own_prefix8(cell_cases(S, l), j);
+}
 
 // Vertices of a staged block, in two phases per window of kListCap local
 // vertex ids: the mixed cubes write (cube, owned edge) at their local id,
 // then every thread takes consecutive ids (position + gathered normal sum).
-__device__ void mc_verts_block(BlockLds& S, int R, Geo geo, int blk, u32 base, u32 nverts, float* V, float* Nrm,
-                               u32 capV) {
+__device__ void mc_verts_block(BlockLds& S, int R, Geo geo, int blk, int c0x, int c0y, int c0z, bool useFn, u32 base,
+                               u32 nverts, float* V, float* Nrm, u32 capV) {
   for (u32 w0 = 0; w0 < nverts; w0 += kListCap) {
-    for (int s = threadIdx.x; s < 4096; s += blockDim.x) {
-      const int v = S.cases[s];
-      if (!mixed(v)) continue;
-      const int l = s >> 3, j = s & 7;
-      u32 lv = S.vOff[l] + prefix_in_cell(S, l, j) - w0;
-      const int c0 = v & 1;
-      const int on[3] = {c0 ^ ((v >> 1) & 1), c0 ^ ((v >> 3) & 1), c0 ^ ((v >> 4) & 1)};
+    for (int l = threadIdx.x; l < 512; l += blockDim.x) {
+      const unsigned long long w = cell_cases(S, l);
+      if (!w) continue;
+      // the cell's vertices, in cube and edge order, from its local base
+      u32 lv = S.vOff[l] - w0;
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        if (!on[q]) continue;
-        if (lv < (u32)kListCap) S.list[lv] = (uint16_t)((s << 3) | q);
-        ++lv;
+      for (int j = 0; j < 8; ++j) {
+        const int v = (int)((w >> (8 * j)) & 0xffu);
+        const int c0 = v & 1;
+        const int on[3] = {c0 ^ ((v >> 1) & 1), c0 ^ ((v >> 3) & 1), c0 ^ ((v >> 4) & 1)};
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          if (!on[q]) continue;
+          if (lv < (u32)kListCap) S.list[lv] = (uint16_t)(((8 * l + j) << 3) | q);
+          ++lv;
+        }
       }
     }
     __syncthreads();
@@ -598,14 +792,14 @@ __device__ void mc_verts_block(BlockLds& S, int R, Geo geo, int blk, u32 base, u
       const int e = S.list[i];
       const int s = e >> 3, q = e & 7, l = s >> 3, j = s & 7;
       int cx, cy, cz;
-      block_cell(blk, l, cx, cy, cz);
-      const int fx = 2 * cx + (j & 1), fy = 2 * cy + ((j >> 1) & 1), fz = 2 * cz + ((j >> 2) & 1);
+      local_cell(l, cx, cy, cz);
+      const int fx = 2 * (c0x + cx) + (j & 1), fy = 2 * (c0y + cy) + ((j >> 1) & 1), fz = 2 * (c0z + cz) + ((j >> 2) & 1);
       float p[3], nn[3];
       const int v = S.cases[s];
       // own edge q runs from corner 0 to corner 1 (x), 3 (y) or 4 (z)
       edge_pos_v(geo, fx, fy, fz, q == 0 ? 0 : (q == 1 ? 3 : 8), (u32)v & 1u,
                  (u32)(v >> (q == 0 ? 1 : (q == 1 ? 3 : 4))) & 1u, p);
-      normal_sum(S.tab, geo, S.box, R, fx, fy, fz, q, v, p, nn);
+      normal_sum(S, geo, R, c0x, c0y, c0z, useFn, fx, fy, fz, q, v, p, nn);
       V[3 * (size_t)vid] = p[0];
       V[3 * (size_t)vid + 1] = p[1];
       V[3 * (size_t)vid + 2] = p[2];
@@ -633,57 +827,62 @@ __device__ __forceinline__ u32 edge_vid(const BlockLds& S, int blk, int c0x, int
   int j = (gx & 1) | ((gy & 1) << 1) | ((gz & 1) << 2);
   int lx = cx - c0x, ly = cy - c0y, lz = cz - c0z;
   if (lx < 8 && ly < 8 && lz < 8) {
-    int l = (int)(part1by2((u32)lx) | (part1by2((u32)ly) << 1) | (part1by2((u32)lz) << 2));
-    return base + S.vOff[l] + prefix_in_cell(S, l, j) + (u32)rank_of(S.cases[8 * l + j], oe);
+    const int l = local_morton(lx, ly, lz);
+    const unsigned long long w = cell_cases(S, l);
+    return base + S.vOff[l] + own_prefix8(w, j) + (u32)rank_of((int)((w >> (8 * j)) & 0xffu), oe);
   }
-  // another block's cell: that block's vertex base + the cell's local base
-  const u32 code = part1by2((u32)cx) | (part1by2((u32)cy) << 1) | (part1by2((u32)cz) << 2);
-  u32 s = blockV[code >> 9] + cellL[(size_t)cx + (size_t)R * ((size_t)cy + (size_t)R * cz)];
-  for (int q = 0; q < j; ++q)
-    s += own_verts(cube_case(S.box, 2 * cx + (q & 1), 2 * cy + ((q >> 1) & 1), 2 * cz + ((q >> 2) & 1)));
-  return s + (u32)rank_of(cube_case(S.box, gx, gy, gz), oe);
+  // a cell just above the block (staged): its global base + in-cell prefix
+  const int k = nb_index(lx, ly, lz);
+  const unsigned long long w = S.nbAbove[k];
+  return S.nbBase[k] + own_prefix8(w, j) + (u32)rank_of((int)((w >> (8 * j)) & 0xffu), oe);
 }
-
-__device__ __forceinline__ u32 prefix_ntri(const BlockLds& S, int l, int j) { return S.tpre[8 * l + j]; }
 
 // Triangles of a staged block, in the same two phases over local triangle
 // slots. Listed-but-uncounted triangles (len > ntri) go to the late list
-// from the fill phase of the first window.
-__device__ void mc_tris_block(BlockLds& S, int R, int blk, int c0x, int c0y, int c0z, u32 vbase, u32 tbase,
-                              u32 ntris, const u32* __restrict__ blockV, const u32* __restrict__ cellL, u32* T,
-                              u32 capT, u32* late, u32 capLate, u32* totals) {
+// from the fill phase of the first window. With useFn (one window) each
+// counted triangle's face normal goes to S.fn at its local index.
+__device__ void mc_tris_block(BlockLds& S, int R, Geo geo, int blk, int c0x, int c0y, int c0z, bool useFn, u32 vbase,
+                              u32 tbase, u32 ntris, const u32* __restrict__ blockV, const u32* __restrict__ cellL,
+                              u32* T, u32 capT, u32* late, u32 capLate, u32* totals) {
   u32 border = 0u;
   for (u32 w0 = 0; w0 < ntris; w0 += kListCap) {
-    for (int s = threadIdx.x; s < 4096; s += blockDim.x) {
-      const int v = S.cases[s];
-      if (!mixed(v)) continue;
-      const int l = s >> 3, j = s & 7;
-      const u32 lt = S.tOff[l] + prefix_ntri(S, l, j);
-      const u32 cnt = S.tab.ntri[v], len = S.tab.len[v];
-      for (u32 q = 0; q < cnt; ++q)
-        if (lt + q - w0 < (u32)kListCap) S.list[lt + q - w0] = (uint16_t)((s << 3) | q);
-      if (w0 == 0 && len > cnt) {
-        int cx, cy, cz;
-        block_cell(blk, l, cx, cy, cz);
-        const int fx = 2 * cx + (j & 1), fy = 2 * cy + ((j >> 1) & 1), fz = 2 * cz + ((j >> 2) & 1);
-        const unsigned long long tri = S.tab.tri[v];
-        for (u32 q = cnt; q < len; ++q) {
-          const int e0 = (int)((tri >> (12 * q)) & 0xf), e1 = (int)((tri >> (12 * q + 4)) & 0xf),
-                    e2 = (int)((tri >> (12 * q + 8)) & 0xf);
-          const u32 a = edge_vid(S, blk, c0x, c0y, c0z, vbase, blockV, cellL, R, fx, fy, fz, e2, &border);
-          const u32 b = edge_vid(S, blk, c0x, c0y, c0z, vbase, blockV, cellL, R, fx, fy, fz, e1, &border);
-          const u32 c = edge_vid(S, blk, c0x, c0y, c0z, vbase, blockV, cellL, R, fx, fy, fz, e0, &border);
-          // listed but not counted: overwrites slot sl after all counted writes
-          const u32 k = atomicAdd(&late[0], 1u);
-          if (k < capLate) {
-            late[1 + 4 * k] = tbase + lt + q;
-            late[2 + 4 * k] = a;
-            late[3 + 4 * k] = b;
-            late[4 + 4 * k] = c;
-          } else {
-            atomicOr(&totals[2], 2u);
+    for (int l = threadIdx.x; l < 512; l += blockDim.x) {
+      const unsigned long long w = cell_cases(S, l);
+      if (!w) continue;
+      // the cell's counted triangles, in cube order, from its local base
+      u32 lt = S.tOff[l];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int v = (int)((w >> (8 * j)) & 0xffu);
+        const int s = 8 * l + j;
+        const u32 cnt = S.tab.ntri[v], len = S.tab.len[v];
+        for (u32 q = 0; q < cnt; ++q)
+          if (lt + q - w0 < (u32)kListCap) S.list[lt + q - w0] = (uint16_t)((s << 3) | q);
+        if (w0 == 0 && len > cnt) {
+          int cx, cy, cz;
+          local_cell(l, cx, cy, cz);
+          const int fx = 2 * (c0x + cx) + (j & 1), fy = 2 * (c0y + cy) + ((j >> 1) & 1),
+                    fz = 2 * (c0z + cz) + ((j >> 2) & 1);
+          const unsigned long long tri = S.tab.tri[v];
+          for (u32 q = cnt; q < len; ++q) {
+            const int e0 = (int)((tri >> (12 * q)) & 0xf), e1 = (int)((tri >> (12 * q + 4)) & 0xf),
+                      e2 = (int)((tri >> (12 * q + 8)) & 0xf);
+            const u32 a = edge_vid(S, blk, c0x, c0y, c0z, vbase, blockV, cellL, R, fx, fy, fz, e2, &border);
+            const u32 b = edge_vid(S, blk, c0x, c0y, c0z, vbase, blockV, cellL, R, fx, fy, fz, e1, &border);
+            const u32 c = edge_vid(S, blk, c0x, c0y, c0z, vbase, blockV, cellL, R, fx, fy, fz, e0, &border);
+            // listed but not counted: overwrites slot sl after all counted writes
+            const u32 k = atomicAdd(&late[0], 1u);
+            if (k < capLate) {
+              late[1 + 4 * k] = tbase + lt + q;
+              late[2 + 4 * k] = a;
+              late[3 + 4 * k] = b;
+              late[4 + 4 * k] = c;
+            } else {
+              atomicOr(&totals[2], 2u);
+            }
           }
         }
+        lt += cnt;
       }
     }
     __syncthreads();
@@ -694,26 +893,36 @@ __device__ void mc_tris_block(BlockLds& S, int R, int blk, int c0x, int c0y, int
       const int e = S.list[i];
       const int s = e >> 3, q = e & 7, l = s >> 3, j = s & 7;
       int cx, cy, cz;
-      block_cell(blk, l, cx, cy, cz);
-      const int fx = 2 * cx + (j & 1), fy = 2 * cy + ((j >> 1) & 1), fz = 2 * cz + ((j >> 2) & 1);
+      local_cell(l, cx, cy, cz);
+      const int fx = 2 * (c0x + cx) + (j & 1), fy = 2 * (c0y + cy) + ((j >> 1) & 1), fz = 2 * (c0z + cz) + ((j >> 2) & 1);
       const unsigned long long tri = S.tab.tri[S.cases[s]];
       const int e0 = (int)((tri >> (12 * q)) & 0xf), e1 = (int)((tri >> (12 * q + 4)) & 0xf),
                 e2 = (int)((tri >> (12 * q + 8)) & 0xf);
       T[3 * (size_t)sl] = edge_vid(S, blk, c0x, c0y, c0z, vbase, blockV, cellL, R, fx, fy, fz, e2, &border);
       T[3 * (size_t)sl + 1] = edge_vid(S, blk, c0x, c0y, c0z, vbase, blockV, cellL, R, fx, fy, fz, e1, &border);
       T[3 * (size_t)sl + 2] = edge_vid(S, blk, c0x, c0y, c0z, vbase, blockV, cellL, R, fx, fy, fz, e0, &border);
+      if (useFn) {
+        const int v = S.cases[s];
+        float p0[3], p1[3], p2[3];
+        local_edge_pos(geo, fx, fy, fz, v, e2, p0);
+        local_edge_pos(geo, fx, fy, fz, v, e1, p1);
+        local_edge_pos(geo, fx, fy, fz, v, e0, p2);
+        face_cross(p0, p1, p2, S.fn[i]);
+      }
     }
     __syncthreads();
   }
   if (border) atomicOr(&totals[3], 1u);
 }
 
-// One listed block: staged once, then its vertices and its triangles.
-__device__ __forceinline__ void mc_emit_block(BlockLds& S, const u32* __restrict__ C, int R, Geo geo, int blk,
-                                              int nblocks, const u32* __restrict__ blockV,
-                                              const u32* __restrict__ blockT, const u32* __restrict__ cellL, float* V,
-                                              float* Nrm, u32 capV, u32* T, u32 capT, u32* late, u32 capLate,
-                                              u32* totals) {
+// Listed block `it` (id blk): staged once, then its triangles and its
+// vertices.
+__device__ __forceinline__ void mc_emit_block(BlockLds& S, int R, Geo geo, u32 it, int blk, int nblocks,
+                                              const u32* __restrict__ blockV, const u32* __restrict__ blockT,
+                                              const u32* __restrict__ cellL,
+                                              const unsigned long long* __restrict__ cellCase,
+                                              const u32* __restrict__ posOf, float* V, float* Nrm, u32 capV, u32* T,
+                                              u32 capT, u32* late, u32 capLate, u32* totals) {
   int c0x, c0y, c0z;
   block_outside(blk, R, c0x, c0y, c0z);
   const u32 base = blockV[blk];
@@ -721,11 +930,14 @@ __device__ __forceinline__ void mc_emit_block(BlockLds& S, const u32* __restrict
   const u32 tbase = blockT[blk];
   const u32 tnext = (blk + 1 < nblocks) ? blockT[blk + 1] : totals[1];
   if (next == base && tnext == tbase) return;  // uniform across the workgroup
-  load_box(S.box, C, 2 * R, 2 * c0x - kLo, 2 * c0y - kLo, 2 * c0z - kLo, kBox, kBox);
-  stage_cases(S, blk, R);
+  stage_block(S, cellCase, posOf, blockV, cellL, R, it, c0x, c0y, c0z);
   scan_cells(S);
-  mc_verts_block(S, R, geo, blk, base, next - base, V, Nrm, capV);
-  mc_tris_block(S, R, blk, c0x, c0y, c0z, base, tbase, tnext - tbase, blockV, cellL, T, capT, late, capLate, totals);
+  // triangles first: their face normals (when they fit in LDS) are then
+  // gathered by the vertices
+  const bool useFn = tnext - tbase <= (u32)kFaceCap && tnext <= capT;
+  mc_tris_block(S, R, geo, blk, c0x, c0y, c0z, useFn, base, tbase, tnext - tbase, blockV, cellL, T, capT, late,
+                capLate, totals);
+  mc_verts_block(S, R, geo, blk, c0x, c0y, c0z, useFn, base, next - base, V, Nrm, capV);
 }
 
 // Vertices and triangles of the active blocks: each listed block is staged
@@ -733,20 +945,25 @@ __device__ __forceinline__ void mc_emit_block(BlockLds& S, const u32* __restrict
 // its triangles are written. Vertex ids on edges owned by another block's
 // cube come from that block's base and the per-cell local bases written by
 // k_mc_count, so no block waits for another.
-__global__ void __launch_bounds__(kMcThreads) k_mc_emit(const u32* __restrict__ C, int R, Geo geo,
-                                                         const u32* __restrict__ list, int nblocks,
+__global__ void __launch_bounds__(kMcThreads) k_mc_emit(int R, Geo geo, const u32* __restrict__ list, int nblocks,
                                                          const u32* __restrict__ blockV,
                                                          const u32* __restrict__ blockT,
-                                                         const u32* __restrict__ cellL, float* V, float* Nrm,
+                                                         const u32* __restrict__ cellL,
+                                                         const unsigned long long* __restrict__ cellCase,
+                                                         const u32* __restrict__ posOf,
+                                                         const u32* __restrict__ order, float* V, float* Nrm,
                                                          u32 capV, u32* T, u32 capT, u32* late, u32 capLate,
                                                          u32* totals) {
   __shared__ BlockLds S;
+  u32 nwork = 0u;  // blocks with output
+  for (int c = 0; c < kOrderClasses; ++c) nwork += order[c];
+  if (blockIdx.x >= nwork) return;  // the grid is sized for every block of the grid
   load_mc_tab(S.tab);
-  const u32 nlist = list[0];
-  for (u32 it = blockIdx.x; it < nlist; it += gridDim.x) {
+  for (u32 i = blockIdx.x; i < nwork; i += gridDim.x) {
     __syncthreads();  // LDS of the previous block is no longer read
-    mc_emit_block(S, C, R, geo, (int)list[1 + it], nblocks, blockV, blockT, cellL, V, Nrm, capV, T, capT, late,
-                  capLate, totals);
+    const u32 it = order_at(order, nblocks, i);
+    mc_emit_block(S, R, geo, it, (int)list[1 + it], nblocks, blockV, blockT, cellL, cellCase, posOf, V, Nrm, capV, T,
+                  capT, late, capLate, totals);
   }
 }
 
@@ -833,7 +1050,7 @@ __global__ void __launch_bounds__(256) k_mc_blocks_b(const MCJob* __restrict__ j
                                                      int n) {
   const int j = batch_job(pre, n, blockIdx.x);
   const MCJob& J = jobs[j];
-  mc_blocks_body(J.m.brick, J.R, J.m.nblocks, J.m.blockV, J.m.blockT, J.m.list,
+  mc_blocks_body(J.m.brick, J.R, J.m.nblocks, J.m.blockV, J.m.blockT, J.m.list, J.m.posOf,
                  (int)(blockIdx.x - pre[j]) * 256 + (int)threadIdx.x);
 }
 // Prefix of the jobs' active-block list lengths (one workgroup).
@@ -851,11 +1068,13 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_count_b(const MCJob* __restri
   __shared__ Box bx;
   __shared__ u32 red[16];
   __shared__ McTab tab;
+  if (blockIdx.x >= lpre[n]) return;
   load_mc_tab(tab);
   for (u32 u = blockIdx.x; u < lpre[n]; u += gridDim.x) {
     const int j = batch_job(lpre, n, u);
     const MCJob& J = jobs[j];
-    mc_count_block(J.C, J.R, (int)J.m.list[1 + u - lpre[j]], J.m.blockV, J.m.blockT, J.m.cellV, bx, red, tab);
+    mc_count_block(J.C, J.R, u - lpre[j], (int)J.m.list[1 + u - lpre[j]], J.m.blockV, J.m.blockT, J.m.cellV,
+                   J.m.cellCase, bx, red, tab);
     __syncthreads();
   }
 }
@@ -878,13 +1097,15 @@ __global__ void __launch_bounds__(256) k_scan_down_b(const MCJob* __restrict__ j
 __global__ void __launch_bounds__(kMcThreads) k_mc_emit_b(const MCJob* __restrict__ jobs, const u32* __restrict__ lpre,
                                                            int n) {
   __shared__ BlockLds S;
+  if (blockIdx.x >= lpre[n]) return;
   load_mc_tab(S.tab);
   for (u32 u = blockIdx.x; u < lpre[n]; u += gridDim.x) {
     __syncthreads();  // LDS of the previous block is no longer read
     const int j = batch_job(lpre, n, u);
     const MCJob& J = jobs[j];
-    mc_emit_block(S, J.C, J.R, J.geo, (int)J.m.list[1 + u - lpre[j]], J.m.nblocks, J.m.blockV, J.m.blockT, J.m.cellV,
-                  J.m.V, J.m.Nrm, J.m.capV, J.m.T, J.m.capT, J.m.late, J.m.capLate, J.m.totals);
+    mc_emit_block(S, J.R, J.geo, u - lpre[j], (int)J.m.list[1 + u - lpre[j]], J.m.nblocks, J.m.blockV, J.m.blockT,
+                  J.m.cellV, J.m.cellCase, J.m.posOf, J.m.V, J.m.Nrm, J.m.capV, J.m.T, J.m.capT, J.m.late,
+                  J.m.capLate, J.m.totals);
   }
 }
 __global__ void k_mc_fixup_b(const MCJob* __restrict__ jobs) {
@@ -960,10 +1181,10 @@ hipError_t launch_mc(const u32* C, int R, const float* bmin3, float cell, const 
   // active Morton blocks from a brick summary of C
   GM_CHECK(launch_brick_summary(C, 2 * R, m.brick, m.list, s));
   hipLaunchKernelGGL(k_mc_blocks, dim3((m.nblocks + 255) / 256), dim3(256), 0, s, m.brick, R, m.nblocks, m.blockV,
-                     m.blockT, m.list);
+                     m.blockT, m.list, m.posOf, m.order);
   GM_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_mc_count, dim3(mc_list_grid(m.nblocks, 2048)), dim3(kMcThreads), 0, s, C, R, m.list, m.blockV,
-                     m.blockT, m.cellV, m.late);
+                     m.blockT, m.cellV, m.cellCase, m.late);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
   int nparts = (int)mc_scan_parts(m.nblocks);
@@ -971,7 +1192,8 @@ hipError_t launch_mc(const u32* C, int R, const float* bmin3, float cell, const 
   GM_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, m.scanPart, nparts, m.totals, m.capV, m.capT);
   GM_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_scan_down, dim3(nparts), dim3(256), 0, s, m.blockV, m.blockT, m.nblocks, m.scanPart);
+  hipLaunchKernelGGL(k_scan_down, dim3(nparts), dim3(256), 0, s, m.blockV, m.blockT, m.nblocks, m.scanPart, m.posOf,
+                     m.order);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   return launch_mc_emit(C, R, bmin3, cell, m, s, ev ? ev + 2 : nullptr);
@@ -1064,9 +1286,9 @@ hipError_t launch_mc_emit(const u32* C, int R, const float* bmin3, float cell, c
                           hipEvent_t* ev) {
   GM_CHECK(upload_tables());
   Geo geo{bmin3[0], bmin3[1], bmin3[2], 0.5f * cell};
-  hipLaunchKernelGGL(k_mc_emit, dim3(mc_list_grid(m.nblocks, 16384)), dim3(kMcThreads), 0, s, C, R, geo, m.list,
-                     m.nblocks, m.blockV, m.blockT, m.cellV, m.V, m.Nrm, m.capV, m.T, m.capT, m.late, m.capLate,
-                     m.totals);
+  hipLaunchKernelGGL(k_mc_emit, dim3(mc_list_grid(m.nblocks, 16384)), dim3(kMcThreads), 0, s, R, geo, m.list,
+                     m.nblocks, m.blockV, m.blockT, m.cellV, m.cellCase, m.posOf, m.order, m.V, m.Nrm, m.capV, m.T,
+                     m.capT, m.late, m.capLate, m.totals);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
   hipLaunchKernelGGL(k_mc_fixup, dim3(1), dim3(64), 0, s, m.V, m.Nrm, m.T, m.late, m.capLate, m.totals, m.capV,

@@ -244,13 +244,24 @@ struct gm_engine {
             group.push_back(s);
           }
         if (group.empty()) continue;
-        const int rc = gm_batch_close_mesh(batch, ctxs.data(), (int)ctxs.size(), se);
-        const std::string msg = rc == GM_OK ? std::string() : std::string(gm_batch_last_error(batch));
+        int rc = gm_batch_close_mesh(batch, ctxs.data(), (int)ctxs.size(), se);
+        std::vector<int> rcs(group.size(), rc);
+        std::vector<std::string> msgs(group.size());
+        if (rc != GM_OK) {
+          // a job of the batch failed: each job's own status comes from
+          // closing it alone (the loaded grids are unchanged by a closing)
+          for (size_t i = 0; i < group.size(); ++i) {
+            gm_ctx* c = group[i]->ctx;
+            rcs[i] = gm_close(c, se);
+            if (rcs[i] == GM_OK) rcs[i] = gm_extract_mesh(c);
+            if (rcs[i] != GM_OK) msgs[i] = gm_last_error(c);
+          }
+        }
         std::lock_guard<std::mutex> lk(bm);
-        for (Slot* s : group) {
-          s->closeRc = rc;
-          s->closeErr = msg;
-          s->closed = true;
+        for (size_t i = 0; i < group.size(); ++i) {
+          group[i]->closeRc = rcs[i];
+          group[i]->closeErr = msgs[i];
+          group[i]->closed = true;
         }
       }
       cvClosed.notify_all();

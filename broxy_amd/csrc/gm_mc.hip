@@ -14,7 +14,7 @@
 //   mc_blocks  list of the blocks that can hold a mixed cube
 //   mc_count   per block vertex/triangle totals, and the block-local vertex
 //              base of every cell with vertices
-//   scan       exclusive scan of the block totals (reduce, top, down-sweep)
+//   scan       exclusive scan of the block totals (chunk reduce, down-sweep)
 //   mc_emit    cube cases in LDS, block scan over cells, then one thread per
 //              output slot: vertex positions with area-weighted normal sums
 //              gathered from the <= 4 cubes around each vertex, then
@@ -518,39 +518,11 @@ __global__ void __launch_bounds__(256) k_scan_reduce(const u32* __restrict__ a, 
   scan_reduce_body(a, b, n, part, blockIdx.x);
 }
 
-// Exclusive scan of the chunk totals (each thread a contiguous run of
-// ceil(nparts / 256) parts), then the overall totals and overflow flag.
-__device__ __forceinline__ void scan_top_body(u32* part, int nparts, u32* totals, u32 capV, u32 capT) {
-  __shared__ u32 red[16];
-  const int per = (nparts + 255) / 256;
-  const int p0 = threadIdx.x * per, p1 = min(p0 + per, nparts);
-  u32 sa = 0, sb = 0;
-  for (int p = p0; p < p1; ++p) {
-    sa += part[2 * p];
-    sb += part[2 * p + 1];
-  }
-  u32 ta, tb;
-  u32 oa = block_exscan(sa, red, &ta);
-  u32 ob = block_exscan(sb, red, &tb);
-  for (int p = p0; p < p1; ++p) {
-    const u32 va = part[2 * p], vb = part[2 * p + 1];
-    part[2 * p] = oa;
-    part[2 * p + 1] = ob;
-    oa += va;
-    ob += vb;
-  }
-  if (threadIdx.x == 0) {
-    totals[0] = ta;
-    totals[1] = tb;
-    totals[2] = (ta > capV || tb > capT) ? 1u : 0u;
-    totals[3] = 0u;
-  }
-}
-__global__ void __launch_bounds__(256) k_scan_top(u32* part, int nparts, u32* totals, u32 capV, u32 capT) {
-  scan_top_body(part, nparts, totals, capV, capT);
-}
-
-__device__ __forceinline__ void scan_down_body(u32* a, u32* b, int n, const u32* __restrict__ part, int chunk,
+// Down-sweep of one chunk: the chunk's offset is the sum of the chunk totals
+// before it (part holds the totals of all chunks); chunk 0 also writes the
+// overall totals and the overflow flag.
+__device__ __forceinline__ void scan_down_body(u32* a, u32* b, int n, const u32* __restrict__ part, int nparts,
+                                               int chunk, u32* totals, u32 capV, u32 capT,
                                                const u32* __restrict__ posOf = nullptr, u32* order = nullptr) {
   __shared__ u32 red[16];
   __shared__ u32 cls[2 * kOrderClasses];
@@ -565,9 +537,27 @@ __device__ __forceinline__ void scan_down_body(u32* a, u32* b, int n, const u32*
     sb += vb[k];
   }
   if (order && threadIdx.x < kOrderClasses) cls[threadIdx.x] = 0u;
+  const int lim = chunk == 0 ? nparts : chunk;  // uniform
+  u32 pa = 0, pb = 0;
+  for (int p = threadIdx.x; p < lim; p += blockDim.x) {
+    pa += part[2 * p];
+    pb += part[2 * p + 1];
+  }
+  u32 offA, offB;
+  block_exscan(pa, red, &offA);
+  block_exscan(pb, red, &offB);
+  if (chunk == 0) {
+    if (threadIdx.x == 0) {
+      totals[0] = offA;
+      totals[1] = offB;
+      totals[2] = (offA > capV || offB > capT) ? 1u : 0u;
+      totals[3] = 0u;
+    }
+    offA = offB = 0u;
+  }
   u32 ta, tb;
-  u32 oa = block_exscan(sa, red, &ta) + part[2 * chunk];
-  u32 ob = block_exscan(sb, red, &tb) + part[2 * chunk + 1];
+  u32 oa = block_exscan(sa, red, &ta) + offA;
+  u32 ob = block_exscan(sb, red, &tb) + offB;
   for (int k = 0; k < 4; ++k) {
     int i = base + k;
     if (i < n) {
@@ -599,9 +589,10 @@ __device__ __forceinline__ void scan_down_body(u32* a, u32* b, int n, const u32*
     }
   }
 }
-__global__ void __launch_bounds__(256) k_scan_down(u32* a, u32* b, int n, const u32* __restrict__ part,
-                                                   const u32* __restrict__ posOf, u32* order) {
-  scan_down_body(a, b, n, part, blockIdx.x, posOf, order);
+__global__ void __launch_bounds__(256) k_scan_down(u32* a, u32* b, int n, const u32* __restrict__ part, int nparts,
+                                                   u32* totals, u32 capV, u32 capT, const u32* __restrict__ posOf,
+                                                   u32* order) {
+  scan_down_body(a, b, n, part, nparts, blockIdx.x, totals, capV, capT, posOf, order);
 }
 
 // Position of the vertex on owned edge e (0: x, 3: y, 8: z) of cube f:
@@ -1084,15 +1075,12 @@ __global__ void __launch_bounds__(256) k_scan_reduce_b(const MCJob* __restrict__
   const MCJob& J = jobs[j];
   scan_reduce_body(J.m.blockV, J.m.blockT, J.m.nblocks, J.m.scanPart, (int)(blockIdx.x - pre[j]));
 }
-__global__ void __launch_bounds__(256) k_scan_top_b(const MCJob* __restrict__ jobs) {
-  const MCJob& J = jobs[blockIdx.x];
-  scan_top_body(J.m.scanPart, (J.m.nblocks + kChunk - 1) / kChunk, J.m.totals, J.m.capV, J.m.capT);
-}
 __global__ void __launch_bounds__(256) k_scan_down_b(const MCJob* __restrict__ jobs, const u32* __restrict__ pre,
                                                      int n) {
   const int j = batch_job(pre, n, blockIdx.x);
   const MCJob& J = jobs[j];
-  scan_down_body(J.m.blockV, J.m.blockT, J.m.nblocks, J.m.scanPart, (int)(blockIdx.x - pre[j]));
+  scan_down_body(J.m.blockV, J.m.blockT, J.m.nblocks, J.m.scanPart, (J.m.nblocks + kChunk - 1) / kChunk,
+                 (int)(blockIdx.x - pre[j]), J.m.totals, J.m.capV, J.m.capT);
 }
 __global__ void __launch_bounds__(kMcThreads) k_mc_emit_b(const MCJob* __restrict__ jobs, const u32* __restrict__ lpre,
                                                            int n) {
@@ -1190,10 +1178,8 @@ hipError_t launch_mc(const u32* C, int R, const float* bmin3, float cell, const 
   int nparts = (int)mc_scan_parts(m.nblocks);
   hipLaunchKernelGGL(k_scan_reduce, dim3(nparts), dim3(256), 0, s, m.blockV, m.blockT, m.nblocks, m.scanPart);
   GM_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, m.scanPart, nparts, m.totals, m.capV, m.capT);
-  GM_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_scan_down, dim3(nparts), dim3(256), 0, s, m.blockV, m.blockT, m.nblocks, m.scanPart, m.posOf,
-                     m.order);
+  hipLaunchKernelGGL(k_scan_down, dim3(nparts), dim3(256), 0, s, m.blockV, m.blockT, m.nblocks, m.scanPart, nparts,
+                     m.totals, m.capV, m.capT, m.posOf, m.order);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   return launch_mc_emit(C, R, bmin3, cell, m, s, ev ? ev + 2 : nullptr);
@@ -1265,8 +1251,6 @@ hipError_t launch_mc_batched(const BatchMCJob* in, int n, void* hBlob, void* dBl
   hipLaunchKernelGGL(k_mc_count_b, dim3(std::min(listGrid, 4096u)), dim3(kMcThreads), 0, s, dj, lpre, n);
   GM_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_scan_reduce_b, dim3(preC[n]), dim3(256), 0, s, dj, dPreC, n);
-  GM_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_scan_top_b, dim3(n), dim3(256), 0, s, dj);
   GM_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_scan_down_b, dim3(preC[n]), dim3(256), 0, s, dj, dPreC, n);
   GM_CHECK(hipGetLastError());

@@ -1041,6 +1041,9 @@ __global__ void __launch_bounds__(256) k_mc_blocks_b(const MCJob* __restrict__ j
                                                      int n) {
   const int j = batch_job(pre, n, blockIdx.x);
   const MCJob& J = jobs[j];
+  // each job's emit order is built too (a mesh that overflows its output is
+  // emitted again by the single-job emit, which walks that order)
+  if (blockIdx.x == pre[j] && threadIdx.x < kOrderClasses) J.m.order[threadIdx.x] = 0u;
   mc_blocks_body(J.m.brick, J.R, J.m.nblocks, J.m.blockV, J.m.blockT, J.m.list, J.m.posOf,
                  (int)(blockIdx.x - pre[j]) * 256 + (int)threadIdx.x);
 }
@@ -1080,7 +1083,7 @@ __global__ void __launch_bounds__(256) k_scan_down_b(const MCJob* __restrict__ j
   const int j = batch_job(pre, n, blockIdx.x);
   const MCJob& J = jobs[j];
   scan_down_body(J.m.blockV, J.m.blockT, J.m.nblocks, J.m.scanPart, (J.m.nblocks + kChunk - 1) / kChunk,
-                 (int)(blockIdx.x - pre[j]), J.m.totals, J.m.capV, J.m.capT);
+                 (int)(blockIdx.x - pre[j]), J.m.totals, J.m.capV, J.m.capT, J.m.posOf, J.m.order);
 }
 __global__ void __launch_bounds__(kMcThreads) k_mc_emit_b(const MCJob* __restrict__ jobs, const u32* __restrict__ lpre,
                                                            int n) {

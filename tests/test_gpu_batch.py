@@ -10,7 +10,7 @@ import pytest
 
 import _oracle as O
 import broxy_amd as B
-from _util import random_blobs
+from _util import cells_from_fine, random_blobs
 
 pytestmark = pytest.mark.gpu
 
@@ -110,4 +110,45 @@ def test_batch_against_oracle_and_regrowth():
             assert V.shape == Vr.shape and np.allclose(V, Vr, rtol=1e-6, atol=0)
             c.close()
     bt.close_mesh([])
+    bt.close()
+
+
+def test_batch_mesh_overflow_emitted_again():
+    """A mesh far larger than a fresh context's initial output capacity,
+    closed in a batch: reading the mesh grows the output and emits it again
+    with the single-job emit, which walks the emit order that the batched
+    scan built. Every result equals the job closed alone."""
+    R = 48
+    N = 2 * R
+    f = np.zeros((N, N, N), bool)
+    for z in range(6, N - 9, 8):  # 11^3 separate 3^3-voxel cubes
+        for y in range(6, N - 9, 8):
+            for x in range(6, N - 9, 8):
+                f[z:z + 3, y:y + 3, x:x + 3] = True
+    I = cells_from_fine(f)
+
+    def make():
+        c = B.Context(0)
+        bmin = np.zeros(3, np.float32)
+        cell = np.float32(1.0 / R)
+        c.grid_init(bmin, bmin + cell * R, R, np.array([R, R, R], np.int32), float(cell))
+        c.set_cells(B.GRID_INPUT, I)
+        c.scale_uniform(2)
+        return c
+
+    say("overflow: alone at R=%d" % R)
+    alone = make()
+    alone.close_grid(B.SE_SPHERE)
+    alone.extract_mesh()
+    V, _, _ = alone.mesh()
+    assert V.shape[0] > 4 * N * N  # beyond the initial output capacity
+    ref = result(alone)
+    alone.close()
+    ctxs = [make(), make()]
+    bt = B.Batch(0)
+    say("overflow: batch of %d fresh contexts" % len(ctxs))
+    bt.close_mesh(ctxs)
+    for c in ctxs:
+        assert result(c) == ref
+        c.close()
     bt.close()

@@ -30,13 +30,21 @@ KERNEL_OF_STAGE = {"brick_summary": "k_brick_summary", "dilate_coarse": "k_pass_
                    "tight_contour": "k_tight", "dilate_fine": "k_pass_ct<1,rho>", "contour_K": "k_contour_K",
                    "erode_coarse": "k_pass_ct<2,rho>", "base_contour": "k_base_contour",
                    "erode_fine": "k_pass_ct<3,rho>", "mc_count": "k_mc_count",
-                   "mc_scan": "k_scan_reduce+k_scan_top+k_scan_down", "mc_emit": "k_mc_emit",
-                   "mc_fixup_normalize": "k_mc_fixup+k_mc_normalize"}
+                   "mc_scan": "k_scan_reduce+k_scan_down", "mc_emit": "k_mc_emit",
+                   "mc_fixup_normalize": "k_mc_fixup+k_mc_normalize",
+                   "dilate_coarse_stream": "k_pass_ct<0,rho>", "dilate_fine_stream": "k_pass_ct<1,rho>",
+                   "erode_coarse_stream": "k_pass_ct<2,rho>", "erode_fine_stream": "k_pass_ct<3,rho>"}
+# Stages timed over exactly one kernel launch: the dominant kernel is chosen
+# among these (a pass stage also holds its tile selection).
+SINGLE_KERNEL_STAGES = ("brick_summary", "tight_contour", "contour_K", "base_contour", "dilate_coarse_stream",
+                        "dilate_fine_stream", "erode_coarse_stream", "erode_fine_stream", "mc_emit")
 
 
-# per-voxel radii run the runtime-table kernels
-VAR_KERNEL = {"dilate_coarse": "k_dil_coarse", "dilate_fine": "k_dil_fine", "erode_coarse": "k_ero_coarse",
-              "erode_fine": "k_ero_fine"}
+# per-voxel radii: one compiled-radius launch per radius value
+VAR_KERNEL = {"dilate_coarse_stream": "k_pass_ct<0,v+1> per radius value v",
+              "dilate_fine_stream": "k_pass_ct<1,v+1> per radius value v",
+              "erode_coarse_stream": "k_pass_ct<2,v-1> per radius value v",
+              "erode_fine_stream": "k_pass_ct<3,v-1> per radius value v"}
 
 
 def algorithmic_bytes(stage, N, nv, nt):
@@ -45,6 +53,7 @@ def algorithmic_bytes(stage, N, nv, nt):
     and triangle."""
     g = N ** 3 / 8.0
     f = N ** 3 / 64.0
+    stage = stage.replace("_stream", "")
     return {
         "brick_summary": g,                 # read I
         "dilate_coarse": g + f,            # read I, write DF
@@ -349,7 +358,7 @@ def main():
         ctx.extract_mesh()
         planes_ms = float(ctx.stage_times()[0] - stage_ms[0])
         ctx.set_timing(False)
-    dom = int(np.argmax(stage_ms))
+    dom = max((i for i, s in enumerate(STAGES) if s in SINGLE_KERNEL_STAGES), key=lambda i: stage_ms[i])
     dname = STAGES[dom]
     dbytes = algorithmic_bytes(dname, N, nv, nt)
     achieved = dbytes / (stage_ms[dom] * 1e-3) / 1e9
@@ -363,7 +372,7 @@ def main():
     if os.path.exists(tf):
         with open(tf) as fh:
             tr = json.load(fh)
-        ent = tr.get("stages", {}).get(dname)
+        ent = tr.get("stages", {}).get(dname.replace("_stream", ""))
         if ent:
             traffic = ent.get("bytes")
 

@@ -219,7 +219,8 @@ class Context:
         self._call("gm_set_timing", 1 if on else 0)
 
     STAGES = ("brick_summary", "dilate_coarse", "tight_contour", "dilate_fine", "contour_K", "erode_coarse",
-              "base_contour", "erode_fine", "mc_count", "mc_scan", "mc_emit", "mc_fixup_normalize")
+              "base_contour", "erode_fine", "mc_count", "mc_scan", "mc_emit", "mc_fixup_normalize",
+              "dilate_coarse_stream", "dilate_fine_stream", "erode_coarse_stream", "erode_fine_stream")
 
     def stage_times(self):
         """Device ms per stage (one kernel each, see STAGES) of the last

@@ -74,7 +74,7 @@ struct gm_ctx {
   uint64_t pyrTexels = 0;
   // timing
   bool timing = false;
-  hipEvent_t ev[16] = {};
+  hipEvent_t ev[20] = {};
   int evValid = 0;
 };
 
@@ -391,7 +391,7 @@ int do_dilate(gm_ctx* c, int se) {
     MorphBuffers b = morph_buffers(c);
     b.tables = c->dCube;
     hipError_t e = launch_dilate(morph_dims(c), b, radius_field(c), 1, c->cubeTabs, lo, hi, c->cubeWords, c->stream,
-                                 c->timing ? &c->ev[0] : nullptr);
+                                 c->timing ? &c->ev[0] : nullptr, c->timing ? &c->ev[15] : nullptr);
     if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("dilation: ") + hipGetErrorString(e));
   } else {
     rc = ensure_tables(c, lo, hi);
@@ -399,7 +399,7 @@ int do_dilate(gm_ctx* c, int se) {
     rc = ensure_var_buffers(c);
     if (rc) return rc;
     hipError_t e = launch_dilate(morph_dims(c), morph_buffers(c), radius_field(c), 0, c->tabs, lo, hi, c->tabWords,
-                                 c->stream, c->timing ? &c->ev[0] : nullptr);
+                                 c->stream, c->timing ? &c->ev[0] : nullptr, c->timing ? &c->ev[15] : nullptr);
     if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("dilation: ") + hipGetErrorString(e));
     if (c->rmode != 0) c->rpValid = true;
   }
@@ -415,7 +415,8 @@ int do_erode(gm_ctx* c) {
   rc = ensure_var_buffers(c);
   if (rc) return rc;
   hipError_t e = launch_erode(morph_dims(c), morph_buffers(c), radius_field(c), c->tabs, c->rlo, c->rhi,
-                              c->tabWords, c->stream, c->timing ? &c->ev[5] : nullptr);
+                              c->tabWords, c->stream, c->timing ? &c->ev[5] : nullptr,
+                              c->timing ? &c->ev[17] : nullptr);
   if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("erosion: ") + hipGetErrorString(e));
   if (c->rmode != 0) c->rpValid = true;
   c->meshPending = false;
@@ -826,16 +827,18 @@ int gm_set_timing(gm_ctx* c, int enable) {
   return GM_OK;
 }
 
-constexpr int kNumStages = 12;
+constexpr int kNumStages = 16;
 int gm_stage_times(gm_ctx* c, float* ms, int n) {
   if (!c || !ms) return GM_ERR_ARG;
   HIPCALL(c, hipStreamSynchronize(c->stream));
-  // events: dilate 0..4, erode 5..9, mc 10..14 (one stage per kernel or scan)
-  const int pairs[kNumStages][2] = {{0, 1}, {1, 2}, {2, 3},   {3, 4},   {5, 6},   {6, 7},
-                                    {7, 8}, {8, 9}, {10, 11}, {11, 12}, {12, 13}, {13, 14}};
+  // events: dilate 0..4, erode 5..9, mc 10..14 (one stage per kernel or
+  // scan); then the four stream kernels alone (from 15..18, recorded after
+  // each pass's tile selection, to the end of the pass)
+  const int pairs[kNumStages][2] = {{0, 1},   {1, 2},   {2, 3},   {3, 4},  {5, 6},  {6, 7},  {7, 8},  {8, 9},
+                                    {10, 11}, {11, 12}, {12, 13}, {13, 14}, {15, 2}, {16, 4}, {17, 7}, {18, 9}};
   for (int i = 0; i < n && i < kNumStages; ++i) {
     ms[i] = -1.f;
-    bool ok = (i < 8) ? (c->evValid & 1) : (c->evValid & 2);
+    bool ok = (i < 8 || i >= 12) ? (c->evValid & 1) : (c->evValid & 2);
     if (!ok) continue;
     float t = 0.f;
     if (hipEventElapsedTime(&t, c->ev[pairs[i][0]], c->ev[pairs[i][1]]) == hipSuccess) ms[i] = t;

@@ -169,16 +169,18 @@ struct SETableSet {
   int ero_corner[129];
   int ero_centre[129];
 };
+// ev (optional): 5 stage events; evs (optional): 2 events recorded just
+// before the coarse and the fine stream kernels (after their tile selection).
 hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf, int se,
                          const SETableSet& tabs, int rho_lo, int rho_hi, int tabWords, hipStream_t s,
-                         hipEvent_t* ev);
+                         hipEvent_t* ev, hipEvent_t* evs);
 // Brick summary of a bit grid of edge N: one byte per 1 word x 8 rows x 8
 // planes, bit0 any voxel set, bit1 all set. clear (optional): a u32 counter
 // set to zero by the same launch.
 hipError_t launch_brick_summary(const u32* G, int N, uint8_t* out, u32* clear, hipStream_t s);
 hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf,
                         const SETableSet& tabs, int rho_lo, int rho_hi, int tabWords, hipStream_t s,
-                        hipEvent_t* ev);
+                        hipEvent_t* ev, hipEvent_t* evs);
 // Batched closing of uniform-radius sphere jobs (radius r in [2, 31]), all
 // jobs on stream s. The caller provides a pinned host blob and a device blob
 // of batch_morph_bytes(n) (per-job argument tables and work prefixes), and a

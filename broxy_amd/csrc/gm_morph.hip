@@ -497,8 +497,10 @@ int device_cus() {
 // finished there, active ones queued), then persistent workgroups of the
 // kernel compiled for chord-table radius rho when there is one (fixedOk:
 // uniform radius, sphere), else of the runtime-table kernel `generic`.
+// evStream (optional) is recorded between the tile selection and the stream
+// kernel, so that the stream kernel can be timed alone.
 hipError_t launch_pass(int pass, const void* generic, MorphArgs a, int* qsel, bool var, bool fixedOk, int rho,
-                       hipStream_t s) {
+                       hipStream_t s, hipEvent_t evStream) {
   a.qsel = *qsel;
   *qsel = 1 - *qsel;
   const void* fn = generic;
@@ -515,6 +517,7 @@ hipError_t launch_pass(int pass, const void* generic, MorphArgs a, int* qsel, bo
   a.qCap = a.queueCap;
   void* args[] = {&a};
   GM_CHECK(hipLaunchKernel(sel[pass], dim3((tiles + 3) / 4), dim3(256), args, 0, s));
+  if (evStream) GM_CHECK(hipEventRecord(evStream, s));
   const int nw = stream_waves(a, var);
   const size_t lds = lds_bytes(a, var, nw);
   // one workgroup per tile at most: workgroup i takes entries i/8, i/8 + G/8,
@@ -530,7 +533,7 @@ hipError_t launch_pass(int pass, const void* generic, MorphArgs a, int* qsel, bo
 // radius of v (dilation v + 1, erosion v - 1). Erosion launches read the
 // source kv, rebuilt for each value first (*vprev: the value kv holds).
 hipError_t launch_pass_var(int pass, MorphArgs a, const MorphBuffers& b, int vlo, int vhi, int* vprev,
-                           hipStream_t s) {
+                           hipStream_t s, hipEvent_t evStream) {
   a.qsel = *b.qsel;
   *b.qsel = 1 - *b.qsel;
   a.rvar = 1;
@@ -543,6 +546,7 @@ hipError_t launch_pass_var(int pass, MorphArgs a, const MorphBuffers& b, int vlo
     void* args[] = {&a};
     GM_CHECK(hipLaunchKernel(sel[pass], dim3((tiles + 3) / 4), dim3(256), args, 0, s));
   }
+  if (evStream) GM_CHECK(hipEventRecord(evStream, s));
   const bool ero = pass >= 2;
   // k_kv: 4 waves per workgroup, about one wave per brick of K (bounded)
   const unsigned bricks = (unsigned)((a.N / 8) * (a.N / 8) * a.W);
@@ -601,7 +605,7 @@ hipError_t launch_brick_summary(const u32* G, int N, uint8_t* out, u32* clear, h
 // the runtime-table kernels.
 hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf, int se,
                          const SETableSet& tabs, int rho_lo, int rho_hi, int tabWords, hipStream_t s,
-                         hipEvent_t* ev) {
+                         hipEvent_t* ev, hipEvent_t* evs) {
   MorphArgs a = base_args(d, b, rf, tabs);
   a.se = se;
   const int rmax = rho_hi + (se == 0 ? 1 : 0);
@@ -629,8 +633,10 @@ hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const Radius
                        b.DF);
     GM_CHECK(hipGetLastError());
   }
-  if (perValue) GM_CHECK(launch_pass_var(0, a, b, vlo, vhi, &vprev, s));
-  else GM_CHECK(launch_pass(0, reinterpret_cast<const void*>(&k_dil_coarse), a, b.qsel, var, fixedOk, rho, s));
+  if (perValue) GM_CHECK(launch_pass_var(0, a, b, vlo, vhi, &vprev, s, evs ? evs[0] : nullptr));
+  else
+    GM_CHECK(launch_pass(0, reinterpret_cast<const void*>(&k_dil_coarse), a, b.qsel, var, fixedOk, rho, s,
+                         evs ? evs[0] : nullptr));
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   a.outFlags = b.TG;
   a.outRows = b.D;
@@ -639,14 +645,16 @@ hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const Radius
   if (ev) GM_CHECK(hipEventRecord(ev[3], s));
   a.ystep = 1;
   a.outRows = b.D;
-  if (perValue) GM_CHECK(launch_pass_var(1, a, b, vlo, vhi, &vprev, s));
-  else GM_CHECK(launch_pass(1, reinterpret_cast<const void*>(&k_dil_fine), a, b.qsel, var, fixedOk, rho, s));
+  if (perValue) GM_CHECK(launch_pass_var(1, a, b, vlo, vhi, &vprev, s, evs ? evs[1] : nullptr));
+  else
+    GM_CHECK(launch_pass(1, reinterpret_cast<const void*>(&k_dil_fine), a, b.qsel, var, fixedOk, rho, s,
+                         evs ? evs[1] : nullptr));
   if (ev) GM_CHECK(hipEventRecord(ev[4], s));
   return hipSuccess;
 }
 
 hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf, const SETableSet& tabs,
-                        int rho_lo, int rho_hi, int tabWords, hipStream_t s, hipEvent_t* ev) {
+                        int rho_lo, int rho_hi, int tabWords, hipStream_t s, hipEvent_t* ev, hipEvent_t* evs) {
   MorphArgs a = base_args(d, b, rf, tabs);
   int rmax = rho_hi - 1;  // erosion radius r-1
   if (rmax < 1) rmax = 1;
@@ -675,8 +683,10 @@ hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusF
   int vprev = -1;
   a.ystep = 2;
   a.outFlags = b.CL;
-  if (perValue) GM_CHECK(launch_pass_var(2, a, b, vlo, vhi, &vprev, s));
-  else GM_CHECK(launch_pass(2, reinterpret_cast<const void*>(&k_ero_coarse), a, b.qsel, var, fixedOk, rho, s));
+  if (perValue) GM_CHECK(launch_pass_var(2, a, b, vlo, vhi, &vprev, s, evs ? evs[0] : nullptr));
+  else
+    GM_CHECK(launch_pass(2, reinterpret_cast<const void*>(&k_ero_coarse), a, b.qsel, var, fixedOk, rho, s,
+                         evs ? evs[0] : nullptr));
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   a.outFlags = b.UG;
   a.outRows = b.C;
@@ -685,8 +695,10 @@ hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusF
   if (ev) GM_CHECK(hipEventRecord(ev[3], s));
   a.ystep = 1;
   a.outRows = b.C;
-  if (perValue) GM_CHECK(launch_pass_var(3, a, b, vlo, vhi, &vprev, s));
-  else GM_CHECK(launch_pass(3, reinterpret_cast<const void*>(&k_ero_fine), a, b.qsel, var, fixedOk, rho, s));
+  if (perValue) GM_CHECK(launch_pass_var(3, a, b, vlo, vhi, &vprev, s, evs ? evs[1] : nullptr));
+  else
+    GM_CHECK(launch_pass(3, reinterpret_cast<const void*>(&k_ero_fine), a, b.qsel, var, fixedOk, rho, s,
+                         evs ? evs[1] : nullptr));
   if (ev) GM_CHECK(hipEventRecord(ev[4], s));
   return hipSuccess;
 }

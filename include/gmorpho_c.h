@@ -115,13 +115,14 @@ int gm_pyramid_get(gm_ctx* ctx, uint64_t* texels);
 
 /* Per-stage device timings of the last gm_close / gm_extract_mesh, in ms,
  * from HIP events on the context stream (the reference prints GET_TIME
- * brackets, GMorpho.cu:7796-7815). 12 stages: brick summary (with the
+ * brackets, GMorpho.cu:7796-7815). 16 stages: brick summary (with the
  * radius planes on the first closing after a per-voxel field is set),
  * dilate coarse, tight contour, dilate fine, contour K, erode coarse, base
- * contour, erode fine, mc count, mc scan (3 small kernels), mc vertices and
- * triangles, mc fixup+normalize. A pass stage holds its tile selection
- * and its streaming launches (one per radius value with per-voxel radii).
- * Timing is off unless enabled. */
+ * contour, erode fine, mc count, mc scan (2 small kernels), mc vertices and
+ * triangles, mc fixup+normalize; then the streaming launches of the four
+ * passes alone (dilate coarse, dilate fine, erode coarse, erode fine). A
+ * pass stage holds its tile selection and its streaming launches (one per
+ * radius value with per-voxel radii). Timing is off unless enabled. */
 int gm_set_timing(gm_ctx* ctx, int enable);
 int gm_stage_times(gm_ctx* ctx, float* ms, int n);
 /* Wait for all work enqueued on the context. */

@@ -250,6 +250,24 @@ def test_mesh_blobs(ctx):
     _mesh_compare(ctx, C, R, bmin, cell)
 
 
+def test_stage_times(ctx):
+    """Per-stage timings: every stage is timed, and each stream kernel alone
+    takes no longer than its pass (tile selection + stream)."""
+    R = 32
+    I = random_blobs(R, seed=22, nballs=8)
+    grid_setup(ctx, R, I)
+    ctx.scale_uniform(4)
+    say("stage times R=32")
+    ctx.set_timing(True)
+    ctx.close_grid(B.SE_SPHERE)
+    ctx.extract_mesh()
+    ms = dict(zip(B.Context.STAGES, ctx.stage_times()))
+    ctx.set_timing(False)
+    assert len(ms) == 16 and all(v >= 0 for v in ms.values()), ms
+    for p in ("dilate_coarse", "dilate_fine", "erode_coarse", "erode_fine"):
+        assert ms[p + "_stream"] <= ms[p], (p, ms)
+
+
 def test_config0_end_to_end(ctx):
     P, T = B.sphere_soup(64, 3, seed=1234)
     R = 64

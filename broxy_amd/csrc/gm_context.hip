@@ -29,7 +29,7 @@ struct gm_ctx {
   float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0}, cell = 0.f;
   int data_res[3] = {0, 0, 0};
   u32 *I = nullptr, *D = nullptr, *K = nullptr, *C = nullptr;
-  u16 *DF = nullptr, *TG = nullptr, *CL = nullptr, *UG = nullptr;
+  u16 *DF = nullptr, *TG = nullptr, *CL = nullptr, *UG = nullptr, *IFL = nullptr, *DAL = nullptr;
   uint8_t *brickI = nullptr, *brickK = nullptr;
   u32* queue = nullptr;  // active-tile queues of the streaming passes
   int qsel = 0;          // counter block of the next streaming pass
@@ -99,7 +99,7 @@ void dfree(T*& p) {
 
 void free_grid(gm_ctx* c) {
   dfree(c->I); dfree(c->D); dfree(c->K); dfree(c->C);
-  dfree(c->DF); dfree(c->TG); dfree(c->CL); dfree(c->UG); dfree(c->brickI); dfree(c->brickK); dfree(c->queue);
+  dfree(c->DF); dfree(c->TG); dfree(c->CL); dfree(c->UG); dfree(c->IFL); dfree(c->DAL); dfree(c->brickI); dfree(c->brickK); dfree(c->queue);
   dfree(c->stage); dfree(c->mark); dfree(c->shell);
   dfree(c->mc.blockV); dfree(c->mc.blockT); dfree(c->mc.cellV); dfree(c->mc.cellCase); dfree(c->mc.posOf); dfree(c->mc.order);
   dfree(c->mc.V); dfree(c->mc.Nrm); dfree(c->mc.T); dfree(c->mc.late); dfree(c->mc.totals); dfree(c->mc.scanPart);
@@ -184,6 +184,8 @@ int alloc_grid(gm_ctx* c, int R) {
   HIPCALL(c, hipMalloc(&c->TG, flagBytes));
   HIPCALL(c, hipMalloc(&c->CL, flagBytes));
   HIPCALL(c, hipMalloc(&c->UG, flagBytes));
+  HIPCALL(c, hipMalloc(&c->IFL, flagBytes));
+  HIPCALL(c, hipMalloc(&c->DAL, flagBytes));
   size_t brickBytes = (size_t)(c->N / 8) * (c->N / 8) * c->W;
   HIPCALL(c, hipMalloc(&c->brickI, brickBytes));
   HIPCALL(c, hipMalloc(&c->brickK, brickBytes));
@@ -335,6 +337,8 @@ MorphBuffers morph_buffers(gm_ctx* c) {
   b.TG = c->TG;
   b.CL = c->CL;
   b.UG = c->UG;
+  b.IFL = c->IFL;
+  b.DAL = c->DAL;
   b.tables = c->dTables;
   b.brickI = c->brickI;
   b.brickK = c->brickK;

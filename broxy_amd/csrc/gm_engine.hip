@@ -107,7 +107,7 @@ uint64_t context_bytes(int R, int L, const gm_job& j) {
   while (P < (uint64_t)R) P <<= 1;
   uint64_t b = 0;
   b += 6 * N * N * W * 4;                         // I, D, K, C, voxelizer marks and shell
-  b += 4 * (uint64_t)R * R * W * 2;               // DF, TG, CL, UG
+  b += 6 * (uint64_t)R * R * W * 2;               // DF, TG, CL, UG, I-full and D-any cells
   b += 2 * (N / 8) * (N / 8) * W;                 // brick summaries
   b += (uint64_t)R * R * R;                       // cell-byte staging
   b += 7 * (P / 8) * (P / 8) * (P / 8) * 4;       // MC block counts, list positions, emit order

@@ -123,6 +123,8 @@ struct MorphBuffers {
   u16* TG;  // tight contour of DF
   u16* CL;  // coarse erosion: cell cleared
   u16* UG;  // base contour of the coarse erosion
+  u16* IFL; // cells full in I
+  u16* DAL; // cells with a voxel of D
   u32* tables;       // device SE tables (concatenated)
   uint8_t* brickI;   // per 1 word x 8 rows x 8 planes brick of I: bit0 any, bit1 all
   uint8_t* brickK;   // same for K (bit0 only)

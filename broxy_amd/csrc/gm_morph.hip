@@ -96,9 +96,12 @@ __global__ void k_tight(MorphArgs a) { tight_body(a, (long long)blockIdx.x * blo
 // that no separate memset is needed.
 // Workgroups may have more than 8*W threads (batched launches size them for
 // the widest job); the extra threads only meet the barriers.
+// ifl (optional): the cell flags of the full cells of G (all 8 fine voxels
+// set), for the 4x4 cells of the brick column (u16 per cell row word).
 __device__ __forceinline__ void brick_summary_body(const u32* __restrict__ G, int N, int W, uint8_t* __restrict__ out,
-                                                   int by, int bz) {
+                                                   int by, int bz, u16* __restrict__ ifl = nullptr) {
   __shared__ u32 anyW[64], allW[64];
+  __shared__ u32 pairW[8][4][64];  // row yl, plane pair k, word: AND of planes 2k, 2k+1
   const int NB = N / 8;
   const bool act = threadIdx.x < 8u * (unsigned)W;
   const int w = threadIdx.x % W, yl = threadIdx.x / W;
@@ -109,25 +112,39 @@ __device__ __forceinline__ void brick_summary_body(const u32* __restrict__ G, in
   __syncthreads();
   if (act) {
     const int y = 8 * by + yl;
-    u32 any = 0u, all = 0xffffffffu;
+    u32 any = 0u, all = 0xffffffffu, pr[4];
     const u32* g = G + ((size_t)(8 * bz) * N + y) * W + w;
 #pragma unroll
     for (int zl = 0; zl < 8; ++zl) {
       u32 v = g[(size_t)zl * N * W];
       any |= v;
       all &= v;
+      pr[zl >> 1] = (zl & 1) ? (pr[zl >> 1] & v) : v;
     }
     if (any) atomicOr(&anyW[w], 1u);
     if (all != 0xffffffffu) atomicAnd(&allW[w], 0u);
+    if (ifl)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) pairW[yl][k][w] = pr[k];
   }
   __syncthreads();
   if (threadIdx.x < W)
     out[((size_t)bz * NB + by) * W + threadIdx.x] = (uint8_t)(anyW[threadIdx.x] | (allW[threadIdx.x] << 1));
+  if (ifl && act && !(yl & 1)) {
+    // cell (4by + yl/2, 4bz + k): rows yl, yl+1 of planes 2k, 2k+1
+    const int R = N / 2;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const u32 f = pairW[yl][k][w] & pairW[yl + 1][k][w];
+      ifl[((size_t)(4 * bz + k) * R + (4 * by + (yl >> 1))) * W + w] = (u16)gather2(f & (f >> 1));
+    }
+  }
 }
 __global__ void __launch_bounds__(512) k_brick_summary(const u32* __restrict__ G, int N, int W,
-                                                        uint8_t* __restrict__ out, u32* clear) {
+                                                        uint8_t* __restrict__ out, u32* clear,
+                                                        u16* __restrict__ ifl = nullptr) {
   if (clear && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *clear = 0u;
-  brick_summary_body(G, N, W, out, blockIdx.x, blockIdx.y);
+  brick_summary_body(G, N, W, out, blockIdx.x, blockIdx.y, ifl);
 }
 
 // Outer contour K of D (Grid.cu:1230-1327) and its brick summary: voxels
@@ -178,6 +195,11 @@ __device__ __forceinline__ void contour_K_rows(const MorphArgs& a, int wx, int y
     const u32 zsib = (z & 1) ? zm : zp;
     const u32 diag = row_word(D, N, W, y ^ 1, z ^ 1, wx);
     const u32 f = c & ysib & zsib & diag;
+    if (!(y & 1) && !(z & 1)) {
+      // the cells with a voxel of D (read by the coarse erosion)
+      const u32 o = c | ysib | zsib | diag;
+      a.DAL[((size_t)(z >> 1) * a.R + (y >> 1)) * W + wx] = (u16)gather2(o | (o >> 1));
+    }
     const u32 cfull = expand2(gather2(f & (f >> 1)));
     const int cz = z >> 1;
     const u32 inner = (yin && cz >= 1 && cz <= a.R - 2) ? xin : 0u;
@@ -381,7 +403,7 @@ __global__ void __launch_bounds__(512) k_brick_summary_b(const MorphArgs* __rest
   const int j = batch_job(pre, n, blockIdx.x);
   const MorphArgs& a = tab[j];
   const int NB = a.N / 8, l = (int)(blockIdx.x - pre[j]);
-  brick_summary_body(a.I, a.N, a.W, const_cast<uint8_t*>(a.brickI), l % NB, l / NB);
+  brick_summary_body(a.I, a.N, a.W, const_cast<uint8_t*>(a.brickI), l % NB, l / NB, a.IFL);
 }
 __global__ void __launch_bounds__(512) k_brick_summary_gb(const BrickJob* __restrict__ jobs,
                                                            const u32* __restrict__ pre, int n) {
@@ -452,6 +474,8 @@ MorphArgs base_args(const MorphDims& d, const MorphBuffers& b, const RadiusField
   a.TG = b.TG;
   a.CL = b.CL;
   a.UG = b.UG;
+  a.IFL = b.IFL;
+  a.DAL = b.DAL;
   a.brickI = b.brickI;
   a.brickK = b.brickK;
   a.queue = b.queue;
@@ -619,7 +643,7 @@ hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const Radius
   if (ev) GM_CHECK(hipEventRecord(ev[0], s));
   if (perValue && b.rpBuild) GM_CHECK(launch_radius_planes(rf, d.N, b.rp, b.rvset, s));
   hipLaunchKernelGGL(k_brick_summary, dim3(d.N / 8, d.N / 8), dim3(8 * d.W), 0, s, b.I, d.N, d.W, b.brickI,
-                     nullptr);
+                     nullptr, b.IFL);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
   const bool fixedOk = !var && se == 0;

@@ -489,6 +489,8 @@ struct MorphArgs {
   const u16* TG;
   const u16* CL;
   const u16* UG;
+  u16* IFL;  // cells full in I (written by the brick summary of I)
+  u16* DAL;  // cells with a voxel of D (written by the contour-K kernel)
   const uint8_t* brickI;  // per brick (1 word x 8 rows x 8 planes): bit0 any, bit1 all
   const uint8_t* brickK;
   u16* outFlags;
@@ -882,19 +884,13 @@ struct DilCoarseFin {  // DF(c) = I(c) full | (c < dR & F(2c))
     u32 full;  // uniform: I-full bits of the 4 rows; per value: DF so far
     u32 m;     // per value: output voxels of value vcur
   };
-  __device__ u32 ifull(int z) const {
-    const u32* I = a->I;
-    const u32 f = ld_row(I, a->N, a->W, y, z, w) & ld_row(I, a->N, a->W, y + 1, z, w) &
-                  ld_row(I, a->N, a->W, y, z + 1, w) & ld_row(I, a->N, a->W, y + 1, z + 1, w);
-    return gather2(f & (f >> 1));
-  }
   __device__ size_t at(int z) const { return ((size_t)(z >> 1) * a->R + (y >> 1)) * a->W + w; }
   __device__ void load(int z, In& in) const {
     if (a->rvar) {
       in.full = ld_flag(a->outFlags, a->R, a->W, y >> 1, z >> 1, w);
       in.m = radius_eq(a->rp, a->N, a->W, y, z, w, a->vcur);
     } else {
-      in.full = ifull(z);
+      in.full = ld_flag(a->IFL, a->R, a->W, y >> 1, z >> 1, w);
       in.m = ~0u;
     }
   }
@@ -933,23 +929,20 @@ struct EroCoarseFin {  // CL(c) = c < dR & D(c) != 0 & I(c) != 0xff & Ec(2c+1)
   int y;  // odd row 2cy+1
   u32 xin;
   struct In {
-    u32 dany, full, old;
+    u32 dany, full, old;  // cells: with a voxel of D, full in I; CL so far
   };
   __device__ size_t at(int z) const { return ((size_t)(z >> 1) * a->R + (y >> 1)) * a->W + w; }
   __device__ void load(int z, In& in) const {
-    const int y0 = y - 1, z0 = z - 1, N = a->N, W = a->W;
-    in.dany = ld_row(a->D, N, W, y0, z0, w) | ld_row(a->D, N, W, y0 + 1, z0, w) | ld_row(a->D, N, W, y0, z0 + 1, w) |
-              ld_row(a->D, N, W, y0 + 1, z0 + 1, w);
-    in.full = ld_row(a->I, N, W, y0, z0, w) & ld_row(a->I, N, W, y0 + 1, z0, w) & ld_row(a->I, N, W, y0, z0 + 1, w) &
-              ld_row(a->I, N, W, y0 + 1, z0 + 1, w);
-    in.old = a->rvar ? ld_flag(a->outFlags, a->R, a->W, y >> 1, z >> 1, w) : 0u;
+    const int cy = y >> 1, cz = z >> 1;
+    in.dany = ld_flag(a->DAL, a->R, a->W, cy, cz, w);
+    in.full = ld_flag(a->IFL, a->R, a->W, cy, cz, w);
+    in.old = a->rvar ? ld_flag(a->outFlags, a->R, a->W, cy, cz, w) : 0u;
   }
   __device__ void store(int z, u32 e, const In& in) const {
     if (y >= a->N) return;
     const int cy = y >> 1, cz = z >> 1;
     const u32 inr = (cy < a->dR1 && cz < a->dR2) ? xin : 0u;
-    a->outFlags[at(z)] = (u16)(in.old | (gather2(e >> 1) & gather2(in.dany | (in.dany >> 1)) &
-                                         ~gather2(in.full & (in.full >> 1)) & inr));
+    a->outFlags[at(z)] = (u16)(in.old | (gather2(e >> 1) & in.dany & ~in.full & inr));
   }
   __device__ void skipped(int z) const {  // no K in reach: E = 0
     if (y >= a->N) return;

@@ -905,21 +905,24 @@ struct DilCoarseFin {  // DF(c) = I(c) full | (c < dR & F(2c))
     a->outFlags[at(z)] = skipValue;
   }
 };
+// The fine epilogues change only the bits they set or clear, with a device
+// atomic on the output word (each word belongs to one tile), so D and C are
+// not read back.
 struct DilFineFin {  // D |= TG & F (& value mask); D holds TG ? I : (DF ? 1 : I)
   const MorphArgs* a;
   int w;
   int y;
   struct In {
-    u32 tg, d, m;
+    u32 tg, m;
   };
   __device__ void load(int z, In& in) const {
     in.tg = ld_flag(a->TG, a->R, a->W, y >> 1, z >> 1, w);
-    in.d = ld_row(a->outRows, a->N, a->W, y, z, w);
     in.m = a->rvar ? radius_eq(a->rp, a->N, a->W, y, z, w, a->vcur) : ~0u;
   }
   __device__ void store(int z, u32 f, const In& in) const {
     if (y >= a->N) return;
-    a->outRows[((size_t)z * a->N + y) * a->W + w] = in.d | (expand2(in.tg) & f & in.m);
+    const u32 v = expand2(in.tg) & f & in.m;
+    if (v) atomicOr(&a->outRows[((size_t)z * a->N + y) * a->W + w], v);
   }
   __device__ void skipped(int) const {}
 };
@@ -954,15 +957,13 @@ struct EroFineFin {  // C &= ~(UG & Ef); C holds C' = D & ~CL, all ones on UG ce
   int w;
   int y;
   struct In {
-    u32 ug, c;
+    u32 ug;
   };
-  __device__ void load(int z, In& in) const {
-    in.ug = ld_flag(a->UG, a->R, a->W, y >> 1, z >> 1, w);
-    in.c = ld_row(a->outRows, a->N, a->W, y, z, w);
-  }
+  __device__ void load(int z, In& in) const { in.ug = ld_flag(a->UG, a->R, a->W, y >> 1, z >> 1, w); }
   __device__ void store(int z, u32 e, const In& in) const {
     if (y >= a->N) return;
-    a->outRows[((size_t)z * a->N + y) * a->W + w] = in.c & ~(expand2(in.ug) & e);
+    const u32 v = expand2(in.ug) & e;
+    if (v) atomicAnd(&a->outRows[((size_t)z * a->N + y) * a->W + w], ~v);
   }
   __device__ void skipped(int) const {}
 };

@@ -418,12 +418,23 @@ __global__ void __launch_bounds__(256) k_mc_blocks(const uint8_t* __restrict__ b
   mc_blocks_body(brick, R, nblocks, blockV, blockT, list, posOf, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// Count kernels: threads per listed block and consecutive cells per thread
+// (128 threads with 4 cells each measured slower).
+constexpr int kCountThreads = 256;
+constexpr int kCountCells = 512 / kCountThreads;
+
+// Triangle counts per case, the only table the count kernels read.
+__device__ __forceinline__ void load_ntri(uint8_t* ntri) {
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) ntri[i] = (uint8_t)g_mc.ntri[i];
+}
+
 // Vertex and triangle counts of listed block `it`, the block-local vertex
 // base of each of its cells with vertices, and the cube cases of its cells
-// (for the emit kernel).
+// (for the emit kernel). kCountThreads threads, kCountCells consecutive
+// Morton cells each.
 __device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R, u32 it, int blk, u32* blockV,
                                                u32* blockT, u32* cellL, unsigned long long* cellCase, Box& bx,
-                                               u32* red, const McTab& tab) {
+                                               u32* red, const uint8_t* ntri) {
   int c0x, c0y, c0z;
   block_outside(blk, R, c0x, c0y, c0z);
   // cells of the block need fine voxels [16b, 16b+17) per axis
@@ -438,35 +449,41 @@ __device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R,
   const bool anyOne = __syncthreads_or(o != 0u) != 0;
   const bool anyZero = __syncthreads_or(a != 0x1ffffu) != 0;
   const bool nonuni = anyOne && anyZero;
-  u32 nvh[2] = {0u, 0u}, nt = 0;
-  unsigned long long w[2] = {0ull, 0ull};
+  u32 nvh[kCountCells] = {}, nv = 0, nt = 0;
+  unsigned long long w[kCountCells] = {};
   if (nonuni) {
-    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+    for (int h = 0; h < kCountCells; ++h) {
       int cx, cy, cz;
-      block_cell(blk, 2 * threadIdx.x + h, cx, cy, cz);
+      block_cell(blk, kCountCells * threadIdx.x + h, cx, cy, cz);
       if (!cell_in_m_range(cx, cy, cz, R)) continue;
       w[h] = cell_cases_box(bx, cx, cy, cz);
       nvh[h] = own_prefix8(w[h], 8);
+      nv += nvh[h];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) nt += tab.ntri[(w[h] >> (8 * j)) & 0xffu];
+      for (int j = 0; j < 8; ++j) nt += ntri[(w[h] >> (8 * j)) & 0xffu];
     }
   }
   {
-    unsigned long long* dst = cellCase + (size_t)it * 512 + 2 * threadIdx.x;
-    dst[0] = w[0];
-    dst[1] = w[1];
+    unsigned long long* dst = cellCase + (size_t)it * 512 + kCountCells * threadIdx.x;
+#pragma unroll
+    for (int h = 0; h < kCountCells; ++h) dst[h] = w[h];
   }
   u32 tv, tt;
-  const u32 off = block_exscan(nvh[0] + nvh[1], red, &tv);
+  u32 off = block_exscan(nv, red, &tv);
   block_exscan(nt, red, &tt);
   // block-local vertex base of the cells with vertices (cells are in Morton
-  // order, two per thread), read for the edges owned by another block's cube
-  for (int h = 0; h < 2; ++h)
+  // order, consecutive per thread), read for the edges owned by another
+  // block's cube
+#pragma unroll
+  for (int h = 0; h < kCountCells; ++h) {
     if (nvh[h]) {
       int cx, cy, cz;
-      block_cell(blk, 2 * threadIdx.x + h, cx, cy, cz);
-      cellL[(size_t)cx + (size_t)R * ((size_t)cy + (size_t)R * cz)] = off + (h ? nvh[0] : 0u);
+      block_cell(blk, kCountCells * threadIdx.x + h, cx, cy, cz);
+      cellL[(size_t)cx + (size_t)R * ((size_t)cy + (size_t)R * cz)] = off;
     }
+    off += nvh[h];
+  }
   if (threadIdx.x == 0) {
     blockV[blk] = tv;
     blockT[blk] = tt;
@@ -475,20 +492,20 @@ __device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R,
 
 // Vertex and triangle counts of the active blocks; the workgroups walk the
 // list (each ends when the list does).
-__global__ void __launch_bounds__(kMcThreads) k_mc_count(const u32* __restrict__ C, int R,
+__global__ void __launch_bounds__(kCountThreads) k_mc_count(const u32* __restrict__ C, int R,
                                                           const u32* __restrict__ list, u32* blockV,
                                                           u32* blockT, u32* cellL, unsigned long long* cellCase,
                                                           u32* late) {
   __shared__ Box bx;
   __shared__ u32 red[16];
-  __shared__ McTab tab;
+  __shared__ uint8_t ntri[256];
   // the late-triangle count starts from zero in the emit kernel that follows
   if (blockIdx.x == 0 && threadIdx.x == 0) late[0] = 0u;
   const u32 n = list[0];
   if (blockIdx.x >= n) return;
-  load_mc_tab(tab);
+  load_ntri(ntri);
   for (u32 it = blockIdx.x; it < n; it += gridDim.x) {
-    mc_count_block(C, R, it, (int)list[1 + it], blockV, blockT, cellL, cellCase, bx, red, tab);
+    mc_count_block(C, R, it, (int)list[1 + it], blockV, blockT, cellL, cellCase, bx, red, ntri);
     __syncthreads();  // the box is reloaded by the next iteration
   }
 }
@@ -1057,18 +1074,18 @@ __global__ void k_list_prefix(const MCJob* __restrict__ jobs, int n, u32* lpre) 
   }
   lpre[n] = acc;
 }
-__global__ void __launch_bounds__(kMcThreads) k_mc_count_b(const MCJob* __restrict__ jobs, const u32* __restrict__ lpre,
+__global__ void __launch_bounds__(kCountThreads) k_mc_count_b(const MCJob* __restrict__ jobs, const u32* __restrict__ lpre,
                                                             int n) {
   __shared__ Box bx;
   __shared__ u32 red[16];
-  __shared__ McTab tab;
+  __shared__ uint8_t ntri[256];
   if (blockIdx.x >= lpre[n]) return;
-  load_mc_tab(tab);
+  load_ntri(ntri);
   for (u32 u = blockIdx.x; u < lpre[n]; u += gridDim.x) {
     const int j = batch_job(lpre, n, u);
     const MCJob& J = jobs[j];
     mc_count_block(J.C, J.R, u - lpre[j], (int)J.m.list[1 + u - lpre[j]], J.m.blockV, J.m.blockT, J.m.cellV,
-                   J.m.cellCase, bx, red, tab);
+                   J.m.cellCase, bx, red, ntri);
     __syncthreads();
   }
 }
@@ -1174,8 +1191,8 @@ hipError_t launch_mc(const u32* C, int R, const float* bmin3, float cell, const 
   hipLaunchKernelGGL(k_mc_blocks, dim3((m.nblocks + 255) / 256), dim3(256), 0, s, m.brick, R, m.nblocks, m.blockV,
                      m.blockT, m.list, m.posOf, m.order);
   GM_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_mc_count, dim3(mc_list_grid(m.nblocks, 2048)), dim3(kMcThreads), 0, s, C, R, m.list, m.blockV,
-                     m.blockT, m.cellV, m.cellCase, m.late);
+  hipLaunchKernelGGL(k_mc_count, dim3(mc_list_grid(m.nblocks, 2048)), dim3(kCountThreads), 0, s, C, R, m.list,
+                     m.blockV, m.blockT, m.cellV, m.cellCase, m.late);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
   int nparts = (int)mc_scan_parts(m.nblocks);
@@ -1251,7 +1268,7 @@ hipError_t launch_mc_batched(const BatchMCJob* in, int n, void* hBlob, void* dBl
   hipLaunchKernelGGL(k_list_prefix, dim3(1), dim3(64), 0, s, dj, n, lpre);
   GM_CHECK(hipGetLastError());
   const unsigned listGrid = (unsigned)std::max<long long>(1, std::min<long long>(blocksAll, 16384));
-  hipLaunchKernelGGL(k_mc_count_b, dim3(std::min(listGrid, 4096u)), dim3(kMcThreads), 0, s, dj, lpre, n);
+  hipLaunchKernelGGL(k_mc_count_b, dim3(std::min(listGrid, 4096u)), dim3(kCountThreads), 0, s, dj, lpre, n);
   GM_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_scan_reduce_b, dim3(preC[n]), dim3(256), 0, s, dj, dPreC, n);
   GM_CHECK(hipGetLastError());

@@ -650,13 +650,30 @@ __device__ __forceinline__ int rank_of(int v, int e) {  // index of owned edge e
 
 // Area-weighted normal sums (GMorpho.cu:6498-6524) gathered per vertex: the
 // triangles that use the vertex's edge are those of the <= 4 cubes around it.
-__constant__ int8_t g_share[3][4][4] = {
+struct ShareCube {
+  int dx, dy, dz, le;
+};
+constexpr ShareCube kShare[3][4] = {
     // owned edge 0 (x): cubes f, f-y, f-z, f-y-z with local edges 0, 2, 4, 6
     {{0, 0, 0, 0}, {0, -1, 0, 2}, {0, 0, -1, 4}, {0, -1, -1, 6}},
     // owned edge 3 (y): cubes f, f-x, f-z, f-x-z with local edges 3, 1, 7, 5
     {{0, 0, 0, 3}, {-1, 0, 0, 1}, {0, 0, -1, 7}, {-1, 0, -1, 5}},
     // owned edge 8 (z): cubes f, f-x, f-y, f-x-y with local edges 8, 9, 11, 10
     {{0, 0, 0, 8}, {-1, 0, 0, 9}, {0, -1, 0, 11}, {-1, -1, 0, 10}}};
+// The 4 cubes of owned edge q packed one byte each: bit0/1/2 = offset -1 in
+// x/y/z, bits 3..6 = the edge's index in that cube. Selected by value (q is
+// per lane), so no table is read from memory.
+constexpr u32 share_pack(int q) {
+  u32 v = 0;
+  for (int s = 0; s < 4; ++s) {
+    const ShareCube& c = kShare[q][s];
+    v |= (u32)((c.dx < 0 ? 1 : 0) | (c.dy < 0 ? 2 : 0) | (c.dz < 0 ? 4 : 0) | (c.le << 3)) << (8 * s);
+  }
+  return v;
+}
+__device__ __forceinline__ u32 share_of(int q) {
+  return q == 0 ? share_pack(0) : (q == 1 ? share_pack(1) : share_pack(2));
+}
 
 // edge_pos with the two voxel values given (v0 at (fx,fy,fz), v1 one step
 // along the edge's axis): the same float operations.
@@ -703,10 +720,12 @@ __device__ __forceinline__ u32 prefix_ntri(const BlockLds& S, int l, int j) { re
 __device__ void normal_sum(const BlockLds& S, const Geo& geo, int R, int c0x, int c0y, int c0z, bool useFn, int fx,
                            int fy, int fz, int q, int v0, const float* pv, float* n) {
   n[0] = n[1] = n[2] = 0.f;
+  const u32 sp = share_of(q);
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
-    int gx = fx + g_share[q][s][0], gy = fy + g_share[q][s][1], gz = fz + g_share[q][s][2];
-    int le = g_share[q][s][3];
+    const u32 e = (sp >> (8 * s)) & 0xffu;
+    const int gx = fx - (int)(e & 1u), gy = fy - (int)((e >> 1) & 1u), gz = fz - (int)((e >> 2) & 1u);
+    const int le = (int)(e >> 3);
     const int cx = gx >> 1, cy = gy >> 1, cz = gz >> 1;
     if (s > 0 && !cell_in_m_range(cx, cy, cz, R)) continue;
     const int lx = cx - c0x, ly = cy - c0y, lz = cz - c0z;

@@ -1055,39 +1055,58 @@ __device__ bool select_tile(const MorphArgs& a, int t, const TileGeo& g, u32& va
 // to I-full beforehand).
 template <int PASS>
 __device__ void select_tiles(const MorphArgs& a) {
+  // queue reservations per workgroup: its tiles (consecutive, mostly of one
+  // slab) take one atomic per queue list they use, so the queue counters see
+  // few atomics. Lists: uniform radius [x], per-voxel radii [8v + x].
+  __shared__ u32 wgCount[256], wgBase[256];
   if (blockIdx.x == 0) {
     // the counter blocks of the next pass (the previous pass has finished)
     if (threadIdx.x < 16) a.queue[16 * (1 - a.qsel) + threadIdx.x] = 0u;
     for (int i = threadIdx.x; i < 256; i += blockDim.x) a.queue[kVarCounters + 256 * (1 - a.qsel) + i] = 0u;
   }
-  u32* qc = a.queue + 16 * a.qsel;
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) wgCount[i] = 0u;
+  __syncthreads();
   const int t = blockIdx.x * (int)(blockDim.x >> 6) + wave_id();
-  if (t >= tile_count(a)) return;
-  const TileGeo g = pass_tile<PASS>(a, t);
   const int lane = lane_id();
-  u32 vals;
-  int blo, bhi;  // lane v: brick planes holding value v
-  int slo, shi;  // brick planes holding source voxels in reach (uniform radius)
-  if (!select_tile<PASS>(a, t, g, vals, blo, bhi, slo, shi)) return;
-  const int x = slab_queue(t / a.W);
-  if (a.rvar) {
-    // lane v queues the tile for radius value v, with the planes that can
-    // contribute: the output planes of value v (dilation, outputs masked by
-    // value) or the input planes of K of value v (erosion, source masked)
-    if (lane < 32 && ((vals >> lane) & 1u)) {
-      int rlo = 8 * blo, rhi = 8 * bhi + 7;
-      if constexpr (PASS < 2) {
-        rlo = max(rlo, g.zfirst);
-        rhi = min(rhi, g.zlast);
+  int list = -1;  // this lane's queue list, or none
+  u32 slot = 0u, entry = 0u;
+  if (t < tile_count(a)) {
+    const TileGeo g = pass_tile<PASS>(a, t);
+    u32 vals;
+    int blo, bhi;  // lane v: brick planes holding value v
+    int slo, shi;  // brick planes holding source voxels in reach (uniform radius)
+    if (select_tile<PASS>(a, t, g, vals, blo, bhi, slo, shi)) {
+      const int x = slab_queue(t / a.W);
+      if (a.rvar) {
+        // lane v queues the tile for radius value v, with the planes that can
+        // contribute: the output planes of value v (dilation, outputs masked
+        // by value) or the input planes of K of value v (erosion, source
+        // masked)
+        if (lane < 32 && ((vals >> lane) & 1u)) {
+          int rlo = 8 * blo, rhi = 8 * bhi + 7;
+          if constexpr (PASS < 2) {
+            rlo = max(rlo, g.zfirst);
+            rhi = min(rhi, g.zlast);
+          }
+          list = lane * 8 + x;
+          entry = entry_pack(t, g.tz0, rlo, rhi);
+        }
+      } else if (lane == 0) {
+        // the stream reads only the input planes holding source voxels
+        list = x;
+        entry = entry_pack(t, g.tz0, 8 * slo, 8 * shi + 7);
       }
-      const int q = lane * 8 + x;
-      const u32 k = atomicAdd(&a.queue[kVarCounters + 256 * a.qsel + q], 1u);
-      a.queue[a.vqBase + (size_t)q * a.vqCap + k] = entry_pack(t, g.tz0, rlo, rhi);
     }
-  } else if (lane == 0) {
-    // the stream reads only the input planes holding source voxels
-    const u32 k = atomicAdd(&qc[x], 1u);
-    a.queue[kQueueList + x * a.queueCap + k] = entry_pack(t, g.tz0, 8 * slo, 8 * shi + 7);
+  }
+  if (list >= 0) slot = atomicAdd(&wgCount[list], 1u);
+  __syncthreads();
+  u32* qc = a.rvar ? a.queue + kVarCounters + 256 * a.qsel : a.queue + 16 * a.qsel;
+  for (int i = threadIdx.x; i < 256; i += blockDim.x)
+    if (wgCount[i]) wgBase[i] = atomicAdd(&qc[i], wgCount[i]);
+  __syncthreads();
+  if (list >= 0) {
+    if (a.rvar) a.queue[a.vqBase + (size_t)list * a.vqCap + wgBase[list] + slot] = entry;
+    else a.queue[kQueueList + list * a.queueCap + wgBase[list] + slot] = entry;
   }
 }
 

@@ -161,10 +161,22 @@ __device__ __forceinline__ void contour_K_body(const MorphArgs& a, uint8_t* __re
   __syncthreads();
   if (threadIdx.x < 8u * (unsigned)W) contour_K_rows(a, wx, yl, by, bz, anyW);
   __syncthreads();
-  if (threadIdx.x < W) {
+  if (threadIdx.x < 64) {  // W <= 64: the first wave holds every word of the row
+    const bool in = (int)threadIdx.x < W;
+    const bool has = in && anyW[threadIdx.x] != 0u;
     const size_t b = ((size_t)bz * NB + by) * W + threadIdx.x;
-    brickK[b] = (uint8_t)anyW[threadIdx.x];
-    if (kbl && anyW[threadIdx.x]) kbl[1 + atomicAdd(kbl, 1u)] = (u32)b;
+    if (in) brickK[b] = (uint8_t)(has ? 1u : 0u);
+    if (kbl) {
+      // one reservation for the row's bricks holding K
+      const unsigned long long m = __ballot(has);
+      if (m) {
+        const int lane = (int)threadIdx.x;
+        u32 base = 0u;
+        if (lane == 0) base = atomicAdd(kbl, (u32)__popcll(m));
+        base = __shfl(base, 0, 64);
+        if (has) kbl[1 + base + (u32)__popcll(m & ((1ull << lane) - 1ull))] = (u32)b;
+      }
+    }
   }
 }
 __global__ void __launch_bounds__(512) k_contour_K(MorphArgs a, uint8_t* __restrict__ brickK, u32* kbl) {

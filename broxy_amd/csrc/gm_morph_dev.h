@@ -1126,22 +1126,34 @@ constexpr int kBatchJobBits = 12;
 template <int PASS>
 __device__ void select_tiles_batched(const MorphArgs* __restrict__ tab, const u32* __restrict__ pre, int n,
                                      const int* __restrict__ rad, u32* queue, int qsel, int listBase, int cap) {
+  // one reservation per workgroup and list, as in select_tiles
+  __shared__ u32 wgCount[256], wgBase[256];
   if (blockIdx.x == 0)
     for (int i = threadIdx.x; i < 256; i += blockDim.x) queue[kVarCounters + 256 * (1 - qsel) + i] = 0u;
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) wgCount[i] = 0u;
+  __syncthreads();
   const u32 u = blockIdx.x * (blockDim.x >> 6) + (u32)wave_id();
-  if (u >= pre[n]) return;
-  const int j = batch_job(pre, n, u);
-  const MorphArgs& a = tab[j];
-  const int t = (int)(u - pre[j]);
-  const TileGeo g = pass_tile<PASS>(a, t);
-  u32 vals;
-  int blo, bhi, slo, shi;
-  if (!select_tile<PASS>(a, t, g, vals, blo, bhi, slo, shi)) return;
-  if (lane_id() == 0) {
-    const int q = rad[j] * 8 + slab_queue(t / a.W);
-    const u32 k = atomicAdd(&queue[kVarCounters + 256 * qsel + q], 1u);
-    queue[listBase + (size_t)q * cap + k] = ((u32)j << kBatchTileBits) | (u32)t;
+  int list = -1;
+  u32 slot = 0u, entry = 0u;
+  if (u < pre[n]) {
+    const int j = batch_job(pre, n, u);
+    const MorphArgs& a = tab[j];
+    const int t = (int)(u - pre[j]);
+    const TileGeo g = pass_tile<PASS>(a, t);
+    u32 vals;
+    int blo, bhi, slo, shi;
+    if (select_tile<PASS>(a, t, g, vals, blo, bhi, slo, shi) && lane_id() == 0) {
+      list = rad[j] * 8 + slab_queue(t / a.W);
+      entry = ((u32)j << kBatchTileBits) | (u32)t;
+    }
   }
+  if (list >= 0) slot = atomicAdd(&wgCount[list], 1u);
+  __syncthreads();
+  u32* qc = queue + kVarCounters + 256 * qsel;
+  for (int i = threadIdx.x; i < 256; i += blockDim.x)
+    if (wgCount[i]) wgBase[i] = atomicAdd(&qc[i], wgCount[i]);
+  __syncthreads();
+  if (list >= 0) queue[listBase + (size_t)list * cap + wgBase[list] + slot] = entry;
 }
 
 // The predicate of one active tile and its epilogue. RHO > 0: uniform radius

@@ -30,7 +30,7 @@ KERNEL_OF_STAGE = {"brick_summary": "k_brick_summary", "dilate_coarse": "k_pass_
                    "tight_contour": "k_tight", "dilate_fine": "k_pass_ct<1,rho>", "contour_K": "k_contour_K",
                    "erode_coarse": "k_pass_ct<2,rho>", "base_contour": "k_base_contour",
                    "erode_fine": "k_pass_ct<3,rho>", "mc_count": "k_mc_count",
-                   "mc_scan": "k_scan_reduce+k_scan_down", "mc_emit": "k_mc_emit",
+                   "mc_scan": "k_scan_down", "mc_emit": "k_mc_emit",
                    "mc_fixup_normalize": "k_mc_fixup+k_mc_normalize",
                    "dilate_coarse_stream": "k_pass_ct<0,rho>", "dilate_fine_stream": "k_pass_ct<1,rho>",
                    "erode_coarse_stream": "k_pass_ct<2,rho>", "erode_fine_stream": "k_pass_ct<3,rho>"}

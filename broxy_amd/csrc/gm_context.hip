@@ -210,7 +210,7 @@ int alloc_grid(gm_ctx* c, int R) {
   HIPCALL(c, hipMalloc(&c->mc.posOf, (size_t)c->mc.nblocks * 4));
   HIPCALL(c, hipMalloc(&c->mc.order, ((size_t)4 * c->mc.nblocks + 4) * 4));
   HIPCALL(c, hipMalloc(&c->mc.totals, 16));
-  HIPCALL(c, hipMalloc(&c->mc.scanPart, mc_scan_parts(c->mc.nblocks) * 8));
+  HIPCALL(c, hipMalloc(&c->mc.scanPart, mc_scan_part_bytes(c->mc.nblocks)));
   HIPCALL(c, hipMalloc(&c->mc.brick, brickBytes));
   HIPCALL(c, hipMalloc(&c->mc.list, ((size_t)c->mc.nblocks + 1) * 4));
   c->mc.capV = c->mc.capT = c->mc.capLate = 0;

@@ -248,7 +248,7 @@ struct MCBuffers {
   u32* T;
   u32* late;     // [0] count, then (slot, a, b, c) quadruples
   u32* totals;   // [0] nV [1] nT [2] overflow [3] border flag
-  u32* scanPart; // 2 u32 per 1024 blocks
+  u32* scanPart; // per 1024 blocks: vertex and triangle totals (mc_scan_part_bytes)
   uint8_t* brick; // brick summary of the closing grid (1 word x 8 rows x 8 planes)
   u32* list;      // [0] number of active blocks, then their ids
   u32 capV, capT, capLate;
@@ -258,6 +258,7 @@ struct MCBuffers {
 hipError_t launch_mc(const u32* C, int R, const float* bmin3, float cell, const MCBuffers& m, hipStream_t s,
                      hipEvent_t* ev);
 size_t mc_scan_parts(int nblocks);
+size_t mc_scan_part_bytes(int nblocks);  // device bytes of MCBuffers::scanPart
 hipError_t launch_mc_emit(const u32* C, int R, const float* bmin3, float cell, const MCBuffers& m,
                           hipStream_t s, hipEvent_t* ev);
 // Marching cubes of many closed grids, all on stream s: the same kernels per

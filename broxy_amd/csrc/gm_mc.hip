@@ -14,7 +14,8 @@
 //   mc_blocks  list of the blocks that can hold a mixed cube
 //   mc_count   per block vertex/triangle totals, and the block-local vertex
 //              base of every cell with vertices
-//   scan       exclusive scan of the block totals (chunk reduce, down-sweep)
+//   scan       exclusive scan of the block totals (chunk totals summed by
+//              mc_count, then one down-sweep kernel)
 //   mc_emit    cube cases in LDS, block scan over cells, then one thread per
 //              output slot: vertex positions with area-weighted normal sums
 //              gathered from the <= 4 cubes around each vertex, then
@@ -355,6 +356,9 @@ __device__ __forceinline__ bool block_outside(int blk, int R, int& c0x, int& c0y
 // blocks start early. order[0..kOrderClasses) count the members of each
 // class; class c lists its members' list positions at
 // order[kOrderClasses + c * nblocks ...]. Built by the scan's down-sweep.
+// Device scan of the block totals: chunks of kChunk blocks.
+constexpr int kChunk = 1024;
+constexpr int kPartStride = 32;  // words per chunk total pair in the single-grid scan (one cache line)
 constexpr int kOrderClasses = 4;
 __device__ __forceinline__ int order_class(u32 work) {
   return work >= 2048u ? 0 : (work >= 1024u ? 1 : (work >= 512u ? 2 : 3));
@@ -412,9 +416,16 @@ __device__ __forceinline__ void mc_blocks_body(const uint8_t* __restrict__ brick
   }
 }
 __global__ void __launch_bounds__(256) k_mc_blocks(const uint8_t* __restrict__ brick, int R, int nblocks,
-                                                   u32* blockV, u32* blockT, u32* list, u32* posOf, u32* order) {
-  // the emit-order class counts start from zero in the scan that follows
+                                                   u32* blockV, u32* blockT, u32* list, u32* posOf, u32* order,
+                                                   u32* part) {
+  // the emit-order class counts start from zero in the scan that follows,
+  // and the chunk totals in the count kernel (kPartStride words per chunk)
   if (blockIdx.x == 0 && threadIdx.x < kOrderClasses) order[threadIdx.x] = 0u;
+  const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (gid < (nblocks + kChunk - 1) / kChunk) {
+    part[kPartStride * gid] = 0u;
+    part[kPartStride * gid + 1] = 0u;
+  }
   mc_blocks_body(brick, R, nblocks, blockV, blockT, list, posOf, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
@@ -431,10 +442,11 @@ __device__ __forceinline__ void load_ntri(uint8_t* ntri) {
 // Vertex and triangle counts of listed block `it`, the block-local vertex
 // base of each of its cells with vertices, and the cube cases of its cells
 // (for the emit kernel). kCountThreads threads, kCountCells consecutive
-// Morton cells each.
+// Morton cells each. part (optional): the block's totals are added to its
+// scan chunk's totals (kPartStride words per chunk, one cache line each).
 __device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R, u32 it, int blk, u32* blockV,
                                                u32* blockT, u32* cellL, unsigned long long* cellCase, Box& bx,
-                                               u32* red, const uint8_t* ntri) {
+                                               u32* red, const uint8_t* ntri, u32* part = nullptr) {
   int c0x, c0y, c0z;
   block_outside(blk, R, c0x, c0y, c0z);
   // cells of the block need fine voxels [16b, 16b+17) per axis
@@ -487,6 +499,11 @@ __device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R,
   if (threadIdx.x == 0) {
     blockV[blk] = tv;
     blockT[blk] = tt;
+    if (part && (tv | tt)) {
+      u32* pc = part + kPartStride * (blk / kChunk);
+      atomicAdd(&pc[0], tv);
+      atomicAdd(&pc[1], tt);
+    }
   }
 }
 
@@ -495,7 +512,7 @@ __device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R,
 __global__ void __launch_bounds__(kCountThreads) k_mc_count(const u32* __restrict__ C, int R,
                                                           const u32* __restrict__ list, u32* blockV,
                                                           u32* blockT, u32* cellL, unsigned long long* cellCase,
-                                                          u32* late) {
+                                                          u32* late, u32* part) {
   __shared__ Box bx;
   __shared__ u32 red[16];
   __shared__ uint8_t ntri[256];
@@ -505,13 +522,12 @@ __global__ void __launch_bounds__(kCountThreads) k_mc_count(const u32* __restric
   if (blockIdx.x >= n) return;
   load_ntri(ntri);
   for (u32 it = blockIdx.x; it < n; it += gridDim.x) {
-    mc_count_block(C, R, it, (int)list[1 + it], blockV, blockT, cellL, cellCase, bx, red, ntri);
+    mc_count_block(C, R, it, (int)list[1 + it], blockV, blockT, cellL, cellCase, bx, red, ntri, part);
     __syncthreads();  // the box is reloaded by the next iteration
   }
 }
 
 // ---- device-wide exclusive scan of (blockV, blockT), 1024 entries per chunk
-constexpr int kChunk = 1024;
 
 __device__ __forceinline__ void scan_reduce_body(const u32* __restrict__ a, const u32* __restrict__ b, int n, u32* part,
                                                  int chunk) {
@@ -530,16 +546,12 @@ __device__ __forceinline__ void scan_reduce_body(const u32* __restrict__ a, cons
     part[2 * chunk + 1] = tb;
   }
 }
-__global__ void __launch_bounds__(256) k_scan_reduce(const u32* __restrict__ a, const u32* __restrict__ b, int n,
-                                                     u32* part) {
-  scan_reduce_body(a, b, n, part, blockIdx.x);
-}
 
 // Down-sweep of one chunk: the chunk's offset is the sum of the chunk totals
 // before it (part holds the totals of all chunks); chunk 0 also writes the
 // overall totals and the overflow flag.
-__device__ __forceinline__ void scan_down_body(u32* a, u32* b, int n, const u32* __restrict__ part, int nparts,
-                                               int chunk, u32* totals, u32 capV, u32 capT,
+__device__ __forceinline__ void scan_down_body(u32* a, u32* b, int n, const u32* __restrict__ part, int pstride,
+                                               int nparts, int chunk, u32* totals, u32 capV, u32 capT,
                                                const u32* __restrict__ posOf = nullptr, u32* order = nullptr) {
   __shared__ u32 red[16];
   __shared__ u32 cls[2 * kOrderClasses];
@@ -557,8 +569,8 @@ __device__ __forceinline__ void scan_down_body(u32* a, u32* b, int n, const u32*
   const int lim = chunk == 0 ? nparts : chunk;  // uniform
   u32 pa = 0, pb = 0;
   for (int p = threadIdx.x; p < lim; p += blockDim.x) {
-    pa += part[2 * p];
-    pb += part[2 * p + 1];
+    pa += part[pstride * p];
+    pb += part[pstride * p + 1];
   }
   u32 offA, offB;
   block_exscan(pa, red, &offA);
@@ -609,7 +621,7 @@ __device__ __forceinline__ void scan_down_body(u32* a, u32* b, int n, const u32*
 __global__ void __launch_bounds__(256) k_scan_down(u32* a, u32* b, int n, const u32* __restrict__ part, int nparts,
                                                    u32* totals, u32 capV, u32 capT, const u32* __restrict__ posOf,
                                                    u32* order) {
-  scan_down_body(a, b, n, part, nparts, blockIdx.x, totals, capV, capT, posOf, order);
+  scan_down_body(a, b, n, part, kPartStride, nparts, blockIdx.x, totals, capV, capT, posOf, order);
 }
 
 // Position of the vertex on owned edge e (0: x, 3: y, 8: z) of cube f:
@@ -1118,7 +1130,7 @@ __global__ void __launch_bounds__(256) k_scan_down_b(const MCJob* __restrict__ j
                                                      int n) {
   const int j = batch_job(pre, n, blockIdx.x);
   const MCJob& J = jobs[j];
-  scan_down_body(J.m.blockV, J.m.blockT, J.m.nblocks, J.m.scanPart, (J.m.nblocks + kChunk - 1) / kChunk,
+  scan_down_body(J.m.blockV, J.m.blockT, J.m.nblocks, J.m.scanPart, 2, (J.m.nblocks + kChunk - 1) / kChunk,
                  (int)(blockIdx.x - pre[j]), J.m.totals, J.m.capV, J.m.capT, J.m.posOf, J.m.order);
 }
 __global__ void __launch_bounds__(kMcThreads) k_mc_emit_b(const MCJob* __restrict__ jobs, const u32* __restrict__ lpre,
@@ -1194,6 +1206,7 @@ hipError_t mc_tables_host(u32* ntri, u32* len, unsigned long long* tri) {
 }
 
 size_t mc_scan_parts(int nblocks) { return (size_t)((nblocks + kChunk - 1) / kChunk); }
+size_t mc_scan_part_bytes(int nblocks) { return mc_scan_parts(nblocks) * kPartStride * 4; }
 
 // Workgroups of the kernels that walk the active-block list: enough to fill
 // the chip several times over, never more than there are blocks. The emit
@@ -1208,15 +1221,14 @@ hipError_t launch_mc(const u32* C, int R, const float* bmin3, float cell, const 
   // active Morton blocks from a brick summary of C
   GM_CHECK(launch_brick_summary(C, 2 * R, m.brick, m.list, s));
   hipLaunchKernelGGL(k_mc_blocks, dim3((m.nblocks + 255) / 256), dim3(256), 0, s, m.brick, R, m.nblocks, m.blockV,
-                     m.blockT, m.list, m.posOf, m.order);
+                     m.blockT, m.list, m.posOf, m.order, m.scanPart);
   GM_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_mc_count, dim3(mc_list_grid(m.nblocks, 2048)), dim3(kCountThreads), 0, s, C, R, m.list,
-                     m.blockV, m.blockT, m.cellV, m.cellCase, m.late);
+                     m.blockV, m.blockT, m.cellV, m.cellCase, m.late, m.scanPart);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
+  // the chunk totals were summed by the count kernel
   int nparts = (int)mc_scan_parts(m.nblocks);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nparts), dim3(256), 0, s, m.blockV, m.blockT, m.nblocks, m.scanPart);
-  GM_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_scan_down, dim3(nparts), dim3(256), 0, s, m.blockV, m.blockT, m.nblocks, m.scanPart, nparts,
                      m.totals, m.capV, m.capT, m.posOf, m.order);
   GM_CHECK(hipGetLastError());

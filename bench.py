@@ -55,14 +55,14 @@ def algorithmic_bytes(stage, N, nv, nt):
     f = N ** 3 / 64.0
     stage = stage.replace("_stream", "")
     return {
-        "brick_summary": g,                 # read I
-        "dilate_coarse": g + f,            # read I, write DF
+        "brick_summary": g + f,            # read I; write I-full cell flags
+        "dilate_coarse": g + 2 * f,        # read I, I-full flags; write DF
         "tight_contour": 2 * f + 2 * g,    # read DF, I; write TG, D
-        "dilate_fine": 2 * g + f,          # read I, TG; rewrite D (upper bound: every tile tagged)
-        "contour_K": 2 * g,                # read D, write K
-        "erode_coarse": 3 * g + f,         # read K, D, I; write CL
+        "dilate_fine": 2 * g + f,          # read I, TG; set bits of D (upper bound: every word)
+        "contour_K": 2 * g + f,            # read D; write K, D-any cell flags
+        "erode_coarse": g + 3 * f,         # read K, D-any and I-full flags; write CL
         "base_contour": 2 * g + 2 * f,     # read D, CL; write UG, C
-        "erode_fine": 3 * g + f,           # read K, UG, C; rewrite C (upper bound)
+        "erode_fine": 2 * g + f,           # read K, UG; clear bits of C (upper bound: every word)
         "mc_count": g,                     # read C
         "mc_scan": 16.0 * N ** 3 / 4096,   # block counts and bases (2 u32 per 8^3 cells, read + write)
         "mc_emit": g + 24 * nv + 12 * nt,  # read C; write positions, normal sums, triangles

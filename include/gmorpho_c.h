@@ -118,7 +118,7 @@ int gm_pyramid_get(gm_ctx* ctx, uint64_t* texels);
  * brackets, GMorpho.cu:7796-7815). 16 stages: brick summary (with the
  * radius planes on the first closing after a per-voxel field is set),
  * dilate coarse, tight contour, dilate fine, contour K, erode coarse, base
- * contour, erode fine, mc count, mc scan (2 small kernels), mc vertices and
+ * contour, erode fine, mc count, mc scan (one small kernel), mc vertices and
  * triangles, mc fixup+normalize; then the streaming launches of the four
  * passes alone (dilate coarse, dilate fine, erode coarse, erode fine). A
  * pass stage holds its tile selection and its streaming launches (one per

@@ -551,6 +551,7 @@ hipError_t launch_pass(int pass, const void* generic, MorphArgs a, int* qsel, bo
   a.qcOff = 16 * a.qsel;
   a.qlOff = kQueueList;
   a.qCap = a.queueCap;
+  a.qBack = 1;
   void* args[] = {&a};
   GM_CHECK(hipLaunchKernel(sel[pass], dim3((tiles + 3) / 4), dim3(256), args, 0, s));
   if (evStream) GM_CHECK(hipEventRecord(evStream, s));

@@ -494,16 +494,17 @@ struct MorphArgs {
   const uint8_t* brickI;  // per brick (1 word x 8 rows x 8 planes): bit0 any, bit1 all
   const uint8_t* brickK;
   u16* outFlags;
-  // active-tile queues: two blocks of counters ([0..7] tiles per queue,
-  // [8..15] next tile to hand out) at words 0 and 16, then (from word 32) 8
-  // lists of queueCap tile ids. A launch uses counter block qsel; its tile
+  // active-tile queues: two blocks of counters ([0..7] tiles at the front
+  // of each queue list, [8..15] tiles at its back) at words 0 and 16, then
+  // 8 lists of queueCap tile ids. A launch uses counter block qsel; its tile
   // selection clears the other block for the next launch.
   u32* queue;
   int queueCap;
   int qsel;
   // queue a streaming launch drains: 8 counters at word qcOff, 8 lists of
-  // qCap entries from word qlOff
-  int qcOff, qlOff, qCap;
+  // qCap entries from word qlOff; qBack != 0: 8 more counters at qcOff + 8
+  // count entries stored from the back of each list (taken after the front)
+  int qcOff, qlOff, qCap, qBack;
   // per-value queues: counter blocks of 32 values x 8 queues at
   // kVarCounters + 256*qsel, lists of vqCap entries at vqBase + (8v + x)*vqCap
   int vqBase, vqCap;
@@ -1092,8 +1093,12 @@ __device__ void select_tiles(const MorphArgs& a) {
           entry = entry_pack(t, g.tz0, rlo, rhi);
         }
       } else if (lane == 0) {
-        // the stream reads only the input planes holding source voxels
-        list = x;
+        // the stream reads only the input planes holding source voxels;
+        // tiles with most of their planes in reach go to the front of the
+        // list (they start first, mostly one per CU), the others to the back
+        const int lo = max(8 * slo, g.zfirst - a.maxoff), hi = min(8 * shi + 7, g.zlast + a.maxoff);
+        const int span = g.zlast - g.zfirst + 1 + 2 * a.maxoff;
+        list = (4 * (hi - lo + 1) >= 3 * span) ? x : 8 + x;
         entry = entry_pack(t, g.tz0, 8 * slo, 8 * shi + 7);
       }
     }
@@ -1105,8 +1110,13 @@ __device__ void select_tiles(const MorphArgs& a) {
     if (wgCount[i]) wgBase[i] = atomicAdd(&qc[i], wgCount[i]);
   __syncthreads();
   if (list >= 0) {
-    if (a.rvar) a.queue[a.vqBase + (size_t)list * a.vqCap + wgBase[list] + slot] = entry;
-    else a.queue[kQueueList + list * a.queueCap + wgBase[list] + slot] = entry;
+    if (a.rvar) {
+      a.queue[a.vqBase + (size_t)list * a.vqCap + wgBase[list] + slot] = entry;
+    } else {
+      const int x = list & 7;
+      const u32 pos = list < 8 ? wgBase[list] + slot : (u32)a.queueCap - 1u - (wgBase[list] + slot);
+      a.queue[kQueueList + x * a.queueCap + pos] = entry;
+    }
   }
 }
 
@@ -1195,8 +1205,9 @@ __device__ void run_pass(const MorphArgs& a) {
   const int x = (int)(blockIdx.x & 7u);
   const u32* qc = a.queue + a.qcOff;
   const u32* ql = a.queue + a.qlOff + (size_t)x * a.qCap;
-  for (u32 k = blockIdx.x >> 3; k < qc[x]; k += gridDim.x >> 3) {
-    run_pass_tile<PASS, RHO>(a, ql[k], smem, set4);
+  const u32 front = qc[x], n = front + (a.qBack ? qc[8 + x] : 0u);
+  for (u32 k = blockIdx.x >> 3; k < n; k += gridDim.x >> 3) {
+    run_pass_tile<PASS, RHO>(a, k < front ? ql[k] : ql[a.qCap - 1 - (k - front)], smem, set4);
     __syncthreads();  // LDS is reused by the next tile
   }
 }

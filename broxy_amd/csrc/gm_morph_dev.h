@@ -634,6 +634,43 @@ __device__ bool wave_tile_has_flags(const u16* F, int R, int W, int wx, int y0, 
   return __any(acc != 0u) != 0;
 }
 
+// As wave_tile_has_flags, and the fine planes [olo, ohi] of the cell planes
+// holding a flag (wave-uniform; olo > ohi when none does).
+__device__ bool wave_tile_flag_planes(const u16* F, int R, int W, int wx, int y0, int y1, int z0, int z1, int& olo,
+                                      int& ohi) {
+  const int cy0 = y0 >> 1, cy1 = min((y1 + 1) >> 1, R);
+  const int cz0 = z0 >> 1, cz1 = min((z1 + 1) >> 1, R);
+  const int ny = cy1 - cy0, nz = cz1 - cz0;
+  int clo = 1 << 20, chi = -1;
+  if (ny > 0 && nz > 0) {
+    const int n = ny * nz;
+    for (int i0 = 0; i0 < n; i0 += 256) {
+      u32 v[4];
+      int cz[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = min(i0 + 64 * q + lane_id(), n - 1);
+        cz[q] = cz0 + i / ny;
+        v[q] = ld_flag(F, R, W, cy0 + i % ny, cz[q], wx);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (v[q]) {
+          clo = min(clo, cz[q]);
+          chi = max(chi, cz[q]);
+        }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    clo = min(clo, __shfl_xor(clo, off));
+    chi = max(chi, __shfl_xor(chi, off));
+  }
+  olo = __builtin_amdgcn_readfirstlane(2 * clo);
+  ohi = __builtin_amdgcn_readfirstlane(2 * chi + 1);
+  return chi >= 0;
+}
+
 // Distinct radii of a tile (128-bit set in LDS). Dilation: radius+add of the
 // output voxels; erosion: radius-1 of the K voxels within the halo.
 __device__ void rho_set_dilate(const RadiusField& rf, int add, const TileGeo& g, int N, u32* set4) {
@@ -993,9 +1030,11 @@ __device__ __forceinline__ TileGeo pass_tile(const MorphArgs& a, int t) {
 // coarse pass stores its constant), returns whether the tile is active. With
 // per-voxel radii, vals gets the radius values that reach the tile and lane v
 // the brick planes [blo, bhi] of value v.
+// olo, ohi: the output planes that need the predicate (fine passes: the
+// planes holding contour tags; coarse passes: all).
 template <int PASS>
 __device__ bool select_tile(const MorphArgs& a, int t, const TileGeo& g, u32& vals, int& blo, int& bhi, int& slo,
-                            int& shi) {
+                            int& shi, int& olo, int& ohi) {
   const int lane = lane_id();
   const int y1 = g.yfirst + a.ystep * kTileRows;
   bool skip;
@@ -1004,6 +1043,8 @@ __device__ bool select_tile(const MorphArgs& a, int t, const TileGeo& g, u32& va
   bhi = -1;
   slo = 0;
   shi = -1;
+  olo = g.zfirst;
+  ohi = g.zlast;
   if constexpr (PASS == 0) {
     // F = 0 when no occupied voxel is in reach; F unused when every cell is full
     const bool empty = !wave_bricks_zrange(a.brickI, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff,
@@ -1030,8 +1071,10 @@ __device__ bool select_tile(const MorphArgs& a, int t, const TileGeo& g, u32& va
       vals = wave_vset_planes(a.rvset, a.brickK, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff, g.zfirst - a.maxoff,
                               g.zlast + 1 + a.maxoff, g.wx - 1, g.wx + 1, blo, bhi);
   } else {
-    skip = !wave_tile_has_flags(PASS == 1 ? a.TG : a.UG, a.R, a.W, g.wx, g.yfirst, g.yfirst + kTileRows, g.zfirst,
-                                g.zlast + 1);
+    skip = !wave_tile_flag_planes(PASS == 1 ? a.TG : a.UG, a.R, a.W, g.wx, g.yfirst, g.yfirst + kTileRows,
+                                  g.zfirst, g.zlast + 1, olo, ohi);
+    olo = max(olo, g.zfirst);
+    ohi = min(ohi, g.zlast);
     if (!skip && !a.rvar)  // the source planes in reach (I, or K)
       wave_bricks_zrange(PASS == 1 ? a.brickI : a.brickK, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff,
                          g.zfirst - a.maxoff, g.zlast + 1 + a.maxoff, g.wx - 1, g.wx + 1, 0, slo, shi);
@@ -1076,7 +1119,8 @@ __device__ void select_tiles(const MorphArgs& a) {
     u32 vals;
     int blo, bhi;  // lane v: brick planes holding value v
     int slo, shi;  // brick planes holding source voxels in reach (uniform radius)
-    if (select_tile<PASS>(a, t, g, vals, blo, bhi, slo, shi)) {
+    int olo, ohi;  // output planes that need the predicate
+    if (select_tile<PASS>(a, t, g, vals, blo, bhi, slo, shi, olo, ohi)) {
       const int x = slab_queue(t / a.W);
       if (a.rvar) {
         // lane v queues the tile for radius value v, with the planes that can
@@ -1086,20 +1130,24 @@ __device__ void select_tiles(const MorphArgs& a) {
         if (lane < 32 && ((vals >> lane) & 1u)) {
           int rlo = 8 * blo, rhi = 8 * bhi + 7;
           if constexpr (PASS < 2) {
-            rlo = max(rlo, g.zfirst);
-            rhi = min(rhi, g.zlast);
+            rlo = max(rlo, olo);
+            rhi = min(rhi, ohi);
+          } else {
+            rlo = max(rlo, olo - a.maxoff);
+            rhi = min(rhi, ohi + a.maxoff);
           }
           list = lane * 8 + x;
           entry = entry_pack(t, g.tz0, rlo, rhi);
         }
       } else if (lane == 0) {
-        // the stream reads only the input planes holding source voxels;
-        // tiles with most of their planes in reach go to the front of the
-        // list (they start first, mostly one per CU), the others to the back
-        const int lo = max(8 * slo, g.zfirst - a.maxoff), hi = min(8 * shi + 7, g.zlast + a.maxoff);
+        // the stream reads only the input planes holding source voxels
+        // within reach of the output planes that need the predicate; tiles
+        // with most of their planes in reach go to the front of the list
+        // (they start first, mostly one per CU), the others to the back
+        const int lo = max(8 * slo, olo - a.maxoff), hi = min(8 * shi + 7, ohi + a.maxoff);
         const int span = g.zlast - g.zfirst + 1 + 2 * a.maxoff;
         list = (4 * (hi - lo + 1) >= 3 * span) ? x : 8 + x;
-        entry = entry_pack(t, g.tz0, 8 * slo, 8 * shi + 7);
+        entry = entry_pack(t, g.tz0, lo, hi);
       }
     }
   }
@@ -1151,8 +1199,8 @@ __device__ void select_tiles_batched(const MorphArgs* __restrict__ tab, const u3
     const int t = (int)(u - pre[j]);
     const TileGeo g = pass_tile<PASS>(a, t);
     u32 vals;
-    int blo, bhi, slo, shi;
-    if (select_tile<PASS>(a, t, g, vals, blo, bhi, slo, shi) && lane_id() == 0) {
+    int blo, bhi, slo, shi, olo, ohi;
+    if (select_tile<PASS>(a, t, g, vals, blo, bhi, slo, shi, olo, ohi) && lane_id() == 0) {
       list = rad[j] * 8 + slab_queue(t / a.W);
       entry = ((u32)j << kBatchTileBits) | (u32)t;
     }

@@ -8,19 +8,21 @@
 // triangles from the triangulation table with reversed winding.
 //
 // MI355X design: the Morton order is walked directly. A workgroup owns an
-// aligned 8x8x8 block of cells (512 consecutive Morton codes) and stages the
-// closed grid around it in LDS, so the reference's tag / iota / copy_if /
-// sort_by_key / scan chain becomes
+// aligned 8x8x8 block of cells (512 consecutive Morton codes), so the
+// reference's tag / iota / copy_if / sort_by_key / scan chain becomes
 //   mc_blocks  list of the blocks that can hold a mixed cube
-//   mc_count   per block vertex/triangle totals, and the block-local vertex
-//              base of every cell with vertices
+//   mc_count   the block's voxels in LDS: the 8 cube cases of every cell
+//              (stored for mc_emit), per block vertex/triangle totals, and
+//              the block-local vertex base of every cell with vertices
 //   scan       exclusive scan of the block totals (chunk totals summed by
-//              mc_count, then one down-sweep kernel)
-//   mc_emit    cube cases in LDS, block scan over cells, then one thread per
-//              output slot: vertex positions with area-weighted normal sums
-//              gathered from the <= 4 cubes around each vertex, then
-//              triangles (vertex ids of other blocks' edges from their block
-//              base + cell base)
+//              mc_count, then one down-sweep kernel, which also sorts the
+//              blocks into work classes, heaviest first)
+//   mc_emit    the stored cases of the block and of the cells around it in
+//              LDS, a block scan over cells, then one thread per triangle
+//              (vertex ids of other blocks' edges from their block base +
+//              cell base; face normals kept in LDS), then one thread per
+//              vertex (position, area-weighted normal sum gathered from the
+//              <= 4 cubes around its edge)
 //   mc_fixup   triangles the count table does not cover, mc_normalize.
 // The per-cell vertex base map is dense (R^3 u32) but only written and read
 // at surface cells, replacing the reference's 4*N^3-byte memset + splat.
@@ -62,8 +64,10 @@ __device__ __forceinline__ void load_mc_tab(McTab& t) {
     for (int e = 0; e < 12; ++e) t.use[i][e] = g_mc.use[i][e];
   }
 }
-constexpr int kBox = 24;  // fine voxels per axis held in LDS (block 16 + halo)
-constexpr int kLo = 2;    // halo below the block
+// Voxel box of one Morton block in LDS (count kernel): the 17 x 17 rows of
+// fine voxels [16b, 16b + 17) per axis, 24 bits of x each.
+constexpr int kBoxRows = 17;
+constexpr int kBoxBits = 24;
 
 __device__ __forceinline__ u32 part1by2(u32 x) {
   x &= 0x3ffu;
@@ -83,18 +87,16 @@ __device__ __forceinline__ u32 compact1by2(u32 x) {
 }
 
 struct Box {
-  u32 rows[kBox * kBox];  // bit i = fine x0+i of row (y0+j, z0+k)
+  u32 rows[kBoxRows * kBoxRows];  // bit i = fine x0+i of row (y0+j, z0+k)
   int x0, y0, z0;
-  __device__ __forceinline__ int at(int x, int y, int z) const {
-    return (rows[(y - y0) + kBox * (z - z0)] >> (x - x0)) & 1;
-  }
   // three consecutive x bits starting at x
   __device__ __forceinline__ u32 at3(int x, int y, int z) const {
-    return (rows[(y - y0) + kBox * (z - z0)] >> (x - x0)) & 7u;
+    return (rows[(y - y0) + kBoxRows * (z - z0)] >> (x - x0)) & 7u;
   }
 };
 
-// Loads rows [fy0, fy0+ny) x [fz0, fz0+nz) of the box (x range 24 bits from fx0).
+// Loads rows [fy0, fy0+ny) x [fz0, fz0+nz) of the box (x range kBoxBits bits
+// from fx0; ny, nz <= kBoxRows).
 __device__ void load_box(Box& bx, const u32* __restrict__ C, int N, int fx0, int fy0, int fz0, int ny, int nz) {
   const int W = N / 32;
   if (threadIdx.x == 0) {
@@ -114,46 +116,24 @@ __device__ void load_box(Box& bx, const u32* __restrict__ C, int N, int fx0, int
       if ((unsigned)(w0 + 1) < (unsigned)W) b = C[r + w0 + 1];
     }
     u64 v = ((u64)b << 32) | a;
-    bx.rows[j + kBox * k] = (u32)(v >> o) & ((1u << kBox) - 1u);
+    bx.rows[j + kBoxRows * k] = (u32)(v >> o) & ((1u << kBoxBits) - 1u);
   }
   __syncthreads();
 }
 
-// Corner k of the cube at f in MarchingCubesMesher.h:39-46 order:
-// (0,0,0),(1,0,0),(1,1,0),(0,1,0),(0,0,1),(1,0,1),(1,1,1),(0,1,1).
-__device__ __forceinline__ int cube_case(const Box& b, int fx, int fy, int fz) {
-  u32 r00 = b.at3(fx, fy, fz) , r10 = b.at3(fx, fy + 1, fz);
-  u32 r01 = b.at3(fx, fy, fz + 1), r11 = b.at3(fx, fy + 1, fz + 1);
-  return (int)((r00 & 1u) | ((r00 >> 1 & 1u) << 1) | ((r10 >> 1 & 1u) << 2) | ((r10 & 1u) << 3) |
-               ((r01 & 1u) << 4) | ((r01 >> 1 & 1u) << 5) | ((r11 >> 1 & 1u) << 6) | ((r11 & 1u) << 7));
-}
-__device__ __forceinline__ int own_verts(int v) {
-  int c0 = v & 1;
-  return (c0 ^ ((v >> 1) & 1)) + (c0 ^ ((v >> 3) & 1)) + (c0 ^ ((v >> 4) & 1));
-}
+// A cube's case: bit k = voxel at corner k, corners in MarchingCubesMesher.h:39-46
+// order (0,0,0),(1,0,0),(1,1,0),(0,1,0),(0,0,1),(1,0,1),(1,1,1),(0,1,1). It
+// owns the vertices on its edges 0, 3, 8 (x, y, z from corner 0) that are cut.
 __device__ __forceinline__ bool mixed(int v) { return v != 0 && v != 255; }
 __device__ __forceinline__ bool cell_in_m_range(int cx, int cy, int cz, int R) {
   return cx >= 1 && cy >= 1 && cz >= 1 && cx <= R - 2 && cy <= R - 2 && cz <= R - 2;
-}
-// The 3x3x3 fine voxels of cell c are uniform (then no cube of c is mixed).
-__device__ __forceinline__ bool cell_uniform(const Box& b, int cx, int cy, int cz) {
-  u32 a = 7u, o = 0u;
-#pragma unroll
-  for (int k = 0; k < 3; ++k)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      u32 v = b.at3(2 * cx, 2 * cy + j, 2 * cz + k);
-      a &= v;
-      o |= v;
-    }
-  return a == 7u || o == 0u;
 }
 
 // swap of the two low bits of p (p < 4)
 __device__ __forceinline__ u32 swap2(u32 p) { return ((p * 5u) >> 1) & 3u; }
 
 // The 8 cube cases of cell (cx,cy,cz) as the bytes of one word (byte j =
-// cube (j&1, j>>1&1, j>>2) of the cell, cube_case corner order), 0 when the
+// cube (j&1, j>>1&1, j>>2) of the cell, corner order above), 0 when the
 // cell's 3x3x3 fine voxels are uniform: the 9 rows of 3 bits are read once
 // and each case takes two bits from four of them.
 __device__ __forceinline__ unsigned long long cell_cases_box(const Box& b, int cx, int cy, int cz) {
@@ -251,7 +231,7 @@ __device__ __forceinline__ void nb_cell(int k, int& lx, int& ly, int& lz) {
 }
 
 // Vertices owned by the cubes before cube j of a cell whose 8 cube cases are
-// the bytes of w (own_verts per byte, summed over bytes < j).
+// the bytes of w (cut edges among 0, 3, 8 per byte, summed over bytes < j).
 __device__ __forceinline__ u32 own_prefix8(unsigned long long w, int j) {
   constexpr unsigned long long m1 = 0x0101010101010101ull;
   const unsigned long long c0 = w & m1;
@@ -450,11 +430,11 @@ __device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R,
   int c0x, c0y, c0z;
   block_outside(blk, R, c0x, c0y, c0z);
   // cells of the block need fine voxels [16b, 16b+17) per axis
-  load_box(bx, C, 2 * R, 2 * c0x, 2 * c0y, 2 * c0z, 17, 17);
+  load_box(bx, C, 2 * R, 2 * c0x, 2 * c0y, 2 * c0z, kBoxRows, kBoxRows);
   // uniform block: no mixed cube
   u32 a = 0x1ffffu, o = 0u;
-  for (int i = threadIdx.x; i < 17 * 17; i += blockDim.x) {
-    u32 v = bx.rows[(i % 17) + kBox * (i / 17)] & 0x1ffffu;
+  for (int i = threadIdx.x; i < kBoxRows * kBoxRows; i += blockDim.x) {
+    u32 v = bx.rows[i] & 0x1ffffu;
     a &= v;
     o |= v;
   }
@@ -700,7 +680,7 @@ __device__ __forceinline__ void edge_pos_v(const Geo& g, int fx, int fy, int fz,
   else if (e == 3) o[1] = lerp_edge(__fadd_rn(py, g.h), py, (float)v1, (float)v0);
   else o[2] = lerp_edge(pz, __fadd_rn(pz, g.h), (float)v0, (float)v1);
 }
-// Corner index (cube_case order) of the corner at offset (ox, oy, oz).
+// Corner index (case bit order) of the corner at offset (ox, oy, oz).
 __device__ __forceinline__ int corner_of(int ox, int oy, int oz) { return 4 * oz + (oy ? (ox ? 2 : 3) : ox); }
 // Position of the vertex on local edge e of cube (fx,fy,fz) with case v: both
 // ends of the edge are corners of the cube, so their values are bits of v.
@@ -794,9 +774,6 @@ __device__ void normal_sum(const BlockLds& S, const Geo& geo, int R, int c0x, in
 
 __device__ __forceinline__ unsigned long long cell_cases(const BlockLds& S, int l) {
   return *reinterpret_cast<const unsigned long long*>(&S.cases[8 * l]);
-}
-__device__ __forceinline__ u32 prefix_in_cell(const BlockLds& S, int l, int j) {
-  return own_prefix8(cell_cases(S, l), j);
 }
 
 // Vertices of a staged block, in two phases per window of kListCap local

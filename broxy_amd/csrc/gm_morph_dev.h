@@ -42,14 +42,8 @@ __device__ __forceinline__ u32 flag_word(const u16* __restrict__ g, int R, int W
   if ((unsigned)cy >= (unsigned)R || (unsigned)cz >= (unsigned)R || (unsigned)w >= (unsigned)W) return 0u;
   return g[((size_t)cz * R + cy) * W + w];
 }
-// Branch-free forms of row_word / flag_word (clamped address, masked value):
-// a run of them issues its loads back to back.
-__device__ __forceinline__ u32 ld_row(const u32* __restrict__ g, int N, int W, int y, int z, int w) {
-  const bool in = (unsigned)y < (unsigned)N && (unsigned)z < (unsigned)N && (unsigned)w < (unsigned)W;
-  const int yc = min(max(y, 0), N - 1), zc = min(max(z, 0), N - 1), wc = min(max(w, 0), W - 1);
-  const u32 v = g[((size_t)zc * N + yc) * W + wc];
-  return in ? v : 0u;
-}
+// Branch-free form of flag_word (clamped address, masked value): a run of
+// them issues its loads back to back.
 __device__ __forceinline__ u32 ld_flag(const u16* __restrict__ g, int R, int W, int cy, int cz, int w) {
   const bool in = (unsigned)cy < (unsigned)R && (unsigned)cz < (unsigned)R && (unsigned)w < (unsigned)W;
   const int yc = min(max(cy, 0), R - 1), zc = min(max(cz, 0), R - 1), wc = min(max(w, 0), W - 1);
@@ -612,30 +606,9 @@ __device__ bool wave_bricks_all(const uint8_t* br, int N, int W, int y0, int y1,
   }
   return __any((all & 2u) == 0u) == 0;
 }
-// Some cell-row flag of word column wx set in rows [y0,y1), planes [z0,z1).
-__device__ bool wave_tile_has_flags(const u16* F, int R, int W, int wx, int y0, int y1, int z0, int z1) {
-  const int cy0 = y0 >> 1, cy1 = min((y1 + 1) >> 1, R);
-  const int cz0 = z0 >> 1, cz1 = min((z1 + 1) >> 1, R);
-  const int ny = cy1 - cy0, nz = cz1 - cz0;
-  u32 acc = 0u;
-  if (ny > 0 && nz > 0) {
-    const int n = ny * nz;
-    for (int i0 = 0; i0 < n; i0 += 256) {
-      u32 v[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int i = min(i0 + 64 * q + lane_id(), n - 1);
-        v[q] = ld_flag(F, R, W, cy0 + i % ny, cz0 + i / ny, wx);
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) acc |= v[q];
-    }
-  }
-  return __any(acc != 0u) != 0;
-}
-
-// As wave_tile_has_flags, and the fine planes [olo, ohi] of the cell planes
-// holding a flag (wave-uniform; olo > ohi when none does).
+// Some cell-row flag of word column wx set in rows [y0,y1), planes [z0,z1),
+// and the fine planes [olo, ohi] of the cell planes holding one
+// (wave-uniform; olo > ohi when none does).
 __device__ bool wave_tile_flag_planes(const u16* F, int R, int W, int wx, int y0, int y1, int z0, int z1, int& olo,
                                       int& ohi) {
   const int cy0 = y0 >> 1, cy1 = min((y1 + 1) >> 1, R);

@@ -63,15 +63,37 @@ __device__ __forceinline__ void tight_body(const MorphArgs& a, long long gid) {
   int wx = (int)(gid % a.W);
   long long t = gid / a.W;
   int cy = (int)(t % a.R), cz = (int)(t / a.R);
+  // flag words (cy + (d & 1), cz + (d >> 1)) at wx and wx + 1: the cell
+  // rows at x and y distance 1 are gids +1 and +W, consecutive over the
+  // lanes of a workgroup (a multiple of 64 threads), so those inside the
+  // wave come from the lanes that hold them
+  const int lane = (int)(threadIdx.x & 63u);
+  u32 v[4], nx[4];
+  v[0] = flag_word(a.DF, a.R, a.W, cy, cz, wx);
+  v[2] = flag_word(a.DF, a.R, a.W, cy, cz + 1, wx);
+  v[1] = __shfl(v[0], min(lane + a.W, 63));
+  v[3] = __shfl(v[2], min(lane + a.W, 63));
+  if (cy + 1 >= a.R) {
+    v[1] = v[3] = 0u;
+  } else if (lane + a.W > 63) {
+    v[1] = flag_word(a.DF, a.R, a.W, cy + 1, cz, wx);
+    v[3] = flag_word(a.DF, a.R, a.W, cy + 1, cz + 1, wx);
+  }
+#pragma unroll
+  for (int d = 0; d < 4; ++d) nx[d] = __shfl(v[d], min(lane + 1, 63));
+  if (wx + 1 >= a.W) {
+#pragma unroll
+    for (int d = 0; d < 4; ++d) nx[d] = 0u;
+  } else if (lane == 63) {
+#pragma unroll
+    for (int d = 0; d < 4; ++d) nx[d] = flag_word(a.DF, a.R, a.W, cy + (d & 1), cz + (d >> 1), wx + 1);
+  }
   u32 all = 0xffffu, any = 0u;
 #pragma unroll
   for (int d = 0; d < 4; ++d) {
-    int yy = cy + (d & 1), zz = cz + (d >> 1);
-    u32 v = flag_word(a.DF, a.R, a.W, yy, zz, wx);
-    u32 nx = flag_word(a.DF, a.R, a.W, yy, zz, wx + 1);
-    u32 sh = ((v >> 1) | (nx << 15)) & 0xffffu;
-    all &= v & sh;
-    any |= v | sh;
+    u32 sh = ((v[d] >> 1) | (nx[d] << 15)) & 0xffffu;
+    all &= v[d] & sh;
+    any |= v[d] | sh;
   }
   u32 valid = cell_range_mask(wx, 0, a.R - 1);
   if (cy > a.R - 2 || cz > a.R - 2) valid = 0;
@@ -79,7 +101,7 @@ __device__ __forceinline__ void tight_body(const MorphArgs& a, long long gid) {
   a.outFlags[gid] = (u16)tg;
   // D outside the tight contour: DF ? 1 : I; on it: I (the fine pass ORs in
   // F, which contains I)
-  const u32 df = expand2(flag_word(a.DF, a.R, a.W, cy, cz, wx) & ~tg);
+  const u32 df = expand2(v[0] & ~tg);
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     const size_t o = ((size_t)(2 * cz + (s >> 1)) * a.N + 2 * cy + (s & 1)) * a.W + wx;
@@ -194,18 +216,33 @@ __device__ __forceinline__ void contour_K_rows(const MorphArgs& a, int wx, int y
   u32 any = 0u;
   u32 zm = row_word(D, N, W, y, 8 * bz - 1, wx);
   u32 c = row_word(D, N, W, y, 8 * bz, wx);
+  // Thread t holds (wx, yl) = (t % W, t / W): the words at x distance 1 are
+  // threads t -+ 1 and those at y distance 1 threads t -+ W; inside the wave
+  // they come from the lanes that hold them, the others are loaded.
+  const int lane = (int)(threadIdx.x & 63u);
+  const int sl = max(lane - 1, 0), sr = min(lane + 1, 63), sym = max(lane - W, 0), syp = min(lane + W, 63);
+  const bool ymIn = yl >= 1 && lane >= W, ypIn = yl <= 6 && lane + W <= 63;
 #pragma unroll 2
   for (int zl = 0; zl < 8; ++zl) {
     const int z = 8 * bz + zl;
     const u32 zp = row_word(D, N, W, y, z + 1, wx);
-    const u32 l = row_word(D, N, W, y, z, wx - 1), r = row_word(D, N, W, y, z, wx + 1);
+    const u32 diag = row_word(D, N, W, y ^ 1, z ^ 1, wx);
+    u32 l = __shfl(c, sl), r = __shfl(c, sr), ym = __shfl(c, sym), yp = __shfl(c, syp);
+    if (wx == 0)
+      l = 0u;
+    else if (lane == 0)
+      l = row_word(D, N, W, y, z, wx - 1);
+    if (wx == W - 1)
+      r = 0u;
+    else if (lane == 63)
+      r = row_word(D, N, W, y, z, wx + 1);
+    if (!ymIn) ym = row_word(D, N, W, y - 1, z, wx);
+    if (!ypIn) yp = row_word(D, N, W, y + 1, z, wx);
     const u32 xn = (c << 1) | (l >> 31) | (c >> 1) | (r << 31);
-    const u32 ym = row_word(D, N, W, y - 1, z, wx), yp = row_word(D, N, W, y + 1, z, wx);
     const u32 n6 = xn | ym | yp | zm | zp;
     // the 2x2 fine rows of this voxel's cell row
     const u32 ysib = (y & 1) ? ym : yp;
     const u32 zsib = (z & 1) ? zm : zp;
-    const u32 diag = row_word(D, N, W, y ^ 1, z ^ 1, wx);
     const u32 f = c & ysib & zsib & diag;
     if (!(y & 1) && !(z & 1)) {
       // the cells with a voxel of D (read by the coarse erosion)
@@ -372,8 +409,19 @@ __device__ __forceinline__ void base_contour_body(const MorphArgs& a, long long 
   int cy = (int)(t % a.R), cz = (int)(t / a.R);
   u32 valid = cell_range_mask(wx, 1, a.R - 1);
   if (cy < 1 || cy > a.R - 2 || cz < 1 || cz > a.R - 2) valid = 0;
-  u32 c[4], l[4], r[4], o[4];
+  u32 c[4], l[4], r[4], yp[4], ym[4], o[4];
   cprime_rows(a, cy, cz, wx, c);
+  // The cell rows at x and y distance 1 are gids +-1 and +-W. Gids are
+  // consecutive over the lanes of a workgroup (a multiple of 64 threads),
+  // so a neighbour inside the wave comes from the lane that holds it.
+  const int lane = (int)(threadIdx.x & 63u);
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    l[s] = __shfl(c[s], max(lane - 1, 0));
+    r[s] = __shfl(c[s], min(lane + 1, 63));
+    yp[s] = __shfl(c[s], min(lane + a.W, 63));
+    ym[s] = __shfl(c[s], max(lane - a.W, 0));
+  }
   // C outside the base contour is C'; on it all ones (the fine pass clears Ef)
   auto put = [&](u32 ug) {
     a.outFlags[gid] = (u16)ug;
@@ -386,19 +434,31 @@ __device__ __forceinline__ void base_contour_body(const MorphArgs& a, long long 
     put(0u);
     return;
   }
-  cprime_rows(a, cy, cz, wx - 1, l);
-  cprime_rows(a, cy, cz, wx + 1, r);
+  // words outside the grid are empty; valid cells have 1 <= cy <= R-2
+  if (wx == 0) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) l[s] = 0u;
+  } else if (lane == 0) {
+    cprime_rows(a, cy, cz, wx - 1, l);
+  }
+  if (wx == a.W - 1) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) r[s] = 0u;
+  } else if (lane == 63) {
+    cprime_rows(a, cy, cz, wx + 1, r);
+  }
+  if (lane + a.W > 63) cprime_rows(a, cy + 1, cz, wx, yp);
+  if (lane < a.W) cprime_rows(a, cy - 1, cz, wx, ym);
   u32 diff = 0u;
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     u32 xp = (c[s] >> 2) | (r[s] << 30);  // cell j+1 aligned on cell j
     u32 xm = (c[s] << 2) | (l[s] >> 30);  // cell j-1
-    diff |= (c[s] ^ xp) | (c[s] ^ xm);
+    diff |= (c[s] ^ xp) | (c[s] ^ xm) | (c[s] ^ yp[s]) | (c[s] ^ ym[s]);
   }
-  const int nb[4][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}};
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    cprime_rows(a, cy + nb[q][0], cz + nb[q][1], wx, o);
+  for (int q = -1; q <= 1; q += 2) {
+    cprime_rows(a, cy, cz + q, wx, o);
 #pragma unroll
     for (int s = 0; s < 4; ++s) diff |= c[s] ^ o[s];
   }

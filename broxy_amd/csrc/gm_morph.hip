@@ -447,8 +447,22 @@ __device__ __forceinline__ void base_contour_body(const MorphArgs& a, long long 
   } else if (lane == 63) {
     cprime_rows(a, cy, cz, wx + 1, r);
   }
-  if (lane + a.W > 63) cprime_rows(a, cy + 1, cz, wx, yp);
-  if (lane < a.W) cprime_rows(a, cy - 1, cz, wx, ym);
+  {
+    // one load for the lanes that need a single y neighbour (W <= 32: all)
+    const bool needP = lane + a.W > 63, needM = lane < a.W;
+    if (needP && needM) {
+      cprime_rows(a, cy + 1, cz, wx, yp);
+      cprime_rows(a, cy - 1, cz, wx, ym);
+    } else if (needP || needM) {
+      u32 t[4];
+      cprime_rows(a, needP ? cy + 1 : cy - 1, cz, wx, t);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        yp[s] = needP ? t[s] : yp[s];
+        ym[s] = needP ? ym[s] : t[s];
+      }
+    }
+  }
   u32 diff = 0u;
 #pragma unroll
   for (int s = 0; s < 4; ++s) {

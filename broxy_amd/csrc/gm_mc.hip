@@ -488,8 +488,9 @@ __device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R,
 }
 
 // Vertex and triangle counts of the active blocks; the workgroups walk the
-// list (each ends when the list does).
-__global__ void __launch_bounds__(kCountThreads) k_mc_count(const u32* __restrict__ C, int R,
+// list (each ends when the list does). 8 waves per SIMD (registers capped to
+// fit), so that the 2048 workgroups of a full list are resident at once.
+__global__ void __launch_bounds__(kCountThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) k_mc_count(const u32* __restrict__ C, int R,
                                                           const u32* __restrict__ list, u32* blockV,
                                                           u32* blockT, u32* cellL, unsigned long long* cellCase,
                                                           u32* late, u32* part) {

@@ -106,17 +106,28 @@ __device__ void load_box(Box& bx, const u32* __restrict__ C, int N, int fx0, int
   }
   int w0 = (fx0 >= 0) ? (fx0 >> 5) : -((31 - fx0) >> 5);
   int o = fx0 - 32 * w0;
-  for (int i = threadIdx.x; i < ny * nz; i += blockDim.x) {
-    int j = i % ny, k = i / ny;
-    int y = fy0 + j, z = fz0 + k;
-    u32 a = 0, b = 0;
+  auto row = [&](int i, u32& a, u32& b) {
+    const int y = fy0 + i % ny, z = fz0 + i / ny;
+    a = b = 0u;
     if ((unsigned)y < (unsigned)N && (unsigned)z < (unsigned)N) {
-      size_t r = ((size_t)z * N + y) * W;
+      const size_t r = ((size_t)z * N + y) * W;
       if ((unsigned)w0 < (unsigned)W) a = C[r + w0];
       if ((unsigned)(w0 + 1) < (unsigned)W) b = C[r + w0 + 1];
     }
-    u64 v = ((u64)b << 32) | a;
-    bx.rows[j + kBoxRows * k] = (u32)(v >> o) & ((1u << kBoxBits) - 1u);
+  };
+  auto put = [&](int i, u32 a, u32 b) {
+    const u64 v = ((u64)b << 32) | a;
+    bx.rows[i % ny + kBoxRows * (i / ny)] = (u32)(v >> o) & ((1u << kBoxBits) - 1u);
+  };
+  // two rows per thread in flight at once (the 17x17 rows of the count
+  // kernel's box take one round on 256 threads instead of two)
+  for (int i = threadIdx.x; i < ny * nz; i += 2 * blockDim.x) {
+    const int i2 = i + (int)blockDim.x;
+    u32 a, b, a2 = 0u, b2 = 0u;
+    row(i, a, b);
+    if (i2 < ny * nz) row(i2, a2, b2);
+    put(i, a, b);
+    if (i2 < ny * nz) put(i2, a2, b2);
   }
   __syncthreads();
 }

@@ -59,6 +59,30 @@ def test_dilate_erode_uniform(ctx, r):
     assert np.array_equal(C, Cref), "erosion differs in %d cells" % int((C != Cref).sum())
 
 
+@pytest.mark.parametrize("R", [16, 80, 144])
+def test_dilate_erode_row_widths(ctx, R):
+    """Grids of 1, 5 and 9 cell-row words per row. The contour kernels take
+    the words of neighbouring rows from the lanes of the wave that hold
+    them (64 / W rows per wave) and load the others, so row widths that do
+    not divide 64 mix both at every lane position."""
+    r = 3
+    I = random_blobs(R, seed=40 + R, nballs=8, rmax=4.0 if R < 32 else 8.0, margin=3)
+    dr = np.array([R, R - 3, R - 5], np.int32)
+    grid_setup(ctx, R, I, dr)
+    ctx.scale_uniform(r)
+    say("dilate/erode row widths r=%d R=%d" % (r, R))
+    ctx.dilate(B.SE_SPHERE)
+    D = ctx.get_cells(B.GRID_DILATION)
+    Dref = O.dilate(I, R, r=r, data_res=dr)
+    assert np.array_equal(D, Dref), "dilation differs in %d cells" % int((D != Dref).sum())
+    ctx.erode()
+    K = ctx.get_cells(B.GRID_CONTOUR)
+    assert np.array_equal(K, O.contour_K(Dref, R))
+    C = ctx.get_cells(B.GRID_CLOSING)
+    Cref = O.erode(I, Dref, R, r=r, data_res=dr)
+    assert np.array_equal(C, Cref), "erosion differs in %d cells" % int((C != Cref).sum())
+
+
 @pytest.mark.parametrize("r", [1, 11, 15, 16, 17, 23, 31])
 def test_dilate_erode_radius_range(ctx, r):
     """Radii across the supported range (both the compiled chord tables and

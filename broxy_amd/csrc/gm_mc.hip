@@ -327,9 +327,12 @@ __device__ void scan_cells(BlockLds& S) {
   // two cells per thread (workgroup of 256)
   int l0 = 2 * threadIdx.x;
   u32 v0 = S.vOff[l0], v1 = S.vOff[l0 + 1], t0 = S.tOff[l0], t1 = S.tOff[l0 + 1];
-  u32 tv, tt;
-  u32 ov = block_exscan(v0 + v1, S.red, &tv);
-  u32 ot = block_exscan(t0 + t1, S.red, &tt);
+  // one scan of (vertices | triangles << 16): a block's 4096 cubes own at
+  // most 3 vertices and count at most 5 triangles each (12288, 20480), so
+  // neither field carries
+  u32 tot;
+  const u32 o = block_exscan((v0 + v1) | ((t0 + t1) << 16), S.red, &tot);
+  const u32 ov = o & 0xffffu, ot = o >> 16;
   S.vOff[l0] = ov;
   S.vOff[l0 + 1] = ov + v0;
   S.tOff[l0] = ot;
@@ -472,9 +475,12 @@ __device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R,
 #pragma unroll
     for (int h = 0; h < kCountCells; ++h) dst[h] = w[h];
   }
-  u32 tv, tt;
-  u32 off = block_exscan(nv, red, &tv);
-  block_exscan(nt, red, &tt);
+  // one scan of (vertices | triangles << 16): a block's 4096 cubes own at
+  // most 3 vertices and count at most 5 triangles each (12288, 20480), so
+  // neither field carries
+  u32 tot;
+  u32 off = block_exscan(nv | (nt << 16), red, &tot) & 0xffffu;
+  const u32 tv = tot & 0xffffu, tt = tot >> 16;
   // block-local vertex base of the cells with vertices (cells are in Morton
   // order, consecutive per thread), read for the edges owned by another
   // block's cube

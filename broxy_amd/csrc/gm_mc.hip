@@ -89,6 +89,9 @@ __device__ __forceinline__ u32 compact1by2(u32 x) {
 struct Box {
   u32 rows[kBoxRows * kBoxRows];  // bit i = fine x0+i of row (y0+j, z0+k)
   int x0, y0, z0;
+  // per wave: bit 0 = some row of its 17-bit cube span has a set voxel,
+  // bit 1 = some row has a clear one
+  u32 uni[16];
   // three consecutive x bits starting at x
   __device__ __forceinline__ u32 at3(int x, int y, int z) const {
     return (rows[(y - y0) + kBoxRows * (z - z0)] >> (x - x0)) & 7u;
@@ -115,9 +118,13 @@ __device__ void load_box(Box& bx, const u32* __restrict__ C, int N, int fx0, int
       if ((unsigned)(w0 + 1) < (unsigned)W) b = C[r + w0 + 1];
     }
   };
+  u32 all = 0x1ffffu, any = 0u;  // over the 17 bits the cubes of a block read
   auto put = [&](int i, u32 a, u32 b) {
     const u64 v = ((u64)b << 32) | a;
-    bx.rows[i % ny + kBoxRows * (i / ny)] = (u32)(v >> o) & ((1u << kBoxBits) - 1u);
+    const u32 row = (u32)(v >> o) & ((1u << kBoxBits) - 1u);
+    bx.rows[i % ny + kBoxRows * (i / ny)] = row;
+    all &= row;
+    any |= row & 0x1ffffu;
   };
   // two rows per thread in flight at once (the 17x17 rows of the count
   // kernel's box take one round on 256 threads instead of two)
@@ -128,6 +135,10 @@ __device__ void load_box(Box& bx, const u32* __restrict__ C, int N, int fx0, int
     if (i2 < ny * nz) row(i2, a2, b2);
     put(i, a, b);
     if (i2 < ny * nz) put(i2, a2, b2);
+  }
+  {
+    const bool one = __ballot(any != 0u) != 0ull, zero = __ballot((all & 0x1ffffu) != 0x1ffffu) != 0ull;
+    if ((threadIdx.x & 63u) == 0u) bx.uni[threadIdx.x >> 6] = (one ? 1u : 0u) | (zero ? 2u : 0u);
   }
   __syncthreads();
 }
@@ -445,16 +456,11 @@ __device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R,
   block_outside(blk, R, c0x, c0y, c0z);
   // cells of the block need fine voxels [16b, 16b+17) per axis
   load_box(bx, C, 2 * R, 2 * c0x, 2 * c0y, 2 * c0z, kBoxRows, kBoxRows);
-  // uniform block: no mixed cube
-  u32 a = 0x1ffffu, o = 0u;
-  for (int i = threadIdx.x; i < kBoxRows * kBoxRows; i += blockDim.x) {
-    u32 v = bx.rows[i] & 0x1ffffu;
-    a &= v;
-    o |= v;
-  }
-  const bool anyOne = __syncthreads_or(o != 0u) != 0;
-  const bool anyZero = __syncthreads_or(a != 0x1ffffu) != 0;
-  const bool nonuni = anyOne && anyZero;
+  // uniform block: no mixed cube (a set and a clear voxel somewhere in the
+  // 17^3 voxels its cubes read)
+  u32 u = 0u;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) u |= bx.uni[w];
+  const bool nonuni = u == 3u;
   u32 nvh[kCountCells] = {}, nv = 0, nt = 0;
   unsigned long long w[kCountCells] = {};
   if (nonuni) {

@@ -201,8 +201,10 @@ __device__ __forceinline__ void block_cell(int blk, int l, int& cx, int& cy, int
 }
 
 // Exclusive scan over the workgroup in thread order of one value per thread;
-// writes the workgroup total to *total.
-__device__ u32 block_exscan(u32 v, u32* lds16, u32* total) {
+// writes the workgroup total to *total. trailing = false leaves out the
+// closing barrier, for callers that pass another barrier before lds16 is
+// written again.
+__device__ u32 block_exscan(u32 v, u32* lds16, u32* total, bool trailing = true) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   u32 x = v;
 #pragma unroll
@@ -219,7 +221,7 @@ __device__ u32 block_exscan(u32 v, u32* lds16, u32* total) {
     if (i < wv) off += s;
     tot += s;
   }
-  __syncthreads();
+  if (trailing) __syncthreads();
   *total = tot;
   return off + x - v;
 }
@@ -342,7 +344,7 @@ __device__ void scan_cells(BlockLds& S) {
   // most 3 vertices and count at most 5 triangles each (12288, 20480), so
   // neither field carries
   u32 tot;
-  const u32 o = block_exscan((v0 + v1) | ((t0 + t1) << 16), S.red, &tot);
+  const u32 o = block_exscan((v0 + v1) | ((t0 + t1) << 16), S.red, &tot, false);  // barrier below
   const u32 ov = o & 0xffffu, ot = o >> 16;
   S.vOff[l0] = ov;
   S.vOff[l0 + 1] = ov + v0;
@@ -485,7 +487,8 @@ __device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R,
   // most 3 vertices and count at most 5 triangles each (12288, 20480), so
   // neither field carries
   u32 tot;
-  u32 off = block_exscan(nv | (nt << 16), red, &tot) & 0xffffu;
+  // (no closing barrier: the caller's loop ends with one)
+  u32 off = block_exscan(nv | (nt << 16), red, &tot, false) & 0xffffu;
   const u32 tv = tot & 0xffffu, tt = tot >> 16;
   // block-local vertex base of the cells with vertices (cells are in Morton
   // order, consecutive per thread), read for the edges owned by another

@@ -448,7 +448,8 @@ __device__ __forceinline__ void base_contour_body(const MorphArgs& a, long long 
     cprime_rows(a, cy, cz, wx + 1, r);
   }
   {
-    // one load for the lanes that need a single y neighbour (W <= 32: all)
+    // the wave's first row loads y - 1 and its last row y + 1 (W = 64: one
+    // row, both loads); a lane that needs one of them issues one load
     const bool needP = lane + a.W > 63, needM = lane < a.W;
     if (needP && needM) {
       cprime_rows(a, cy + 1, cz, wx, yp);

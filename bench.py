@@ -92,7 +92,8 @@ def build_workload(args):
 
 
 def cpu_baseline(I, R, r, dr, bmin, cell, N, lat=None):
-    """The oracle (CPU restatement, OpenMP) on the same closing + mesh."""
+    """The oracle (CPU restatement, OpenMP) on the same closing + mesh. Its
+    grids and mesh are kept: every bench line compares the GPU's with them."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
     sc = None
@@ -105,7 +106,27 @@ def cpu_baseline(I, R, r, dr, bmin, cell, N, lat=None):
     V, Nn, T = O.mc(C, R, bmin, cell)
     dt = time.perf_counter() - t0
     return {"value": round(N ** 3 / dt / 1e9, 4), "unit": "Gvoxels/s", "cores": int(O.num_threads()),
-            "kind": "port", "sample": "full closing + mesh of the same grid, once (%.1f s)" % dt}, C, (V, T)
+            "kind": "port", "sample": "full closing + mesh of the same grid, once (%.1f s)" % dt}, D, C, (V, Nn, T)
+
+
+def parity(ctx, P, T, R, bmin, bmax, D, C, mesh):
+    """GPU grids and mesh of the last step against the oracle's: voxelized
+    grid, dilation and closing bit for bit, triangles exactly, vertices within
+    1e-6 relative, unit normals within 2e-5."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    Vr, Nr, Tr = mesh
+    V, Nn, Tt = ctx.mesh()
+    Iref = O.voxelize(P, T, bmin, float((bmax - bmin).max()), R)
+    same = V.shape == Vr.shape and Tt.shape == Tr.shape
+    out = {"voxelized_bit_exact": bool(np.array_equal(ctx.get_cells(B.GRID_INPUT), Iref)),
+           "dilation_bit_exact": bool(np.array_equal(ctx.get_cells(B.GRID_DILATION), D)),
+           "closing_bit_exact": bool(np.array_equal(ctx.get_cells(B.GRID_CLOSING), C)),
+           "triangles_exact": bool(same and np.array_equal(Tt, Tr)),
+           "vertices_rtol1e-6": bool(same and np.allclose(V, Vr, rtol=1e-6, atol=0)),
+           "normals_atol2e-5": bool(same and np.allclose(Nn, Nr, atol=2e-5))}
+    out["all"] = all(out.values())
+    return out
 
 
 def _mesh_hash(V, N, T):
@@ -284,8 +305,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="large")
     ap.add_argument("--seed", type=int, default=2017)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--check", action="store_true", help="compare the closing grid with the oracle")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle parity check")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-jobs measurement")
     ap.add_argument("--no-served", action="store_true", help="skip the served-requests measurement")
     ap.add_argument("--batch-jobs", type=int, default=256)
@@ -409,14 +429,9 @@ def main():
         out["radius_planes_ms_once_per_field"] = round(planes_ms, 4)
     if not args.no_cpu:
         I = ctx.get_cells(B.GRID_INPUT)
-        cb, Cref, (Vr, Tr) = cpu_baseline(I, R, r, dr, bmin, cell, N, lat)
+        cb, Dref, Cref, mesh = cpu_baseline(I, R, r, dr, bmin, cell, N, lat)
         out["cpu_baseline"] = cb
-        if args.check:
-            C = ctx.get_cells(B.GRID_CLOSING)
-            V, Nn, Tt = ctx.mesh()
-            out["parity"] = {"closing_bit_exact": bool(np.array_equal(C, Cref)),
-                             "triangles_exact": bool(np.array_equal(Tt, Tr)),
-                             "vertices_rtol1e-6": bool(V.shape == Vr.shape and np.allclose(V, Vr, rtol=1e-6, atol=0))}
+        out["parity"] = parity(ctx, P, T, R, bmin, bmax, Dref, Cref, mesh)
     ctx.close()
     if not args.no_batched:
         out["batched"] = batched(args)

@@ -132,7 +132,8 @@ int alloc_mesh_out(gm_ctx* c, u32 capV, u32 capT, u32 capLate) {
   }
   if (capLate > c->mc.capLate || !c->mc.late) {
     dfree(c->mc.late);
-    HIPCALL(c, hipMalloc(&c->mc.late, (size_t)(1 + 4 * capLate) * 4));
+    // entries, then their copy in slot order (k_mc_fixup)
+    HIPCALL(c, hipMalloc(&c->mc.late, (size_t)(1 + 8 * capLate) * 4));
     c->mc.capLate = capLate;
   }
   return GM_OK;
@@ -760,6 +761,10 @@ int gm_mesh_counts(gm_ctx* c, uint32_t* nv, uint32_t* nt) {
     if (rc) return rc;
     u32 tot[4] = {c->hTotals[0], c->hTotals[1], 0u, 0u};
     HIPCALL(c, hipMemcpyAsync(c->mc.totals, tot, 16, hipMemcpyHostToDevice, c->stream));
+    // the emit appends to the late list from zero (the count kernel cleared
+    // it before the first emit; a regrown list is fresh memory)
+    const u32 zero = 0u;
+    HIPCALL(c, hipMemcpyAsync(c->mc.late, &zero, 4, hipMemcpyHostToDevice, c->stream));
     hipError_t e = launch_mc_emit(c->C, c->R, c->bmin, c->cell, c->mc, c->stream, nullptr);
     if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("mesh extraction: ") + hipGetErrorString(e));
     HIPCALL(c, hipMemcpyAsync(c->hTotals, c->mc.totals, 16, hipMemcpyDeviceToHost, c->stream));

@@ -246,7 +246,7 @@ struct MCBuffers {
   float* V;
   float* Nrm;
   u32* T;
-  u32* late;     // [0] count, then (slot, a, b, c) quadruples
+  u32* late;     // [0] count, capLate (slot, a, b, c) quadruples, then their copy in slot order
   u32* totals;   // [0] nV [1] nT [2] overflow [3] border flag
   u32* scanPart; // per 1024 blocks: vertex and triangle totals (mc_scan_part_bytes)
   uint8_t* brick; // brick summary of the closing grid (1 word x 8 rows x 8 planes)

@@ -1021,44 +1021,53 @@ __device__ __forceinline__ void face(const float* V, const u32* t, float* n) {
   n[2] = __fsub_rn(__fmul_rn(a0, b1), __fmul_rn(a1, b0));
 }
 
-// Applies the listed-but-uncounted triangles in slot order (single thread):
-// each replaces the triangle in its slot, and the normal sums follow.
+// Applies the listed-but-uncounted triangles in slot order: each replaces the
+// triangle in its slot, and the normal sums follow. The workgroup first ranks
+// the entries by slot (slots are unique: each is the first slot of the next
+// cube with triangles) into the list's second half, then one lane applies
+// them in that order, so the float sums are deterministic. Entries whose slot
+// or vertex ids fall outside the mesh are skipped.
 __device__ __forceinline__ void mc_fixup_body(const float* V, float* Nrm, u32* T, u32* late, u32 capLate,
                                               u32* totals, u32 capV, u32 capT) {
   u32 n = late[0];
   if (n > capLate) n = capLate;
   if (n == 0) return;
-  // insertion sort by slot (the list is tiny)
-  for (u32 i = 1; i < n; ++i) {
-    u32 e[4] = {late[1 + 4 * i], late[2 + 4 * i], late[3 + 4 * i], late[4 + 4 * i]};
-    u32 j = i;
-    while (j > 0 && late[1 + 4 * (j - 1)] > e[0]) {
-      for (int q = 0; q < 4; ++q) late[1 + 4 * j + q] = late[1 + 4 * (j - 1) + q];
-      --j;
+  u32* sorted = late + 1 + 4 * (size_t)capLate;
+  for (u32 i = threadIdx.x; i < n; i += blockDim.x) {
+    const u32 si = late[1 + 4 * i];
+    u32 rank = 0;
+    for (u32 j = 0; j < n; ++j) {
+      const u32 sj = late[1 + 4 * j];
+      rank += (sj < si || (sj == si && j < i)) ? 1u : 0u;
     }
-    for (int q = 0; q < 4; ++q) late[1 + 4 * j + q] = e[q];
+    for (int q = 0; q < 4; ++q) sorted[4 * rank + q] = late[1 + 4 * i + q];
   }
+  __threadfence_block();
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   const u32 nT = totals[1], nV = totals[0];
   if (nT > capT || nV > capV) return;
   for (u32 i = 0; i < n; ++i) {
-    u32 s = late[1 + 4 * i];
-    if (s >= nT) continue;
+    const u32 s = sorted[4 * i];
+    const u32 a = sorted[4 * i + 1], b = sorted[4 * i + 2], c = sorted[4 * i + 3];
+    if (s >= nT || a >= nV || b >= nV || c >= nV) continue;
     u32* t = T + 3 * (size_t)s;
+    if (t[0] >= nV || t[1] >= nV || t[2] >= nV) continue;
     float f[3];
     face(V, t, f);
     for (int k = 0; k < 3; ++k)
       for (int q = 0; q < 3; ++q) Nrm[3 * (size_t)t[k] + q] = __fsub_rn(Nrm[3 * (size_t)t[k] + q], f[q]);
-    t[0] = late[2 + 4 * i];
-    t[1] = late[3 + 4 * i];
-    t[2] = late[4 + 4 * i];
+    t[0] = a;
+    t[1] = b;
+    t[2] = c;
     face(V, t, f);
     for (int k = 0; k < 3; ++k)
       for (int q = 0; q < 3; ++q) Nrm[3 * (size_t)t[k] + q] = __fadd_rn(Nrm[3 * (size_t)t[k] + q], f[q]);
   }
 }
-__global__ void k_mc_fixup(const float* V, float* Nrm, u32* T, u32* late, u32 capLate, u32* totals, u32 capV,
-                           u32 capT) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+constexpr int kFixupThreads = 256;
+__global__ void __launch_bounds__(kFixupThreads) k_mc_fixup(const float* V, float* Nrm, u32* T, u32* late,
+                                                            u32 capLate, u32* totals, u32 capV, u32 capT) {
   mc_fixup_body(V, Nrm, T, late, capLate, totals, capV, capT);
 }
 
@@ -1151,8 +1160,7 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_emit_b(const MCJob* __restric
                   J.m.capLate, J.m.totals);
   }
 }
-__global__ void k_mc_fixup_b(const MCJob* __restrict__ jobs) {
-  if (threadIdx.x != 0) return;
+__global__ void __launch_bounds__(kFixupThreads) k_mc_fixup_b(const MCJob* __restrict__ jobs) {
   const MCJob& J = jobs[blockIdx.x];
   mc_fixup_body(J.m.V, J.m.Nrm, J.m.T, J.m.late, J.m.capLate, J.m.totals, J.m.capV, J.m.capT);
 }
@@ -1311,7 +1319,7 @@ hipError_t launch_mc_batched(const BatchMCJob* in, int n, void* hBlob, void* dBl
   GM_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_mc_emit_b, dim3(listGrid), dim3(kMcThreads), 0, s, dj, lpre, n);
   GM_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_mc_fixup_b, dim3(n), dim3(64), 0, s, dj);
+  hipLaunchKernelGGL(k_mc_fixup_b, dim3(n), dim3(kFixupThreads), 0, s, dj);
   GM_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_mc_normalize_b, dim3(n * kNormWgPerJob), dim3(256), 0, s, dj);
   GM_CHECK(hipGetLastError());
@@ -1330,7 +1338,7 @@ hipError_t launch_mc_emit(const u32* C, int R, const float* bmin3, float cell, c
                      m.capT, m.late, m.capLate, m.totals);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
-  hipLaunchKernelGGL(k_mc_fixup, dim3(1), dim3(64), 0, s, m.V, m.Nrm, m.T, m.late, m.capLate, m.totals, m.capV,
+  hipLaunchKernelGGL(k_mc_fixup, dim3(1), dim3(kFixupThreads), 0, s, m.V, m.Nrm, m.T, m.late, m.capLate, m.totals, m.capV,
                      m.capT);
   GM_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_mc_normalize, dim3(1024), dim3(256), 0, s, m.Nrm, m.totals, m.capV);

@@ -5,7 +5,8 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(ROOT, "oracle", "_build", "libgmorpho_oracle.so")
+# GM_ORACLE_LIB selects another build of the same sources (e.g. the sanitized one).
+LIB_PATH = os.environ.get("GM_ORACLE_LIB") or os.path.join(ROOT, "oracle", "_build", "libgmorpho_oracle.so")
 
 _lib = None
 

@@ -94,6 +94,171 @@ def test_erode_dfs_equals_brute(seed):
     assert np.array_equal(O.erode(I, D2, R, scale=sc), O.erode(I, D2, R, scale=sc, brute=1))
 
 
+def _hole_ball(p, r, N):
+    """Fine voxels v whose centre lies closer than r-1 to the box [p, p+1]^3:
+    per axis max(0, |v - p| - 1/2), in doubled units (2|t| - 1 for t != 0)."""
+    z, y, x = np.mgrid[0:N, 0:N, 0:N]
+
+    def d2(t):
+        t = np.abs(t)
+        return np.where(t == 0, 0, (2 * t - 1) ** 2)
+
+    return d2(x - p[0]) + d2(y - p[1]) + d2(z - p[2]) < 4 * (r - 1) ** 2
+
+
+@pytest.mark.parametrize("r", [2, 3, 4, 6, 9])
+def test_erosion_single_contour_voxel_closed_form(r):
+    # D full except fine voxel p, so K = {p}; I empty. Reference semantics
+    # (GMorpho.cu:6627-6702): the coarse pass clears a cell when its centre
+    # 2c+1 is closer than r-1 to p's box (GMorpho.cu:4372, radius r-1 at
+    # 4430-4437); the fine pass rewrites every cell of the 6-connected base
+    # contour of that result (Grid.cu:1499-1570) with the sub-voxels whose
+    # centre is not closer than r-1 (GMorpho.cu:5081-5345). The distances to a
+    # single box are closed forms: per axis max(0, |t| - 1/2) from a centre.
+    R = 16
+    N = 2 * R
+    p = (15, 16, 17)
+    f = np.ones((N, N, N), bool)
+    f[p[2], p[1], p[0]] = False
+    D = cells_from_fine(f)
+    K = fine_grid(O.contour_K(D, R), R)
+    assert K.sum() == 1 and K[p[2], p[1], p[0]]
+    I = np.zeros_like(D)
+    C = fine_grid(O.erode(I, D, R, r=r), R)
+    ball = _hole_ball(p, r, N)  # sub-voxel centres in reach
+    # coarse: cell centres (2c+1) in reach, in doubled units |2(2c+1) - (2p+1)|
+    cz, cy, cx = np.mgrid[0:R, 0:R, 0:R]
+
+    def cd2(cc, pp):
+        t = np.abs(2 * (2 * cc + 1) - (2 * pp + 1))
+        return np.maximum(0, t - 1) ** 2
+
+    cleared = cd2(cx, p[0]) + cd2(cy, p[1]) + cd2(cz, p[2]) < 4 * (r - 1) ** 2
+    inner = np.zeros_like(cleared)
+    inner[1:-1, 1:-1, 1:-1] = True
+    byte = np.where(cleared, 0, 1)
+    diff = np.zeros_like(cleared)
+    for ax in range(3):
+        for s in (1, -1):
+            diff |= byte != np.roll(byte, s, axis=ax)
+    U = diff & inner
+    up = lambda a: np.repeat(np.repeat(np.repeat(a, 2, 0), 2, 1), 2, 2)  # noqa: E731
+    want = np.where(up(U), ~ball, np.where(up(cleared), False, f))
+    assert np.array_equal(C, want)
+    # never beyond the ideal ball: every eroded voxel is in reach of p
+    assert not np.any(~C & ~ball & f)
+    # the ideal ball's closed-form count #{t : sum max(0,|t_a|-1/2)^2 < (r-1)^2}
+    rr = r - 1
+    cnt = sum(1 for a in range(-rr, rr + 1) for b in range(-rr, rr + 1) for c in range(-rr, rr + 1)
+              if sum(0 if t == 0 else (2 * abs(t) - 1) ** 2 for t in (a, b, c)) < 4 * rr * rr)
+    assert ball.sum() == cnt
+    if r == 2:
+        # the 3x3x3 block around p: every sum of up to three 1/4 terms is < 1
+        assert cnt == 27
+    if r == 3:
+        # the 5x5x5 block less the voxels two steps out on at least two axes
+        # (9/4 + 9/4 >= 4): 8 corners and 3 * 4 * 3 = 36 with exactly two
+        assert cnt == 125 - 8 - 36
+
+
+def test_erosion_coarse_query_at_cell_centre():
+    # Coarse erosion queries the cell centre 2c+1 (GMorpho.cu:4372), not the
+    # corner 2c. Border cells (x = 0) are outside the base contour
+    # ([1,R-2]^3, Grid.cu:1499-1570), so the coarse result stands there.
+    # Contour voxel p = (2, 2cy, 2cz) in cell x = 1; cell (0, cy, cz): centre
+    # x = 1 is at distance 1 from [2,3], the corner x = 0 at distance 2.
+    R = 16
+    N = 2 * R
+    cy, cz = 7, 8
+    p = (2, 2 * cy, 2 * cz)
+    f = np.ones((N, N, N), bool)
+    f[p[2], p[1], p[0]] = False
+    D = cells_from_fine(f)
+    I = np.zeros_like(D)
+    idx = 0 + R * (cy + R * cz)
+    # r = 3: 1 < (r-1)^2 = 4 clears the border cell; a corner query (4 < 4)
+    # would not
+    C = O.erode(I, D, R, r=3)
+    assert C[idx] == 0
+    # r = 2: 1 < 1 fails, the border cell keeps D
+    C = O.erode(I, D, R, r=2)
+    assert C[idx] == 0xFF
+    # a cell full in I is never cleared by the coarse pass (GMorpho.cu:4349ff)
+    I2 = I.copy()
+    I2[idx] = 0xFF
+    assert O.erode(I2, D, R, r=3)[idx] == 0xFF
+
+
+def test_erosion_fine_result_is_not_anded_with_dilation():
+    # The fine pass writes the surviving sub-voxels of a base-contour cell
+    # whatever D held there (GMorpho.cu:5347-5360 store the byte, no AND).
+    # D: one fine voxel f0 = 2c' (bit 0 of cell c'); I full at c' so the
+    # coarse pass leaves it; K = the 6 neighbours of f0; r = 2 erodes the
+    # 3x3x3 block of every K voxel. The base contour is c' and its six
+    # neighbours. In the +x neighbour the voxels at x = 2c'x + 3 are 1.5 from
+    # the nearest contour box, so they survive although D is empty there:
+    # bits with x offset 1 (0xAA); likewise +y (0xCC) and +z (0xF0). c', the
+    # -x/-y/-z neighbours and every other cell end empty.
+    R = 16
+    c = (7, 8, 6)
+    D = np.zeros(R ** 3, np.uint8)
+    ci = lambda x, y, z: x + R * (y + R * z)  # noqa: E731
+    D[ci(*c)] = 0x01
+    I = np.zeros_like(D)
+    I[ci(*c)] = 0xFF
+    C = O.erode(I, D, R, r=2)
+    want = np.zeros_like(D)
+    want[ci(c[0] + 1, c[1], c[2])] = 0xAA
+    want[ci(c[0], c[1] + 1, c[2])] = 0xCC
+    want[ci(c[0], c[1], c[2] + 1)] = 0xF0
+    assert np.array_equal(C, want)
+
+
+def test_contour_K_hand_bytes():
+    # TagVoxelsContourByErosion (Grid.cu:1230-1327) on hand-built 3x3x3
+    # neighbourhoods; expected bytes written out by hand.
+    R = 8
+    ci = lambda x, y, z: x + R * (y + R * z)  # noqa: E731
+    c = (3, 4, 3)
+    # one full cell: each face neighbour gets the 4 voxels touching the face,
+    # diagonal neighbours and the full cell itself get nothing
+    D = np.zeros(R ** 3, np.uint8)
+    D[ci(*c)] = 0xFF
+    K = O.contour_K(D, R)
+    want = np.zeros_like(D)
+    want[ci(c[0] - 1, c[1], c[2])] = 0xAA
+    want[ci(c[0] + 1, c[1], c[2])] = 0x55
+    want[ci(c[0], c[1] - 1, c[2])] = 0xCC
+    want[ci(c[0], c[1] + 1, c[2])] = 0x33
+    want[ci(c[0], c[1], c[2] - 1)] = 0xF0
+    want[ci(c[0], c[1], c[2] + 1)] = 0x0F
+    assert np.array_equal(K, want)
+    # one voxel (bit 0): its neighbours inside the cell (bits 1, 2, 4) and
+    # across the -x, -y, -z faces (bit 1, 2, 4 of those cells)
+    D = np.zeros(R ** 3, np.uint8)
+    D[ci(*c)] = 0x01
+    K = O.contour_K(D, R)
+    want = np.zeros_like(D)
+    want[ci(*c)] = 0x16
+    want[ci(c[0] - 1, c[1], c[2])] = 0x02
+    want[ci(c[0], c[1] - 1, c[2])] = 0x04
+    want[ci(c[0], c[1], c[2] - 1)] = 0x10
+    assert np.array_equal(K, want)
+    # a full cell less voxel 0: the hole itself, and in the -x neighbour the
+    # face voxels next to set voxels (bits 3, 5, 7; bit 1 faces the hole)
+    D = np.zeros(R ** 3, np.uint8)
+    D[ci(*c)] = 0xFE
+    K = O.contour_K(D, R)
+    assert K[ci(*c)] == 0x01
+    assert K[ci(c[0] - 1, c[1], c[2])] == 0xA8
+    assert K[ci(c[0] + 1, c[1], c[2])] == 0x55
+    # cells on the grid border ([1,R-2]^3 only) are never tagged
+    D = np.zeros(R ** 3, np.uint8)
+    D[ci(1, 4, 4)] = 0xFF
+    K = O.contour_K(D, R)
+    assert K[ci(0, 4, 4)] == 0 and K[ci(2, 4, 4)] == 0x55
+
+
 def test_contour_K_definition():
     R = 16
     D = random_blobs(R, seed=9, rmin=3, rmax=7, margin=2)

@@ -17,6 +17,7 @@ MESHGEN_LIB = os.path.join(LIBDIR, "libgmorpho_meshgen.so")
 
 GRID_INPUT, GRID_DILATION, GRID_CONTOUR, GRID_CLOSING = 0, 1, 2, 3
 SE_SPHERE, SE_CUBE = 0, 1
+BRUSH_NONE, BRUSH_MAX, BRUSH_MIN, BRUSH_ZERO = 0, 1, 2, 3
 
 _hip = None
 _meshgen = None
@@ -181,6 +182,21 @@ class Context:
         lat = np.ascontiguousarray(lat, np.float32)
         self._call("gm_scale_lattice_f32", _ptr(lat), L)
 
+    # ScaleField editing: UpdateGridGlobalScale, AddPoint + UpdateGrid, GetScaleGridCPU
+    def scale_fill(self, value):
+        self._call("gm_scale_grid_fill", int(value))
+
+    def scale_brush(self, pos, scale, support, global_scale=0.0, mode=BRUSH_MAX):
+        p = np.ascontiguousarray(pos, np.float32)
+        f = ctypes.c_float
+        self._call("gm_scale_brush", _ptr(p), f(scale), f(support), f(global_scale), int(mode))
+
+    def scale_grid_get(self):
+        N = 2 * self.info()[2][0]
+        out = np.zeros(N ** 3, np.int8)
+        self._call("gm_scale_grid_get", _ptr(out))
+        return out
+
     def dilate(self, se=SE_SPHERE):
         self._call("gm_dilate", se)
 
@@ -205,6 +221,10 @@ class Context:
         T = np.zeros((nt, 3), np.uint32)
         self._call("gm_mesh_copy", _ptr(V), _ptr(N), _ptr(T))
         return V, N, T
+
+    def filter_normals(self, iters):
+        """Bilateral normal filtering of the extracted mesh (ExtractClosingMesh)."""
+        self._call("gm_filter_normals", int(iters))
 
     def build_pyramid(self, which):
         self._call("gm_build_pyramid", which)

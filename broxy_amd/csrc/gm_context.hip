@@ -45,6 +45,10 @@ struct gm_ctx {
   int rmode = 0, r = 0, rlo = 0, rhi = 0, L = 0;
   int8_t* dGrid = nullptr;
   size_t capGrid = 0;  // bytes of dGrid
+  // counts of the clamped radius values of dGrid (device and host copies),
+  // kept by the fill and the brush so the range needs no grid scan
+  unsigned long long* dHist = nullptr;
+  unsigned long long hHist[128] = {};
   int32_t* dLat = nullptr;
   long long latMaxAbs = 0;  // largest |fixed-point lattice value|
   // per-voxel radii by value: radius planes, per-brick value sets, masked
@@ -67,6 +71,11 @@ struct gm_ctx {
   u32* hTotals = nullptr;
   bool meshPending = false;
   int Pmc = 0;
+  // bilateral normal filtering: incidence lists and a second normal buffer
+  u32* bilScratch = nullptr;
+  size_t bilWords = 0;
+  float* bilN = nullptr;
+  size_t bilNCap = 0;  // vertices of bilN
   // pyramid
   u64* dPyr = nullptr;
   uint64_t capPyr = 0;  // texels of dPyr
@@ -386,6 +395,33 @@ int ensure_var_buffers(gm_ctx* c) {
   return GM_OK;
 }
 
+// Device radius grid (N^3 signed chars) and its value histogram.
+int ensure_scale_grid(gm_ctx* c) {
+  const size_t n = (size_t)c->N * c->N * c->N;
+  if (n > c->capGrid) {
+    dfree(c->dGrid);
+    c->capGrid = 0;
+    HIPCALL(c, hipMalloc(&c->dGrid, n));
+    c->capGrid = n;
+  }
+  if (!c->dHist) HIPCALL(c, hipMalloc(&c->dHist, sizeof(c->hHist)));
+  return GM_OK;
+}
+int upload_hist(gm_ctx* c) {
+  HIPCALL(c, hipMemcpyAsync(c->dHist, c->hHist, sizeof(c->hHist), hipMemcpyHostToDevice, c->stream));
+  return GM_OK;
+}
+void set_range_from_hist(gm_ctx* c) {
+  int lo = 127, hi = 2;
+  for (int v = 2; v < 128; ++v)
+    if (c->hHist[v]) {
+      lo = std::min(lo, v);
+      hi = std::max(hi, v);
+    }
+  c->rlo = std::min(lo, hi);
+  c->rhi = hi;
+}
+
 int do_dilate(gm_ctx* c, int se) {
   int rc = check_radius(c);
   if (rc) return rc;
@@ -461,6 +497,9 @@ int gm_destroy(gm_ctx* c) {
   dfree(c->dLat);
   dfree(c->dTables);
   dfree(c->dCube);
+  dfree(c->bilScratch);
+  dfree(c->bilN);
+  dfree(c->dHist);
   for (auto& e : c->ev) hipEventDestroy(e);
   if (c->hTotals) hipHostFree(c->hTotals);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -630,26 +669,109 @@ int gm_scale_grid_i8(gm_ctx* c, const int8_t* grid) {
   if (!grid) return fail(c, GM_ERR_ARG, "null scale grid");
   hipSetDevice(c->device);
   size_t n = (size_t)c->N * c->N * c->N;
-  if (n > c->capGrid) {
-    dfree(c->dGrid);
-    c->capGrid = 0;
-    HIPCALL(c, hipMalloc(&c->dGrid, n));
-    c->capGrid = n;
-  }
+  rc = ensure_scale_grid(c);
+  if (rc) return rc;
   HIPCALL(c, hipMemcpyAsync(c->dGrid, grid, n, hipMemcpyHostToDevice, c->stream));
-  int lo = 127, hi = 2;
+  std::memset(c->hHist, 0, sizeof(c->hHist));
   for (size_t i = 0; i < n; ++i) {
     int v = grid[i];
-    v = v < 2 ? 2 : (v > 127 ? 127 : v);
-    lo = std::min(lo, v);
-    hi = std::max(hi, v);
+    ++c->hHist[v < 2 ? 2 : (v > 127 ? 127 : v)];
   }
+  rc = upload_hist(c);
+  if (rc) return rc;
   HIPCALL(c, hipStreamSynchronize(c->stream));
   c->rmode = 1;
-  c->rlo = lo;
-  c->rhi = hi;
+  set_range_from_hist(c);
   c->meshPending = false;
   c->rpValid = false;
+  return GM_OK;
+}
+
+int gm_scale_grid_fill(gm_ctx* c, int value) {
+  int rc = need_grid(c);
+  if (rc) return rc;
+  if (value < -128 || value > 127) return fail(c, GM_ERR_ARG, "radius grid value must fit a signed char");
+  hipSetDevice(c->device);
+  rc = ensure_scale_grid(c);
+  if (rc) return rc;
+  const size_t n = (size_t)c->N * c->N * c->N;
+  HIPCALL(c, launch_scale_fill(c->dGrid, n, (int8_t)value, c->stream));
+  std::memset(c->hHist, 0, sizeof(c->hHist));
+  c->hHist[value < 2 ? 2 : (value > 127 ? 127 : value)] = n;
+  rc = upload_hist(c);
+  if (rc) return rc;
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  c->rmode = 1;
+  set_range_from_hist(c);
+  c->meshPending = false;
+  c->rpValid = false;
+  return GM_OK;
+}
+
+int gm_scale_brush(gm_ctx* c, const float position[3], float scale, float support, float global_scale, int mode) {
+  int rc = need_grid(c);
+  if (rc) return rc;
+  if (!position) return fail(c, GM_ERR_ARG, "null brush position");
+  if (mode < GM_BRUSH_NONE || mode > GM_BRUSH_ZERO) return fail(c, GM_ERR_ARG, "unknown brush mode");
+  if (!(c->cell > 0.f)) return fail(c, GM_ERR_STATE, "grid attributes not set");
+  hipSetDevice(c->device);
+  if (c->rmode != 1) {
+    // the reference's scale grid always exists: a uniform radius is a filled grid
+    rc = gm_scale_grid_fill(c, c->rmode == 0 ? std::min(c->r, 127) : 0);
+    if (rc) return rc;
+  }
+  if (mode == GM_BRUSH_NONE) return GM_OK;
+  ScaleBrush br{};
+  const float fc = 0.5f * c->cell;  // the scale field's cell: half a grid cell
+  for (int a = 0; a < 3; ++a) {
+    br.bmin[a] = c->bmin[a];
+    br.pos[a] = position[a];
+  }
+  br.cell = fc;
+  br.scale = scale;
+  volatile float rt = std::sqrt(1.f - std::sqrt(0.5f));
+  br.h = (support - scale) / (float)rt;
+  br.global_scale = global_scale;
+  br.mode = mode;
+  // voxels whose sample point can lie closer than scale + h
+  const double reach = (double)scale + (double)br.h;
+  int box[6] = {0, 0, 0, 0, 0, 0};
+  bool empty = !(reach > 0.0);
+  for (int a = 0; a < 3 && !empty; ++a) {
+    const double lo = std::ceil(((double)position[a] - c->bmin[a] - reach) / fc) - 1;
+    const double hi = std::floor(((double)position[a] - c->bmin[a] + reach) / fc) + 1;
+    const int l = (int)std::max(0.0, std::min(lo, (double)c->N));
+    const int h = (int)std::max(-1.0, std::min(hi, (double)c->N - 1));
+    if (h < l) empty = true;
+    box[a] = l;
+    box[3 + a] = h - l + 1;
+  }
+  if (empty) return GM_OK;
+  HIPCALL(c, launch_scale_brush(c->dGrid, c->N, br, box, c->dHist, c->stream));
+  HIPCALL(c, hipMemcpyAsync(c->hHist, c->dHist, sizeof(c->hHist), hipMemcpyDeviceToHost, c->stream));
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  const int oldHi = c->rhi;
+  set_range_from_hist(c);
+  c->meshPending = false;
+  if (c->rpValid && c->capVarR >= c->R && c->rhi <= 31 && oldHi <= 31) {
+    // rebuild the radius planes of the bricks the brush touched only
+    const int bb[6] = {box[0] / 32, box[1] / 8, box[2] / 8, (box[0] + box[3] - 1) / 32 - box[0] / 32 + 1,
+                       (box[1] + box[4] - 1) / 8 - box[1] / 8 + 1, (box[2] + box[5] - 1) / 8 - box[2] / 8 + 1};
+    HIPCALL(c, launch_radius_planes(radius_field(c), c->N, c->rp, c->rvset, c->stream, bb));
+  } else {
+    c->rpValid = false;
+  }
+  return GM_OK;
+}
+
+int gm_scale_grid_get(gm_ctx* c, int8_t* grid) {
+  int rc = need_grid(c);
+  if (rc) return rc;
+  if (!grid) return fail(c, GM_ERR_ARG, "null buffer");
+  if (c->rmode != 1) return fail(c, GM_ERR_STATE, "the radius field is not a scale grid");
+  hipSetDevice(c->device);
+  HIPCALL(c, hipMemcpyAsync(grid, c->dGrid, (size_t)c->N * c->N * c->N, hipMemcpyDeviceToHost, c->stream));
+  HIPCALL(c, hipStreamSynchronize(c->stream));
   return GM_OK;
 }
 
@@ -783,6 +905,32 @@ int gm_mesh_copy(gm_ctx* c, float* V, float* Nrm, uint32_t* T) {
   if (V && nv) HIPCALL(c, hipMemcpyAsync(V, c->mc.V, (size_t)nv * 12, hipMemcpyDeviceToHost, c->stream));
   if (Nrm && nv) HIPCALL(c, hipMemcpyAsync(Nrm, c->mc.Nrm, (size_t)nv * 12, hipMemcpyDeviceToHost, c->stream));
   if (T && nt) HIPCALL(c, hipMemcpyAsync(T, c->mc.T, (size_t)nt * 12, hipMemcpyDeviceToHost, c->stream));
+  HIPCALL(c, hipStreamSynchronize(c->stream));
+  return GM_OK;
+}
+
+int gm_filter_normals(gm_ctx* c, int iters) {
+  if (!c) return GM_ERR_ARG;
+  if (iters < 0) return fail(c, GM_ERR_ARG, "negative iteration count");
+  uint32_t nv = 0, nt = 0;
+  int rc = gm_mesh_counts(c, &nv, &nt);
+  if (rc) return rc;
+  if (iters == 0 || nv == 0) return GM_OK;
+  const size_t words = bilateral_scratch_words(nv, nt);
+  if (words > c->bilWords) {
+    dfree(c->bilScratch);
+    c->bilWords = 0;
+    HIPCALL(c, hipMalloc(&c->bilScratch, words * 4));
+    c->bilWords = words;
+  }
+  if (nv > c->bilNCap) {
+    dfree(c->bilN);
+    c->bilNCap = 0;
+    HIPCALL(c, hipMalloc(&c->bilN, (size_t)nv * 12));
+    c->bilNCap = nv;
+  }
+  hipError_t e = launch_bilateral(c->mc.V, c->mc.Nrm, c->mc.T, nv, nt, iters, c->bilScratch, c->bilN, c->stream);
+  if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("normal filtering: ") + hipGetErrorString(e));
   HIPCALL(c, hipStreamSynchronize(c->stream));
   return GM_OK;
 }

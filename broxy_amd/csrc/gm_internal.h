@@ -209,8 +209,28 @@ size_t batch_queue_cap(const BatchMorphJob* jobs, int n);
 hipError_t launch_close_batched(const BatchMorphJob* jobs, int n, void* hBlob, void* dBlob, u32* queue, int cap,
                                 int* qsel, hipStream_t s);
 // Radius planes and per-brick value sets of a per-voxel radius field
-// (values clamped to [2,31]; the caller has checked the range).
-hipError_t launch_radius_planes(const RadiusField& rf, int N, u32* rp, u32* rvset, hipStream_t s);
+// (values clamped to [2,31]; the caller has checked the range). brickBox
+// (optional): {w0, by0, bz0, nw, nby, nbz}, the bricks (1 word x 8 rows x 8
+// planes) to rebuild; all of them when null.
+hipError_t launch_radius_planes(const RadiusField& rf, int N, u32* rp, u32* rvset, hipStream_t s,
+                                const int* brickBox = nullptr);
+
+// ---- ScaleField editing (gm_scale.hip)
+// One brush point (ScaleField::UpdateScaleGrid): fine voxel i samples at
+// bmin + cell * i; h = (support - scale) / sqrt(1 - sqrt(0.5)); mode 1 MAX,
+// 2 MIN, 3 ZERO.
+struct ScaleBrush {
+  float bmin[3];
+  float cell;
+  float pos[3];
+  float scale, h, global_scale;
+  int mode;
+};
+hipError_t launch_scale_fill(int8_t* grid, size_t n, int8_t value, hipStream_t s);
+// box: {x0, y0, z0, nx, ny, nz} of the voxels that can change; hist: 128
+// counts of clamped radius values, updated by the changed voxels.
+hipError_t launch_scale_brush(int8_t* grid, int N, const ScaleBrush& br, const int box[6],
+                              unsigned long long* hist, hipStream_t s);
 
 // Host check that the chord tables compiled into the streaming kernels equal
 // the ones build_se_table produces.
@@ -273,6 +293,13 @@ struct BatchMCJob {
 };
 size_t batch_mc_bytes(int n);
 hipError_t launch_mc_batched(const BatchMCJob* jobs, int n, void* hBlob, void* dBlob, u32* hTotals, hipStream_t s);
+
+// ---- bilateral normal filtering (gm_normals.hip): iters iterations on the
+// unit normals Nrm of a mesh (V, T) with nv vertices and nt triangles.
+// scratch holds bilateral_scratch_words(nv, nt) words, Ntmp nv * 3 floats.
+size_t bilateral_scratch_words(u32 nv, u32 nt);
+hipError_t launch_bilateral(const float* V, float* Nrm, const u32* T, u32 nv, u32 nt, int iters, u32* scratch,
+                            float* Ntmp, hipStream_t s);
 
 // ---- reference-format max-mipmap (gm_pyramid.hip)
 hipError_t launch_pyramid(const u32* rows, int R, const RadiusField& rf, u64* levels, hipStream_t s);

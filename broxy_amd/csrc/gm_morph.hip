@@ -311,17 +311,20 @@ __device__ __forceinline__ void lattice_word(const RadiusField& rf, int y, int z
 // (s - fx) A(ix) + fx A(ix+1), A the y/z-weighted column sums, and the
 // division by s^3 * 256 (a power of two) is an arithmetic shift.
 __global__ void __launch_bounds__(256) k_radius_planes(RadiusField rf, int N, int W, u32* __restrict__ rp,
-                                                       u32* __restrict__ vset) {
+                                                       u32* __restrict__ vset, int bw0, int bnw, int by0, int bnby,
+                                                       int bz0, int bnbz) {
   const int NB = N / 8;
-  const size_t nbricks = (size_t)NB * NB * W;
-  // one wave per brick, lane = row + 8 * plane: the 5 plane stores of a wave
-  // are each one contiguous 256-byte brick
-  const size_t b = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (b >= nbricks) return;
+  // one wave per brick of the box (bricks bw0.., by0.., bz0..), lane = row +
+  // 8 * plane: the 5 plane stores of a wave are each one contiguous 256-byte
+  // brick
+  const size_t i = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= (size_t)bnw * bnby * bnbz) return;
   const int lane = lane_id();
-  const int w = (int)(b % W);
-  const size_t t = b / W;
-  const int y = 8 * (int)(t % NB) + (lane & 7), z = 8 * (int)(t / NB) + (lane >> 3);
+  const int w = bw0 + (int)(i % bnw);
+  const size_t t = i / bnw;
+  const int by = by0 + (int)(t % bnby), bz = bz0 + (int)(t / bnby);
+  const size_t b = ((size_t)bz * NB + by) * W + w;
+  const int y = 8 * by + (lane & 7), z = 8 * bz + (lane >> 3);
   const size_t plane = (size_t)N * N * W;
   u32 set = 0u;
   u32 p[5] = {0u, 0u, 0u, 0u, 0u};
@@ -692,10 +695,17 @@ hipError_t launch_pass_var(int pass, MorphArgs a, const MorphBuffers& b, int vlo
 
 }  // namespace
 
-hipError_t launch_radius_planes(const RadiusField& rf, int N, u32* rp, u32* rvset, hipStream_t s) {
+hipError_t launch_radius_planes(const RadiusField& rf, int N, u32* rp, u32* rvset, hipStream_t s,
+                                const int* brickBox) {
   if (rf.mode != 1 && rf.mode != 2) return hipErrorInvalidValue;
-  const size_t bricks = (size_t)(N / 8) * (N / 8) * (N / 32);
-  hipLaunchKernelGGL(k_radius_planes, dim3((unsigned)((bricks + 3) / 4)), dim3(256), 0, s, rf, N, N / 32, rp, rvset);
+  const int NB = N / 8, W = N / 32;
+  int bx[6] = {0, 0, 0, W, NB, NB};
+  if (brickBox)
+    for (int k = 0; k < 6; ++k) bx[k] = brickBox[k];
+  const size_t bricks = (size_t)bx[3] * bx[4] * bx[5];
+  if (bricks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_radius_planes, dim3((unsigned)((bricks + 3) / 4)), dim3(256), 0, s, rf, N, W, rp, rvset, bx[0],
+                     bx[3], bx[1], bx[4], bx[2], bx[5]);
   return hipGetLastError();
 }
 

@@ -6,19 +6,23 @@
 //   GMorpho            GMorpho/GMorpho.h:48-170 (Load, Update, Dilate*, Erode*,
 //                      ExtractClosingMesh, bbox/res/cell_size, setters)
 //   ScaleField         GMorpho/ScaleField.h:33-140 (Init, global scale,
-//                      UpdateGridGlobalScale); per-voxel radii are given as a
-//                      host int8 grid or an FP32 lattice instead of brush points
-//   Mesh               Common/Mesh.h:47 (P, N, T vectors)
+//                      UpdateGridGlobalScale, AddPoint + UpdateGrid brush
+//                      strokes applied on the device); per-voxel radii may
+//                      also be given as a host int8 grid or an FP32 lattice
+//   Mesh               Common/Mesh.h:47 (P, N, T vectors; OFF and PN load and
+//                      store, 580-662; recomputeNormals, 297-306)
 //   AxisAlignedBoundingBox  Common/BoundingVolume.h:104 (min, max)
 // Errors surface as GMorpho::Exception ("GMorpho Error: ..."), as in the
-// reference (GMorpho.h:52). Methods that the reference drives through OpenGL
-// or that this build does not carry (frame-field dilation, bilateral normal
-// filtering) throw instead of silently doing something else.
+// reference (GMorpho.h:52). The frame-field (rotated cube) dilation, which
+// this build does not carry, throws instead of silently doing something else.
 #ifndef GMORPHO_HPP
 #define GMORPHO_HPP
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <fstream>
 #include <string>
 #include <utility>
 #include <vector>
@@ -57,6 +61,14 @@ class AxisAlignedBoundingBox {
 
 class Mesh {
  public:
+  class Exception {
+   public:
+    explicit Exception(const std::string& msg) : msg_(msg) {}
+    const std::string& msg() const { return msg_; }
+
+   private:
+    std::string msg_;
+  };
   typedef Vec3ui Triangle;
   std::vector<Vec3f>& P() { return P_; }
   std::vector<Vec3f>& N() { return N_; }
@@ -70,7 +82,114 @@ class Mesh {
     T_.clear();
   }
 
+  // Mesh::recomputeNormals (Common/Mesh.h:297-306): per vertex, the sum of
+  // the unit normals of its triangles (in triangle order), normalized; a zero
+  // vector stays zero.
+  void recomputeNormals() {
+    N_.assign(P_.size(), Vec3f());
+    for (const Triangle& t : T_) {
+      if (t[0] >= P_.size() || t[1] >= P_.size() || t[2] >= P_.size()) continue;
+      Vec3f n = unit(cross(sub(P_[t[1]], P_[t[0]]), sub(P_[t[2]], P_[t[0]])));
+      for (int j = 0; j < 3; ++j)
+        for (int a = 0; a < 3; ++a) N_[t[j]][a] += n[a];
+    }
+    for (Vec3f& n : N_) n = unit(n);
+  }
+
+  // Mesh::load / store (Common/Mesh.h:580-600): by the file name's suffix.
+  void load(const std::string& filename) {
+    const std::string f = suffix(filename);
+    if (f == "off") loadOFF(filename);
+    else if (f == ".pn") loadPN(filename);
+    else throw Exception("Error loading mesh: unrecognized file format.");
+  }
+  void store(const std::string& filename) const {
+    const std::string f = suffix(filename);
+    if (f == "off") storeOFF(filename);
+    else if (f == ".pn") storePN(filename);
+    else throw Exception("Error storing mesh: unrecognized file format.");
+  }
+
+  // Mesh::loadOFF (Common/Mesh.h:602-622): "OFF", counts, positions, then
+  // faces of s vertices each, fan-triangulated as (v0, v[j-1], v[j]); then
+  // recomputeNormals. Indices out of range and truncated files are errors.
+  void loadOFF(const std::string& filename) {
+    clear();
+    std::ifstream in(filename.c_str());
+    if (!in) throw Exception("Error loading OFF file: " + filename);
+    std::string head;
+    unsigned long long nv = 0, nf = 0, ne = 0;
+    in >> head >> nv >> nf >> ne;
+    if (!in || head != "OFF") throw Exception("Error loading OFF file: bad header in " + filename);
+    P_.resize(nv);
+    for (auto& p : P_) in >> p[0] >> p[1] >> p[2];
+    for (unsigned long long i = 0; i < nf && in; ++i) {
+      unsigned int sz = 0;
+      in >> sz;
+      std::vector<unsigned int> v(sz);
+      for (auto& x : v) in >> x;
+      for (unsigned int x : v)
+        if (x >= nv) throw Exception("Error loading OFF file: vertex index out of range in " + filename);
+      for (unsigned int j = 2; j < sz; ++j) T_.push_back(Triangle(v[0], v[j - 1], v[j]));
+    }
+    if (!in) throw Exception("Error loading OFF file: truncated " + filename);
+    recomputeNormals();
+  }
+  // Mesh::storeOFF (Common/Mesh.h:624-636).
+  void storeOFF(const std::string& filename) const {
+    std::ofstream out(filename.c_str());
+    if (!out) throw Exception("Error storing OFF file: " + filename);
+    out.precision(9);
+    out << "OFF\n" << P_.size() << " " << T_.size() << " 0\n";
+    for (const auto& p : P_) out << p[0] << " " << p[1] << " " << p[2] << "\n";
+    for (const auto& t : T_) out << "3 " << t[0] << " " << t[1] << " " << t[2] << "\n";
+    if (!out) throw Exception("Error storing OFF file: " + filename);
+  }
+  // Mesh::loadPN / storePN (Common/Mesh.h:638-672): raw float32 records
+  // (px py pz nx ny nz) per point, no triangles.
+  void loadPN(const std::string& filename) {
+    clear();
+    FILE* f = std::fopen(filename.c_str(), "rb");
+    if (!f) throw Exception("Error loading a PN file: Cannot read file" + filename);
+    std::fseek(f, 0, SEEK_END);
+    const long bytes = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    const size_t n = bytes > 0 ? (size_t)bytes / (6 * sizeof(float)) : 0;
+    std::vector<float> pn(6 * n);
+    const size_t got = n ? std::fread(pn.data(), sizeof(float), 6 * n, f) : 0;
+    std::fclose(f);
+    if (got != 6 * n) throw Exception("Error loading a PN file: Cannot read file" + filename);
+    P_.resize(n);
+    N_.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+      P_[i] = Vec3f(pn[6 * i], pn[6 * i + 1], pn[6 * i + 2]);
+      N_[i] = Vec3f(pn[6 * i + 3], pn[6 * i + 4], pn[6 * i + 5]);
+    }
+  }
+  void storePN(const std::string& filename) const {
+    FILE* f = std::fopen(filename.c_str(), "wb");
+    if (!f) throw Exception("Error storing a PN file: Cannot open file" + filename);
+    for (size_t i = 0; i < P_.size(); ++i) {
+      const Vec3f n = i < N_.size() ? N_[i] : Vec3f();
+      const float buf[6] = {P_[i][0], P_[i][1], P_[i][2], n[0], n[1], n[2]};
+      std::fwrite(buf, sizeof(float), 6, f);
+    }
+    std::fclose(f);
+  }
+
  private:
+  static std::string suffix(const std::string& f) { return f.size() >= 3 ? f.substr(f.size() - 3) : f; }
+  static Vec3f sub(const Vec3f& a, const Vec3f& b) { return Vec3f(a[0] - b[0], a[1] - b[1], a[2] - b[2]); }
+  static Vec3f cross(const Vec3f& a, const Vec3f& b) {
+    return Vec3f(a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]);
+  }
+  // Vec3::normalize (Common/Vec3.h:208-217): times 1/length, zero stays zero.
+  static Vec3f unit(const Vec3f& a) {
+    const float l = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    if (l == 0.f) return a;
+    const float inv = 1.f / l;
+    return Vec3f(a[0] * inv, a[1] * inv, a[2] * inv);
+  }
   std::vector<Vec3f> P_, N_;
   std::vector<Triangle> T_;
 };
@@ -79,6 +198,10 @@ class Mesh {
 // is not part of this build; the type exists so that Update's signature is
 // the reference's.
 class FrameField {};
+
+// EditMode (ScaleField.h:32).
+typedef enum { NO_EDITION = GM_BRUSH_NONE, MAX_BRUSH = GM_BRUSH_MAX, MIN_BRUSH = GM_BRUSH_MIN,
+               ZERO_BRUSH = GM_BRUSH_ZERO } EditMode;
 
 class ScaleField {
  public:
@@ -90,8 +213,31 @@ class ScaleField {
    private:
     std::string msg_;
   };
-  enum Mode { UNIFORM = 0, GRID_I8 = 1, LATTICE_F32 = 2 };
+  // ScaleField::ScalePoint (ScaleField.h:43-80).
+  class ScalePoint {
+   public:
+    ScalePoint() {}
+    ScalePoint(const Vec3f& position, float scale) : position_(position), scale_(scale), support_(scale) {}
+    ScalePoint(const Vec3f& position, float scale, float support)
+        : position_(position), scale_(scale), support_(support) {}
+    const Vec3f& position() const { return position_; }
+    Vec3f& position() { return position_; }
+    float scale() const { return scale_; }
+    float support() const { return support_; }
+    EditMode edit_mode() const { return edit_mode_; }
+    void set_position(const Vec3f& position) { position_ = position; }
+    void set_scale(float scale) { scale_ = scale; }
+    void set_support(float support) { support_ = support; }
+    void set_edit_mode(EditMode edit_mode) { edit_mode_ = edit_mode; }
 
+   private:
+    Vec3f position_;
+    float scale_ = 0.f, support_ = 0.f;
+    EditMode edit_mode_ = MAX_BRUSH;
+  };
+  enum Mode { UNIFORM = 0, GRID_I8 = 1, LATTICE_F32 = 2, BRUSHED = 3 };
+
+  ScaleField() : id_(next_id()) {}
   float cell_size() const { return cell_size_; }
   const Vec3f& bbox_min() const { return bbox_min_; }
   const Vec3f& bbox_max() const { return bbox_max_; }
@@ -106,26 +252,36 @@ class ScaleField {
     bbox_max_ = bbox_max;
     res_ = res;
     cell_size_ = cell_size;
-    mode_ = UNIFORM;
-    grid_.clear();
-    lattice_.clear();
+    reset(UNIFORM);
   }
   // ScaleField::UpdateGridGlobalScale (ScaleField.cu:143-146): every fine
   // voxel gets floor(global_scale / cell_size).
-  void UpdateGridGlobalScale() {
-    mode_ = UNIFORM;
-    grid_.clear();
-    lattice_.clear();
+  void UpdateGridGlobalScale() { reset(UNIFORM); }
+  // ScaleField::AddPoint (ScaleField.h:102-109).
+  void AddPoint(const Vec3f& position, float scale) { points_.push_back(ScalePoint(position, scale)); }
+  void AddPoint(const Vec3f& position, float scale, float support) {
+    points_.push_back(ScalePoint(position, scale, support));
+  }
+  const std::vector<ScalePoint>& points() const { return points_; }
+  std::vector<ScalePoint>& points() { return points_; }
+  int GetNumberOfPoints() const { return (int)points_.size(); }
+  // ScaleField::UpdateGrid (ScaleField.cu:148-192): the last point is
+  // brushed into the grid (on the device, at the next GMorpho call that
+  // reads the field; strokes accumulate until UpdateGridGlobalScale).
+  void UpdateGrid() {
+    if (points_.empty()) return;
+    if (mode_ != BRUSHED) reset(BRUSHED);
+    strokes_.push_back(points_.back());
   }
   // Per-fine-voxel radii (the reference's scale_grid_, ScaleField.h:135),
   // N^3 signed chars, x fastest.
   void SetScaleGrid(std::vector<int8_t> grid) {
-    mode_ = GRID_I8;
+    reset(GRID_I8);
     grid_ = std::move(grid);
   }
   // Radii sampled from an L^3 FP32 lattice (trilinear, floor, clamp [2,127]).
   void SetScaleLattice(std::vector<float> lattice, int L) {
-    mode_ = LATTICE_F32;
+    reset(LATTICE_F32);
     lattice_ = std::move(lattice);
     L_ = L;
   }
@@ -134,8 +290,23 @@ class ScaleField {
   const std::vector<float>& lattice() const { return lattice_; }
   int lattice_res() const { return L_; }
   int uniform_radius() const { return (int)std::floor(global_scale_ / cell_size_); }
+  // Brush strokes since the last reset, and which field state they belong to.
+  const std::vector<ScalePoint>& strokes() const { return strokes_; }
+  unsigned long long id() const { return id_; }
+  unsigned long long generation() const { return generation_; }
 
  private:
+  static unsigned long long next_id() {
+    static unsigned long long n = 0;
+    return ++n;
+  }
+  void reset(Mode m) {
+    mode_ = m;
+    grid_.clear();
+    lattice_.clear();
+    strokes_.clear();
+    ++generation_;
+  }
   float global_scale_ = 0.f, cell_size_ = 1.f;
   Vec3f bbox_min_, bbox_max_;
   Vec3ui res_;
@@ -143,6 +314,8 @@ class ScaleField {
   std::vector<int8_t> grid_;
   std::vector<float> lattice_;
   int L_ = 0;
+  std::vector<ScalePoint> points_, strokes_;
+  unsigned long long id_ = 0, generation_ = 0;
 };
 
 class GMorpho {
@@ -236,11 +409,12 @@ class GMorpho {
     check(gm_erode(ctx_));
   }
   // GMorpho::ExtractClosingMesh (GMorpho.cu:6814-7001): marching cubes of the
-  // closing grid with area-weighted unit normals, copied into mesh.
+  // closing grid with area-weighted unit normals, then, when bilateral
+  // filtering is on, num_bilateral_iters filtering iterations of the normals
+  // (GMorpho.cu:6957-6978), copied into mesh.
   void ExtractClosingMesh(Mesh& mesh, int num_bilateral_iters = 20) {
-    (void)num_bilateral_iters;
-    if (bilateral_filtering_) throw Exception("bilateral normal filtering is not part of this build");
     check(gm_extract_mesh(ctx_));
+    if (bilateral_filtering_) check(gm_filter_normals(ctx_, num_bilateral_iters));
     uint32_t nv = 0, nt = 0;
     check(gm_mesh_counts(ctx_, &nv, &nt));
     std::vector<float> V(3 * (size_t)nv), Nr(3 * (size_t)nv);
@@ -268,7 +442,24 @@ class GMorpho {
     }
   }
   void apply_scale(const ScaleField& sf) {
+    if (sf.mode() != ScaleField::BRUSHED) brushed_id_ = 0;
     switch (sf.mode()) {
+      case ScaleField::BRUSHED: {
+        // the strokes not yet on the device grid; a new field state starts
+        // from the global scale, as UpdateGridGlobalScale does
+        if (brushed_id_ != sf.id() || brushed_gen_ != sf.generation() || brushed_n_ > sf.strokes().size()) {
+          check(gm_scale_grid_fill(ctx_, std::max(-128, std::min(127, sf.uniform_radius()))));
+          brushed_id_ = sf.id();
+          brushed_gen_ = sf.generation();
+          brushed_n_ = 0;
+        }
+        for (; brushed_n_ < sf.strokes().size(); ++brushed_n_) {
+          const ScaleField::ScalePoint& p = sf.strokes()[brushed_n_];
+          const float pos[3] = {p.position()[0], p.position()[1], p.position()[2]};
+          check(gm_scale_brush(ctx_, pos, p.scale(), p.support(), sf.global_scale(), (int)p.edit_mode()));
+        }
+        break;
+      }
       case ScaleField::GRID_I8:
         check(gm_scale_grid_i8(ctx_, sf.scale_grid_host().data()));
         break;
@@ -286,6 +477,9 @@ class GMorpho {
   float cell_size_ = 0.f, se_size_ = 0.f;
   bool use_asymmetric_closing_ = false, use_frame_field_ = false, bilateral_filtering_ = false;
   Mesh morpho_mesh_;
+  // brush strokes of which field state are on the context's grid
+  unsigned long long brushed_id_ = 0, brushed_gen_ = 0;
+  size_t brushed_n_ = 0;
 };
 
 }  // namespace MorphoGraphics

@@ -88,6 +88,28 @@ int gm_scale_uniform(gm_ctx* ctx, int r_fine);
 int gm_scale_grid_i8(gm_ctx* ctx, const int8_t* grid);
 int gm_scale_lattice_f32(gm_ctx* ctx, const float* lattice, int L);
 
+/* ScaleField editing on the device grid of signed chars (the reference's
+ * scale_grid_, ScaleField.h:135). The scale field's voxel i has its sample
+ * point at bbox_min + (cell_size / 2) * i (ScaleField::Init with twice the
+ * grid's resolution and half its cell, BroxyViewer.cpp:133-136).
+ *  - fill: ScaleField::UpdateGridGlobalScale (ScaleField.cu:143-146), every
+ *    voxel set to value (the reference memsets floor(global_scale /
+ *    cell_size)); the grid becomes the context's radius field.
+ *  - brush: ScaleField::AddPoint + UpdateGrid (ScaleField.h:102-109,
+ *    ScaleField.cu:148-192) for one point, kernel UpdateScaleGrid
+ *    (ScaleField.cu:81-141) with the tukey falloff (cuda_math.h:696-704):
+ *    radius `scale` and support `support` in world units, edit mode
+ *    GM_BRUSH_* (EditMode, ScaleField.h:32). Only voxels closer than
+ *    scale + h are visited; the radius planes of a per-voxel closing are
+ *    rebuilt for the touched bricks only. A context holding a uniform radius
+ *    is first filled with it.
+ *  - get: the grid back to the host (ScaleField::GetScaleGridCPU,
+ *    ScaleField.cu:74-79). */
+enum { GM_BRUSH_NONE = 0, GM_BRUSH_MAX = 1, GM_BRUSH_MIN = 2, GM_BRUSH_ZERO = 3 };
+int gm_scale_grid_fill(gm_ctx* ctx, int value);
+int gm_scale_brush(gm_ctx* ctx, const float position[3], float scale, float support, float global_scale, int mode);
+int gm_scale_grid_get(gm_ctx* ctx, int8_t* grid);
+
 /* GMorpho::DilateBySphereMipmap / DilateByCubeMipmap (GMorpho.cu:3538, 3231):
  * GM_GRID_DILATION from GM_GRID_INPUT. */
 int gm_dilate(gm_ctx* ctx, int se);
@@ -105,6 +127,16 @@ int gm_extract_mesh(gm_ctx* ctx);
 int gm_mesh_counts(gm_ctx* ctx, uint32_t* num_vertices, uint32_t* num_triangles);
 /* Copy the mesh out (GMorpho.cu:6985-6995): V, Nrm float[3*nv], T uint32[3*nt]. */
 int gm_mesh_copy(gm_ctx* ctx, float* V, float* Nrm, uint32_t* T);
+
+/* The bilateral normal filtering of GMorpho::ExtractClosingMesh
+ * (GMorpho.cu:6946-6978, kernels BilateralMeshFiltering and
+ * BilateralMeshFilteringNormalize 6539-6620): num_iters iterations on the
+ * extracted mesh's unit normals, sigma_p = 1, sigma_n = 1e10 for the first
+ * two iterations and 0.2 after (GMorpho::Update passes 200 iterations for the
+ * asymmetric closing and 20 otherwise, GMorpho.cu:7809-7812). Waits for the
+ * extraction. Each vertex adds its triangles' terms in triangle order, so the
+ * result is the same every run (the reference's float atomics are not). */
+int gm_filter_normals(gm_ctx* ctx, int num_iters);
 
 /* Grid::BuildTexMipmaps (Grid.cu:1000-1154) of grid `which` with the current
  * radius field: levels 0..log2(R), level L = (R>>L)^3 64-bit texels (x

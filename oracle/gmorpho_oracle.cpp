@@ -844,6 +844,132 @@ void orc_mc(const uint8_t* C, int R, const float* bmin3, float cell, float* V, f
   }
 }
 
+
+// BilateralMeshFiltering + BilateralMeshFilteringNormalize (GMorpho.cu:6539-6620)
+// iterated as ExtractClosingMesh does (GMorpho.cu:6957-6978): sigma_p = 1 and
+// sigma_n = 1e10 for the first two iterations, 0.2 after. Per triangle and
+// corner v with other corners a, b (in corner order), sumN = w(v,a) n_a +
+// w(v,b) n_b with w(v,u) = exp(-(|p_v-p_u|^2 / sigma_p^2 + |n_v-n_u|^2 /
+// sigma_n^2)) (exp as exp_weight); the sums of a vertex are added in triangle
+// order (the reference's float atomics, serialized), then normalized. A zero
+// sum stays zero (the reference would divide by zero).
+// exp(x) of a bilateral weight (x <= 0) in float: Cody-Waite reduction
+// x = k ln2 + r with ln2 split as 355/512 + (ln2 - 355/512) (k ln2_hi exact
+// for |k| < 2^15), exp(r) by its Taylor series to degree 7 in Horner form
+// (|r| <= 0.35: truncation below 1e-8 relative), times 2^k. Below -87 the
+// result would leave the normal range and is 0. The HIP kernel evaluates the
+// same operations in the same order, so both give the same bits; the test
+// suite holds this function to numpy's exp within a few ulp.
+static float exp_weight(float x) {
+  if (!(x >= -87.f)) return 0.f;
+  if (x > 0.f) x = 0.f;
+  const float k = std::nearbyint((float)(x * 1.44269504f));
+  volatile float a = k * 0.693359375f;
+  volatile float b = k * -2.12194440e-4f;
+  volatile float r0 = x - (float)a;
+  volatile float r = (float)r0 - (float)b;
+  const float c[8] = {1.f / 5040.f, 1.f / 720.f, 1.f / 120.f, 1.f / 24.f, 1.f / 6.f, 0.5f, 1.f, 1.f};
+  volatile float p = c[0];
+  for (int i = 1; i < 8; ++i) {
+    volatile float m = (float)p * (float)r;
+    p = (float)m + c[i];
+  }
+  return std::ldexp((float)p, (int)k);
+}
+
+void orc_exp_weight(const float* x, float* out, int n) {
+  for (int i = 0; i < n; ++i) out[i] = exp_weight(x[i]);
+}
+
+void orc_bilateral_normals(const float* V, float* Nrm, const uint32_t* T, int nv, int nt, int iters) {
+  std::vector<float> acc((size_t)3 * nv);
+  auto d2 = [](const float* a, const float* b) {
+    volatile float x = a[0] - b[0], y = a[1] - b[1], z = a[2] - b[2];
+    volatile float xx = x * x, yy = y * y, zz = z * z;
+    volatile float s = xx + yy;
+    return (float)(s + zz);
+  };
+  for (int it = 0; it < iters; ++it) {
+    const float sp = 1.f, sn = it < 2 ? 1e10f : 0.2f;
+    const float sp2 = sp * sp, sn2 = sn * sn;
+    std::fill(acc.begin(), acc.end(), 0.f);
+    for (int t = 0; t < nt; ++t) {
+      const uint32_t* tri = T + 3 * (size_t)t;
+      for (int c = 0; c < 3; ++c) {
+        const uint32_t v = tri[c], a = tri[c == 0 ? 1 : 0], b = tri[c == 2 ? 1 : 2];
+        const float *pv = V + 3 * (size_t)v, *pa = V + 3 * (size_t)a, *pb = V + 3 * (size_t)b;
+        const float *nv_ = Nrm + 3 * (size_t)v, *na = Nrm + 3 * (size_t)a, *nb = Nrm + 3 * (size_t)b;
+        volatile float ea = d2(pv, pa) / sp2 + d2(nv_, na) / sn2;
+        volatile float eb = d2(pv, pb) / sp2 + d2(nv_, nb) / sn2;
+        const float wa = exp_weight(-(float)ea), wb = exp_weight(-(float)eb);
+        for (int q = 0; q < 3; ++q) {
+          volatile float s = wa * na[q];
+          volatile float u = wb * nb[q];
+          volatile float sum = s + u;
+          acc[3 * (size_t)v + q] += sum;
+        }
+      }
+    }
+    for (int i = 0; i < nv; ++i) {
+      float* a = &acc[3 * (size_t)i];
+      volatile float xx = a[0] * a[0], yy = a[1] * a[1], zz = a[2] * a[2];
+      volatile float s = xx + yy;
+      const float l = std::sqrt((float)(s + zz));
+      for (int q = 0; q < 3; ++q) Nrm[3 * (size_t)i + q] = l > 0.f ? a[q] / l : a[q];
+    }
+  }
+}
+
+
+// ScaleField::UpdateScaleGrid (ScaleField.cu:81-141) for one brush point over
+// every fine voxel, with tukey (cuda_math.h:696-704) and the edit modes of
+// ScaleField.h:32 (1 MAX, 2 MIN, 3 ZERO). The voxel's sample point is its
+// corner bbox_min + cell * id; h = (support - scale) / sqrt(1 - sqrt(0.5)),
+// all in float. Voxels at distance >= scale + h keep their value.
+void orc_scale_brush(int8_t* grid, int N, const float* bmin3, float cell, const float* pos3, float scale,
+                     float support, float global_scale, int mode) {
+  const float v = scale;
+  volatile float rt = std::sqrt(1.f - std::sqrt(0.5f));
+  volatile float h = (support - v) / (float)rt;
+#pragma omp parallel for schedule(static)
+  for (int z = 0; z < N; ++z)
+    for (int y = 0; y < N; ++y)
+      for (int x = 0; x < N; ++x) {
+        const int id[3] = {x, y, z};
+        float d2 = 0.f;
+        for (int a = 0; a < 3; ++a) {
+          volatile float m = cell * (float)id[a];
+          volatile float u = m + bmin3[a];
+          volatile float r = u - pos3[a];
+          volatile float rr = r * r;
+          volatile float acc = d2 + rr;
+          d2 = acc;
+        }
+        const float dist = std::sqrt(d2);
+        volatile float reach = v + (float)h;
+        if (!(dist < reach)) continue;
+        float distrib;
+        if (dist < v) {
+          distrib = 1.f;
+        } else {
+          volatile float q = (dist - v) / (float)h;
+          volatile float q2 = q * q;
+          volatile float o = 1.f - q2;
+          distrib = o * o;
+        }
+        int8_t& g = grid[(size_t)x + (size_t)N * ((size_t)y + (size_t)N * z)];
+        const float prev = (float)g;
+        const float target = mode == 3 ? global_scale / cell : scale / cell;
+        volatile float keep = prev * (1.f - distrib);
+        volatile float add = distrib * target;
+        float d = std::floor((float)(keep + add));
+        d = std::max(std::min(d, 127.f), 0.f);
+        if (mode == 1) g = (int8_t)std::floor(std::max(d, prev));
+        else if (mode == 2) g = (int8_t)std::floor(std::min(d, prev));
+        else if (mode == 3) g = (int8_t)d;
+      }
+}
+
 int64_t orc_stage_slab(int stage, const uint8_t* I, const uint8_t* D, const uint8_t* C, int R,
                        const int* data_res3, const int8_t* scale, int r, int z0, int z1,
                        uint8_t* out) {

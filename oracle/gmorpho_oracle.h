@@ -73,6 +73,20 @@ void orc_mc(const uint8_t* C, int R, const float* bmin3, float cell, float* V, f
  * case, 16 edge ids per case terminated by -1). */
 void orc_mc_tables(uint16_t* edge256, uint8_t* ntri256, int8_t* tri4096);
 
+/* Bilateral normal filtering of ExtractClosingMesh (GMorpho.cu:6539-6620,
+ * 6957-6978): iters iterations on the unit normals Nrm (in place), each
+ * vertex's sums added in triangle order. */
+void orc_bilateral_normals(const float* V, float* Nrm, const uint32_t* T, int nv, int nt, int iters);
+/* The float exp of the bilateral weights (x <= 0; 0 below -87). */
+void orc_exp_weight(const float* x, float* out, int n);
+
+/* ScaleField::UpdateScaleGrid (ScaleField.cu:81-141): one brush point applied
+ * to the N^3 signed-char radius grid (x fastest) whose voxel i has its corner
+ * at bmin + cell * i; mode 1 MAX_BRUSH, 2 MIN_BRUSH, 3 ZERO_BRUSH
+ * (ScaleField.h:32). */
+void orc_scale_brush(int8_t* grid, int N, const float* bmin3, float cell, const float* pos3, float scale,
+                     float support, float global_scale, int mode);
+
 /* Timed sub-steps for the CPU baseline: run the named stage on the cell slab
  * z in [z0,z1) only. stage 0 = coarse+fine dilation of those cells,
  * 1 = coarse+fine erosion of those cells, 2 = MC classification/counting.

@@ -109,6 +109,30 @@ def mc(C, R, bmin, cell):
     return V, N, T
 
 
+def bilateral_normals(V, N, T, iters):
+    """Filtered copy of the unit normals N after `iters` iterations."""
+    out = np.ascontiguousarray(N, np.float32).copy()
+    lib().orc_bilateral_normals(_ptr(np.ascontiguousarray(V, np.float32)), _ptr(out),
+                                _ptr(np.ascontiguousarray(T, np.uint32)), len(V), len(T), iters)
+    return out
+
+
+def exp_weight(x):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros_like(x)
+    lib().orc_exp_weight(_ptr(x), _ptr(out), len(x))
+    return out
+
+
+def scale_brush(grid, N, bmin, cell, pos, scale, support, global_scale, mode):
+    """Copy of the int8 radius grid after one brush point."""
+    out = np.ascontiguousarray(grid, np.int8).copy()
+    f = ctypes.c_float
+    lib().orc_scale_brush(_ptr(out), N, _ptr(np.ascontiguousarray(bmin, np.float32)), f(cell),
+                          _ptr(np.ascontiguousarray(pos, np.float32)), f(scale), f(support), f(global_scale), mode)
+    return out
+
+
 def mc_tables():
     e = np.zeros(256, np.uint16)
     n = np.zeros(256, np.uint8)

@@ -1,10 +1,18 @@
 // Drives the C++ facade (include/gmorpho.hpp) the way the reference's viewer
 // drives GMorpho (BroxyViewer.cpp:126-157): Load, ScaleField Init + global
-// scale, Update -> Mesh. Writes the closing grid (cell bytes) and the
-// triangle list to files so that a test can compare them with the oracle's
-// record. Usage: facade_closing <out_prefix> [seed]
+// scale, Update -> Mesh. Writes the closing grid (cell bytes), the triangle
+// list, the mesh as OFF and as PN (positions + normals) so that a test can
+// compare them with the oracle's, then reads the OFF file back.
+// Usage: facade_closing <out_prefix> [seed] [mode]
+//   mode sphere (default): ball closing, no normal filtering;
+//   viewer: the viewer's defaults, asymmetric (cube) closing with bilateral
+//           normal filtering (BroxyViewer.cpp:66, 125, 138), 200 iterations;
+//   fixed:  Update(Mesh&, float se_size) (GMorpho.cu:7819-7845);
+//   brush:  ball closing with a field edited by brush strokes (AddPoint +
+//           UpdateGrid, BroxyViewer.cpp:670-671), the radius grid written too.
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "gmorpho.hpp"
@@ -18,6 +26,7 @@ int main(int argc, char** argv) {
     return 2;
   }
   const unsigned long long seed = argc > 2 ? std::strtoull(argv[2], nullptr, 10) : 2017ull;
+  const std::string mode = argc > 3 ? argv[3] : "sphere";
   int nv = 0, nf = 0;
   if (gmg_icosphere_counts(3, &nv, &nf)) return 1;
   const int n = 64;
@@ -34,9 +43,33 @@ int main(int argc, char** argv) {
     sf.set_global_scale(4.25f * cell);
     sf.Init(gmorpho.bbox().min(), gmorpho.bbox().max(), res, cell);
     sf.UpdateGridGlobalScale();
-    gmorpho.set_use_asymmetric_closing(false);
     Mesh mesh;
-    gmorpho.Update(mesh, sf, FrameField());
+    if (mode == "viewer") {
+      gmorpho.set_use_asymmetric_closing(true);
+      gmorpho.set_bilateral_filtering(true);
+      gmorpho.Update(mesh, sf, FrameField());
+    } else if (mode == "brush") {
+      gmorpho.set_use_asymmetric_closing(false);
+      sf.AddPoint(Vec3f(0.5f, 0.5f, 0.5f), 0.12f, 0.2f);
+      sf.UpdateGrid();
+      gmorpho.Update(mesh, sf, FrameField());
+      sf.AddPoint(Vec3f(0.35f, 0.6f, 0.45f), 0.05f, 0.09f);
+      sf.points().back().set_edit_mode(MIN_BRUSH);
+      sf.UpdateGrid();
+      gmorpho.Update(mesh, sf, FrameField());
+      const size_t n = (size_t)res[0] * res[1] * res[2];
+      std::vector<int8_t> g(n);
+      if (gm_scale_grid_get(gmorpho.handle(), g.data())) return 1;
+      FILE* fg = std::fopen((std::string(argv[1]) + "_scale.i8").c_str(), "wb");
+      std::fwrite(g.data(), 1, n, fg);
+      std::fclose(fg);
+    } else if (mode == "fixed") {
+      // se_size in world units: radius floor(2 * se_size / cell_size)
+      gmorpho.Update(mesh, 4.25f * cell);
+    } else {
+      gmorpho.set_use_asymmetric_closing(false);
+      gmorpho.Update(mesh, sf, FrameField());
+    }
     const int R = (int)gmorpho.res()[0];
     std::vector<uint8_t> C((size_t)R * R * R);
     if (gm_grid_get_cells(gmorpho.handle(), GM_GRID_CLOSING, C.data())) return 1;
@@ -51,8 +84,22 @@ int main(int argc, char** argv) {
         std::fwrite(&v, 4, 1, f);
       }
     std::fclose(f);
+    mesh.store(pre + "_mesh.off");
+    mesh.store(pre + "_mesh.pn");
+    Mesh back;
+    back.load(pre + "_mesh.off");
+    bool same = back.P().size() == mesh.P().size() && back.T().size() == mesh.T().size();
+    for (size_t i = 0; same && i < mesh.T().size(); ++i)
+      for (int k = 0; k < 3; ++k) same = same && back.T()[i][k] == mesh.T()[i][k];
+    if (!same) {
+      std::fprintf(stderr, "OFF round trip differs\n");
+      return 1;
+    }
     std::printf("%zu %zu\n", mesh.P().size(), mesh.T().size());
   } catch (const GMorpho::Exception& e) {
+    std::fprintf(stderr, "%s\n", e.msg().c_str());
+    return 1;
+  } catch (const Mesh::Exception& e) {
     std::fprintf(stderr, "%s\n", e.msg().c_str());
     return 1;
   }

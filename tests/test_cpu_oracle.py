@@ -603,3 +603,124 @@ def test_context_without_device_fails_loudly():
         pytest.skip("a HIP device may be present")
     with pytest.raises(B.GMorphoError):
         B.Context(0)
+
+
+# ---------------------------------------------------------------- bilateral normals
+def _pairwise_bilateral(P, N, nbrs, iters):
+    """Edge-pair form of BilateralMeshFiltering (GMorpho.cu:6539-6620) for a
+    closed mesh where every edge lies in exactly two triangles: each vertex
+    sums w(v,u) n_u over its edge neighbours u (twice, which normalization
+    removes), w(v,u) = exp(-(|p_v-p_u|^2 + |n_v-n_u|^2 / sigma_n^2)), with
+    sigma_n = 1e10 for the first two iterations and 0.2 after
+    (GMorpho.cu:6957-6978)."""
+    N = N.astype(np.float64)
+    for it in range(iters):
+        sn2 = (1e10 if it < 2 else 0.2) ** 2
+        out = np.zeros_like(N)
+        for v, us in enumerate(nbrs):
+            for u in us:
+                w = np.exp(-(np.sum((P[v] - P[u]) ** 2) + np.sum((N[v] - N[u]) ** 2) / sn2))
+                out[v] += w * N[u]
+        N = out / np.linalg.norm(out, axis=1, keepdims=True)
+    return N
+
+
+def test_bilateral_tetrahedron_pairwise():
+    P = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float32) * np.float32(0.7)
+    T = np.array([[0, 2, 1], [0, 1, 3], [0, 3, 2], [1, 2, 3]], np.uint32)
+    rng = np.random.default_rng(5)
+    N = rng.normal(size=(4, 3))
+    N = (N / np.linalg.norm(N, axis=1, keepdims=True)).astype(np.float32)
+    nbrs = [[u for u in range(4) if u != v] for v in range(4)]
+    for iters in (1, 2, 3, 5):
+        got = O.bilateral_normals(P, N, T, iters)
+        want = _pairwise_bilateral(P.astype(np.float64), N, nbrs, iters)
+        # float vs double: with sigma_n = 0.2 the exponent reaches ~100, so a
+        # rounding of the exponent moves a weight by ~1e-5 relative
+        assert np.allclose(got, want, atol=2e-5), iters
+    # a vertex's own normal takes no part: with every other normal equal the
+    # result is that normal whatever the vertex held
+    N2 = np.tile(np.array([[0, 0, 1]], np.float32), (4, 1))
+    N2[0] = [1, 0, 0]
+    assert np.allclose(O.bilateral_normals(P, N2, T, 1)[0], [0, 0, 1], atol=1e-7)
+
+
+def test_bilateral_planar_patch_is_fixed():
+    # a flat grid of triangles with equal normals stays unchanged
+    n = 6
+    P = np.array([[x * 0.1, y * 0.1, 0.3] for y in range(n) for x in range(n)], np.float32)
+    T = []
+    for y in range(n - 1):
+        for x in range(n - 1):
+            a = y * n + x
+            T += [[a, a + 1, a + n + 1], [a, a + n + 1, a + n]]
+    T = np.array(T, np.uint32)
+    N = np.tile(np.array([[0, 0, 1]], np.float32), (len(P), 1))
+    assert np.array_equal(O.bilateral_normals(P, N, T, 7), N)
+
+
+# ---------------------------------------------------------------- scale-field brush
+def _brush_case(mode, prev, scale=0.5, support=0.8, gscale=0.125):
+    N = 32
+    cs = 0.0625
+    bmin = np.array([-0.25, 0.0, 0.125], np.float32)
+    pos = np.array([0.77, 1.03, 1.21], np.float32)
+    g0 = np.full(N ** 3, prev, np.int8)
+    g = O.scale_brush(g0, N, bmin, cs, pos, scale, support, gscale, mode).reshape(N, N, N)
+    z, y, x = np.mgrid[0:N, 0:N, 0:N]
+    u = np.stack([x, y, z], -1) * cs + bmin.astype(np.float64)
+    dist = np.linalg.norm(u - pos.astype(np.float64), axis=-1)
+    h = (support - scale) / np.sqrt(1 - np.sqrt(0.5))
+    return g, dist, scale, h, cs
+
+
+def test_scale_brush_max_closed_forms():
+    # ScaleField.cu:81-141, MAX_BRUSH over a uniform field of 4: inside the
+    # brush radius the field takes floor(scale / cell) = 8; beyond scale + h
+    # it is untouched; in the tukey band (cuda_math.h:696-704) it is
+    # floor(prev (1 - t) + t * 8), t = (1 - ((d - v)/h)^2)^2, never below prev
+    g, d, v, h, cs = _brush_case(1, 4)
+    safe = lambda a, b: np.abs(a - b) > 1e-5  # noqa: E731
+    inside = (d < v) & safe(d, v)
+    outside = (d > v + h) & safe(d, v + h)
+    assert inside.sum() > 100 and outside.sum() > 1000
+    assert np.all(g[inside] == 8)
+    assert np.all(g[outside] == 4)
+    band = (d > v) & (d < v + h) & safe(d, v) & safe(d, v + h)
+    t = (1 - ((d - v) / h) ** 2) ** 2
+    val = 4 * (1 - t) + t * 8
+    ok = band & (np.abs(val - np.round(val)) > 1e-4)
+    assert ok.sum() > 100
+    assert np.array_equal(g[ok], np.maximum(np.floor(val[ok]), 4).astype(np.int8))
+    assert np.all((g >= 4) & (g <= 8))
+
+
+def test_scale_brush_min_zero_modes():
+    # MIN_BRUSH from 12 only lowers (to 8 inside); ZERO_BRUSH pulls toward
+    # floor(global_scale / cell) = 2 inside, whatever the field held
+    g, d, v, h, cs = _brush_case(2, 12)
+    inside = (d < v - 1e-5)
+    assert np.all(g[inside] == 8) and np.all((g >= 8) & (g <= 12))
+    assert np.all(g[d > v + h + 1e-5] == 12)
+    g, d, v, h, cs = _brush_case(3, 12)
+    assert np.all(g[d < v - 1e-5] == 2)
+    assert np.all(g[d > v + h + 1e-5] == 12)
+    # NO_EDITION changes nothing
+    g, d, v, h, cs = _brush_case(0, 7)
+    assert np.all(g == 7)
+    # values are clamped to [0, 127]: a brush of 200 cells is 127 inside
+    g, d, v, h, cs = _brush_case(1, 3, scale=0.5, support=0.6, gscale=0.0)
+    g2 = O.scale_brush(np.full(32 ** 3, 3, np.int8), 32, np.zeros(3, np.float32), 0.0025, np.zeros(3, np.float32),
+                       0.5, 0.6, 0.0, 1)
+    assert g2.max() == 127
+
+
+def test_exp_weight_accuracy():
+    # the float exp of the bilateral weights: within 3 ulp of exp over its
+    # range, exactly 1 at 0, 0 below -87
+    x = -np.concatenate([np.linspace(0, 87, 200001), np.geomspace(1e-8, 1e-2, 1000)]).astype(np.float32)
+    got = O.exp_weight(x).astype(np.float64)
+    ref = np.exp(x.astype(np.float64))
+    ulp = np.spacing(ref.astype(np.float32)).astype(np.float64)
+    assert np.max(np.abs(got - ref) / ulp) <= 3.0
+    assert O.exp_weight(np.array([0.0, -88.0, -1e30], np.float32)).tolist() == [1.0, 0.0, 0.0]

@@ -381,6 +381,37 @@ def test_empty_grid(ctx):
     assert len(V) == 0 and len(T) == 0
 
 
+def _case91_grid(R):
+    """Fine pattern (x+y+z) % 4 != 2 in an inner box: about one cube in eight
+    is case 91, whose 5th listed triangle is not counted (tables.h:412)."""
+    N = 2 * R
+    z, y, x = np.mgrid[0:N, 0:N, 0:N]
+    f = (x + y + z) % 4 != 2
+    m = np.zeros_like(f)
+    m[8:N - 8, 8:N - 8, 8:N - 8] = True
+    from _util import cells_from_fine
+    return cells_from_fine(f & m)
+
+
+def test_mesh_overflow_with_late_triangles(ctx):
+    """A mesh beyond the initial output (4 N^2 vertices, 8 N^2 triangles)
+    with over 20,000 late (case-91) triangles, more than the late list
+    holds at first: reading the mesh grows the output and the list and emits
+    again from an empty late list. Checked against the oracle twice in a row
+    (the second extraction starts from the grown buffers)."""
+    R = 32
+    N = 2 * R
+    C0 = _case91_grid(R)
+    bmin, cell, dr = grid_setup(ctx, R, C0)
+    ctx.scale_uniform(0)  # radius <= 1: the closing is the input
+    for k in range(2):
+        say("late-triangle overflow R=%d, extraction %d" % (R, k))
+        ctx.close_grid(B.SE_SPHERE)
+        ctx.extract_mesh()
+        V, T = _mesh_compare(ctx, C0, R, bmin, cell)
+        assert len(V) > 4 * N * N and len(T) > 8 * N * N
+
+
 def test_largest_grid_closed_surface():
     """The largest supported grid (base_res 1024, N = 2048): a sphere closes
     to one closed genus-0 surface (2V - T = 4), every index in range, unit
@@ -399,3 +430,169 @@ def test_largest_grid_closed_surface():
     assert 2 * len(V) - len(Tm) == 4
     n = np.linalg.norm(N[:: max(1, len(N) // 100000)], axis=1)
     assert np.allclose(n, 1.0, atol=1e-5)
+
+
+BIL_ATOL = 0.0  # filtered unit normals: the same float operations as the oracle
+
+
+def test_bilateral_normals(ctx):
+    """gm_filter_normals against the oracle's restatement of the reference's
+    filter (GMorpho.cu:6539-6620, 6957-6978), from the same unit normals;
+    twice, to show that the result is the same every run."""
+    P, T = B.sphere_soup(64, 3, seed=1234)
+    R = 64
+    ctx.load(P, T, R, 0.3)
+    ctx.scale_uniform(4)
+    ctx.close_grid(B.SE_SPHERE)
+    ctx.extract_mesh()
+    V, N0, Tt = ctx.mesh()
+    for iters in (1, 2, 20):
+        say("bilateral normal filtering, %d iterations" % iters)
+        ctx.extract_mesh()
+        ctx.filter_normals(iters)
+        V1, N1, T1 = ctx.mesh()
+        assert np.array_equal(V1, V) and np.array_equal(T1, Tt)
+        ref = O.bilateral_normals(V, N0, Tt, iters)
+        err = float(np.abs(N1 - ref).max())
+        say("  max |n - n_oracle| = %.3g" % err)
+        assert err <= BIL_ATOL
+        ctx.extract_mesh()
+        ctx.filter_normals(iters)
+        assert np.array_equal(ctx.mesh()[1], N1)
+
+
+def _run_facade(tmp_path, mode):
+    import os
+    import subprocess
+    exe = os.path.join(B.LIBDIR, "bin", "facade_closing")
+    pre = str(tmp_path / ("facade_" + mode))
+    p = subprocess.run([exe, pre, "2017", mode], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    nv, nt = map(int, p.stdout.split())
+    C = np.fromfile(pre + "_closing.u8", np.uint8)
+    Tt = np.fromfile(pre + "_tris.u32", np.uint32).reshape(-1, 3)
+    pn = np.fromfile(pre + "_mesh.pn", np.float32).reshape(-1, 6)
+    assert (len(pn), len(Tt)) == (nv, nt)
+    return C, Tt, pn[:, :3], pn[:, 3:]
+
+
+def _facade_oracle(se):
+    P, T = B.sphere_soup(64, 3, seed=2017)
+    R = 64
+    bmin, bmax, cell, dr = O.grid_attributes(P, R, 0.3 + 2.0 / (2 * R))
+    I = O.voxelize(P, T, bmin, float((bmax - bmin).max()), R)
+    D = O.dilate(I, R, r=4, data_res=dr, se=se)
+    C = O.erode(I, D, R, r=4, data_res=dr)
+    return (C,) + O.mc(C, R, bmin, cell)
+
+
+def test_facade_viewer_defaults(tmp_path):
+    """The viewer's default configuration through the C++ facade: asymmetric
+    (cube) closing and bilateral normal filtering, 200 iterations
+    (BroxyViewer.cpp:66, 125, 138; GMorpho.cu:7809-7812), checked against the
+    oracle; the mesh also goes through the facade's OFF store and load."""
+    say("C++ facade driver, viewer defaults (cube closing + bilateral normals)")
+    C, Tt, V, Nn = _run_facade(tmp_path, "viewer")
+    Cr, Vr, Nr, Tr = _facade_oracle(1)
+    assert np.array_equal(C, Cr)
+    assert np.array_equal(Tt, Tr)
+    assert np.allclose(V, Vr, rtol=VERT_RTOL, atol=0)
+    # the filter starts from the GPU's unfiltered unit normals (within 2e-5 of
+    # the oracle's, summed in another order): 200 iterations at sigma_n = 0.2
+    # amplify such differences, so the oracle filters the same starting normals
+    c = B.Context(0)
+    try:
+        P, T = B.sphere_soup(64, 3, seed=2017)
+        c.load(P, T, 64, 0.3)
+        c.scale_uniform(4)
+        c.close_grid(B.SE_CUBE)
+        c.extract_mesh()
+        V0, N0, T0 = c.mesh()
+    finally:
+        c.close()
+    assert np.array_equal(V0, V) and np.array_equal(T0, Tt)
+    assert np.allclose(N0, Nr, atol=NORMAL_ATOL)
+    ref = O.bilateral_normals(V0, N0, T0, 200)
+    err = float(np.abs(Nn - ref).max())
+    say("  max |n - n_oracle| after 200 iterations = %.3g" % err)
+    assert err <= BIL_ATOL
+
+
+def test_facade_fixed_radius_update(tmp_path):
+    """GMorpho::Update(Mesh&, float se_size) (GMorpho.cu:7819-7845) through
+    the facade: cube dilation and ball erosion at floor(2 se_size / cell)."""
+    say("C++ facade driver, Update(mesh, se_size)")
+    C, Tt, V, Nn = _run_facade(tmp_path, "fixed")
+    Cr, Vr, Nr, Tr = _facade_oracle(1)
+    assert np.array_equal(C, Cr)
+    assert np.array_equal(Tt, Tr)
+    assert np.allclose(V, Vr, rtol=VERT_RTOL, atol=0)
+    assert np.allclose(Nn, Nr, atol=NORMAL_ATOL)
+
+
+def test_scale_brush_edits_and_closing(ctx):
+    """ScaleField brush strokes on the device (UpdateScaleGrid,
+    ScaleField.cu:81-141) against the oracle after every stroke, and the
+    closings with the edited field; the second closing follows strokes that
+    rebuilt the radius planes of the touched bricks only."""
+    P, T = B.sphere_soup(64, 3, seed=99)
+    R = 64
+    N = 2 * R
+    ctx.load(P, T, R, 0.3)
+    bmin, bmax, res, dr, cell = ctx.info()
+    fc = np.float32(0.5 * cell)
+    I = ctx.get_cells(B.GRID_INPUT)
+    say("scale brush: fill 3, MAX stroke")
+    ctx.scale_fill(3)
+    g = np.full(N ** 3, 3, np.int8)
+    assert np.array_equal(ctx.scale_grid_get(), g)
+    strokes = [((0.5, 0.5, 0.5), 0.12, 0.2, 0.0, B.BRUSH_MAX)]
+    later = [((0.4, 0.6, 0.5), 0.05, 0.1, 0.0, B.BRUSH_MIN),
+             ((0.62, 0.41, 0.55), 0.08, 0.12, 0.03, B.BRUSH_ZERO),
+             ((0.3, 0.35, 0.7), 0.2, 0.21, 0.0, B.BRUSH_MAX)]
+
+    def apply(st):
+        nonlocal g
+        for pos, sc, sup, gs, mode in st:
+            ctx.scale_brush(pos, sc, sup, gs, mode)
+            g = O.scale_brush(g, N, bmin, fc, np.array(pos, np.float32), sc, sup, gs, mode)
+            assert np.array_equal(ctx.scale_grid_get(), g), "brush mode %d differs" % mode
+
+    def closing():
+        ctx.close_grid(B.SE_SPHERE)
+        Dref = O.dilate(I, R, scale=g, data_res=dr)
+        assert np.array_equal(ctx.get_cells(B.GRID_DILATION), Dref)
+        assert np.array_equal(ctx.get_cells(B.GRID_CLOSING), O.erode(I, Dref, R, scale=g, data_res=dr))
+
+    apply(strokes)
+    assert g.max() == 8
+    closing()
+    say("scale brush: MIN, ZERO and MAX strokes, closing again")
+    apply(later)
+    closing()
+
+
+def test_facade_brush_strokes(tmp_path):
+    """Brush strokes through the facade's ScaleField (AddPoint + UpdateGrid,
+    a MAX then a MIN stroke, each followed by Update) against the oracle's
+    brush and closing."""
+    import os
+    say("C++ facade driver, brush strokes")
+    C, Tt, V, Nn = _run_facade(tmp_path, "brush")
+    P, T = B.sphere_soup(64, 3, seed=2017)
+    R = 64
+    N = 2 * R
+    bmin, bmax, cell, dr = O.grid_attributes(P, R, 0.3 + 2.0 / (2 * R))
+    fc = np.float32(0.5 * cell)
+    g = np.full(N ** 3, int(np.floor(np.float32(4.25) * fc / fc)), np.int8)
+    g = O.scale_brush(g, N, bmin, fc, np.array([0.5, 0.5, 0.5], np.float32), 0.12, 0.2, 4.25 * fc, B.BRUSH_MAX)
+    g = O.scale_brush(g, N, bmin, fc, np.array([0.35, 0.6, 0.45], np.float32), 0.05, 0.09, 4.25 * fc, B.BRUSH_MIN)
+    got = np.fromfile(str(tmp_path / "facade_brush_scale.i8"), np.int8)
+    assert np.array_equal(got, g)
+    I = O.voxelize(P, T, bmin, float((bmax - bmin).max()), R)
+    D = O.dilate(I, R, scale=g, data_res=dr)
+    Cr = O.erode(I, D, R, scale=g, data_res=dr)
+    assert np.array_equal(C, Cr)
+    Vr, Nr, Tr = O.mc(Cr, R, bmin, cell)
+    assert np.array_equal(Tt, Tr)
+    assert np.allclose(V, Vr, rtol=VERT_RTOL, atol=0)

@@ -192,7 +192,7 @@ class Context:
         self._call("gm_scale_brush", _ptr(p), f(scale), f(support), f(global_scale), int(mode))
 
     def scale_grid_get(self):
-        N = 2 * self.info()[2][0]
+        N = 2 * self.info()[2]
         out = np.zeros(N ** 3, np.int8)
         self._call("gm_scale_grid_get", _ptr(out))
         return out
@@ -235,6 +235,19 @@ class Context:
         self._call("gm_pyramid_get", _ptr(out))
         return out
 
+    def autotune(self, se=SE_SPHERE, reps=3, path=None):
+        """Times the closing with each launch shape of the streaming passes,
+        records the fastest for this grid size and radius (process-wide;
+        written to path when given); returns (planes per tile, waves)."""
+        z, w = ctypes.c_int(), ctypes.c_int()
+        self._call("gm_autotune", int(se), int(reps), path.encode() if path else None, ctypes.byref(z),
+                   ctypes.byref(w))
+        return z.value, w.value
+
+    def set_graphs(self, on=True):
+        """Replay gm_close / gm_extract_mesh as captured HIP graphs (default on)."""
+        self._call("gm_set_graphs", 1 if on else 0)
+
     def set_timing(self, on=True):
         self._call("gm_set_timing", 1 if on else 0)
 
@@ -251,6 +264,77 @@ class Context:
 
     def sync(self):
         self._call("gm_sync")
+
+
+def tune_load(path):
+    """Reads launch shapes recorded by Context.autotune into the process table."""
+    rc = hip_lib().gm_tune_load(path.encode())
+    if rc:
+        raise GMorphoError("gm_tune_load(%s) -> %d" % (path, rc))
+
+
+_DECIM = None
+
+
+def decimate_lib():
+    global _DECIM
+    if _DECIM is None:
+        path = os.path.join(LIBDIR, "libgmorpho_decimate.so")
+        if not os.path.exists(path):
+            raise GMorphoError("libgmorpho_decimate.so not built: run __graft_entry__.build()")
+        _DECIM = ctypes.CDLL(path)
+        _DECIM.gmd_last_error.restype = ctypes.c_char_p
+        _DECIM.gmd_last_error.argtypes = [_P]
+    return _DECIM
+
+
+class Decimator:
+    """Proxy decimation of a mesh (the reference's Decimator): HC smoothing,
+    then bounded quadric edge collapses; host code (libgmorpho_decimate)."""
+
+    def __init__(self, P, T, target_faces, target_error, use_linear_constraints=True, max_edge_length=1.0,
+                 bbox_min=None, res=None, cell=1.0, scale_grid=None, hc_iters=3):
+        self._lib = decimate_lib()
+        P = np.ascontiguousarray(P, np.float32).reshape(-1, 3)
+        T = np.ascontiguousarray(T, np.uint32).reshape(-1, 3)
+        bm = None if bbox_min is None else np.ascontiguousarray(bbox_min, np.float32)
+        rs = None if res is None else np.ascontiguousarray(res, np.int32)
+        sg = None if scale_grid is None else np.ascontiguousarray(scale_grid, np.int8)
+        h = _P()
+        rc = self._lib.gmd_create(_ptr(P), len(P), _ptr(T), len(T), int(target_faces), ctypes.c_float(target_error),
+                                  1 if use_linear_constraints else 0, ctypes.c_float(max_edge_length), _ptr(bm),
+                                  _ptr(rs), ctypes.c_float(cell), _ptr(sg), int(hc_iters), ctypes.byref(h))
+        if rc:
+            raise GMorphoError("gmd_create -> %d" % rc)
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.gmd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def optimize(self, budget=0.0):
+        more = ctypes.c_int()
+        self._lib.gmd_optimize(self._h, ctypes.c_double(budget), ctypes.byref(more))
+        return bool(more.value)
+
+    def state(self):
+        nf, nv, err = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        self._lib.gmd_state(self._h, ctypes.byref(nf), ctypes.byref(nv), ctypes.byref(err))
+        return nf.value, nv.value, err.value
+
+    def mesh(self):
+        nf, nv, _ = self.state()
+        P = np.zeros((nv, 3), np.float32)
+        T = np.zeros((nf, 3), np.uint32)
+        self._lib.gmd_get_mesh(self._h, _ptr(P), _ptr(T))
+        return P, T
 
 
 def mc_tables():
@@ -372,6 +456,12 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+    def set_graphs(self, on=True):
+        """Replay the batched launches as a captured HIP graph (default on)."""
+        rc = self._lib.gm_batch_set_graphs(self._h, 1 if on else 0)
+        if rc:
+            raise GMorphoError("gm_batch_set_graphs -> %d" % rc)
 
     def close_mesh(self, contexts, se=SE_SPHERE):
         """gm_close + gm_extract_mesh for every context; meshes are then read

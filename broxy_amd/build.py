@@ -2,6 +2,7 @@
 
 libgmorpho_hip.so     HIP kernels + C-ABI (hipcc, gfx950)
 libgmorpho_meshgen.so seeded workload generators (host C++)
+libgmorpho_decimate.so proxy decimation of a closing mesh (host C++)
 """
 import os
 import subprocess
@@ -38,6 +39,14 @@ def build_meshgen():
     out = os.path.join(LIBDIR, "libgmorpho_meshgen.so")
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
           os.path.join(CSRC, "meshgen.cpp"), "-o", out])
+    return out
+
+
+def build_decimator():
+    """Host proxy decimation (csrc/decimator.cpp)."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = os.path.join(LIBDIR, "libgmorpho_decimate.so")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", os.path.join(CSRC, "decimator.cpp"), "-o", out])
     return out
 
 
@@ -90,6 +99,7 @@ def build_oracle():
 
 def build_all():
     build_meshgen()
+    build_decimator()
     build_hip()
     build_driver()
     build_oracle()
@@ -98,5 +108,6 @@ def build_all():
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "hip": build_hip, "meshgen": build_meshgen, "driver": build_driver,
+        {"all": build_all, "hip": build_hip, "meshgen": build_meshgen, "decimator": build_decimator,
+         "driver": build_driver,
          "oracle": build_oracle}[w]()

@@ -2,10 +2,15 @@
 // orchestration of GMorpho::Load / Update / ExtractClosingMesh
 // (GMorpho.cu:7731-7845, 6814-7001) on one HIP stream.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/gmorpho_c.h"
@@ -41,6 +46,9 @@ struct gm_ctx {
   int nv = 0, nf = 0;
   size_t capP = 0, capTri = 0;
   u32 *mark = nullptr, *shell = nullptr;
+  u32* dCheck = nullptr;          // mesh check results (8 u32)
+  uint8_t* hStage = nullptr;      // pinned upload staging of the mesh
+  size_t capStage = 0;
   // radius field
   int rmode = 0, r = 0, rlo = 0, rhi = 0, L = 0;
   int8_t* dGrid = nullptr;
@@ -81,6 +89,20 @@ struct gm_ctx {
   uint64_t capPyr = 0;  // texels of dPyr
   int pyrLevels = 0;
   uint64_t pyrTexels = 0;
+  // Captured launch sequences (hipGraph) of gm_close (per structuring
+  // element) and gm_extract_mesh, replayed while their key holds. graphGen
+  // changes whenever a launch argument could (buffers, sizes, radius field,
+  // tables, grid attributes).
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+    uint64_t key[4] = {0, 0, 0, 0};
+  };
+  Graph gClose[2], gMesh;
+  uint64_t graphGen = 1;
+  bool graphs = true;
+  // launch shape forced while gm_autotune measures (tileZ 0: the table's)
+  int forceTileZ = 0, forceWaveCap = 0;
+  int tuneSe = 0;  // structuring element of the closing being launched (tuning-table key)
   // timing
   bool timing = false;
   hipEvent_t ev[20] = {};
@@ -88,6 +110,20 @@ struct gm_ctx {
 };
 
 namespace {
+
+// Launch shapes chosen by gm_autotune, per (base resolution, uniform radius
+// or -1 for per-voxel radii, structuring element); shared by the contexts of
+// the process.
+struct TuneKey {
+  int R, r, se;
+  bool operator<(const TuneKey& o) const { return std::tie(R, r, se) < std::tie(o.R, o.r, o.se); }
+};
+struct TuneShape {
+  int tileZ = 0, waveCap = 0;
+  double ms = 0.0;
+};
+std::mutex g_tuneMu;
+std::map<TuneKey, TuneShape> g_tune;
 
 int fail(gm_ctx* c, int code, const std::string& msg) {
   if (c) c->err = "GMorpho Error: " + msg;
@@ -106,7 +142,47 @@ void dfree(T*& p) {
   p = nullptr;
 }
 
+// Launch arguments may have changed: captured sequences are out of date.
+void touch(gm_ctx* c) { ++c->graphGen; }
+
+void drop_graph(gm_ctx::Graph& g) {
+  if (g.exec) hipGraphExecDestroy(g.exec);
+  g.exec = nullptr;
+}
+
+// Runs enqueue() (kernel launches and async copies on c->stream only) as a
+// captured graph: replayed when the key matches the last capture, else
+// captured again. Without graphs (or with per-stage timing) it runs eagerly.
+template <class F>
+int run_captured(gm_ctx* c, gm_ctx::Graph& g, const uint64_t key[4], F enqueue) {
+  if (!c->graphs || c->timing) return enqueue();
+  if (g.exec && std::equal(key, key + 4, g.key)) {
+    HIPCALL(c, hipGraphLaunch(g.exec, c->stream));
+    return GM_OK;
+  }
+  drop_graph(g);
+  HIPCALL(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+  const int rc = enqueue();
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  if (rc != GM_OK || e != hipSuccess) {
+    if (graph) hipGraphDestroy(graph);
+    if (rc != GM_OK) return rc;
+    return fail(c, GM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  }
+  const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    g.exec = nullptr;
+    return fail(c, GM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  }
+  std::copy(key, key + 4, g.key);
+  HIPCALL(c, hipGraphLaunch(g.exec, c->stream));
+  return GM_OK;
+}
+
 void free_grid(gm_ctx* c) {
+  touch(c);
   dfree(c->I); dfree(c->D); dfree(c->K); dfree(c->C);
   dfree(c->DF); dfree(c->TG); dfree(c->CL); dfree(c->UG); dfree(c->IFL); dfree(c->DAL); dfree(c->brickI); dfree(c->brickK); dfree(c->queue);
   dfree(c->stage); dfree(c->mark); dfree(c->shell);
@@ -127,6 +203,7 @@ void free_grid(gm_ctx* c) {
 }
 
 int alloc_mesh_out(gm_ctx* c, u32 capV, u32 capT, u32 capLate) {
+  if (capV > c->mc.capV || capT > c->mc.capT || capLate > c->mc.capLate || !c->mc.late) touch(c);
   if (capV > c->mc.capV) {
     dfree(c->mc.V);
     dfree(c->mc.Nrm);
@@ -148,8 +225,23 @@ int alloc_mesh_out(gm_ctx* c, u32 capV, u32 capT, u32 capLate) {
   return GM_OK;
 }
 
+// Per-listed-block marching-cubes buffers (cube cases, vertex bases) for cap
+// listed blocks.
+int alloc_mc_list(gm_ctx* c, u32 cap) {
+  if (cap <= c->mc.capList && c->mc.cellCase) return GM_OK;
+  touch(c);
+  dfree(c->mc.cellV);
+  dfree(c->mc.cellCase);
+  c->mc.capList = 0;
+  HIPCALL(c, hipMalloc(&c->mc.cellV, (size_t)cap * 512 * 4));
+  HIPCALL(c, hipMalloc(&c->mc.cellCase, (size_t)cap * 512 * 8));
+  c->mc.capList = cap;
+  return GM_OK;
+}
+
 // Grid dimensions of base resolution R inside buffers sized for c->capR.
 void set_dims(gm_ctx* c, int R) {
+  touch(c);
   if (c->R != R && c->rmode != 0) {
     // a per-voxel radius field belongs to the previous grid size
     c->rmode = 0;
@@ -215,11 +307,17 @@ int alloc_grid(gm_ctx* c, int R) {
   c->mc.nblocks = (P / 8) * (P / 8) * (P / 8);
   HIPCALL(c, hipMalloc(&c->mc.blockV, (size_t)c->mc.nblocks * 4));
   HIPCALL(c, hipMalloc(&c->mc.blockT, (size_t)c->mc.nblocks * 4));
-  HIPCALL(c, hipMalloc(&c->mc.cellV, (size_t)R * R * R * 4));
-  HIPCALL(c, hipMalloc(&c->mc.cellCase, (size_t)c->mc.nblocks * 512 * 8));
+  c->mc.capList = 0;
+  {
+    // a closed surface lists on the order of 6 (P/8)^2 blocks; more grow the
+    // per-block buffers when the mesh is read
+    const size_t B = (size_t)P / 8;
+    int rcl = alloc_mc_list(c, (u32)std::min<size_t>((size_t)c->mc.nblocks, std::max<size_t>(64, 8 * B * B)));
+    if (rcl) return rcl;
+  }
   HIPCALL(c, hipMalloc(&c->mc.posOf, (size_t)c->mc.nblocks * 4));
   HIPCALL(c, hipMalloc(&c->mc.order, ((size_t)4 * c->mc.nblocks + 4) * 4));
-  HIPCALL(c, hipMalloc(&c->mc.totals, 16));
+  HIPCALL(c, hipMalloc(&c->mc.totals, 32));  // nV, nT, overflow, border, late cubes, spare
   HIPCALL(c, hipMalloc(&c->mc.scanPart, mc_scan_part_bytes(c->mc.nblocks)));
   HIPCALL(c, hipMalloc(&c->mc.brick, brickBytes));
   HIPCALL(c, hipMalloc(&c->mc.list, ((size_t)c->mc.nblocks + 1) * 4));
@@ -301,6 +399,7 @@ int ensure_tables(gm_ctx* c, int rlo, int rhi) {
   }
   HIPCALL(c, hipMemcpyAsync(c->dTables, all.data(), bytes, hipMemcpyHostToDevice, c->stream));
   HIPCALL(c, hipStreamSynchronize(c->stream));
+  touch(c);
   c->tabLo = rlo;
   c->tabHi = rhi;
   c->tabWords = longest;
@@ -331,6 +430,7 @@ int cube_tables(gm_ctx* c, int rlo, int rhi) {
   }
   HIPCALL(c, hipMemcpyAsync(c->dCube, all.data(), bytes, hipMemcpyHostToDevice, c->stream));
   HIPCALL(c, hipStreamSynchronize(c->stream));
+  touch(c);
   c->cubeLo = rlo;
   c->cubeHi = rhi;
   c->cubeWords = longest;
@@ -355,6 +455,18 @@ MorphBuffers morph_buffers(gm_ctx* c) {
   b.queue = c->queue;
   b.queueCap = (int)tile_count_max(c->R);
   b.qsel = &c->qsel;
+  if (c->forceTileZ) {
+    b.tileZ = c->forceTileZ;
+    b.waveCap = c->forceWaveCap;
+  } else {
+    std::lock_guard<std::mutex> lk(g_tuneMu);
+    const int r = c->rmode == 0 ? c->r : -1;
+    auto it = g_tune.find(TuneKey{c->R, r, c->tuneSe});
+    if (it != g_tune.end()) {
+      b.tileZ = it->second.tileZ;
+      b.waveCap = it->second.waveCap;
+    }
+  }
   if (c->rmode != 0 && c->capVarR >= c->R) {
     b.rp = c->rp;
     b.rvset = c->rvset;
@@ -392,6 +504,7 @@ int ensure_var_buffers(gm_ctx* c) {
   HIPCALL(c, hipMalloc(&c->kv, (size_t)R * R * R / 4 * 4));
   HIPCALL(c, hipMalloc(&c->kbl, (brick_words(R) + 1) * 4));
   c->capVarR = R;
+  touch(c);
   return GM_OK;
 }
 
@@ -422,46 +535,66 @@ void set_range_from_hist(gm_ctx* c) {
   c->rhi = hi;
 }
 
-int do_dilate(gm_ctx* c, int se) {
+// Host-synchronous set-up of a dilation (tables, per-value buffers): done
+// before any capture.
+int prep_dilate(gm_ctx* c, int se) {
   int rc = check_radius(c);
   if (rc) return rc;
-  int lo = std::max(c->rlo, 0), hi = c->rhi;
-  if (se == GM_SE_CUBE) {
-    rc = cube_tables(c, lo, hi);
-    if (rc) return rc;
-    MorphBuffers b = morph_buffers(c);
-    b.tables = c->dCube;
-    hipError_t e = launch_dilate(morph_dims(c), b, radius_field(c), 1, c->cubeTabs, lo, hi, c->cubeWords, c->stream,
-                                 c->timing ? &c->ev[0] : nullptr, c->timing ? &c->ev[15] : nullptr);
-    if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("dilation: ") + hipGetErrorString(e));
-  } else {
-    rc = ensure_tables(c, lo, hi);
-    if (rc) return rc;
-    rc = ensure_var_buffers(c);
-    if (rc) return rc;
-    hipError_t e = launch_dilate(morph_dims(c), morph_buffers(c), radius_field(c), 0, c->tabs, lo, hi, c->tabWords,
-                                 c->stream, c->timing ? &c->ev[0] : nullptr, c->timing ? &c->ev[15] : nullptr);
-    if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("dilation: ") + hipGetErrorString(e));
-    if (c->rmode != 0) c->rpValid = true;
-  }
-  c->meshPending = false;
-  return GM_OK;
+  const int lo = std::max(c->rlo, 0), hi = c->rhi;
+  if (se == GM_SE_CUBE) return cube_tables(c, lo, hi);
+  rc = ensure_tables(c, lo, hi);
+  if (rc) return rc;
+  return ensure_var_buffers(c);
 }
-
-int do_erode(gm_ctx* c) {
+int prep_erode(gm_ctx* c) {
   int rc = check_radius(c);
   if (rc) return rc;
   rc = ensure_tables(c, std::max(c->rlo, 0), c->rhi);
   if (rc) return rc;
-  rc = ensure_var_buffers(c);
-  if (rc) return rc;
+  return ensure_var_buffers(c);
+}
+// Launches only (capturable).
+int enq_dilate(gm_ctx* c, int se) {
+  const int lo = std::max(c->rlo, 0), hi = c->rhi;
+  hipError_t e;
+  if (se == GM_SE_CUBE) {
+    MorphBuffers b = morph_buffers(c);
+    b.tables = c->dCube;
+    e = launch_dilate(morph_dims(c), b, radius_field(c), 1, c->cubeTabs, lo, hi, c->cubeWords, c->stream,
+                      c->timing ? &c->ev[0] : nullptr, c->timing ? &c->ev[15] : nullptr);
+  } else {
+    e = launch_dilate(morph_dims(c), morph_buffers(c), radius_field(c), 0, c->tabs, lo, hi, c->tabWords, c->stream,
+                      c->timing ? &c->ev[0] : nullptr, c->timing ? &c->ev[15] : nullptr);
+  }
+  if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("dilation: ") + hipGetErrorString(e));
+  return GM_OK;
+}
+int enq_erode(gm_ctx* c) {
   hipError_t e = launch_erode(morph_dims(c), morph_buffers(c), radius_field(c), c->tabs, c->rlo, c->rhi,
                               c->tabWords, c->stream, c->timing ? &c->ev[5] : nullptr,
                               c->timing ? &c->ev[17] : nullptr);
   if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("erosion: ") + hipGetErrorString(e));
-  if (c->rmode != 0) c->rpValid = true;
-  c->meshPending = false;
   return GM_OK;
+}
+// State after a dilation or erosion was enqueued: the sphere's per-value
+// passes built the radius planes.
+void after_morph(gm_ctx* c, bool planes) {
+  if (planes && c->rmode != 0 && c->capVarR >= c->R) c->rpValid = true;
+  c->meshPending = false;
+}
+
+int do_dilate(gm_ctx* c, int se) {
+  int rc = prep_dilate(c, se);
+  if (rc == GM_OK) rc = enq_dilate(c, se);
+  if (rc == GM_OK) after_morph(c, se == GM_SE_SPHERE);
+  return rc;
+}
+
+int do_erode(gm_ctx* c) {
+  int rc = prep_erode(c);
+  if (rc == GM_OK) rc = enq_erode(c);
+  if (rc == GM_OK) after_morph(c, true);
+  return rc;
 }
 
 }  // namespace
@@ -481,7 +614,8 @@ int gm_create(int device, gm_ctx** out) {
     return GM_ERR_HIP;
   }
   for (auto& e : c->ev) hipEventCreate(&e);
-  hipHostMalloc((void**)&c->hTotals, 16, hipHostMallocDefault);
+  // pinned: 4 mesh totals, then 8 words of the mesh check
+  hipHostMalloc((void**)&c->hTotals, 48, hipHostMallocDefault);
   *out = c;
   return GM_OK;
 }
@@ -490,10 +624,15 @@ int gm_destroy(gm_ctx* c) {
   if (!c) return GM_ERR_ARG;
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
+  drop_graph(c->gClose[0]);
+  drop_graph(c->gClose[1]);
+  drop_graph(c->gMesh);
   free_grid(c);
   dfree(c->dP);
   dfree(c->dT);
   dfree(c->dSnap);
+  dfree(c->dCheck);
+  if (c->hStage) hipHostFree(c->hStage);
   dfree(c->dLat);
   dfree(c->dTables);
   dfree(c->dCube);
@@ -513,42 +652,60 @@ int gm_load_mesh(gm_ctx* c, const float* P, const float* N, int nv, const uint32
   (void)N;
   if (!c) return GM_ERR_ARG;
   if (!P || !T || nv <= 0 || nf <= 0) return fail(c, GM_ERR_ARG, "empty mesh");
-  for (size_t i = 0; i < (size_t)3 * nf; ++i)
-    if (T[i] >= (uint32_t)nv) return fail(c, GM_ERR_ARG, "triangle index out of range");
   hipSetDevice(c->device);
   if ((size_t)nv > c->capP) {
     dfree(c->dP);
     dfree(c->dSnap);
+    c->capP = 0;
     HIPCALL(c, hipMalloc(&c->dP, (size_t)nv * 12));
     HIPCALL(c, hipMalloc(&c->dSnap, (size_t)nv * 12));
     c->capP = nv;
   }
   if ((size_t)nf > c->capTri) {
     dfree(c->dT);
+    c->capTri = 0;
     HIPCALL(c, hipMalloc(&c->dT, (size_t)nf * 12));
     c->capTri = nf;
   }
-  HIPCALL(c, hipMemcpyAsync(c->dP, P, (size_t)nv * 12, hipMemcpyHostToDevice, c->stream));
-  HIPCALL(c, hipMemcpyAsync(c->dT, T, (size_t)nf * 12, hipMemcpyHostToDevice, c->stream));
+  if (!c->dCheck) HIPCALL(c, hipMalloc(&c->dCheck, 32));
+  // through pinned staging: one host copy, then asynchronous DMA
+  const size_t bytes = (size_t)nv * 12 + (size_t)nf * 12;
+  if (bytes > c->capStage) {
+    if (c->hStage) hipHostFree(c->hStage);
+    c->hStage = nullptr;
+    c->capStage = 0;
+    HIPCALL(c, hipHostMalloc((void**)&c->hStage, bytes, hipHostMallocDefault));
+    c->capStage = bytes;
+  }
+  // the previous upload from the staging buffer has finished (every call
+  // below ends with a synchronization)
+  std::memcpy(c->hStage, P, (size_t)nv * 12);
+  std::memcpy(c->hStage + (size_t)nv * 12, T, (size_t)nf * 12);
+  HIPCALL(c, hipMemcpyAsync(c->dP, c->hStage, (size_t)nv * 12, hipMemcpyHostToDevice, c->stream));
+  HIPCALL(c, hipMemcpyAsync(c->dT, c->hStage + (size_t)nv * 12, (size_t)nf * 12, hipMemcpyHostToDevice, c->stream));
+  // index check and bounding box on the device (kept for ComputeGridAttributes)
+  const u32 init[8] = {0u, 0u, ~0u, ~0u, ~0u, 0u, 0u, 0u};
+  std::memcpy(c->hTotals + 4, init, sizeof(init));
+  HIPCALL(c, hipMemcpyAsync(c->dCheck, c->hTotals + 4, 32, hipMemcpyHostToDevice, c->stream));
+  HIPCALL(c, launch_mesh_check(c->dP, nv, c->dT, nf, c->dCheck, c->stream));
+  HIPCALL(c, hipMemcpyAsync(c->hTotals + 4, c->dCheck, 32, hipMemcpyDeviceToHost, c->stream));
   HIPCALL(c, hipStreamSynchronize(c->stream));
+  const u32* chk = c->hTotals + 4;
+  if (chk[0]) return fail(c, GM_ERR_ARG, "triangle index out of range");
+  if (chk[1]) return fail(c, GM_ERR_ARG, "vertex coordinate not finite");
   c->nv = nv;
   c->nf = nf;
-  // bounding box of the mesh, kept for ComputeGridAttributes
-  float mn[3] = {P[0], P[1], P[2]}, mx[3] = {P[0], P[1], P[2]};
-  for (int i = 0; i < nv; ++i)
-    for (int a = 0; a < 3; ++a) {
-      float v = P[3 * i + a];
-      if (v > mx[a]) mx[a] = v;
-      if (v < mn[a]) mn[a] = v;
-    }
-  std::memcpy(c->bmin, mn, sizeof(mn));
-  std::memcpy(c->bmax, mx, sizeof(mx));
+  for (int a = 0; a < 3; ++a) {
+    c->bmin[a] = mesh_key_float(chk[2 + a]);
+    c->bmax[a] = mesh_key_float(chk[5 + a]);
+  }
   c->cell = -1.f;  // grid attributes not computed yet
   return GM_OK;
 }
 
 int gm_voxelize(gm_ctx* c, int base_res, float margin) {
   if (!c) return GM_ERR_ARG;
+  touch(c);
   if (c->nv <= 0) return fail(c, GM_ERR_STATE, "no mesh: call gm_load_mesh first");
   hipSetDevice(c->device);
   // Voxelizer::ComputeGridAttributes (Voxelizer.cu:431-455), in float.
@@ -598,6 +755,7 @@ int gm_load(gm_ctx* c, const float* P, const float* N, int nv, const uint32_t* T
 int gm_grid_init(gm_ctx* c, const float bmin[3], const float bmax[3], int base_res, const int data_res[3],
                  float cell) {
   if (!c || !bmin || !bmax || !data_res) return GM_ERR_ARG;
+  touch(c);
   hipSetDevice(c->device);
   int rc = alloc_grid(c, base_res);
   if (rc) return rc;
@@ -654,6 +812,7 @@ int gm_grid_info(const gm_ctx* c, float bmin[3], float bmax[3], int res[3], int 
 int gm_scale_uniform(gm_ctx* c, int r) {
   int rc = need_grid(c);
   if (rc) return rc;
+  touch(c);
   if (r < 0 || r > 127) return fail(c, GM_ERR_ARG, "uniform radius must be in [0,127]");
   c->rmode = 0;
   c->r = r;
@@ -666,6 +825,7 @@ int gm_scale_uniform(gm_ctx* c, int r) {
 int gm_scale_grid_i8(gm_ctx* c, const int8_t* grid) {
   int rc = need_grid(c);
   if (rc) return rc;
+  touch(c);
   if (!grid) return fail(c, GM_ERR_ARG, "null scale grid");
   hipSetDevice(c->device);
   size_t n = (size_t)c->N * c->N * c->N;
@@ -690,6 +850,7 @@ int gm_scale_grid_i8(gm_ctx* c, const int8_t* grid) {
 int gm_scale_grid_fill(gm_ctx* c, int value) {
   int rc = need_grid(c);
   if (rc) return rc;
+  touch(c);
   if (value < -128 || value > 127) return fail(c, GM_ERR_ARG, "radius grid value must fit a signed char");
   hipSetDevice(c->device);
   rc = ensure_scale_grid(c);
@@ -711,6 +872,7 @@ int gm_scale_grid_fill(gm_ctx* c, int value) {
 int gm_scale_brush(gm_ctx* c, const float position[3], float scale, float support, float global_scale, int mode) {
   int rc = need_grid(c);
   if (rc) return rc;
+  touch(c);
   if (!position) return fail(c, GM_ERR_ARG, "null brush position");
   if (mode < GM_BRUSH_NONE || mode > GM_BRUSH_ZERO) return fail(c, GM_ERR_ARG, "unknown brush mode");
   if (!(c->cell > 0.f)) return fail(c, GM_ERR_STATE, "grid attributes not set");
@@ -778,6 +940,7 @@ int gm_scale_grid_get(gm_ctx* c, int8_t* grid) {
 int gm_scale_lattice_f32(gm_ctx* c, const float* lat, int L) {
   int rc = need_grid(c);
   if (rc) return rc;
+  touch(c);
   if (!lat || L < 2 || L > c->N || (c->N % L) != 0 || ((c->N / L) & (c->N / L - 1)) != 0)
     return fail(c, GM_ERR_ARG, "lattice size must divide N with a power-of-two ratio");
   hipSetDevice(c->device);
@@ -813,6 +976,7 @@ int gm_dilate(gm_ctx* c, int se) {
   if (rc) return rc;
   if (se != GM_SE_SPHERE && se != GM_SE_CUBE) return fail(c, GM_ERR_ARG, "unknown structuring element");
   hipSetDevice(c->device);
+  c->tuneSe = se;
   c->evValid = 0;
   rc = do_dilate(c, se);
   if (rc) return rc;
@@ -836,6 +1000,7 @@ int gm_close(gm_ctx* c, int se) {
   if (rc) return rc;
   if (se != GM_SE_SPHERE && se != GM_SE_CUBE) return fail(c, GM_ERR_ARG, "unknown structuring element");
   hipSetDevice(c->device);
+  c->tuneSe = se;
   c->evValid = 0;
   size_t rowBytes = (size_t)c->N * c->N * c->W * 4;
   if (c->rmode == 0 && c->r <= 1) {
@@ -846,10 +1011,20 @@ int gm_close(gm_ctx* c, int se) {
     c->meshPending = false;
     return GM_OK;
   }
-  rc = do_dilate(c, se);
+  rc = prep_dilate(c, se);
+  if (rc == GM_OK) rc = prep_erode(c);
   if (rc) return rc;
-  rc = do_erode(c);
+  // the tile-queue toggle returns to its value after the four passes, so a
+  // captured sequence stays valid while it starts from the same value
+  const MorphBuffers shape = morph_buffers(c);
+  const uint64_t key[4] = {c->graphGen, (uint64_t)se | ((uint64_t)shape.tileZ << 8) | ((uint64_t)shape.waveCap << 16),
+                           (uint64_t)c->qsel, (uint64_t)(c->rpValid ? 1 : 0)};
+  rc = run_captured(c, c->gClose[se], key, [&]() -> int {
+    int r = enq_dilate(c, se);
+    return r == GM_OK ? enq_erode(c) : r;
+  });
   if (rc) return rc;
+  after_morph(c, true);
   if (c->timing) c->evValid |= 1;
   return GM_OK;
 }
@@ -858,9 +1033,15 @@ int gm_extract_mesh(gm_ctx* c) {
   int rc = need_grid(c);
   if (rc) return rc;
   hipSetDevice(c->device);
-  hipError_t e = launch_mc(c->C, c->R, c->bmin, c->cell, c->mc, c->stream, c->timing ? &c->ev[10] : nullptr);
-  if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("mesh extraction: ") + hipGetErrorString(e));
-  HIPCALL(c, hipMemcpyAsync(c->hTotals, c->mc.totals, 16, hipMemcpyDeviceToHost, c->stream));
+  HIPCALL(c, mc_prepare());
+  const uint64_t key[4] = {c->graphGen, 0, 0, 0};
+  rc = run_captured(c, c->gMesh, key, [&]() -> int {
+    hipError_t e = launch_mc(c->C, c->R, c->bmin, c->cell, c->mc, c->stream, c->timing ? &c->ev[10] : nullptr);
+    if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("mesh extraction: ") + hipGetErrorString(e));
+    HIPCALL(c, hipMemcpyAsync(c->hTotals, c->mc.totals, 16, hipMemcpyDeviceToHost, c->stream));
+    return GM_OK;
+  });
+  if (rc) return rc;
   c->meshPending = true;
   if (c->timing) c->evValid |= 2;
   return GM_OK;
@@ -872,6 +1053,20 @@ int gm_mesh_counts(gm_ctx* c, uint32_t* nv, uint32_t* nt) {
   if (!c->meshPending) return fail(c, GM_ERR_STATE, "no mesh extracted since the last grid change");
   hipSetDevice(c->device);
   HIPCALL(c, hipStreamSynchronize(c->stream));
+  if (c->hTotals[2] & 4u) {
+    // more listed blocks than the per-block buffers hold: grow them and
+    // extract again (nothing was stored)
+    u32 listed = 0;
+    HIPCALL(c, hipMemcpyAsync(&listed, c->mc.list, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    rc = alloc_mc_list(c, std::min<u32>((u32)c->mc.nblocks, listed + listed / 8 + 64));
+    if (rc) return rc;
+    hipError_t e = launch_mc(c->C, c->R, c->bmin, c->cell, c->mc, c->stream, nullptr);
+    if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("mesh extraction: ") + hipGetErrorString(e));
+    HIPCALL(c, hipMemcpyAsync(c->hTotals, c->mc.totals, 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(c, hipStreamSynchronize(c->stream));
+    if (c->hTotals[2] & 4u) return fail(c, GM_ERR_CAPACITY, "marching-cubes block list overflow");
+  }
   // grow and re-emit when the output did not fit
   for (int attempt = 0; attempt < 3 && c->hTotals[2] != 0; ++attempt) {
     u32 lateN = 0;
@@ -977,6 +1172,12 @@ int gm_pyramid_get(gm_ctx* c, uint64_t* out) {
   return GM_OK;
 }
 
+int gm_set_graphs(gm_ctx* c, int enable) {
+  if (!c) return GM_ERR_ARG;
+  c->graphs = enable != 0;
+  return GM_OK;
+}
+
 int gm_set_timing(gm_ctx* c, int enable) {
   if (!c) return GM_ERR_ARG;
   c->timing = enable != 0;
@@ -1003,6 +1204,109 @@ int gm_stage_times(gm_ctx* c, float* ms, int n) {
   return GM_OK;
 }
 
+// ---- Launch-shape tuning of the streaming passes.
+namespace {
+const int kShapes[][2] = {{kTileZ, 8}, {kTileZ, 4}, {kTileZ, 2}, {kTileZMin, 8}, {kTileZMin, 4}, {kTileZMin, 2}};
+
+uint64_t grid_hash(gm_ctx* c, u32* rows) {
+  // FNV-1a over the closing grid (device -> pinned staging of the mesh check)
+  const size_t words = (size_t)c->N * c->N * c->W;
+  std::vector<u32> h(words);
+  if (hipMemcpyAsync(h.data(), rows, words * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return 0;
+  uint64_t x = 1469598103934665603ull;
+  for (u32 w : h) x = (x ^ w) * 1099511628211ull;
+  return x;
+}
+
+void write_tune_file(const char* path) {
+  std::lock_guard<std::mutex> lk(g_tuneMu);
+  FILE* f = std::fopen(path, "w");
+  if (!f) return;
+  for (const auto& kv : g_tune)
+    std::fprintf(f, "{\"base_res\": %d, \"radius\": %d, \"se\": %d, \"tile_planes\": %d, \"waves\": %d, \"ms\": %.5f}\n",
+                 kv.first.R, kv.first.r, kv.first.se, kv.second.tileZ, kv.second.waveCap, kv.second.ms);
+  std::fclose(f);
+}
+}  // namespace
+
+int gm_autotune(gm_ctx* c, int se, int reps, const char* path, int* tile_planes, int* waves) {
+  int rc = need_grid(c);
+  if (rc) return rc;
+  if (se != GM_SE_SPHERE && se != GM_SE_CUBE) return fail(c, GM_ERR_ARG, "unknown structuring element");
+  if (c->rmode == 0 && c->r <= 1) return fail(c, GM_ERR_STATE, "radius <= 1: no morphology to tune");
+  if (reps < 1) reps = 1;
+  hipSetDevice(c->device);
+  const bool timing = c->timing;
+  c->timing = false;
+  double best = 1e30;
+  int bestZ = 0, bestW = 0;
+  uint64_t ref = 0;
+  for (const auto& shp : kShapes) {
+    c->forceTileZ = shp[0];
+    c->forceWaveCap = shp[1];
+    touch(c);
+    rc = gm_close(c, se);  // warm (capture, code objects)
+    if (rc == GM_OK) rc = gm_sync(c);
+    if (rc) break;
+    const uint64_t hsh = grid_hash(c, c->C);
+    if (!ref) ref = hsh;
+    if (hsh != ref) {  // every shape must give the same closing
+      rc = fail(c, GM_ERR_STATE, "launch shape changed the closing");
+      break;
+    }
+    std::vector<double> t;
+    for (int k = 0; k < reps && rc == GM_OK; ++k) {
+      const auto t0 = std::chrono::steady_clock::now();
+      rc = gm_close(c, se);
+      if (rc == GM_OK) rc = gm_sync(c);
+      t.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
+    if (rc) break;
+    std::sort(t.begin(), t.end());
+    const double ms = t[t.size() / 2];
+    if (ms < best) {
+      best = ms;
+      bestZ = shp[0];
+      bestW = shp[1];
+    }
+  }
+  c->forceTileZ = c->forceWaveCap = 0;
+  c->timing = timing;
+  touch(c);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lk(g_tuneMu);
+    g_tune[TuneKey{c->R, c->rmode == 0 ? c->r : -1, se}] = TuneShape{bestZ, bestW, best};
+  }
+  if (path && *path) write_tune_file(path);
+  if (tile_planes) *tile_planes = bestZ;
+  if (waves) *waves = bestW;
+  return GM_OK;
+}
+
+int gm_tune_load(const char* path) {
+  if (!path) return GM_ERR_ARG;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return GM_ERR_ARG;
+  char line[512];
+  int n = 0;
+  while (std::fgets(line, sizeof(line), f)) {
+    int R = 0, r = 0, se = 0, z = 0, w = 0;
+    double ms = 0.0;
+    if (std::sscanf(line, " {\"base_res\": %d, \"radius\": %d, \"se\": %d, \"tile_planes\": %d, \"waves\": %d, \"ms\": %lf",
+                    &R, &r, &se, &z, &w, &ms) == 6 &&
+        (z == kTileZ || z == kTileZMin) && w >= 1 && w <= 8 && (se == 0 || se == 1)) {
+      std::lock_guard<std::mutex> lk(g_tuneMu);
+      g_tune[TuneKey{R, r, se}] = TuneShape{z, w, ms};
+      ++n;
+    }
+  }
+  std::fclose(f);
+  return n > 0 ? GM_OK : GM_ERR_ARG;
+}
+
 // ---- Batched closing + mesh of many contexts (one launch per step).
 struct gm_batch {
   int device = 0;
@@ -1017,6 +1321,12 @@ struct gm_batch {
   u32* queue = nullptr;
   size_t capQueue = 0;  // words
   int qsel = 0;
+  // the captured launch sequence of the last batch and what it was built for
+  // (the contexts, their launch-argument generations, the element, the
+  // queue toggle): replayed when the same batch is closed again
+  hipGraphExec_t exec = nullptr;
+  std::vector<uint64_t> key;
+  bool graphs = true;
 };
 
 namespace {
@@ -1089,12 +1399,19 @@ int gm_batch_destroy(gm_batch* b) {
   dfree(b->dMorph);
   dfree(b->dMc);
   dfree(b->queue);
+  if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
   return GM_OK;
 }
 
 const char* gm_batch_last_error(const gm_batch* b) { return b ? b->err.c_str() : ""; }
+
+int gm_batch_set_graphs(gm_batch* b, int enable) {
+  if (!b) return GM_ERR_ARG;
+  b->graphs = enable != 0;
+  return GM_OK;
+}
 
 int gm_batch_close_mesh(gm_batch* b, gm_ctx* const* ctxs, int n, int se) {
   if (!b || (!ctxs && n > 0) || n < 0) return GM_ERR_ARG;
@@ -1128,6 +1445,8 @@ int gm_batch_close_mesh(gm_batch* b, gm_ctx* const* ctxs, int n, int se) {
       gm_ctx* c = grouped[k];
       mj[k].d = morph_dims(c);
       mj[k].b = morph_buffers(c);
+      mj[k].b.tileZ = 0;  // the batched launch counts tiles of the default depth
+      mj[k].b.waveCap = 0;
       mj[k].r = c->r;
       cj[k].C = c->C;
       cj[k].R = c->R;
@@ -1146,6 +1465,7 @@ int gm_batch_close_mesh(gm_batch* b, gm_ctx* const* ctxs, int n, int se) {
       BHIP(b, hipHostMalloc((void**)&b->hTot, (size_t)m * 16, hipHostMallocDefault));
       b->capTot = (size_t)m * 4;
     }
+    BHIP(b, mc_prepare());
     const size_t cap = batch_queue_cap(mj.data(), m);
     const size_t words = kQueueLists + 256 * cap;
     if (words > b->capQueue) {
@@ -1157,10 +1477,48 @@ int gm_batch_close_mesh(gm_batch* b, gm_ctx* const* ctxs, int n, int se) {
       b->capQueue = words;
       b->qsel = 0;
     }
-    hipError_t e = launch_close_batched(mj.data(), m, b->hMorph, b->dMorph, b->queue, (int)cap, &b->qsel, b->stream);
-    if (e != hipSuccess) return bfail(b, GM_ERR_HIP, std::string("batched closing: ") + hipGetErrorString(e));
-    e = launch_mc_batched(cj.data(), m, b->hMc, b->dMc, b->hTot, b->stream);
-    if (e != hipSuccess) return bfail(b, GM_ERR_HIP, std::string("batched mesh extraction: ") + hipGetErrorString(e));
+    std::vector<uint64_t> key = {(uint64_t)m, (uint64_t)se, (uint64_t)b->qsel, (uint64_t)(uintptr_t)b->queue,
+                                 (uint64_t)(uintptr_t)b->dMorph, (uint64_t)(uintptr_t)b->dMc};
+    for (gm_ctx* c : grouped) {
+      key.push_back((uint64_t)(uintptr_t)c);
+      key.push_back(c->graphGen);
+    }
+    auto enqueue = [&]() -> int {
+      hipError_t e = launch_close_batched(mj.data(), m, b->hMorph, b->dMorph, b->queue, (int)cap, &b->qsel, b->stream);
+      if (e != hipSuccess) return bfail(b, GM_ERR_HIP, std::string("batched closing: ") + hipGetErrorString(e));
+      e = launch_mc_batched(cj.data(), m, b->hMc, b->dMc, b->hTot, b->stream);
+      if (e != hipSuccess) return bfail(b, GM_ERR_HIP, std::string("batched mesh extraction: ") + hipGetErrorString(e));
+      return GM_OK;
+    };
+    if (!b->graphs) {
+      int rc2 = enqueue();
+      if (rc2) return rc2;
+    } else if (b->exec && key == b->key) {
+      // the host tables of this batch are still in the pinned blobs the
+      // captured copies read
+      BHIP(b, hipGraphLaunch(b->exec, b->stream));
+    } else {
+      if (b->exec) hipGraphExecDestroy(b->exec);
+      b->exec = nullptr;
+      b->key.clear();
+      BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+      const int rc2 = enqueue();
+      hipGraph_t graph = nullptr;
+      const hipError_t ec = hipStreamEndCapture(b->stream, &graph);
+      if (rc2 != GM_OK || ec != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        if (rc2) return rc2;
+        return bfail(b, GM_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+      }
+      const hipError_t ei = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      if (ei != hipSuccess) {
+        b->exec = nullptr;
+        return bfail(b, GM_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+      }
+      b->key = key;
+      BHIP(b, hipGraphLaunch(b->exec, b->stream));
+    }
     BHIP(b, hipStreamSynchronize(b->stream));
     for (int k = 0; k < m; ++k) {
       std::memcpy(grouped[k]->hTotals, b->hTot + 4 * k, 16);

@@ -85,8 +85,8 @@ struct Slot {
   int capR = 0;            // largest base resolution this slot has run
   uint64_t reserved = 0;   // device bytes the slot holds (estimate)
   std::thread th;
-  Pinned<float> inP, outV, outN;
-  Pinned<uint32_t> inT, outT;
+  Pinned<float> outV, outN;
+  Pinned<uint32_t> outT;
   // hand-over to the batcher: the loaded job's structuring element, then the
   // outcome of its closing
   int se = 0;
@@ -111,8 +111,8 @@ uint64_t context_bytes(int R, int L, const gm_job& j) {
   b += 2 * (N / 8) * (N / 8) * W;                 // brick summaries
   b += (uint64_t)R * R * R;                       // cell-byte staging
   b += 7 * (P / 8) * (P / 8) * (P / 8) * 4;       // MC block counts, list positions, emit order
-  b += (P / 8) * (P / 8) * (P / 8) * 512 * 8;     // MC cube cases of listed blocks
-  b += (uint64_t)R * R * R * 4;                   // MC cell bases
+  const uint64_t listCap = std::min<uint64_t>((P / 8) * (P / 8) * (P / 8), std::max<uint64_t>(64, 8 * (P / 8) * (P / 8)));
+  b += listCap * 512 * 12;                        // MC cube cases and cell bases of listed blocks (initial)
   b += 4 * N * N * 24 + 8 * N * N * 12;           // mesh output (initial capacity)
   b += gm::tile_queue_words(R) * 4;               // active-tile queues
   if (L > 0) {
@@ -186,10 +186,9 @@ struct gm_engine {
       j->status = GM_ERR_HIP;
       j->err = "GMorpho Error: pinned host buffer allocation failed";
     };
-    if (!s.inP.reserve(j->P.size()) || !s.inT.reserve(j->T.size())) return oom();
-    std::memcpy(s.inP.p, j->P.data(), j->P.size() * sizeof(float));
-    std::memcpy(s.inT.p, j->T.data(), j->T.size() * sizeof(uint32_t));
-    if (!step(gm_load(c, s.inP.p, nullptr, d.num_of_vertices, s.inT.p, d.num_of_faces, d.base_res, d.se_size)))
+    // gm_load uploads through the context's own pinned staging
+    if (!step(gm_load(c, j->P.data(), nullptr, d.num_of_vertices, j->T.data(), d.num_of_faces, d.base_res,
+                      d.se_size)))
       return;
     if (!j->lattice.empty()) {
       if (!step(gm_scale_lattice_f32(c, j->lattice.data(), d.lattice_res))) return;
@@ -347,8 +346,6 @@ int gm_engine_destroy(gm_engine* e) {
   if (e->batch) gm_batch_destroy(e->batch);
   for (auto& s : e->slots) {
     if (s.ctx) gm_destroy(s.ctx);
-    s.inP.release();
-    s.inT.release();
     s.outV.release();
     s.outN.release();
     s.outT.release();

@@ -91,7 +91,8 @@ constexpr int kMaxRhoFast = 32;  // streaming kernel supports m, |offset| <= 32
 
 // Launch configuration of the streaming morphology kernels.
 constexpr int kTileRows = 64;  // lanes per workgroup (one output row each)
-constexpr int kTileZ = 32;     // planes per tile
+constexpr int kTileZ = 32;     // planes per tile (default)
+constexpr int kTileZMin = 16;  // the smallest tile depth a launch may use (queues are sized for it)
 
 struct MorphDims {
   int R, N, W;
@@ -110,6 +111,17 @@ hipError_t launch_cells_to_rows(const uint8_t* cells, u32* rows, int R, hipStrea
 hipError_t launch_rows_to_cells(const u32* rows, uint8_t* cells, int R, hipStream_t s);
 
 // ---- voxelizer (gm_voxelize.hip)
+// Index check and bounding box of a device mesh: out (8 u32, initialised to
+// 0, 0, ~0, ~0, ~0, 0, 0, 0) receives the bad-index flag, the non-finite
+// flag and order-preserving keys of the per-axis min and max coordinates.
+hipError_t launch_mesh_check(const float* P, int nv, const u32* T, int nt, u32* out, hipStream_t s);
+// The float a launch_mesh_check key stands for.
+inline float mesh_key_float(u32 k) {
+  const u32 u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
 hipError_t launch_voxelize(const float* P, int nv, const u32* T, int nt, const float* bmin3, double scale,
                            int R, int32_t* snap, u32* mark, u32* shell, u32* rows, hipStream_t s);
 
@@ -142,6 +154,10 @@ struct MorphBuffers {
   u32* kv;
   u32* kbl;
   int rpBuild;  // rebuild rp and rvset from the radius field first
+  // launch shape of the streaming passes (0: defaults): planes per tile
+  // (kTileZMin or kTileZ) and the most waves per workgroup
+  int tileZ;
+  int waveCap;
 };
 // Active-tile queues for base resolution R (u32 words): two blocks of 16
 // counters (uniform radius) at word 0, two blocks of 32 x 8 counters (one
@@ -152,13 +168,13 @@ constexpr int kVarCounters = 32;
 constexpr int kQueueLists = 544;
 inline size_t tile_count_max(int R) {
   const size_t N = 2 * (size_t)R, W = N / 32;
-  return W * ((N + kTileRows - 1) / kTileRows) * ((N + kTileZ - 1) / kTileZ);
+  return W * ((N + kTileRows - 1) / kTileRows) * ((N + kTileZMin - 1) / kTileZMin);
 }
 inline size_t var_queue_base(int R) { return kQueueLists + 8 * tile_count_max(R); }
 // Tiles of one per-XCD queue: each run of 8 slabs puts one slab in every queue.
 inline size_t var_queue_cap(int R) {
   const size_t N = 2 * (size_t)R, W = N / 32;
-  const size_t slabs = ((N + kTileRows - 1) / kTileRows) * ((N + kTileZ - 1) / kTileZ);
+  const size_t slabs = ((N + kTileRows - 1) / kTileRows) * ((N + kTileZMin - 1) / kTileZMin);
   return (slabs + 7) / 8 * W;
 }
 inline size_t tile_queue_words(int R) { return var_queue_base(R) + 256 * var_queue_cap(R); }
@@ -259,7 +275,7 @@ void ct_register_part7(CtKernels&);
 struct MCBuffers {
   u32* blockV;   // per 8^3 Morton block: vertex count, then exclusive offset
   u32* blockT;
-  u32* cellV;    // dense R^3: block-local vertex base of cells with vertices
+  u32* cellV;    // per listed block (list order) and cell (local Morton index): block-local vertex base
   unsigned long long* cellCase;  // per listed block (list order) and cell: its 8 cube cases
   u32* posOf;    // per block: its index in list, ~0u when not listed
   u32* order;    // emit order of the listed blocks: 4 class counts, then 4 x nblocks list positions
@@ -267,14 +283,18 @@ struct MCBuffers {
   float* Nrm;
   u32* T;
   u32* late;     // [0] count, capLate (slot, a, b, c) quadruples, then their copy in slot order
-  u32* totals;   // [0] nV [1] nT [2] overflow [3] border flag
+  u32* totals;   // [0] nV [1] nT [2] overflow (1 output, 2 late list, 4 block list) [3] border flag
+                 // [4] cubes listing more triangles than they count (8 words)
   u32* scanPart; // per 1024 blocks: vertex and triangle totals (mc_scan_part_bytes)
   uint8_t* brick; // brick summary of the closing grid (1 word x 8 rows x 8 planes)
   u32* list;      // [0] number of active blocks, then their ids
   u32 capV, capT, capLate;
+  u32 capList;   // listed blocks cellV and cellCase hold
   int nblocks;   // (P/8)^3
   int P;         // padded power-of-two cells per axis
 };
+// One-time set-up of the marching-cubes tables (call before a capture).
+hipError_t mc_prepare();
 hipError_t launch_mc(const u32* C, int R, const float* bmin3, float cell, const MCBuffers& m, hipStream_t s,
                      hipEvent_t* ev);
 size_t mc_scan_parts(int nblocks);

@@ -24,8 +24,9 @@
 //              vertex (position, area-weighted normal sum gathered from the
 //              <= 4 cubes around its edge)
 //   mc_fixup   triangles the count table does not cover, mc_normalize.
-// The per-cell vertex base map is dense (R^3 u32) but only written and read
-// at surface cells, replacing the reference's 4*N^3-byte memset + splat.
+// The cube cases and block-local vertex bases are stored per listed block
+// (512 cells each, by list position), replacing the reference's 4*N^3-byte
+// memset + splat; their capacity grows when more blocks are listed.
 #include <algorithm>
 #include <mutex>
 
@@ -287,11 +288,11 @@ struct BlockLds {
 // without mixed cubes); *blkOut = its block.
 __device__ __forceinline__ unsigned long long listed_cell_cases(const unsigned long long* __restrict__ cellCase,
                                                                 const u32* __restrict__ posOf, int R, int cx, int cy,
-                                                                int cz, u32* code) {
+                                                                int cz, u32* code, u32* pos) {
   if (!cell_in_m_range(cx, cy, cz, R)) return 0ull;
   *code = part1by2((u32)cx) | (part1by2((u32)cy) << 1) | (part1by2((u32)cz) << 2);
-  const u32 pos = posOf[*code >> 9];
-  return pos == ~0u ? 0ull : cellCase[(size_t)pos * 512 + (*code & 511u)];
+  *pos = posOf[*code >> 9];
+  return *pos == ~0u ? 0ull : cellCase[(size_t)*pos * 512 + (*code & 511u)];
 }
 
 // Stages listed block `it` (origin cell c0): the cube cases of its 512 cells
@@ -307,10 +308,10 @@ __device__ void stage_block(BlockLds& S, const unsigned long long* __restrict__ 
   if (t < kNbCells) {
     int lx, ly, lz;
     nb_cell(t, lx, ly, lz);
-    u32 code = 0u;
-    above = listed_cell_cases(cellCase, posOf, R, c0x + lx, c0y + ly, c0z + lz, &code);
-    if (above) nbB = blockV[code >> 9] + cellL[(size_t)(c0x + lx) + (size_t)R * ((size_t)(c0y + ly) + (size_t)R * (c0z + lz))];
-    below = listed_cell_cases(cellCase, posOf, R, c0x + 7 - lx, c0y + 7 - ly, c0z + 7 - lz, &code);
+    u32 code = 0u, pos = 0u;
+    above = listed_cell_cases(cellCase, posOf, R, c0x + lx, c0y + ly, c0z + lz, &code, &pos);
+    if (above) nbB = blockV[code >> 9] + cellL[(size_t)pos * 512 + (code & 511u)];
+    below = listed_cell_cases(cellCase, posOf, R, c0x + 7 - lx, c0y + 7 - ly, c0z + 7 - lz, &code, &pos);
   }
   const unsigned long long* own = cellCase + (size_t)it * 512;
   for (int l = t; l < 512; l += blockDim.x) {
@@ -424,10 +425,11 @@ __device__ __forceinline__ void mc_blocks_body(const uint8_t* __restrict__ brick
 }
 __global__ void __launch_bounds__(256) k_mc_blocks(const uint8_t* __restrict__ brick, int R, int nblocks,
                                                    u32* blockV, u32* blockT, u32* list, u32* posOf, u32* order,
-                                                   u32* part) {
+                                                   u32* part, u32* totals) {
   // the emit-order class counts start from zero in the scan that follows,
   // and the chunk totals in the count kernel (kPartStride words per chunk)
   if (blockIdx.x == 0 && threadIdx.x < kOrderClasses) order[threadIdx.x] = 0u;
+  if (blockIdx.x == 0 && threadIdx.x == 0) totals[4] = 0u;  // late cubes, tallied by the count kernel
   const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (gid < (nblocks + kChunk - 1) / kChunk) {
     part[kPartStride * gid] = 0u;
@@ -441,9 +443,13 @@ __global__ void __launch_bounds__(256) k_mc_blocks(const uint8_t* __restrict__ b
 constexpr int kCountThreads = 256;
 constexpr int kCountCells = 512 / kCountThreads;
 
-// Triangle counts per case, the only table the count kernels read.
-__device__ __forceinline__ void load_ntri(uint8_t* ntri) {
-  for (int i = threadIdx.x; i < 256; i += blockDim.x) ntri[i] = (uint8_t)g_mc.ntri[i];
+// Triangle counts per case and the listed triangles beyond them (case 91),
+// the only tables the count kernels read.
+__device__ __forceinline__ void load_ntri(uint8_t* ntri, uint8_t* xtra) {
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+    ntri[i] = (uint8_t)g_mc.ntri[i];
+    xtra[i] = (uint8_t)(g_mc.len[i] - g_mc.ntri[i]);
+  }
 }
 
 // Vertex and triangle counts of listed block `it`, the block-local vertex
@@ -453,7 +459,8 @@ __device__ __forceinline__ void load_ntri(uint8_t* ntri) {
 // scan chunk's totals (kPartStride words per chunk, one cache line each).
 __device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R, u32 it, int blk, u32* blockV,
                                                u32* blockT, u32* cellL, unsigned long long* cellCase, Box& bx,
-                                               u32* red, const uint8_t* ntri, u32* part = nullptr) {
+                                               u32* red, const uint8_t* ntri, const uint8_t* xtra, u32* totals,
+                                               u32* part = nullptr) {
   int c0x, c0y, c0z;
   block_outside(blk, R, c0x, c0y, c0z);
   // cells of the block need fine voxels [16b, 16b+17) per axis
@@ -463,7 +470,7 @@ __device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R,
   u32 u = 0u;
   for (int w = 0; w < (int)(blockDim.x >> 6); ++w) u |= bx.uni[w];
   const bool nonuni = u == 3u;
-  u32 nvh[kCountCells] = {}, nv = 0, nt = 0;
+  u32 nvh[kCountCells] = {}, nv = 0, nt = 0, nx = 0;
   unsigned long long w[kCountCells] = {};
   if (nonuni) {
 #pragma unroll
@@ -475,9 +482,15 @@ __device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R,
       nvh[h] = own_prefix8(w[h], 8);
       nv += nvh[h];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) nt += ntri[(w[h] >> (8 * j)) & 0xffu];
+      for (int j = 0; j < 8; ++j) {
+        nt += ntri[(w[h] >> (8 * j)) & 0xffu];
+        nx += xtra[(w[h] >> (8 * j)) & 0xffu];
+      }
     }
   }
+  // listed-but-uncounted triangles anywhere: the emit then leaves the normal
+  // sums for the fixup (rare; one atomic per thread that has any)
+  if (nx) atomicAdd(&totals[4], nx);
   {
     unsigned long long* dst = cellCase + (size_t)it * 512 + kCountCells * threadIdx.x;
 #pragma unroll
@@ -491,15 +504,11 @@ __device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R,
   u32 off = block_exscan(nv | (nt << 16), red, &tot, false) & 0xffffu;
   const u32 tv = tot & 0xffffu, tt = tot >> 16;
   // block-local vertex base of the cells with vertices (cells are in Morton
-  // order, consecutive per thread), read for the edges owned by another
-  // block's cube
+  // order, consecutive per thread; stored by list position and local Morton
+  // index), read for the edges owned by another block's cube
 #pragma unroll
   for (int h = 0; h < kCountCells; ++h) {
-    if (nvh[h]) {
-      int cx, cy, cz;
-      block_cell(blk, kCountCells * threadIdx.x + h, cx, cy, cz);
-      cellL[(size_t)cx + (size_t)R * ((size_t)cy + (size_t)R * cz)] = off;
-    }
+    if (nvh[h]) cellL[(size_t)it * 512 + kCountCells * threadIdx.x + h] = off;
     off += nvh[h];
   }
   if (threadIdx.x == 0) {
@@ -519,17 +528,19 @@ __device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R,
 __global__ void __launch_bounds__(kCountThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) k_mc_count(const u32* __restrict__ C, int R,
                                                           const u32* __restrict__ list, u32* blockV,
                                                           u32* blockT, u32* cellL, unsigned long long* cellCase,
-                                                          u32* late, u32* part) {
+                                                          u32* late, u32* part, u32 capList, u32* totals) {
   __shared__ Box bx;
   __shared__ u32 red[16];
-  __shared__ uint8_t ntri[256];
+  __shared__ uint8_t ntri[256], xtra[256];
   // the late-triangle count starts from zero in the emit kernel that follows
   if (blockIdx.x == 0 && threadIdx.x == 0) late[0] = 0u;
   const u32 n = list[0];
-  if (blockIdx.x >= n) return;
-  load_ntri(ntri);
+  // more listed blocks than the per-block buffers hold: nothing is stored,
+  // the scan flags it and the host grows them and extracts again
+  if (blockIdx.x >= n || n > capList) return;
+  load_ntri(ntri, xtra);
   for (u32 it = blockIdx.x; it < n; it += gridDim.x) {
-    mc_count_block(C, R, it, (int)list[1 + it], blockV, blockT, cellL, cellCase, bx, red, ntri, part);
+    mc_count_block(C, R, it, (int)list[1 + it], blockV, blockT, cellL, cellCase, bx, red, ntri, xtra, totals, part);
     __syncthreads();  // the box is reloaded by the next iteration
   }
 }
@@ -559,7 +570,8 @@ __device__ __forceinline__ void scan_reduce_body(const u32* __restrict__ a, cons
 // overall totals and the overflow flag.
 __device__ __forceinline__ void scan_down_body(u32* a, u32* b, int n, const u32* __restrict__ part, int pstride,
                                                int nparts, int chunk, u32* totals, u32 capV, u32 capT,
-                                               const u32* __restrict__ posOf = nullptr, u32* order = nullptr) {
+                                               const u32* __restrict__ posOf = nullptr, u32* order = nullptr,
+                                               const u32* __restrict__ list = nullptr, u32 capList = 0) {
   __shared__ u32 red[16];
   __shared__ u32 cls[2 * kOrderClasses];
   int base = chunk * kChunk + 4 * threadIdx.x;
@@ -586,7 +598,7 @@ __device__ __forceinline__ void scan_down_body(u32* a, u32* b, int n, const u32*
     if (threadIdx.x == 0) {
       totals[0] = offA;
       totals[1] = offB;
-      totals[2] = (offA > capV || offB > capT) ? 1u : 0u;
+      totals[2] = ((offA > capV || offB > capT) ? 1u : 0u) | (list && list[0] > capList ? 4u : 0u);
       totals[3] = 0u;
     }
     offA = offB = 0u;
@@ -627,8 +639,8 @@ __device__ __forceinline__ void scan_down_body(u32* a, u32* b, int n, const u32*
 }
 __global__ void __launch_bounds__(256) k_scan_down(u32* a, u32* b, int n, const u32* __restrict__ part, int nparts,
                                                    u32* totals, u32 capV, u32 capT, const u32* __restrict__ posOf,
-                                                   u32* order) {
-  scan_down_body(a, b, n, part, kPartStride, nparts, blockIdx.x, totals, capV, capT, posOf, order);
+                                                   u32* order, const u32* __restrict__ list, u32 capList) {
+  scan_down_body(a, b, n, part, kPartStride, nparts, blockIdx.x, totals, capV, capT, posOf, order, list, capList);
 }
 
 // Position of the vertex on owned edge e (0: x, 3: y, 8: z) of cube f:
@@ -806,8 +818,19 @@ __device__ __forceinline__ unsigned long long cell_cases(const BlockLds& S, int 
 // Vertices of a staged block, in two phases per window of kListCap local
 // vertex ids: the mixed cubes write (cube, owned edge) at their local id,
 // then every thread takes consecutive ids (position + gathered normal sum).
+__device__ __forceinline__ void unit3(float* n) {
+  const float l = sqrtf(__fadd_rn(__fadd_rn(__fmul_rn(n[0], n[0]), __fmul_rn(n[1], n[1])), __fmul_rn(n[2], n[2])));
+  if (l > 0.f) {
+    n[0] = __fdiv_rn(n[0], l);
+    n[1] = __fdiv_rn(n[1], l);
+    n[2] = __fdiv_rn(n[2], l);
+  }
+}
+
+// unit: the sums are final (no listed-but-uncounted triangle anywhere), so
+// they are stored normalized (NormalizeMeshNormals, GMorpho.cu:6526-6537).
 __device__ void mc_verts_block(BlockLds& S, int R, Geo geo, int blk, int c0x, int c0y, int c0z, bool useFn, u32 base,
-                               u32 nverts, float* V, float* Nrm, u32 capV) {
+                               u32 nverts, float* V, float* Nrm, u32 capV, bool unit) {
   for (u32 w0 = 0; w0 < nverts; w0 += kListCap) {
     for (int l = threadIdx.x; l < 512; l += blockDim.x) {
       const unsigned long long w = cell_cases(S, l);
@@ -843,6 +866,7 @@ __device__ void mc_verts_block(BlockLds& S, int R, Geo geo, int blk, int c0x, in
       edge_pos_v(geo, fx, fy, fz, q == 0 ? 0 : (q == 1 ? 3 : 8), (u32)v & 1u,
                  (u32)(v >> (q == 0 ? 1 : (q == 1 ? 3 : 4))) & 1u, p);
       normal_sum(S, geo, R, c0x, c0y, c0z, useFn, fx, fy, fz, q, v, p, nn);
+      if (unit) unit3(nn);
       V[3 * (size_t)vid] = p[0];
       V[3 * (size_t)vid + 1] = p[1];
       V[3 * (size_t)vid + 2] = p[2];
@@ -980,7 +1004,7 @@ __device__ __forceinline__ void mc_emit_block(BlockLds& S, int R, Geo geo, u32 i
   const bool useFn = tnext - tbase <= (u32)kFaceCap && tnext <= capT;
   mc_tris_block(S, R, geo, blk, c0x, c0y, c0z, useFn, base, tbase, tnext - tbase, blockV, cellL, T, capT, late,
                 capLate, totals);
-  mc_verts_block(S, R, geo, blk, c0x, c0y, c0z, useFn, base, next - base, V, Nrm, capV);
+  mc_verts_block(S, R, geo, blk, c0x, c0y, c0z, useFn, base, next - base, V, Nrm, capV, totals[4] == 0u);
 }
 
 // Vertices and triangles of the active blocks: each listed block is staged
@@ -996,8 +1020,9 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_emit(int R, Geo geo, const u3
                                                          const u32* __restrict__ posOf,
                                                          const u32* __restrict__ order, float* V, float* Nrm,
                                                          u32 capV, u32* T, u32 capT, u32* late, u32 capLate,
-                                                         u32* totals) {
+                                                         u32* totals, u32 capList) {
   __shared__ BlockLds S;
+  if (list[0] > capList) return;  // the per-block buffers overflowed: extracted again
   u32 nwork = 0u;  // blocks with output
   for (int c = 0; c < kOrderClasses; ++c) nwork += order[c];
   if (blockIdx.x >= nwork) return;  // the grid is sized for every block of the grid
@@ -1031,7 +1056,7 @@ __device__ __forceinline__ void mc_fixup_body(const float* V, float* Nrm, u32* T
                                               u32* totals, u32 capV, u32 capT) {
   u32 n = late[0];
   if (n > capLate) n = capLate;
-  if (n == 0) return;
+  if (n == 0) return;  // uniform over the workgroup
   u32* sorted = late + 1 + 4 * (size_t)capLate;
   for (u32 i = threadIdx.x; i < n; i += blockDim.x) {
     const u32 si = late[1 + 4 * i];
@@ -1044,9 +1069,8 @@ __device__ __forceinline__ void mc_fixup_body(const float* V, float* Nrm, u32* T
   }
   __threadfence_block();
   __syncthreads();
-  if (threadIdx.x != 0) return;
   const u32 nT = totals[1], nV = totals[0];
-  if (nT > capT || nV > capV) return;
+  if (threadIdx.x != 0 || nT > capT || nV > capV) return;
   for (u32 i = 0; i < n; ++i) {
     const u32 s = sorted[4 * i];
     const u32 a = sorted[4 * i + 1], b = sorted[4 * i + 2], c = sorted[4 * i + 3];
@@ -1065,31 +1089,36 @@ __device__ __forceinline__ void mc_fixup_body(const float* V, float* Nrm, u32* T
       for (int q = 0; q < 3; ++q) Nrm[3 * (size_t)t[k] + q] = __fadd_rn(Nrm[3 * (size_t)t[k] + q], f[q]);
   }
 }
-constexpr int kFixupThreads = 256;
-__global__ void __launch_bounds__(kFixupThreads) k_mc_fixup(const float* V, float* Nrm, u32* T, u32* late,
-                                                            u32 capLate, u32* totals, u32 capV, u32 capT) {
-  mc_fixup_body(V, Nrm, T, late, capLate, totals, capV, capT);
-}
-
 // NormalizeMeshNormals (GMorpho.cu:6526-6537).
 __device__ __forceinline__ void mc_normalize_body(float* Nrm, const u32* totals, u32 capV, size_t first,
                                                   size_t stride) {
   u32 nV = totals[0];
   if (nV > capV) nV = capV;
   for (size_t i = first; i < nV; i += stride) {
-    float x = Nrm[3 * i], y = Nrm[3 * i + 1], z = Nrm[3 * i + 2];
-    float l = sqrtf(__fadd_rn(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y)), __fmul_rn(z, z)));
-    if (l > 0.f) {
-      Nrm[3 * i] = __fdiv_rn(x, l);
-      Nrm[3 * i + 1] = __fdiv_rn(y, l);
-      Nrm[3 * i + 2] = __fdiv_rn(z, l);
-    }
+    float n[3] = {Nrm[3 * i], Nrm[3 * i + 1], Nrm[3 * i + 2]};
+    unit3(n);
+    Nrm[3 * i] = n[0];
+    Nrm[3 * i + 1] = n[1];
+    Nrm[3 * i + 2] = n[2];
   }
 }
-__global__ void k_mc_normalize(float* Nrm, const u32* totals, u32 capV) {
-  mc_normalize_body(Nrm, totals, capV, (size_t)blockIdx.x * blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x);
-}
 
+// The mesh's last step: nothing when no cube lists more triangles than it
+// counts (the emit stored unit normals); otherwise the late triangles are
+// applied, then every normal is normalized, by one workgroup (rare).
+constexpr int kFinishThreads = 1024;
+__device__ __forceinline__ void mc_finish_body(const float* V, float* Nrm, u32* T, u32* late, u32 capLate,
+                                               u32* totals, u32 capV, u32 capT) {
+  if (totals[4] == 0u) return;
+  mc_fixup_body(V, Nrm, T, late, capLate, totals, capV, capT);
+  __threadfence_block();
+  __syncthreads();
+  mc_normalize_body(Nrm, totals, capV, threadIdx.x, blockDim.x);
+}
+__global__ void __launch_bounds__(kFinishThreads) k_mc_finish(const float* V, float* Nrm, u32* T, u32* late,
+                                                              u32 capLate, u32* totals, u32 capV, u32 capT) {
+  mc_finish_body(V, Nrm, T, late, capLate, totals, capV, capT);
+}
 
 // ---- Batched forms: many closed grids in one launch per step.
 struct MCJob {
@@ -1105,6 +1134,7 @@ __global__ void __launch_bounds__(256) k_mc_blocks_b(const MCJob* __restrict__ j
   // each job's emit order is built too (a mesh that overflows its output is
   // emitted again by the single-job emit, which walks that order)
   if (blockIdx.x == pre[j] && threadIdx.x < kOrderClasses) J.m.order[threadIdx.x] = 0u;
+  if (blockIdx.x == pre[j] && threadIdx.x == 0) J.m.totals[4] = 0u;
   mc_blocks_body(J.m.brick, J.R, J.m.nblocks, J.m.blockV, J.m.blockT, J.m.list, J.m.posOf,
                  (int)(blockIdx.x - pre[j]) * 256 + (int)threadIdx.x);
 }
@@ -1122,14 +1152,15 @@ __global__ void __launch_bounds__(kCountThreads) k_mc_count_b(const MCJob* __res
                                                             int n) {
   __shared__ Box bx;
   __shared__ u32 red[16];
-  __shared__ uint8_t ntri[256];
+  __shared__ uint8_t ntri[256], xtra[256];
   if (blockIdx.x >= lpre[n]) return;
-  load_ntri(ntri);
+  load_ntri(ntri, xtra);
   for (u32 u = blockIdx.x; u < lpre[n]; u += gridDim.x) {
     const int j = batch_job(lpre, n, u);
     const MCJob& J = jobs[j];
+    if (J.m.list[0] > J.m.capList) continue;  // uniform per workgroup; flagged by the scan
     mc_count_block(J.C, J.R, u - lpre[j], (int)J.m.list[1 + u - lpre[j]], J.m.blockV, J.m.blockT, J.m.cellV,
-                   J.m.cellCase, bx, red, ntri);
+                   J.m.cellCase, bx, red, ntri, xtra, J.m.totals);
     __syncthreads();
   }
 }
@@ -1144,7 +1175,8 @@ __global__ void __launch_bounds__(256) k_scan_down_b(const MCJob* __restrict__ j
   const int j = batch_job(pre, n, blockIdx.x);
   const MCJob& J = jobs[j];
   scan_down_body(J.m.blockV, J.m.blockT, J.m.nblocks, J.m.scanPart, 2, (J.m.nblocks + kChunk - 1) / kChunk,
-                 (int)(blockIdx.x - pre[j]), J.m.totals, J.m.capV, J.m.capT, J.m.posOf, J.m.order);
+                 (int)(blockIdx.x - pre[j]), J.m.totals, J.m.capV, J.m.capT, J.m.posOf, J.m.order, J.m.list,
+                 J.m.capList);
 }
 __global__ void __launch_bounds__(kMcThreads) k_mc_emit_b(const MCJob* __restrict__ jobs, const u32* __restrict__ lpre,
                                                            int n) {
@@ -1155,21 +1187,15 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_emit_b(const MCJob* __restric
     __syncthreads();  // LDS of the previous block is no longer read
     const int j = batch_job(lpre, n, u);
     const MCJob& J = jobs[j];
+    if (J.m.list[0] > J.m.capList) continue;  // uniform per workgroup; flagged by the scan
     mc_emit_block(S, J.R, J.geo, u - lpre[j], (int)J.m.list[1 + u - lpre[j]], J.m.nblocks, J.m.blockV, J.m.blockT,
                   J.m.cellV, J.m.cellCase, J.m.posOf, J.m.V, J.m.Nrm, J.m.capV, J.m.T, J.m.capT, J.m.late,
                   J.m.capLate, J.m.totals);
   }
 }
-__global__ void __launch_bounds__(kFixupThreads) k_mc_fixup_b(const MCJob* __restrict__ jobs) {
+__global__ void __launch_bounds__(kFinishThreads) k_mc_finish_b(const MCJob* __restrict__ jobs) {
   const MCJob& J = jobs[blockIdx.x];
-  mc_fixup_body(J.m.V, J.m.Nrm, J.m.T, J.m.late, J.m.capLate, J.m.totals, J.m.capV, J.m.capT);
-}
-constexpr int kNormWgPerJob = 16;
-__global__ void __launch_bounds__(256) k_mc_normalize_b(const MCJob* __restrict__ jobs) {
-  const MCJob& J = jobs[blockIdx.x / kNormWgPerJob];
-  const int l = (int)(blockIdx.x % kNormWgPerJob);
-  mc_normalize_body(J.m.Nrm, J.m.totals, J.m.capV, (size_t)l * blockDim.x + threadIdx.x,
-                    (size_t)kNormWgPerJob * blockDim.x);
+  mc_finish_body(J.m.V, J.m.Nrm, J.m.T, J.m.late, J.m.capLate, J.m.totals, J.m.capV, J.m.capT);
 }
 __global__ void k_totals_b(const MCJob* __restrict__ jobs, int n, u32* out) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1205,6 +1231,8 @@ hipError_t upload_tables() {
 
 }  // namespace
 
+hipError_t mc_prepare() { return upload_tables(); }
+
 hipError_t mc_tables_host(u32* ntri, u32* len, unsigned long long* tri) {
   for (int c = 0; c < 256; ++c) {
     int n = 0;
@@ -1233,16 +1261,16 @@ hipError_t launch_mc(const u32* C, int R, const float* bmin3, float cell, const 
   // active Morton blocks from a brick summary of C
   GM_CHECK(launch_brick_summary(C, 2 * R, m.brick, m.list, s));
   hipLaunchKernelGGL(k_mc_blocks, dim3((m.nblocks + 255) / 256), dim3(256), 0, s, m.brick, R, m.nblocks, m.blockV,
-                     m.blockT, m.list, m.posOf, m.order, m.scanPart);
+                     m.blockT, m.list, m.posOf, m.order, m.scanPart, m.totals);
   GM_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_mc_count, dim3(mc_list_grid(m.nblocks, 2048)), dim3(kCountThreads), 0, s, C, R, m.list,
-                     m.blockV, m.blockT, m.cellV, m.cellCase, m.late, m.scanPart);
+                     m.blockV, m.blockT, m.cellV, m.cellCase, m.late, m.scanPart, m.capList, m.totals);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
   // the chunk totals were summed by the count kernel
   int nparts = (int)mc_scan_parts(m.nblocks);
   hipLaunchKernelGGL(k_scan_down, dim3(nparts), dim3(256), 0, s, m.blockV, m.blockT, m.nblocks, m.scanPart, nparts,
-                     m.totals, m.capV, m.capT, m.posOf, m.order);
+                     m.totals, m.capV, m.capT, m.posOf, m.order, m.list, m.capList);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   return launch_mc_emit(C, R, bmin3, cell, m, s, ev ? ev + 2 : nullptr);
@@ -1319,9 +1347,7 @@ hipError_t launch_mc_batched(const BatchMCJob* in, int n, void* hBlob, void* dBl
   GM_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_mc_emit_b, dim3(listGrid), dim3(kMcThreads), 0, s, dj, lpre, n);
   GM_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_mc_fixup_b, dim3(n), dim3(kFixupThreads), 0, s, dj);
-  GM_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_mc_normalize_b, dim3(n * kNormWgPerJob), dim3(256), 0, s, dj);
+  hipLaunchKernelGGL(k_mc_finish_b, dim3(n), dim3(kFinishThreads), 0, s, dj);
   GM_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_totals_b, dim3((n + 255) / 256), dim3(256), 0, s, dj, n, tot);
   GM_CHECK(hipGetLastError());
@@ -1335,13 +1361,11 @@ hipError_t launch_mc_emit(const u32* C, int R, const float* bmin3, float cell, c
   Geo geo{bmin3[0], bmin3[1], bmin3[2], 0.5f * cell};
   hipLaunchKernelGGL(k_mc_emit, dim3(mc_list_grid(m.nblocks, 16384)), dim3(kMcThreads), 0, s, R, geo, m.list,
                      m.nblocks, m.blockV, m.blockT, m.cellV, m.cellCase, m.posOf, m.order, m.V, m.Nrm, m.capV, m.T,
-                     m.capT, m.late, m.capLate, m.totals);
+                     m.capT, m.late, m.capLate, m.totals, m.capList);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
-  hipLaunchKernelGGL(k_mc_fixup, dim3(1), dim3(kFixupThreads), 0, s, m.V, m.Nrm, m.T, m.late, m.capLate, m.totals, m.capV,
-                     m.capT);
-  GM_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_mc_normalize, dim3(1024), dim3(256), 0, s, m.Nrm, m.totals, m.capV);
+  hipLaunchKernelGGL(k_mc_finish, dim3(1), dim3(kFinishThreads), 0, s, m.V, m.Nrm, m.T, m.late, m.capLate, m.totals,
+                     m.capV, m.capT);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   return hipSuccess;

@@ -538,9 +538,10 @@ size_t lds_bytes(const MorphArgs& a, bool var, int waves) {
   if (var) words += (size_t)nplanes * kTileRows;
   return words * 4;
 }
-// Waves per workgroup: as many as fit a 64 KiB LDS budget, at most 8.
+// Waves per workgroup: as many as fit a 64 KiB LDS budget, at most 8 (or the
+// launch shape's cap).
 int stream_waves(const MorphArgs& a, bool var) {
-  int w = kMaxWaves;
+  int w = a.waveCap > 0 ? std::min(a.waveCap, kMaxWaves) : kMaxWaves;
   while (w > 1 && lds_bytes(a, var, w) > 64 * 1024) --w;
   return w;
 }
@@ -553,7 +554,8 @@ MorphArgs base_args(const MorphDims& d, const MorphBuffers& b, const RadiusField
   a.dR0 = d.dR[0];
   a.dR1 = d.dR[1];
   a.dR2 = d.dR[2];
-  a.Zt = kTileZ;
+  a.Zt = b.tileZ == kTileZMin ? kTileZMin : kTileZ;
+  a.waveCap = b.waveCap;
   a.rf = rf;
   a.tables = b.tables;
   a.tabs = tabs;

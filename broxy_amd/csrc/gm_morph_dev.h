@@ -469,6 +469,7 @@ struct MorphArgs {
   int R, N, W;
   int dR0, dR1, dR2;
   int Zt;
+  int waveCap;  // most waves per streaming workgroup (host-side launch choice)
   RadiusField rf;
   int se;  // 0 sphere, 1 cube (dilation)
   const u32* tables;

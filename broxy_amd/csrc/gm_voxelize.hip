@@ -14,6 +14,8 @@
 // is the closed triangle/pixel-square overlap, depths are floored plane
 // values. The parity pass writes one toggle bit per covered pixel at the
 // triangle's depth; a suffix XOR along z then yields the filled columns.
+#include <algorithm>
+
 #include "gm_internal.h"
 
 namespace gm {
@@ -162,22 +164,88 @@ __global__ void k_vox_tris(const int32_t* __restrict__ X, const u32* __restrict_
 }
 
 // Suffix XOR along z per (row y, word wx) column, then OR with the shell.
-__global__ void k_vox_fill(const u32* __restrict__ mark, const u32* __restrict__ shell, u32* __restrict__ rows,
-                           int N) {
+// A workgroup owns 64 consecutive columns (one per lane, coalesced) and
+// splits z into kFillChunks chunks, one wave each: each wave XORs its chunk,
+// the chunk parities give every chunk the XOR of the chunks above it (LDS),
+// then each wave walks its chunk down from that value.
+constexpr int kFillChunks = 16;
+__global__ void __launch_bounds__(64 * kFillChunks) k_vox_fill(const u32* __restrict__ mark,
+                                                               const u32* __restrict__ shell, u32* __restrict__ rows,
+                                                               int N) {
+  __shared__ u32 par[kFillChunks][64];
   const int W = N / 32;
-  int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= N * W) return;
-  size_t col = (size_t)gid;  // = y*W + wx
-  size_t plane = (size_t)N * W;
-  u32 acc = 0;
-  for (int z = N - 1; z >= 0; --z) {
-    size_t i = (size_t)z * plane + col;
+  const int lane = (int)(threadIdx.x & 63), chunk = (int)(threadIdx.x >> 6);
+  const size_t col = (size_t)blockIdx.x * 64 + lane;  // = y*W + wx
+  const bool ok = col < (size_t)N * W;
+  const size_t plane = (size_t)N * W;
+  const int zc = N / kFillChunks, z0 = chunk * zc;
+  u32 x = 0u;
+  if (ok)
+    for (int z = z0; z < z0 + zc; ++z) x ^= mark[(size_t)z * plane + col];
+  par[chunk][lane] = x;
+  __syncthreads();
+  u32 acc = 0u;  // XOR of the chunks above this one
+  for (int c = chunk + 1; c < kFillChunks; ++c) acc ^= par[c][lane];
+  if (!ok) return;
+  for (int z = z0 + zc - 1; z >= z0; --z) {
+    const size_t i = (size_t)z * plane + col;
     acc ^= mark[i];
     rows[i] = acc | shell[i];
   }
 }
 
+// Float keys that order like the floats (for min / max by integer atomics).
+__device__ __forceinline__ u32 fkey(float f) {
+  const u32 u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// Mesh check before voxelization: out[0] = 1 if a triangle index is >= nv,
+// out[1] = 1 if a coordinate is not finite, out[2..4] / out[5..7] = keys of
+// the per-axis minimum / maximum coordinate (out must hold 0, 0, ~0 x 3,
+// 0 x 3). Per-workgroup reductions, then one atomic per value.
+__global__ void __launch_bounds__(256) k_mesh_check(const float* __restrict__ P, int nv, const u32* __restrict__ T,
+                                                     int nt, u32* out) {
+  __shared__ u32 red[8];
+  if (threadIdx.x < 8) red[threadIdx.x] = (threadIdx.x >= 2 && threadIdx.x < 5) ? ~0u : 0u;
+  __syncthreads();
+  u32 bad = 0u, nonfinite = 0u, mn[3] = {~0u, ~0u, ~0u}, mx[3] = {0u, 0u, 0u};
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < 3 * (size_t)nt; i += stride)
+    bad |= T[i] >= (u32)nv ? 1u : 0u;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < (size_t)nv; i += stride)
+    for (int a = 0; a < 3; ++a) {
+      const float v = P[3 * i + a];
+      if (!isfinite(v)) nonfinite = 1u;
+      const u32 k = fkey(v);
+      mn[a] = min(mn[a], k);
+      mx[a] = max(mx[a], k);
+    }
+  if (bad) atomicOr(&red[0], 1u);
+  if (nonfinite) atomicOr(&red[1], 1u);
+  for (int a = 0; a < 3; ++a) {
+    atomicMin(&red[2 + a], mn[a]);
+    atomicMax(&red[5 + a], mx[a]);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (red[0]) atomicOr(&out[0], 1u);
+    if (red[1]) atomicOr(&out[1], 1u);
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&out[2 + a], red[2 + a]);
+      atomicMax(&out[5 + a], red[5 + a]);
+    }
+  }
+}
+
 }  // namespace
+
+hipError_t launch_mesh_check(const float* P, int nv, const u32* T, int nt, u32* out, hipStream_t s) {
+  const size_t work = std::max<size_t>(3 * (size_t)nt, (size_t)nv);
+  const unsigned blocks = (unsigned)std::min<size_t>((work + 255) / 256, 2048);
+  hipLaunchKernelGGL(k_mesh_check, dim3(std::max(blocks, 1u)), dim3(256), 0, s, P, nv, T, nt, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_voxelize(const float* P, int nv, const u32* T, int nt, const float* bmin3, double scale,
                            int R, int32_t* snap, u32* mark, u32* shell, u32* rows, hipStream_t s) {
@@ -191,7 +259,9 @@ hipError_t launch_voxelize(const float* P, int nv, const u32* T, int nt, const f
   GM_CHECK(hipGetLastError());
   if (nt > 0) hipLaunchKernelGGL(k_vox_tris, dim3((nt + 127) / 128), dim3(128), 0, s, snap, T, nt, N, mark, shell);
   GM_CHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_vox_fill, dim3((N * W + 255) / 256), dim3(256), 0, s, mark, shell, rows, N);
+  // N is a multiple of 32, so each of the kFillChunks chunks has N / 16 planes
+  hipLaunchKernelGGL(k_vox_fill, dim3((unsigned)((words / N + 63) / 64)), dim3(64 * kFillChunks), 0, s, mark, shell,
+                     rows, N);
   return hipGetLastError();
 }
 

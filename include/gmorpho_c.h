@@ -145,17 +145,38 @@ int gm_build_pyramid(gm_ctx* ctx, int which);
 int gm_pyramid_size(const gm_ctx* ctx, int* levels, uint64_t* texels);
 int gm_pyramid_get(gm_ctx* ctx, uint64_t* texels);
 
+/* Launch-shape tuning of the streaming dilation and erosion passes. No
+ * counterpart in the reference (its kernels have fixed block shapes).
+ * gm_autotune times gm_close on the context's current grid and radius with
+ * each candidate shape (planes per tile 32 or 16; at most 8, 4 or 2 waves per
+ * workgroup), checks that every shape gives the same closing, and records the
+ * fastest for (base resolution, uniform radius or -1 for per-voxel radii,
+ * se) in a table of the process, which every context's gm_close / gm_dilate
+ * / gm_erode then use; with path it also writes the table (one JSON object a
+ * line). gm_tune_load reads such a file into the table. Batched closings use
+ * the default shape. */
+int gm_autotune(gm_ctx* ctx, int se, int reps, const char* path, int* tile_planes, int* waves);
+int gm_tune_load(const char* path);
+
 /* Per-stage device timings of the last gm_close / gm_extract_mesh, in ms,
  * from HIP events on the context stream (the reference prints GET_TIME
  * brackets, GMorpho.cu:7796-7815). 16 stages: brick summary (with the
  * radius planes on the first closing after a per-voxel field is set),
  * dilate coarse, tight contour, dilate fine, contour K, erode coarse, base
  * contour, erode fine, mc count, mc scan (one small kernel), mc vertices and
- * triangles, mc fixup+normalize; then the streaming launches of the four
+ * triangles, mc finish (late triangles and normalization when a cube lists
+ * more triangles than it counts; the emit normalizes otherwise); then the
+ * streaming launches of the four
  * passes alone (dilate coarse, dilate fine, erode coarse, erode fine). A
  * pass stage holds its tile selection and its streaming launches (one per
  * radius value with per-voxel radii). Timing is off unless enabled. */
 int gm_set_timing(gm_ctx* ctx, int enable);
+/* gm_close and gm_extract_mesh replay their launch sequence as a captured
+ * HIP graph once it was captured for the context's current buffers, sizes,
+ * radius field and grid attributes (on by default; off runs every launch
+ * eagerly; per-stage timing also runs eagerly). No counterpart in the
+ * reference, which launches every kernel from the host. */
+int gm_set_graphs(gm_ctx* ctx, int enable);
 int gm_stage_times(gm_ctx* ctx, float* ms, int n);
 /* Wait for all work enqueued on the context. */
 int gm_sync(gm_ctx* ctx);
@@ -181,6 +202,10 @@ int gm_batch_create(int device, gm_batch** out);
 int gm_batch_destroy(gm_batch* b);
 const char* gm_batch_last_error(const gm_batch* b);
 int gm_batch_close_mesh(gm_batch* b, gm_ctx* const* ctxs, int n, int se);
+/* The batched launches are captured as a HIP graph and replayed when the same
+ * contexts (with unchanged buffers, sizes and radii) are closed again (on by
+ * default). */
+int gm_batch_set_graphs(gm_batch* b, int enable);
 
 /* ---- Job engine. No counterpart in the reference, which runs one closing
  * at a time from its viewer (GMorpho::Update, GMorpho.cu:7775); it is for

@@ -1,0 +1,153 @@
+"""Proxy decimation (libgmorpho_decimate, the reference's Decimator): closed
+forms and invariants. The reference's decimator runs on VCG and a Fortran QP
+solver, neither of which is in its tree, so its exact collapse sequence is
+parity unpinned; these pin what the method guarantees."""
+import numpy as np
+import pytest
+
+import broxy_amd as B
+
+
+def subdivided_cube(n):
+    """Closed unit cube with every face cut into n x n quads (2 triangles each),
+    outward counter-clockwise."""
+    P, T, idx = [], [], {}
+
+    def vid(p):
+        key = tuple(np.round(p, 9))
+        if key not in idx:
+            idx[key] = len(P)
+            P.append(p)
+        return idx[key]
+
+    for axis in range(3):
+        for side in (0.0, 1.0):
+            u, v = (axis + 1) % 3, (axis + 2) % 3
+            for i in range(n):
+                for j in range(n):
+                    q = []
+                    for di, dj in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                        p = np.zeros(3)
+                        p[axis] = side
+                        p[u] = (i + di) / n
+                        p[v] = (j + dj) / n
+                        q.append(vid(p))
+                    a, b, c, d = q
+                    if side == 0.0:
+                        T += [[a, c, b], [a, d, c]]
+                    else:
+                        T += [[a, b, c], [a, c, d]]
+    return np.array(P, np.float32), np.array(T, np.uint32)
+
+
+def closed_manifold(T):
+    e = {}
+    for t in T:
+        for k in range(3):
+            a, b = int(t[k]), int(t[(k + 1) % 3])
+            e[(a, b)] = e.get((a, b), 0) + 1
+    return all(v == 1 for v in e.values()) and all((b, a) in e for (a, b) in e)
+
+
+def inside_count(P, T, pts):
+    """Points strictly inside a closed mesh (ray parity along a tilted axis)."""
+    d = np.array([0.5773, 0.5774, 0.5775])
+    d /= np.linalg.norm(d)
+    A, Bv, C = P[T[:, 0]].astype(np.float64), P[T[:, 1]].astype(np.float64), P[T[:, 2]].astype(np.float64)
+    e1, e2 = Bv - A, C - A
+    h = np.cross(d, e2)
+    det = np.einsum("ij,ij->i", e1, h)
+    out = []
+    for o in pts.astype(np.float64):
+        s = o - A
+        with np.errstate(divide="ignore", invalid="ignore"):
+            u = np.einsum("ij,ij->i", s, h) / det
+            q = np.cross(s, e1)
+            v = (q @ d) / det
+            t = np.einsum("ij,ij->i", e2, q) / det
+        hit = (np.abs(det) > 1e-12) & (u >= 0) & (v >= 0) & (u + v <= 1) & (t > 1e-9)
+        out.append(int(hit.sum()) % 2 == 1)
+    return np.array(out)
+
+
+def test_cube_reduces_to_its_corners():
+    P, T = subdivided_cube(4)
+    assert closed_manifold(T) and len(T) == 192
+    for constrained in (True, False):
+        d = B.Decimator(P, T, 12, 1e-6, use_linear_constraints=constrained, max_edge_length=1e9, hc_iters=0)
+        d.optimize()
+        nf, nv, err = d.state()
+        Po, To = d.mesh()
+        assert (nf, nv) == (12, 8) and closed_manifold(To)
+        corners = {(x, y, z) for x in (0.0, 1.0) for y in (0.0, 1.0) for z in (0.0, 1.0)}
+        assert {tuple(np.round(p, 5)) for p in Po} == corners
+        d.close()
+
+
+def test_bounding_constraints_keep_input_inside():
+    # the unit icosphere's vertices all lie on the sphere; a bounded proxy
+    # keeps every one of them inside or on it, a plain quadric decimation
+    # (placements inside the chords) does not
+    P, T = B.icosphere(3)
+    res = {}
+    for constrained in (True, False):
+        d = B.Decimator(P, T, 120, 1e9, use_linear_constraints=constrained, max_edge_length=1e9, hc_iters=0)
+        d.optimize()
+        nf, nv, err = d.state()
+        Po, To = d.mesh()
+        assert nf <= 122 and closed_manifold(To)
+        assert nv - (nf * 3 // 2) + nf == 2  # Euler characteristic of a sphere
+        # every input vertex (pulled in by 1e-6 so that points on the
+        # proxy's faces count as inside) against the proxy, by ray parity
+        inside = inside_count(Po, To, P * 0.999999)
+        res[constrained] = inside.mean()
+        d.close()
+    assert res[True] == 1.0
+    assert res[False] < 1.0
+
+
+def test_scale_grid_limits_edge_length():
+    # a scale grid of 1 over the whole box with cell 0.05 and alpha 1: every
+    # collapse of an edge longer than 0.05 gets priority 1e11 and never runs,
+    # so the decimation stops well above the target; without the grid it
+    # reaches it
+    P, T = B.icosphere(3)
+    res = np.array([64, 64, 64], np.int32)
+    bmin = np.array([-1.6, -1.6, -1.6], np.float32)
+    grid = np.ones(64 ** 3, np.int8)
+    d = B.Decimator(P, T, 100, 1e9, max_edge_length=1.0, bbox_min=bmin, res=res, cell=0.05, scale_grid=grid,
+                    hc_iters=0)
+    d.optimize()
+    limited = d.state()[0]
+    d.close()
+    assert limited > 1000  # icosphere 3 edges are ~0.1 long: few collapse
+    d = B.Decimator(P, T, 100, 1e9, max_edge_length=1.0, hc_iters=0)
+    d.optimize()
+    assert d.state()[0] <= 102
+    d.close()
+
+
+def test_decimation_deterministic_and_target_error():
+    P, T = B.sphere_soup(4, 3, seed=3)
+    out = []
+    for _ in range(2):
+        d = B.Decimator(P, T, 500, 1e9, hc_iters=3)
+        d.optimize()
+        out.append(d.mesh())
+        d.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    # a tiny target error stops at the first collapse costing more
+    d = B.Decimator(P, T, 10, 1e-12, hc_iters=0)
+    d.optimize()
+    nf, nv, err = d.state()
+    assert nf > 10 and err <= 1e-12
+    d.close()
+
+
+def test_library_exports_declared_symbols():
+    import os
+    import re
+    hdr = open(os.path.join(os.path.dirname(B.HERE), "include", "gmorpho_decimate.h")).read()
+    names = set(re.findall(r"\b(gmd_[a-z0-9_]+)\s*\(", hdr))
+    lib = B.decimate_lib()
+    assert len(names) >= 6 and all(hasattr(lib, n) for n in names)

@@ -152,3 +152,47 @@ def test_batch_mesh_overflow_emitted_again():
         assert result(c) == ref
         c.close()
     bt.close()
+
+
+def test_batch_graph_replays():
+    """The same batch closed four times: the first call captures the batched
+    launches, the others replay them, with eager single-context work and
+    grid uploads between the calls; every replay equals the batch run without
+    graphs. A changed radius (new capture) is checked the same way."""
+    jobs = make_jobs(11, 12)
+    ctxs = [B.Context(0) for _ in jobs]
+    ref_ctxs = [B.Context(0) for _ in jobs]
+    bt, bt_ref = B.Batch(0), B.Batch(0)
+    try:
+        bt_ref.set_graphs(False)
+        for c, c2, j in zip(ctxs, ref_ctxs, jobs):
+            load(c, j)
+            load(c2, j)
+        for rep in range(4):
+            say("batch graph replay %d" % rep)
+            if rep == 2:  # grids change between replays (same buffers): the replay reads them
+                for c, c2, j in zip(ctxs, ref_ctxs, jobs):
+                    I = c.get_cells(B.GRID_INPUT)
+                    I2 = np.roll(I, 1)
+                    c.set_cells(B.GRID_INPUT, I2)
+                    c2.set_cells(B.GRID_INPUT, I2)
+            if rep == 3:
+                for c, c2 in zip(ctxs, ref_ctxs):
+                    c.scale_uniform(3)
+                    c2.scale_uniform(3)
+            bt.close_mesh(ctxs)
+            bt_ref.close_mesh(ref_ctxs)
+            # eager work on a context between the calls
+            ctxs[0].close_grid(B.SE_SPHERE)
+            ctxs[0].extract_mesh()
+            ctxs[0].mesh_counts()
+            ref_ctxs[0].close_grid(B.SE_SPHERE)
+            ref_ctxs[0].extract_mesh()
+            ref_ctxs[0].mesh_counts()
+            for c, c2 in zip(ctxs, ref_ctxs):
+                assert result(c) == result(c2)
+    finally:
+        bt.close()
+        bt_ref.close()
+        for c in ctxs + ref_ctxs:
+            c.close()

@@ -207,6 +207,30 @@ int gm_batch_close_mesh(gm_batch* b, gm_ctx* const* ctxs, int n, int se);
  * default). */
 int gm_batch_set_graphs(gm_batch* b, int enable);
 
+/* ---- Allocation-free closing. No counterpart in the reference; for callers
+ * that cannot allocate device memory: code under HIP graph capture, serving
+ * loops and harnesses that forbid hipMalloc. gm_close_cells_noalloc does what
+ * gm_close does for a uniform ball of radius `radius` (GMorpho.cu:7775-7806;
+ * radius <= 1 copies the input), on grids the caller owns: cells_in and
+ * cells_out are device buffers of base_res^3 bytes in the cell-byte layout
+ * above (16-byte aligned; they may not overlap), data_res as in
+ * gm_grid_init. It enqueues on `stream` (a hipStream_t, NULL for the null
+ * stream) and returns; the result is ready when the stream reaches it. Every
+ * intermediate buffer lives in one static device array of
+ * GM_NOALLOC_SCRATCH_MIB MiB per device, held until the process ends, so
+ * base_res is limited to GM_NOALLOC_MAX_RES (GM_ERR_CAPACITY beyond) and
+ * radius to [0, 31]. Calls on one device share that array: calls that overlap
+ * on one device (on different streams or host threads) race and must be
+ * serialized by the caller. gm_noalloc_init takes the array's device address
+ * (optional: the first call on a device does it); gm_noalloc_last_error
+ * gives the message of the calling thread's last failure. */
+#define GM_NOALLOC_MAX_RES 128
+#define GM_NOALLOC_SCRATCH_MIB 10
+int gm_noalloc_init(int device);
+int gm_close_cells_noalloc(int device, const uint8_t* cells_in, uint8_t* cells_out, int base_res,
+                           const int data_res[3], int radius, void* stream);
+const char* gm_noalloc_last_error(void);
+
 /* ---- Job engine. No counterpart in the reference, which runs one closing
  * at a time from its viewer (GMorpho::Update, GMorpho.cu:7775); it is for
  * callers with many jobs: batches of uneven grids and radii, and a serving

@@ -359,12 +359,27 @@ __device__ __forceinline__ void ct_scatter_all(const u32* disk, int zi, const Ti
 
 // Words wx-1, wx, wx+1 of row (y, z) of a bit grid, zero outside the grid.
 // Branch-free (clamped address, masked value) so that the loads of a plane
-// issue back to back.
+// issue back to back. Lanes own rows one row (W words) apart, so each lane
+// touches its own cache line: the three words come in one 12-byte load
+// (one address and one tag lookup per lane instead of three), from the
+// wave-uniform word s = clamp(wx - 1, 0, W - 3) of the row.
+typedef u32 u32x3a4 __attribute__((ext_vector_type(3), aligned(4)));
 __device__ __forceinline__ void load3(const u32* __restrict__ G, int N, int W, int y, int z, int wx, u32& l, u32& c,
                                       u32& r) {
   const bool in = (unsigned)y < (unsigned)N && (unsigned)z < (unsigned)N;
   const int yc = min(max(y, 0), N - 1), zc = min(max(z, 0), N - 1);
   const u32* row = G + ((size_t)zc * N + yc) * W;
+  if (W >= 3) {
+    const int s = min(max(wx - 1, 0), W - 3);
+    const u32x3a4 v = *reinterpret_cast<const u32x3a4*>(row + s);
+    const u32 lv = wx == 0 ? 0u : (wx == W - 1 ? v.y : v.x);
+    const u32 cv = wx == 0 ? v.x : (wx == W - 1 ? v.z : v.y);
+    const u32 rv = wx == 0 ? v.y : (wx == W - 1 ? 0u : v.z);
+    l = in ? lv : 0u;
+    c = in ? cv : 0u;
+    r = in ? rv : 0u;
+    return;
+  }
   const u32 lv = row[max(wx - 1, 0)], cv = row[wx], rv = row[min(wx + 1, W - 1)];
   l = (in && wx > 0) ? lv : 0u;
   c = in ? cv : 0u;

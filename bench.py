@@ -75,6 +75,16 @@ def rho_of_stage(stage, r):
     return r + 1 if stage.startswith("dilate") else r - 1
 
 
+def dominant_kernel(stage, r, se):
+    """rocprof name of the kernel a stage times (cube dilations run the
+    runtime-table stream kernels)."""
+    if se == "cube" and stage in ("dilate_coarse_stream", "dilate_fine_stream"):
+        return "k_dil_coarse" if "coarse" in stage else "k_dil_fine"
+    if r:
+        return KERNEL_OF_STAGE[stage].replace("rho", str(rho_of_stage(stage, r)))
+    return VAR_KERNEL.get(stage, KERNEL_OF_STAGE[stage])
+
+
 def build_workload(args):
     """(P, T, base_res, uniform radius or None, radius lattice or None, description)."""
     if args.config == "large":
@@ -91,7 +101,7 @@ def build_workload(args):
     raise SystemExit("unknown config")
 
 
-def cpu_baseline(I, R, r, dr, bmin, cell, N, lat=None):
+def cpu_baseline(I, R, r, dr, bmin, cell, N, lat=None, se=0):
     """The oracle (CPU restatement, OpenMP) on the same closing + mesh. Its
     grids and mesh are kept: every bench line compares the GPU's with them."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -101,7 +111,7 @@ def cpu_baseline(I, R, r, dr, bmin, cell, N, lat=None):
         L = int(round(lat.size ** (1 / 3.0)))
         sc = O.scale_from_lattice(lat.reshape(L, L, L), N)
     t0 = time.perf_counter()
-    D = O.dilate(I, R, r=r or 0, data_res=dr, scale=sc)
+    D = O.dilate(I, R, r=r or 0, data_res=dr, scale=sc, se=se)
     C = O.erode(I, D, R, r=r or 0, data_res=dr, scale=sc)
     V, Nn, T = O.mc(C, R, bmin, cell)
     dt = time.perf_counter() - t0
@@ -311,6 +321,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="large")
+    ap.add_argument("--se", choices=["sphere", "cube"], default="sphere",
+                    help="dilation structuring element (cube: the asymmetric closing, cube dilation + ball erosion)")
     ap.add_argument("--seed", type=int, default=2017)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle parity check")
     ap.add_argument("--no-tune", action="store_true", help="default launch shape (no autotuning before timing)")
@@ -340,6 +352,9 @@ def main():
         sys.exit(2)
 
     P, T, R, r, lat, desc = build_workload(args)
+    se = B.SE_CUBE if args.se == "cube" else B.SE_SPHERE
+    if args.se == "cube":
+        desc = desc.replace("ball r=", "cube dilation + ball erosion r=")
     N = 2 * R
     ctx = B.Context(0)
     t0 = time.perf_counter()
@@ -363,16 +378,16 @@ def main():
     if not args.no_tune:
         # launch shape of the streaming passes for this grid and radius,
         # chosen by timing every candidate (outside the timed region)
-        shape = ctx.autotune(B.SE_SPHERE, 5)
+        shape = ctx.autotune(se, 5)
     for _ in range(args.warmup):
-        ctx.close_grid(B.SE_SPHERE)
+        ctx.close_grid(se)
         ctx.extract_mesh()
         nv, nt = ctx.mesh_counts()
 
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        ctx.close_grid(B.SE_SPHERE)
+        ctx.close_grid(se)
         ctx.extract_mesh()
     ctx.sync()
     t1 = time.perf_counter()
@@ -384,7 +399,7 @@ def main():
     acc = np.zeros(len(STAGES))
     reps = max(3, min(args.steps, 10))
     for _ in range(reps):
-        ctx.close_grid(B.SE_SPHERE)
+        ctx.close_grid(se)
         ctx.extract_mesh()
         acc += ctx.stage_times()
     ctx.set_timing(False)
@@ -395,7 +410,7 @@ def main():
         # planes (inside the brick_summary stage); later closings reuse them
         ctx.scale_lattice(lat, int(round(lat.size ** (1 / 3.0))))
         ctx.set_timing(True)
-        ctx.close_grid(B.SE_SPHERE)
+        ctx.close_grid(se)
         ctx.extract_mesh()
         planes_ms = float(ctx.stage_times()[0] - stage_ms[0])
         ctx.set_timing(False)
@@ -431,13 +446,12 @@ def main():
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic (seeded mesh generator)",
-        "config": {"workload": desc, "N": N, "radius": r if r is not None else "lattice [2,24]", "se": "sphere",
+        "config": {"workload": desc, "N": N, "radius": r if r is not None else "lattice [2,24]", "se": args.se,
                    "mesh_vertices": nv,
                    "mesh_triangles": nt, "parallelism": "single GPU",
                    "launch_shape": ({"tile_planes": shape[0], "waves": shape[1]} if shape else "default")},
         "roofline": {"bound": "hbm",
-                     "kernel": (KERNEL_OF_STAGE[dname].replace("rho", str(rho_of_stage(dname, r))) if r
-                                else VAR_KERNEL.get(dname, KERNEL_OF_STAGE[dname])),
+                     "kernel": dominant_kernel(dname, r, args.se),
                      "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes": int(dbytes),
@@ -452,7 +466,7 @@ def main():
         out["radius_planes_ms_once_per_field"] = round(planes_ms, 4)
     if not args.no_cpu:
         I = ctx.get_cells(B.GRID_INPUT)
-        cb, Dref, Cref, mesh = cpu_baseline(I, R, r, dr, bmin, cell, N, lat)
+        cb, Dref, Cref, mesh = cpu_baseline(I, R, r, dr, bmin, cell, N, lat, se)
         out["cpu_baseline"] = cb
         out["parity"] = parity(ctx, P, T, R, bmin, bmax, Dref, Cref, mesh)
     ctx.close()

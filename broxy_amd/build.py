@@ -93,6 +93,18 @@ def build_driver():
     return out
 
 
+def build_decimate_driver():
+    """The C++ Decimator facade driver (tests/cpp/decimate_facade.cpp), linked
+    against the host decimation library only; run by the CPU tests."""
+    bindir = os.path.join(LIBDIR, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    out = os.path.join(bindir, "decimate_facade")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+          os.path.join(ROOT, "tests", "cpp", "decimate_facade.cpp"), "-o", out,
+          "-L", LIBDIR, "-lgmorpho_decimate", "-Wl,-rpath,$ORIGIN/.."])
+    return out
+
+
 def build_oracle():
     _run(["make", "-C", os.path.join(ROOT, "oracle")])
 
@@ -100,6 +112,7 @@ def build_oracle():
 def build_all():
     build_meshgen()
     build_decimator()
+    build_decimate_driver()
     build_hip()
     build_driver()
     build_oracle()
@@ -109,5 +122,5 @@ if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
         {"all": build_all, "hip": build_hip, "meshgen": build_meshgen, "decimator": build_decimator,
-         "driver": build_driver,
+         "driver": build_driver, "decimate_driver": build_decimate_driver,
          "oracle": build_oracle}[w]()

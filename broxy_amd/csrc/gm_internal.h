@@ -86,7 +86,10 @@ __device__ __forceinline__ int batch_job(const u32* __restrict__ pre, int n, u32
 //          class b stands for {b, -b} (just {0} for b = 0); window [-m, m].
 // Layout (u32): [0] kind, [1] nclass, [2] max m, [3] max |offset| in y/z,
 // then per class: header (b | npairs<<16), npairs words (a | m<<16).
-enum { SE_CORNER = 0, SE_CENTRE = 1 };
+// SE_BOX: the cube's chords (every chord of half-width rho in a
+// (2 rho + 2)^3 corner-geometry window), for the compiled-table kernels only;
+// the runtime cube tables are SE_CORNER tables built with cube = true.
+enum { SE_CORNER = 0, SE_CENTRE = 1, SE_BOX = 2 };
 constexpr int kMaxRhoFast = 32;  // streaming kernel supports m, |offset| <= 32
 
 // Launch configuration of the streaming morphology kernels.
@@ -261,6 +264,7 @@ constexpr int kCtParts = 8;
 struct CtKernels {
   const void* fn[4][kCtMaxRho + 1];
   const void* fnB[4][kCtMaxRho + 1];  // batched forms (argument table + job/tile queue)
+  const void* fnBox[2][kCtMaxRho + 1];  // cube dilation, coarse and fine (uniform radius rho)
 };
 void ct_register_part0(CtKernels&);
 void ct_register_part1(CtKernels&);

@@ -277,9 +277,11 @@ struct BlockLds {
   unsigned long long nbAbove[kNbCells];
   unsigned long long nbBelow[kNbCells];
   u32 nbBase[kNbCells];
-  u32 vOff[512];
-  u32 tOff[512];
+  u32 vOff[512];   // block-local vertex base of each cell with a mixed cube
+  u32 tOff[512];   // block-local triangle base of each cell with a mixed cube
   u32 red[16];
+  uint16_t cl[512];  // the cells with a mixed cube (local Morton index), S.ncl of them
+  u32 ncl;
   uint16_t list[kListCap];  // cube index << 3 | item, by local output index
   float fn[kFaceCap][3];     // face normal of each local triangle (when it fits)
 };
@@ -296,9 +298,12 @@ __device__ __forceinline__ unsigned long long listed_cell_cases(const unsigned l
 }
 
 // Stages listed block `it` (origin cell c0): the cube cases of its 512 cells
-// (written by the count kernel), per-cell vertex / triangle counts into
-// vOff/tOff with the in-cell triangle prefix, and the cells just above and
-// below it.
+// (written by the count kernel), the block-local vertex and triangle bases
+// of the cells with a mixed cube (also from the count kernel) and the list
+// of those cells (S.cl, S.ncl; the later phases walk it instead of all 512
+// cells), and the cells just above and below the block. Two cells per
+// thread (kMcThreads = 256).
+static_assert(kMcThreads == 256, "stage_block takes cells t and t + 256");
 __device__ void stage_block(BlockLds& S, const unsigned long long* __restrict__ cellCase,
                             const u32* __restrict__ posOf, const u32* __restrict__ blockV,
                             const u32* __restrict__ cellL, int R, u32 it, int c0x, int c0y, int c0z) {
@@ -310,23 +315,30 @@ __device__ void stage_block(BlockLds& S, const unsigned long long* __restrict__ 
     nb_cell(t, lx, ly, lz);
     u32 code = 0u, pos = 0u;
     above = listed_cell_cases(cellCase, posOf, R, c0x + lx, c0y + ly, c0z + lz, &code, &pos);
-    if (above) nbB = blockV[code >> 9] + cellL[(size_t)pos * 512 + (code & 511u)];
+    if (above) nbB = blockV[code >> 9] + (cellL[(size_t)pos * 512 + (code & 511u)] & 0xffffu);
     below = listed_cell_cases(cellCase, posOf, R, c0x + 7 - lx, c0y + 7 - ly, c0z + 7 - lz, &code, &pos);
   }
   const unsigned long long* own = cellCase + (size_t)it * 512;
-  for (int l = t; l < 512; l += blockDim.x) {
-    const unsigned long long w = own[l];
-    u32 nt = 0u;
-    unsigned long long tp = 0ull;
+  const u32* ownL = cellL + (size_t)it * 512;
+  unsigned long long w[2];
+  u32 base[2];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      tp |= (unsigned long long)nt << (8 * j);
-      nt += S.tab.ntri[(w >> (8 * j)) & 0xffu];
+  for (int h = 0; h < 2; ++h) {
+    w[h] = own[t + 256 * h];
+    base[h] = ownL[t + 256 * h];  // written only for cells with a mixed cube
+  }
+  const int lane = t & 63, wv = t >> 6;
+  unsigned long long m[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int l = t + 256 * h;
+    *reinterpret_cast<unsigned long long*>(&S.cases[8 * l]) = w[h];
+    if (w[h]) {
+      S.vOff[l] = base[h] & 0xffffu;
+      S.tOff[l] = base[h] >> 16;
     }
-    *reinterpret_cast<unsigned long long*>(&S.cases[8 * l]) = w;
-    *reinterpret_cast<unsigned long long*>(&S.tpre[8 * l]) = tp;
-    S.vOff[l] = own_prefix8(w, 8);
-    S.tOff[l] = nt;
+    m[h] = __ballot(w[h] != 0ull);
+    if (lane == 0) S.red[4 * h + wv] = (u32)__popcll(m[h]);
   }
   if (t < kNbCells) {
     S.nbAbove[t] = above;
@@ -334,23 +346,21 @@ __device__ void stage_block(BlockLds& S, const unsigned long long* __restrict__ 
     S.nbBase[t] = nbB;
   }
   __syncthreads();
-}
-
-// Exclusive scans of vOff/tOff over the 512 cells in Morton order.
-__device__ void scan_cells(BlockLds& S) {
-  // two cells per thread (workgroup of 256)
-  int l0 = 2 * threadIdx.x;
-  u32 v0 = S.vOff[l0], v1 = S.vOff[l0 + 1], t0 = S.tOff[l0], t1 = S.tOff[l0 + 1];
-  // one scan of (vertices | triangles << 16): a block's 4096 cubes own at
-  // most 3 vertices and count at most 5 triangles each (12288, 20480), so
-  // neither field carries
-  u32 tot;
-  const u32 o = block_exscan((v0 + v1) | ((t0 + t1) << 16), S.red, &tot, false);  // barrier below
-  const u32 ov = o & 0xffffu, ot = o >> 16;
-  S.vOff[l0] = ov;
-  S.vOff[l0 + 1] = ov + v0;
-  S.tOff[l0] = ot;
-  S.tOff[l0 + 1] = ot + t0;
+  // list position: cells with a mixed cube of the earlier waves, then of
+  // this wave's earlier lanes
+  u32 pre[2] = {0u, 0u}, n = 0u;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const u32 c = S.red[k];
+    pre[0] += k < wv ? c : 0u;
+    pre[1] += k < 4 + wv ? c : 0u;
+    n += c;
+  }
+  const unsigned long long below_lane = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+    if (w[h]) S.cl[pre[h] + (u32)__popcll(m[h] & below_lane)] = (uint16_t)(t + 256 * h);
+  if (t == 0) S.ncl = n;
   __syncthreads();
 }
 
@@ -470,7 +480,7 @@ __device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R,
   u32 u = 0u;
   for (int w = 0; w < (int)(blockDim.x >> 6); ++w) u |= bx.uni[w];
   const bool nonuni = u == 3u;
-  u32 nvh[kCountCells] = {}, nv = 0, nt = 0, nx = 0;
+  u32 nvh[kCountCells] = {}, nth[kCountCells] = {}, nv = 0, nt = 0, nx = 0;
   unsigned long long w[kCountCells] = {};
   if (nonuni) {
 #pragma unroll
@@ -483,9 +493,10 @@ __device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R,
       nv += nvh[h];
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        nt += ntri[(w[h] >> (8 * j)) & 0xffu];
+        nth[h] += ntri[(w[h] >> (8 * j)) & 0xffu];
         nx += xtra[(w[h] >> (8 * j)) & 0xffu];
       }
+      nt += nth[h];
     }
   }
   // listed-but-uncounted triangles anywhere: the emit then leaves the normal
@@ -501,15 +512,17 @@ __device__ __forceinline__ void mc_count_block(const u32* __restrict__ C, int R,
   // neither field carries
   u32 tot;
   // (no closing barrier: the caller's loop ends with one)
-  u32 off = block_exscan(nv | (nt << 16), red, &tot, false) & 0xffffu;
+  u32 off = block_exscan(nv | (nt << 16), red, &tot, false);
   const u32 tv = tot & 0xffffu, tt = tot >> 16;
-  // block-local vertex base of the cells with vertices (cells are in Morton
-  // order, consecutive per thread; stored by list position and local Morton
-  // index), read for the edges owned by another block's cube
+  // block-local vertex and triangle bases (vertex | triangle << 16) of the
+  // cells with a mixed cube (cells are in Morton order, consecutive per
+  // thread; stored by list position and local Morton index): the emit of
+  // this block stages its cells from them, and the emit of other blocks reads
+  // the vertex base for the edges owned by this block's cubes
 #pragma unroll
   for (int h = 0; h < kCountCells; ++h) {
-    if (nvh[h]) cellL[(size_t)it * 512 + kCountCells * threadIdx.x + h] = off;
-    off += nvh[h];
+    if (w[h]) cellL[(size_t)it * 512 + kCountCells * threadIdx.x + h] = off;
+    off += nvh[h] | (nth[h] << 16);
   }
   if (threadIdx.x == 0) {
     blockV[blk] = tv;
@@ -832,9 +845,9 @@ __device__ __forceinline__ void unit3(float* n) {
 __device__ void mc_verts_block(BlockLds& S, int R, Geo geo, int blk, int c0x, int c0y, int c0z, bool useFn, u32 base,
                                u32 nverts, float* V, float* Nrm, u32 capV, bool unit) {
   for (u32 w0 = 0; w0 < nverts; w0 += kListCap) {
-    for (int l = threadIdx.x; l < 512; l += blockDim.x) {
+    for (u32 i = threadIdx.x; i < S.ncl; i += blockDim.x) {
+      const int l = S.cl[i];
       const unsigned long long w = cell_cases(S, l);
-      if (!w) continue;
       // the cell's vertices, in cube and edge order, from its local base
       u32 lv = S.vOff[l] - w0;
 #pragma unroll
@@ -913,16 +926,20 @@ __device__ void mc_tris_block(BlockLds& S, int R, Geo geo, int blk, int c0x, int
                               u32* T, u32 capT, u32* late, u32 capLate, u32* totals) {
   u32 border = 0u;
   for (u32 w0 = 0; w0 < ntris; w0 += kListCap) {
-    for (int l = threadIdx.x; l < 512; l += blockDim.x) {
+    for (u32 i = threadIdx.x; i < S.ncl; i += blockDim.x) {
+      const int l = S.cl[i];
       const unsigned long long w = cell_cases(S, l);
-      if (!w) continue;
-      // the cell's counted triangles, in cube order, from its local base
-      u32 lt = S.tOff[l];
+      // the cell's counted triangles, in cube order, from its local base;
+      // the first window also stores the in-cell prefix per cube
+      const u32 lt0 = S.tOff[l];
+      u32 lt = lt0;
+      unsigned long long tp = 0ull;
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int v = (int)((w >> (8 * j)) & 0xffu);
         const int s = 8 * l + j;
         const u32 cnt = S.tab.ntri[v], len = S.tab.len[v];
+        tp |= (unsigned long long)(lt - lt0) << (8 * j);
         for (u32 q = 0; q < cnt; ++q)
           if (lt + q - w0 < (u32)kListCap) S.list[lt + q - w0] = (uint16_t)((s << 3) | q);
         if (w0 == 0 && len > cnt) {
@@ -951,6 +968,7 @@ __device__ void mc_tris_block(BlockLds& S, int R, Geo geo, int blk, int c0x, int
         }
         lt += cnt;
       }
+      if (w0 == 0) *reinterpret_cast<unsigned long long*>(&S.tpre[8 * l]) = tp;
     }
     __syncthreads();
     const u32 n = min((u32)kListCap, ntris - w0);
@@ -998,7 +1016,6 @@ __device__ __forceinline__ void mc_emit_block(BlockLds& S, int R, Geo geo, u32 i
   const u32 tnext = (blk + 1 < nblocks) ? blockT[blk + 1] : totals[1];
   if (next == base && tnext == tbase) return;  // uniform across the workgroup
   stage_block(S, cellCase, posOf, blockV, cellL, R, it, c0x, c0y, c0z);
-  scan_cells(S);
   // triangles first: their face normals (when they fit in LDS) are then
   // gathered by the vertices
   const bool useFn = tnext - tbase <= (u32)kFaceCap && tnext <= capT;

@@ -616,12 +616,15 @@ int device_cus() {
 // evStream (optional) is recorded between the tile selection and the stream
 // kernel, so that the stream kernel can be timed alone.
 hipError_t launch_pass(int pass, const void* generic, MorphArgs a, int* qsel, bool var, bool fixedOk, int rho,
-                       hipStream_t s, hipEvent_t evStream) {
+                       hipStream_t s, hipEvent_t evStream, bool box = false) {
   a.qsel = *qsel;
   *qsel = 1 - *qsel;
   const void* fn = generic;
-  if (fixedOk && rho >= 1 && rho <= kCtMaxRho && ct_kernels().fn[pass][rho]) {
-    fn = ct_kernels().fn[pass][rho];
+  const void* compiled = nullptr;
+  if (fixedOk && rho >= 1 && rho <= kCtMaxRho)
+    compiled = box ? (pass < 2 ? ct_kernels().fnBox[pass][rho] : nullptr) : ct_kernels().fn[pass][rho];
+  if (compiled) {
+    fn = compiled;
     a.tabWords = 0;  // the chords are compiled in: no table in LDS
   }
   const void* sel[4] = {reinterpret_cast<const void*>(&k_select<0>), reinterpret_cast<const void*>(&k_select<1>),
@@ -746,8 +749,10 @@ hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const Radius
                      nullptr, b.IFL);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
-  const bool fixedOk = !var && se == 0;
-  const int rho = (rf.r & 0x7f) + 1;
+  // uniform radius: the compiled chord tables (ball rho = r + 1, cube rho = r)
+  const bool fixedOk = !var;
+  const int rho = (rf.r & 0x7f) + (se == 0 ? 1 : 0);
+  const bool box = se != 0;
   const int vlo = std::max(rho_lo, 2), vhi = rho_hi;
   int vprev = -1;
   a.ystep = 2;
@@ -760,7 +765,7 @@ hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const Radius
   if (perValue) GM_CHECK(launch_pass_var(0, a, b, vlo, vhi, &vprev, s, evs ? evs[0] : nullptr));
   else
     GM_CHECK(launch_pass(0, reinterpret_cast<const void*>(&k_dil_coarse), a, b.qsel, var, fixedOk, rho, s,
-                         evs ? evs[0] : nullptr));
+                         evs ? evs[0] : nullptr, box));
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   a.outFlags = b.TG;
   a.outRows = b.D;
@@ -772,7 +777,7 @@ hipError_t launch_dilate(const MorphDims& d, const MorphBuffers& b, const Radius
   if (perValue) GM_CHECK(launch_pass_var(1, a, b, vlo, vhi, &vprev, s, evs ? evs[1] : nullptr));
   else
     GM_CHECK(launch_pass(1, reinterpret_cast<const void*>(&k_dil_fine), a, b.qsel, var, fixedOk, rho, s,
-                         evs ? evs[1] : nullptr));
+                         evs ? evs[1] : nullptr, box));
   if (ev) GM_CHECK(hipEventRecord(ev[4], s));
   return hipSuccess;
 }
@@ -1016,7 +1021,8 @@ std::vector<u32> build_se_table(int kind, int rho, bool cube) {
 template <int KIND, int RHO>
 static bool ct_matches() {
   constexpr SETabCT<KIND, RHO> T{};
-  const std::vector<u32> h = build_se_table(KIND, RHO, false);
+  // the box is the runtime cube table (corner geometry, cube = true)
+  const std::vector<u32> h = build_se_table(KIND == SE_BOX ? SE_CORNER : KIND, RHO, KIND == SE_BOX);
   if ((int)h[2] != T.maxm || (int)h[3] != T.hy) return false;
   size_t p = 5;
   int seen = 0;
@@ -1035,7 +1041,8 @@ static bool ct_matches() {
 }
 template <int... Rs>
 static bool ct_matches_all(std::integer_sequence<int, Rs...>) {
-  return ((ct_matches<SE_CORNER, Rs + 1>() && ct_matches<SE_CENTRE, Rs + 1>()) && ...);
+  return ((ct_matches<SE_CORNER, Rs + 1>() && ct_matches<SE_CENTRE, Rs + 1>() && ct_matches<SE_BOX, Rs + 1>()) &&
+          ...);
 }
 bool ct_tables_ok() { return ct_matches_all(std::make_integer_sequence<int, kCtMaxRho>{}); }
 

@@ -27,7 +27,8 @@ __global__ void __launch_bounds__(64 * kMaxWaves) k_pass_ct_b(const MorphArgs* _
 }
 
 // Dilation uses rho = r + 1 <= 32 (corner tables); erosion rho = r - 1 <= 30
-// (corner tables for the coarse pass, centre tables for the fine pass).
+// (corner tables for the coarse pass, centre tables for the fine pass); the
+// cube dilation rho = r <= 32 (box tables, passes 4 and 5).
 template <int RHO>
 void register_rho(CtKernels& k) {
   if constexpr (RHO % kCtParts == GM_CT_PART) {
@@ -35,6 +36,8 @@ void register_rho(CtKernels& k) {
     k.fn[1][RHO] = reinterpret_cast<const void*>(&k_pass_ct<1, RHO>);
     k.fnB[0][RHO] = reinterpret_cast<const void*>(&k_pass_ct_b<0, RHO>);
     k.fnB[1][RHO] = reinterpret_cast<const void*>(&k_pass_ct_b<1, RHO>);
+    k.fnBox[0][RHO] = reinterpret_cast<const void*>(&k_pass_ct<4, RHO>);
+    k.fnBox[1][RHO] = reinterpret_cast<const void*>(&k_pass_ct<5, RHO>);
     if constexpr (RHO <= kCtMaxRho - 2) {
       k.fn[2][RHO] = reinterpret_cast<const void*>(&k_pass_ct<2, RHO>);
       k.fn[3][RHO] = reinterpret_cast<const void*>(&k_pass_ct<3, RHO>);

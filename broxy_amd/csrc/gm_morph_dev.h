@@ -87,6 +87,39 @@ __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
 __device__ __forceinline__ int num_waves() { return (int)(blockDim.x >> 6); }
 
+// Buffer resource of a bit grid (N x N rows of W words): loads through it
+// return 0 at byte offsets at or past the grid's end, so an offset of
+// kOutOfGrid reads a zero word without a branch. Grids are at most 2048^3
+// bits (1 GiB), below kOutOfGrid.
+constexpr int kOutOfGrid = 0x7fff0000;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t grid_rsrc(const u32* G, int N, int W) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<u32*>(G), (short)0, N * N * W * 4, 0x00020000);
+}
+// Words wx-1, wx, wx+1 of row (y, z) of a bit grid, zero outside the grid,
+// in two steps: row_window issues one 12-byte load of the words from wx - 1
+// (from wx when wx = 0, a wave-uniform choice, so that the window never
+// starts before the grid), and pick3 selects the three words where they are
+// used. Words past the grid's end read as 0 (the buffer range check is per
+// dword) and are masked anyway. Lanes own rows one row (W words) apart, so
+// each lane touches its own cache line: one address and one tag lookup per
+// lane instead of three. The load is a buffer-load intrinsic with an
+// out-of-grid offset for rows outside the grid, and nothing selects on its
+// result before the use: with a plain load the compiler turns
+// "in ? load : 0" into a branch around the load and waits for it at once,
+// which serializes a plane's loads and defeats the prefetch of the planes
+// ahead.
+typedef u32 u32x3 __attribute__((ext_vector_type(3)));
+__device__ __forceinline__ void row_window(__amdgpu_buffer_rsrc_t G, int N, int W, int y, int z, int wx, u32x3& v) {
+  const bool in = (unsigned)y < (unsigned)N && (unsigned)z < (unsigned)N;
+  const int st = (z * N + y) * W + (wx > 0 ? wx - 1 : 0);
+  v = __builtin_amdgcn_raw_buffer_load_b96(G, in ? 4 * st : kOutOfGrid, 0, 0);
+}
+__device__ __forceinline__ void pick3(const u32x3& v, int wx, int W, u32& l, u32& c, u32& r) {
+  l = wx > 0 ? v.x : 0u;
+  c = wx > 0 ? v.y : v.x;
+  r = wx + 1 < W ? (wx > 0 ? v.z : v.y) : 0u;
+}
+
 // One streaming pass of the table tl (in LDS) over the source src, ORed into
 // acc[plane index][lane] (plane index = (zo - zfirst) / zstep). The waves of
 // the workgroup take the input planes round robin; each builds the G table of
@@ -103,17 +136,15 @@ __device__ void stream(const u32* tl, const Src& src, u32* gtAll, int gtStride, 
   const int zbeg = g.zfirst + dzl, zend = g.zlast + dzh;
   const int zshift = g.zstep == 2 ? 1 : 0;
   const int base = g.ystep * lane + HY;
-  u32 pre[4][3];
+  // row windows of the wave's next plane (rows past the window and planes
+  // past zend read as rows outside the grid: zero, and no branch)
+  typename Src::Win pre[4];
+  const __amdgpu_buffer_rsrc_t rs = grid_rsrc(src.g, src.N, src.W);
   int zi = zbeg + wv;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    int gr = lane + kTileRows * k;
-    pre[k][0] = pre[k][1] = pre[k][2] = 0u;
-    if (gr < GR && zi <= zend) {
-      pre[k][0] = src(ybase + gr, zi, g.wx - 1);
-      pre[k][1] = src(ybase + gr, zi, g.wx);
-      pre[k][2] = src(ybase + gr, zi, g.wx + 1);
-    }
+    const int gr = lane + kTileRows * k;
+    src.window(rs, (gr < GR && zi <= zend) ? ybase + gr : -1, zi, g.wx, pre[k]);
   }
   for (; zi <= zend; zi += nw) {
     // stage 1: x-window dilations of this plane's rows into the wave's slice
@@ -121,7 +152,8 @@ __device__ void stream(const u32* tl, const Src& src, u32* gtAll, int gtStride, 
     for (int k = 0; k < 4; ++k) {
       int gr = lane + kTileRows * k;
       if (gr < GR) {
-        const u32 Lw = pre[k][0], Cw = pre[k][1], Rw = pre[k][2];
+        u32 Lw, Cw, Rw;
+        src.pick(pre[k], g.wx, Lw, Cw, Rw);
         u32 gg;
         if (kind == SE_CORNER) {
           gg = Cw | shl_k(Lw, Cw, 1);
@@ -142,16 +174,10 @@ __device__ void stream(const u32* tl, const Src& src, u32* gtAll, int gtStride, 
     }
     // prefetch this wave's next plane while this one is reduced
     const int zn = zi + nw;
-    if (zn <= zend) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        int gr = lane + kTileRows * k;
-        if (gr < GR) {
-          pre[k][0] = src(ybase + gr, zn, g.wx - 1);
-          pre[k][1] = src(ybase + gr, zn, g.wx);
-          pre[k][2] = src(ybase + gr, zn, g.wx + 1);
-        }
-      }
+    for (int k = 0; k < 4; ++k) {
+      const int gr = lane + kTileRows * k;
+      src.window(rs, (gr < GR && zn <= zend) ? ybase + gr : -1, zn, g.wx, pre[k]);
     }
     // the slice is written and read by this wave only: in-order LDS suffices
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -201,10 +227,16 @@ __device__ void stream(const u32* tl, const Src& src, u32* gtAll, int gtStride, 
   }
 }
 
+// Sources of a stream: window() starts the loads of words wx-1..wx+1 of row
+// (y, z) (zero outside the grid), pick() gives them once they are used.
 struct GridSrc {
   const u32* g;
   int N, W;
-  __device__ __forceinline__ u32 operator()(int y, int z, int w) const { return row_word(g, N, W, y, z, w); }
+  typedef u32x3 Win;
+  __device__ __forceinline__ void window(__amdgpu_buffer_rsrc_t rs, int y, int z, int wx, Win& v) const {
+    row_window(rs, N, W, y, z, wx, v);
+  }
+  __device__ __forceinline__ void pick(const Win& v, int wx, u32& l, u32& c, u32& r) const { pick3(v, wx, W, l, c, r); }
 };
 
 // ---------------------------------------------------------------------------
@@ -229,7 +261,15 @@ struct SETabCT {
       np[b] = 0;
       for (int a = 0; a < NMAX; ++a) m[b][a] = -1;
     }
-    if (KIND == SE_CORNER) {
+    if (KIND == SE_BOX) {
+      for (int b = 0; b <= RHO; ++b) {
+        for (int a = 0; a <= RHO; ++a) m[b][a] = RHO;
+        np[b] = RHO + 1;
+      }
+      nb = RHO + 1;
+      maxm = RHO;
+      hy = RHO + 1;
+    } else if (KIND == SE_CORNER) {
       const long long r2 = (long long)RHO * RHO;
       for (int b = 0; b <= RHO - 1; ++b) {
         for (int a = 0; (long long)a * a + (long long)b * b < r2; ++a) {
@@ -298,7 +338,7 @@ __device__ __forceinline__ u32 ct_pair(const u32* gl) {
   } else if constexpr (KIND == SE_CENTRE && A == 0) {
     return gl[C::off(m, 0)];
   } else {
-    constexpr int d2 = KIND == SE_CORNER ? -A - 1 : -A;
+    constexpr int d2 = KIND != SE_CENTRE ? -A - 1 : -A;
     return gl[C::off(m, A)] | gl[C::off(m, d2)];
   }
 }
@@ -343,7 +383,7 @@ __device__ __forceinline__ void ct_scatter(u32 disk, int zi, const TileGeo& g, u
     int zo = zi - B;
     if (zo >= g.zfirst && zo <= g.zlast && ((zo - g.zfirst) & (g.zstep - 1)) == 0)
       atomicOr(&acc[((zo - g.zfirst) >> zshift) * kTileRows + lane], disk);
-    constexpr int dz1 = KIND == SE_CORNER ? -B - 1 : -B;
+    constexpr int dz1 = KIND != SE_CENTRE ? -B - 1 : -B;
     if constexpr (dz1 != B) {
       zo = zi - dz1;
       if (zo >= g.zfirst && zo <= g.zlast && ((zo - g.zfirst) & (g.zstep - 1)) == 0)
@@ -357,34 +397,6 @@ __device__ __forceinline__ void ct_scatter_all(const u32* disk, int zi, const Ti
   (ct_scatter<C, KIND, Bs>(disk[Bs], zi, g, acc, lane, zshift), ...);
 }
 
-// Words wx-1, wx, wx+1 of row (y, z) of a bit grid, zero outside the grid.
-// Branch-free (clamped address, masked value) so that the loads of a plane
-// issue back to back. Lanes own rows one row (W words) apart, so each lane
-// touches its own cache line: the three words come in one 12-byte load
-// (one address and one tag lookup per lane instead of three), from the
-// wave-uniform word s = clamp(wx - 1, 0, W - 3) of the row.
-typedef u32 u32x3a4 __attribute__((ext_vector_type(3), aligned(4)));
-__device__ __forceinline__ void load3(const u32* __restrict__ G, int N, int W, int y, int z, int wx, u32& l, u32& c,
-                                      u32& r) {
-  const bool in = (unsigned)y < (unsigned)N && (unsigned)z < (unsigned)N;
-  const int yc = min(max(y, 0), N - 1), zc = min(max(z, 0), N - 1);
-  const u32* row = G + ((size_t)zc * N + yc) * W;
-  if (W >= 3) {
-    const int s = min(max(wx - 1, 0), W - 3);
-    const u32x3a4 v = *reinterpret_cast<const u32x3a4*>(row + s);
-    const u32 lv = wx == 0 ? 0u : (wx == W - 1 ? v.y : v.x);
-    const u32 cv = wx == 0 ? v.x : (wx == W - 1 ? v.z : v.y);
-    const u32 rv = wx == 0 ? v.y : (wx == W - 1 ? 0u : v.z);
-    l = in ? lv : 0u;
-    c = in ? cv : 0u;
-    r = in ? rv : 0u;
-    return;
-  }
-  const u32 lv = row[max(wx - 1, 0)], cv = row[wx], rv = row[min(wx + 1, W - 1)];
-  l = (in && wx > 0) ? lv : 0u;
-  c = in ? cv : 0u;
-  r = (in && wx + 1 < W) ? rv : 0u;
-}
 
 // Fixed-radius version of stream(): same plane schedule and accumulation,
 // chords and G-table geometry fixed at compile time.
@@ -393,7 +405,7 @@ __device__ void stream_ct(const GridSrc& src, u32* gtAll, int gtStride, u32* acc
   using C = StreamCT<KIND, RHO, YS>;
   const int lane = lane_id(), wv = wave_id(), nw = num_waves();
   constexpr int HY = C::HY;
-  constexpr int dzl = -HY, dzh = (KIND == SE_CORNER) ? HY - 1 : HY;
+  constexpr int dzl = -HY, dzh = (KIND != SE_CENTRE) ? HY - 1 : HY;
   const int ybase = g.yfirst - HY;
   u32* gt = gtAll + wv * gtStride;
   const u32* gl = gt + lane * C::M1;
@@ -404,27 +416,28 @@ __device__ void stream_ct(const GridSrc& src, u32* gtAll, int gtStride, u32* acc
   static_assert((size_t)C::M1 * (YS == 1 ? C::GR : 2 * C::GRH) <= (size_t)ct_slice_words(C::MAXM, C::HY, YS),
                 "slice too small");
   // the rows of the wave's next two planes are in flight while a plane is
-  // reduced: buffer a holds plane zi, buffer b plane zi + nw
-  u32 pa[C::NK][3], pb[C::NK][3];
+  // reduced: buffer a holds plane zi, buffer b plane zi + nw (row windows)
+  u32x3 pa[C::NK], pb[C::NK];
   int zi = zbeg + wv;
-  auto load = [&](u32 (&buf)[C::NK][3], int z) {
+  const __amdgpu_buffer_rsrc_t rs = grid_rsrc(src.g, src.N, src.W);
+  auto load = [&](u32x3 (&buf)[C::NK], int z) {
 #pragma unroll
-    for (int k = 0; k < C::NK; ++k)
-      load3(src.g, src.N, src.W, ybase + lane + kTileRows * k, z, g.wx, buf[k][0], buf[k][1], buf[k][2]);
+    for (int k = 0; k < C::NK; ++k) row_window(rs, src.N, src.W, ybase + lane + kTileRows * k, z, g.wx, buf[k]);
   };
   // x-window dilations of one plane's rows into the wave's slice
-  auto build = [&](const u32 (&buf)[C::NK][3]) {
+  auto build = [&](const u32x3 (&buf)[C::NK]) {
 #pragma unroll
     for (int k = 0; k < C::NK; ++k) {
       const int gr = lane + kTileRows * k;
       if (k + 1 < C::NK || gr < C::GR) {
-        const u32 Lw = buf[k][0], Cw = buf[k][1], Rw = buf[k][2];
+        u32 Lw, Cw, Rw;
+        pick3(buf[k], g.wx, src.W, Lw, Cw, Rw);
         u32* gw = gt + (YS == 1 ? gr * C::M1 : ((gr & 1) * C::GRH + (gr >> 1)) * C::M1);
-        u32 gg = KIND == SE_CORNER ? (Cw | shl_k(Lw, Cw, 1)) : Cw;
+        u32 gg = KIND != SE_CENTRE ? (Cw | shl_k(Lw, Cw, 1)) : Cw;
         gw[0] = gg;
 #pragma unroll
         for (int m = 1; m <= C::MAXM; ++m) {
-          gg |= shr_k(Cw, Rw, m) | shl_k(Lw, Cw, KIND == SE_CORNER ? m + 1 : m);
+          gg |= shr_k(Cw, Rw, m) | shl_k(Lw, Cw, KIND != SE_CENTRE ? m + 1 : m);
           gw[m] = gg;
         }
       }
@@ -475,6 +488,17 @@ struct MaskedSrc {
       if (radius_at(rf, 32 * w + b, y, z) - 1 == rho) out |= 1u << b;
     }
     return out;
+  }
+  typedef u32x3 Win;  // the three masked words, computed at once
+  __device__ __forceinline__ void window(__amdgpu_buffer_rsrc_t, int y, int z, int wx, Win& v) const {
+    v.x = (*this)(y, z, wx - 1);
+    v.y = (*this)(y, z, wx);
+    v.z = (*this)(y, z, wx + 1);
+  }
+  __device__ __forceinline__ void pick(const Win& v, int, u32& l, u32& c, u32& r) const {
+    l = v.x;
+    c = v.y;
+    r = v.z;
   }
 };
 
@@ -768,7 +792,7 @@ __device__ u32 wave_vset_planes(const u32* vset, const uint8_t* gate, int N, int
 // Runs the predicate of a pass over the tile and calls fin(zo, F) for every
 // output plane of the thread's row (F = 0 when skip is set). which: 0
 // dilation table (radius+add), 1 erosion corner, 2 erosion centre (radius-1).
-template <int WHICH, int YS, int RHO, class Fin>
+template <int WHICH, int YS, int RHO, int KIND = -1, class Fin>
 __device__ void run_tile(const MorphArgs& a, Lds L, u32* set4, const TileGeo& g, const u32* src, bool skip,
                          Fin& fin) {
   constexpr int which = WHICH;
@@ -780,7 +804,9 @@ __device__ void run_tile(const MorphArgs& a, Lds L, u32* set4, const TileGeo& g,
     if constexpr (RHO > 0) {
       // uniform radius with a compiled chord table (the host picks the kernel)
       __syncthreads();
-      stream_ct<WHICH == 2 ? SE_CENTRE : SE_CORNER, RHO, YS>(GridSrc{src, a.N, a.W}, L.gt, L.gtStride, L.out, g);
+      // (KIND >= 0: the compiled table of that kind, the cube's for the box dilations)
+      stream_ct<KIND >= 0 ? KIND : (WHICH == 2 ? SE_CENTRE : SE_CORNER), RHO, YS>(GridSrc{src, a.N, a.W}, L.gt,
+                                                                                   L.gtStride, L.out, g);
     } else if (a.rf.mode == 0) {
       const int rho = (a.rf.r & 0x7f) + add;
       const int off = table_off(a, which, rho);
@@ -1205,21 +1231,24 @@ __device__ void select_tiles_batched(const MorphArgs* __restrict__ tab, const u3
 
 // The predicate of one active tile and its epilogue. RHO > 0: uniform radius
 // with the compiled chord table of that radius; RHO = 0: runtime tables
-// (uniform or per-voxel radii, sphere or cube).
+// (uniform or per-voxel radii, sphere or cube). PASS 4 and 5 are passes 0 and
+// 1 (dilation) with the cube's compiled table.
 template <int PASS, int RHO>
 __device__ __forceinline__ void run_pass_tile(const MorphArgs& a, u32 entry, u32* smem, u32* set4) {
-  TileGeo g = pass_tile<PASS>(a, entry_tile(entry));
+  constexpr int P = PASS & 3;
+  constexpr int KIND = PASS >= 4 ? SE_BOX : -1;
+  TileGeo g = pass_tile<P>(a, entry_tile(entry));
   // per-voxel radii: a dilation entry holds the output planes of its value;
   // otherwise entries hold the input planes with source voxels
-  entry_range(entry, g, (PASS < 2 && a.rvar) ? 1 : 0);
+  entry_range(entry, g, (P < 2 && a.rvar) ? 1 : 0);
   Lds L = carve(smem, a, g.nout());
-  if constexpr (PASS == 0) {
+  if constexpr (P == 0) {
     DilCoarseFin fin{&a, g.wx, g.yfirst + 2 * lane_id(), cell_range_mask(g.wx, 0, a.dR0), 0};
-    run_tile<0, 2, RHO>(a, L, set4, g, a.I, false, fin);
-  } else if constexpr (PASS == 1) {
+    run_tile<0, 2, RHO, KIND>(a, L, set4, g, a.I, false, fin);
+  } else if constexpr (P == 1) {
     DilFineFin fin{&a, g.wx, g.yfirst + lane_id()};
-    run_tile<0, 1, RHO>(a, L, set4, g, a.I, false, fin);
-  } else if constexpr (PASS == 2) {
+    run_tile<0, 1, RHO, KIND>(a, L, set4, g, a.I, false, fin);
+  } else if constexpr (P == 2) {
     EroCoarseFin fin{&a, g.wx, g.yfirst + 2 * lane_id(), cell_range_mask(g.wx, 0, a.dR0)};
     run_tile<1, 2, RHO>(a, L, set4, g, a.K, false, fin);
   } else {

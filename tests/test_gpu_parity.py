@@ -117,15 +117,17 @@ def test_sparse_input(ctx, r):
     assert np.array_equal(ctx.get_cells(B.GRID_CLOSING), O.erode(I, Dref, R, r=r))
 
 
-def test_cube_dilation(ctx):
-    R = 32
-    I = random_blobs(R, seed=3)
+@pytest.mark.parametrize("R", [32, 48, 64])
+def test_cube_dilation(ctx, R):
+    # uniform radii run the compiled box kernels (rho = r, 1..31); N = 64
+    # has 2 words per row, N = 96 and 128 three and four
+    I = random_blobs(R, seed=3 + R)
     grid_setup(ctx, R, I)
-    for r in (2, 5):
+    for r in ((1, 2, 3, 5, 8, 13, 21, 31) if R == 32 else (2, 7, 12)):
         ctx.scale_uniform(r)
-        say("cube dilation r=%d" % r)
+        say("cube dilation R=%d r=%d" % (R, r))
         ctx.dilate(B.SE_CUBE)
-        assert np.array_equal(ctx.get_cells(B.GRID_DILATION), O.dilate(I, R, r=r, se=1))
+        assert np.array_equal(ctx.get_cells(B.GRID_DILATION), O.dilate(I, R, r=r, se=1)), r
 
 
 def test_scale_grid_i8(ctx):

@@ -88,7 +88,7 @@ def build_driver():
     out = os.path.join(bindir, "facade_closing")
     _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
           os.path.join(ROOT, "tests", "cpp", "facade_closing.cpp"), "-o", out,
-          "-L", LIBDIR, "-lgmorpho_hip", "-lgmorpho_meshgen", "-L/opt/rocm/lib",
+          "-L", LIBDIR, "-lgmorpho_hip", "-lgmorpho_meshgen", "-lgmorpho_decimate", "-L/opt/rocm/lib",
           "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib"])
     return out
 

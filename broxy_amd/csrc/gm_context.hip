@@ -46,7 +46,7 @@ struct gm_ctx {
   int nv = 0, nf = 0;
   size_t capP = 0, capTri = 0;
   u32 *mark = nullptr, *shell = nullptr;
-  u32* dCheck = nullptr;          // mesh check results (8 u32)
+  u32* dCheck = nullptr;          // mesh check results (8 u32, then the per-workgroup parts)
   uint8_t* hStage = nullptr;      // pinned upload staging of the mesh
   size_t capStage = 0;
   // radius field
@@ -667,7 +667,7 @@ int gm_load_mesh(gm_ctx* c, const float* P, const float* N, int nv, const uint32
     HIPCALL(c, hipMalloc(&c->dT, (size_t)nf * 12));
     c->capTri = nf;
   }
-  if (!c->dCheck) HIPCALL(c, hipMalloc(&c->dCheck, 32));
+  if (!c->dCheck) HIPCALL(c, hipMalloc(&c->dCheck, mesh_check_words() * 4));
   // through pinned staging: one host copy, then asynchronous DMA
   const size_t bytes = (size_t)nv * 12 + (size_t)nf * 12;
   if (bytes > c->capStage) {
@@ -684,9 +684,6 @@ int gm_load_mesh(gm_ctx* c, const float* P, const float* N, int nv, const uint32
   HIPCALL(c, hipMemcpyAsync(c->dP, c->hStage, (size_t)nv * 12, hipMemcpyHostToDevice, c->stream));
   HIPCALL(c, hipMemcpyAsync(c->dT, c->hStage + (size_t)nv * 12, (size_t)nf * 12, hipMemcpyHostToDevice, c->stream));
   // index check and bounding box on the device (kept for ComputeGridAttributes)
-  const u32 init[8] = {0u, 0u, ~0u, ~0u, ~0u, 0u, 0u, 0u};
-  std::memcpy(c->hTotals + 4, init, sizeof(init));
-  HIPCALL(c, hipMemcpyAsync(c->dCheck, c->hTotals + 4, 32, hipMemcpyHostToDevice, c->stream));
   HIPCALL(c, launch_mesh_check(c->dP, nv, c->dT, nf, c->dCheck, c->stream));
   HIPCALL(c, hipMemcpyAsync(c->hTotals + 4, c->dCheck, 32, hipMemcpyDeviceToHost, c->stream));
   HIPCALL(c, hipStreamSynchronize(c->stream));

@@ -117,6 +117,8 @@ hipError_t launch_rows_to_cells(const u32* rows, uint8_t* cells, int R, hipStrea
 // Index check and bounding box of a device mesh: out (8 u32, initialised to
 // 0, 0, ~0, ~0, ~0, 0, 0, 0) receives the bad-index flag, the non-finite
 // flag and order-preserving keys of the per-axis min and max coordinates.
+// out: mesh_check_words() u32 (the 8 results first, then per-workgroup parts).
+size_t mesh_check_words();
 hipError_t launch_mesh_check(const float* P, int nv, const u32* T, int nt, u32* out, hipStream_t s);
 // The float a launch_mesh_check key stands for.
 inline float mesh_key_float(u32 k) {

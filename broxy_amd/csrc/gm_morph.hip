@@ -44,9 +44,14 @@ namespace gm {
 
 namespace {
 
-// Tile selection of each pass (one wave per tile).
+// Tile selection of each pass (one wave per tile). A workgroup of 16 waves
+// covers a slab of 16 word tiles at N = 512, so it reserves its queue slots
+// with one or two device atomics: those atomics, from every workgroup on
+// one cache line, serialize across the XCDs, and with 4 waves per workgroup
+// (4 times the atomics) they bounded the selection.
+constexpr int kSelectWaves = 16;
 template <int PASS>
-__global__ void __launch_bounds__(256) k_select(MorphArgs a) {
+__global__ void __launch_bounds__(64 * kSelectWaves) k_select(MorphArgs a) {
   select_tiles<PASS>(a);
 }
 
@@ -636,7 +641,8 @@ hipError_t launch_pass(int pass, const void* generic, MorphArgs a, int* qsel, bo
   a.qCap = a.queueCap;
   a.qBack = 1;
   void* args[] = {&a};
-  GM_CHECK(hipLaunchKernel(sel[pass], dim3((tiles + 3) / 4), dim3(256), args, 0, s));
+  GM_CHECK(hipLaunchKernel(sel[pass], dim3((tiles + kSelectWaves - 1) / kSelectWaves), dim3(64 * kSelectWaves),
+                            args, 0, s));
   if (evStream) GM_CHECK(hipEventRecord(evStream, s));
   const int nw = stream_waves(a, var);
   const size_t lds = lds_bytes(a, var, nw);
@@ -664,7 +670,8 @@ hipError_t launch_pass_var(int pass, MorphArgs a, const MorphBuffers& b, int vlo
   const unsigned tiles = stream_grid(a).x;
   {
     void* args[] = {&a};
-    GM_CHECK(hipLaunchKernel(sel[pass], dim3((tiles + 3) / 4), dim3(256), args, 0, s));
+    GM_CHECK(hipLaunchKernel(sel[pass], dim3((tiles + kSelectWaves - 1) / kSelectWaves), dim3(64 * kSelectWaves),
+                            args, 0, s));
   }
   if (evStream) GM_CHECK(hipEventRecord(evStream, s));
   const bool ero = pass >= 2;

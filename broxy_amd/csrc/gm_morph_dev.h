@@ -109,6 +109,11 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t grid_rsrc(const u32* G, int N,
 // which serializes a plane's loads and defeats the prefetch of the planes
 // ahead.
 typedef u32 u32x3 __attribute__((ext_vector_type(3)));
+// Planes whose row loads a wave of a compiled stream keeps in flight.
+#ifndef GM_STREAM_DEPTH
+#define GM_STREAM_DEPTH 2
+#endif
+constexpr int kStreamDepth = GM_STREAM_DEPTH;
 __device__ __forceinline__ void row_window(__amdgpu_buffer_rsrc_t G, int N, int W, int y, int z, int wx, u32x3& v) {
   const bool in = (unsigned)y < (unsigned)N && (unsigned)z < (unsigned)N;
   const int st = (z * N + y) * W + (wx > 0 ? wx - 1 : 0);
@@ -415,9 +420,10 @@ __device__ void stream_ct(const GridSrc& src, u32* gtAll, int gtStride, u32* acc
   static_assert(ct_monotone<C>(), "chord half-widths must not grow with |dz|");
   static_assert((size_t)C::M1 * (YS == 1 ? C::GR : 2 * C::GRH) <= (size_t)ct_slice_words(C::MAXM, C::HY, YS),
                 "slice too small");
-  // the rows of the wave's next two planes are in flight while a plane is
-  // reduced: buffer a holds plane zi, buffer b plane zi + nw (row windows)
-  u32x3 pa[C::NK], pb[C::NK];
+  // the rows of the wave's next kStreamDepth planes are in flight while a
+  // plane is reduced: buffer s holds the row windows of plane
+  // zi + ((s - i) mod kStreamDepth) nw at step i
+  u32x3 buf[kStreamDepth][C::NK];
   int zi = zbeg + wv;
   const __amdgpu_buffer_rsrc_t rs = grid_rsrc(src.g, src.N, src.W);
   auto load = [&](u32x3 (&buf)[C::NK], int z) {
@@ -456,20 +462,30 @@ __device__ void stream_ct(const GridSrc& src, u32* gtAll, int gtStride, u32* acc
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   };
-  // planes past zend are loaded (in bounds, masked) and not used
-  load(pa, zi);
-  load(pb, zi + nw);
+  // planes past zend are loaded (in bounds, masked) and not used. The
+  // scheduling barriers keep each buffer's loads together and in buffer
+  // order, so the wait before a build covers that buffer's loads only.
+#pragma unroll
+  for (int s = 0; s < kStreamDepth; ++s) {
+    load(buf[s], zi + s * nw);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  // Whole rounds of kStreamDepth steps: every step issues its loads (a
+  // step past zend loads planes it never uses), so the number of loads
+  // issued after a buffer's is the same on every path and each wait is
+  // exact; only the build and the reduction of a plane past zend are
+  // skipped (a wave-uniform test).
   zi = __builtin_amdgcn_readfirstlane(zi);
   while (zi <= zend) {
-    build(pa);
-    load(pa, zi + 2 * nw);
-    reduce(zi);
-    zi += nw;
-    if (zi > zend) break;
-    build(pb);
-    load(pb, zi + 2 * nw);
-    reduce(zi);
-    zi += nw;
+#pragma unroll
+    for (int s = 0; s < kStreamDepth; ++s) {
+      const bool live = zi <= zend;
+      if (live) build(buf[s]);
+      load(buf[s], zi + kStreamDepth * nw);
+      __builtin_amdgcn_sched_barrier(0);
+      if (live) reduce(zi);
+      zi += nw;
+    }
   }
 }
 

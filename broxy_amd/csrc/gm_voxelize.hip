@@ -200,50 +200,97 @@ __device__ __forceinline__ u32 fkey(float f) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-// Mesh check before voxelization: out[0] = 1 if a triangle index is >= nv,
-// out[1] = 1 if a coordinate is not finite, out[2..4] / out[5..7] = keys of
-// the per-axis minimum / maximum coordinate (out must hold 0, 0, ~0 x 3,
-// 0 x 3). Per-workgroup reductions, then one atomic per value.
-__global__ void __launch_bounds__(256) k_mesh_check(const float* __restrict__ P, int nv, const u32* __restrict__ T,
-                                                     int nt, u32* out) {
-  __shared__ u32 red[8];
-  if (threadIdx.x < 8) red[threadIdx.x] = (threadIdx.x >= 2 && threadIdx.x < 5) ? ~0u : 0u;
+// Mesh check before voxelization, in two kernels without atomics (device
+// atomics on one cache line from every workgroup serialize across the XCDs:
+// 154 us for config 1 with one set per workgroup). k_mesh_check: each of
+// kCheckBlocks workgroups reduces its share (wave shuffles, then LDS) into 8
+// words at part + 8 * block: [0] some triangle index >= nv, [1] some
+// coordinate not finite, [2..4] / [5..7] keys of the per-axis minimum /
+// maximum coordinate. k_mesh_check_final reduces the parts into out[0..7].
+constexpr int kCheckBlocks = 256;
+__device__ __forceinline__ u32 wave_or(u32 v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= (u32)__shfl_xor((int)v, o);
+  return v;
+}
+__device__ __forceinline__ u32 wave_min(u32 v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, (u32)__shfl_xor((int)v, o));
+  return v;
+}
+__device__ __forceinline__ u32 wave_max(u32 v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (u32)__shfl_xor((int)v, o));
+  return v;
+}
+// The 8 check words of this thread reduced over the workgroup (256 threads):
+// the result in w[] of thread 0.
+__device__ __forceinline__ void block_check_reduce(u32 (&w)[8]) {
+  __shared__ u32 red[4][8];
+  const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
+  w[0] = wave_or(w[0]);
+  w[1] = wave_or(w[1]);
+  for (int a = 0; a < 3; ++a) {
+    w[2 + a] = wave_min(w[2 + a]);
+    w[5 + a] = wave_max(w[5 + a]);
+  }
+  if (lane == 0)
+    for (int k = 0; k < 8; ++k) red[wv][k] = w[k];
   __syncthreads();
-  u32 bad = 0u, nonfinite = 0u, mn[3] = {~0u, ~0u, ~0u}, mx[3] = {0u, 0u, 0u};
+  if (threadIdx.x == 0)
+    for (int q = 1; q < 4; ++q) {
+      w[0] |= red[q][0];
+      w[1] |= red[q][1];
+      for (int a = 0; a < 3; ++a) {
+        w[2 + a] = min(w[2 + a], red[q][2 + a]);
+        w[5 + a] = max(w[5 + a], red[q][5 + a]);
+      }
+    }
+}
+__global__ void __launch_bounds__(256) k_mesh_check(const float* __restrict__ P, int nv, const u32* __restrict__ T,
+                                                     int nt, u32* __restrict__ part) {
+  u32 w[8] = {0u, 0u, ~0u, ~0u, ~0u, 0u, 0u, 0u};
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < 3 * (size_t)nt; i += stride)
-    bad |= T[i] >= (u32)nv ? 1u : 0u;
+    w[0] |= T[i] >= (u32)nv ? 1u : 0u;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < (size_t)nv; i += stride)
     for (int a = 0; a < 3; ++a) {
       const float v = P[3 * i + a];
-      if (!isfinite(v)) nonfinite = 1u;
+      if (!isfinite(v)) w[1] = 1u;
       const u32 k = fkey(v);
-      mn[a] = min(mn[a], k);
-      mx[a] = max(mx[a], k);
+      w[2 + a] = min(w[2 + a], k);
+      w[5 + a] = max(w[5 + a], k);
     }
-  if (bad) atomicOr(&red[0], 1u);
-  if (nonfinite) atomicOr(&red[1], 1u);
-  for (int a = 0; a < 3; ++a) {
-    atomicMin(&red[2 + a], mn[a]);
-    atomicMax(&red[5 + a], mx[a]);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (red[0]) atomicOr(&out[0], 1u);
-    if (red[1]) atomicOr(&out[1], 1u);
+  block_check_reduce(w);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 8; ++k) part[8 * blockIdx.x + k] = w[k];
+}
+__global__ void __launch_bounds__(256) k_mesh_check_final(const u32* __restrict__ part, int nparts, u32* out) {
+  u32 w[8] = {0u, 0u, ~0u, ~0u, ~0u, 0u, 0u, 0u};
+  for (int b = (int)threadIdx.x; b < nparts; b += blockDim.x) {
+    const u32* p = part + 8 * b;
+    w[0] |= p[0];
+    w[1] |= p[1];
     for (int a = 0; a < 3; ++a) {
-      atomicMin(&out[2 + a], red[2 + a]);
-      atomicMax(&out[5 + a], red[5 + a]);
+      w[2 + a] = min(w[2 + a], p[2 + a]);
+      w[5 + a] = max(w[5 + a], p[5 + a]);
     }
   }
+  block_check_reduce(w);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 8; ++k) out[k] = w[k];
 }
 
 }  // namespace
 
+size_t mesh_check_words() { return 8 + 8 * (size_t)kCheckBlocks; }
+
 hipError_t launch_mesh_check(const float* P, int nv, const u32* T, int nt, u32* out, hipStream_t s) {
   const size_t work = std::max<size_t>(3 * (size_t)nt, (size_t)nv);
-  const unsigned blocks = (unsigned)std::min<size_t>((work + 255) / 256, 2048);
-  hipLaunchKernelGGL(k_mesh_check, dim3(std::max(blocks, 1u)), dim3(256), 0, s, P, nv, T, nt, out);
+  const int blocks = (int)std::max<size_t>(1, std::min<size_t>((work + 255) / 256, kCheckBlocks));
+  hipLaunchKernelGGL(k_mesh_check, dim3(blocks), dim3(256), 0, s, P, nv, T, nt, out + 8);
+  GM_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_mesh_check_final, dim3(1), dim3(256), 0, s, out + 8, blocks, out);
   return hipGetLastError();
 }
 

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <string>
 #include <utility>
@@ -294,6 +295,30 @@ class ScaleField {
   const std::vector<ScalePoint>& strokes() const { return strokes_; }
   unsigned long long id() const { return id_; }
   unsigned long long generation() const { return generation_; }
+  // ScaleField::GetScaleGridCPU (ScaleField.cu:74-79): a new[] copy of the
+  // res^3 grid, which the caller deletes (the viewer hands it to the
+  // Decimator). A brushed field lives on the device of the GMorpho that last
+  // applied it (GMorpho::Update); a uniform field is the memset byte of
+  // UpdateGridGlobalScale; a lattice field has no int8 grid here.
+  void GetScaleGridCPU(char** scale_grid_cpu_ptr) const {
+    const size_t n = (size_t)res_[0] * res_[1] * res_[2];
+    if (mode_ == LATTICE_F32) throw Exception("a lattice field has no int8 scale grid");
+    if (mode_ == BRUSHED && !device_) throw Exception("brush strokes reach the grid at GMorpho::Update first");
+    if (mode_ == GRID_I8 && grid_.size() != n) throw Exception("the scale grid does not match res");
+    char* g = new char[n];
+    if (mode_ == UNIFORM) {
+      std::memset(g, uniform_radius() & 0xff, n);
+    } else if (mode_ == GRID_I8) {
+      std::memcpy(g, grid_.data(), n);
+    } else if (gm_scale_grid_get(device_, reinterpret_cast<int8_t*>(g)) != GM_OK) {
+      const std::string m = gm_last_error(device_);
+      delete[] g;
+      throw Exception(m);
+    }
+    *scale_grid_cpu_ptr = g;
+  }
+  // the context that last applied this field (set by GMorpho)
+  void set_device(gm_ctx* ctx) const { device_ = ctx; }
 
  private:
   static unsigned long long next_id() {
@@ -316,6 +341,7 @@ class ScaleField {
   int L_ = 0;
   std::vector<ScalePoint> points_, strokes_;
   unsigned long long id_ = 0, generation_ = 0;
+  mutable gm_ctx* device_ = nullptr;
 };
 
 class GMorpho {
@@ -442,6 +468,7 @@ class GMorpho {
     }
   }
   void apply_scale(const ScaleField& sf) {
+    sf.set_device(ctx_);
     if (sf.mode() != ScaleField::BRUSHED) brushed_id_ = 0;
     switch (sf.mode()) {
       case ScaleField::BRUSHED: {

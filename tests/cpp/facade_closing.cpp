@@ -9,12 +9,16 @@
 //           normal filtering (BroxyViewer.cpp:66, 125, 138), 200 iterations;
 //   fixed:  Update(Mesh&, float se_size) (GMorpho.cu:7819-7845);
 //   brush:  ball closing with a field edited by brush strokes (AddPoint +
-//           UpdateGrid, BroxyViewer.cpp:670-671), the radius grid written too.
+//           UpdateGrid, BroxyViewer.cpp:670-671), the radius grid written too
+//           (ScaleField::GetScaleGridCPU), then the viewer's proxy: the
+//           Decimator loop on the closing mesh with that grid
+//           (BroxyViewer.cpp:1060-1125), stored as <out_prefix>_proxy.off.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
 #include <vector>
 
+#include "decimator.hpp"
 #include "gmorpho.hpp"
 #include "gmorpho_meshgen.h"
 
@@ -58,11 +62,24 @@ int main(int argc, char** argv) {
       sf.UpdateGrid();
       gmorpho.Update(mesh, sf, FrameField());
       const size_t n = (size_t)res[0] * res[1] * res[2];
-      std::vector<int8_t> g(n);
-      if (gm_scale_grid_get(gmorpho.handle(), g.data())) return 1;
+      char* g = nullptr;
+      sf.GetScaleGridCPU(&g);
       FILE* fg = std::fopen((std::string(argv[1]) + "_scale.i8").c_str(), "wb");
-      std::fwrite(g.data(), 1, n, fg);
+      std::fwrite(g, 1, n, fg);
       std::fclose(fg);
+      // the viewer's proxy (ball closing: target error 0.001, 20 faces,
+      // edge-length alpha 1, linear constraints, no features, 3 HC iterations)
+      std::vector<bool> tags(mesh.P().size(), false);
+      Decimator decimator(mesh, 20, 0.001f, true, false, 1.f, sf.bbox_min(), sf.bbox_max(), sf.res(),
+                          sf.cell_size(), g, tags, 3);
+      bool more = true;
+      while (more && decimator.num_faces() > decimator.target_num_faces() &&
+             decimator.error() < decimator.target_error())
+        more = decimator.Optimize();
+      Mesh proxy;
+      decimator.GetMesh(proxy);
+      proxy.store(std::string(argv[1]) + "_proxy.off");
+      delete[] g;
     } else if (mode == "fixed") {
       // se_size in world units: radius floor(2 * se_size / cell_size)
       gmorpho.Update(mesh, 4.25f * cell);
@@ -100,6 +117,12 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "%s\n", e.msg().c_str());
     return 1;
   } catch (const Mesh::Exception& e) {
+    std::fprintf(stderr, "%s\n", e.msg().c_str());
+    return 1;
+  } catch (const ScaleField::Exception& e) {
+    std::fprintf(stderr, "%s\n", e.msg().c_str());
+    return 1;
+  } catch (const Decimator::Exception& e) {
     std::fprintf(stderr, "%s\n", e.msg().c_str());
     return 1;
   }

@@ -598,6 +598,25 @@ def test_facade_brush_strokes(tmp_path):
     Vr, Nr, Tr = O.mc(Cr, R, bmin, cell)
     assert np.array_equal(Tt, Tr)
     assert np.allclose(V, Vr, rtol=VERT_RTOL, atol=0)
+    # the viewer's proxy (BroxyViewer.cpp:1060-1125): the facade's Decimator
+    # loop on the closing mesh and the grid from GetScaleGridCPU equals the
+    # library's on the same input, and is a smaller closed triangle mesh
+    d = B.Decimator(V, Tt, 20, 0.001, use_linear_constraints=True, max_edge_length=1.0,
+                    bbox_min=np.asarray(bmin, np.float32), res=(N, N, N), cell=float(fc), scale_grid=g, hc_iters=3)
+    more = True
+    while more and d.state()[0] > 20 and d.state()[2] < 0.001:
+        more = d.optimize()
+    Pp, Tp = d.mesh()
+    d.close()
+    tok = open(str(tmp_path / "facade_brush_proxy.off")).read().split()
+    nv, nf = int(tok[1]), int(tok[2])
+    Pf = np.array(tok[4:4 + 3 * nv], np.float32).reshape(nv, 3)
+    Ff = np.array(tok[4 + 3 * nv:], np.int64).reshape(nf, 4)
+    assert np.array_equal(Ff[:, 1:], Tp.astype(np.int64)) and np.array_equal(Pf, Pp)
+    assert 0 < nf < len(Tt)
+    edges = np.sort(np.concatenate([Tp[:, [0, 1]], Tp[:, [1, 2]], Tp[:, [2, 0]]]), axis=1)
+    _, cnt = np.unique(edges, axis=0, return_counts=True)
+    assert np.all(cnt == 2)  # every edge shared by exactly two faces
 
 
 def test_graph_replays_match_eager(ctx):

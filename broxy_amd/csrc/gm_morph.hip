@@ -61,13 +61,23 @@ __global__ void __launch_bounds__(64 * kMaxWaves) k_dil_fine(MorphArgs a) { run_
 __global__ void __launch_bounds__(64 * kMaxWaves) k_ero_coarse(MorphArgs a) { run_pass<2, 0>(a); }
 __global__ void __launch_bounds__(64 * kMaxWaves) k_ero_fine(MorphArgs a) { run_pass<3, 0>(a); }
 
+// (wx, cy, cz) of cell-row word g of an R x R x W flag grid. R*R*W <= 2^26,
+// so 32-bit divisions do (64-bit ones are expanded into long VALU
+// sequences and made these kernels VALU-bound).
+__device__ __forceinline__ void cell_word_at(u32 g, u32 R, u32 W, int& wx, int& cy, int& cz) {
+  const u32 t = g / W;
+  const u32 z = t / R;
+  wx = (int)(g - t * W);
+  cy = (int)(t - z * R);
+  cz = (int)z;
+}
+
 // Tight contour of DF -> TG flags (one thread per cell-row word).
 __device__ __forceinline__ void tight_body(const MorphArgs& a, long long gid) {
   long long total = (long long)a.R * a.R * a.W;
   if (gid >= total) return;
-  int wx = (int)(gid % a.W);
-  long long t = gid / a.W;
-  int cy = (int)(t % a.R), cz = (int)(t / a.R);
+  int wx, cy, cz;
+  cell_word_at((u32)gid, (u32)a.R, (u32)a.W, wx, cy, cz);
   // flag words (cy + (d & 1), cz + (d >> 1)) at wx and wx + 1: the cell
   // rows at x and y distance 1 are gids +1 and +W, consecutive over the
   // lanes of a workgroup (a multiple of 64 threads), so those inside the
@@ -363,9 +373,8 @@ __global__ void __launch_bounds__(256) k_radius_planes(RadiusField rf, int N, in
 __global__ void k_ifull(const u32* __restrict__ I, int R, int N, int W, u16* __restrict__ DF) {
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (long long)R * R * W) return;
-  const int wx = (int)(gid % W);
-  const long long t = gid / W;
-  const int cy = (int)(t % R), cz = (int)(t / R);
+  int wx, cy, cz;
+  cell_word_at((u32)gid, (u32)R, (u32)W, wx, cy, cz);
   const u32* r = I + ((size_t)(2 * cz) * N + 2 * cy) * W + wx;
   const size_t pz = (size_t)N * W;
   const u32 f = r[0] & r[W] & r[pz] & r[pz + W];
@@ -412,9 +421,8 @@ __device__ __forceinline__ void cprime_rows(const MorphArgs& a, int cy, int cz, 
 __device__ __forceinline__ void base_contour_body(const MorphArgs& a, long long gid) {
   long long total = (long long)a.R * a.R * a.W;
   if (gid >= total) return;
-  int wx = (int)(gid % a.W);
-  long long t = gid / a.W;
-  int cy = (int)(t % a.R), cz = (int)(t / a.R);
+  int wx, cy, cz;
+  cell_word_at((u32)gid, (u32)a.R, (u32)a.W, wx, cy, cz);
   u32 valid = cell_range_mask(wx, 1, a.R - 1);
   if (cy < 1 || cy > a.R - 2 || cz < 1 || cz > a.R - 2) valid = 0;
   u32 c[4], l[4], r[4], yp[4], ym[4], o[4];

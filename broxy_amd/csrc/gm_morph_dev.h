@@ -309,6 +309,17 @@ struct SETabCT {
   }
 };
 
+// Rows of one parity half of a ystep-2 G table (GR rows): padded so that the
+// two halves start 16 mod 32 rows apart. The build stores of a wave write
+// lanes 2j and 2j + 1 to rows j of the two halves; with that offset (and an
+// odd row stride) the 32 lanes of each half-wave store to 32 distinct banks.
+#ifndef GM_GRH_ALIGN
+#define GM_GRH_ALIGN 0
+#endif
+__host__ __device__ constexpr int ct_grh(int GR) {
+  const int h = (GR + 1) / 2;
+  return GM_GRH_ALIGN == 1 ? h + ((16 - h % 32) + 32) % 32 : (GM_GRH_ALIGN == 2 ? h + ((32 - h % 64) + 64) % 64 : h);
+}
 // G-table geometry of a fixed-radius stream. Rows of the tile window are
 // r = 0..GR-1 (row yfirst - HY + r); row r holds G_0..G_MAXM in consecutive
 // words (stride M1, odd, so that lanes reading rows lane + c hit distinct
@@ -322,7 +333,7 @@ struct StreamCT {
   static constexpr int HY = T.hy, MAXM = T.maxm, NB = T.nb, NA = SETabCT<KIND, RHO>::NMAX;
   static constexpr int M1 = (MAXM + 1) | 1;
   static constexpr int GR = YS * (kTileRows - 1) + 1 + 2 * HY;
-  static constexpr int GRH = (GR + 1) / 2;
+  static constexpr int GRH = ct_grh(GR);
   static constexpr int NK = (GR + kTileRows - 1) / kTileRows;
   static constexpr int row_at(int r) { return YS == 1 ? r * M1 : ((r & 1) * GRH + (r >> 1)) * M1; }
   // word of G_m at the row offset d from the lane's row, relative to gt + lane*M1
@@ -332,7 +343,8 @@ struct StreamCT {
 };
 // Words per wave slice of a fixed-radius stream (upper bound, any kind).
 __host__ __device__ constexpr int ct_slice_words(int maxm, int maxoff, int ystep) {
-  return (maxm + 2) * (((ystep * (kTileRows - 1) + 1 + 2 * maxoff) + 1) & ~1);
+  return (maxm + 2) * (ystep == 2 ? 2 * ct_grh(ystep * (kTileRows - 1) + 1 + 2 * maxoff)
+                                  : (((ystep * (kTileRows - 1) + 1 + 2 * maxoff) + 1) & ~1));
 }
 
 template <class C, int KIND, int B, int A>
@@ -641,27 +653,60 @@ __device__ bool wave_bricks_zrange(const uint8_t* br, int N, int W, int y0, int 
   hi = __builtin_amdgcn_readfirstlane(lhi);
   return hi >= lo;
 }
-// Every brick of the region has the "all" bit.
-__device__ bool wave_bricks_all(const uint8_t* br, int N, int W, int y0, int y1, int z0, int z1, int w) {
+// wave_bricks_zrange of bit 0 over the region, and at the same time whether
+// every brick of the inner region (rows [ay0,ay1), planes [az0,az1), word aw;
+// inside the region) has bit 1 (all): one pass over the bricks, 8 loads in
+// flight per lane, unsigned index arithmetic.
+__device__ bool wave_bricks_zrange_all(const uint8_t* br, int N, int W, int y0, int y1, int z0, int z1, int w0,
+                                       int w1, int ay0, int ay1, int az0, int az1, int aw, int& lo, int& hi,
+                                       bool& all) {
   const int NB = N / 8;
   const int by0 = max(y0, 0) >> 3, by1 = (min(y1, N) - 1) >> 3;
   const int bz0 = max(z0, 0) >> 3, bz1 = (min(z1, N) - 1) >> 3;
-  const int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1;
-  u32 all = 2u;
-  const int n = ny * nz;
-  for (int i0 = 0; i0 < n; i0 += 256) {
-    u32 v[4];
+  const int aby0 = max(ay0, 0) >> 3, aby1 = (min(ay1, N) - 1) >> 3;
+  const int abz0 = max(az0, 0) >> 3, abz1 = (min(az1, N) - 1) >> 3;
+  w0 = max(w0, 0);
+  w1 = min(w1, W - 1);
+  const int ny = by1 - by0 + 1, nz = bz1 - bz0 + 1, nw = w1 - w0 + 1;
+  int llo = 1 << 20, lhi = -1;
+  u32 inner = 2u;
+  if (ny > 0 && nz > 0 && nw > 0) {
+    const u32 n = (u32)(ny * nz * nw);
+    for (u32 i0 = 0; i0 < n; i0 += 512) {
+      u32 v[8];
+      int zb[8], in[8];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int i = min(i0 + 64 * q + lane_id(), n - 1);
-      const int by = by0 + i % ny, bz = bz0 + i / ny;
-      v[q] = br[((size_t)bz * NB + by) * W + w];
+      for (int q = 0; q < 8; ++q) {
+        const u32 i = min(i0 + 64u * q + (u32)lane_id(), n - 1u);
+        const u32 t = i / (u32)nw;
+        const int w = w0 + (int)(i - t * (u32)nw);
+        const u32 tz = t / (u32)ny;
+        const int by = by0 + (int)(t - tz * (u32)ny), bz = bz0 + (int)tz;
+        v[q] = br[((size_t)bz * NB + by) * W + w];
+        zb[q] = bz;
+        in[q] = w == aw && by >= aby0 && by <= aby1 && bz >= abz0 && bz <= abz1;
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        if (v[q] & 1u) {
+          llo = min(llo, zb[q]);
+          lhi = max(lhi, zb[q]);
+        }
+        if (in[q]) inner &= v[q];
+      }
     }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) all &= v[q];
   }
-  return __any((all & 2u) == 0u) == 0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    llo = min(llo, __shfl_xor(llo, off));
+    lhi = max(lhi, __shfl_xor(lhi, off));
+  }
+  lo = __builtin_amdgcn_readfirstlane(llo);
+  hi = __builtin_amdgcn_readfirstlane(lhi);
+  all = __any((inner & 2u) == 0u) == 0;
+  return hi >= lo;
 }
+
 // Some cell-row flag of word column wx set in rows [y0,y1), planes [z0,z1),
 // and the fine planes [olo, ohi] of the cell planes holding one
 // (wave-uniform; olo > ohi when none does).
@@ -1078,10 +1123,11 @@ __device__ bool select_tile(const MorphArgs& a, int t, const TileGeo& g, u32& va
   ohi = g.zlast;
   if constexpr (PASS == 0) {
     // F = 0 when no occupied voxel is in reach; F unused when every cell is full
-    const bool empty = !wave_bricks_zrange(a.brickI, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff,
-                                           g.zfirst - a.maxoff, g.zlast + 2 + a.maxoff, g.wx - 1, g.wx + 1, 0, slo,
-                                           shi);
-    const bool full = !empty && wave_bricks_all(a.brickI, a.N, a.W, g.yfirst, y1, g.zfirst, g.zlast + 2, g.wx);
+    bool allFull;
+    const bool empty = !wave_bricks_zrange_all(a.brickI, a.N, a.W, g.yfirst - a.maxoff, y1 + a.maxoff,
+                                               g.zfirst - a.maxoff, g.zlast + 2 + a.maxoff, g.wx - 1, g.wx + 1,
+                                               g.yfirst, y1, g.zfirst, g.zlast + 2, g.wx, slo, shi, allFull);
+    const bool full = !empty && allFull;
     skip = empty || full;
     if (a.rvar) {
       // DF already holds I-full, which is also the value of a skipped tile

@@ -245,8 +245,10 @@ class Context:
         return z.value, w.value
 
     def set_graphs(self, on=True):
-        """Replay gm_close / gm_extract_mesh as captured HIP graphs (default on)."""
-        self._call("gm_set_graphs", 1 if on else 0)
+        """Replay gm_close / gm_extract_mesh as captured HIP graphs (default
+        on). on=True captures on the first call, "repeat" only when the same
+        launch sequence comes a second time, False runs eagerly."""
+        self._call("gm_set_graphs", 2 if on == "repeat" else (1 if on else 0))
 
     def set_timing(self, on=True):
         self._call("gm_set_timing", 1 if on else 0)
@@ -364,11 +366,13 @@ class Engine:
         if rc:
             raise GMorphoError("gm_engine_create failed (%d): no usable HIP device" % rc)
         self._h = h
+        self._held = {}  # borrowed inputs of jobs not yet waited for
 
     def close(self):
         if self._h:
             self._lib.gm_engine_destroy(self._h)
             self._h = None
+            self._held = {}
 
     def __del__(self):
         try:
@@ -383,8 +387,11 @@ class Engine:
             raise GMorphoError("%s -> %d: %s" % (name, rc, msg.decode() if msg else ""))
         return rc
 
-    def submit(self, P, T, base_res, se_size=0.3, radius=0, lattice=None, se=SE_SPHERE):
-        """Queue one closing job (voxelize at base_res, close, mesh); returns its id."""
+    def submit(self, P, T, base_res, se_size=0.3, radius=0, lattice=None, se=SE_SPHERE, borrow=False):
+        """Queue one closing job (voxelize at base_res, close, mesh); returns
+        its id. borrow=True hands the arrays to the engine without a copy; this
+        object keeps them alive until the job is waited for, viewed, taken or
+        released."""
         P = np.ascontiguousarray(P, np.float32)
         T = np.ascontiguousarray(T, np.uint32)
         L = 0
@@ -395,14 +402,53 @@ class Engine:
         job = _Job(P.ctypes.data, len(P), T.ctypes.data, len(T), base_res, se_size, radius,
                    None if lat is None else lat.ctypes.data, L, se)
         jid = ctypes.c_uint64()
-        self._call("gm_engine_submit", ctypes.byref(job), ctypes.byref(jid))
+        self._call("gm_engine_submit_borrowed" if borrow else "gm_engine_submit", ctypes.byref(job),
+                   ctypes.byref(jid))
+        if borrow:
+            self._held[jid.value] = (P, T, lat)
         return jid.value
 
     def wait(self, jid):
         nv = ctypes.c_uint32()
         nt = ctypes.c_uint32()
-        self._call("gm_engine_wait", ctypes.c_uint64(jid), ctypes.byref(nv), ctypes.byref(nt))
+        try:
+            self._call("gm_engine_wait", ctypes.c_uint64(jid), ctypes.byref(nv), ctypes.byref(nt))
+        finally:
+            self._held.pop(jid, None)
         return nv.value, nt.value
+
+    def view(self, jid):
+        """Mesh (V, N, T) of a finished job as arrays over the engine's pinned
+        result block (no copy); valid until release(jid)."""
+        V, N, T = _P(), _P(), _P()
+        nv = ctypes.c_uint32()
+        nt = ctypes.c_uint32()
+        try:
+            self._call("gm_engine_view", ctypes.c_uint64(jid), ctypes.byref(V), ctypes.byref(N), ctypes.byref(T),
+                       ctypes.byref(nv), ctypes.byref(nt))
+        finally:
+            self._held.pop(jid, None)
+
+        def arr(p, n, ct, dt):
+            if n == 0:
+                return np.zeros((0, 3), dt)
+            return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ct)), shape=(n, 3))
+
+        return (arr(V, nv.value, ctypes.c_float, np.float32), arr(N, nv.value, ctypes.c_float, np.float32),
+                arr(T, nt.value, ctypes.c_uint32, np.uint32))
+
+    def release(self, jid):
+        """Forget a job; arrays from view(jid) are invalid afterwards."""
+        try:
+            self._call("gm_engine_release", ctypes.c_uint64(jid))
+        finally:
+            self._held.pop(jid, None)
+
+    def phases(self, jid):
+        """(submit, start, loaded, meshed, done) seconds on the engine clock."""
+        t = np.zeros(5, np.float64)
+        self._call("gm_engine_job_phases", ctypes.c_uint64(jid), _ptr(t))
+        return t
 
     def times(self, jid):
         """(submit, start, done) seconds on the engine clock."""
@@ -422,6 +468,7 @@ class Engine:
         N = np.zeros((nv, 3), np.float32)
         T = np.zeros((nt, 3), np.uint32)
         self._call("gm_engine_take", ctypes.c_uint64(jid), _ptr(V), _ptr(N), _ptr(T))
+        self._held.pop(jid, None)
         return V, N, T
 
     def stats(self):
@@ -458,8 +505,9 @@ class Batch:
             pass
 
     def set_graphs(self, on=True):
-        """Replay the batched launches as a captured HIP graph (default on)."""
-        rc = self._lib.gm_batch_set_graphs(self._h, 1 if on else 0)
+        """Replay the batched launches as a captured HIP graph (default on;
+        "repeat": capture when the same batch comes a second time)."""
+        rc = self._lib.gm_batch_set_graphs(self._h, 2 if on == "repeat" else (1 if on else 0))
         if rc:
             raise GMorphoError("gm_batch_set_graphs -> %d" % rc)
 

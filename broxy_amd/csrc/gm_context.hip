@@ -11,6 +11,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/gmorpho_c.h"
@@ -46,6 +47,7 @@ struct gm_ctx {
   int nv = 0, nf = 0;
   size_t capP = 0, capTri = 0;
   u32 *mark = nullptr, *shell = nullptr;
+  bool voxDirty = true;  // mark / shell may hold bits (fresh or interrupted)
   u32* dCheck = nullptr;          // mesh check results (8 u32, then the per-workgroup parts)
   uint8_t* hStage = nullptr;      // pinned upload staging of the mesh
   size_t capStage = 0;
@@ -96,13 +98,19 @@ struct gm_ctx {
   struct Graph {
     hipGraphExec_t exec = nullptr;
     uint64_t key[4] = {0, 0, 0, 0};
+    uint64_t seen[4] = {0, 0, 0, 0};  // key of the last eager run (repeat mode)
   };
   Graph gClose[2], gMesh;
   uint64_t graphGen = 1;
-  bool graphs = true;
+  // 0: eager; 1: capture at once; 2: capture when a key repeats (a context
+  // that runs a different job every time never pays for a capture)
+  int graphs = 1;
   // launch shape forced while gm_autotune measures (tileZ 0: the table's)
   int forceTileZ = 0, forceWaveCap = 0;
   int tuneSe = 0;  // structuring element of the closing being launched (tuning-table key)
+  // device bytes held by the context's own allocations
+  std::unordered_map<void*, size_t> allocBytes;
+  uint64_t devBytes = 0;
   // timing
   bool timing = false;
   hipEvent_t ev[20] = {};
@@ -142,6 +150,29 @@ void dfree(T*& p) {
   p = nullptr;
 }
 
+// Context allocations, counted so that a job engine can charge a slot for
+// what its context really holds after buffers grew.
+template <class T>
+hipError_t calloc_dev(gm_ctx* c, T** p, size_t bytes) {
+  const hipError_t e = hipMalloc((void**)p, bytes);
+  if (e == hipSuccess) {
+    c->allocBytes[(void*)*p] = bytes;
+    c->devBytes += bytes;
+  }
+  return e;
+}
+template <class T>
+void cfree(gm_ctx* c, T*& p) {
+  if (!p) return;
+  auto it = c->allocBytes.find((void*)p);
+  if (it != c->allocBytes.end()) {
+    c->devBytes -= it->second;
+    c->allocBytes.erase(it);
+  }
+  hipFree(p);
+  p = nullptr;
+}
+
 // Launch arguments may have changed: captured sequences are out of date.
 void touch(gm_ctx* c) { ++c->graphGen; }
 
@@ -161,6 +192,10 @@ int run_captured(gm_ctx* c, gm_ctx::Graph& g, const uint64_t key[4], F enqueue) 
     return GM_OK;
   }
   drop_graph(g);
+  if (c->graphs == 2 && !std::equal(key, key + 4, g.seen)) {
+    std::copy(key, key + 4, g.seen);
+    return enqueue();
+  }
   HIPCALL(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
   const int rc = enqueue();
   hipGraph_t graph = nullptr;
@@ -183,17 +218,17 @@ int run_captured(gm_ctx* c, gm_ctx::Graph& g, const uint64_t key[4], F enqueue) 
 
 void free_grid(gm_ctx* c) {
   touch(c);
-  dfree(c->I); dfree(c->D); dfree(c->K); dfree(c->C);
-  dfree(c->DF); dfree(c->TG); dfree(c->CL); dfree(c->UG); dfree(c->IFL); dfree(c->DAL); dfree(c->brickI); dfree(c->brickK); dfree(c->queue);
-  dfree(c->stage); dfree(c->mark); dfree(c->shell);
-  dfree(c->mc.blockV); dfree(c->mc.blockT); dfree(c->mc.cellV); dfree(c->mc.cellCase); dfree(c->mc.posOf); dfree(c->mc.order);
-  dfree(c->mc.V); dfree(c->mc.Nrm); dfree(c->mc.T); dfree(c->mc.late); dfree(c->mc.totals); dfree(c->mc.scanPart);
-  dfree(c->mc.brick); dfree(c->mc.list);
-  dfree(c->dPyr);
+  cfree(c, c->I); cfree(c, c->D); cfree(c, c->K); cfree(c, c->C);
+  cfree(c, c->DF); cfree(c, c->TG); cfree(c, c->CL); cfree(c, c->UG); cfree(c, c->IFL); cfree(c, c->DAL); cfree(c, c->brickI); cfree(c, c->brickK); cfree(c, c->queue);
+  cfree(c, c->stage); cfree(c, c->mark); cfree(c, c->shell);
+  cfree(c, c->mc.blockV); cfree(c, c->mc.blockT); cfree(c, c->mc.cellV); cfree(c, c->mc.cellCase); cfree(c, c->mc.posOf); cfree(c, c->mc.order);
+  cfree(c, c->mc.V); cfree(c, c->mc.Nrm); cfree(c, c->mc.T); cfree(c, c->mc.late); cfree(c, c->mc.totals); cfree(c, c->mc.scanPart);
+  cfree(c, c->mc.brick); cfree(c, c->mc.list);
+  cfree(c, c->dPyr);
   c->capPyr = 0;
-  dfree(c->dGrid);
+  cfree(c, c->dGrid);
   c->capGrid = 0;
-  dfree(c->rp); dfree(c->rvset); dfree(c->kv); dfree(c->kbl);
+  cfree(c, c->rp); cfree(c, c->rvset); cfree(c, c->kv); cfree(c, c->kbl);
   c->capVarR = 0;
   c->rpValid = false;
   c->has_grid = false;
@@ -205,24 +240,32 @@ void free_grid(gm_ctx* c) {
 int alloc_mesh_out(gm_ctx* c, u32 capV, u32 capT, u32 capLate) {
   if (capV > c->mc.capV || capT > c->mc.capT || capLate > c->mc.capLate || !c->mc.late) touch(c);
   if (capV > c->mc.capV) {
-    dfree(c->mc.V);
-    dfree(c->mc.Nrm);
-    HIPCALL(c, hipMalloc(&c->mc.V, (size_t)capV * 12));
-    HIPCALL(c, hipMalloc(&c->mc.Nrm, (size_t)capV * 12));
+    cfree(c, c->mc.V);
+    cfree(c, c->mc.Nrm);
+    HIPCALL(c, calloc_dev(c, &c->mc.V, (size_t)capV * 12));
+    HIPCALL(c, calloc_dev(c, &c->mc.Nrm, (size_t)capV * 12));
     c->mc.capV = capV;
   }
   if (capT > c->mc.capT) {
-    dfree(c->mc.T);
-    HIPCALL(c, hipMalloc(&c->mc.T, (size_t)capT * 12));
+    cfree(c, c->mc.T);
+    HIPCALL(c, calloc_dev(c, &c->mc.T, (size_t)capT * 12));
     c->mc.capT = capT;
   }
   if (capLate > c->mc.capLate || !c->mc.late) {
-    dfree(c->mc.late);
+    cfree(c, c->mc.late);
     // entries, then their copy in slot order (k_mc_fixup)
-    HIPCALL(c, hipMalloc(&c->mc.late, (size_t)(1 + 8 * capLate) * 4));
+    HIPCALL(c, calloc_dev(c, &c->mc.late, (size_t)(1 + 8 * capLate) * 4));
     c->mc.capLate = capLate;
   }
   return GM_OK;
+}
+
+// Capacity after an overflow: the need plus a quarter, and at least half as
+// much again as before, so a context that runs jobs of varying output sizes
+// regrows (and frees, which waits for the whole device) only a few times.
+u32 grow_cap(u32 cap, u32 need) {
+  const uint64_t a = (uint64_t)need + need / 4 + 1024, b = (uint64_t)cap + cap / 2;
+  return (u32)std::min<uint64_t>(std::max(a, b), 0xFFFFFFF0u);
 }
 
 // Per-listed-block marching-cubes buffers (cube cases, vertex bases) for cap
@@ -230,11 +273,11 @@ int alloc_mesh_out(gm_ctx* c, u32 capV, u32 capT, u32 capLate) {
 int alloc_mc_list(gm_ctx* c, u32 cap) {
   if (cap <= c->mc.capList && c->mc.cellCase) return GM_OK;
   touch(c);
-  dfree(c->mc.cellV);
-  dfree(c->mc.cellCase);
+  cfree(c, c->mc.cellV);
+  cfree(c, c->mc.cellCase);
   c->mc.capList = 0;
-  HIPCALL(c, hipMalloc(&c->mc.cellV, (size_t)cap * 512 * 4));
-  HIPCALL(c, hipMalloc(&c->mc.cellCase, (size_t)cap * 512 * 8));
+  HIPCALL(c, calloc_dev(c, &c->mc.cellV, (size_t)cap * 512 * 4));
+  HIPCALL(c, calloc_dev(c, &c->mc.cellCase, (size_t)cap * 512 * 8));
   c->mc.capList = cap;
   return GM_OK;
 }
@@ -266,8 +309,9 @@ void set_dims(gm_ctx* c, int R) {
 int alloc_grid(gm_ctx* c, int R) {
   if (R < 16 || R > 1024 || (R % 16) != 0)
     return fail(c, GM_ERR_UNSUPPORTED, "base resolution must be a multiple of 16 in [16,1024]");
-  if (c->has_grid && R <= c->capR) {
+  if (c->capR > 0 && R <= c->capR) {
     set_dims(c, R);
+    c->has_grid = true;
     return GM_OK;
   }
   free_grid(c);
@@ -276,26 +320,27 @@ int alloc_grid(gm_ctx* c, int R) {
   c->W = c->N / 32;
   size_t rowBytes = (size_t)c->N * c->N * c->W * 4;
   size_t flagBytes = (size_t)R * R * c->W * 2;
-  HIPCALL(c, hipMalloc(&c->I, rowBytes));
-  HIPCALL(c, hipMalloc(&c->D, rowBytes));
-  HIPCALL(c, hipMalloc(&c->K, rowBytes));
-  HIPCALL(c, hipMalloc(&c->C, rowBytes));
-  HIPCALL(c, hipMalloc(&c->mark, rowBytes));
-  HIPCALL(c, hipMalloc(&c->shell, rowBytes));
-  HIPCALL(c, hipMalloc(&c->DF, flagBytes));
-  HIPCALL(c, hipMalloc(&c->TG, flagBytes));
-  HIPCALL(c, hipMalloc(&c->CL, flagBytes));
-  HIPCALL(c, hipMalloc(&c->UG, flagBytes));
-  HIPCALL(c, hipMalloc(&c->IFL, flagBytes));
-  HIPCALL(c, hipMalloc(&c->DAL, flagBytes));
+  HIPCALL(c, calloc_dev(c, &c->I, rowBytes));
+  HIPCALL(c, calloc_dev(c, &c->D, rowBytes));
+  HIPCALL(c, calloc_dev(c, &c->K, rowBytes));
+  HIPCALL(c, calloc_dev(c, &c->C, rowBytes));
+  HIPCALL(c, calloc_dev(c, &c->mark, rowBytes));
+  HIPCALL(c, calloc_dev(c, &c->shell, rowBytes));
+  c->voxDirty = true;
+  HIPCALL(c, calloc_dev(c, &c->DF, flagBytes));
+  HIPCALL(c, calloc_dev(c, &c->TG, flagBytes));
+  HIPCALL(c, calloc_dev(c, &c->CL, flagBytes));
+  HIPCALL(c, calloc_dev(c, &c->UG, flagBytes));
+  HIPCALL(c, calloc_dev(c, &c->IFL, flagBytes));
+  HIPCALL(c, calloc_dev(c, &c->DAL, flagBytes));
   size_t brickBytes = (size_t)(c->N / 8) * (c->N / 8) * c->W;
-  HIPCALL(c, hipMalloc(&c->brickI, brickBytes));
-  HIPCALL(c, hipMalloc(&c->brickK, brickBytes));
-  HIPCALL(c, hipMalloc(&c->queue, tile_queue_words(R) * 4));
+  HIPCALL(c, calloc_dev(c, &c->brickI, brickBytes));
+  HIPCALL(c, calloc_dev(c, &c->brickK, brickBytes));
+  HIPCALL(c, calloc_dev(c, &c->queue, tile_queue_words(R) * 4));
   // both counter blocks start at zero (each pass clears the other one)
   HIPCALL(c, hipMemsetAsync(c->queue, 0, kQueueLists * 4, c->stream));
   c->qsel = 0;
-  HIPCALL(c, hipMalloc(&c->stage, (size_t)R * R * R));
+  HIPCALL(c, calloc_dev(c, &c->stage, (size_t)R * R * R));
   HIPCALL(c, hipMemsetAsync(c->I, 0, rowBytes, c->stream));
   HIPCALL(c, hipMemsetAsync(c->D, 0, rowBytes, c->stream));
   HIPCALL(c, hipMemsetAsync(c->K, 0, rowBytes, c->stream));
@@ -305,8 +350,8 @@ int alloc_grid(gm_ctx* c, int R) {
   c->Pmc = P;
   c->mc.P = P;
   c->mc.nblocks = (P / 8) * (P / 8) * (P / 8);
-  HIPCALL(c, hipMalloc(&c->mc.blockV, (size_t)c->mc.nblocks * 4));
-  HIPCALL(c, hipMalloc(&c->mc.blockT, (size_t)c->mc.nblocks * 4));
+  HIPCALL(c, calloc_dev(c, &c->mc.blockV, (size_t)c->mc.nblocks * 4));
+  HIPCALL(c, calloc_dev(c, &c->mc.blockT, (size_t)c->mc.nblocks * 4));
   c->mc.capList = 0;
   {
     // a closed surface lists on the order of 6 (P/8)^2 blocks; more grow the
@@ -315,12 +360,12 @@ int alloc_grid(gm_ctx* c, int R) {
     int rcl = alloc_mc_list(c, (u32)std::min<size_t>((size_t)c->mc.nblocks, std::max<size_t>(64, 8 * B * B)));
     if (rcl) return rcl;
   }
-  HIPCALL(c, hipMalloc(&c->mc.posOf, (size_t)c->mc.nblocks * 4));
-  HIPCALL(c, hipMalloc(&c->mc.order, ((size_t)4 * c->mc.nblocks + 4) * 4));
-  HIPCALL(c, hipMalloc(&c->mc.totals, 32));  // nV, nT, overflow, border, late cubes, spare
-  HIPCALL(c, hipMalloc(&c->mc.scanPart, mc_scan_part_bytes(c->mc.nblocks)));
-  HIPCALL(c, hipMalloc(&c->mc.brick, brickBytes));
-  HIPCALL(c, hipMalloc(&c->mc.list, ((size_t)c->mc.nblocks + 1) * 4));
+  HIPCALL(c, calloc_dev(c, &c->mc.posOf, (size_t)c->mc.nblocks * 4));
+  HIPCALL(c, calloc_dev(c, &c->mc.order, ((size_t)4 * c->mc.nblocks + 4) * 4));
+  HIPCALL(c, calloc_dev(c, &c->mc.totals, 32));  // nV, nT, overflow, border, late cubes, spare
+  HIPCALL(c, calloc_dev(c, &c->mc.scanPart, mc_scan_part_bytes(c->mc.nblocks)));
+  HIPCALL(c, calloc_dev(c, &c->mc.brick, brickBytes));
+  HIPCALL(c, calloc_dev(c, &c->mc.list, ((size_t)c->mc.nblocks + 1) * 4));
   c->mc.capV = c->mc.capT = c->mc.capLate = 0;
   int rc = alloc_mesh_out(c, (u32)std::min<size_t>((size_t)4 * c->N * c->N, 1u << 30),
                           (u32)std::min<size_t>((size_t)8 * c->N * c->N, 1u << 30), 4096);
@@ -393,8 +438,8 @@ int ensure_tables(gm_ctx* c, int rlo, int rhi) {
   c->tabs = t;
   size_t bytes = all.size() * 4;
   if (bytes > c->tablesCap) {
-    dfree(c->dTables);
-    HIPCALL(c, hipMalloc(&c->dTables, bytes));
+    cfree(c, c->dTables);
+    HIPCALL(c, calloc_dev(c, &c->dTables, bytes));
     c->tablesCap = bytes;
   }
   HIPCALL(c, hipMemcpyAsync(c->dTables, all.data(), bytes, hipMemcpyHostToDevice, c->stream));
@@ -423,9 +468,9 @@ int cube_tables(gm_ctx* c, int rlo, int rhi) {
   }
   const size_t bytes = all.size() * 4;
   if (bytes > c->cubeCap) {
-    dfree(c->dCube);
+    cfree(c, c->dCube);
     c->cubeCap = 0;
-    HIPCALL(c, hipMalloc(&c->dCube, bytes));
+    HIPCALL(c, calloc_dev(c, &c->dCube, bytes));
     c->cubeCap = bytes;
   }
   HIPCALL(c, hipMemcpyAsync(c->dCube, all.data(), bytes, hipMemcpyHostToDevice, c->stream));
@@ -495,14 +540,14 @@ int check_radius(gm_ctx* c) {
 // largest size used, like the grid buffers).
 int ensure_var_buffers(gm_ctx* c) {
   if (c->rmode == 0 || c->capVarR >= c->R) return GM_OK;
-  dfree(c->rp); dfree(c->rvset); dfree(c->kv); dfree(c->kbl);
+  cfree(c, c->rp); cfree(c, c->rvset); cfree(c, c->kv); cfree(c, c->kbl);
   c->capVarR = 0;
   c->rpValid = false;
   const int R = c->R;
-  HIPCALL(c, hipMalloc(&c->rp, radius_plane_words(R) * 4));
-  HIPCALL(c, hipMalloc(&c->rvset, brick_words(R) * 4));
-  HIPCALL(c, hipMalloc(&c->kv, (size_t)R * R * R / 4 * 4));
-  HIPCALL(c, hipMalloc(&c->kbl, (brick_words(R) + 1) * 4));
+  HIPCALL(c, calloc_dev(c, &c->rp, radius_plane_words(R) * 4));
+  HIPCALL(c, calloc_dev(c, &c->rvset, brick_words(R) * 4));
+  HIPCALL(c, calloc_dev(c, &c->kv, (size_t)R * R * R / 4 * 4));
+  HIPCALL(c, calloc_dev(c, &c->kbl, (brick_words(R) + 1) * 4));
   c->capVarR = R;
   touch(c);
   return GM_OK;
@@ -512,12 +557,12 @@ int ensure_var_buffers(gm_ctx* c) {
 int ensure_scale_grid(gm_ctx* c) {
   const size_t n = (size_t)c->N * c->N * c->N;
   if (n > c->capGrid) {
-    dfree(c->dGrid);
+    cfree(c, c->dGrid);
     c->capGrid = 0;
-    HIPCALL(c, hipMalloc(&c->dGrid, n));
+    HIPCALL(c, calloc_dev(c, &c->dGrid, n));
     c->capGrid = n;
   }
-  if (!c->dHist) HIPCALL(c, hipMalloc(&c->dHist, sizeof(c->hHist)));
+  if (!c->dHist) HIPCALL(c, calloc_dev(c, &c->dHist, sizeof(c->hHist)));
   return GM_OK;
 }
 int upload_hist(gm_ctx* c) {
@@ -628,17 +673,17 @@ int gm_destroy(gm_ctx* c) {
   drop_graph(c->gClose[1]);
   drop_graph(c->gMesh);
   free_grid(c);
-  dfree(c->dP);
-  dfree(c->dT);
-  dfree(c->dSnap);
-  dfree(c->dCheck);
+  cfree(c, c->dP);
+  cfree(c, c->dT);
+  cfree(c, c->dSnap);
+  cfree(c, c->dCheck);
   if (c->hStage) hipHostFree(c->hStage);
-  dfree(c->dLat);
-  dfree(c->dTables);
-  dfree(c->dCube);
-  dfree(c->bilScratch);
-  dfree(c->bilN);
-  dfree(c->dHist);
+  cfree(c, c->dLat);
+  cfree(c, c->dTables);
+  cfree(c, c->dCube);
+  cfree(c, c->bilScratch);
+  cfree(c, c->bilN);
+  cfree(c, c->dHist);
   for (auto& e : c->ev) hipEventDestroy(e);
   if (c->hTotals) hipHostFree(c->hTotals);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -654,20 +699,20 @@ int gm_load_mesh(gm_ctx* c, const float* P, const float* N, int nv, const uint32
   if (!P || !T || nv <= 0 || nf <= 0) return fail(c, GM_ERR_ARG, "empty mesh");
   hipSetDevice(c->device);
   if ((size_t)nv > c->capP) {
-    dfree(c->dP);
-    dfree(c->dSnap);
+    cfree(c, c->dP);
+    cfree(c, c->dSnap);
     c->capP = 0;
-    HIPCALL(c, hipMalloc(&c->dP, (size_t)nv * 12));
-    HIPCALL(c, hipMalloc(&c->dSnap, (size_t)nv * 12));
+    HIPCALL(c, calloc_dev(c, &c->dP, (size_t)nv * 12));
+    HIPCALL(c, calloc_dev(c, &c->dSnap, (size_t)nv * 12));
     c->capP = nv;
   }
   if ((size_t)nf > c->capTri) {
-    dfree(c->dT);
+    cfree(c, c->dT);
     c->capTri = 0;
-    HIPCALL(c, hipMalloc(&c->dT, (size_t)nf * 12));
+    HIPCALL(c, calloc_dev(c, &c->dT, (size_t)nf * 12));
     c->capTri = nf;
   }
-  if (!c->dCheck) HIPCALL(c, hipMalloc(&c->dCheck, mesh_check_words() * 4));
+  if (!c->dCheck) HIPCALL(c, calloc_dev(c, &c->dCheck, mesh_check_words() * 4));
   // through pinned staging: one host copy, then asynchronous DMA
   const size_t bytes = (size_t)nv * 12 + (size_t)nf * 12;
   if (bytes > c->capStage) {
@@ -732,9 +777,14 @@ int gm_voxelize(gm_ctx* c, int base_res, float margin) {
   c->cell = cs;
   float maxext = std::max(std::fabs(mx[0] - mn[0]), std::max(std::fabs(mx[1] - mn[1]), std::fabs(mx[2] - mn[2])));
   double scale = (double)c->N / (double)maxext;
+  // the fill leaves the marks and the shell zero; they are cleared here only
+  // after allocation or a voxelization that did not complete its launches
+  const bool clear = c->voxDirty;
+  c->voxDirty = true;
   hipError_t e = launch_voxelize(c->dP, c->nv, c->dT, c->nf, c->bmin, scale, c->R, c->dSnap, c->mark, c->shell, c->I,
-                                 c->stream);
+                                 c->stream, clear);
   if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("voxelize: ") + hipGetErrorString(e));
+  c->voxDirty = false;
   HIPCALL(c, hipStreamSynchronize(c->stream));
   c->meshPending = false;
   return GM_OK;
@@ -952,8 +1002,8 @@ int gm_scale_lattice_f32(gm_ctx* c, const float* lat, int L) {
     mx = std::max<long long>(mx, q[i]);
   }
   if (c->L != L) {
-    dfree(c->dLat);
-    HIPCALL(c, hipMalloc(&c->dLat, n * 4));
+    cfree(c, c->dLat);
+    HIPCALL(c, calloc_dev(c, &c->dLat, n * 4));
   }
   HIPCALL(c, hipMemcpyAsync(c->dLat, q.data(), n * 4, hipMemcpyHostToDevice, c->stream));
   HIPCALL(c, hipStreamSynchronize(c->stream));
@@ -1056,7 +1106,7 @@ int gm_mesh_counts(gm_ctx* c, uint32_t* nv, uint32_t* nt) {
     u32 listed = 0;
     HIPCALL(c, hipMemcpyAsync(&listed, c->mc.list, 4, hipMemcpyDeviceToHost, c->stream));
     HIPCALL(c, hipStreamSynchronize(c->stream));
-    rc = alloc_mc_list(c, std::min<u32>((u32)c->mc.nblocks, listed + listed / 8 + 64));
+    rc = alloc_mc_list(c, std::min<u32>((u32)c->mc.nblocks, grow_cap(c->mc.capList, listed)));
     if (rc) return rc;
     hipError_t e = launch_mc(c->C, c->R, c->bmin, c->cell, c->mc, c->stream, nullptr);
     if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("mesh extraction: ") + hipGetErrorString(e));
@@ -1069,9 +1119,9 @@ int gm_mesh_counts(gm_ctx* c, uint32_t* nv, uint32_t* nt) {
     u32 lateN = 0;
     HIPCALL(c, hipMemcpyAsync(&lateN, c->mc.late, 4, hipMemcpyDeviceToHost, c->stream));
     HIPCALL(c, hipStreamSynchronize(c->stream));
-    rc = alloc_mesh_out(c, std::max(c->mc.capV, c->hTotals[0] + c->hTotals[0] / 8 + 1024),
-                        std::max(c->mc.capT, c->hTotals[1] + c->hTotals[1] / 8 + 1024),
-                        std::max(c->mc.capLate, lateN + 1024));
+    rc = alloc_mesh_out(c, std::max(c->mc.capV, grow_cap(c->mc.capV, c->hTotals[0])),
+                        std::max(c->mc.capT, grow_cap(c->mc.capT, c->hTotals[1])),
+                        std::max(c->mc.capLate, grow_cap(c->mc.capLate, lateN)));
     if (rc) return rc;
     u32 tot[4] = {c->hTotals[0], c->hTotals[1], 0u, 0u};
     HIPCALL(c, hipMemcpyAsync(c->mc.totals, tot, 16, hipMemcpyHostToDevice, c->stream));
@@ -1110,15 +1160,15 @@ int gm_filter_normals(gm_ctx* c, int iters) {
   if (iters == 0 || nv == 0) return GM_OK;
   const size_t words = bilateral_scratch_words(nv, nt);
   if (words > c->bilWords) {
-    dfree(c->bilScratch);
+    cfree(c, c->bilScratch);
     c->bilWords = 0;
-    HIPCALL(c, hipMalloc(&c->bilScratch, words * 4));
+    HIPCALL(c, calloc_dev(c, &c->bilScratch, words * 4));
     c->bilWords = words;
   }
   if (nv > c->bilNCap) {
-    dfree(c->bilN);
+    cfree(c, c->bilN);
     c->bilNCap = 0;
-    HIPCALL(c, hipMalloc(&c->bilN, (size_t)nv * 12));
+    HIPCALL(c, calloc_dev(c, &c->bilN, (size_t)nv * 12));
     c->bilNCap = nv;
   }
   hipError_t e = launch_bilateral(c->mc.V, c->mc.Nrm, c->mc.T, nv, nt, iters, c->bilScratch, c->bilN, c->stream);
@@ -1141,9 +1191,9 @@ int gm_build_pyramid(gm_ctx* c, int which) {
     tex += (uint64_t)s * s * s;
   }
   if (tex > c->capPyr) {
-    dfree(c->dPyr);
+    cfree(c, c->dPyr);
     c->capPyr = 0;
-    HIPCALL(c, hipMalloc(&c->dPyr, tex * 8));
+    HIPCALL(c, calloc_dev(c, &c->dPyr, tex * 8));
     c->capPyr = tex;
   }
   HIPCALL(c, launch_pyramid(rows, c->R, radius_field(c), c->dPyr, c->stream));
@@ -1169,9 +1219,24 @@ int gm_pyramid_get(gm_ctx* c, uint64_t* out) {
   return GM_OK;
 }
 
-int gm_set_graphs(gm_ctx* c, int enable) {
+uint64_t gm_ctx_device_bytes(const gm_ctx* c) { return c ? c->devBytes : 0; }
+
+// Buffers for grids up to base resolution R, allocated now (a job engine
+// sizes a context once for the largest job it will batch, so that jobs of
+// mixed sizes never reallocate: every hipFree waits for the whole device).
+// The context holds no grid afterwards.
+int gm_ctx_reserve(gm_ctx* c, int R) {
   if (!c) return GM_ERR_ARG;
-  c->graphs = enable != 0;
+  if (R <= c->capR) return GM_OK;
+  hipSetDevice(c->device);
+  const int rc = alloc_grid(c, R);
+  c->has_grid = false;
+  return rc;
+}
+
+int gm_set_graphs(gm_ctx* c, int enable) {
+  if (!c || enable < 0 || enable > 2) return GM_ERR_ARG;
+  c->graphs = enable;
   return GM_OK;
 }
 
@@ -1322,8 +1387,9 @@ struct gm_batch {
   // (the contexts, their launch-argument generations, the element, the
   // queue toggle): replayed when the same batch is closed again
   hipGraphExec_t exec = nullptr;
-  std::vector<uint64_t> key;
-  bool graphs = true;
+  std::vector<uint64_t> key, seen;
+  int graphs = 1;  // as gm_ctx::graphs
+  SideStreams side;
 };
 
 namespace {
@@ -1382,6 +1448,16 @@ int gm_batch_create(int device, gm_batch** out) {
     delete b;
     return GM_ERR_HIP;
   }
+  bool ok = hipEventCreateWithFlags(&b->side.fork, hipEventDisableTiming) == hipSuccess;
+  for (int k = 0; k < 3 && ok; ++k) {
+    ok = hipStreamCreateWithFlags(&b->side.s[k], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&b->side.join[k], hipEventDisableTiming) == hipSuccess;
+    if (ok) b->side.n = k + 1;
+  }
+  if (!ok) {
+    gm_batch_destroy(b);
+    return GM_ERR_HIP;
+  }
   *out = b;
   return GM_OK;
 }
@@ -1397,6 +1473,14 @@ int gm_batch_destroy(gm_batch* b) {
   dfree(b->dMc);
   dfree(b->queue);
   if (b->exec) hipGraphExecDestroy(b->exec);
+  for (int k = 0; k < 3; ++k) {
+    if (b->side.s[k]) {
+      hipStreamSynchronize(b->side.s[k]);
+      hipStreamDestroy(b->side.s[k]);
+    }
+    if (b->side.join[k]) hipEventDestroy(b->side.join[k]);
+  }
+  if (b->side.fork) hipEventDestroy(b->side.fork);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
   return GM_OK;
@@ -1405,8 +1489,8 @@ int gm_batch_destroy(gm_batch* b) {
 const char* gm_batch_last_error(const gm_batch* b) { return b ? b->err.c_str() : ""; }
 
 int gm_batch_set_graphs(gm_batch* b, int enable) {
-  if (!b) return GM_ERR_ARG;
-  b->graphs = enable != 0;
+  if (!b || enable < 0 || enable > 2) return GM_ERR_ARG;
+  b->graphs = enable;
   return GM_OK;
 }
 
@@ -1481,13 +1565,23 @@ int gm_batch_close_mesh(gm_batch* b, gm_ctx* const* ctxs, int n, int se) {
       key.push_back(c->graphGen);
     }
     auto enqueue = [&]() -> int {
-      hipError_t e = launch_close_batched(mj.data(), m, b->hMorph, b->dMorph, b->queue, (int)cap, &b->qsel, b->stream);
+      hipError_t e =
+          launch_close_batched(mj.data(), m, b->hMorph, b->dMorph, b->queue, (int)cap, &b->qsel, b->stream, b->side);
       if (e != hipSuccess) return bfail(b, GM_ERR_HIP, std::string("batched closing: ") + hipGetErrorString(e));
       e = launch_mc_batched(cj.data(), m, b->hMc, b->dMc, b->hTot, b->stream);
       if (e != hipSuccess) return bfail(b, GM_ERR_HIP, std::string("batched mesh extraction: ") + hipGetErrorString(e));
       return GM_OK;
     };
-    if (!b->graphs) {
+    const bool firstSeen = b->graphs == 2 && !(b->exec && key == b->key) && key != b->seen;
+    if (firstSeen) {
+      // the eager launches below rewrite the pinned tables a captured batch
+      // would copy from: that capture is stale from here on
+      b->seen = key;
+      if (b->exec) hipGraphExecDestroy(b->exec);
+      b->exec = nullptr;
+      b->key.clear();
+    }
+    if (!b->graphs || firstSeen) {
       int rc2 = enqueue();
       if (rc2) return rc2;
     } else if (b->exec && key == b->key) {

@@ -128,7 +128,7 @@ inline float mesh_key_float(u32 k) {
   return f;
 }
 hipError_t launch_voxelize(const float* P, int nv, const u32* T, int nt, const float* bmin3, double scale,
-                           int R, int32_t* snap, u32* mark, u32* shell, u32* rows, hipStream_t s);
+                           int R, int32_t* snap, u32* mark, u32* shell, u32* rows, hipStream_t s, bool clear);
 
 // ---- morphology (gm_morph.hip)
 struct MorphBuffers {
@@ -227,8 +227,18 @@ struct BrickJob {
 hipError_t launch_brick_summary_batched(const BrickJob* dJobs, const u32* dPre, int n, u32 blocks, int wmax,
                                         hipStream_t s);
 size_t batch_queue_cap(const BatchMorphJob* jobs, int n);
+// Side streams of the batched closing: the per-radius launches of a pass are
+// independent (each job has one radius and its own buffers), so they are
+// spread over s and these streams, forked from and joined back into s with
+// the events (which keeps the sequence capturable as a graph).
+struct SideStreams {
+  int n = 0;
+  hipStream_t s[3] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[3] = {};
+};
 hipError_t launch_close_batched(const BatchMorphJob* jobs, int n, void* hBlob, void* dBlob, u32* queue, int cap,
-                                int* qsel, hipStream_t s);
+                                int* qsel, hipStream_t s, const SideStreams& side);
 // Radius planes and per-brick value sets of a per-voxel radius field
 // (values clamped to [2,31]; the caller has checked the range). brickBox
 // (optional): {w0, by0, bz0, nw, nby, nbz}, the bricks (1 word x 8 rows x 8

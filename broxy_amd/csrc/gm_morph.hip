@@ -861,7 +861,7 @@ size_t batch_queue_cap(const BatchMorphJob* jobs, int n) {
 }
 
 hipError_t launch_close_batched(const BatchMorphJob* jobs, int n, void* hBlob, void* dBlob, u32* queue, int cap,
-                                int* qsel, hipStream_t s) {
+                                int* qsel, hipStream_t s, const SideStreams& side) {
   if (n <= 0) return hipSuccess;
   MorphArgs* hT = static_cast<MorphArgs*>(hBlob);
   u32* hP = reinterpret_cast<u32*>(hT + 7 * n);
@@ -935,8 +935,19 @@ hipError_t launch_close_batched(const BatchMorphJob* jobs, int n, void* hBlob, v
     int nn = n, qq = q, lb = kQueueLists, cp = cap;
     void* sargs[] = {&tab, &pre, &nn, &dR, &queue, &qq, &lb, &cp};
     GM_CHECK(hipLaunchKernel(sel[p], dim3((units + 3) / 4), dim3(256), sargs, 0, s));
+    // a launch per radius present, round robin over s and the side streams
+    int nr = 0;
+    for (int r = 2; r <= kCtMaxRho - 1; ++r) nr += present[r] ? 1 : 0;
+    const int ns = std::min(nr - 1, side.n);
+    if (ns > 0) {
+      GM_CHECK(hipEventRecord(side.fork, s));
+      for (int k = 0; k < ns; ++k) GM_CHECK(hipStreamWaitEvent(side.s[k], side.fork, 0));
+    }
+    int slot = 0;
     for (int r = 2; r <= kCtMaxRho - 1; ++r) {
       if (!present[r]) continue;
+      const hipStream_t sr = slot == 0 ? s : side.s[slot - 1];
+      slot = slot == ns ? 0 : slot + 1;
       const int rho = p < 2 ? r + 1 : r - 1;
       const void* fn = ct.fnB[p][rho];
       if (!fn) return hipErrorInvalidValue;
@@ -953,7 +964,11 @@ hipError_t launch_close_batched(const BatchMorphJob* jobs, int n, void* hBlob, v
       int qc = kVarCounters + 256 * q + 8 * r, ql = kQueueLists + 8 * r * cap, qcap = cap;
       const u32* cq = queue;
       void* args[] = {&tab, &cq, &qc, &ql, &qcap};
-      GM_CHECK(hipLaunchKernel(fn, dim3(grid), dim3(64 * nw), args, lds, s));
+      GM_CHECK(hipLaunchKernel(fn, dim3(grid), dim3(64 * nw), args, lds, sr));
+    }
+    for (int k = 0; k < ns; ++k) {
+      GM_CHECK(hipEventRecord(side.join[k], side.s[k]));
+      GM_CHECK(hipStreamWaitEvent(s, side.join[k], 0));
     }
     return hipGetLastError();
   };

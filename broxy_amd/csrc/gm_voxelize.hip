@@ -24,9 +24,26 @@ namespace {
 
 typedef long long i64;
 
-__device__ __forceinline__ i64 fdiv(i64 a, i64 b) {  // floor(a/b), b > 0
-  i64 q = a / b;
-  if ((a % b != 0) && (a < 0)) --q;
+// floor(a / 256): an arithmetic shift (the snap grid has 256 steps per voxel).
+__device__ __forceinline__ i64 fdiv256(i64 a) { return a >> 8; }
+
+// floor(a / b) for b > 0, with rb = 1 / b in double: the rounded quotient is
+// corrected against the exact remainder, so the result is the integer floor.
+// (A 64-bit integer division is a long software sequence on the GPU; here it
+// ran once per covered pixel.) The quotients are voxel depths (|q| < 2^13)
+// and |a| < 2^62, so the double estimate is off by less than one and the
+// loops run at most once.
+__device__ __forceinline__ i64 fdiv_r(i64 a, i64 b, double rb) {
+  i64 q = (i64)floor((double)a * rb);
+  i64 r = a - q * b;
+  while (r < 0) {
+    --q;
+    r += b;
+  }
+  while (r >= b) {
+    ++q;
+    r -= b;
+  }
   return q;
 }
 
@@ -77,8 +94,10 @@ __global__ void k_vox_tris(const int32_t* __restrict__ X, const u32* __restrict_
       }
       i64 xmn = min(a0, min(b0, c0)), xmx = max(a0, max(b0, c0));
       i64 ymn = min(a1, min(b1, c1)), ymx = max(a1, max(b1, c1));
-      i64 i0 = max((i64)0, fdiv(xmn - S / 2 + S - 1, S)), i1 = min((i64)N - 1, fdiv(xmx - S / 2, S));
-      i64 j0 = max((i64)0, fdiv(ymn - S / 2 + S - 1, S)), j1 = min((i64)N - 1, fdiv(ymx - S / 2, S));
+      i64 i0 = max((i64)0, fdiv256(xmn - S / 2 + S - 1)), i1 = min((i64)N - 1, fdiv256(xmx - S / 2));
+      i64 j0 = max((i64)0, fdiv256(ymn - S / 2 + S - 1)), j1 = min((i64)N - 1, fdiv256(ymx - S / 2));
+      const i64 dz = area * S;
+      const double rdz = 1.0 / (double)dz;
       const i64 vx[3] = {a0, b0, c0}, vy[3] = {a1, b1, c1}, vz[3] = {a2, b2, c2};
       for (i64 j = j0; j <= j1; ++j)
         for (i64 i = i0; i <= i1; ++i) {
@@ -96,7 +115,7 @@ __global__ void k_vox_tris(const int32_t* __restrict__ X, const u32* __restrict_
           }
           if (!in) continue;
           i64 num = w[0] * vz[0] + w[1] * vz[1] + w[2] * vz[2];
-          i64 zc = fdiv(num, area * S);
+          i64 zc = fdiv_r(num, dz, rdz);
           if (zc < 0) continue;
           if (zc > N - 1) zc = N - 1;
           toggle(mark, N, W, (int)i, (int)j, (int)zc);
@@ -132,10 +151,13 @@ __global__ void k_vox_tris(const int32_t* __restrict__ X, const u32* __restrict_
     }
     i64 umn = min(U[0], min(U[1], U[2])), umx = max(U[0], max(U[1], U[2]));
     i64 vmn = min(Vv[0], min(Vv[1], Vv[2])), vmx = max(Vv[0], max(Vv[1], Vv[2]));
-    i64 iu0 = max((i64)0, fdiv(umn - 1, S)), iu1 = min((i64)N - 1, fdiv(umx, S));
-    i64 iv0 = max((i64)0, fdiv(vmn - 1, S)), iv1 = min((i64)N - 1, fdiv(vmx, S));
+    i64 iu0 = max((i64)0, fdiv256(umn - 1)), iu1 = min((i64)N - 1, fdiv256(umx));
+    i64 iv0 = max((i64)0, fdiv256(vmn - 1)), iv1 = min((i64)N - 1, fdiv256(vmx));
     i64 nd = n[ax], nu = n[ua], nvv = n[va];
     i64 Ad = A[ax], Au = A[ua], Av = A[va];
+    // depth q = floor(num / (nd S)), taken with a positive divisor
+    const i64 sgn = nd > 0 ? 1 : -1, dq = sgn * nd * S;
+    const double rdq = 1.0 / (double)dq;
     for (i64 iv = iv0; iv <= iv1; ++iv)
       for (i64 iu = iu0; iu <= iu1; ++iu) {
         i64 ulo = iu * S, uhi = ulo + S, vlo = iv * S, vhi = vlo + S;
@@ -152,7 +174,7 @@ __global__ void k_vox_tris(const int32_t* __restrict__ X, const u32* __restrict_
         if (!ok) continue;
         i64 pu = ulo + S / 2, pv = vlo + S / 2;
         i64 num = Ad * nd - (nu * (pu - Au) + nvv * (pv - Av));
-        i64 q = nd > 0 ? fdiv(num, nd * S) : fdiv(-num, -nd * S);
+        i64 q = fdiv_r(sgn * num, dq, rdq);
         if (q < 0 || q >= N) continue;
         int c3[3];
         c3[ua] = (int)iu;
@@ -167,10 +189,12 @@ __global__ void k_vox_tris(const int32_t* __restrict__ X, const u32* __restrict_
 // A workgroup owns 64 consecutive columns (one per lane, coalesced) and
 // splits z into kFillChunks chunks, one wave each: each wave XORs its chunk,
 // the chunk parities give every chunk the XOR of the chunks above it (LDS),
-// then each wave walks its chunk down from that value.
+// then each wave walks its chunk down from that value. The walk clears the
+// marks and the shell behind it, so they are zero for the next voxelization
+// without a separate clearing pass.
 constexpr int kFillChunks = 16;
-__global__ void __launch_bounds__(64 * kFillChunks) k_vox_fill(const u32* __restrict__ mark,
-                                                               const u32* __restrict__ shell, u32* __restrict__ rows,
+__global__ void __launch_bounds__(64 * kFillChunks) k_vox_fill(u32* __restrict__ mark,
+                                                               u32* __restrict__ shell, u32* __restrict__ rows,
                                                                int N) {
   __shared__ u32 par[kFillChunks][64];
   const int W = N / 32;
@@ -191,6 +215,8 @@ __global__ void __launch_bounds__(64 * kFillChunks) k_vox_fill(const u32* __rest
     const size_t i = (size_t)z * plane + col;
     acc ^= mark[i];
     rows[i] = acc | shell[i];
+    mark[i] = 0u;
+    shell[i] = 0u;
   }
 }
 
@@ -295,11 +321,13 @@ hipError_t launch_mesh_check(const float* P, int nv, const u32* T, int nt, u32* 
 }
 
 hipError_t launch_voxelize(const float* P, int nv, const u32* T, int nt, const float* bmin3, double scale,
-                           int R, int32_t* snap, u32* mark, u32* shell, u32* rows, hipStream_t s) {
+                           int R, int32_t* snap, u32* mark, u32* shell, u32* rows, hipStream_t s, bool clear) {
   const int N = 2 * R, W = N / 32;
   size_t words = (size_t)N * N * W;
-  GM_CHECK(hipMemsetAsync(mark, 0, words * 4, s));
-  GM_CHECK(hipMemsetAsync(shell, 0, words * 4, s));
+  if (clear) {
+    GM_CHECK(hipMemsetAsync(mark, 0, words * 4, s));
+    GM_CHECK(hipMemsetAsync(shell, 0, words * 4, s));
+  }
   if (nv > 0)
     hipLaunchKernelGGL(k_snap, dim3((nv + 255) / 256), dim3(256), 0, s, P, nv, bmin3[0], bmin3[1], bmin3[2],
                        scale, snap);

@@ -173,10 +173,14 @@ int gm_tune_load(const char* path);
 int gm_set_timing(gm_ctx* ctx, int enable);
 /* gm_close and gm_extract_mesh replay their launch sequence as a captured
  * HIP graph once it was captured for the context's current buffers, sizes,
- * radius field and grid attributes (on by default; off runs every launch
- * eagerly; per-stage timing also runs eagerly). No counterpart in the
- * reference, which launches every kernel from the host. */
-int gm_set_graphs(gm_ctx* ctx, int enable);
+ * radius field and grid attributes. mode 1 (the default) captures on the
+ * first call of a sequence; mode 2 runs a sequence eagerly the first time
+ * and captures it when it comes again unchanged (for contexts that run a
+ * different job every call, where a capture would never be replayed);
+ * mode 0 runs every launch eagerly. Per-stage timing also runs eagerly.
+ * No counterpart in the reference, which launches every kernel from the
+ * host. */
+int gm_set_graphs(gm_ctx* ctx, int mode);
 int gm_stage_times(gm_ctx* ctx, float* ms, int n);
 /* Wait for all work enqueued on the context. */
 int gm_sync(gm_ctx* ctx);
@@ -203,9 +207,9 @@ int gm_batch_destroy(gm_batch* b);
 const char* gm_batch_last_error(const gm_batch* b);
 int gm_batch_close_mesh(gm_batch* b, gm_ctx* const* ctxs, int n, int se);
 /* The batched launches are captured as a HIP graph and replayed when the same
- * contexts (with unchanged buffers, sizes and radii) are closed again (on by
- * default). */
-int gm_batch_set_graphs(gm_batch* b, int enable);
+ * contexts (with unchanged buffers, sizes and radii) are closed again; the
+ * modes are those of gm_set_graphs (1 by default). */
+int gm_batch_set_graphs(gm_batch* b, int mode);
 
 /* ---- Allocation-free closing. No counterpart in the reference; for callers
  * that cannot allocate device memory: code under HIP graph capture, serving
@@ -242,7 +246,7 @@ const char* gm_noalloc_last_error(void);
  * A job is admitted when a slot is free and the device memory the slot would
  * then hold fits hbm_budget_bytes; smaller jobs may pass one that does not
  * fit yet. Slots keep their largest buffers. Every job's mesh is identical to
- * running it alone. Inputs are copied at submit. */
+ * running it alone. */
 typedef struct gm_engine gm_engine;
 typedef struct {
   const float* P;          /* float[3*num_of_vertices] */
@@ -260,15 +264,29 @@ int gm_engine_create(int device, int slots, uint64_t hbm_budget_bytes, gm_engine
 /* Running jobs finish; queued ones are dropped. */
 int gm_engine_destroy(gm_engine* e);
 const char* gm_engine_last_error(const gm_engine* e);
-/* Queue a job; returns at once with its id. */
+/* Queue a job; returns at once with its id. The inputs are copied. */
 int gm_engine_submit(gm_engine* e, const gm_job* job, uint64_t* id);
+/* Queue a job without copying its inputs: P, T and lattice must stay valid
+ * and unchanged until gm_engine_wait, gm_engine_view or gm_engine_take
+ * returns for the job (or gm_engine_destroy does). */
+int gm_engine_submit_borrowed(gm_engine* e, const gm_job* job, uint64_t* id);
 /* Wait for a job; returns its status and mesh sizes. */
 int gm_engine_wait(gm_engine* e, uint64_t id, uint32_t* num_vertices, uint32_t* num_triangles);
 /* Copy a finished job's mesh out (V, Nrm float[3*nv], T uint32[3*nt]) and
  * release the job. */
 int gm_engine_take(gm_engine* e, uint64_t id, float* V, float* Nrm, uint32_t* T);
+/* Wait for a job and point at its mesh where the engine holds it (pinned
+ * host memory the device copied it into, so no host copy is made); the
+ * arrays stay valid until gm_engine_release(e, id). */
+int gm_engine_view(gm_engine* e, uint64_t id, const float** V, const float** Nrm, const uint32_t** T,
+                   uint32_t* num_vertices, uint32_t* num_triangles);
+/* Release a job (waits for it first); its id is unknown afterwards. */
+int gm_engine_release(gm_engine* e, uint64_t id);
 /* Submit, start and completion time of a job (seconds on the engine's clock). */
 int gm_engine_job_times(gm_engine* e, uint64_t id, double t[3]);
+/* Submit, start, loaded (voxelized, radius set), meshed (counts known) and
+ * done (mesh on the host) times of a job, seconds on the engine's clock. */
+int gm_engine_job_phases(gm_engine* e, uint64_t id, double t[5]);
 /* The engine's clock now (seconds). */
 int gm_engine_now(gm_engine* e, double* t);
 /* Device bytes reserved by the slots, jobs running, jobs queued. */

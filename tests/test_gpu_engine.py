@@ -104,3 +104,40 @@ def test_context_reuse_across_sizes(ctx):
         Dref = O.dilate(I, R, scale=sc)
         assert np.array_equal(ctx.get_cells(B.GRID_DILATION), Dref)
         assert np.array_equal(ctx.get_cells(B.GRID_CLOSING), O.erode(I, Dref, R, scale=sc))
+
+
+def test_engine_borrowed_views_and_repeats(ctx):
+    """Borrowed inputs and in-place result views give the same meshes as
+    copied inputs and take(); the same job three times on one slot (whose
+    context captures launch sequences only when they repeat) gives the same
+    mesh each time; phase times are ordered."""
+    # lattice jobs need N / 8 to be a power of two
+    J = [j for j in jobs(9, 10) if j[4] is None or j[2] in (32, 64)]
+    say("engine: borrowed inputs, views, %d jobs, 3 slots" % len(J))
+    eng = B.Engine(0, slots=3, hbm_budget=8 << 30)
+    try:
+        ids = [eng.submit(P, T, br, 0.3, radius=r, lattice=lat, borrow=True) for P, T, br, r, lat in J]
+        got = []
+        for i in ids:
+            V, N, Tt = eng.view(i)
+            got.append(h(V, N, Tt))
+            t = eng.phases(i)
+            assert np.all(np.diff(t) >= 0) and t[4] > t[0]
+            eng.release(i)
+        with pytest.raises(B.GMorphoError):
+            eng.view(ids[0])  # released
+        for k, (P, T, br, r, lat) in enumerate(J):
+            assert got[k] == h(*alone(ctx, P, T, br, r, lat)), "job %d differs from its single-context run" % k
+    finally:
+        eng.close()
+    P, T, br, r, lat = J[0]
+    ref = h(*alone(ctx, P, T, br, r, lat))
+    eng = B.Engine(0, slots=1, hbm_budget=8 << 30)
+    try:
+        for rep in range(3):
+            say("engine repeat %d" % rep)
+            i = eng.submit(P, T, br, 0.3, radius=r, borrow=True)
+            assert h(*eng.view(i)) == ref
+            eng.release(i)
+    finally:
+        eng.close()

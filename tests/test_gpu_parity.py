@@ -657,6 +657,35 @@ def test_graph_replays_match_eager(ctx):
         eager.close()
 
 
+def test_graph_repeat_mode_matches_eager(ctx):
+    """Capture-on-repeat: the first closing runs eagerly, the second (same
+    launch sequence, new grid contents) is captured, the later ones replay;
+    every one equals the eager context's result."""
+    R = 48
+    grids = [random_blobs(R, seed=70 + k, nballs=8) for k in range(4)]
+    eager = B.Context(0)
+    try:
+        eager.set_graphs(False)
+        grid_setup(eager, R, grids[0])
+        grid_setup(ctx, R, grids[0])
+        ctx.set_graphs("repeat")
+        eager.scale_uniform(5)
+        ctx.scale_uniform(5)
+        for k, I in enumerate(grids):
+            say("graph repeat mode: closing %d" % k)
+            out = []
+            for c in (ctx, eager):
+                c.set_cells(B.GRID_INPUT, I)
+                c.close_grid(B.SE_SPHERE)
+                c.extract_mesh()
+                out.append((c.get_cells(B.GRID_CLOSING),) + tuple(c.mesh()))
+            for a, b in zip(*out):
+                assert np.array_equal(a, b)
+    finally:
+        ctx.set_graphs(True)
+        eager.close()
+
+
 def test_mc_block_list_grows(ctx):
     """One voxel in every 8^3-cell Morton block of a 128^3-cell grid: all
     4096 blocks are listed, twice the per-block buffers' initial capacity,

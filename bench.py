@@ -77,6 +77,10 @@ def rho_of_stage(stage, r):
     return r + 1 if stage.startswith("dilate") else r - 1
 
 
+# rocprofv3 counter summaries per (config, structuring element)
+COUNTER_FILES = {("large", "sphere"): "counters_sh512_r8.json"}
+
+
 def dominant_kernel(stage, r, se):
     """rocprof name of the kernel a stage times (cube dilations run the
     runtime-table stream kernels)."""
@@ -370,16 +374,11 @@ def main():
     ap.add_argument("--slots", type=int, default=8, help="engine slots for the served measurement")
     ap.add_argument("--hbm-budget", type=int, default=200, help="engine HBM budget in GiB")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="only run 3 device-to-device copies of MIB MiB (calibrates HBM byte counters)")
+                    help="only run 3 copies of MIB MiB by the library's copy kernel (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.copy_probe:
-        import torch
-        src = torch.ones(args.copy_probe << 20, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes": args.copy_probe << 20}), flush=True)
+        B.copy_probe(args.copy_probe << 20, reps=3)
+        print(json.dumps({"copy_probe_bytes": args.copy_probe << 20, "kernel": "k_copy_probe"}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the closing path is single-device (--gpus %d refused)" % args.gpus,
@@ -456,16 +455,30 @@ def main():
     whole_bytes = 5 * N ** 3 / 8.0 + 12 * nv + 12 * nt
     whole_gbs = whole_bytes / (ms * 1e-3) / 1e9
 
-    # HBM bytes per launch of that kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE
-    # passes (scaled by a copy of known size), committed beside the profiles
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.config)
-    if os.path.exists(tf):
-        with open(tf) as fh:
-            tr = json.load(fh)
-        ent = tr.get("stages", {}).get(dname.replace("_stream", ""))
-        if ent:
-            traffic = ent.get("bytes")
+    # HBM bytes per launch of that kernel, and its VALU / LDS use, from the
+    # rocprofv3 counter passes committed beside the profiles: bytes from the
+    # L2-to-memory request counts (128 B per read request, 64 B per write
+    # request, both checked on a copy of known size), else FETCH_SIZE /
+    # WRITE_SIZE scaled by that copy
+    traffic, counters = None, None
+    kname = dominant_kernel(dname, r, args.se).replace(" ", "")
+    cf = os.path.join(ROOT, "profiles", COUNTER_FILES.get((args.config, args.se), "-"))
+    if os.path.exists(cf):
+        with open(cf) as fh:
+            for k in json.load(fh)["kernels"]:
+                if k["kernel"].replace(" ", "") == kname:
+                    traffic = k["read_bytes_from_requests"] + k["write_bytes_from_requests"]
+                    counters = {"valu_busy_pct": k["VALUBusy_pct"], "valu_utilization_pct": k["VALUUtilization_pct"],
+                                "lds_util_pct": k["LdsUtil_pct"], "lds_bank_conflict_pct": k["LDSBankConflict_pct"],
+                                "occupancy_pct": k["Occupancy_pct"], "source": "profiles/" + os.path.basename(cf)}
+    if traffic is None:
+        tf = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.config)
+        if os.path.exists(tf):
+            with open(tf) as fh:
+                tr = json.load(fh)
+            ent = tr.get("stages", {}).get(dname.replace("_stream", ""))
+            if ent:
+                traffic = ent.get("bytes")
 
     out = {
         "metric": ("Gvoxels/s closing 512^3 r=8 (+%HBM roof); batched jobs/s vs 1-by-1; p99 lat"
@@ -490,7 +503,7 @@ def main():
                      "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes": int(dbytes),
-                     "avg_launch_ms": round(float(stage_ms[dom]), 5)},
+                     "avg_launch_ms": round(float(stage_ms[dom]), 5), "counters": counters},
         "pipeline_roofline": {"bytes": int(whole_bytes), "achieved_GBs": round(whole_gbs, 1),
                               "frac": round(whole_gbs / HBM_PEAK_GBS, 4)},
         "stage_ms": {s: round(float(v), 5) for s, v in zip(STAGES, stage_ms)},

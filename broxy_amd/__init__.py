@@ -45,6 +45,14 @@ def hip_lib():
     return _hip
 
 
+def copy_probe(nbytes, reps=3, device=0):
+    """reps device copies of nbytes by the library's copy kernel (the
+    known-size reference for HBM byte counters)."""
+    rc = hip_lib().gm_diag_copy_probe(device, ctypes.c_uint64(nbytes), reps)
+    if rc:
+        raise GMorphoError("gm_diag_copy_probe -> %d" % rc)
+
+
 def meshgen_lib():
     global _meshgen
     if _meshgen is None:

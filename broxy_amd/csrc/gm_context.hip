@@ -1429,6 +1429,23 @@ int gm_get_stream(gm_ctx* c, void** s) {
   return GM_OK;
 }
 
+int gm_diag_copy_probe(int device, uint64_t bytes, int reps) {
+  if (bytes == 0 || (bytes % 16) != 0 || reps < 1) return GM_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return GM_ERR_HIP;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, bytes) != hipSuccess) return GM_ERR_HIP;
+  if (hipMalloc(&b, bytes) != hipSuccess) {
+    hipFree(a);
+    return GM_ERR_HIP;
+  }
+  hipError_t e = hipMemset(a, 1, bytes);
+  for (int k = 0; k < reps && e == hipSuccess; ++k) e = launch_copy_probe(a, b, bytes, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  hipFree(a);
+  hipFree(b);
+  return e == hipSuccess ? GM_OK : GM_ERR_HIP;
+}
+
 int gm_mc_tables(uint32_t* ntri, uint32_t* len, uint64_t* tri) {
   if (!ntri || !len || !tri) return GM_ERR_ARG;
   mc_tables_host(ntri, len, (unsigned long long*)tri);

@@ -112,6 +112,8 @@ struct MorphDims {
 // ---- layout kernels (gm_layout.hip)
 hipError_t launch_cells_to_rows(const uint8_t* cells, u32* rows, int R, hipStream_t s);
 hipError_t launch_rows_to_cells(const u32* rows, uint8_t* cells, int R, hipStream_t s);
+// bytes: a multiple of 16
+hipError_t launch_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 // ---- voxelizer (gm_voxelize.hip)
 // Index check and bounding box of a device mesh: out (8 u32, initialised to

@@ -92,6 +92,21 @@ hipError_t launch_cells_to_rows(const uint8_t* cells, u32* rows, int R, hipStrea
   return hipGetLastError();
 }
 
+// A plain device copy of whole 16-byte words, one per lane per step: the
+// reference point for the HBM byte counters (its bytes are known exactly).
+__global__ void __launch_bounds__(256) k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                    size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy_probe(const void* src, void* dst, size_t bytes, hipStream_t s) {
+  const size_t n = bytes / 16;
+  hipLaunchKernelGGL(k_copy_probe, dim3(8192), dim3(256), 0, s, static_cast<const uint4*>(src),
+                     static_cast<uint4*>(dst), n);
+  return hipGetLastError();
+}
+
 hipError_t launch_rows_to_cells(const u32* rows, uint8_t* cells, int R, hipStream_t s) {
   long long total = (long long)R * R * (R / 16);
   int bs = 256;

@@ -187,6 +187,11 @@ int gm_sync(gm_ctx* ctx);
 /* The context's HIP stream (hipStream_t), for callers that enqueue around it. */
 int gm_get_stream(gm_ctx* ctx, void** stream);
 
+/* Diagnostics, no counterpart in the reference: `reps` device copies of
+ * `bytes` (a multiple of 16) between two fresh buffers by a plain 16-byte
+ * per lane copy kernel, the known-size reference for HBM byte counters. */
+int gm_diag_copy_probe(int device, uint64_t bytes, int reps);
+
 /* The marching-cubes tables in use: per case the counted triangles, the listed
  * triangles and the packed edge ids (nibble 3q+i = edge i of triangle q, 0xf
  * terminates). Host only. */

@@ -243,35 +243,35 @@ def batched(args):
     # Each mode first runs the jobs once untimed with every mesh hashed (also
     # warming buffers), then times them without hashing.
     big = max(jobs, key=lambda j: j["base_res"])
+    def median_time(fn, reps=3):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
     ctx = B.Context(0)
     run_alone(ctx, big)
     ref_e = [_mesh_hash(*run_alone(ctx, j)) for j in jobs]
-    t0 = time.perf_counter()
-    for j in jobs:
-        run_alone(ctx, j)
-    t_ctx = time.perf_counter() - t0
+    t_ctx = median_time(lambda: [run_alone(ctx, j) for j in jobs])
     ctx.close()
     eng = B.Engine(0, slots=1, hbm_budget=args.hbm_budget << 30)
     same_e = run_engine(eng, jobs) == ref_e
-    t0 = time.perf_counter()
-    run_engine_timed(eng, jobs)
-    t_eng1 = time.perf_counter() - t0
+    t_eng1 = median_time(lambda: run_engine_timed(eng, jobs))
     eng.close()
+    # slot count tuned on the first half of the jobs
     tune = {}
-    for k in (16, 32, 64, 128):
+    for k in (16, 32, 64):
         eng = B.Engine(0, slots=k, hbm_budget=args.hbm_budget << 30)
         run_engine(eng, jobs[:128])
-        t0 = time.perf_counter()
-        run_engine_timed(eng, jobs[:128])
-        tune[k] = time.perf_counter() - t0
+        tune[k] = median_time(lambda: run_engine_timed(eng, jobs[:128]))
         eng.close()
     kbest = min(tune, key=tune.get)
     progress("batched end to end: slot tuning %s" % {k: round(v, 3) for k, v in tune.items()})
     eng = B.Engine(0, slots=kbest, hbm_budget=args.hbm_budget << 30)
     same_e = same_e and run_engine(eng, jobs) == ref_e
-    t0 = time.perf_counter()
-    run_engine_timed(eng, jobs)
-    t_eng = time.perf_counter() - t0
+    t_eng = median_time(lambda: run_engine_timed(eng, jobs))
     eng.close()
     t_seq = min(t_ctx, t_eng1)
     n = len(jobs)
@@ -289,7 +289,7 @@ def batched(args):
                            "mode": "from host meshes: voxelize + close + mesh + copy-back per job; engine: inputs "
                                    "borrowed, each mesh copied by the device into pinned host memory and released "
                                    "once there; one by one: one context, meshes copied into numpy; each mode is "
-                                   "run once hashed (bit_exact) and then timed"},
+                                   "run once hashed (bit_exact), then timed 3 times (median)"},
             "workload": "sphere-union jobs (8-128 icospheres), N log-uniform [64,384] step 32, r U{2..16}"}
 
 

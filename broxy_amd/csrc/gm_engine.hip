@@ -69,13 +69,13 @@ struct Job {
 };
 
 // Pinned host blocks for finished meshes, in power-of-two size classes from
-// 64 KiB to 32 MiB, reused after a job is released: the device copies the
+// 64 KiB to 256 MiB, reused after a job is released: the device copies the
 // mesh straight into the block a caller then reads (gm_engine_view) or
 // copies out once (gm_engine_take). hipHostMalloc is slow and pins pages, so
 // blocks are kept until the engine is destroyed; larger meshes go to
-// pageable memory.
+// pageable memory (a pageable copy runs at a fraction of the DMA rate).
 struct ResultPool {
-  static constexpr int kClasses = 10;
+  static constexpr int kClasses = 13;
   static constexpr size_t kMin = size_t(64) << 10;
   std::vector<void*> free[kClasses];
   std::vector<std::pair<void*, int>> all;
@@ -331,6 +331,8 @@ struct gm_engine {
   void batch_loop() {
     std::unique_lock<std::mutex> lk(m);
     for (;;) {
+      // everything ready goes at once (holding a batch back until more jobs
+      // are loaded left the device idle and measured slower)
       cvReady.wait(lk, [&] { return bstop || !ready.empty(); });
       if (ready.empty()) return;  // stopping
       std::vector<Slot*> take;
@@ -401,15 +403,19 @@ struct gm_engine {
       lk.unlock();
       bool batched = false;
       const bool ok = load(*s, j, batched);
-      if (ok && !batched) finish(*s, j);
       lk.lock();
       if (ok && batched) {
         ready.push_back(s);
         ++batchedPending;
         cvReady.notify_one();
-      } else {
-        complete(*s, j);
+        continue;
       }
+      if (ok) {
+        lk.unlock();
+        finish(*s, j);
+        lk.lock();
+      }
+      complete(*s, j);
     }
   }
 };

@@ -95,7 +95,8 @@ def build_workload(args):
     """(P, T, base_res, uniform radius or None, radius lattice or None, description)."""
     if args.config == "large":
         P, T = B.sh_blob(7, seed=args.seed, lmax=8, amp=0.1)
-        return P, T, 256, 8, None, "sh-icosphere subdiv7 (327,680 tris), N=512, ball r=8"
+        r = args.radius if args.radius else 8
+        return P, T, 256, r, None, "sh-icosphere subdiv7 (327,680 tris), N=512, ball r=%d" % r
     if args.config == "plain":
         P, T = B.sphere_soup(64, 3, seed=args.seed)
         return P, T, 64, 4, None, "64-icosphere soup subdiv3, N=128, ball r=4"
@@ -363,6 +364,8 @@ def main():
     ap.add_argument("--se", choices=["sphere", "cube"], default="sphere",
                     help="dilation structuring element (cube: the asymmetric closing, cube dilation + ball erosion)")
     ap.add_argument("--seed", type=int, default=2017)
+    ap.add_argument("--radius", type=int, default=0,
+                    help="uniform radius for --config large (default 8; above 31 the distance-transform passes run)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle parity check")
     ap.add_argument("--no-tune", action="store_true", help="default launch shape (no autotuning before timing)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-jobs measurement")
@@ -409,9 +412,10 @@ def main():
         ctx.scale_uniform(r)
 
     shape = None
-    if not args.no_tune:
+    if not args.no_tune and (r is None or r <= 31):
         # launch shape of the streaming passes for this grid and radius,
-        # chosen by timing every candidate (outside the timed region)
+        # chosen by timing every candidate (outside the timed region); radii
+        # above 31 run the distance-transform passes, which have none
         shape = ctx.autotune(se, 5)
     for _ in range(args.warmup):
         ctx.close_grid(se)

@@ -15,7 +15,7 @@ LIBDIR = os.path.join(HERE, "_lib")
 
 HIP_SOURCES = ["gm_context.hip", "gm_layout.hip", "gm_voxelize.hip", "gm_morph.hip", "gm_mc.hip",
                "gm_pyramid.hip", "gm_engine.hip", "gm_normals.hip",
-               "gm_scale.hip", "gm_noalloc.hip"]
+               "gm_scale.hip", "gm_noalloc.hip", "gm_edt.hip"]
 # gm_morph_ct.hip is compiled once per part (the compiled-radius kernels)
 CT_SOURCE = "gm_morph_ct.hip"
 CT_PARTS = 8

@@ -66,6 +66,9 @@ struct gm_ctx {
   u32 *rp = nullptr, *rvset = nullptr, *kv = nullptr, *kbl = nullptr;
   int capVarR = 0;
   bool rpValid = false;  // rp/rvset hold the current radius field
+  // radii above the chord tables: distance-transform scratch (N^3 ints each)
+  int *edtTmp = nullptr, *edtOut = nullptr;
+  size_t capEdt = 0;
   // structuring-element tables
   u32* dTables = nullptr;
   size_t tablesCap = 0;
@@ -231,6 +234,8 @@ void free_grid(gm_ctx* c) {
   cfree(c, c->rp); cfree(c, c->rvset); cfree(c, c->kv); cfree(c, c->kbl);
   c->capVarR = 0;
   c->rpValid = false;
+  cfree(c, c->edtTmp); cfree(c, c->edtOut);
+  c->capEdt = 0;
   c->has_grid = false;
   c->capR = 0;
   c->meshPending = false;
@@ -531,8 +536,24 @@ MorphDims morph_dims(gm_ctx* c) {
   return d;
 }
 
-int check_radius(gm_ctx* c) {
-  if (c->rhi > 31) return fail(c, GM_ERR_UNSUPPORTED, "radius above 31 fine voxels is not supported by this build");
+// Radii above 31 fine voxels (the chord tables' range) go through the
+// distance-transform passes, for the ball; the cube stays within the tables.
+bool use_edt(const gm_ctx* c) { return c->rhi > 31; }
+int check_radius(gm_ctx* c, int se) {
+  if (use_edt(c) && se == GM_SE_CUBE)
+    return fail(c, GM_ERR_UNSUPPORTED, "cube dilation above radius 31 fine voxels is not supported by this build");
+  return GM_OK;
+}
+int ensure_edt_buffers(gm_ctx* c) {
+  const size_t n = edt_scratch_ints(c->N);
+  if (n <= c->capEdt) return GM_OK;
+  cfree(c, c->edtTmp);
+  cfree(c, c->edtOut);
+  c->capEdt = 0;
+  HIPCALL(c, calloc_dev(c, &c->edtTmp, n * 4));
+  HIPCALL(c, calloc_dev(c, &c->edtOut, n * 4));
+  c->capEdt = n;
+  touch(c);
   return GM_OK;
 }
 
@@ -583,8 +604,9 @@ void set_range_from_hist(gm_ctx* c) {
 // Host-synchronous set-up of a dilation (tables, per-value buffers): done
 // before any capture.
 int prep_dilate(gm_ctx* c, int se) {
-  int rc = check_radius(c);
+  int rc = check_radius(c, se);
   if (rc) return rc;
+  if (use_edt(c)) return ensure_edt_buffers(c);
   const int lo = std::max(c->rlo, 0), hi = c->rhi;
   if (se == GM_SE_CUBE) return cube_tables(c, lo, hi);
   rc = ensure_tables(c, lo, hi);
@@ -592,8 +614,9 @@ int prep_dilate(gm_ctx* c, int se) {
   return ensure_var_buffers(c);
 }
 int prep_erode(gm_ctx* c) {
-  int rc = check_radius(c);
+  int rc = check_radius(c, GM_SE_SPHERE);
   if (rc) return rc;
+  if (use_edt(c)) return ensure_edt_buffers(c);
   rc = ensure_tables(c, std::max(c->rlo, 0), c->rhi);
   if (rc) return rc;
   return ensure_var_buffers(c);
@@ -602,7 +625,10 @@ int prep_erode(gm_ctx* c) {
 int enq_dilate(gm_ctx* c, int se) {
   const int lo = std::max(c->rlo, 0), hi = c->rhi;
   hipError_t e;
-  if (se == GM_SE_CUBE) {
+  if (use_edt(c)) {
+    e = launch_dilate_edt(morph_dims(c), morph_buffers(c), radius_field(c), hi, c->edtTmp, c->edtOut, c->stream,
+                          c->timing ? &c->ev[0] : nullptr, c->timing ? &c->ev[15] : nullptr);
+  } else if (se == GM_SE_CUBE) {
     MorphBuffers b = morph_buffers(c);
     b.tables = c->dCube;
     e = launch_dilate(morph_dims(c), b, radius_field(c), 1, c->cubeTabs, lo, hi, c->cubeWords, c->stream,
@@ -615,16 +641,19 @@ int enq_dilate(gm_ctx* c, int se) {
   return GM_OK;
 }
 int enq_erode(gm_ctx* c) {
-  hipError_t e = launch_erode(morph_dims(c), morph_buffers(c), radius_field(c), c->tabs, c->rlo, c->rhi,
-                              c->tabWords, c->stream, c->timing ? &c->ev[5] : nullptr,
-                              c->timing ? &c->ev[17] : nullptr);
+  hipError_t e =
+      use_edt(c) ? launch_erode_edt(morph_dims(c), morph_buffers(c), radius_field(c), c->rhi, c->edtTmp, c->edtOut,
+                                    c->stream, c->timing ? &c->ev[5] : nullptr, c->timing ? &c->ev[17] : nullptr)
+                 : launch_erode(morph_dims(c), morph_buffers(c), radius_field(c), c->tabs, c->rlo, c->rhi,
+                                c->tabWords, c->stream, c->timing ? &c->ev[5] : nullptr,
+                                c->timing ? &c->ev[17] : nullptr);
   if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("erosion: ") + hipGetErrorString(e));
   return GM_OK;
 }
 // State after a dilation or erosion was enqueued: the sphere's per-value
 // passes built the radius planes.
 void after_morph(gm_ctx* c, bool planes) {
-  if (planes && c->rmode != 0 && c->capVarR >= c->R) c->rpValid = true;
+  if (planes && !use_edt(c) && c->rmode != 0 && c->capVarR >= c->R) c->rpValid = true;
   c->meshPending = false;
 }
 
@@ -1298,6 +1327,7 @@ int gm_autotune(gm_ctx* c, int se, int reps, const char* path, int* tile_planes,
   if (rc) return rc;
   if (se != GM_SE_SPHERE && se != GM_SE_CUBE) return fail(c, GM_ERR_ARG, "unknown structuring element");
   if (c->rmode == 0 && c->r <= 1) return fail(c, GM_ERR_STATE, "radius <= 1: no morphology to tune");
+  if (use_edt(c)) return fail(c, GM_ERR_STATE, "radii above 31 run the distance-transform passes: no launch shape to tune");
   if (reps < 1) reps = 1;
   hipSetDevice(c->device);
   const bool timing = c->timing;
@@ -1523,7 +1553,6 @@ int gm_batch_close_mesh(gm_batch* b, gm_ctx* const* ctxs, int n, int se) {
     gm_ctx* c = ctxs[i];
     if (!c || !c->has_grid) return bfail(b, GM_ERR_STATE, "context without a grid in the batch");
     if (c->device != b->device) return bfail(b, GM_ERR_ARG, "context on another device");
-    if (c->rhi > 31) return bfail(b, GM_ERR_UNSUPPORTED, "radius above 31 fine voxels is not supported by this build");
     c->evValid = 0;
     if (se == GM_SE_SPHERE && c->rmode == 0 && c->r >= 2 && c->r <= 31) grouped.push_back(c);
     else single.push_back(c);

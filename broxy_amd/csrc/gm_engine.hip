@@ -156,6 +156,7 @@ uint64_t context_bytes(int R, int L, const gm_job& j) {
     b += (gm::radius_plane_words(R) + (uint64_t)R * R * R / 4) * 4;  // radius planes, masked K
     b += 2 * (gm::brick_words(R) + 1) * 4;                // value sets, K brick list
   }
+  if (j.radius > 31) b += 8 * N * N * N;          // distance-transform scratch (radii above the chord tables)
   b += (uint64_t)j.num_of_vertices * 24 + (uint64_t)j.num_of_faces * 12;
   return b + (16ull << 20);                       // tables, scans, slack
 }

@@ -847,6 +847,70 @@ hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusF
   return hipSuccess;
 }
 
+// Ball dilation and erosion for radii above the chord tables' range, through
+// the separable distance transforms of gm_edt.hip; the elementwise passes
+// (brick summary, tight contour, contour K, base contour) are the same as
+// above, and each transform pass writes what the chord pass it replaces
+// writes (DF, D bits on TG cells, CL, C bits on UG cells). tmp and out hold
+// N^3 ints each.
+// ev / evs (optional): the stage events of launch_dilate / launch_erode.
+hipError_t launch_dilate_edt(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf, int rho_hi, int* tmp,
+                             int* out, hipStream_t s, hipEvent_t* ev, hipEvent_t* evs) {
+  MorphArgs a = base_args(d, b, rf, SETableSet{});
+  const long long cellwords = (long long)d.R * d.R * d.W;
+  if (ev) GM_CHECK(hipEventRecord(ev[0], s));
+  hipLaunchKernelGGL(k_brick_summary, dim3(d.N / 8, d.N / 8), dim3(8 * d.W), 0, s, b.I, d.N, d.W, b.brickI,
+                     nullptr, b.IFL);
+  GM_CHECK(hipGetLastError());
+  if (ev) GM_CHECK(hipEventRecord(ev[1], s));
+  if (evs) GM_CHECK(hipEventRecord(evs[0], s));
+  // fine units, corner geometry: a source reaches at most rho = r + 1 voxels
+  GM_CHECK(launch_edt(b.I, d.N, 1, 1, 0, d.N, rho_hi + 2, false, rf, tmp, out, s));
+  GM_CHECK(launch_edt_dil_coarse(out, d, rf, b.IFL, b.DF, s));
+  if (ev) GM_CHECK(hipEventRecord(ev[2], s));
+  a.outFlags = b.TG;
+  a.outRows = b.D;
+  hipLaunchKernelGGL(k_tight, dim3((unsigned)((cellwords + 255) / 256)), dim3(256), 0, s, a);
+  GM_CHECK(hipGetLastError());
+  if (ev) GM_CHECK(hipEventRecord(ev[3], s));
+  if (evs) GM_CHECK(hipEventRecord(evs[1], s));
+  GM_CHECK(launch_edt_dil_fine(out, d, rf, b.TG, b.D, s));
+  if (ev) GM_CHECK(hipEventRecord(ev[4], s));
+  return hipSuccess;
+}
+
+hipError_t launch_erode_edt(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf, int rho_hi, int* tmp,
+                            int* out, hipStream_t s, hipEvent_t* ev, hipEvent_t* evs) {
+  MorphArgs a = base_args(d, b, rf, SETableSet{});
+  const long long cellwords = (long long)d.R * d.R * d.W;
+  if (ev) GM_CHECK(hipEventRecord(ev[0], s));
+  a.outRows = b.K;
+  hipLaunchKernelGGL(k_contour_K, dim3(d.N / 8, d.N / 8), dim3(8 * d.W), 0, s, a, b.brickK, nullptr);
+  GM_CHECK(hipGetLastError());
+  if (ev) GM_CHECK(hipEventRecord(ev[1], s));
+  if (evs) GM_CHECK(hipEventRecord(evs[0], s));
+  // doubled units: a source of radius r reaches 2 (r - 1); with a uniform
+  // radius the sources are binary and the threshold is 4 (r - 1)^2, with
+  // per-voxel radii each source carries its weight and the threshold is 0
+  const bool weighted = rf.mode != 0;
+  const int rr = (rf.r & 0x7f) - 1;
+  const int thr = weighted ? 0 : 4 * rr * rr;
+  const int reach = rho_hi + 2;
+  GM_CHECK(launch_edt(b.K, d.N, 2, 4, 2, d.R, reach, weighted, rf, tmp, out, s));
+  GM_CHECK(launch_edt_ero_coarse(out, thr, d, b.DAL, b.IFL, b.CL, s));
+  if (ev) GM_CHECK(hipEventRecord(ev[2], s));
+  a.outFlags = b.UG;
+  a.outRows = b.C;
+  hipLaunchKernelGGL(k_base_contour, dim3((unsigned)((cellwords + 255) / 256)), dim3(256), 0, s, a);
+  GM_CHECK(hipGetLastError());
+  if (ev) GM_CHECK(hipEventRecord(ev[3], s));
+  if (evs) GM_CHECK(hipEventRecord(evs[1], s));
+  GM_CHECK(launch_edt(b.K, d.N, 2, 2, 1, d.N, reach, weighted, rf, tmp, out, s));
+  GM_CHECK(launch_edt_ero_fine(out, thr, d, b.UG, b.C, s));
+  if (ev) GM_CHECK(hipEventRecord(ev[4], s));
+  return hipSuccess;
+}
+
 // ---------------------------------------------------------------------------
 // Batched closing of many uniform-radius sphere jobs: every step is one
 // launch over all jobs (per radius value for the streaming passes), with the

@@ -111,7 +111,8 @@ int gm_scale_brush(gm_ctx* ctx, const float position[3], float scale, float supp
 int gm_scale_grid_get(gm_ctx* ctx, int8_t* grid);
 
 /* GMorpho::DilateBySphereMipmap / DilateByCubeMipmap (GMorpho.cu:3538, 3231):
- * GM_GRID_DILATION from GM_GRID_INPUT. */
+ * GM_GRID_DILATION from GM_GRID_INPUT. Ball radii up to 127 (radii above 31
+ * run separable distance-transform passes); cube radii up to 31. */
 int gm_dilate(gm_ctx* ctx, int se);
 /* GMorpho::ErodeBySphereMipmap (GMorpho.cu:6627): GM_GRID_CONTOUR and
  * GM_GRID_CLOSING from GM_GRID_DILATION. */

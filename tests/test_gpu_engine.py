@@ -75,7 +75,7 @@ def test_engine_budget_and_errors():
             eng.submit(P, T, 512, 0.3, radius=4)  # does not fit 256 MiB
         with pytest.raises(B.GMorphoError):
             eng.submit(P, T, 40, 0.3, radius=4)   # not a multiple of 16
-        bad = eng.submit(P, T, 32, 0.3, radius=40)  # radius above the supported range
+        bad = eng.submit(P, T, 32, 0.3, radius=200)  # radius above the reference's 7-bit range
         ok = eng.submit(P, T, 32, 0.3, radius=3)
         with pytest.raises(B.GMorphoError):
             eng.wait(bad)

@@ -104,6 +104,69 @@ def test_dilate_erode_radius_range(ctx, r):
     assert np.array_equal(C, Cref), "erosion differs in %d cells" % int((C != Cref).sum())
 
 
+@pytest.mark.parametrize("R,r", [(48, 32), (64, 47), (96, 63), (96, 127)])
+def test_large_radius_uniform(ctx, R, r):
+    """Radii above the chord tables (32..127) run the distance-transform
+    passes: dilation, K, erosion and the mesh equal the oracle's, with a
+    clipped data region. The sources are a sparse cluster near one corner,
+    so that the ball reaches the grid's faces but not the far corner and
+    the closing is neither empty nor the whole grid."""
+    from _util import cells_from_fine
+    N = 2 * R
+    rng = np.random.default_rng(300 + r)
+    f = np.zeros((N, N, N), bool)
+    f[6:20, 8:22, 5:18] = rng.random((14, 14, 13)) < 0.05
+    f[30:34, 10:13, 12:15] = True
+    I = cells_from_fine(f)
+    # a clipped data region, except at the largest radius, where the clipped
+    # faces' contour would erode everything
+    dr = np.array([R - 3, R, R - 5] if r < 100 else [R, R, R], np.int32)
+    bmin, cell, _ = grid_setup(ctx, R, I, dr)
+    ctx.scale_uniform(r)
+    say("large radius R=%d r=%d" % (R, r))
+    ctx.close_grid(B.SE_SPHERE)
+    Dref = O.dilate(I, R, r=r, data_res=dr)
+    D = ctx.get_cells(B.GRID_DILATION)
+    assert np.array_equal(D, Dref), "dilation differs in %d cells" % int((D != Dref).sum())
+    assert np.array_equal(ctx.get_cells(B.GRID_CONTOUR), O.contour_K(Dref, R))
+    Cref = O.erode(I, Dref, R, r=r, data_res=dr)
+    C = ctx.get_cells(B.GRID_CLOSING)
+    assert np.array_equal(C, Cref), "erosion differs in %d cells" % int((C != Cref).sum())
+    assert 0 < int((Cref == 255).sum()) < R ** 3
+    ctx.extract_mesh()
+    V, Nn, T = ctx.mesh()
+    Vr, Nr, Tr = O.mc(Cref, R, bmin, cell)
+    assert np.array_equal(T, Tr) and np.allclose(V, Vr, rtol=1e-6, atol=0)
+
+
+def test_large_radius_scale_grid(ctx):
+    """A per-voxel radius field reaching above 31 (values 2..80): the
+    dilation takes each query's radius, the erosion weighs every source by
+    its own; both equal the oracle's. Then the same context closes with a
+    small uniform radius again (chord path), also equal."""
+    R = 48
+    N = 2 * R
+    I = random_blobs(R, seed=77, nballs=5, rmin=2.0, rmax=6.0, margin=4)
+    rng = np.random.default_rng(78)
+    sc = rng.integers(2, 81, N ** 3).astype(np.int8)
+    sc[rng.random(N ** 3) < 0.6] = 3
+    grid_setup(ctx, R, I)
+    ctx.scale_grid(sc)
+    say("large radius scale grid R=%d" % R)
+    ctx.close_grid(B.SE_SPHERE)
+    Dref = O.dilate(I, R, scale=sc)
+    assert np.array_equal(ctx.get_cells(B.GRID_DILATION), Dref)
+    assert np.array_equal(ctx.get_cells(B.GRID_CLOSING), O.erode(I, Dref, R, scale=sc))
+    ctx.scale_uniform(6)
+    ctx.close_grid(B.SE_SPHERE)
+    Dref = O.dilate(I, R, r=6)
+    assert np.array_equal(ctx.get_cells(B.GRID_DILATION), Dref)
+    assert np.array_equal(ctx.get_cells(B.GRID_CLOSING), O.erode(I, Dref, R, r=6))
+    ctx.scale_uniform(40)
+    with pytest.raises(B.GMorphoError):
+        ctx.dilate(B.SE_CUBE)  # the cube stays within the chord tables
+
+
 @pytest.mark.parametrize("r", [2, 4, 7])
 def test_sparse_input(ctx, r):
     R = 32

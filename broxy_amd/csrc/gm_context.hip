@@ -537,13 +537,9 @@ MorphDims morph_dims(gm_ctx* c) {
 }
 
 // Radii above 31 fine voxels (the chord tables' range) go through the
-// distance-transform passes, for the ball; the cube stays within the tables.
+// distance-transform passes.
 bool use_edt(const gm_ctx* c) { return c->rhi > 31; }
-int check_radius(gm_ctx* c, int se) {
-  if (use_edt(c) && se == GM_SE_CUBE)
-    return fail(c, GM_ERR_UNSUPPORTED, "cube dilation above radius 31 fine voxels is not supported by this build");
-  return GM_OK;
-}
+int check_radius(gm_ctx*, int) { return GM_OK; }
 int ensure_edt_buffers(gm_ctx* c) {
   const size_t n = edt_scratch_ints(c->N);
   if (n <= c->capEdt) return GM_OK;
@@ -626,8 +622,8 @@ int enq_dilate(gm_ctx* c, int se) {
   const int lo = std::max(c->rlo, 0), hi = c->rhi;
   hipError_t e;
   if (use_edt(c)) {
-    e = launch_dilate_edt(morph_dims(c), morph_buffers(c), radius_field(c), hi, c->edtTmp, c->edtOut, c->stream,
-                          c->timing ? &c->ev[0] : nullptr, c->timing ? &c->ev[15] : nullptr);
+    e = launch_dilate_edt(morph_dims(c), morph_buffers(c), radius_field(c), se == GM_SE_CUBE ? 1 : 0, hi, c->edtTmp,
+                          c->edtOut, c->stream, c->timing ? &c->ev[0] : nullptr, c->timing ? &c->ev[15] : nullptr);
   } else if (se == GM_SE_CUBE) {
     MorphBuffers b = morph_buffers(c);
     b.tables = c->dCube;
@@ -1249,6 +1245,16 @@ int gm_pyramid_get(gm_ctx* c, uint64_t* out) {
 }
 
 uint64_t gm_ctx_device_bytes(const gm_ctx* c) { return c ? c->devBytes : 0; }
+
+// Frees the context's grid buffers (a job engine releasing an idle
+// context's memory for a job that would not fit the budget otherwise).
+int gm_ctx_release(gm_ctx* c) {
+  if (!c) return GM_ERR_ARG;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  free_grid(c);
+  return GM_OK;
+}
 
 // Buffers for grids up to base resolution R, allocated now (a job engine
 // sizes a context once for the largest job it will batch, so that jobs of

@@ -12,6 +12,8 @@
 //
 //   dilation (gather, radius of the query point p, fine units):
 //     F(p) = min_{q in I} sum_a dist(p_a, [q_a, q_a + 1])^2 < (r(p) + 1)^2
+//     and, for the cube (GMorpho.cu:1838-1934), with the largest per-axis
+//     distance in place of the sum: F(p) = min_q max_a dist(...) <= r(p)
 //   erosion (scatter, radius of the source, doubled units so that the
 //   sub-voxel centres are integers):
 //     E(Q) = min_{q in K} [sum_a dist(Q_a, [2 q_a, 2 q_a + 2])^2 - 4 (r(q) - 1)^2] < 0
@@ -73,7 +75,7 @@ __device__ __forceinline__ int first_set_at_or_above(const u32* __restrict__ row
 // and query i. WEIGHTED: min over the row's sources in reach of
 // dist^2 - 4 (r(q) - 1)^2 (erosion with per-voxel radii); otherwise the
 // squared distance to the nearest source.
-template <bool WEIGHTED>
+template <bool WEIGHTED, bool CHEB>
 __global__ void __launch_bounds__(256) k_edt_x(const u32* __restrict__ src, int N, int W, int s, int a, int b,
                                                int nq, int reach, RadiusField rf, int* __restrict__ out) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -90,12 +92,12 @@ __global__ void __launch_bounds__(256) k_edt_x(const u32* __restrict__ src, int 
     const int jl = last_set_at_or_below(row, min(jx, N - 1), reach + 1);
     if (jl >= 0) {
       const int d = seg_dist(X, jl, s);
-      best = min(best, d * d);
+      best = min(best, CHEB ? d : d * d);
     }
     const int jr = first_set_at_or_above(row, W, jx + 1, reach + 1);
     if (jr >= 0) {
       const int d = seg_dist(X, jr, s);
-      best = min(best, d * d);
+      best = min(best, CHEB ? d : d * d);
     }
   } else {
     const int y = (int)(rowid % (size_t)N), z = (int)(rowid / (size_t)N);
@@ -117,7 +119,9 @@ __global__ void __launch_bounds__(256) k_edt_x(const u32* __restrict__ src, int 
 }
 
 // Stage y or z: out[(o nq + k) inner + ii] = min over sources j in reach of
-// in[(o nsrc + j) inner + ii] + dist(a k + b, [s j, s j + s])^2.
+// in[(o nsrc + j) inner + ii] + dist(a k + b, [s j, s j + s])^2 (CHEB: the
+// larger of the two, the cube's per-axis distance bound).
+template <bool CHEB>
 __global__ void __launch_bounds__(256) k_edt_axis(const int* __restrict__ in, int* __restrict__ out, int nouter,
                                                   int nsrc, int nq, long long inner, int s, int a, int b,
                                                   int reach) {
@@ -136,7 +140,7 @@ __global__ void __launch_bounds__(256) k_edt_axis(const int* __restrict__ in, in
     const int v = p[(size_t)j * (size_t)inner];
     if (v >= kInf) continue;
     const int d = seg_dist(Y, j, s);
-    best = min(best, v + d * d);
+    best = min(best, CHEB ? max(v, d) : v + d * d);
   }
   out[t] = best;
 }
@@ -145,8 +149,9 @@ __global__ void __launch_bounds__(256) k_edt_axis(const int* __restrict__ in, in
 
 // DF(c) = I-full(c) | (c in the data region & F(2c)); one thread per
 // cell-row word (16 cells).
+// cube: dt is the per-axis (Chebyshev) distance, F(p) = dt <= r(p)
 __global__ void __launch_bounds__(256) k_edt_dil_coarse(const int* __restrict__ dt, MorphDims d, RadiusField rf,
-                                                        const u16* __restrict__ ifl, u16* __restrict__ df) {
+                                                        int cube, const u16* __restrict__ ifl, u16* __restrict__ df) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t total = (size_t)d.R * d.R * d.W;
   if (t >= total) return;
@@ -158,8 +163,8 @@ __global__ void __launch_bounds__(256) k_edt_dil_coarse(const int* __restrict__ 
       const int cx = 16 * w + j;
       if (cx >= d.dR[0]) break;
       const int px = 2 * cx, py = 2 * cy, pz = 2 * cz;
-      const int rho = radius_at(rf, px, py, pz) + 1;
-      if (dt[((size_t)pz * d.N + py) * d.N + px] < rho * rho) f |= 1u << j;
+      const int r = radius_at(rf, px, py, pz), v = dt[((size_t)pz * d.N + py) * d.N + px];
+      if (cube ? v <= r : v < (r + 1) * (r + 1)) f |= 1u << j;
     }
   }
   df[t] = (u16)(ifl[t] | f);
@@ -168,7 +173,7 @@ __global__ void __launch_bounds__(256) k_edt_dil_coarse(const int* __restrict__ 
 // D |= expand(TG) & F on the fine voxels of tagged cells; one thread per
 // fine row word.
 __global__ void __launch_bounds__(256) k_edt_dil_fine(const int* __restrict__ dt, MorphDims d, RadiusField rf,
-                                                      const u16* __restrict__ tg, u32* __restrict__ D) {
+                                                      int cube, const u16* __restrict__ tg, u32* __restrict__ D) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t total = (size_t)d.N * d.N * d.W;
   if (t >= total) return;
@@ -180,8 +185,8 @@ __global__ void __launch_bounds__(256) k_edt_dil_fine(const int* __restrict__ dt
   for (int j = 0; j < 32; ++j) {
     if (!((tags >> (j >> 1)) & 1u)) continue;
     const int x = 32 * w + j;
-    const int rho = radius_at(rf, x, y, z) + 1;
-    if (dt[((size_t)z * d.N + y) * d.N + x] < rho * rho) v |= 1u << j;
+    const int r = radius_at(rf, x, y, z), e = dt[((size_t)z * d.N + y) * d.N + x];
+    if (cube ? e <= r : e < (r + 1) * (r + 1)) v |= 1u << j;
   }
   if (v) D[t] |= v;
 }
@@ -235,38 +240,50 @@ size_t edt_scratch_ints(int N) { return (size_t)N * N * N; }
 // Transform of the bit grid src at the query lattice X = a i + b (nq per
 // axis) into out (nq^3, x fastest); tmp holds N^2 nq ints. weighted: per-voxel
 // erosion weights from rf.
-hipError_t launch_edt(const u32* src, int N, int s, int a, int b, int nq, int reach, bool weighted,
+hipError_t launch_edt(const u32* src, int N, int s, int a, int b, int nq, int reach, bool weighted, bool cheb,
                       const RadiusField& rf, int* tmp, int* out, hipStream_t st) {
   const int W = N / 32;
   const size_t n1 = (size_t)N * N * nq;
   if (weighted)
-    hipLaunchKernelGGL(k_edt_x<true>, dim3(blocks_for(n1)), dim3(256), 0, st, src, N, W, s, a, b, nq, reach, rf,
-                       out);
+    hipLaunchKernelGGL((k_edt_x<true, false>), dim3(blocks_for(n1)), dim3(256), 0, st, src, N, W, s, a, b, nq, reach,
+                       rf, out);
+  else if (cheb)
+    hipLaunchKernelGGL((k_edt_x<false, true>), dim3(blocks_for(n1)), dim3(256), 0, st, src, N, W, s, a, b, nq, reach,
+                       rf, out);
   else
-    hipLaunchKernelGGL(k_edt_x<false>, dim3(blocks_for(n1)), dim3(256), 0, st, src, N, W, s, a, b, nq, reach, rf,
-                       out);
+    hipLaunchKernelGGL((k_edt_x<false, false>), dim3(blocks_for(n1)), dim3(256), 0, st, src, N, W, s, a, b, nq,
+                       reach, rf, out);
   GM_CHECK(hipGetLastError());
   // y: [z][j][i] -> [z][k][i]
   const size_t n2 = (size_t)N * nq * nq;
-  hipLaunchKernelGGL(k_edt_axis, dim3(blocks_for(n2)), dim3(256), 0, st, out, tmp, N, N, nq, (long long)nq, s, a, b,
-                     reach);
+  if (cheb)
+    hipLaunchKernelGGL(k_edt_axis<true>, dim3(blocks_for(n2)), dim3(256), 0, st, out, tmp, N, N, nq, (long long)nq, s,
+                       a, b, reach);
+  else
+    hipLaunchKernelGGL(k_edt_axis<false>, dim3(blocks_for(n2)), dim3(256), 0, st, out, tmp, N, N, nq, (long long)nq,
+                       s, a, b, reach);
   GM_CHECK(hipGetLastError());
   // z: [j][k][i] -> [k'][k][i]
   const size_t n3 = (size_t)nq * nq * nq;
-  hipLaunchKernelGGL(k_edt_axis, dim3(blocks_for(n3)), dim3(256), 0, st, tmp, out, 1, N, nq,
-                     (long long)nq * nq, s, a, b, reach);
+  if (cheb)
+    hipLaunchKernelGGL(k_edt_axis<true>, dim3(blocks_for(n3)), dim3(256), 0, st, tmp, out, 1, N, nq,
+                       (long long)nq * nq, s, a, b, reach);
+  else
+    hipLaunchKernelGGL(k_edt_axis<false>, dim3(blocks_for(n3)), dim3(256), 0, st, tmp, out, 1, N, nq,
+                       (long long)nq * nq, s, a, b, reach);
   return hipGetLastError();
 }
 
-hipError_t launch_edt_dil_coarse(const int* dt, const MorphDims& d, const RadiusField& rf, const u16* ifl, u16* df,
-                                 hipStream_t st) {
-  hipLaunchKernelGGL(k_edt_dil_coarse, dim3(blocks_for((size_t)d.R * d.R * d.W)), dim3(256), 0, st, dt, d, rf, ifl,
-                     df);
+hipError_t launch_edt_dil_coarse(const int* dt, const MorphDims& d, const RadiusField& rf, bool cube, const u16* ifl,
+                                 u16* df, hipStream_t st) {
+  hipLaunchKernelGGL(k_edt_dil_coarse, dim3(blocks_for((size_t)d.R * d.R * d.W)), dim3(256), 0, st, dt, d, rf,
+                     cube ? 1 : 0, ifl, df);
   return hipGetLastError();
 }
-hipError_t launch_edt_dil_fine(const int* dt, const MorphDims& d, const RadiusField& rf, const u16* tg, u32* D,
-                               hipStream_t st) {
-  hipLaunchKernelGGL(k_edt_dil_fine, dim3(blocks_for((size_t)d.N * d.N * d.W)), dim3(256), 0, st, dt, d, rf, tg, D);
+hipError_t launch_edt_dil_fine(const int* dt, const MorphDims& d, const RadiusField& rf, bool cube, const u16* tg,
+                               u32* D, hipStream_t st) {
+  hipLaunchKernelGGL(k_edt_dil_fine, dim3(blocks_for((size_t)d.N * d.N * d.W)), dim3(256), 0, st, dt, d, rf,
+                     cube ? 1 : 0, tg, D);
   return hipGetLastError();
 }
 hipError_t launch_edt_ero_coarse(const int* et, int thr, const MorphDims& d, const u16* dal, const u16* ifl, u16* cl,

@@ -39,6 +39,7 @@
 
 extern "C" uint64_t gm_ctx_device_bytes(const gm_ctx* c);
 extern "C" int gm_ctx_reserve(gm_ctx* c, int R);
+extern "C" int gm_ctx_release(gm_ctx* c);
 
 namespace {
 
@@ -131,12 +132,14 @@ struct Slot {
 // be many more, so that jobs keep loading while earlier ones wait for a batch.
 constexpr int kMaxWorkers = 16;
 
-// Device bytes of one context at base resolution R (the buffers alloc_grid,
-// the mesh output and the radius field allocate), plus the mesh input.
 // Jobs up to this base resolution are closed in batches; larger grids keep
 // the GPU busy on their own.
 constexpr int kBatchMaxRes = 192;
 
+// Device bytes of one context whose buffers are sized for base resolution R
+// (the buffers alloc_grid, the mesh output and the radius field allocate),
+// running job j: plus its mesh input and, for a radius above the chord
+// tables, the distance-transform scratch of its own grid.
 uint64_t context_bytes(int R, int L, const gm_job& j) {
   const uint64_t N = 2ull * (uint64_t)R, W = N / 32;
   uint64_t P = 1;
@@ -156,7 +159,10 @@ uint64_t context_bytes(int R, int L, const gm_job& j) {
     b += (gm::radius_plane_words(R) + (uint64_t)R * R * R / 4) * 4;  // radius planes, masked K
     b += 2 * (gm::brick_words(R) + 1) * 4;                // value sets, K brick list
   }
-  if (j.radius > 31) b += 8 * N * N * N;          // distance-transform scratch (radii above the chord tables)
+  if (j.radius > 31 && j.radius <= 127) {
+    const uint64_t Nj = 2ull * (uint64_t)j.base_res;
+    b += 8 * Nj * Nj * Nj;                        // distance-transform scratch (radii above the chord tables)
+  }
   b += (uint64_t)j.num_of_vertices * 24 + (uint64_t)j.num_of_faces * 12;
   return b + (16ull << 20);                       // tables, scans, slack
 }
@@ -188,6 +194,7 @@ struct gm_engine {
   std::vector<Slot*> ready;
   std::deque<Slot*> closed;
   int batchedPending = 0;  // slots handed to the batcher, not yet in `closed`
+  int evicting = 0;        // idle slots whose buffers a worker is releasing
   bool bstop = false;
   ResultPool pool;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
@@ -195,6 +202,13 @@ struct gm_engine {
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
+
+  // Buffer size of a context running j: a context that batches is sized
+  // once for the largest batched grid the budget allows every slot to hold
+  // (reserveRes, set at creation), so jobs of mixed small sizes never
+  // reallocate.
+  int reserveRes = kBatchMaxRes;
+  int buffer_res(const Job* j) const { return std::max(j->R, j->R <= kBatchMaxRes ? reserveRes : j->R); }
 
   // First queued job that some free slot can run within the budget, on the
   // free slot whose reservation grows least (caller holds m). A job that
@@ -204,8 +218,7 @@ struct gm_engine {
       Job* j = *it;
       Slot* best = nullptr;
       uint64_t bestGrow = 0, bestNeed = 0;
-      // a context that batches is sized once for the largest batched grid
-      const int Rj = j->R <= kBatchMaxRes ? kBatchMaxRes : j->R;
+      const int Rj = buffer_res(j);
       for (Slot& s : slots) {
         if (s.busy) continue;
         const int R = std::max(s.capR, Rj);
@@ -376,12 +389,52 @@ struct gm_engine {
     for (;;) {
       Slot* s = nullptr;
       Job* j = nullptr;
+      Slot* evict = nullptr;
       // closed batches first: copying their meshes back frees their slots
       cvWork.wait(lk, [&] {
         if (!closed.empty()) return true;
         if (stop) return batchedPending == 0;
-        return pick(s, j);
+        for (;;) {
+          if (pick(s, j)) return true;
+          // jobs wait, none fits, and no slot holds a job: the buffers the
+          // idle slots keep are what blocks the budget; release the largest
+          if (queue.empty() || running != 0) return false;
+          for (Slot& t : slots)
+            if (!t.busy && t.capR > 0 && (!evict || t.reserved > evict->reserved)) evict = &t;
+          if (evict) {
+            evict->busy = true;
+            ++evicting;
+            return true;
+          }
+          // another worker is still releasing a slot: wait for it
+          if (evicting) return false;
+          // nothing left to release: the first queued job cannot run here
+          Job* jf = queue.front();
+          queue.pop_front();
+          jf->status = GM_ERR_CAPACITY;
+          jf->err = "GMorpho Error: job does not fit the engine's HBM budget beside the buffers other slots keep";
+          jf->srcP = nullptr;
+          jf->srcT = nullptr;
+          jf->srcLat = nullptr;
+          jf->tStart = jf->tLoaded = jf->tMeshed = jf->tDone = now();
+          jf->done = true;
+          cvDone.notify_all();
+        }
       });
+      if (evict) {
+        lk.unlock();
+        gm_ctx_release(evict->ctx);
+        const uint64_t held = gm_ctx_device_bytes(evict->ctx);
+        lk.lock();
+        reservedTotal -= evict->reserved;
+        evict->reserved = held;
+        reservedTotal += held;
+        evict->capR = 0;
+        evict->busy = false;
+        --evicting;
+        cvWork.notify_all();
+        continue;
+      }
       if (!closed.empty()) {
         s = closed.front();
         closed.pop_front();
@@ -437,6 +490,11 @@ int gm_engine_create(int device, int slots, uint64_t hbm_budget_bytes, gm_engine
   e->device = device;
   e->budget = hbm_budget_bytes;
   e->slots.resize(slots);
+  {
+    gm_job none{};
+    while (e->reserveRes > 16 && (uint64_t)slots * context_bytes(e->reserveRes, 0, none) > hbm_budget_bytes)
+      e->reserveRes -= 16;
+  }
   for (auto& s : e->slots) {
     if (gm_create(device, &s.ctx) == GM_OK) {
       // a slot runs a different job almost every time: capture only what repeats
@@ -517,7 +575,9 @@ int submit_job(gm_engine* e, const gm_job* job, uint64_t* id, bool borrow) {
   j->desc.P = nullptr;
   j->desc.T = nullptr;
   j->desc.lattice = nullptr;
-  if (context_bytes(j->R, j->desc.lattice_res, j->desc) > e->budget) {
+  // what the job needs on a fresh slot (the sizing pick uses): a job that
+  // could never be admitted is refused here rather than left queued
+  if (context_bytes(e->buffer_res(j), j->desc.lattice_res, j->desc) > e->budget) {
     delete j;
     return efail(e, GM_ERR_CAPACITY, "job does not fit the engine's HBM budget");
   }

@@ -231,17 +231,17 @@ hipError_t launch_brick_summary_batched(const BrickJob* dJobs, const u32* dPre, 
 size_t batch_queue_cap(const BatchMorphJob* jobs, int n);
 // ---- radii above the chord tables (gm_edt.hip, orchestration in gm_morph.hip)
 size_t edt_scratch_ints(int N);
-hipError_t launch_edt(const u32* src, int N, int s, int a, int b, int nq, int reach, bool weighted,
+hipError_t launch_edt(const u32* src, int N, int s, int a, int b, int nq, int reach, bool weighted, bool cheb,
                       const RadiusField& rf, int* tmp, int* out, hipStream_t st);
-hipError_t launch_edt_dil_coarse(const int* dt, const MorphDims& d, const RadiusField& rf, const u16* ifl, u16* df,
-                                 hipStream_t st);
-hipError_t launch_edt_dil_fine(const int* dt, const MorphDims& d, const RadiusField& rf, const u16* tg, u32* D,
-                               hipStream_t st);
+hipError_t launch_edt_dil_coarse(const int* dt, const MorphDims& d, const RadiusField& rf, bool cube, const u16* ifl,
+                                 u16* df, hipStream_t st);
+hipError_t launch_edt_dil_fine(const int* dt, const MorphDims& d, const RadiusField& rf, bool cube, const u16* tg,
+                               u32* D, hipStream_t st);
 hipError_t launch_edt_ero_coarse(const int* et, int thr, const MorphDims& d, const u16* dal, const u16* ifl, u16* cl,
                                  hipStream_t st);
 hipError_t launch_edt_ero_fine(const int* et, int thr, const MorphDims& d, const u16* ug, u32* C, hipStream_t st);
-hipError_t launch_dilate_edt(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf, int rho_hi, int* tmp,
-                             int* out, hipStream_t s, hipEvent_t* ev, hipEvent_t* evs);
+hipError_t launch_dilate_edt(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf, int se, int rho_hi,
+                             int* tmp, int* out, hipStream_t s, hipEvent_t* ev, hipEvent_t* evs);
 hipError_t launch_erode_edt(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf, int rho_hi, int* tmp,
                             int* out, hipStream_t s, hipEvent_t* ev, hipEvent_t* evs);
 

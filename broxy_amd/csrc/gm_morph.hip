@@ -854,8 +854,9 @@ hipError_t launch_erode(const MorphDims& d, const MorphBuffers& b, const RadiusF
 // writes (DF, D bits on TG cells, CL, C bits on UG cells). tmp and out hold
 // N^3 ints each.
 // ev / evs (optional): the stage events of launch_dilate / launch_erode.
-hipError_t launch_dilate_edt(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf, int rho_hi, int* tmp,
-                             int* out, hipStream_t s, hipEvent_t* ev, hipEvent_t* evs) {
+hipError_t launch_dilate_edt(const MorphDims& d, const MorphBuffers& b, const RadiusField& rf, int se, int rho_hi,
+                             int* tmp, int* out, hipStream_t s, hipEvent_t* ev, hipEvent_t* evs) {
+  const bool cube = se != 0;
   MorphArgs a = base_args(d, b, rf, SETableSet{});
   const long long cellwords = (long long)d.R * d.R * d.W;
   if (ev) GM_CHECK(hipEventRecord(ev[0], s));
@@ -864,9 +865,9 @@ hipError_t launch_dilate_edt(const MorphDims& d, const MorphBuffers& b, const Ra
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
   if (evs) GM_CHECK(hipEventRecord(evs[0], s));
-  // fine units, corner geometry: a source reaches at most rho = r + 1 voxels
-  GM_CHECK(launch_edt(b.I, d.N, 1, 1, 0, d.N, rho_hi + 2, false, rf, tmp, out, s));
-  GM_CHECK(launch_edt_dil_coarse(out, d, rf, b.IFL, b.DF, s));
+  // fine units, corner geometry: a source reaches at most r + 1 voxels
+  GM_CHECK(launch_edt(b.I, d.N, 1, 1, 0, d.N, rho_hi + 2, false, cube, rf, tmp, out, s));
+  GM_CHECK(launch_edt_dil_coarse(out, d, rf, cube, b.IFL, b.DF, s));
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   a.outFlags = b.TG;
   a.outRows = b.D;
@@ -874,7 +875,7 @@ hipError_t launch_dilate_edt(const MorphDims& d, const MorphBuffers& b, const Ra
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[3], s));
   if (evs) GM_CHECK(hipEventRecord(evs[1], s));
-  GM_CHECK(launch_edt_dil_fine(out, d, rf, b.TG, b.D, s));
+  GM_CHECK(launch_edt_dil_fine(out, d, rf, cube, b.TG, b.D, s));
   if (ev) GM_CHECK(hipEventRecord(ev[4], s));
   return hipSuccess;
 }
@@ -896,7 +897,7 @@ hipError_t launch_erode_edt(const MorphDims& d, const MorphBuffers& b, const Rad
   const int rr = (rf.r & 0x7f) - 1;
   const int thr = weighted ? 0 : 4 * rr * rr;
   const int reach = rho_hi + 2;
-  GM_CHECK(launch_edt(b.K, d.N, 2, 4, 2, d.R, reach, weighted, rf, tmp, out, s));
+  GM_CHECK(launch_edt(b.K, d.N, 2, 4, 2, d.R, reach, weighted, false, rf, tmp, out, s));
   GM_CHECK(launch_edt_ero_coarse(out, thr, d, b.DAL, b.IFL, b.CL, s));
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   a.outFlags = b.UG;
@@ -905,7 +906,7 @@ hipError_t launch_erode_edt(const MorphDims& d, const MorphBuffers& b, const Rad
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[3], s));
   if (evs) GM_CHECK(hipEventRecord(evs[1], s));
-  GM_CHECK(launch_edt(b.K, d.N, 2, 2, 1, d.N, reach, weighted, rf, tmp, out, s));
+  GM_CHECK(launch_edt(b.K, d.N, 2, 2, 1, d.N, reach, weighted, false, rf, tmp, out, s));
   GM_CHECK(launch_edt_ero_fine(out, thr, d, b.UG, b.C, s));
   if (ev) GM_CHECK(hipEventRecord(ev[4], s));
   return hipSuccess;

@@ -83,6 +83,21 @@ def test_engine_budget_and_errors():
         assert nt > 0
         reserved, running, queued = eng.stats()
         assert reserved <= 256 << 20 and running == 0 and queued == 0
+        # a radius above the chord tables needs distance-transform scratch
+        # of its grid (8 N^3 bytes): refused at submit when that cannot fit,
+        # never left queued
+        with pytest.raises(B.GMorphoError):
+            eng.submit(P, T, 192, 0.3, radius=100)
+        # two small jobs leave both slots holding buffers; a larger grid that
+        # fits only once an idle slot's buffers are released still runs
+        ids = [eng.submit(P, T, 32, 0.3, radius=3) for _ in range(2)]
+        for i in ids:
+            eng.wait(i)
+        big = eng.submit(P, T, 240, 0.3, radius=3)
+        nv2, nt2 = eng.wait(big)
+        assert nt2 > 0
+        reserved, running, queued = eng.stats()
+        assert reserved <= 256 << 20 and running == 0 and queued == 0
     finally:
         eng.close()
 

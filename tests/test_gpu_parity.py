@@ -143,7 +143,8 @@ def test_large_radius_scale_grid(ctx):
     """A per-voxel radius field reaching above 31 (values 2..80): the
     dilation takes each query's radius, the erosion weighs every source by
     its own; both equal the oracle's. Then the same context closes with a
-    small uniform radius again (chord path), also equal."""
+    small uniform radius again (chord path), and dilates with the cube at
+    radius 40 and with the field, all equal to the oracle."""
     R = 48
     N = 2 * R
     I = random_blobs(R, seed=77, nballs=5, rmin=2.0, rmax=6.0, margin=4)
@@ -162,9 +163,13 @@ def test_large_radius_scale_grid(ctx):
     Dref = O.dilate(I, R, r=6)
     assert np.array_equal(ctx.get_cells(B.GRID_DILATION), Dref)
     assert np.array_equal(ctx.get_cells(B.GRID_CLOSING), O.erode(I, Dref, R, r=6))
+    # the cube above radius 31: the per-axis (Chebyshev) transform
     ctx.scale_uniform(40)
-    with pytest.raises(B.GMorphoError):
-        ctx.dilate(B.SE_CUBE)  # the cube stays within the chord tables
+    ctx.dilate(B.SE_CUBE)
+    assert np.array_equal(ctx.get_cells(B.GRID_DILATION), O.dilate(I, R, r=40, se=1))
+    ctx.scale_grid(sc)
+    ctx.dilate(B.SE_CUBE)
+    assert np.array_equal(ctx.get_cells(B.GRID_DILATION), O.dilate(I, R, scale=sc, se=1))
 
 
 @pytest.mark.parametrize("r", [2, 4, 7])

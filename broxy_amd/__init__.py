@@ -151,8 +151,10 @@ class Context:
         T = np.ascontiguousarray(T, np.uint32)
         self._call("gm_load_mesh", _ptr(P), None, len(P), _ptr(T), len(T))
 
-    def voxelize(self, base_res, margin):
-        self._call("gm_voxelize", base_res, ctypes.c_float(margin))
+    def voxelize(self, base_res, margin, multislice=False):
+        """GM_GRID_INPUT from the loaded mesh: the conservative voxelization,
+        or with multislice=True the reference's alternative MultiSlicing."""
+        self._call("gm_voxelize_multislice" if multislice else "gm_voxelize", base_res, ctypes.c_float(margin))
 
     def grid_init(self, bmin, bmax, base_res, data_res, cell):
         bmin = np.ascontiguousarray(bmin, np.float32)

@@ -770,7 +770,12 @@ int gm_load_mesh(gm_ctx* c, const float* P, const float* N, int nv, const uint32
   return GM_OK;
 }
 
-int gm_voxelize(gm_ctx* c, int base_res, float margin) {
+}  // extern "C"
+
+namespace {
+// GM_GRID_INPUT from the loaded mesh: the conservative voxelization (parity
+// fill along z plus the surface shell), or MultiSlicing (parity along x).
+int voxelize_impl(gm_ctx* c, int base_res, float margin, bool multislice) {
   if (!c) return GM_ERR_ARG;
   touch(c);
   if (c->nv <= 0) return fail(c, GM_ERR_STATE, "no mesh: call gm_load_mesh first");
@@ -806,14 +811,23 @@ int gm_voxelize(gm_ctx* c, int base_res, float margin) {
   // after allocation or a voxelization that did not complete its launches
   const bool clear = c->voxDirty;
   c->voxDirty = true;
-  hipError_t e = launch_voxelize(c->dP, c->nv, c->dT, c->nf, c->bmin, scale, c->R, c->dSnap, c->mark, c->shell, c->I,
-                                 c->stream, clear);
+  hipError_t e = multislice ? launch_voxelize_multislice(c->dP, c->nv, c->dT, c->nf, c->bmin, scale, c->R, c->dSnap,
+                                                         c->mark, c->I, c->stream, clear)
+                            : launch_voxelize(c->dP, c->nv, c->dT, c->nf, c->bmin, scale, c->R, c->dSnap, c->mark,
+                                              c->shell, c->I, c->stream, clear);
   if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("voxelize: ") + hipGetErrorString(e));
   c->voxDirty = false;
   HIPCALL(c, hipStreamSynchronize(c->stream));
   c->meshPending = false;
   return GM_OK;
 }
+}  // namespace
+
+extern "C" {
+
+int gm_voxelize(gm_ctx* c, int base_res, float margin) { return voxelize_impl(c, base_res, margin, false); }
+
+int gm_voxelize_multislice(gm_ctx* c, int base_res, float margin) { return voxelize_impl(c, base_res, margin, true); }
 
 int gm_load(gm_ctx* c, const float* P, const float* N, int nv, const uint32_t* T, int nf, int base_res,
             float se_size) {

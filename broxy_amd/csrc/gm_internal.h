@@ -131,6 +131,9 @@ inline float mesh_key_float(u32 k) {
 }
 hipError_t launch_voxelize(const float* P, int nv, const u32* T, int nt, const float* bmin3, double scale,
                            int R, int32_t* snap, u32* mark, u32* shell, u32* rows, hipStream_t s, bool clear);
+// MultiSlicing: parity along x only (the marks are cleared by its fill too)
+hipError_t launch_voxelize_multislice(const float* P, int nv, const u32* T, int nt, const float* bmin3, double scale,
+                                      int R, int32_t* snap, u32* mark, u32* rows, hipStream_t s, bool clear);
 
 // ---- morphology (gm_morph.hip)
 struct MorphBuffers {

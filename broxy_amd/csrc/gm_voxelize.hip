@@ -185,6 +185,85 @@ __global__ void k_vox_tris(const int32_t* __restrict__ X, const u32* __restrict_
   }
 }
 
+// MultiSlicing (Voxelizer::VoxelizeByMultiSlicing, Voxelizer.cu:188-331):
+// rays along +x through the fine column centres (y + 1/2, z + 1/2); a
+// triangle covering a centre under the top-left rule toggles the column from
+// the voxel holding its crossing on, as one toggle bit at that voxel (the fill
+// below takes the running XOR along x). No surface shell. Same snapped
+// vertices and exact integer rule as the conservative path.
+__global__ void k_vox_ms_tris(const int32_t* __restrict__ X, const u32* __restrict__ T, int nt, int N,
+                              u32* __restrict__ mark) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const i64 S = 256;
+  const int W = N / 32;
+  i64 A[3], B[3], C[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    A[a] = X[3 * (size_t)T[3 * (size_t)t] + a];
+    B[a] = X[3 * (size_t)T[3 * (size_t)t + 1] + a];
+    C[a] = X[3 * (size_t)T[3 * (size_t)t + 2] + a];
+  }
+  i64 area = (B[1] - A[1]) * (C[2] - A[2]) - (B[2] - A[2]) * (C[1] - A[1]);
+  if (area == 0) return;
+  i64 vy[3] = {A[1], B[1], C[1]}, vz[3] = {A[2], B[2], C[2]}, vx[3] = {A[0], B[0], C[0]};
+  if (area < 0) {  // counter-clockwise in (y, z)
+    i64 t0 = vy[1], t1 = vz[1], t2 = vx[1];
+    vy[1] = vy[2]; vz[1] = vz[2]; vx[1] = vx[2];
+    vy[2] = t0; vz[2] = t1; vx[2] = t2;
+    area = -area;
+  }
+  const i64 ymn = min(vy[0], min(vy[1], vy[2])), ymx = max(vy[0], max(vy[1], vy[2]));
+  const i64 zmn = min(vz[0], min(vz[1], vz[2])), zmx = max(vz[0], max(vz[1], vz[2]));
+  const i64 j0 = max((i64)0, fdiv256(ymn - S / 2 + S - 1)), j1 = min((i64)N - 1, fdiv256(ymx - S / 2));
+  const i64 k0 = max((i64)0, fdiv256(zmn - S / 2 + S - 1)), k1 = min((i64)N - 1, fdiv256(zmx - S / 2));
+  const i64 dx = area * S;
+  const double rdx = 1.0 / (double)dx;
+  for (i64 k = k0; k <= k1; ++k)
+    for (i64 j = j0; j <= j1; ++j) {
+      const i64 py = j * S + S / 2, pz = k * S + S / 2;
+      i64 w[3];
+      bool in = true;
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        const int e1 = e == 2 ? 0 : e + 1;
+        const i64 ey = vy[e1] - vy[e], ez = vz[e1] - vz[e];
+        const i64 f = ey * (pz - vz[e]) - ez * (py - vy[e]);
+        const bool tl = (ez < 0) || (ez == 0 && ey < 0);
+        if (f < 0 || (f == 0 && !tl)) in = false;
+        w[e == 0 ? 2 : e - 1] = f;
+      }
+      if (!in) continue;
+      i64 xq = fdiv_r(w[0] * vx[0] + w[1] * vx[1] + w[2] * vx[2], dx, rdx);
+      if (xq >= N) continue;
+      if (xq < 0) xq = 0;
+      atomicXor(&mark[((size_t)k * N + (size_t)j) * W + (size_t)(xq >> 5)], 1u << (xq & 31));
+    }
+}
+
+// Running XOR of the toggle bits along x, one thread per row (prefix XOR of
+// the bits inside a word, then the parity carried from the words before);
+// clears the toggles behind it.
+__global__ void k_vox_ms_fill(u32* __restrict__ mark, u32* __restrict__ rows, int N) {
+  const size_t row = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= (size_t)N * N) return;
+  const int W = N / 32;
+  u32 carry = 0u;
+  for (int w = 0; w < W; ++w) {
+    const size_t i = row * W + w;
+    u32 v = mark[i];
+    mark[i] = 0u;
+    v ^= v << 1;
+    v ^= v << 2;
+    v ^= v << 4;
+    v ^= v << 8;
+    v ^= v << 16;
+    if (carry) v = ~v;
+    rows[i] = v;
+    carry = v >> 31;
+  }
+}
+
 // Suffix XOR along z per (row y, word wx) column, then OR with the shell.
 // A workgroup owns 64 consecutive columns (one per lane, coalesced) and
 // splits z into kFillChunks chunks, one wave each: each wave XORs its chunk,
@@ -310,6 +389,22 @@ __global__ void __launch_bounds__(256) k_mesh_check_final(const u32* __restrict_
 }  // namespace
 
 size_t mesh_check_words() { return 8 + 8 * (size_t)kCheckBlocks; }
+
+hipError_t launch_voxelize_multislice(const float* P, int nv, const u32* T, int nt, const float* bmin3, double scale,
+                                      int R, int32_t* snap, u32* mark, u32* rows, hipStream_t s, bool clear) {
+  const int N = 2 * R;
+  const size_t words = (size_t)N * N * (N / 32);
+  if (clear) GM_CHECK(hipMemsetAsync(mark, 0, words * 4, s));
+  if (nv > 0)
+    hipLaunchKernelGGL(k_snap, dim3((nv + 255) / 256), dim3(256), 0, s, P, nv, bmin3[0], bmin3[1], bmin3[2],
+                       scale, snap);
+  GM_CHECK(hipGetLastError());
+  if (nt > 0) hipLaunchKernelGGL(k_vox_ms_tris, dim3((nt + 127) / 128), dim3(128), 0, s, snap, T, nt, N, mark);
+  GM_CHECK(hipGetLastError());
+  const size_t nrows = (size_t)N * N;
+  hipLaunchKernelGGL(k_vox_ms_fill, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, s, mark, rows, N);
+  return hipGetLastError();
+}
 
 hipError_t launch_mesh_check(const float* P, int nv, const u32* T, int nt, u32* out, hipStream_t s) {
   const size_t work = std::max<size_t>(3 * (size_t)nt, (size_t)nv);

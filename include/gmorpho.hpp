@@ -394,6 +394,23 @@ class GMorpho {
     cell_size_ = cell;
   }
 
+  // GMorpho::Load with the voxelizer's MultiSlicing in place of the
+  // conservative voxelization (the alternative left commented out at
+  // GMorpho.cu:7741; Voxelizer::VoxelizeByMultiSlicing, Voxelizer.cu:375-400),
+  // with GMorpho::Load's margin.
+  void LoadByMultiSlicing(const float* P, const float* N, int num_of_vertices, const unsigned int* T,
+                          int num_of_faces, int base_res, float se_size) {
+    se_size_ = se_size;
+    check(gm_load_mesh(ctx_, P, N, num_of_vertices, T, num_of_faces));
+    check(gm_voxelize_multislice(ctx_, base_res, se_size + 2.f / (2 * (float)base_res)));
+    float mn[3], mx[3], cell;
+    int res[3], dres[3];
+    check(gm_grid_info(ctx_, mn, mx, res, dres, &cell));
+    bbox_ = BoundingBox(Vec3f(mn[0], mn[1], mn[2]), Vec3f(mx[0], mx[1], mx[2]));
+    res_ = Vec3ui(res[0], res[1], res[2]);
+    cell_size_ = cell;
+  }
+
   // GMorpho::Update (GMorpho.cu:7775-7817): dilation (cube when asymmetric
   // closing is on, else sphere) and erosion with the scale field, both
   // skipped when floor(2 * global_scale / cell_size) <= 1, then the mesh.

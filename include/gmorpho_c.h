@@ -61,6 +61,11 @@ int gm_load_mesh(gm_ctx* ctx, const float* P, const float* N, int num_of_vertice
 /* Voxelizer::VoxelizeConservative (Voxelizer.cu:724-796) with
  * ComputeGridAttributes (431-455): fills GM_GRID_INPUT. */
 int gm_voxelize(gm_ctx* ctx, int base_res, float margin);
+/* Voxelizer::VoxelizeByMultiSlicing (Voxelizer.cu:375-400, kernel 188-331;
+ * the alternative GMorpho::Load leaves commented out, GMorpho.cu:7741): the
+ * same grid attributes, GM_GRID_INPUT from the parity of rays along x
+ * through the fine column centres, without the conservative shell. */
+int gm_voxelize_multislice(gm_ctx* ctx, int base_res, float margin);
 
 /* Grid::Init (Grid.cu:107-131) for a caller-supplied input grid: bounding
  * box, base resolution (cells per axis), data resolution (cells of the data

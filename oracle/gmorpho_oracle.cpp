@@ -644,6 +644,83 @@ void orc_voxelize(const float* P, int nv, const uint32_t* T, int nt, const float
           cells[cidx(R, x >> 1, y >> 1, z >> 1)] |= (uint8_t)(1 << ((x & 1) | ((y & 1) << 1) | ((z & 1) << 2)));
 }
 
+void orc_voxelize_multislice(const float* P, int nv, const uint32_t* T, int nt, const float* bmin3, float maxext,
+                             int base_res, uint8_t* cells) {
+  // Voxelizer::VoxelizeByMultiSlicing (Voxelizer.cu:375-400, kernel
+  // MultiSlicing at 188-331): rays along +x through the centres of the fine
+  // voxel columns (fine y, z) of each cell row (Voxelizer.cu:262-266); a
+  // triangle covering a ray's centre under the top-left rule (the FLT_MIN
+  // bias of the edge tests, 252-255) toggles every voxel of the column from
+  // the one holding the crossing on (qi <= x, 288-306). No surface shell.
+  // Restated on the snapped vertices of orc_voxelize with exact integer edge
+  // functions and depths; a crossing before the grid toggles the whole
+  // column, one past it nothing.
+  const int N = 2 * base_res;
+  const i64 S = 256;
+  const double sc = (double)N / (double)maxext;
+  std::vector<i64> X((size_t)nv * 3);
+  for (int i = 0; i < nv; ++i)
+    for (int a = 0; a < 3; ++a) {
+      double v = ((double)P[3 * i + a] - (double)bmin3[a]) * sc;
+      X[3 * i + a] = (i64)std::floor(v * 256.0 + 0.5);
+    }
+  auto fdiv = [](i64 a, i64 b) -> i64 {  // floor(a/b), b > 0
+    i64 q = a / b;
+    if ((a % b != 0) && (a < 0)) --q;
+    return q;
+  };
+  // one toggle bit per (x, y, z): the column toggles from x on
+  std::vector<uint8_t> mark((size_t)N * N * N, 0);
+  for (int t = 0; t < nt; ++t) {
+    const i64* a = &X[3 * (size_t)T[3 * t]];
+    const i64* b = &X[3 * (size_t)T[3 * t + 1]];
+    const i64* c = &X[3 * (size_t)T[3 * t + 2]];
+    // projection on the (y, z) plane, counter-clockwise
+    i64 area = (b[1] - a[1]) * (c[2] - a[2]) - (b[2] - a[2]) * (c[1] - a[1]);
+    if (area == 0) continue;
+    if (area < 0) {
+      std::swap(b, c);
+      area = -area;
+    }
+    const i64* V[3] = {a, b, c};
+    i64 ymn = std::min(a[1], std::min(b[1], c[1])), ymx = std::max(a[1], std::max(b[1], c[1]));
+    i64 zmn = std::min(a[2], std::min(b[2], c[2])), zmx = std::max(a[2], std::max(b[2], c[2]));
+    i64 j0 = std::max<i64>(0, fdiv(ymn - S / 2 + S - 1, S)), j1 = std::min<i64>(N - 1, fdiv(ymx - S / 2, S));
+    i64 k0 = std::max<i64>(0, fdiv(zmn - S / 2 + S - 1, S)), k1 = std::min<i64>(N - 1, fdiv(zmx - S / 2, S));
+    for (i64 k = k0; k <= k1; ++k)
+      for (i64 j = j0; j <= j1; ++j) {
+        const i64 py = j * S + S / 2, pz = k * S + S / 2;
+        i64 w[3];
+        bool in = true;
+        for (int e = 0; e < 3 && in; ++e) {
+          const i64* p0 = V[e];
+          const i64* p1 = V[(e + 1) % 3];
+          const i64 ey = p1[1] - p0[1], ez = p1[2] - p0[2];
+          const i64 f = ey * (pz - p0[2]) - ez * (py - p0[1]);
+          const bool tl = (ez < 0) || (ez == 0 && ey < 0);
+          if (f < 0 || (f == 0 && !tl)) in = false;
+          w[(e + 2) % 3] = f;
+        }
+        if (!in) continue;
+        // the voxel holding the crossing: floor(x / S) of the plane's x
+        i64 xq = fdiv(w[0] * a[0] + w[1] * b[0] + w[2] * c[0], area * S);
+        if (xq >= N) continue;
+        if (xq < 0) xq = 0;
+        mark[((size_t)k * N + (size_t)j) * N + (size_t)xq] ^= 1;
+      }
+  }
+  const int R = base_res;
+  std::memset(cells, 0, (size_t)R * R * R);
+  for (int z = 0; z < N; ++z)
+    for (int y = 0; y < N; ++y) {
+      uint8_t acc = 0;
+      for (int x = 0; x < N; ++x) {
+        acc ^= mark[((size_t)z * N + y) * N + x];
+        if (acc) cells[cidx(R, x >> 1, y >> 1, z >> 1)] |= (uint8_t)(1 << ((x & 1) | ((y & 1) << 1) | ((z & 1) << 2)));
+      }
+    }
+}
+
 void orc_scale_from_lattice(const float* lattice, int L, int N, int8_t* out) {
   // Fixed-point trilinear sampling: lattice node i sits on fine voxel i*N/L
   // (N/L a power of two >= 1), values are first rounded to 1/256, samples use

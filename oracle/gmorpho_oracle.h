@@ -35,6 +35,13 @@ void orc_grid_attributes(const float* P, int nv, int base_res, float margin, flo
 void orc_voxelize(const float* P, int nv, const uint32_t* T, int nt, const float* bmin3,
                   float maxext, int base_res, uint8_t* cells);
 
+/* Voxelizer::VoxelizeByMultiSlicing (Voxelizer.cu:188-331, 375-400; not
+ * called by the reference's GMorpho::Load, GMorpho.cu:7741): parity along x
+ * from each covered fine column's crossing voxel on, on the same snapped
+ * vertices and exact integer rule as orc_voxelize; no surface shell. */
+void orc_voxelize_multislice(const float* P, int nv, const uint32_t* T, int nt, const float* bmin3, float maxext,
+                             int base_res, uint8_t* cells);
+
 /* Config-2 style ScaleField: FP32 L^3 lattice -> N^3 signed-char radii,
  * fixed-point trilinear sampling then floor, clamped to [-128,127]. */
 void orc_scale_from_lattice(const float* lattice, int L, int N, int8_t* out);

@@ -49,6 +49,16 @@ def voxelize(P_, T_, bmin, maxext, R):
     return cells
 
 
+def voxelize_multislice(P_, T_, bmin, maxext, R):
+    P_ = np.ascontiguousarray(P_, np.float32)
+    T_ = np.ascontiguousarray(T_, np.uint32)
+    bmin = np.ascontiguousarray(bmin, np.float32)
+    cells = np.zeros(R ** 3, np.uint8)
+    lib().orc_voxelize_multislice(_ptr(P_), len(P_), _ptr(T_), len(T_), _ptr(bmin), ctypes.c_float(maxext), R,
+                                  _ptr(cells))
+    return cells
+
+
 def scale_from_lattice(lat, N):
     L = lat.shape[0]
     out = np.zeros(N ** 3, np.int8)

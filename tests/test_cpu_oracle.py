@@ -724,3 +724,99 @@ def test_exp_weight_accuracy():
     ulp = np.spacing(ref.astype(np.float32)).astype(np.float64)
     assert np.max(np.abs(got - ref) / ulp) <= 3.0
     assert O.exp_weight(np.array([0.0, -88.0, -1e30], np.float32)).tolist() == [1.0, 0.0, 0.0]
+
+
+def _box_mesh(lo, hi):
+    """Closed axis-aligned box, outward-facing triangles."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    P = np.array([[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0],
+                  [x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]], np.float32)
+    T = np.array([[0, 2, 1], [0, 3, 2], [4, 5, 6], [4, 6, 7], [0, 1, 5], [0, 5, 4],
+                  [3, 7, 6], [3, 6, 2], [0, 4, 7], [0, 7, 3], [1, 2, 6], [1, 6, 5]], np.uint32)
+    return P, T
+
+
+def _ms_grid(P, T, R):
+    # bmin = 0 and maxext = N: mesh coordinates are fine-voxel units
+    N = 2 * R
+    return fine_grid(O.voxelize_multislice(P, T, np.zeros(3, np.float32), float(N), R), R)
+
+
+def test_multislice_box_closed_form():
+    """MultiSlicing of a box: voxel (x, y, z) is set iff floor(x0) <= x <
+    floor(x1) (the crossing voxel toggles the column from there on) and the
+    column centre (y + 1/2, z + 1/2) lies inside the face; no shell."""
+    R = 24
+    lo, hi = (10.3, 5.2, 7.1), (30.7, 20.9, 15.6)
+    f = _ms_grid(*_box_mesh(lo, hi), R)
+    want = np.zeros_like(f)
+    want[7:16, 5:21, 10:30] = True  # [z][y][x]: k + 1/2 in (7.1, 15.6), j + 1/2 in (5.2, 20.9)
+    assert np.array_equal(f, want)
+    # triangle order and winding of the input do not matter
+    P, T = _box_mesh(lo, hi)
+    assert np.array_equal(_ms_grid(P, T[::-1, ::-1].copy(), R), want)
+
+
+def test_multislice_tie_rule_and_parity():
+    """Faces through column centres: the top-left rule keeps exactly one of
+    the two boundary columns per axis (15 of the 16 centres on [5.5, 20.5]),
+    and two touching boxes cover the union's columns once each; a box inside
+    another leaves a hollow (parity)."""
+    R = 24
+    f = _ms_grid(*_box_mesh((10.3, 5.5, 7.5), (30.7, 20.5, 15.5)), R)
+    cols = f.any(axis=2)  # [z][y]
+    assert cols.sum() == 15 * 8
+    assert cols[:, 5:21].sum() == 15 * 8 and cols[7:16].sum() == 15 * 8
+    # two boxes sharing the face y = 12.5: together exactly the columns of one
+    Pa, Ta = _box_mesh((10.3, 5.5, 7.5), (30.7, 12.5, 15.5))
+    Pb, Tb = _box_mesh((10.3, 12.5, 7.5), (30.7, 20.5, 15.5))
+    P2 = np.concatenate([Pa, Pb])
+    T2 = np.concatenate([Ta, Tb + 8]).astype(np.uint32)
+    assert np.array_equal(_ms_grid(P2, T2, R), f)
+    # nested boxes: parity leaves the inner box empty
+    Po, To = _box_mesh((4.3, 4.2, 4.1), (40.7, 40.9, 40.6))
+    Pi, Ti = _box_mesh((12.3, 12.2, 12.1), (20.7, 20.9, 20.6))
+    g = _ms_grid(np.concatenate([Po, Pi]), np.concatenate([To, Ti + 8]).astype(np.uint32), R)
+    outer = np.zeros_like(g)
+    outer[4:41, 4:41, 4:40] = True
+    inner = np.zeros_like(g)
+    inner[12:21, 12:21, 12:20] = True
+    assert np.array_equal(g, outer & ~inner)
+
+
+def test_multislice_convex_matches_crossings():
+    """For a closed convex mesh, each covered column is set exactly from the
+    voxel of its entry crossing up to (not including) the voxel of its exit
+    crossing, with the crossings computed here from the snapped triangles in
+    exact rational arithmetic."""
+    from fractions import Fraction
+    P, T = B.icosphere(1)
+    R = 12
+    N = 2 * R
+    P = (P * 9.0 + np.array([12.137, 11.891, 12.073])).astype(np.float32)  # fine-voxel units, inside [2, 22]
+    f = _ms_grid(P, T, R)
+    X = _snap(P, np.zeros(3, np.float32), float(N), N)
+    want = np.zeros_like(f)
+    for k in range(N):
+        for j in range(N):
+            py, pz = 256 * j + 128, 256 * k + 128
+            xs = []
+            for a, b, c in X[T]:
+                d = (b[1] - a[1]) * (c[2] - a[2]) - (b[2] - a[2]) * (c[1] - a[1])
+                if d == 0:
+                    continue
+                # barycentric weights of (py, pz) in the (y, z) projection
+                wb = Fraction(int((py - a[1]) * (c[2] - a[2]) - (pz - a[2]) * (c[1] - a[1])), int(d))
+                wc = Fraction(int((b[1] - a[1]) * (pz - a[2]) - (b[2] - a[2]) * (py - a[1])), int(d))
+                wa = 1 - wb - wc
+                if min(wa, wb, wc) < 0:
+                    continue
+                if min(wa, wb, wc) == 0:
+                    raise AssertionError("column centre on an edge: pick another scale")
+                xs.append((wa * int(a[0]) + wb * int(b[0]) + wc * int(c[0])) / 256)
+            if xs:
+                assert len(xs) == 2
+                lo, hi = min(xs), max(xs)
+                want[k, j, int(np.floor(float(lo))):int(np.floor(float(hi)))] = True
+    assert np.array_equal(f, want)

@@ -322,6 +322,32 @@ def test_voxelize_soup(ctx):
     assert np.array_equal(I, Iref), "voxelization differs in %d cells" % int((I != Iref).sum())
 
 
+@pytest.mark.parametrize("R,kind", [(64, "soup"), (128, "blob"), (48, "soup")])
+def test_voxelize_multislice(ctx, R, kind):
+    """MultiSlicing (the reference's alternative voxelizer): the same grid
+    attributes, the parity grid bit-exact with the oracle's; then closed and
+    meshed like any input grid."""
+    P, T = B.sphere_soup(12, 3, seed=41) if kind == "soup" else B.sh_blob(5, seed=42)
+    say("voxelize multislice R=%d %s" % (R, kind))
+    ctx.load_mesh(P, T)
+    margin = 0.3 + 2.0 / (2 * R)
+    ctx.voxelize(R, margin, multislice=True)
+    bmin, bmax, res, dr, cell = ctx.info()
+    rb, rB, rc, rd = O.grid_attributes(P, R, margin)
+    assert np.array_equal(bmin, rb) and np.array_equal(bmax, rB) and cell == np.float32(rc)
+    Iref = O.voxelize_multislice(P, T, rb, float((rB - rb).max()), R)
+    I = ctx.get_cells(B.GRID_INPUT)
+    assert np.array_equal(I, Iref), "voxelization differs in %d cells" % int((I != Iref).sum())
+    assert int((Iref != 0).sum()) > 100
+    ctx.scale_uniform(3)
+    ctx.close_grid(B.SE_SPHERE)
+    Dref = O.dilate(Iref, R, r=3, data_res=rd)
+    assert np.array_equal(ctx.get_cells(B.GRID_CLOSING), O.erode(Iref, Dref, R, r=3, data_res=rd))
+    # the conservative path afterwards still starts from clear marks
+    ctx.load(P, T, R, 0.3)
+    assert np.array_equal(ctx.get_cells(B.GRID_INPUT), O.voxelize(P, T, rb, float((rB - rb).max()), R))
+
+
 def _mesh_compare(ctx, C, R, bmin, cell):
     V, N, T = ctx.mesh()
     Vr, Nr, Tr = O.mc(C, R, bmin, cell)

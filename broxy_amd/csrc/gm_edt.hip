@@ -120,29 +120,41 @@ __global__ void __launch_bounds__(256) k_edt_x(const u32* __restrict__ src, int 
 
 // Stage y or z: out[(o nq + k) inner + ii] = min over sources j in reach of
 // in[(o nsrc + j) inner + ii] + dist(a k + b, [s j, s j + s])^2 (CHEB: the
-// larger of the two, the cube's per-axis distance bound).
+// larger of the two, the cube's per-axis distance bound). A thread takes
+// kEdtBlock consecutive queries k of one (o, ii): each source value is loaded
+// once for all of them (loads per query fall by the block size; the
+// minimum-plus work per query is unchanged).
+constexpr int kEdtBlock = 8;
 template <bool CHEB>
 __global__ void __launch_bounds__(256) k_edt_axis(const int* __restrict__ in, int* __restrict__ out, int nouter,
                                                   int nsrc, int nq, long long inner, int s, int a, int b,
                                                   int reach) {
+  const int nkb = (nq + kEdtBlock - 1) / kEdtBlock;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t total = (size_t)nouter * nq * (size_t)inner;
+  const size_t total = (size_t)nouter * nkb * (size_t)inner;
   if (t >= total) return;
   const size_t ii = t % (size_t)inner;
   const size_t ok = t / (size_t)inner;
-  const int k = (int)(ok % (size_t)nq), o = (int)(ok / (size_t)nq);
-  const int Y = a * k + b;
-  const int jc = Y / s;
-  const int j0 = max(0, jc - reach - 1), j1 = min(nsrc - 1, jc + reach + 1);
+  const int k0 = (int)(ok % (size_t)nkb) * kEdtBlock, o = (int)(ok / (size_t)nkb);
+  const int k1 = min(nq - 1, k0 + kEdtBlock - 1);
+  const int j0 = max(0, (a * k0 + b) / s - reach - 1), j1 = min(nsrc - 1, (a * k1 + b) / s + reach + 1);
   const int* p = in + ((size_t)o * nsrc) * (size_t)inner + ii;
-  int best = kInf;
+  int best[kEdtBlock];
+#pragma unroll
+  for (int q = 0; q < kEdtBlock; ++q) best[q] = kInf;
   for (int j = j0; j <= j1; ++j) {
     const int v = p[(size_t)j * (size_t)inner];
     if (v >= kInf) continue;
-    const int d = seg_dist(Y, j, s);
-    best = min(best, CHEB ? max(v, d) : v + d * d);
+#pragma unroll
+    for (int q = 0; q < kEdtBlock; ++q) {
+      const int d = seg_dist(a * (k0 + q) + b, j, s);
+      best[q] = min(best[q], CHEB ? max(v, d) : v + d * d);
+    }
   }
-  out[t] = best;
+  int* po = out + ((size_t)o * nq + k0) * (size_t)inner + ii;
+#pragma unroll
+  for (int q = 0; q < kEdtBlock; ++q)
+    if (k0 + q <= k1) po[(size_t)q * (size_t)inner] = best[q];
 }
 
 // ---- pass outputs (the same contracts as the chord kernels' epilogues)
@@ -254,8 +266,9 @@ hipError_t launch_edt(const u32* src, int N, int s, int a, int b, int nq, int re
     hipLaunchKernelGGL((k_edt_x<false, false>), dim3(blocks_for(n1)), dim3(256), 0, st, src, N, W, s, a, b, nq,
                        reach, rf, out);
   GM_CHECK(hipGetLastError());
-  // y: [z][j][i] -> [z][k][i]
-  const size_t n2 = (size_t)N * nq * nq;
+  // y: [z][j][i] -> [z][k][i], kEdtBlock queries k per thread
+  const size_t nkb = (size_t)((nq + kEdtBlock - 1) / kEdtBlock);
+  const size_t n2 = (size_t)N * nkb * nq;
   if (cheb)
     hipLaunchKernelGGL(k_edt_axis<true>, dim3(blocks_for(n2)), dim3(256), 0, st, out, tmp, N, N, nq, (long long)nq, s,
                        a, b, reach);
@@ -264,7 +277,7 @@ hipError_t launch_edt(const u32* src, int N, int s, int a, int b, int nq, int re
                        s, a, b, reach);
   GM_CHECK(hipGetLastError());
   // z: [j][k][i] -> [k'][k][i]
-  const size_t n3 = (size_t)nq * nq * nq;
+  const size_t n3 = nkb * (size_t)nq * nq;
   if (cheb)
     hipLaunchKernelGGL(k_edt_axis<true>, dim3(blocks_for(n3)), dim3(256), 0, st, tmp, out, 1, N, nq,
                        (long long)nq * nq, s, a, b, reach);

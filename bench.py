@@ -49,13 +49,25 @@ VAR_KERNEL = {"dilate_coarse_stream": "k_pass_ct<0,v+1> per radius value v",
               "erode_fine_stream": "k_pass_ct<3,v-1> per radius value v"}
 
 
-def algorithmic_bytes(stage, N, nv, nt):
+def edt_bytes(N, nq):
+    """One separable transform of a bit grid to nq^3 query points: the bit
+    grid read, then each axis stage's 4-byte values written and read back."""
+    return N ** 3 / 8.0 + 8.0 * N * N * nq + 8.0 * N * nq * nq + 4.0 * nq ** 3
+
+
+def algorithmic_bytes(stage, N, nv, nt, edt=False):
     """Compulsory HBM bytes of one launch of a stage's kernel: bit grids N^3/8
     bytes, cell flags N^3/64 bytes, mesh 12 bytes per vertex position, normal
-    and triangle."""
+    and triangle. edt: radii above 31, whose four passes are distance
+    transforms (plus the threshold pass that writes the flags or bits)."""
     g = N ** 3 / 8.0
     f = N ** 3 / 64.0
     stage = stage.replace("_stream", "")
+    if edt and stage in ("dilate_coarse", "dilate_fine", "erode_coarse", "erode_fine"):
+        return {"dilate_coarse": edt_bytes(N, N) + 4.0 * (N // 2) ** 3 + 2 * f,
+                "dilate_fine": 4.0 * N ** 3 + 2 * g + f,
+                "erode_coarse": edt_bytes(N, N // 2) + 4.0 * (N // 2) ** 3 + 3 * f,
+                "erode_fine": edt_bytes(N, N) + 4.0 * N ** 3 + 2 * g + f}[stage]
     return {
         "brick_summary": g + f,            # read I; write I-full cell flags
         "dilate_coarse": g + 2 * f,        # read I, I-full flags; write DF
@@ -82,9 +94,14 @@ COUNTER_FILES = {("large", "sphere"): "counters_sh512_r8.json"}
 
 
 def dominant_kernel(stage, r, se):
-    """rocprof name of the kernel a stage times (cube dilations run the
-    runtime-table stream kernels)."""
+    """rocprof name of the kernel a stage times (a uniform cube radius runs
+    the compiled box kernels, passes 4 and 5; radii above 31 the
+    distance-transform passes)."""
+    if r and r > 31 and stage.endswith("_stream"):
+        return "k_edt_x + k_edt_axis x2 + threshold (distance-transform passes)"
     if se == "cube" and stage in ("dilate_coarse_stream", "dilate_fine_stream"):
+        if r:
+            return "k_pass_ct<%d,%d>" % (4 if "coarse" in stage else 5, r)
         return "k_dil_coarse" if "coarse" in stage else "k_dil_fine"
     if r:
         return KERNEL_OF_STAGE[stage].replace("rho", str(rho_of_stage(stage, r)))
@@ -442,7 +459,7 @@ def main():
         acc += ctx.stage_times()
     ctx.set_timing(False)
     stage_ms = acc / reps
-    planes_ms = None
+    planes_ms = field_step_ms = None
     if lat is not None:
         # a new radius field: its first closing also builds the radius bit
         # planes (inside the brick_summary stage); later closings reuse them
@@ -452,9 +469,20 @@ def main():
         ctx.extract_mesh()
         planes_ms = float(ctx.stage_times()[0] - stage_ms[0])
         ctx.set_timing(False)
+        # the same steps with a new radius field before each one (host upload
+        # of the lattice, radius planes rebuilt on the device, then the step)
+        L = int(round(lat.size ** (1 / 3.0)))
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.scale_lattice(lat, L)
+            ctx.close_grid(se)
+            ctx.extract_mesh()
+        ctx.sync()
+        field_step_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     dom = max((i for i, s in enumerate(STAGES) if s in SINGLE_KERNEL_STAGES), key=lambda i: stage_ms[i])
     dname = STAGES[dom]
-    dbytes = algorithmic_bytes(dname, N, nv, nt)
+    dbytes = algorithmic_bytes(dname, N, nv, nt, edt=r is not None and r > 31)
     achieved = dbytes / (stage_ms[dom] * 1e-3) / 1e9
     whole_bytes = 5 * N ** 3 / 8.0 + 12 * nv + 12 * nt
     whole_gbs = whole_bytes / (ms * 1e-3) / 1e9
@@ -516,6 +544,7 @@ def main():
     }
     if planes_ms is not None:
         out["radius_planes_ms_once_per_field"] = round(planes_ms, 4)
+        out["ms_per_step_new_field_each_step"] = round(field_step_ms, 4)
     if not args.no_cpu:
         I = ctx.get_cells(B.GRID_INPUT)
         cb, Dref, Cref, mesh = cpu_baseline(I, R, r, dr, bmin, cell, N, lat, se)

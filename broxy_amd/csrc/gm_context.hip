@@ -1026,7 +1026,11 @@ int gm_scale_grid_get(gm_ctx* c, int8_t* grid) {
 int gm_scale_lattice_f32(gm_ctx* c, const float* lat, int L) {
   int rc = need_grid(c);
   if (rc) return rc;
-  touch(c);
+  // a new field of the same lattice size, radius range and arithmetic width
+  // keeps the launch arguments (the kernels read the lattice through its
+  // device pointer): a captured closing is then replayed, not captured again
+  const RadiusField before = radius_field(c);
+  const int beforeLo = c->rlo, beforeHi = c->rhi;
   if (!lat || L < 2 || L > c->N || (c->N % L) != 0 || ((c->N / L) & (c->N / L - 1)) != 0)
     return fail(c, GM_ERR_ARG, "lattice size must divide N with a power-of-two ratio");
   hipSetDevice(c->device);
@@ -1041,6 +1045,7 @@ int gm_scale_lattice_f32(gm_ctx* c, const float* lat, int L) {
     mx = std::max<long long>(mx, q[i]);
   }
   if (c->L != L) {
+    touch(c);  // the lattice moves
     cfree(c, c->dLat);
     HIPCALL(c, calloc_dev(c, &c->dLat, n * 4));
   }
@@ -1054,6 +1059,10 @@ int gm_scale_lattice_f32(gm_ctx* c, const float* lat, int L) {
   c->rlo = (int)std::max<long long>(2, std::min<long long>(127, fl(mn)));
   c->rhi = (int)std::max<long long>(2, std::min<long long>(127, fl(mx)));
   c->meshPending = false;
+  const RadiusField after = radius_field(c);
+  if (before.mode != after.mode || before.L != after.L || before.lat != after.lat || before.narrow != after.narrow ||
+      beforeLo != c->rlo || beforeHi != c->rhi)
+    touch(c);
   return GM_OK;
 }
 

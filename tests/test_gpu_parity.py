@@ -751,6 +751,38 @@ def test_graph_replays_match_eager(ctx):
         eager.close()
 
 
+def test_graph_replay_new_lattice_values(ctx):
+    """A new lattice of the same size and radius range keeps the captured
+    closing (its launch arguments do not change): the replay must read the
+    new values. Each field's closing equals the eager context's and the
+    oracle's."""
+    R = 32
+    N = 2 * R
+    I = random_blobs(R, seed=91, nballs=6)
+    lat_a = B.scale_lattice(8, seed=5, lo=2.0, hi=9.0, smooth=1)
+    rng = np.random.default_rng(6)
+    lat_b = rng.permutation(lat_a.reshape(-1)).reshape(lat_a.shape).astype(np.float32)  # same range
+    eager = B.Context(0)
+    try:
+        eager.set_graphs(False)
+        grid_setup(eager, R, I)
+        grid_setup(ctx, R, I)
+        ctx.set_graphs(True)
+        for k, lat in enumerate((lat_a, lat_b, lat_a, lat_b)):
+            say("graph replay with lattice %d" % k)
+            out = []
+            for c in (ctx, eager):
+                c.scale_lattice(lat, 8)
+                c.close_grid(B.SE_SPHERE)
+                out.append(c.get_cells(B.GRID_CLOSING))
+            assert np.array_equal(out[0], out[1]), "replay %d differs from eager" % k
+            sc = O.scale_from_lattice(lat.reshape(8, 8, 8), N)
+            Dref = O.dilate(I, R, scale=sc)
+            assert np.array_equal(out[0], O.erode(I, Dref, R, scale=sc))
+    finally:
+        eager.close()
+
+
 def test_graph_repeat_mode_matches_eager(ctx):
     """Capture-on-repeat: the first closing runs eagerly, the second (same
     launch sequence, new grid contents) is captured, the later ones replay;

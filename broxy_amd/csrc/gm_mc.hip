@@ -227,8 +227,9 @@ __device__ u32 block_exscan(u32 v, u32* lds16, u32* total, bool trailing = true)
   return off + x - v;
 }
 
-constexpr int kListCap = 2048;  // work-list window (vertices or triangles of a block)
+constexpr int kListCap = 1024;  // work-list window (vertices or triangles of a block)
 constexpr int kFaceCap = 1024;  // face normals kept in LDS (blocks with at most this many triangles)
+static_assert(kFaceCap <= kListCap, "the triangles whose face normals are kept fit one window");
 
 constexpr int kNbCells = 9 * 9 * 9 - 8 * 8 * 8;
 static_assert(kNbCells <= kMcThreads, "one thread per cell just above a block");
@@ -255,6 +256,12 @@ __device__ __forceinline__ void nb_cell(int k, int& lx, int& ly, int& lz) {
   }
 }
 
+// Axes (bit 0/1/2 = x/y/z) in which the cell at local coordinates 7 - l lies
+// below the block, for l = nb_cell of a cell just above it.
+__device__ __forceinline__ int shell_axes(int lx, int ly, int lz) {
+  return (lx == 8 ? 1 : 0) | (ly == 8 ? 2 : 0) | (lz == 8 ? 4 : 0);
+}
+
 // Vertices owned by the cubes before cube j of a cell whose 8 cube cases are
 // the bytes of w (cut edges among 0, 3, 8 per byte, summed over bytes < j).
 __device__ __forceinline__ u32 own_prefix8(unsigned long long w, int j) {
@@ -277,12 +284,19 @@ struct BlockLds {
   unsigned long long nbAbove[kNbCells];
   unsigned long long nbBelow[kNbCells];
   u32 nbBase[kNbCells];
-  u32 vOff[512];   // block-local vertex base of each cell with a mixed cube
-  u32 tOff[512];   // block-local triangle base of each cell with a mixed cube
+  // Shell: the cubes of the cells below the block that share vertices with
+  // it (fine coordinate one below the block in every axis where the cell is
+  // below it). Their counted triangles are numbered after the block's own
+  // (shell base of the cell, then byte j = in-cell prefix of cube j), so
+  // their face normals can follow the block's in fn.
+  u32 nbTB[kNbCells];
+  unsigned long long nbTP[kNbCells];
+  u32 nshell;
+  u32 vtOff[512];  // per cell with a mixed cube: block-local vertex base (low 16 bits), triangle base (high 16)
   u32 red[16];
   uint16_t cl[512];  // the cells with a mixed cube (local Morton index), S.ncl of them
   u32 ncl;
-  uint16_t list[kListCap];  // cube index << 3 | item, by local output index
+  uint16_t list[kListCap];  // cube index << 3 | item, by local output index (shell cubes: 4096 + 8 k + j)
   float fn[kFaceCap][3];     // face normal of each local triangle (when it fits)
 };
 
@@ -308,8 +322,8 @@ __device__ void stage_block(BlockLds& S, const unsigned long long* __restrict__ 
                             const u32* __restrict__ posOf, const u32* __restrict__ blockV,
                             const u32* __restrict__ cellL, int R, u32 it, int c0x, int c0y, int c0z) {
   const int t = (int)threadIdx.x;
-  unsigned long long above = 0ull, below = 0ull;
-  u32 nbB = 0u;
+  unsigned long long above = 0ull, below = 0ull, shellPre = 0ull;
+  u32 nbB = 0u, shellCnt = 0u;
   if (t < kNbCells) {
     int lx, ly, lz;
     nb_cell(t, lx, ly, lz);
@@ -317,6 +331,14 @@ __device__ void stage_block(BlockLds& S, const unsigned long long* __restrict__ 
     above = listed_cell_cases(cellCase, posOf, R, c0x + lx, c0y + ly, c0z + lz, &code, &pos);
     if (above) nbB = blockV[code >> 9] + (cellL[(size_t)pos * 512 + (code & 511u)] & 0xffffu);
     below = listed_cell_cases(cellCase, posOf, R, c0x + 7 - lx, c0y + 7 - ly, c0z + 7 - lz, &code, &pos);
+    // the below cell's shell cubes: sub-position bit set in each axis where
+    // the cell is below the block
+    const int need = shell_axes(lx, ly, lz);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      shellPre |= (unsigned long long)shellCnt << (8 * j);
+      if ((j & need) == need) shellCnt += S.tab.ntri[(below >> (8 * j)) & 0xffull];
+    }
   }
   const unsigned long long* own = cellCase + (size_t)it * 512;
   const u32* ownL = cellL + (size_t)it * 512;
@@ -333,19 +355,36 @@ __device__ void stage_block(BlockLds& S, const unsigned long long* __restrict__ 
   for (int h = 0; h < 2; ++h) {
     const int l = t + 256 * h;
     *reinterpret_cast<unsigned long long*>(&S.cases[8 * l]) = w[h];
-    if (w[h]) {
-      S.vOff[l] = base[h] & 0xffffu;
-      S.tOff[l] = base[h] >> 16;
-    }
+    if (w[h]) S.vtOff[l] = base[h];
     m[h] = __ballot(w[h] != 0ull);
     if (lane == 0) S.red[4 * h + wv] = (u32)__popcll(m[h]);
   }
+  // shell triangles: inclusive scan over the wave, wave totals in red[8..11]
+  u32 shellInc = shellCnt;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const u32 y = __shfl_up(shellInc, d, 64);
+    if (lane >= d) shellInc += y;
+  }
+  if (lane == 63) S.red[8 + wv] = shellInc;
   if (t < kNbCells) {
     S.nbAbove[t] = above;
     S.nbBelow[t] = below;
     S.nbBase[t] = nbB;
+    S.nbTP[t] = shellPre;
   }
   __syncthreads();
+  {
+    u32 off = shellInc - shellCnt, tot = 0u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const u32 c = S.red[8 + k];
+      off += k < wv ? c : 0u;
+      tot += c;
+    }
+    if (t < kNbCells) S.nbTB[t] = off;
+    if (t == 0) S.nshell = tot;
+  }
   // list position: cells with a mixed cube of the earlier waves, then of
   // this wave's earlier lanes
   u32 pre[2] = {0u, 0u}, n = 0u;
@@ -759,10 +798,12 @@ __device__ __forceinline__ u32 prefix_ntri(const BlockLds& S, int l, int j) { re
 // v0: the case of cube f; pv: the vertex's own position (the same float
 // values local_edge_pos gives for its edge in every sharing cube). With
 // useFn, the face normals of the block's own cubes come from S.fn (written
-// by the triangle phase from the same positions); cubes of other blocks are
-// computed here. The sum runs in the same order either way.
-__device__ void normal_sum(const BlockLds& S, const Geo& geo, int R, int c0x, int c0y, int c0z, bool useFn, int fx,
-                           int fy, int fz, int q, int v0, const float* pv, float* n) {
+// by the triangle phase from the same positions), and with useShell those
+// of the shell cubes too, from shellBase on; otherwise cubes of other blocks
+// are computed here. The sum runs in the same order either way.
+__device__ void normal_sum(const BlockLds& S, const Geo& geo, int R, int c0x, int c0y, int c0z, bool useFn,
+                           bool useShell, u32 shellBase, int fx, int fy, int fz, int q, int v0, const float* pv,
+                           float* n) {
   n[0] = n[1] = n[2] = 0.f;
   const u32 sp = share_of(q);
 #pragma unroll
@@ -781,8 +822,14 @@ __device__ void normal_sum(const BlockLds& S, const Geo& geo, int R, int c0x, in
                           : (inBlock ? (int)S.cases[8 * l + j]
                                      : (int)((S.nbBelow[nb_index(7 - lx, 7 - ly, 7 - lz)] >> (8 * j)) & 0xffu));
     if (!mixed(gv)) continue;
-    if (useFn && inBlock) {
-      const u32 lt = S.tOff[l] + prefix_ntri(S, l, j);
+    if (inBlock ? useFn : useShell) {
+      u32 lt;
+      if (inBlock) {
+        lt = (S.vtOff[l] >> 16) + prefix_ntri(S, l, j);
+      } else {
+        const int k = nb_index(7 - lx, 7 - ly, 7 - lz);
+        lt = shellBase + S.nbTB[k] + (u32)((S.nbTP[k] >> (8 * j)) & 0xffull);
+      }
       u32 use = S.tab.use[gv][le];
       while (use) {
         const u32 t = (u32)__builtin_ctz(use);
@@ -842,14 +889,15 @@ __device__ __forceinline__ void unit3(float* n) {
 
 // unit: the sums are final (no listed-but-uncounted triangle anywhere), so
 // they are stored normalized (NormalizeMeshNormals, GMorpho.cu:6526-6537).
-__device__ void mc_verts_block(BlockLds& S, int R, Geo geo, int blk, int c0x, int c0y, int c0z, bool useFn, u32 base,
-                               u32 nverts, float* V, float* Nrm, u32 capV, bool unit) {
+__device__ void mc_verts_block(BlockLds& S, int R, Geo geo, int blk, int c0x, int c0y, int c0z, bool useFn,
+                               bool useShell, u32 shellBase, u32 base, u32 nverts, float* V, float* Nrm, u32 capV,
+                               bool unit) {
   for (u32 w0 = 0; w0 < nverts; w0 += kListCap) {
     for (u32 i = threadIdx.x; i < S.ncl; i += blockDim.x) {
       const int l = S.cl[i];
       const unsigned long long w = cell_cases(S, l);
       // the cell's vertices, in cube and edge order, from its local base
-      u32 lv = S.vOff[l] - w0;
+      u32 lv = (S.vtOff[l] & 0xffffu) - w0;
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int v = (int)((w >> (8 * j)) & 0xffu);
@@ -878,7 +926,7 @@ __device__ void mc_verts_block(BlockLds& S, int R, Geo geo, int blk, int c0x, in
       // own edge q runs from corner 0 to corner 1 (x), 3 (y) or 4 (z)
       edge_pos_v(geo, fx, fy, fz, q == 0 ? 0 : (q == 1 ? 3 : 8), (u32)v & 1u,
                  (u32)(v >> (q == 0 ? 1 : (q == 1 ? 3 : 4))) & 1u, p);
-      normal_sum(S, geo, R, c0x, c0y, c0z, useFn, fx, fy, fz, q, v, p, nn);
+      normal_sum(S, geo, R, c0x, c0y, c0z, useFn, useShell, shellBase, fx, fy, fz, q, v, p, nn);
       if (unit) unit3(nn);
       V[3 * (size_t)vid] = p[0];
       V[3 * (size_t)vid + 1] = p[1];
@@ -909,7 +957,7 @@ __device__ __forceinline__ u32 edge_vid(const BlockLds& S, int blk, int c0x, int
   if (lx < 8 && ly < 8 && lz < 8) {
     const int l = local_morton(lx, ly, lz);
     const unsigned long long w = cell_cases(S, l);
-    return base + S.vOff[l] + own_prefix8(w, j) + (u32)rank_of((int)((w >> (8 * j)) & 0xffu), oe);
+    return base + (S.vtOff[l] & 0xffffu) + own_prefix8(w, j) + (u32)rank_of((int)((w >> (8 * j)) & 0xffu), oe);
   }
   // a cell just above the block (staged): its global base + in-cell prefix
   const int k = nb_index(lx, ly, lz);
@@ -920,18 +968,35 @@ __device__ __forceinline__ u32 edge_vid(const BlockLds& S, int blk, int c0x, int
 // Triangles of a staged block, in the same two phases over local triangle
 // slots. Listed-but-uncounted triangles (len > ntri) go to the late list
 // from the fill phase of the first window. With useFn (one window) each
-// counted triangle's face normal goes to S.fn at its local index.
-__device__ void mc_tris_block(BlockLds& S, int R, Geo geo, int blk, int c0x, int c0y, int c0z, bool useFn, u32 vbase,
-                              u32 tbase, u32 ntris, const u32* __restrict__ blockV, const u32* __restrict__ cellL,
-                              u32* T, u32 capT, u32* late, u32 capLate, u32* totals) {
+// counted triangle's face normal goes to S.fn at its local index; with
+// useShell the counted triangles of the shell cubes follow as extra items
+// (face normal only, at ntris + their shell index).
+__device__ void mc_tris_block(BlockLds& S, int R, Geo geo, int blk, int c0x, int c0y, int c0z, bool useFn,
+                              bool useShell, u32 vbase, u32 tbase, u32 ntris, const u32* __restrict__ blockV,
+                              const u32* __restrict__ cellL, u32* T, u32 capT, u32* late, u32 capLate, u32* totals) {
   u32 border = 0u;
-  for (u32 w0 = 0; w0 < ntris; w0 += kListCap) {
+  for (u32 w0 = 0; w0 < ntris || w0 == 0; w0 += kListCap) {
+    if (useShell)  // one window: ntris + nshell <= kFaceCap <= kListCap
+      for (u32 k = threadIdx.x; k < (u32)kNbCells; k += blockDim.x) {
+        const unsigned long long wb = S.nbBelow[k];
+        if (!wb) continue;
+        int lx, ly, lz;
+        nb_cell((int)k, lx, ly, lz);
+        const int need = shell_axes(lx, ly, lz);
+        u32 lt = ntris + S.nbTB[k];
+        for (int j = 0; j < 8; ++j) {
+          if ((j & need) != need) continue;
+          const u32 cnt = S.tab.ntri[(wb >> (8 * j)) & 0xffull];
+          for (u32 q = 0; q < cnt; ++q) S.list[lt + q] = (uint16_t)(((4096u + 8u * k + (u32)j) << 3) | q);
+          lt += cnt;
+        }
+      }
     for (u32 i = threadIdx.x; i < S.ncl; i += blockDim.x) {
       const int l = S.cl[i];
       const unsigned long long w = cell_cases(S, l);
       // the cell's counted triangles, in cube order, from its local base;
       // the first window also stores the in-cell prefix per cube
-      const u32 lt0 = S.tOff[l];
+      const u32 lt0 = S.vtOff[l] >> 16;
       u32 lt = lt0;
       unsigned long long tp = 0ull;
 #pragma unroll
@@ -972,10 +1037,27 @@ __device__ void mc_tris_block(BlockLds& S, int R, Geo geo, int blk, int c0x, int
     }
     __syncthreads();
     const u32 n = min((u32)kListCap, ntris - w0);
-    for (u32 i = threadIdx.x; i < n; i += blockDim.x) {
-      const u32 sl = tbase + w0 + i;
-      if (sl >= capT) break;
+    const u32 nx = n + (useShell ? S.nshell : 0u);
+    for (u32 i = threadIdx.x; i < nx; i += blockDim.x) {
       const int e = S.list[i];
+      if (i >= n) {
+        // a shell cube's triangle: its face normal only
+        const int s = (e >> 3) - 4096, q = e & 7, k = s >> 3, j = s & 7;
+        int lx, ly, lz;
+        nb_cell(k, lx, ly, lz);
+        const int fx = 2 * (c0x + 7 - lx) + (j & 1), fy = 2 * (c0y + 7 - ly) + ((j >> 1) & 1),
+                  fz = 2 * (c0z + 7 - lz) + ((j >> 2) & 1);
+        const int v = (int)((S.nbBelow[k] >> (8 * j)) & 0xffull);
+        const unsigned long long tri = S.tab.tri[v];
+        float p0[3], p1[3], p2[3];
+        local_edge_pos(geo, fx, fy, fz, v, (int)((tri >> (12 * q + 8)) & 0xf), p0);
+        local_edge_pos(geo, fx, fy, fz, v, (int)((tri >> (12 * q + 4)) & 0xf), p1);
+        local_edge_pos(geo, fx, fy, fz, v, (int)((tri >> (12 * q)) & 0xf), p2);
+        face_cross(p0, p1, p2, S.fn[i]);
+        continue;
+      }
+      const u32 sl = tbase + w0 + i;
+      if (sl >= capT) continue;
       const int s = e >> 3, q = e & 7, l = s >> 3, j = s & 7;
       int cx, cy, cz;
       local_cell(l, cx, cy, cz);
@@ -1018,10 +1100,14 @@ __device__ __forceinline__ void mc_emit_block(BlockLds& S, int R, Geo geo, u32 i
   stage_block(S, cellCase, posOf, blockV, cellL, R, it, c0x, c0y, c0z);
   // triangles first: their face normals (when they fit in LDS) are then
   // gathered by the vertices
-  const bool useFn = tnext - tbase <= (u32)kFaceCap && tnext <= capT;
-  mc_tris_block(S, R, geo, blk, c0x, c0y, c0z, useFn, base, tbase, tnext - tbase, blockV, cellL, T, capT, late,
+  const u32 ntris = tnext - tbase;
+  const bool useFn = ntris <= (u32)kFaceCap && tnext <= capT;
+  // the shell's face normals too, when they fit after the block's
+  const bool useShell = useFn && ntris + S.nshell <= (u32)kFaceCap;
+  mc_tris_block(S, R, geo, blk, c0x, c0y, c0z, useFn, useShell, base, tbase, ntris, blockV, cellL, T, capT, late,
                 capLate, totals);
-  mc_verts_block(S, R, geo, blk, c0x, c0y, c0z, useFn, base, next - base, V, Nrm, capV, totals[4] == 0u);
+  mc_verts_block(S, R, geo, blk, c0x, c0y, c0z, useFn, useShell, ntris, base, next - base, V, Nrm, capV,
+                 totals[4] == 0u);
 }
 
 // Vertices and triangles of the active blocks: each listed block is staged
@@ -1029,7 +1115,7 @@ __device__ __forceinline__ void mc_emit_block(BlockLds& S, int R, Geo geo, u32 i
 // its triangles are written. Vertex ids on edges owned by another block's
 // cube come from that block's base and the per-cell local bases written by
 // k_mc_count, so no block waits for another.
-__global__ void __launch_bounds__(kMcThreads) k_mc_emit(int R, Geo geo, const u32* __restrict__ list, int nblocks,
+__global__ void __launch_bounds__(kMcThreads) __attribute__((amdgpu_waves_per_eu(4))) k_mc_emit(int R, Geo geo, const u32* __restrict__ list, int nblocks,
                                                          const u32* __restrict__ blockV,
                                                          const u32* __restrict__ blockT,
                                                          const u32* __restrict__ cellL,
@@ -1195,7 +1281,7 @@ __global__ void __launch_bounds__(256) k_scan_down_b(const MCJob* __restrict__ j
                  (int)(blockIdx.x - pre[j]), J.m.totals, J.m.capV, J.m.capT, J.m.posOf, J.m.order, J.m.list,
                  J.m.capList);
 }
-__global__ void __launch_bounds__(kMcThreads) k_mc_emit_b(const MCJob* __restrict__ jobs, const u32* __restrict__ lpre,
+__global__ void __launch_bounds__(kMcThreads) __attribute__((amdgpu_waves_per_eu(4))) k_mc_emit_b(const MCJob* __restrict__ jobs, const u32* __restrict__ lpre,
                                                            int n) {
   __shared__ BlockLds S;
   if (blockIdx.x >= lpre[n]) return;

@@ -367,10 +367,13 @@ int alloc_grid(gm_ctx* c, int R) {
   }
   HIPCALL(c, calloc_dev(c, &c->mc.posOf, (size_t)c->mc.nblocks * 4));
   HIPCALL(c, calloc_dev(c, &c->mc.order, ((size_t)4 * c->mc.nblocks + 4) * 4));
-  HIPCALL(c, calloc_dev(c, &c->mc.totals, 32));  // nV, nT, overflow, border, late cubes, spare
+  HIPCALL(c, calloc_dev(c, &c->mc.totals, 32));  // nV, nT, overflow, border, late cubes, listed blocks, spare
   HIPCALL(c, calloc_dev(c, &c->mc.scanPart, mc_scan_part_bytes(c->mc.nblocks)));
   HIPCALL(c, calloc_dev(c, &c->mc.brick, brickBytes));
   HIPCALL(c, calloc_dev(c, &c->mc.list, ((size_t)c->mc.nblocks + 1) * 4));
+  // the block-list count starts from zero (each extraction's last kernel
+  // clears it for the next)
+  HIPCALL(c, hipMemsetAsync(c->mc.list, 0, 4, c->stream));
   c->mc.capV = c->mc.capT = c->mc.capLate = 0;
   int rc = alloc_mesh_out(c, (u32)std::min<size_t>((size_t)4 * c->N * c->N, 1u << 30),
                           (u32)std::min<size_t>((size_t)8 * c->N * c->N, 1u << 30), 4096);
@@ -1152,7 +1155,7 @@ int gm_mesh_counts(gm_ctx* c, uint32_t* nv, uint32_t* nt) {
     // more listed blocks than the per-block buffers hold: grow them and
     // extract again (nothing was stored)
     u32 listed = 0;
-    HIPCALL(c, hipMemcpyAsync(&listed, c->mc.list, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCALL(c, hipMemcpyAsync(&listed, c->mc.totals + 5, 4, hipMemcpyDeviceToHost, c->stream));
     HIPCALL(c, hipStreamSynchronize(c->stream));
     rc = alloc_mc_list(c, std::min<u32>((u32)c->mc.nblocks, grow_cap(c->mc.capList, listed)));
     if (rc) return rc;

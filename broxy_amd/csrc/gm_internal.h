@@ -321,10 +321,10 @@ struct MCBuffers {
   u32* T;
   u32* late;     // [0] count, capLate (slot, a, b, c) quadruples, then their copy in slot order
   u32* totals;   // [0] nV [1] nT [2] overflow (1 output, 2 late list, 4 block list) [3] border flag
-                 // [4] cubes listing more triangles than they count (8 words)
+                 // [4] cubes listing more triangles than they count [5] listed blocks (8 words)
   u32* scanPart; // per 1024 blocks: vertex and triangle totals (mc_scan_part_bytes)
-  uint8_t* brick; // brick summary of the closing grid (1 word x 8 rows x 8 planes)
-  u32* list;      // [0] number of active blocks, then their ids
+  uint8_t* brick; // brick summary of the closing grid (1 word x 8 rows x 8 planes; batched extraction)
+  u32* list;      // [0] number of active blocks (zero between extractions), then their ids
   u32 capV, capT, capLate;
   u32 capList;   // listed blocks cellV and cellCase hold
   int nblocks;   // (P/8)^3

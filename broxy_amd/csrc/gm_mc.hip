@@ -472,19 +472,93 @@ __device__ __forceinline__ void mc_blocks_body(const uint8_t* __restrict__ brick
     }
   }
 }
-__global__ void __launch_bounds__(256) k_mc_blocks(const uint8_t* __restrict__ brick, int R, int nblocks,
-                                                   u32* blockV, u32* blockT, u32* list, u32* posOf, u32* order,
-                                                   u32* part, u32* totals) {
-  // the emit-order class counts start from zero in the scan that follows,
-  // and the chunk totals in the count kernel (kPartStride words per chunk)
-  if (blockIdx.x == 0 && threadIdx.x < kOrderClasses) order[threadIdx.x] = 0u;
-  if (blockIdx.x == 0 && threadIdx.x == 0) totals[4] = 0u;  // late cubes, tallied by the count kernel
-  const int gid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (gid < (nblocks + kChunk - 1) / kChunk) {
-    part[kPartStride * gid] = 0u;
-    part[kPartStride * gid + 1] = 0u;
+
+// Active Morton blocks straight from C (one launch instead of a brick
+// summary and a block pass): workgroup (BY, BZ) takes the blocks (bx, BY, BZ)
+// of every bx. A block is active when its 17^3 fine voxels (the corners of
+// its 16^3 cubes, clamped to the grid) are neither all clear nor all set; a
+// block with no mixed cube has no vertex, so listing exactly these blocks
+// gives the same mesh as any superset. Thread (word w, row y) walks the 17
+// planes and folds its word into the blocks it touches: 2w (bits 0..16),
+// 2w + 1 (bits 16..31) and 2w - 1 (bit 0). list[0] is zero on entry (the
+// finish kernel of the previous extraction clears it).
+constexpr int kMaxBlocksPerAxis = 128;
+__global__ void __launch_bounds__(512) k_mc_blocks_c(const u32* __restrict__ C, int R, int nb, int nblocks,
+                                                      u32* blockV, u32* blockT, u32* list, u32* posOf, u32* order,
+                                                      u32* part, u32* totals) {
+  __shared__ u32 anyB[kMaxBlocksPerAxis], allB[kMaxBlocksPerAxis];
+  if (blockIdx.x == 0 && blockIdx.y == 0) {
+    if (threadIdx.x < kOrderClasses) order[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) totals[4] = 0u;  // late cubes, tallied by the count kernel
+    for (int i = threadIdx.x; i < (nblocks + kChunk - 1) / kChunk; i += blockDim.x) {
+      part[kPartStride * i] = 0u;
+      part[kPartStride * i + 1] = 0u;
+    }
   }
-  mc_blocks_body(brick, R, nblocks, blockV, blockT, list, posOf, blockIdx.x * blockDim.x + threadIdx.x);
+  const int N = 2 * R, W = N / 32;
+  const int BY = blockIdx.x, BZ = blockIdx.y;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    anyB[i] = 0u;
+    allB[i] = 1u;
+  }
+  __syncthreads();
+  const int fy0 = 16 * BY, fz0 = 16 * BZ;
+  if (fy0 < N && fz0 < N) {
+    const int ny = min(17, N - fy0), nz = min(17, N - fz0);
+    for (int p = threadIdx.x; p < ny * W; p += blockDim.x) {
+      const int w = p % W, y = fy0 + p / W;
+      const u32* g = C + ((size_t)fz0 * N + y) * W + w;
+      u32 v[17];  // the 17 loads in flight together
+#pragma unroll
+      for (int z = 0; z < 17; ++z) v[z] = z < nz ? g[(size_t)z * N * W] : v[0];
+      u32 any = 0u, all = ~0u;
+#pragma unroll
+      for (int z = 0; z < 17; ++z) {
+        any |= v[z];
+        all &= v[z];
+      }
+      // block 2w: bits 0..16; 2w + 1: bits 16..31 (and bit 0 of w + 1); 2w - 1: bit 0
+      constexpr u32 kLo = 0x1ffffu, kHi = 0xffff0000u;
+      if (any & kLo) atomicOr(&anyB[2 * w], 1u);
+      if ((all & kLo) != kLo) atomicAnd(&allB[2 * w], 0u);
+      if (any & kHi) atomicOr(&anyB[2 * w + 1], 1u);
+      if ((all & kHi) != kHi) atomicAnd(&allB[2 * w + 1], 0u);
+      if (w > 0) {
+        if (any & 1u) atomicOr(&anyB[2 * w - 1], 1u);
+        if (!(all & 1u)) atomicAnd(&allB[2 * w - 1], 0u);
+      }
+    }
+  }
+  __syncthreads();
+  // one wave lists the row of blocks (nb <= 128: two rounds at most)
+  if (threadIdx.x >= 64) return;
+  const int lane = (int)threadIdx.x;
+  for (int bx0 = 0; bx0 < nb; bx0 += 64) {
+    const int bx = bx0 + lane;
+    bool act = false;
+    u32 blk = 0u;
+    if (bx < nb) {
+      blk = part1by2((u32)bx) | (part1by2((u32)BY) << 1) | (part1by2((u32)BZ) << 2);
+      act = 16 * bx < N && fy0 < N && fz0 < N && anyB[bx] != 0u && allB[bx] == 0u;
+      if (!act) {
+        blockV[blk] = 0u;
+        blockT[blk] = 0u;
+        posOf[blk] = ~0u;
+      }
+    }
+    const unsigned long long m = __ballot(act);
+    if (m) {
+      const int leader = __ffsll((long long)m) - 1;
+      u32 base = 0u;
+      if (lane == leader) base = atomicAdd(&list[0], (u32)__popcll(m));
+      base = __shfl(base, leader, 64);
+      if (act) {
+        const u32 pos = base + (u32)__popcll(m & ((1ull << lane) - 1ull));
+        list[1 + pos] = blk;
+        posOf[blk] = pos;
+      }
+    }
+  }
 }
 
 // Count kernels: threads per listed block and consecutive cells per thread
@@ -1210,8 +1284,17 @@ __device__ __forceinline__ void mc_normalize_body(float* Nrm, const u32* totals,
 // counts (the emit stored unit normals); otherwise the late triangles are
 // applied, then every normal is normalized, by one workgroup (rare).
 constexpr int kFinishThreads = 1024;
+// It also clears the block-list count for the next extraction, keeping it
+// in totals[5] for the host (read when the list overflowed).
 __device__ __forceinline__ void mc_finish_body(const float* V, float* Nrm, u32* T, u32* late, u32 capLate,
-                                               u32* totals, u32 capV, u32 capT) {
+                                               u32* totals, u32 capV, u32 capT, u32* list) {
+  if (threadIdx.x == 0) {
+    const u32 listed = list[0];
+    if (listed) {
+      totals[5] = listed;
+      list[0] = 0u;
+    }
+  }
   if (totals[4] == 0u) return;
   mc_fixup_body(V, Nrm, T, late, capLate, totals, capV, capT);
   __threadfence_block();
@@ -1219,8 +1302,9 @@ __device__ __forceinline__ void mc_finish_body(const float* V, float* Nrm, u32* 
   mc_normalize_body(Nrm, totals, capV, threadIdx.x, blockDim.x);
 }
 __global__ void __launch_bounds__(kFinishThreads) k_mc_finish(const float* V, float* Nrm, u32* T, u32* late,
-                                                              u32 capLate, u32* totals, u32 capV, u32 capT) {
-  mc_finish_body(V, Nrm, T, late, capLate, totals, capV, capT);
+                                                              u32 capLate, u32* totals, u32 capV, u32 capT,
+                                                              u32* list) {
+  mc_finish_body(V, Nrm, T, late, capLate, totals, capV, capT, list);
 }
 
 // ---- Batched forms: many closed grids in one launch per step.
@@ -1298,7 +1382,7 @@ __global__ void __launch_bounds__(kMcThreads) __attribute__((amdgpu_waves_per_eu
 }
 __global__ void __launch_bounds__(kFinishThreads) k_mc_finish_b(const MCJob* __restrict__ jobs) {
   const MCJob& J = jobs[blockIdx.x];
-  mc_finish_body(J.m.V, J.m.Nrm, J.m.T, J.m.late, J.m.capLate, J.m.totals, J.m.capV, J.m.capT);
+  mc_finish_body(J.m.V, J.m.Nrm, J.m.T, J.m.late, J.m.capLate, J.m.totals, J.m.capV, J.m.capT, J.m.list);
 }
 __global__ void k_totals_b(const MCJob* __restrict__ jobs, int n, u32* out) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1361,10 +1445,11 @@ hipError_t launch_mc(const u32* C, int R, const float* bmin3, float cell, const 
                      hipEvent_t* ev) {
   GM_CHECK(upload_tables());
   if (ev) GM_CHECK(hipEventRecord(ev[0], s));
-  // active Morton blocks from a brick summary of C
-  GM_CHECK(launch_brick_summary(C, 2 * R, m.brick, m.list, s));
-  hipLaunchKernelGGL(k_mc_blocks, dim3((m.nblocks + 255) / 256), dim3(256), 0, s, m.brick, R, m.nblocks, m.blockV,
-                     m.blockT, m.list, m.posOf, m.order, m.scanPart, m.totals);
+  // active Morton blocks, straight from C
+  const int nb = m.P / 8;
+  if (nb > kMaxBlocksPerAxis) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_mc_blocks_c, dim3(nb, nb), dim3(512), 0, s, C, R, nb, m.nblocks, m.blockV, m.blockT, m.list,
+                     m.posOf, m.order, m.scanPart, m.totals);
   GM_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_mc_count, dim3(mc_list_grid(m.nblocks, 2048)), dim3(kCountThreads), 0, s, C, R, m.list,
                      m.blockV, m.blockT, m.cellV, m.cellCase, m.late, m.scanPart, m.capList, m.totals);
@@ -1468,7 +1553,7 @@ hipError_t launch_mc_emit(const u32* C, int R, const float* bmin3, float cell, c
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
   hipLaunchKernelGGL(k_mc_finish, dim3(1), dim3(kFinishThreads), 0, s, m.V, m.Nrm, m.T, m.late, m.capLate, m.totals,
-                     m.capV, m.capT);
+                     m.capV, m.capT, m.list);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   return hipSuccess;

@@ -697,7 +697,7 @@ __device__ __forceinline__ void scan_reduce_body(const u32* __restrict__ a, cons
 __device__ __forceinline__ void scan_down_body(u32* a, u32* b, int n, const u32* __restrict__ part, int pstride,
                                                int nparts, int chunk, u32* totals, u32 capV, u32 capT,
                                                const u32* __restrict__ posOf = nullptr, u32* order = nullptr,
-                                               const u32* __restrict__ list = nullptr, u32 capList = 0) {
+                                               u32* list = nullptr, u32 capList = 0, bool clearList = false) {
   __shared__ u32 red[16];
   __shared__ u32 cls[2 * kOrderClasses];
   int base = chunk * kChunk + 4 * threadIdx.x;
@@ -722,10 +722,18 @@ __device__ __forceinline__ void scan_down_body(u32* a, u32* b, int n, const u32*
   block_exscan(pb, red, &offB);
   if (chunk == 0) {
     if (threadIdx.x == 0) {
+      const u32 listed = list ? list[0] : 0u;
       totals[0] = offA;
       totals[1] = offB;
-      totals[2] = ((offA > capV || offB > capT) ? 1u : 0u) | (list && list[0] > capList ? 4u : 0u);
+      totals[2] = ((offA > capV || offB > capT) ? 1u : 0u) | (listed > capList ? 4u : 0u);
       totals[3] = 0u;
+      if (clearList) {
+        // the count kernel has read it: kept for the host, cleared for the
+        // next extraction; the emit's completion counter starts at zero
+        totals[5] = listed;
+        totals[6] = 0u;
+        list[0] = 0u;
+      }
     }
     offA = offB = 0u;
   }
@@ -765,8 +773,9 @@ __device__ __forceinline__ void scan_down_body(u32* a, u32* b, int n, const u32*
 }
 __global__ void __launch_bounds__(256) k_scan_down(u32* a, u32* b, int n, const u32* __restrict__ part, int nparts,
                                                    u32* totals, u32 capV, u32 capT, const u32* __restrict__ posOf,
-                                                   u32* order, const u32* __restrict__ list, u32 capList) {
-  scan_down_body(a, b, n, part, kPartStride, nparts, blockIdx.x, totals, capV, capT, posOf, order, list, capList);
+                                                   u32* order, u32* list, u32 capList) {
+  scan_down_body(a, b, n, part, kPartStride, nparts, blockIdx.x, totals, capV, capT, posOf, order, list, capList,
+                 true);
 }
 
 // Position of the vertex on owned edge e (0: x, 3: y, 8: z) of cube f:
@@ -1184,34 +1193,6 @@ __device__ __forceinline__ void mc_emit_block(BlockLds& S, int R, Geo geo, u32 i
                  totals[4] == 0u);
 }
 
-// Vertices and triangles of the active blocks: each listed block is staged
-// once (voxel box, cube cases, per-cell prefix sums), then its vertices and
-// its triangles are written. Vertex ids on edges owned by another block's
-// cube come from that block's base and the per-cell local bases written by
-// k_mc_count, so no block waits for another.
-__global__ void __launch_bounds__(kMcThreads) __attribute__((amdgpu_waves_per_eu(4))) k_mc_emit(int R, Geo geo, const u32* __restrict__ list, int nblocks,
-                                                         const u32* __restrict__ blockV,
-                                                         const u32* __restrict__ blockT,
-                                                         const u32* __restrict__ cellL,
-                                                         const unsigned long long* __restrict__ cellCase,
-                                                         const u32* __restrict__ posOf,
-                                                         const u32* __restrict__ order, float* V, float* Nrm,
-                                                         u32 capV, u32* T, u32 capT, u32* late, u32 capLate,
-                                                         u32* totals, u32 capList) {
-  __shared__ BlockLds S;
-  if (list[0] > capList) return;  // the per-block buffers overflowed: extracted again
-  u32 nwork = 0u;  // blocks with output
-  for (int c = 0; c < kOrderClasses; ++c) nwork += order[c];
-  if (blockIdx.x >= nwork) return;  // the grid is sized for every block of the grid
-  load_mc_tab(S.tab);
-  for (u32 i = blockIdx.x; i < nwork; i += gridDim.x) {
-    __syncthreads();  // LDS of the previous block is no longer read
-    const u32 it = order_at(order, nblocks, i);
-    mc_emit_block(S, R, geo, it, (int)list[1 + it], nblocks, blockV, blockT, cellL, cellCase, posOf, V, Nrm, capV, T,
-                  capT, late, capLate, totals);
-  }
-}
-
 __device__ __forceinline__ void face(const float* V, const u32* t, float* n) {
   const float* p0 = V + 3 * (size_t)t[0];
   const float* p1 = V + 3 * (size_t)t[1];
@@ -1282,13 +1263,14 @@ __device__ __forceinline__ void mc_normalize_body(float* Nrm, const u32* totals,
 
 // The mesh's last step: nothing when no cube lists more triangles than it
 // counts (the emit stored unit normals); otherwise the late triangles are
-// applied, then every normal is normalized, by one workgroup (rare).
+// applied, then every normal is normalized, by one workgroup (rare). With a
+// list (batched extraction) it also clears the block-list count for the
+// next extraction, keeping it in totals[5] for the host (read when the list
+// overflowed); a single extraction's scan does that.
 constexpr int kFinishThreads = 1024;
-// It also clears the block-list count for the next extraction, keeping it
-// in totals[5] for the host (read when the list overflowed).
 __device__ __forceinline__ void mc_finish_body(const float* V, float* Nrm, u32* T, u32* late, u32 capLate,
                                                u32* totals, u32 capV, u32 capT, u32* list) {
-  if (threadIdx.x == 0) {
+  if (list && threadIdx.x == 0) {
     const u32 listed = list[0];
     if (listed) {
       totals[5] = listed;
@@ -1301,10 +1283,51 @@ __device__ __forceinline__ void mc_finish_body(const float* V, float* Nrm, u32* 
   __syncthreads();
   mc_normalize_body(Nrm, totals, capV, threadIdx.x, blockDim.x);
 }
-__global__ void __launch_bounds__(kFinishThreads) k_mc_finish(const float* V, float* Nrm, u32* T, u32* late,
-                                                              u32 capLate, u32* totals, u32 capV, u32 capT,
-                                                              u32* list) {
-  mc_finish_body(V, Nrm, T, late, capLate, totals, capV, capT, list);
+
+// Vertices and triangles of the active blocks: each listed block is staged
+// once (voxel box, cube cases, per-cell prefix sums), then its vertices and
+// its triangles are written. Vertex ids on edges owned by another block's
+// cube come from that block's base and the per-cell local bases written by
+// k_mc_count, so no block waits for another.
+__global__ void __launch_bounds__(kMcThreads) __attribute__((amdgpu_waves_per_eu(4))) k_mc_emit(int R, Geo geo, const u32* __restrict__ list, int nblocks,
+                                                         const u32* __restrict__ blockV,
+                                                         const u32* __restrict__ blockT,
+                                                         const u32* __restrict__ cellL,
+                                                         const unsigned long long* __restrict__ cellCase,
+                                                         const u32* __restrict__ posOf,
+                                                         const u32* __restrict__ order, float* V, float* Nrm,
+                                                         u32 capV, u32* T, u32 capT, u32* late, u32 capLate,
+                                                         u32* totals, u32 capList) {
+  __shared__ BlockLds S;
+  __shared__ u32 lastGroup;
+  if (totals[2] & 4u) return;  // the per-block buffers overflowed: extracted again
+  u32 nwork = 0u;  // blocks with output
+  for (int c = 0; c < kOrderClasses; ++c) nwork += order[c];
+  if (blockIdx.x >= nwork) return;  // the grid is sized for every block of the grid
+  load_mc_tab(S.tab);
+  for (u32 i = blockIdx.x; i < nwork; i += gridDim.x) {
+    __syncthreads();  // LDS of the previous block is no longer read
+    const u32 it = order_at(order, nblocks, i);
+    mc_emit_block(S, R, geo, it, (int)list[1 + it], nblocks, blockV, blockT, cellL, cellCase, posOf, V, Nrm, capV, T,
+                  capT, late, capLate, totals);
+  }
+  // Cubes listing more triangles than they count (rare; the count kernel
+  // tallied them): every block stored raw normal sums, and the last
+  // workgroup to finish applies the late triangles and normalizes. Only
+  // then do the workgroups publish their writes (release, then the
+  // completion count in totals[6], zeroed by the scan; acquire in the last).
+  if (totals[4] == 0u) return;  // uniform: unit normals are already stored
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const u32 done = atomicAdd(&totals[6], 1u) + 1u;
+    lastGroup = done == min(gridDim.x, nwork) ? 1u : 0u;
+  }
+  __syncthreads();
+  if (!lastGroup) return;
+  __threadfence();
+  mc_finish_body(V, Nrm, T, late, capLate, totals, capV, capT, nullptr);
+  if (threadIdx.x == 0) totals[6] = 0u;  // a re-emit into grown buffers counts again
 }
 
 // ---- Batched forms: many closed grids in one launch per step.
@@ -1552,9 +1575,8 @@ hipError_t launch_mc_emit(const u32* C, int R, const float* bmin3, float cell, c
                      m.capT, m.late, m.capLate, m.totals, m.capList);
   GM_CHECK(hipGetLastError());
   if (ev) GM_CHECK(hipEventRecord(ev[1], s));
-  hipLaunchKernelGGL(k_mc_finish, dim3(1), dim3(kFinishThreads), 0, s, m.V, m.Nrm, m.T, m.late, m.capLate, m.totals,
-                     m.capV, m.capT, m.list);
-  GM_CHECK(hipGetLastError());
+  // the late triangles and normalization, when any, ran in the emit's last
+  // workgroup: the stage after it is empty
   if (ev) GM_CHECK(hipEventRecord(ev[2], s));
   return hipSuccess;
 }

@@ -181,8 +181,7 @@ def run_alone(ctx, j):
         ctx.scale_lattice(j["lattice"], int(round(j["lattice"].size ** (1 / 3.0))))
     else:
         ctx.scale_uniform(j["radius"])
-    ctx.close_grid(B.SE_SPHERE)
-    ctx.extract_mesh()
+    ctx.update(B.SE_SPHERE)
     return ctx.mesh()
 
 
@@ -229,8 +228,7 @@ def batched(args):
 
     def one_by_one():
         for c in ctxs:
-            c.close_grid(B.SE_SPHERE)
-            c.extract_mesh()
+            c.update(B.SE_SPHERE)
             c.mesh_counts()
 
     def all_at_once():
@@ -434,16 +432,16 @@ def main():
         # chosen by timing every candidate (outside the timed region); radii
         # above 31 run the distance-transform passes, which have none
         shape = ctx.autotune(se, 5)
+    # a step: GMorpho::Update up to the mesh (closing, then marching cubes),
+    # enqueued as one captured sequence
     for _ in range(args.warmup):
-        ctx.close_grid(se)
-        ctx.extract_mesh()
+        ctx.update(se)
         nv, nt = ctx.mesh_counts()
 
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        ctx.close_grid(se)
-        ctx.extract_mesh()
+        ctx.update(se)
     ctx.sync()
     t1 = time.perf_counter()
     nv, nt = ctx.mesh_counts()
@@ -476,8 +474,7 @@ def main():
         t0 = time.perf_counter()
         for _ in range(args.steps):
             ctx.scale_lattice(lat, L)
-            ctx.close_grid(se)
-            ctx.extract_mesh()
+            ctx.update(se)
         ctx.sync()
         field_step_ms = (time.perf_counter() - t0) * 1e3 / args.steps
     dom = max((i for i, s in enumerate(STAGES) if s in SINGLE_KERNEL_STAGES), key=lambda i: stage_ms[i])

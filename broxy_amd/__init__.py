@@ -219,6 +219,10 @@ class Context:
     def extract_mesh(self):
         self._call("gm_extract_mesh")
 
+    def update(self, se=SE_SPHERE):
+        """gm_update: close_grid then extract_mesh as one enqueued sequence."""
+        self._call("gm_update", se)
+
     def mesh_counts(self):
         nv, nt = ctypes.c_uint32(), ctypes.c_uint32()
         self._call("gm_mesh_counts", ctypes.byref(nv), ctypes.byref(nt))

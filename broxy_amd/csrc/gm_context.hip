@@ -103,7 +103,7 @@ struct gm_ctx {
     uint64_t key[4] = {0, 0, 0, 0};
     uint64_t seen[4] = {0, 0, 0, 0};  // key of the last eager run (repeat mode)
   };
-  Graph gClose[2], gMesh;
+  Graph gClose[2], gMesh, gUpdate[2];
   uint64_t graphGen = 1;
   // 0: eager; 1: capture at once; 2: capture when a key repeats (a context
   // that runs a different job every time never pays for a capture)
@@ -700,6 +700,8 @@ int gm_destroy(gm_ctx* c) {
   drop_graph(c->gClose[0]);
   drop_graph(c->gClose[1]);
   drop_graph(c->gMesh);
+  drop_graph(c->gUpdate[0]);
+  drop_graph(c->gUpdate[1]);
   free_grid(c);
   cfree(c, c->dP);
   cfree(c, c->dT);
@@ -1093,13 +1095,17 @@ int gm_erode(gm_ctx* c) {
   return GM_OK;
 }
 
-int gm_close(gm_ctx* c, int se) {
+// The closing's set-up shared by gm_close and gm_update: *copied = true when
+// the radius skips the morphology (the closing is the input, copied here);
+// otherwise the passes are prepared and key is the captured sequence's key.
+static int close_setup(gm_ctx* c, int se, bool* copied, uint64_t key[4]) {
   int rc = need_grid(c);
   if (rc) return rc;
   if (se != GM_SE_SPHERE && se != GM_SE_CUBE) return fail(c, GM_ERR_ARG, "unknown structuring element");
   hipSetDevice(c->device);
   c->tuneSe = se;
   c->evValid = 0;
+  *copied = false;
   size_t rowBytes = (size_t)c->N * c->N * c->W * 4;
   if (c->rmode == 0 && c->r <= 1) {
     // GMorpho.cu:7784, 7803: no morphology below two fine voxels.
@@ -1107,6 +1113,7 @@ int gm_close(gm_ctx* c, int se) {
     HIPCALL(c, hipMemsetAsync(c->K, 0, rowBytes, c->stream));
     HIPCALL(c, hipMemcpyAsync(c->C, c->I, rowBytes, hipMemcpyDeviceToDevice, c->stream));
     c->meshPending = false;
+    *copied = true;
     return GM_OK;
   }
   rc = prep_dilate(c, se);
@@ -1115,8 +1122,26 @@ int gm_close(gm_ctx* c, int se) {
   // the tile-queue toggle returns to its value after the four passes, so a
   // captured sequence stays valid while it starts from the same value
   const MorphBuffers shape = morph_buffers(c);
-  const uint64_t key[4] = {c->graphGen, (uint64_t)se | ((uint64_t)shape.tileZ << 8) | ((uint64_t)shape.waveCap << 16),
-                           (uint64_t)c->qsel, (uint64_t)(c->rpValid ? 1 : 0)};
+  key[0] = c->graphGen;
+  key[1] = (uint64_t)se | ((uint64_t)shape.tileZ << 8) | ((uint64_t)shape.waveCap << 16);
+  key[2] = (uint64_t)c->qsel;
+  key[3] = (uint64_t)(c->rpValid ? 1 : 0);
+  return GM_OK;
+}
+
+// Marching cubes of C and the copy of its totals, enqueued (GMorpho.cu:6814-7001).
+static int enq_mesh(gm_ctx* c) {
+  hipError_t e = launch_mc(c->C, c->R, c->bmin, c->cell, c->mc, c->stream, c->timing ? &c->ev[10] : nullptr);
+  if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("mesh extraction: ") + hipGetErrorString(e));
+  HIPCALL(c, hipMemcpyAsync(c->hTotals, c->mc.totals, 16, hipMemcpyDeviceToHost, c->stream));
+  return GM_OK;
+}
+
+int gm_close(gm_ctx* c, int se) {
+  bool copied = false;
+  uint64_t key[4] = {0, 0, 0, 0};
+  int rc = close_setup(c, se, &copied, key);
+  if (rc || copied) return rc;
   rc = run_captured(c, c->gClose[se], key, [&]() -> int {
     int r = enq_dilate(c, se);
     return r == GM_OK ? enq_erode(c) : r;
@@ -1133,15 +1158,29 @@ int gm_extract_mesh(gm_ctx* c) {
   hipSetDevice(c->device);
   HIPCALL(c, mc_prepare());
   const uint64_t key[4] = {c->graphGen, 0, 0, 0};
-  rc = run_captured(c, c->gMesh, key, [&]() -> int {
-    hipError_t e = launch_mc(c->C, c->R, c->bmin, c->cell, c->mc, c->stream, c->timing ? &c->ev[10] : nullptr);
-    if (e != hipSuccess) return fail(c, GM_ERR_HIP, std::string("mesh extraction: ") + hipGetErrorString(e));
-    HIPCALL(c, hipMemcpyAsync(c->hTotals, c->mc.totals, 16, hipMemcpyDeviceToHost, c->stream));
-    return GM_OK;
-  });
+  rc = run_captured(c, c->gMesh, key, [&]() -> int { return enq_mesh(c); });
   if (rc) return rc;
   c->meshPending = true;
   if (c->timing) c->evValid |= 2;
+  return GM_OK;
+}
+
+int gm_update(gm_ctx* c, int se) {
+  bool copied = false;
+  uint64_t key[4] = {0, 0, 0, 0};
+  int rc = close_setup(c, se, &copied, key);
+  if (rc) return rc;
+  if (copied) return gm_extract_mesh(c);
+  HIPCALL(c, mc_prepare());
+  rc = run_captured(c, c->gUpdate[se], key, [&]() -> int {
+    int r = enq_dilate(c, se);
+    if (r == GM_OK) r = enq_erode(c);
+    return r == GM_OK ? enq_mesh(c) : r;
+  });
+  if (rc) return rc;
+  after_morph(c, true);
+  c->meshPending = true;
+  if (c->timing) c->evValid |= 3;
   return GM_OK;
 }
 
@@ -1592,8 +1631,7 @@ int gm_batch_close_mesh(gm_batch* b, gm_ctx* const* ctxs, int n, int se) {
   // the contexts' own streams are idle before the batch stream touches their buffers
   for (int i = 0; i < n; ++i) BHIP(b, hipStreamSynchronize(ctxs[i]->stream));
   for (gm_ctx* c : single) {
-    int rc = gm_close(c, se);
-    if (rc == GM_OK) rc = gm_extract_mesh(c);
+    const int rc = gm_update(c, se);
     if (rc != GM_OK) return bfail(b, rc, c->err);
   }
   const int m = (int)grouped.size();

@@ -280,8 +280,7 @@ struct gm_engine {
       batched = true;
       return true;
     }
-    if (!step(s, j, gm_close(c, d.se))) return false;
-    return step(s, j, gm_extract_mesh(c));
+    return step(s, j, gm_update(c, d.se));
   }
 
   // Mesh counts, then the mesh copied by the device into a pinned result
@@ -371,8 +370,7 @@ struct gm_engine {
           // a job of the batch failed: each job's own status comes from
           // closing it alone (the loaded grids are unchanged by a closing)
           for (Slot* s : group) {
-            s->closeRc = gm_close(s->ctx, se);
-            if (s->closeRc == GM_OK) s->closeRc = gm_extract_mesh(s->ctx);
+            s->closeRc = gm_update(s->ctx, se);
             if (s->closeRc != GM_OK) s->closeErr = gm_last_error(s->ctx);
           }
         }

@@ -420,21 +420,26 @@ class GMorpho {
       throw Exception("frame-field (rotated cube) dilation is not part of this build");
     const int gate = (int)std::floor(2.f * scale_field.global_scale() / cell_size_);
     const int se = use_asymmetric_closing_ ? GM_SE_CUBE : GM_SE_SPHERE;
-    if (gate > 1) {
+    if (gate > 1)
       apply_scale(scale_field);
-      check(gm_dilate(ctx_, se));
-      check(gm_erode(ctx_));
-    } else {
-      check(gm_scale_uniform(ctx_, gate));
-      check(gm_close(ctx_, se));  // radius <= 1: the closing is the input
-    }
-    ExtractClosingMesh(mesh, use_asymmetric_closing_ ? 200 : 20);
+    else
+      check(gm_scale_uniform(ctx_, gate));  // radius <= 1: the closing is the input
+    // dilation, erosion and marching cubes enqueued as one sequence
+    check(gm_update(ctx_, se));
+    ReadClosingMesh(mesh, use_asymmetric_closing_ ? 200 : 20);
   }
   // GMorpho::Update (GMorpho.cu:7819-7845): cube dilation and sphere erosion
   // with the fixed radius floor(2 * se_size / cell_size) (Grid.cu:725).
   void Update(Mesh& mesh, float se_size) {
     se_size_ = se_size;
-    check(gm_scale_uniform(ctx_, (int)std::floor(2.f * se_size / cell_size_)));
+    const int r = (int)std::floor(2.f * se_size / cell_size_);
+    check(gm_scale_uniform(ctx_, r));
+    if (r > 1) {
+      check(gm_update(ctx_, GM_SE_CUBE));  // as gm_dilate, gm_erode, gm_extract_mesh
+      ReadClosingMesh(mesh);
+      return;
+    }
+    // this overload runs the passes at every radius
     check(gm_dilate(ctx_, GM_SE_CUBE));
     check(gm_erode(ctx_));
     ExtractClosingMesh(mesh);
@@ -457,6 +462,11 @@ class GMorpho {
   // (GMorpho.cu:6957-6978), copied into mesh.
   void ExtractClosingMesh(Mesh& mesh, int num_bilateral_iters = 20) {
     check(gm_extract_mesh(ctx_));
+    ReadClosingMesh(mesh, num_bilateral_iters);
+  }
+  // The extracted mesh, its normals filtered when bilateral filtering is on,
+  // copied into mesh.
+  void ReadClosingMesh(Mesh& mesh, int num_bilateral_iters = 20) {
     if (bilateral_filtering_) check(gm_filter_normals(ctx_, num_bilateral_iters));
     uint32_t nv = 0, nt = 0;
     check(gm_mesh_counts(ctx_, &nv, &nt));

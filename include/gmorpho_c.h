@@ -129,6 +129,12 @@ int gm_close(gm_ctx* ctx, int se);
  * vertices, area-weighted unit normals and triangles stay on the device.
  * Enqueue only; gm_mesh_counts waits for it. */
 int gm_extract_mesh(gm_ctx* ctx);
+/* GMorpho::Update (GMorpho.cu:7775-7817) up to the mesh: gm_close(ctx, se)
+ * then gm_extract_mesh(ctx), with identical results, enqueued as one
+ * sequence (with graphs on, one captured launch per call instead of two).
+ * Enqueue only; gm_mesh_counts waits for it. The normal filtering the
+ * reference ends with is gm_filter_normals. */
+int gm_update(gm_ctx* ctx, int se);
 /* Sizes of the extracted mesh (waits for the extraction). */
 int gm_mesh_counts(gm_ctx* ctx, uint32_t* num_vertices, uint32_t* num_triangles);
 /* Copy the mesh out (GMorpho.cu:6985-6995): V, Nrm float[3*nv], T uint32[3*nt]. */

@@ -751,6 +751,55 @@ def test_graph_replays_match_eager(ctx):
         eager.close()
 
 
+def test_update_matches_close_then_extract(ctx):
+    """gm_update (closing and marching cubes as one captured sequence) gives
+    the closing grid and mesh of gm_close then gm_extract_mesh on an eager
+    context, replay after replay with eager work between the replays, across
+    a radius change, per-voxel radii, the cube element, a radius <= 1 (no
+    morphology) and a radius above the chord tables; the closing is also
+    the oracle's."""
+    R = 48
+    grids = [random_blobs(R, seed=70 + k, nballs=8) for k in range(2)]
+    sc = np.random.default_rng(5).integers(1, 7, (2 * R) ** 3).astype(np.int8)
+
+    def setup(c, I, r, s_):
+        c.set_cells(B.GRID_INPUT, I)
+        if s_ is not None:
+            c.scale_grid(s_)
+        else:
+            c.scale_uniform(r)
+
+    def read(c):
+        V, Nn, T = c.mesh()
+        return c.get_cells(B.GRID_CLOSING), V, Nn, T
+
+    eager = B.Context(0)
+    try:
+        eager.set_graphs(False)
+        bmin, cell, dr = grid_setup(eager, R, grids[0])
+        grid_setup(ctx, R, grids[0])
+        ctx.set_graphs(True)
+        plan = [(0, 4, None, B.SE_SPHERE), (1, 4, None, B.SE_SPHERE), (0, 4, None, B.SE_SPHERE),
+                (1, 6, None, B.SE_CUBE), (0, None, sc, B.SE_SPHERE), (1, 1, None, B.SE_SPHERE),
+                (0, 40, None, B.SE_SPHERE)]
+        for k, (g, r, s_, se) in enumerate(plan):
+            say("update %d: grid %d radius %s se %d" % (k, g, r if r is not None else "per-voxel", se))
+            setup(ctx, grids[g], r, s_)
+            ctx.update(se)
+            got = read(ctx)
+            setup(eager, grids[g], r, s_)
+            eager.close_grid(se)
+            eager.extract_mesh()
+            ref = read(eager)
+            for a, b in zip(got, ref):
+                assert np.array_equal(a, b)
+            if se == B.SE_SPHERE and s_ is None and r > 1:
+                Dref = O.dilate(grids[g], R, r=r, data_res=dr)
+                assert np.array_equal(got[0], O.erode(grids[g], Dref, R, r=r, data_res=dr))
+    finally:
+        eager.close()
+
+
 def test_graph_replay_new_lattice_values(ctx):
     """A new lattice of the same size and radius range keeps the captured
     closing (its launch arguments do not change): the replay must read the

@@ -44,12 +44,13 @@ namespace gm {
 
 namespace {
 
-// Tile selection of each pass (one wave per tile). A workgroup of 16 waves
-// covers a slab of 16 word tiles at N = 512, so it reserves its queue slots
-// with one or two device atomics: those atomics, from every workgroup on
-// one cache line, serialize across the XCDs, and with 4 waves per workgroup
-// (4 times the atomics) they bounded the selection.
-constexpr int kSelectWaves = 16;
+// Tile selection of each pass (one wave per tile). A workgroup of 8 waves
+// covers half a slab of 16 word tiles at N = 512, so it reserves its queue
+// slots with one or two device atomics: those atomics, from every workgroup
+// on one cache line, serialize across the XCDs. With 4 waves per workgroup
+// (twice the atomics again) they bounded the selection; with 16, the 2048
+// tiles of N = 512 fill only 128 workgroups, half the CUs.
+constexpr int kSelectWaves = 8;
 template <int PASS>
 __global__ void __launch_bounds__(64 * kSelectWaves) k_select(MorphArgs a) {
   select_tiles<PASS>(a);

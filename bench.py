@@ -31,9 +31,11 @@ STAGES = list(B.Context.STAGES)
 KERNEL_OF_STAGE = {"brick_summary": "k_brick_summary", "dilate_coarse": "k_pass_ct<0,rho>",
                    "tight_contour": "k_tight", "dilate_fine": "k_pass_ct<1,rho>", "contour_K": "k_contour_K",
                    "erode_coarse": "k_pass_ct<2,rho>", "base_contour": "k_base_contour",
-                   "erode_fine": "k_pass_ct<3,rho>", "mc_count": "k_mc_count",
+                   "erode_fine": "k_pass_ct<3,rho>", "mc_count": "k_mc_blocks_c + k_mc_count",
                    "mc_scan": "k_scan_down", "mc_emit": "k_mc_emit",
-                   "mc_fixup_normalize": "k_mc_finish",
+                   # no kernel of its own: the emit's last workgroup, only when a
+                   # cube lists more triangles than it counts
+                   "mc_fixup_normalize": "k_mc_emit (last workgroup)",
                    "dilate_coarse_stream": "k_pass_ct<0,rho>", "dilate_fine_stream": "k_pass_ct<1,rho>",
                    "erode_coarse_stream": "k_pass_ct<2,rho>", "erode_fine_stream": "k_pass_ct<3,rho>"}
 # Stages timed over exactly one kernel launch: the dominant kernel is chosen
@@ -77,7 +79,7 @@ def algorithmic_bytes(stage, N, nv, nt, edt=False):
         "erode_coarse": g + 3 * f,         # read K, D-any and I-full flags; write CL
         "base_contour": 2 * g + 2 * f,     # read D, CL; write UG, C
         "erode_fine": 2 * g + f,           # read K, UG; clear bits of C (upper bound: every word)
-        "mc_count": g,                     # read C
+        "mc_count": 2 * g,                 # read C (block list, then the count)
         "mc_scan": 16.0 * N ** 3 / 4096,   # block counts and bases (2 u32 per 8^3 cells, read + write)
         "mc_emit": g + 24 * nv + 12 * nt,  # read C; write positions, normal sums, triangles
         "mc_fixup_normalize": 24 * nv,     # normals read + write

@@ -258,7 +258,7 @@ int alloc_mesh_out(gm_ctx* c, u32 capV, u32 capT, u32 capLate) {
   }
   if (capLate > c->mc.capLate || !c->mc.late) {
     cfree(c, c->mc.late);
-    // entries, then their copy in slot order (k_mc_fixup)
+    // entries, then their copy in slot order (the late-triangle fixup)
     HIPCALL(c, calloc_dev(c, &c->mc.late, (size_t)(1 + 8 * capLate) * 4));
     c->mc.capLate = capLate;
   }
